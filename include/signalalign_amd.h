@@ -1,0 +1,178 @@
+/*
+ * signalalign_amd.h -- C-ABI of the MI355X-native banded signal-to-reference
+ * pair-HMM forward-backward (libsignalalign_amd.so).
+ *
+ * The library replaces signalAlign's getAlignedPairsUsingAnchors path
+ * (impl/pairwiseAligner.c:1871-1899 and everything below it down to
+ * stateMachine3_cellCalculate, impl/stateMachine.c:1194-1325) with HIP kernels
+ * for gfx950.  Plain pointers and sizes only; all functions return SA_OK (0) or
+ * a negative status and never exit the process.
+ *
+ * Threading: a context serialises its own calls with an internal mutex; use one
+ * context per host thread for concurrency.
+ */
+#ifndef SIGNALALIGN_AMD_H
+#define SIGNALALIGN_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SA_OK 0
+#define SA_ERR_INVALID_ARGUMENT (-1) /* bad sizes, odd expansion, bad parameters */
+#define SA_ERR_HIP (-2)              /* a HIP runtime call failed */
+#define SA_ERR_OUT_OF_MEMORY (-3)
+#define SA_ERR_BAD_KMER (-4)         /* a reference k-mer has a character outside the alphabet
+                                        (the reference exits, impl/nanopore_hdp.c:376-381) */
+#define SA_ERR_ANCHORS (-5)          /* anchors not strictly increasing / outside the matrix
+                                        (asserts at impl/pairwiseAligner.c:161-167) */
+#define SA_ERR_UNSUPPORTED (-6)      /* band wider than the kernels support, unknown emission */
+#define SA_ERR_NO_DEVICE (-7)
+
+/* Arithmetic of the recurrence.  SA_FP64 follows the reference's doubles;
+ * SA_FP32 runs the cell recurrence in float log space relative to a per-diagonal
+ * FP64 offset, with FP64 totals and posterior normalisation. */
+typedef enum { SA_FP64 = 0, SA_FP32 = 1 } sa_precision;
+
+/* Emission functions of the reference's three-state machines:
+ *   SA_EMIT_GAUSS_DESCALED  emissions_signal_strawManGetKmerEventMatchProbWithDescaling
+ *                           (impl/stateMachine.c:498-545), signalMachine's choice
+ *                           (getStateMachine3_descaled, :1627-1643)
+ *   SA_EMIT_GAUSS           emissions_signal_strawManGetKmerEventMatchProb (:547-589)
+ *                           (getStateMachine3, :1645-1653)
+ *   SA_EMIT_HDP             emissions_signal_getHdpKmerDensity (:468-495) with
+ *                           stateMachine3HDP_cellCalculate (:1259-1325) */
+typedef enum { SA_EMIT_GAUSS_DESCALED = 0, SA_EMIT_GAUSS = 1, SA_EMIT_HDP = 2 } sa_emission_kind;
+
+/* Transition slots (natural-log probabilities) in the field order of struct
+ * _StateMachine3 (inc/stateMachine.h:141-160). */
+enum {
+    SA_T_MATCH_CONTINUE = 0, SA_T_MATCH_FROM_GAP_X, SA_T_MATCH_FROM_GAP_Y,
+    SA_T_GAP_OPEN_X, SA_T_GAP_OPEN_Y, SA_T_GAP_EXTEND_X, SA_T_GAP_EXTEND_Y,
+    SA_T_GAP_SWITCH_TO_X, SA_T_GAP_SWITCH_TO_Y
+};
+
+/* Mirrors PairwiseAlignmentParameters (inc/pairwiseAligner.h:123-134): the
+ * fields the banded forward-backward reads. */
+typedef struct {
+    double threshold;                      /* minimum posterior match probability kept */
+    int64_t min_diags_between_traceback;   /* 1000 */
+    int64_t traceback_diagonals;           /* 40 */
+    int64_t diagonal_expansion;            /* even */
+    int64_t split_matrix_bigger_than_this; /* 3000*3000 */
+} sa_params;
+
+/* Defaults of pairwiseAlignmentBandingParameters_construct (impl/pairwiseAligner.c:1843-1856). */
+void sa_params_default(sa_params *p);
+
+/* Model tables.  Gaussian kinds: match and gapy hold 5 doubles per k-mer
+ * (level_mean, level_sd, noise_mean, noise_sd, noise_lambda) indexed by kmer_id over
+ * the sorted alphabet; gapx holds the log skip emission per k-mer (NULL = log 0.1).
+ * HDP kind: a uniform density grid (grid_length knots from grid[0]), per-row density
+ * and spline slopes, the row of each k-mer's first observed ancestor DP (kmer_row),
+ * and the per-k-mer level mean used for descaling (level_mean). */
+typedef struct {
+    int32_t kind;           /* sa_emission_kind */
+    int32_t kmer_length;
+    const char *alphabet;   /* NUL-terminated, any order, distinct characters */
+    double transitions[9];
+    const double *match;
+    const double *gapy;
+    const double *gapx;
+    int64_t grid_length;
+    const double *grid;
+    int64_t n_rows;
+    const double *density;  /* n_rows * grid_length */
+    const double *slopes;   /* n_rows * grid_length */
+    const int64_t *kmer_row;
+    const double *level_mean;
+} sa_model_desc;
+
+/* One read (one strand) to align, the arguments of getAlignedPairsUsingAnchors
+ * in flat form.  Anchors are 0-based (k-mer index, event index) pairs, strictly
+ * increasing in both coordinates. */
+typedef struct {
+    const char *ref;          /* nucleotides; k-mer i is ref[i .. i+k) */
+    int64_t ref_length;       /* nucleotides: lX = ref_length - (k - 1) k-mers */
+    const double *events;     /* n_events records of event_stride doubles: mean at [0], noise at [1] */
+    int64_t n_events;
+    int64_t event_stride;     /* 4 = NB_EVENT_PARAMS (inc/nanopore.h:4) */
+    const int64_t *anchors;   /* 2 * n_anchors */
+    int64_t n_anchors;
+    double scale, shift, var; /* per-read descaling (StateMachine scale/shift/var) */
+    int32_t ragged_left, ragged_right;
+} sa_read;
+
+/* Aligned pair: the reference tuple (floor(p*1e7), x, y, pathKmer) with the
+ * k-mer pointer replaced by x (the k-mer is ref[x .. x+k)). */
+typedef struct {
+    int32_t p;
+    int32_t x;
+    int32_t y;
+} sa_pair;
+
+typedef struct {
+    const sa_pair *pairs;   /* in the reference's list order (see DESIGN/INTEGRATION docs) */
+    int64_t n_pairs;
+    double log_likelihood;  /* sum over sub-alignments of the total at the final diagonal */
+    int64_t cells;          /* band cells = work units */
+    int64_t n_windows;
+    int64_t n_subalignments;
+} sa_read_result;
+
+typedef struct {
+    double prep_ms;        /* band, schedule, k-mer ids, event constants (GPU) */
+    double forward_ms;     /* forward kernel */
+    double backward_ms;    /* backward + posterior kernel */
+    double collect_ms;     /* pair compaction */
+    double total_ms;       /* whole call on the device, inputs resident to outputs resident */
+    int64_t cells;
+    int64_t forward_launches;
+    int64_t backward_launches;
+} sa_timing;
+
+typedef struct sa_context sa_context;
+typedef struct sa_model sa_model;
+typedef struct sa_result sa_result;
+
+int sa_context_create(int device, sa_context **out);
+void sa_context_destroy(sa_context *ctx);
+int sa_device_count(int *count);
+
+/* Uploads a model's tables to HBM (resident until sa_model_destroy). */
+int sa_model_create(sa_context *ctx, const sa_model_desc *desc, sa_model **out);
+void sa_model_destroy(sa_model *model);
+
+/* Plain per-read call: the drop-in for one getAlignedPairsUsingAnchors call. */
+int sa_align(sa_context *ctx, const sa_model *model, const sa_read *read, const sa_params *params,
+             sa_precision precision, sa_result **out);
+
+/* Batched call: all reads in one length-sorted launch sequence. */
+int sa_align_batch(sa_context *ctx, const sa_model *model, const sa_read *reads, int64_t n_reads,
+                   const sa_params *params, sa_precision precision, sa_result **out);
+
+/* Split form of sa_align_batch for callers that keep inputs resident: create
+ * splits the reads into sub-alignments and uploads the raw inputs to HBM; run does
+ * all device work (band, schedule, forward, backward, pairs) and copies the pairs
+ * back; a batch can be run any number of times. */
+typedef struct sa_batch sa_batch;
+int sa_batch_create(sa_context *ctx, const sa_model *model, const sa_read *reads, int64_t n_reads,
+                    const sa_params *params, sa_batch **out);
+int sa_batch_run(sa_batch *batch, sa_precision precision, sa_result **out);
+void sa_batch_destroy(sa_batch *batch);
+
+int64_t sa_result_n_reads(const sa_result *res);
+int sa_result_read(const sa_result *res, int64_t i, sa_read_result *out);
+int sa_result_timing(const sa_result *res, sa_timing *out);
+void sa_result_free(sa_result *res);
+
+/* Text of the last error on this context (valid until the next call). */
+const char *sa_last_error(const sa_context *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

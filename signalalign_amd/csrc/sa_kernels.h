@@ -1,0 +1,80 @@
+/*
+ * sa_kernels.h -- kernel argument blocks and launchers (internal to the library).
+ */
+#ifndef SA_KERNELS_H
+#define SA_KERNELS_H
+
+#include "sa_common.h"
+
+namespace sa {
+
+template <typename T>
+struct Model {
+    int32_t kind;          /* sa_emission_kind */
+    int32_t kmer_length;
+    int32_t alphabet_size;
+    int32_t n_kmers;
+    /* Gaussian: per-k-mer constants of the match (m*) and extra-event (y*) tables:
+     * mu = level_mean, isd = 1/level_sd (0 when sd == 0), ca = -log(sqrt(2 pi)) - log(sd)
+     * + (log(lambda) - log(2 pi))/2 (-inf when sd == 0), nmu = noise_mean,
+     * nb = lambda / (2 noise_mean^2); gx = log skip emission. */
+    const T *mmu, *misd, *mca, *mnmu, *mnb;
+    const T *ymu, *yisd, *yca, *ynmu, *ynb;
+    const T *gx;
+    /* HDP: per-k-mer level mean and density row; grid knots and interleaved
+     * (density, slope) pairs per row. */
+    const T *hmu;
+    const int32_t *hrow;
+    const T *gridx;
+    const T *ys;
+    int32_t grid_length;
+    T inv_dx;
+    T trans[9];
+    T endv[3];
+    T rendv[3];
+};
+
+template <typename T>
+struct Args {
+    Model<T> m;
+    const TaskDev *tasks;
+    const WinDev *wins;
+    const int32_t *task_order;
+    const int32_t *win_order;
+    /* raw inputs, packed for the batch */
+    const char *ref;
+    const double *events;
+    int64_t event_stride;
+    const int32_t *anc_x;   /* matrix coordinates (x+1) */
+    const int32_t *anc_y;
+    const int8_t *lut;      /* character -> sorted-alphabet index, -1 outside */
+    /* derived per position / event / diagonal */
+    int32_t *kid;
+    T *evm, *evn, *evl, *evi;
+    int32_t *xlo, *xhi, *doff;
+    double *foff;
+    int64_t *task_cells;
+    int32_t *task_maxw;
+    int32_t *task_nwin;
+    WinDev *win_slots;
+    int32_t *err;
+    /* forward planes */
+    T *fM, *fX, *fY;
+    /* window outputs: 3 int32 (p, x, y) per pair */
+    int32_t *out;
+    int32_t *win_count;
+    double *win_total;
+    double threshold;
+    int64_t min_diags, traceback, expansion;
+    int32_t ring;
+};
+
+template <typename T> hipError_t launch_prep(const Args<T> &a, int n_tasks, hipStream_t st);
+template <typename T> hipError_t launch_forward(const Args<T> &a, int n_tasks, int nt, int cpt, hipStream_t st);
+template <typename T> hipError_t launch_backward(const Args<T> &a, int n_wins, int cpt, hipStream_t st);
+hipError_t launch_collect(const WinDev *wins, const int32_t *win_count, const int64_t *win_dst, const int32_t *src,
+                          int32_t *dst, const int32_t *task_x1, const int32_t *task_y1, int n_wins, hipStream_t st);
+
+} // namespace sa
+
+#endif

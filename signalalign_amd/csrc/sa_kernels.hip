@@ -1,0 +1,682 @@
+/*
+ * sa_kernels.hip -- gfx950 kernels of the banded signal-to-reference pair-HMM
+ * forward-backward (signalAlign's getAlignedPairsUsingAnchors path).
+ *
+ * Kernels, in launch order:
+ *   prep_kernel      one workgroup per sub-alignment: k-mer ids, per-event
+ *                    constants, band geometry (band_construct,
+ *                    impl/pairwiseAligner.c:131-182), cell counts and the traceback
+ *                    window schedule (getPosteriorProbsWithBanding, :1319-1410).
+ *   forward_kernel   one workgroup per sub-alignment sweeps its band anti-diagonal
+ *                    by anti-diagonal; the three states of the last two diagonals
+ *                    live in an LDS ring indexed by x; every diagonal's cells are
+ *                    streamed to HBM for the posterior pass.
+ *   backward_kernel  one wavefront per traceback window: the windowed backward
+ *                    pass (re-initialised with the end-state values at the window
+ *                    top), the every-10th-diagonal total refresh with the
+ *                    match-flux term (:1182-1200, :1370-1380), and the thresholded
+ *                    posterior pairs (:1202-1253) compacted with wave ballots.
+ *   collect_kernel   moves each window's pairs to their place in the reference's
+ *                    list order (alignedPairCoordinateCorrectionFn, :1862-1869).
+ *
+ * Cell recurrence: stateMachine3_cellCalculate (impl/stateMachine.c:1194-1257) in
+ * pull form; each state accumulates its terms in the order the reference's
+ * doTransitionForward/Backward calls reach it, with the lookup log-add of
+ * impl/pairwiseAligner.c:237-254.  No MFMA: the recurrence is not a contraction.
+ */
+#include "sa_kernels.h"
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+namespace sa {
+
+/* ------------------------------------------------------------------------ */
+/* numerics                                                                  */
+/* ------------------------------------------------------------------------ */
+
+template <typename T> __device__ __forceinline__ T ninf();
+template <> __device__ __forceinline__ float ninf<float>() { return -__builtin_huge_valf(); }
+template <> __device__ __forceinline__ double ninf<double>() { return -__builtin_huge_val(); }
+
+__device__ __forceinline__ double poly3(double d, double c0, double c1, double c2, double c3) {
+#pragma clang fp contract(off)
+    return ((c0 * d + c1) * d + c2) * d + c3;
+}
+__device__ __forceinline__ float poly3(float d, float c0, float c1, float c2, float c3) {
+    return __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(c0, d, c1), d, c2), d, c3);
+}
+
+/* logAdd (impl/pairwiseAligner.c:237-254): lo + lookup(hi - lo) while hi - lo < 7.5,
+ * else hi.  A -inf operand makes the difference inf or NaN, which selects hi. */
+template <typename T>
+__device__ __forceinline__ T logadd(T a, T b) {
+    T hi = a > b ? a : b;
+    T lo = a > b ? b : a;
+    T d = hi - lo;
+    const bool p1 = d <= (T)1.00f, p2 = d <= (T)2.50f, p3 = d <= (T)4.50f;
+    T c0 = p1 ? (T)-0.009350833524763f : p2 ? (T)-0.014532321752540f : p3 ? (T)-0.004605031767994f : (T)-0.000458661602210f;
+    T c1 = p1 ? (T)0.130659527668286f : p2 ? (T)0.139942324101744f : p3 ? (T)0.063427417320019f : (T)0.009695946122598f;
+    T c2 = p1 ? (T)0.498799810682272f : p2 ? (T)0.495635523139337f : p3 ? (T)0.695956496475118f : (T)0.930734667215156f;
+    T c3 = p1 ? (T)0.693203116424741f : p2 ? (T)0.692140569840976f : p3 ? (T)0.514272634594009f : (T)0.168037164329057f;
+    T r = poly3(d, c0, c1, c2, c3) + lo;
+    return d < (T)7.5 ? r : hi;
+}
+
+__device__ __forceinline__ double flog(double v) { return log(v); }
+__device__ __forceinline__ float flog(float v) { return logf(v); }
+__device__ __forceinline__ double fexp(double v) { return exp(v); }
+__device__ __forceinline__ float fexp(float v) { return expf(v); }
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        T u = __shfl_xor(v, o, 64);
+        v = u > v ? u : v;
+    }
+    return v;
+}
+
+/* ------------------------------------------------------------------------ */
+/* emissions                                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* emissions_signal_strawManGetKmerEventMatchProb[WithDescaling]
+ * (impl/stateMachine.c:498-589): log N(descaled mean; mu, sd) + log IG(noise; nmu, lambda),
+ * rearranged with per-k-mer and per-event constants:
+ *   a = (evm - s*mu) / sd,  evm = (mean - shift)/var, s = scale/var (1, 0, 1 undescaled)
+ *   e = ca - a^2/2 + evl - nb (noise - nmu)^2 / noise,  evl = -1.5 log(noise). */
+template <typename T>
+__device__ __forceinline__ T gauss_emit(T mu, T isd, T ca, T nmu, T nb, T evm, T evn, T evl, T evi, T s) {
+    T a = (evm - s * mu) * isd;
+    T u = evn - nmu;
+    return (ca - (T)0.5 * a * a) + evl - nb * (u * u * evi);
+}
+
+/* emissions_signal_getHdpKmerDensity (impl/stateMachine.c:468-495) ->
+ * dir_proc_density (impl/hdp.c:2583-2607) -> grid_spline_interp
+ * (impl/hdp_math_utils.c:471-496): Hermite cubic on the row's uniform grid with linear
+ * extrapolation, clamped at 0, divided by var, in log space. */
+template <typename T>
+__device__ __forceinline__ T hdp_emit(const Model<T> &m, int32_t kid, T evm, T s, T inv_var) {
+    T u = evm + m.hmu[kid] * ((T)1 - s);
+    const int32_t G = m.grid_length;
+    const T *ys = m.ys + (int64_t)m.hrow[kid] * 2 * G;
+    const T x0 = m.gridx[0], xn = m.gridx[G - 1];
+    T interp;
+    if (u <= x0) {
+        interp = ys[0] - ys[1] * (x0 - u);
+    } else if (u >= xn) {
+        interp = ys[2 * (G - 1)] + ys[2 * (G - 1) + 1] * (u - xn);
+    } else {
+        int32_t il = (int32_t)((u - x0) * m.inv_dx);
+        il = il > G - 2 ? G - 2 : il;
+        T y0 = ys[2 * il], s0 = ys[2 * il + 1], y1 = ys[2 * il + 2], s1 = ys[2 * il + 3];
+        T dx = m.gridx[1] - m.gridx[0];
+        T dy = y1 - y0;
+        T a = s0 * dx - dy;
+        T b = dy - s1 * dx;
+        T tl = (u - m.gridx[il]) * m.inv_dx;
+        T tr = (T)1 - tl;
+        interp = tr * y0 + tl * y1 + tl * tr * (a * tr + b * tl);
+    }
+    T dens = interp > (T)0 ? interp : (T)0;
+    return flog(inv_var * dens);
+}
+
+template <typename T>
+struct Emit {
+    /* match and extra-event emissions of k-mer `kid` against event j */
+    __device__ __forceinline__ static void eval(const Model<T> &m, const Args<T> &a, int32_t kid, int64_t j, T s,
+                                                T inv_var, T &em, T &ey) {
+        T evm = a.evm[j];
+        if (m.kind == 2) {
+            em = hdp_emit(m, kid, evm, s, inv_var);
+            ey = em;
+            return;
+        }
+        T evn = a.evn[j], evl = a.evl[j], evi = a.evi[j];
+        em = gauss_emit(m.mmu[kid], m.misd[kid], m.mca[kid], m.mnmu[kid], m.mnb[kid], evm, evn, evl, evi, s);
+        ey = gauss_emit(m.ymu[kid], m.yisd[kid], m.yca[kid], m.ynmu[kid], m.ynb[kid], evm, evn, evl, evi, s);
+    }
+    __device__ __forceinline__ static T match(const Model<T> &m, const Args<T> &a, int32_t kid, int64_t j, T s,
+                                              T inv_var) {
+        T evm = a.evm[j];
+        if (m.kind == 2) return hdp_emit(m, kid, evm, s, inv_var);
+        return gauss_emit(m.mmu[kid], m.misd[kid], m.mca[kid], m.mnmu[kid], m.mnb[kid], evm, a.evn[j], a.evl[j],
+                          a.evi[j], s);
+    }
+    __device__ __forceinline__ static T gapy(const Model<T> &m, const Args<T> &a, int32_t kid, int64_t j, T s,
+                                             T inv_var) {
+        T evm = a.evm[j];
+        if (m.kind == 2) return hdp_emit(m, kid, evm, s, inv_var);
+        return gauss_emit(m.ymu[kid], m.yisd[kid], m.yca[kid], m.ynmu[kid], m.ynb[kid], evm, a.evn[j], a.evl[j],
+                          a.evi[j], s);
+    }
+    /* gapX: emissions_kmer_getGapProb (impl/stateMachine.c:155-172) or the HDP
+     * machine's constant log(0.1) (:1282) */
+    __device__ __forceinline__ static T gapx(const Model<T> &m, int32_t kid) {
+        if (m.kind == 2) return (T)-2.3025850929940455;
+        return m.gx[kid];
+    }
+};
+
+/* ------------------------------------------------------------------------ */
+/* prep: k-mer ids, event constants, band, cell counts, windows             */
+/* ------------------------------------------------------------------------ */
+
+template <typename T>
+__global__ void __launch_bounds__(256) prep_kernel(Args<T> a) {
+    const int ti = blockIdx.x;
+    const TaskDev t = a.tasks[ti];
+    const int tid = threadIdx.x;
+    const int nt = blockDim.x;
+    const int k = a.m.kmer_length;
+    const int A = a.m.alphabet_size;
+    __shared__ long long s_cells;
+    __shared__ int s_maxw;
+    __shared__ int s_err;
+    if (tid == 0) { s_cells = 0; s_maxw = 0; s_err = 0; }
+    __syncthreads();
+    int err = 0;
+    for (int p = tid; p < t.lX; p += nt) {
+        int32_t id = 0;
+        bool bad = false;
+        for (int j = 0; j < k; j++) {
+            int c = a.lut[(uint8_t)a.ref[t.ref_off + p + j]];
+            bad |= c < 0;
+            id = id * A + (c < 0 ? 0 : c);
+        }
+        a.kid[t.pos_off + p] = bad ? 0 : id;
+        if (bad) err |= ERR_BAD_KMER;
+    }
+    const double inv_var = 1.0 / t.var;
+    for (int j = tid; j < t.lY; j += nt) {
+        const double *ev = a.events + (t.ev_off + j) * a.event_stride;
+        double mean = ev[0], noise = ev[1];
+        double evm = a.m.kind == 1 ? mean : (mean - t.shift) * inv_var;
+        a.evm[t.evc_off + j] = (T)evm;
+        a.evn[t.evc_off + j] = (T)noise;
+        a.evl[t.evc_off + j] = (T)(-1.5 * log(noise));
+        a.evi[t.evc_off + j] = (T)(1.0 / noise);
+    }
+    const int64_t N = (int64_t)t.lX + t.lY;
+    long long cells = 0;
+    int maxw = 0;
+    for (int64_t d = tid; d <= N; d += nt) {
+        int64_t L = 0, R = 0;
+        bool ok = band_at(a.anc_x + t.anc_off, a.anc_y + t.anc_off, t.n_anc, t.lX, t.lY, a.expansion, d, &L, &R);
+        if (!ok) { err |= ERR_BAND; L = -d; R = -d; }
+        int32_t lo = (int32_t)div2(d + L), hi = (int32_t)div2(d + R);
+        a.xlo[t.diag_off + d] = lo;
+        a.xhi[t.diag_off + d] = hi;
+        int w = hi - lo + 1;
+        cells += w;
+        maxw = w > maxw ? w : maxw;
+    }
+    atomicAdd((unsigned long long *)&s_cells, (unsigned long long)cells);
+    atomicMax(&s_maxw, maxw);
+    if (err) atomicOr(&s_err, err);
+    __syncthreads();
+    if (tid == 0) {
+        const int32_t *xlo = a.xlo + t.diag_off;
+        const int32_t *xhi = a.xhi + t.diag_off;
+        auto width = [&](int64_t d) { return (int64_t)(xhi[d] - xlo[d] + 1); };
+        int32_t nw = window_schedule(N, a.min_diags, a.traceback, a.expansion, width, a.win_slots + t.win_off,
+                                     t.win_cap, ti);
+        if (nw < 0) { s_err |= ERR_WINDOWS; nw = 0; }
+        a.task_nwin[ti] = nw;
+        a.task_cells[ti] = s_cells;
+        a.task_maxw[ti] = s_maxw;
+        if (s_err) atomicOr(a.err, s_err);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* forward                                                                   */
+/* ------------------------------------------------------------------------ */
+
+template <typename T, int NT, int CPT>
+__global__ void __launch_bounds__(NT) forward_kernel(Args<T> a) {
+    constexpr bool NORM = sizeof(T) == 4;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int R = a.ring, RM = R - 1;
+    T *ring = reinterpret_cast<T *>(smem);               /* [3 diag][3 state][R] */
+    T *red = ring + 9 * R;                              /* NT/64 wave maxima */
+    const int ti = a.task_order[blockIdx.x];
+    const TaskDev t = a.tasks[ti];
+    const Model<T> &m = a.m;
+    const int tid = threadIdx.x;
+    const int64_t N = (int64_t)t.lX + t.lY;
+    const int32_t *xlo = a.xlo + t.diag_off;
+    const int32_t *xhi = a.xhi + t.diag_off;
+    int32_t *doff = a.doff + t.diag_off;
+    double *foff = a.foff + t.diag_off;
+    const int64_t base = t.cell_off;
+    const int32_t *kid = a.kid + t.pos_off;
+    const int64_t evc = t.evc_off;
+    const T s = (T)(m.kind == 1 ? 1.0 : t.scale / t.var);
+    const T inv_var = (T)(1.0 / t.var);
+    const T tMC = m.trans[0], tMFX = m.trans[1], tMFY = m.trans[2], tGOX = m.trans[3], tGOY = m.trans[4];
+    const T tGEX = m.trans[5], tGEY = m.trans[6], tSWX = m.trans[7];
+    const T NI = ninf<T>();
+
+    /* diagonal 0: start or ragged-start state values (impl/stateMachine.c:1022-1031) */
+    {
+        const int lo = xlo[0], w = xhi[0] - lo + 1;
+        for (int i = tid; i < w; i += NT) {
+            const int x = lo + i;
+            T v0 = t.ragged_left ? NI : (T)0, v12 = t.ragged_left ? (T)0 : NI;
+            ring[0 * R + (x & RM)] = v0;
+            ring[1 * R + (x & RM)] = v12;
+            ring[2 * R + (x & RM)] = v12;
+            a.fM[base + i] = v0;
+            a.fX[base + i] = v12;
+            a.fY[base + i] = v12;
+        }
+        if (tid == 0) { doff[0] = 0; foff[0] = 0.0; }
+    }
+    int64_t run = xhi[0] - xlo[0] + 1;
+    double off1 = 0.0, off2 = 0.0;
+    __syncthreads();
+
+    for (int64_t d = 1; d <= N; d++) {
+        const int lo = xlo[d], hi = xhi[d], w = hi - lo + 1;
+        const int lo1 = xlo[d - 1], hi1 = xhi[d - 1];
+        const int lo2 = d >= 2 ? xlo[d - 2] : 1, hi2 = d >= 2 ? xhi[d - 2] : 0;
+        const int c = (int)(d % 3), c1 = (int)((d + 2) % 3), c2 = (int)((d + 1) % 3);
+        T *rc = ring + c * 3 * R, *r1 = ring + c1 * 3 * R, *r2 = ring + c2 * 3 * R;
+        const T dl2 = NORM ? (T)(off2 - off1) : (T)0;
+        T vM[CPT], vX[CPT], vY[CPT];
+        T mx = NI;
+#pragma unroll
+        for (int q = 0; q < CPT; q++) {
+            const int i = tid + q * NT;
+            vM[q] = NI; vX[q] = NI; vY[q] = NI;
+            if (i < w) {
+                const int x = lo + i;
+                const int64_t y = d - x;
+                const bool hasL = x - 1 >= lo1 && x - 1 <= hi1;
+                const bool hasU = x >= lo1 && x <= hi1;
+                const bool hasM = x - 1 >= lo2 && x - 1 <= hi2;
+                const T LM = hasL ? r1[(x - 1) & RM] : NI, LX = hasL ? r1[R + ((x - 1) & RM)] : NI,
+                        LY = hasL ? r1[2 * R + ((x - 1) & RM)] : NI;
+                const T UM = hasU ? r1[x & RM] : NI, UY = hasU ? r1[2 * R + (x & RM)] : NI;
+                T MM = hasM ? r2[(x - 1) & RM] : NI, MX = hasM ? r2[R + ((x - 1) & RM)] : NI,
+                  MY = hasM ? r2[2 * R + ((x - 1) & RM)] : NI;
+                if (NORM) { MM += dl2; MX += dl2; MY += dl2; }
+                const bool kx = x >= 1, ky = y >= 1;
+                const int32_t kk = kid[kx ? x - 1 : 0];
+                const int64_t jj = evc + (ky ? y - 1 : 0);
+                T eX = kx ? Emit<T>::gapx(m, kk) : NI;
+                T eM = NI, eY = NI;
+                if (kx && ky && (hasM || hasU)) Emit<T>::eval(m, a, kk, jj, s, inv_var, eM, eY);
+                /* lower -> gapX, middle -> match, upper -> gapY (impl/stateMachine.c:1209-1256) */
+                T X = LM + (eX + tGOX);
+                X = logadd(X, LX + (eX + tGEX));
+                X = logadd(X, LY + (eX + tSWX));
+                T M = MM + (eM + tMC);
+                M = logadd(M, MX + (eM + tMFX));
+                M = logadd(M, MY + (eM + tMFY));
+                T Y = UM + (eY + tGOY);
+                Y = logadd(Y, UY + (eY + tGEY));
+                vM[q] = M; vX[q] = X; vY[q] = Y;
+                if (NORM) {
+                    T m3 = M > X ? M : X;
+                    m3 = m3 > Y ? m3 : Y;
+                    mx = m3 > mx ? m3 : mx;
+                }
+            }
+        }
+        double offd = 0.0;
+        if (NORM) {
+            mx = wave_max(mx);
+            if (NT > 64) {
+                if ((tid & 63) == 0) red[tid >> 6] = mx;
+                __syncthreads();
+                mx = red[0];
+#pragma unroll
+                for (int q = 1; q < NT / 64; q++) mx = red[q] > mx ? red[q] : mx;
+            }
+            const T sh = mx > NI ? mx : (T)0;
+#pragma unroll
+            for (int q = 0; q < CPT; q++) { vM[q] -= sh; vX[q] -= sh; vY[q] -= sh; }
+            offd = off1 + (double)sh;
+        }
+#pragma unroll
+        for (int q = 0; q < CPT; q++) {
+            const int i = tid + q * NT;
+            if (i < w) {
+                const int x = lo + i;
+                rc[x & RM] = vM[q];
+                rc[R + (x & RM)] = vX[q];
+                rc[2 * R + (x & RM)] = vY[q];
+                a.fM[base + run + i] = vM[q];
+                a.fX[base + run + i] = vX[q];
+                a.fY[base + run + i] = vY[q];
+            }
+        }
+        if (tid == 0) {
+            doff[d] = (int32_t)run;
+            foff[d] = offd;
+        }
+        run += w;
+        off2 = off1;
+        off1 = offd;
+        __syncthreads();
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* backward + total refresh + posterior pairs, one wavefront per window     */
+/* ------------------------------------------------------------------------ */
+
+/* Sequential log-add chain of the reference's dpDiagonal_dotProduct
+ * (impl/pairwiseAligner.c:1014-1027).  Terms at least 7.5 below the running
+ * prefix maximum are exactly no-ops (logAdd returns its accumulator) and are
+ * skipped without changing the result. */
+__device__ __forceinline__ double chain(const double *term, int n) {
+    double acc = -__builtin_huge_val();
+    double pm = -__builtin_huge_val();
+    for (int i = 0; i < n; i++) {
+        const double v = term[i];
+        if (!(v > pm - 7.5)) continue;
+        acc = logadd(acc, v);
+        pm = v > pm ? v : pm;
+    }
+    return acc;
+}
+
+template <typename T, int CPT>
+__global__ void __launch_bounds__(64) backward_kernel(Args<T> a) {
+    constexpr bool NORM = sizeof(T) == 4;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int R = a.ring, RM = R - 1;
+    double *termA = reinterpret_cast<double *>(smem);   /* R */
+    double *termB = termA + R;                          /* R */
+    double *bcast = termB + R;                          /* 2 */
+    T *ring = reinterpret_cast<T *>(bcast + 2);         /* [3 diag][3 state][R] */
+    const int wi = a.win_order[blockIdx.x];
+    const WinDev W = a.wins[wi];
+    const TaskDev t = a.tasks[W.task];
+    const Model<T> &m = a.m;
+    const int lane = threadIdx.x;
+    const int32_t *xlo = a.xlo + t.diag_off;
+    const int32_t *xhi = a.xhi + t.diag_off;
+    const int32_t *doff = a.doff + t.diag_off;
+    const double *foff = a.foff + t.diag_off;
+    const int64_t base = t.cell_off;
+    const int32_t *kid = a.kid + t.pos_off;
+    const int64_t evc = t.evc_off;
+    const T s = (T)(m.kind == 1 ? 1.0 : t.scale / t.var);
+    const T inv_var = (T)(1.0 / t.var);
+    const T tMC = m.trans[0], tMFX = m.trans[1], tMFY = m.trans[2], tGOX = m.trans[3], tGOY = m.trans[4];
+    const T tGEX = m.trans[5], tGEY = m.trans[6], tSWX = m.trans[7];
+    const T NI = ninf<T>();
+    const T thr = (T)a.threshold;
+    const int D = W.D, TF = W.TF, TB = W.T;
+    const bool ragged_end = W.last && t.ragged_right;
+    const T e0 = ragged_end ? m.rendv[0] : m.endv[0];
+    const T e1 = ragged_end ? m.rendv[1] : m.endv[1];
+    const T e2 = ragged_end ? m.rendv[2] : m.endv[2];
+    int32_t *out = a.out + 3 * W.out_off;
+    int cursor = 0;
+    int posterior_count = 0;
+    double total = -__builtin_huge_val();
+    double ob1 = 0.0, ob2 = 0.0; /* offsets of B[e+1], B[e+2] */
+
+    for (int e = D; e > TB; e--) {
+        const int lo = xlo[e], hi = xhi[e], w = hi - lo + 1;
+        const bool h1 = e + 1 <= D, h2 = e + 2 <= D;
+        const int lo1 = h1 ? xlo[e + 1] : 1, hi1 = h1 ? xhi[e + 1] : 0;
+        const int lo2 = h2 ? xlo[e + 2] : 1, hi2 = h2 ? xhi[e + 2] : 0;
+        const int c = e % 3, c1 = (e + 1) % 3, c2 = (e + 2) % 3;
+        T *rc = ring + c * 3 * R, *r1 = ring + c1 * 3 * R, *r2 = ring + c2 * 3 * R;
+        T vM[CPT], vX[CPT], vY[CPT];
+        T mx = NI;
+        if (e == D) {
+#pragma unroll
+            for (int q = 0; q < CPT; q++) { vM[q] = e0; vX[q] = e1; vY[q] = e2; }
+        } else {
+            const T dl2 = NORM ? (T)(ob2 - ob1) : (T)0;
+#pragma unroll
+            for (int q = 0; q < CPT; q++) {
+                const int i = lane + q * 64;
+                vM[q] = NI; vX[q] = NI; vY[q] = NI;
+                if (i < w) {
+                    const int x = lo + i;
+                    const int y = e - x;
+                    const bool hasLow = x + 1 >= lo1 && x + 1 <= hi1; /* C = (x+1, y) on e+1 */
+                    const bool hasUp = x >= lo1 && x <= hi1;          /* C = (x, y+1) on e+1 */
+                    const bool hasMid = x + 1 >= lo2 && x + 1 <= hi2; /* C = (x+1, y+1) on e+2 */
+                    const T ClowX = hasLow ? r1[R + ((x + 1) & RM)] : NI;
+                    const T CupY = hasUp ? r1[2 * R + (x & RM)] : NI;
+                    T CmidM = hasMid ? r2[(x + 1) & RM] : NI;
+                    if (NORM) CmidM += dl2;
+                    const int32_t kx = kid[x < t.lX ? x : 0];
+                    const int32_t kxm1 = kid[x >= 1 ? x - 1 : 0];
+                    const int64_t jj = evc + (y < t.lY ? y : 0);
+                    const T eX = hasLow ? Emit<T>::gapx(m, kx) : NI;
+                    const T eM = hasMid ? Emit<T>::match(m, a, kx, jj, s, inv_var) : NI;
+                    const T eY = (hasUp && x >= 1) ? Emit<T>::gapy(m, a, kxm1, jj, s, inv_var) : NI;
+                    /* contributions arrive middle (from e+2), then upper, then lower */
+                    T M = CmidM + (eM + tMC);
+                    M = logadd(M, CupY + (eY + tGOY));
+                    M = logadd(M, ClowX + (eX + tGOX));
+                    T X = CmidM + (eM + tMFX);
+                    X = logadd(X, ClowX + (eX + tGEX));
+                    T Y = CmidM + (eM + tMFY);
+                    Y = logadd(Y, CupY + (eY + tGEY));
+                    Y = logadd(Y, ClowX + (eX + tSWX));
+                    vM[q] = M; vX[q] = X; vY[q] = Y;
+                    if (NORM) {
+                        T m3 = M > X ? M : X;
+                        m3 = m3 > Y ? m3 : Y;
+                        mx = m3 > mx ? m3 : mx;
+                    }
+                }
+            }
+        }
+        double obe = 0.0;
+        if (NORM && e != D) {
+            mx = wave_max(mx);
+            const T sh = mx > NI ? mx : (T)0;
+#pragma unroll
+            for (int q = 0; q < CPT; q++) { vM[q] -= sh; vX[q] -= sh; vY[q] -= sh; }
+            obe = ob1 + (double)sh;
+        }
+#pragma unroll
+        for (int q = 0; q < CPT; q++) {
+            const int i = lane + q * 64;
+            if (i < w) {
+                const int x = lo + i;
+                rc[x & RM] = vM[q];
+                rc[R + (x & RM)] = vX[q];
+                rc[2 * R + (x & RM)] = vY[q];
+            }
+        }
+        __syncthreads();
+
+        if (e <= TF) {
+            const int64_t fb = base + doff[e];
+            const double offe = NORM ? foff[e] + obe : 0.0;
+            if (posterior_count % 10 == 0) {
+                /* diagonalCalculationTotalProbability (impl/pairwiseAligner.c:1182-1200) */
+                for (int i = lane; i < w; i += 64) {
+                    const int x = lo + i;
+                    T cell = a.fM[fb + i] + rc[x & RM];
+                    cell = logadd(cell, a.fX[fb + i] + rc[R + (x & RM)]);
+                    cell = logadd(cell, a.fY[fb + i] + rc[2 * R + (x & RM)]);
+                    termA[i] = (double)cell + offe;
+                }
+                int w1 = 0;
+                if (h1) {
+                    const int lo1e = xlo[e + 1];
+                    w1 = xhi[e + 1] - lo1e + 1;
+                    const int lom = xlo[e - 1], him = xhi[e - 1];
+                    const int64_t fbm = base + doff[e - 1];
+                    const double offm = NORM ? foff[e - 1] + ob1 : 0.0;
+                    for (int i = lane; i < w1; i += 64) {
+                        const int x = lo1e + i;        /* cell (x, y) on e+1, middle (x-1, y-1) on e-1 */
+                        const int y = e + 1 - x;
+                        const bool hm = x - 1 >= lom && x - 1 <= him && x >= 1 && y >= 1;
+                        T Mv = NI;
+                        if (hm) {
+                            const int64_t fi = fbm + (x - 1 - lom);
+                            const T eM = Emit<T>::match(m, a, kid[x - 1], evc + (y - 1), s, inv_var);
+                            Mv = a.fM[fi] + (eM + tMC);
+                            Mv = logadd(Mv, a.fX[fi] + (eM + tMFX));
+                            Mv = logadd(Mv, a.fY[fi] + (eM + tMFY));
+                        }
+                        termB[i] = (double)(Mv + r1[x & RM]) + offm;
+                    }
+                }
+                __syncthreads();
+                if (lane == 0) {
+                    double tot = chain(termA, w);
+                    if (h1) tot = logadd(tot, chain(termB, w1));
+                    bcast[0] = tot;
+                }
+                __syncthreads();
+                total = bcast[0];
+                if (e == TF && lane == 0) a.win_total[wi] = total;
+            }
+            posterior_count++;
+            /* diagonalCalculationPosteriorMatchProbs (impl/pairwiseAligner.c:1202-1253) */
+            const T cst = NORM ? (T)(offe - total) : (T)0;
+            for (int i0 = 0; i0 < w; i0 += 64) {
+                const int i = i0 + lane;
+                bool keep = false;
+                double p = 0.0;
+                int x = 0, y = 0;
+                if (i < w) {
+                    x = lo + i;
+                    y = e - x;
+                    if (x > 0 && y > 0) {
+                        T pv;
+                        if (NORM) pv = fexp((a.fM[fb + i] + rc[x & RM]) + cst);
+                        else pv = fexp((a.fM[fb + i] + rc[x & RM]) - (T)total);
+                        keep = pv >= thr;
+                        p = (double)pv;
+                    }
+                }
+                const unsigned long long bal = __ballot(keep);
+                if (keep) {
+                    const int idx = cursor + __popcll(bal & ((1ull << lane) - 1ull));
+                    if (idx < W.out_cap) {
+                        if (p > 1.0) p = 1.0;
+                        out[3 * idx + 0] = (int32_t)floor(p * (double)kPairProb1);
+                        out[3 * idx + 1] = x - 1;
+                        out[3 * idx + 2] = y - 1;
+                    }
+                }
+                cursor += __popcll(bal);
+            }
+        }
+        ob2 = ob1;
+        ob1 = obe;
+    }
+    if (lane == 0) a.win_count[wi] = cursor;
+}
+
+/* ------------------------------------------------------------------------ */
+/* collect: reference list order                                             */
+/* ------------------------------------------------------------------------ */
+
+__global__ void __launch_bounds__(256) collect_kernel(const WinDev *wins, const int32_t *win_count,
+                                                      const int64_t *win_dst, const int32_t *src, int32_t *dst,
+                                                      const int32_t *task_x1, const int32_t *task_y1, int n_wins) {
+    const int wi = blockIdx.x;
+    if (wi >= n_wins) return;
+    const WinDev W = wins[wi];
+    const int n = win_count[wi];
+    const int64_t last = win_dst[wi];
+    const int32_t x1 = task_x1[W.task], y1 = task_y1[W.task];
+    const int32_t *s = src + 3 * W.out_off;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        const int64_t o = last - j;
+        dst[3 * o + 0] = s[3 * j + 0];
+        dst[3 * o + 1] = s[3 * j + 1] + x1;
+        dst[3 * o + 2] = s[3 * j + 2] + y1;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* launchers                                                                 */
+/* ------------------------------------------------------------------------ */
+
+template <typename T>
+hipError_t launch_prep(const Args<T> &a, int n_tasks, hipStream_t st) {
+    hipLaunchKernelGGL(prep_kernel<T>, dim3(n_tasks), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+template <typename T, int NT, int CPT>
+static hipError_t fwd_inst(const Args<T> &a, int n, size_t lds, hipStream_t st) {
+    hipLaunchKernelGGL((forward_kernel<T, NT, CPT>), dim3(n), dim3(NT), lds, st, a);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_forward(const Args<T> &a, int n_tasks, int nt, int cpt, hipStream_t st) {
+    const size_t lds = (size_t)9 * a.ring * sizeof(T) + 16 * sizeof(T);
+    if (nt == 64) {
+        switch (cpt) {
+        case 1: return fwd_inst<T, 64, 1>(a, n_tasks, lds, st);
+        case 2: return fwd_inst<T, 64, 2>(a, n_tasks, lds, st);
+        case 3: return fwd_inst<T, 64, 3>(a, n_tasks, lds, st);
+        case 4: return fwd_inst<T, 64, 4>(a, n_tasks, lds, st);
+        default: break;
+        }
+    } else if (nt == 256) {
+        switch (cpt) {
+        case 1: return fwd_inst<T, 256, 1>(a, n_tasks, lds, st);
+        case 2: return fwd_inst<T, 256, 2>(a, n_tasks, lds, st);
+        case 3: return fwd_inst<T, 256, 3>(a, n_tasks, lds, st);
+        case 4: return fwd_inst<T, 256, 4>(a, n_tasks, lds, st);
+        default: break;
+        }
+    }
+    return hipErrorInvalidValue;
+}
+
+template <typename T, int CPT>
+static hipError_t bwd_inst(const Args<T> &a, int n, size_t lds, hipStream_t st) {
+    hipLaunchKernelGGL((backward_kernel<T, CPT>), dim3(n), dim3(64), lds, st, a);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_backward(const Args<T> &a, int n_wins, int cpt, hipStream_t st) {
+    const size_t lds = (size_t)2 * a.ring * sizeof(double) + 2 * sizeof(double) + (size_t)9 * a.ring * sizeof(T);
+    switch (cpt) {
+    case 1: return bwd_inst<T, 1>(a, n_wins, lds, st);
+    case 2: return bwd_inst<T, 2>(a, n_wins, lds, st);
+    case 3: return bwd_inst<T, 3>(a, n_wins, lds, st);
+    case 4: return bwd_inst<T, 4>(a, n_wins, lds, st);
+    case 5: return bwd_inst<T, 5>(a, n_wins, lds, st);
+    case 6: return bwd_inst<T, 6>(a, n_wins, lds, st);
+    case 7: return bwd_inst<T, 7>(a, n_wins, lds, st);
+    case 8: return bwd_inst<T, 8>(a, n_wins, lds, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_collect(const WinDev *wins, const int32_t *win_count, const int64_t *win_dst, const int32_t *src,
+                          int32_t *dst, const int32_t *task_x1, const int32_t *task_y1, int n_wins, hipStream_t st) {
+    if (n_wins == 0) return hipSuccess;
+    hipLaunchKernelGGL(collect_kernel, dim3(n_wins), dim3(256), 0, st, wins, win_count, win_dst, src, dst, task_x1,
+                       task_y1, n_wins);
+    return hipGetLastError();
+}
+
+template hipError_t launch_prep<float>(const Args<float> &, int, hipStream_t);
+template hipError_t launch_prep<double>(const Args<double> &, int, hipStream_t);
+template hipError_t launch_forward<float>(const Args<float> &, int, int, int, hipStream_t);
+template hipError_t launch_forward<double>(const Args<double> &, int, int, int, hipStream_t);
+template hipError_t launch_backward<float>(const Args<float> &, int, int, hipStream_t);
+template hipError_t launch_backward<double>(const Args<double> &, int, int, hipStream_t);
+
+} // namespace sa

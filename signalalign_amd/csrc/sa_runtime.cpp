@@ -1,0 +1,837 @@
+/*
+ * sa_runtime.cpp -- host runtime of libsignalalign_amd: contexts, resident model
+ * tables, batches, split points, launch planning and result assembly behind the
+ * C-ABI of include/signalalign_amd.h.
+ *
+ * The host side does only O(reads + anchors) integer work: the split points of
+ * getSplitPoints (impl/pairwiseAligner.c:1706-1757), the sub-alignment anchor lists
+ * of getPosteriorProbsWithBandingSplittingAlignmentsByLargeGaps (:1773-1837) and
+ * the launch plan.  Everything proportional to diagonals or cells runs on the GPU.
+ */
+#include "signalalign_amd.h"
+
+#include "sa_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+using sa::TaskDev;
+using sa::WinDev;
+
+namespace {
+
+struct DBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap && p) return hipSuccess;
+        if (p) { (void)hipFree(p); p = nullptr; cap = 0; }
+        size_t want = std::max<size_t>(bytes, 256);
+        hipError_t e = hipMalloc(&p, want);
+        if (e == hipSuccess) cap = want;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <typename U> U *as() const { return reinterpret_cast<U *>(p); }
+};
+
+template <typename T>
+struct DevTables {
+    DBuf mmu, misd, mca, mnmu, mnb, ymu, yisd, yca, ynmu, ynb, gx, hmu, hrow, gridx, ys;
+    void release() {
+        DBuf *all[] = {&mmu, &misd, &mca, &mnmu, &mnb, &ymu, &yisd, &yca, &ynmu, &ynb, &gx, &hmu, &hrow, &gridx, &ys};
+        for (DBuf *b : all) b->release();
+    }
+};
+
+} // namespace
+
+struct sa_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::string err;
+    hipEvent_t ev[8] = {};
+    int n_cu = 0;
+};
+
+struct sa_model {
+    sa_context *ctx = nullptr;
+    int32_t kind = 0;
+    int32_t k = 0;
+    int32_t A = 0;
+    int32_t n_kmers = 0;
+    std::string alphabet;
+    int8_t lut[256];
+    double trans[9];
+    double endv[3], rendv[3];
+    int32_t grid_length = 0;
+    double inv_dx = 0.0;
+    DBuf d_lut;
+    DevTables<double> t64;
+    DevTables<float> t32;
+};
+
+struct ReadPlan {
+    int64_t first_task = 0;
+    int64_t n_tasks = 0;
+};
+
+struct TaskHost {
+    int32_t read;
+    int32_t x1, y1;
+};
+
+struct sa_batch {
+    sa_context *ctx = nullptr;
+    const sa_model *model = nullptr;
+    sa_params params;
+    int64_t n_reads = 0;
+    std::vector<ReadPlan> reads;
+    std::vector<TaskDev> tasks;
+    std::vector<TaskHost> thost;
+    int64_t total_pos = 0, total_ev = 0, total_diag = 0, total_wslots = 0, total_anc = 0;
+    int64_t ref_bytes = 0, ev_records = 0;
+    int64_t stride = 4;
+    /* raw inputs resident in HBM */
+    DBuf d_ref, d_events, d_ancx, d_ancy, d_tasks, d_x1, d_y1;
+    /* derived */
+    DBuf d_kid, d_evm, d_evn, d_evl, d_evi, d_xlo, d_xhi, d_doff, d_foff;
+    DBuf d_task_cells, d_task_maxw, d_task_nwin, d_win_slots, d_err;
+    DBuf d_fM, d_fX, d_fY;
+    DBuf d_wins, d_task_order, d_win_order, d_out, d_win_count, d_win_total, d_win_dst, d_packed;
+    ~sa_batch() {
+        DBuf *all[] = {&d_ref, &d_events, &d_ancx, &d_ancy, &d_tasks, &d_x1, &d_y1, &d_kid, &d_evm, &d_evn,
+                       &d_evl, &d_evi, &d_xlo, &d_xhi, &d_doff, &d_foff, &d_task_cells, &d_task_maxw,
+                       &d_task_nwin, &d_win_slots, &d_err, &d_fM, &d_fX, &d_fY, &d_wins, &d_task_order,
+                       &d_win_order, &d_out, &d_win_count, &d_win_total, &d_win_dst, &d_packed};
+        for (DBuf *b : all) b->release();
+    }
+};
+
+struct ReadOut {
+    int64_t first_pair = 0, n_pairs = 0;
+    double log_likelihood = 0.0;
+    int64_t cells = 0, n_windows = 0, n_sub = 0;
+};
+
+struct sa_result {
+    std::vector<sa_pair> pairs;
+    std::vector<ReadOut> reads;
+    sa_timing timing;
+};
+
+namespace {
+
+int fail(sa_context *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define SA_HIP(ctx, call)                                                                                     \
+    do {                                                                                                      \
+        hipError_t _e = (call);                                                                               \
+        if (_e != hipSuccess)                                                                                 \
+            return fail((ctx), _e == hipErrorOutOfMemory ? SA_ERR_OUT_OF_MEMORY : SA_ERR_HIP,                  \
+                        std::string(#call) + ": " + hipGetErrorString(_e));                                   \
+    } while (0)
+
+template <typename T>
+int upload(sa_context *ctx, DBuf &b, const std::vector<T> &v) {
+    SA_HIP(ctx, b.ensure(sizeof(T) * std::max<size_t>(v.size(), 1)));
+    if (!v.empty()) SA_HIP(ctx, hipMemcpy(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    return SA_OK;
+}
+
+template <typename T>
+int upload_tables(sa_context *ctx, sa_model *m, DevTables<T> &t, const sa_model_desc *d) {
+    const int n = m->n_kmers;
+    if (m->kind != SA_EMIT_HDP) {
+        std::vector<T> mu(n), isd(n), ca(n), nmu(n), nb(n);
+        auto fill = [&](const double *tab) {
+            for (int i = 0; i < n; i++) {
+                const double lm = tab[5 * i + 0], sd = tab[5 * i + 1], nm = tab[5 * i + 2], lam = tab[5 * i + 4];
+                mu[i] = (T)lm;
+                isd[i] = sd != 0.0 ? (T)(1.0 / sd) : (T)0;
+                ca[i] = sd != 0.0 ? (T)((-0.91893853320467267 - std::log(sd)) +
+                                        0.5 * (std::log(lam) - 1.8378770664093453))
+                                  : (T)-INFINITY;
+                nmu[i] = (T)nm;
+                nb[i] = (T)(0.5 * lam / (nm * nm));
+            }
+        };
+        fill(d->match);
+        int rc;
+        if ((rc = upload(ctx, t.mmu, mu)) || (rc = upload(ctx, t.misd, isd)) || (rc = upload(ctx, t.mca, ca)) ||
+            (rc = upload(ctx, t.mnmu, nmu)) || (rc = upload(ctx, t.mnb, nb)))
+            return rc;
+        fill(d->gapy ? d->gapy : d->match);
+        if (!d->gapy) /* level_sd x 1.75 as stateMachine3_loadFromFile does */
+            for (int i = 0; i < n; i++) {
+                const double sd = d->match[5 * i + 1] * 1.75, lam = d->match[5 * i + 4];
+                isd[i] = sd != 0.0 ? (T)(1.0 / sd) : (T)0;
+                ca[i] = sd != 0.0 ? (T)((-0.91893853320467267 - std::log(sd)) +
+                                        0.5 * (std::log(lam) - 1.8378770664093453))
+                                  : (T)-INFINITY;
+            }
+        if ((rc = upload(ctx, t.ymu, mu)) || (rc = upload(ctx, t.yisd, isd)) || (rc = upload(ctx, t.yca, ca)) ||
+            (rc = upload(ctx, t.ynmu, nmu)) || (rc = upload(ctx, t.ynb, nb)))
+            return rc;
+        std::vector<T> gx(n);
+        for (int i = 0; i < n; i++) gx[i] = (T)(d->gapx ? d->gapx[i] : -2.3025850929940455);
+        if ((rc = upload(ctx, t.gx, gx))) return rc;
+    } else {
+        std::vector<T> hmu(n);
+        std::vector<int32_t> hrow(n);
+        for (int i = 0; i < n; i++) {
+            hmu[i] = (T)d->level_mean[i];
+            hrow[i] = (int32_t)d->kmer_row[i];
+        }
+        const int64_t G = d->grid_length;
+        std::vector<T> gx(G), ys((size_t)2 * G * d->n_rows);
+        for (int64_t i = 0; i < G; i++) gx[i] = (T)d->grid[i];
+        for (int64_t r = 0; r < d->n_rows; r++)
+            for (int64_t i = 0; i < G; i++) {
+                ys[(size_t)(2 * (r * G + i))] = (T)d->density[r * G + i];
+                ys[(size_t)(2 * (r * G + i) + 1)] = (T)d->slopes[r * G + i];
+            }
+        int rc;
+        if ((rc = upload(ctx, t.hmu, hmu)) || (rc = upload(ctx, t.hrow, hrow)) || (rc = upload(ctx, t.gridx, gx)) ||
+            (rc = upload(ctx, t.ys, ys)))
+            return rc;
+    }
+    return SA_OK;
+}
+
+template <typename T>
+sa::Model<T> device_model(const sa_model *m, const DevTables<T> &t) {
+    sa::Model<T> r;
+    std::memset(&r, 0, sizeof(r));
+    r.kind = m->kind;
+    r.kmer_length = m->k;
+    r.alphabet_size = m->A;
+    r.n_kmers = m->n_kmers;
+    r.mmu = (const T *)t.mmu.p; r.misd = (const T *)t.misd.p; r.mca = (const T *)t.mca.p; r.mnmu = (const T *)t.mnmu.p;
+    r.mnb = (const T *)t.mnb.p;
+    r.ymu = (const T *)t.ymu.p; r.yisd = (const T *)t.yisd.p; r.yca = (const T *)t.yca.p; r.ynmu = (const T *)t.ynmu.p;
+    r.ynb = (const T *)t.ynb.p;
+    r.gx = (const T *)t.gx.p;
+    r.hmu = (const T *)t.hmu.p;
+    r.hrow = (const int32_t *)t.hrow.p;
+    r.gridx = (const T *)t.gridx.p;
+    r.ys = (const T *)t.ys.p;
+    r.grid_length = m->grid_length;
+    r.inv_dx = (T)m->inv_dx;
+    for (int i = 0; i < 9; i++) r.trans[i] = (T)m->trans[i];
+    for (int i = 0; i < 3; i++) { r.endv[i] = (T)m->endv[i]; r.rendv[i] = (T)m->rendv[i]; }
+    return r;
+}
+
+template <typename T> const DevTables<T> &tables(const sa_model *m);
+template <> const DevTables<double> &tables<double>(const sa_model *m) { return m->t64; }
+template <> const DevTables<float> &tables<float>(const sa_model *m) { return m->t32; }
+
+int64_t correct_length(int64_t ref_length, int k) {
+    /* sequence_correctSeqLength (impl/pairwiseAligner.c:456-471) for a k-mer sequence */
+    if (ref_length <= 0) return 0;
+    int64_t l = ref_length - (k - 1);
+    return l > 0 ? l : 0;
+}
+
+/* getSplitPoints (impl/pairwiseAligner.c:1706-1757).  Returns false on anchors the
+ * reference asserts against. */
+bool split_points(const int64_t *anc, int64_t n, int64_t lX, int64_t lY, int64_t split, bool rl, bool rr,
+                  std::vector<int64_t> &out) {
+    out.clear();
+    int64_t x1 = 0, y1 = 0, x2 = 0, y2 = 0;
+    auto P = [&](int64_t x2_, int64_t y2_, int64_t x3, int64_t y3, bool skip) {
+        int64_t lX2 = x3 - x2_, lY2 = y3 - y2_;
+        if (lX2 * lY2 > split) {
+            int64_t ml = (int64_t)std::sqrt((double)split);
+            int64_t hX = lX2 / 2 > ml ? ml : lX2 / 2;
+            int64_t hY = lY2 / 2 > ml ? ml : lY2 / 2;
+            if (!skip) { out.push_back(x1); out.push_back(y1); out.push_back(x2_ + hX); out.push_back(y2_ + hY); }
+            x1 = x3 - hX;
+            y1 = y3 - hY;
+            return true;
+        }
+        return false;
+    };
+    for (int64_t i = 0; i < n; i++) {
+        int64_t x3 = anc[2 * i], y3 = anc[2 * i + 1];
+        P(x2, y2, x3, y3, rl && i == 0);
+        if (x3 < x2 || y3 < y2 || x3 >= lX || y3 >= lY) return false;
+        x2 = x3 + 1;
+        y2 = y3 + 1;
+    }
+    if (!P(x2, y2, lX, lY, rl && n == 0) || !rr) {
+        out.push_back(x1); out.push_back(y1); out.push_back(lX); out.push_back(lY);
+    }
+    return true;
+}
+
+int next_pow2(int v) {
+    int r = 1;
+    while (r < v) r <<= 1;
+    return r;
+}
+
+} // namespace
+
+extern "C" {
+
+void sa_params_default(sa_params *p) {
+    p->threshold = 0.01;
+    p->min_diags_between_traceback = 1000;
+    p->traceback_diagonals = 40;
+    p->diagonal_expansion = 20;
+    p->split_matrix_bigger_than_this = (int64_t)3000 * 3000;
+}
+
+int sa_device_count(int *count) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) { *count = 0; return SA_ERR_NO_DEVICE; }
+    *count = n;
+    return SA_OK;
+}
+
+int sa_context_create(int device, sa_context **out) {
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return SA_ERR_NO_DEVICE;
+    sa_context *c = new sa_context();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+        delete c;
+        return SA_ERR_HIP;
+    }
+    for (auto &e : c->ev) (void)hipEventCreate(&e);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->n_cu = prop.multiProcessorCount;
+    *out = c;
+    return SA_OK;
+}
+
+void sa_context_destroy(sa_context *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    for (auto &e : ctx->ev) if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *sa_last_error(const sa_context *ctx) { return ctx ? ctx->err.c_str() : ""; }
+
+int sa_model_create(sa_context *ctx, const sa_model_desc *d, sa_model **out) {
+    *out = nullptr;
+    if (!ctx || !d || !d->alphabet) return SA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    if (d->kind < 0 || d->kind > 2) return fail(ctx, SA_ERR_UNSUPPORTED, "unknown emission kind");
+    std::string alpha(d->alphabet);
+    std::sort(alpha.begin(), alpha.end()); /* sequence_prepareAlphabet (impl/pairwiseAligner.c:294-323) */
+    if (alpha.empty() || std::adjacent_find(alpha.begin(), alpha.end()) != alpha.end())
+        return fail(ctx, SA_ERR_INVALID_ARGUMENT, "alphabet characters must be distinct");
+    if (d->kmer_length < 1 || d->kmer_length > sa::kMaxKmer)
+        return fail(ctx, SA_ERR_INVALID_ARGUMENT, "kmer_length out of range");
+    double nk = std::pow((double)alpha.size(), (double)d->kmer_length);
+    if (nk > 2.0e8) return fail(ctx, SA_ERR_UNSUPPORTED, "too many k-mers");
+    if (d->kind != SA_EMIT_HDP && !d->match) return fail(ctx, SA_ERR_INVALID_ARGUMENT, "match table missing");
+    if (d->kind == SA_EMIT_HDP &&
+        (!d->grid || d->grid_length < 2 || !d->density || !d->slopes || !d->kmer_row || !d->level_mean ||
+         d->n_rows < 1))
+        return fail(ctx, SA_ERR_INVALID_ARGUMENT, "HDP grid incomplete");
+    sa_model *m = new sa_model();
+    m->ctx = ctx;
+    m->kind = d->kind;
+    m->k = d->kmer_length;
+    m->A = (int32_t)alpha.size();
+    m->n_kmers = (int32_t)nk;
+    m->alphabet = alpha;
+    std::memset(m->lut, -1, sizeof(m->lut));
+    for (size_t i = 0; i < alpha.size(); i++) m->lut[(uint8_t)alpha[i]] = (int8_t)i;
+    for (int i = 0; i < 9; i++) m->trans[i] = d->transitions[i];
+    /* stateMachine3_endStateProb / raggedEndStateProb (impl/stateMachine.c:1033-1061) */
+    m->endv[0] = d->transitions[SA_T_MATCH_CONTINUE];
+    m->endv[1] = d->transitions[SA_T_MATCH_FROM_GAP_X];
+    m->endv[2] = d->transitions[SA_T_MATCH_FROM_GAP_Y];
+    m->rendv[0] = (d->transitions[SA_T_GAP_OPEN_X] + d->transitions[SA_T_GAP_OPEN_Y]) / 2.0;
+    m->rendv[1] = d->transitions[SA_T_GAP_EXTEND_X];
+    m->rendv[2] = d->transitions[SA_T_GAP_EXTEND_Y];
+    if (d->kind == SA_EMIT_HDP) {
+        m->grid_length = (int32_t)d->grid_length;
+        m->inv_dx = 1.0 / (d->grid[1] - d->grid[0]);
+        for (int64_t i = 0; i < m->n_kmers; i++)
+            if (d->kmer_row[i] < 0 || d->kmer_row[i] >= d->n_rows) {
+                delete m;
+                return fail(ctx, SA_ERR_INVALID_ARGUMENT, "kmer_row out of range");
+            }
+    }
+    std::vector<int8_t> lut(m->lut, m->lut + 256);
+    int rc = upload(ctx, m->d_lut, lut);
+    if (!rc) rc = upload_tables<double>(ctx, m, m->t64, d);
+    if (!rc) rc = upload_tables<float>(ctx, m, m->t32, d);
+    if (rc) {
+        m->t64.release(); m->t32.release(); m->d_lut.release();
+        delete m;
+        return rc;
+    }
+    *out = m;
+    return SA_OK;
+}
+
+void sa_model_destroy(sa_model *m) {
+    if (!m) return;
+    (void)hipSetDevice(m->ctx->device);
+    m->t64.release();
+    m->t32.release();
+    m->d_lut.release();
+    delete m;
+}
+
+int sa_batch_create(sa_context *ctx, const sa_model *model, const sa_read *reads, int64_t n_reads,
+                    const sa_params *params, sa_batch **out);
+int sa_batch_run(sa_batch *b, sa_precision precision, sa_result **out);
+void sa_batch_destroy(sa_batch *b);
+
+int sa_batch_create(sa_context *ctx, const sa_model *model, const sa_read *reads, int64_t n_reads,
+                    const sa_params *params, sa_batch **out) {
+    *out = nullptr;
+    if (!ctx || !model || (!reads && n_reads > 0) || n_reads < 0) return SA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    sa_params p;
+    if (params) p = *params; else sa_params_default(&p);
+    /* getPosteriorProbsWithBanding prerequisites (impl/pairwiseAligner.c:1291-1295) */
+    if (!(p.traceback_diagonals >= 1) || !(p.diagonal_expansion >= 0) || p.diagonal_expansion % 2 != 0 ||
+        !(p.min_diags_between_traceback >= 2) || !(p.traceback_diagonals + 1 < p.min_diags_between_traceback) ||
+        !(p.threshold >= 0.0 && p.threshold <= 1.0))
+        return fail(ctx, SA_ERR_INVALID_ARGUMENT, "invalid banding parameters");
+    sa_batch *b = new sa_batch();
+    b->ctx = ctx;
+    b->model = model;
+    b->params = p;
+    b->n_reads = n_reads;
+    b->reads.resize(n_reads);
+    const int k = model->k;
+    std::vector<int64_t> splits;
+    std::vector<int32_t> ancx, ancy;
+    int64_t ref_off = 0, ev_off = 0;
+    int64_t stride = -1;
+    for (int64_t r = 0; r < n_reads; r++) {
+        const sa_read &rd = reads[r];
+        if (rd.event_stride < 2 || (stride >= 0 && rd.event_stride != stride) || rd.n_events < 0 ||
+            rd.ref_length < 0 || rd.n_anchors < 0 || !(rd.var != 0.0)) {
+            delete b;
+            return fail(ctx, SA_ERR_INVALID_ARGUMENT, "read " + std::to_string(r) + ": bad sizes");
+        }
+        stride = rd.event_stride;
+        const int64_t lX = correct_length(rd.ref_length, k), lY = rd.n_events;
+        if (!split_points(rd.anchors, rd.n_anchors, lX, lY, p.split_matrix_bigger_than_this, rd.ragged_left,
+                          rd.ragged_right, splits)) {
+            delete b;
+            return fail(ctx, SA_ERR_ANCHORS, "read " + std::to_string(r) + ": anchors out of order or range");
+        }
+        const int64_t nsplit = (int64_t)splits.size() / 4;
+        b->reads[r].first_task = (int64_t)b->tasks.size();
+        b->reads[r].n_tasks = nsplit;
+        int64_t j = 0;
+        for (int64_t i = 0; i < nsplit; i++) {
+            const int64_t x1 = splits[4 * i], y1 = splits[4 * i + 1], x2 = splits[4 * i + 2], y2 = splits[4 * i + 3];
+            TaskDev t;
+            std::memset(&t, 0, sizeof(t));
+            t.lX = (int32_t)(x2 - x1);
+            t.lY = (int32_t)(y2 - y1);
+            if (x2 - x1 > INT32_MAX / 4 || y2 - y1 > INT32_MAX / 4) {
+                delete b;
+                return fail(ctx, SA_ERR_UNSUPPORTED, "sub-alignment too long");
+            }
+            t.ref_off = ref_off + x1;
+            t.ev_off = ev_off + y1;
+            t.pos_off = b->total_pos;
+            t.evc_off = b->total_ev;
+            t.diag_off = b->total_diag;
+            t.anc_off = (int64_t)ancx.size();
+            int32_t na = 0;
+            int64_t px = -1, py = -1;
+            while (j < rd.n_anchors) {
+                const int64_t x = rd.anchors[2 * j], y = rd.anchors[2 * j + 1];
+                if (x + y >= x2 + y2) break;
+                if (!(x >= x1 && x < x2 && y >= y1 && y < y2) || x - x1 <= px || y - y1 <= py) {
+                    delete b;
+                    return fail(ctx, SA_ERR_ANCHORS, "read " + std::to_string(r) + ": anchors out of order");
+                }
+                px = x - x1;
+                py = y - y1;
+                ancx.push_back((int32_t)(x - x1 + 1));
+                ancy.push_back((int32_t)(y - y1 + 1));
+                na++;
+                j++;
+            }
+            t.n_anc = na;
+            t.ragged_left = rd.ragged_left || i > 0;
+            t.ragged_right = rd.ragged_right || i < nsplit - 1;
+            t.read = (int32_t)r;
+            const int64_t N = (int64_t)t.lX + t.lY;
+            t.win_off = (int32_t)b->total_wslots;
+            t.win_cap = sa::window_cap(N, p.min_diags_between_traceback, p.traceback_diagonals);
+            t.scale = rd.scale;
+            t.shift = rd.shift;
+            t.var = rd.var;
+            b->total_pos += t.lX;
+            b->total_ev += t.lY;
+            b->total_diag += N + 1;
+            b->total_wslots += t.win_cap;
+            if (N > 0) {
+                b->tasks.push_back(t);
+                b->thost.push_back({(int32_t)r, (int32_t)x1, (int32_t)y1});
+            } else {
+                b->reads[r].n_tasks--;
+            }
+        }
+        if (j != rd.n_anchors) {
+            delete b;
+            return fail(ctx, SA_ERR_ANCHORS, "read " + std::to_string(r) + ": anchors not consumed");
+        }
+        ref_off += rd.ref_length;
+        ev_off += rd.n_events;
+    }
+    b->ref_bytes = ref_off;
+    b->ev_records = ev_off;
+    b->total_anc = (int64_t)ancx.size();
+    if (stride < 0) stride = 4;
+    b->stride = stride;
+    /* pack raw inputs into HBM */
+    std::vector<char> ref((size_t)std::max<int64_t>(ref_off, 1));
+    std::vector<double> ev((size_t)std::max<int64_t>(ev_off * stride, 1));
+    int64_t ro = 0, eo = 0;
+    for (int64_t r = 0; r < n_reads; r++) {
+        if (reads[r].ref_length) std::memcpy(ref.data() + ro, reads[r].ref, (size_t)reads[r].ref_length);
+        if (reads[r].n_events)
+            std::memcpy(ev.data() + eo * stride, reads[r].events, sizeof(double) * (size_t)(reads[r].n_events * stride));
+        ro += reads[r].ref_length;
+        eo += reads[r].n_events;
+    }
+    std::vector<int32_t> x1(b->tasks.size()), y1(b->tasks.size());
+    for (size_t i = 0; i < b->tasks.size(); i++) { x1[i] = b->thost[i].x1; y1[i] = b->thost[i].y1; }
+    int rc;
+    if ((rc = upload(ctx, b->d_ref, ref)) || (rc = upload(ctx, b->d_events, ev)) ||
+        (rc = upload(ctx, b->d_ancx, ancx)) || (rc = upload(ctx, b->d_ancy, ancy)) ||
+        (rc = upload(ctx, b->d_tasks, b->tasks)) || (rc = upload(ctx, b->d_x1, x1)) ||
+        (rc = upload(ctx, b->d_y1, y1))) {
+        delete b;
+        return rc;
+    }
+
+    *out = b;
+    return SA_OK;
+}
+
+void sa_batch_destroy(sa_batch *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    delete b;
+}
+
+} // extern "C"
+
+namespace {
+
+template <typename T>
+int run_batch(sa_batch *b, sa_result *res) {
+    sa_context *ctx = b->ctx;
+    const sa_model *model = b->model;
+    hipStream_t st = ctx->stream;
+    const int64_t nt = (int64_t)b->tasks.size();
+    sa_timing &tm = res->timing;
+    std::memset(&tm, 0, sizeof(tm));
+    res->reads.assign(b->n_reads, ReadOut());
+    if (nt == 0) {
+        for (int64_t r = 0; r < b->n_reads; r++) res->reads[r].n_sub = 0;
+        return SA_OK;
+    }
+    /* derived buffers */
+    /* +64 slots: a block with no k-mers (or no events) still reads slot 0 of its range */
+    SA_HIP(ctx, b->d_kid.ensure(sizeof(int32_t) * (b->total_pos + 64)));
+    SA_HIP(ctx, b->d_evm.ensure(sizeof(T) * (b->total_ev + 64)));
+    SA_HIP(ctx, b->d_evn.ensure(sizeof(T) * (b->total_ev + 64)));
+    SA_HIP(ctx, b->d_evl.ensure(sizeof(T) * (b->total_ev + 64)));
+    SA_HIP(ctx, b->d_evi.ensure(sizeof(T) * (b->total_ev + 64)));
+    SA_HIP(ctx, hipMemsetAsync(b->d_kid.p, 0, sizeof(int32_t) * (b->total_pos + 64), st));
+    SA_HIP(ctx, hipMemsetAsync(b->d_evm.p, 0, sizeof(T) * (b->total_ev + 64), st));
+    SA_HIP(ctx, hipMemsetAsync(b->d_evn.p, 0, sizeof(T) * (b->total_ev + 64), st));
+    SA_HIP(ctx, hipMemsetAsync(b->d_evl.p, 0, sizeof(T) * (b->total_ev + 64), st));
+    SA_HIP(ctx, hipMemsetAsync(b->d_evi.p, 0, sizeof(T) * (b->total_ev + 64), st));
+    SA_HIP(ctx, b->d_xlo.ensure(sizeof(int32_t) * b->total_diag));
+    SA_HIP(ctx, b->d_xhi.ensure(sizeof(int32_t) * b->total_diag));
+    SA_HIP(ctx, b->d_doff.ensure(sizeof(int32_t) * b->total_diag));
+    SA_HIP(ctx, b->d_foff.ensure(sizeof(double) * b->total_diag));
+    SA_HIP(ctx, b->d_task_cells.ensure(sizeof(int64_t) * nt));
+    SA_HIP(ctx, b->d_task_maxw.ensure(sizeof(int32_t) * nt));
+    SA_HIP(ctx, b->d_task_nwin.ensure(sizeof(int32_t) * nt));
+    SA_HIP(ctx, b->d_win_slots.ensure(sizeof(WinDev) * std::max<int64_t>(b->total_wslots, 1)));
+    SA_HIP(ctx, b->d_err.ensure(sizeof(int32_t)));
+    SA_HIP(ctx, hipMemsetAsync(b->d_err.p, 0, sizeof(int32_t), st));
+
+    const DevTables<T> &tab = tables<T>(model);
+    sa::Args<T> a;
+    std::memset(&a, 0, sizeof(a));
+    a.m = device_model<T>(model, tab);
+    a.tasks = b->d_tasks.as<TaskDev>();
+    a.ref = b->d_ref.as<char>();
+    a.events = b->d_events.as<double>();
+    a.event_stride = b->stride;
+    a.anc_x = b->d_ancx.as<int32_t>();
+    a.anc_y = b->d_ancy.as<int32_t>();
+    a.lut = model->d_lut.as<int8_t>();
+    a.kid = b->d_kid.as<int32_t>();
+    a.evm = b->d_evm.as<T>(); a.evn = b->d_evn.as<T>(); a.evl = b->d_evl.as<T>(); a.evi = b->d_evi.as<T>();
+    a.xlo = b->d_xlo.as<int32_t>(); a.xhi = b->d_xhi.as<int32_t>(); a.doff = b->d_doff.as<int32_t>();
+    a.foff = b->d_foff.as<double>();
+    a.task_cells = b->d_task_cells.as<int64_t>();
+    a.task_maxw = b->d_task_maxw.as<int32_t>();
+    a.task_nwin = b->d_task_nwin.as<int32_t>();
+    a.win_slots = b->d_win_slots.as<WinDev>();
+    a.err = b->d_err.as<int32_t>();
+    a.threshold = b->params.threshold;
+    a.min_diags = b->params.min_diags_between_traceback;
+    a.traceback = b->params.traceback_diagonals;
+    a.expansion = b->params.diagonal_expansion;
+
+    SA_HIP(ctx, hipEventRecord(ctx->ev[0], st));
+    SA_HIP(ctx, sa::launch_prep<T>(a, (int)nt, st));
+    SA_HIP(ctx, hipEventRecord(ctx->ev[1], st));
+    std::vector<int64_t> cells(nt);
+    std::vector<int32_t> maxw(nt), nwin(nt);
+    std::vector<WinDev> slots(b->total_wslots);
+    int32_t err = 0;
+    SA_HIP(ctx, hipMemcpyAsync(cells.data(), b->d_task_cells.p, sizeof(int64_t) * nt, hipMemcpyDeviceToHost, st));
+    SA_HIP(ctx, hipMemcpyAsync(maxw.data(), b->d_task_maxw.p, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, st));
+    SA_HIP(ctx, hipMemcpyAsync(nwin.data(), b->d_task_nwin.p, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, st));
+    SA_HIP(ctx, hipMemcpyAsync(slots.data(), b->d_win_slots.p, sizeof(WinDev) * b->total_wslots,
+                               hipMemcpyDeviceToHost, st));
+    SA_HIP(ctx, hipMemcpyAsync(&err, b->d_err.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    SA_HIP(ctx, hipStreamSynchronize(st));
+    if (err & sa::ERR_BAD_KMER) return fail(ctx, SA_ERR_BAD_KMER, "reference k-mer with a character outside the alphabet");
+    if (err & sa::ERR_BAND) return fail(ctx, SA_ERR_ANCHORS, "invalid band (anchors outside the matrix or out of order)");
+    if (err & sa::ERR_WINDOWS) return fail(ctx, SA_ERR_HIP, "window schedule overflow");
+
+    /* plan: forward planes, windows, outputs */
+    int32_t gmaxw = 1;
+    int64_t total_cells = 0;
+    std::vector<TaskDev> tasks = b->tasks;
+    for (int64_t i = 0; i < nt; i++) {
+        tasks[i].cell_off = total_cells;
+        total_cells += cells[i];
+        gmaxw = std::max(gmaxw, maxw[i]);
+    }
+    const int ring = next_pow2(gmaxw);
+    const int ring_max = sizeof(T) == 8 ? 512 : 1024;
+    if (ring > ring_max) return fail(ctx, SA_ERR_UNSUPPORTED, "band diagonal wider than " + std::to_string(ring_max));
+    std::vector<WinDev> wins;
+    std::vector<int32_t> task_first_win(nt);
+    int64_t out_total = 0;
+    for (int64_t i = 0; i < nt; i++) {
+        task_first_win[i] = (int32_t)wins.size();
+        for (int32_t w = 0; w < nwin[i]; w++) {
+            WinDev W = slots[tasks[i].win_off + w];
+            W.task = (int32_t)i;
+            W.out_off = out_total;
+            W.out_cap = 2 * (W.TF - W.T) + 64;
+            out_total += W.out_cap;
+            wins.push_back(W);
+        }
+    }
+    const int64_t nw = (int64_t)wins.size();
+    std::vector<int32_t> task_order(nt), win_order(nw);
+    std::iota(task_order.begin(), task_order.end(), 0);
+    std::stable_sort(task_order.begin(), task_order.end(), [&](int32_t x, int32_t y) { return cells[x] > cells[y]; });
+    std::iota(win_order.begin(), win_order.end(), 0);
+    std::stable_sort(win_order.begin(), win_order.end(), [&](int32_t x, int32_t y) {
+        return (int64_t)(wins[x].D - wins[x].T) * maxw[wins[x].task] > (int64_t)(wins[y].D - wins[y].T) * maxw[wins[y].task];
+    });
+    SA_HIP(ctx, b->d_fM.ensure(sizeof(T) * total_cells));
+    SA_HIP(ctx, b->d_fX.ensure(sizeof(T) * total_cells));
+    SA_HIP(ctx, b->d_fY.ensure(sizeof(T) * total_cells));
+    int rc;
+    if ((rc = upload(ctx, b->d_tasks, tasks)) || (rc = upload(ctx, b->d_wins, wins)) ||
+        (rc = upload(ctx, b->d_task_order, task_order)) || (rc = upload(ctx, b->d_win_order, win_order)))
+        return rc;
+    SA_HIP(ctx, b->d_out.ensure(sizeof(int32_t) * 3 * std::max<int64_t>(out_total, 1)));
+    SA_HIP(ctx, b->d_win_count.ensure(sizeof(int32_t) * std::max<int64_t>(nw, 1)));
+    SA_HIP(ctx, b->d_win_total.ensure(sizeof(double) * std::max<int64_t>(nw, 1)));
+    a.tasks = b->d_tasks.as<TaskDev>();
+    a.wins = b->d_wins.as<WinDev>();
+    a.task_order = b->d_task_order.as<int32_t>();
+    a.win_order = b->d_win_order.as<int32_t>();
+    a.fM = b->d_fM.as<T>(); a.fX = b->d_fX.as<T>(); a.fY = b->d_fY.as<T>();
+    a.out = b->d_out.as<int32_t>();
+    a.win_count = b->d_win_count.as<int32_t>();
+    a.win_total = b->d_win_total.as<double>();
+    a.ring = ring;
+    const int fnt = gmaxw > 128 ? 256 : 64;
+    const int fcpt = (gmaxw + fnt - 1) / fnt;
+    const int bcpt = (gmaxw + 63) / 64;
+    if (fcpt > 4 || bcpt > 8) return fail(ctx, SA_ERR_UNSUPPORTED, "band too wide for the kernels");
+
+    SA_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+    SA_HIP(ctx, sa::launch_forward<T>(a, (int)nt, fnt, fcpt, st));
+    SA_HIP(ctx, hipEventRecord(ctx->ev[3], st));
+    SA_HIP(ctx, sa::launch_backward<T>(a, (int)nw, bcpt, st));
+    SA_HIP(ctx, hipEventRecord(ctx->ev[4], st));
+    tm.forward_launches = 1;
+    tm.backward_launches = 1;
+    std::vector<int32_t> wcount(nw);
+    SA_HIP(ctx, hipMemcpyAsync(wcount.data(), b->d_win_count.p, sizeof(int32_t) * nw, hipMemcpyDeviceToHost, st));
+    SA_HIP(ctx, hipStreamSynchronize(st));
+    /* windows whose pairs overflowed their slot run again with the exact count */
+    std::vector<int32_t> redo;
+    for (int64_t w = 0; w < nw; w++)
+        if (wcount[w] > wins[w].out_cap) redo.push_back((int32_t)w);
+    if (!redo.empty()) {
+        int64_t extra = 0;
+        for (int32_t w : redo) {
+            wins[w].out_off = out_total + extra;
+            wins[w].out_cap = wcount[w];
+            extra += wcount[w];
+        }
+        SA_HIP(ctx, b->d_out.ensure(sizeof(int32_t) * 3 * (out_total + extra)));
+        /* ensure() may have reallocated: the first-pass pairs of the other windows are gone,
+           so every window runs again into the resized buffer */
+        std::vector<int32_t> all(nw);
+        std::iota(all.begin(), all.end(), 0);
+        if ((rc = upload(ctx, b->d_wins, wins)) || (rc = upload(ctx, b->d_win_order, all))) return rc;
+        a.out = b->d_out.as<int32_t>();
+        a.wins = b->d_wins.as<WinDev>();
+        a.win_order = b->d_win_order.as<int32_t>();
+        SA_HIP(ctx, sa::launch_backward<T>(a, (int)nw, bcpt, st));
+        tm.backward_launches++;
+        SA_HIP(ctx, hipMemcpyAsync(wcount.data(), b->d_win_count.p, sizeof(int32_t) * nw, hipMemcpyDeviceToHost, st));
+        SA_HIP(ctx, hipStreamSynchronize(st));
+    }
+    /* final positions in the reference's list order: per read, sub-alignments in
+       split order, each in reverse emission order */
+    std::vector<int64_t> win_dst(nw);
+    int64_t pos = 0;
+    for (int64_t r = 0; r < b->n_reads; r++) {
+        ReadOut &ro = res->reads[r];
+        ro.first_pair = pos;
+        ro.n_sub = b->reads[r].n_tasks;
+        for (int64_t ti = b->reads[r].first_task; ti < b->reads[r].first_task + b->reads[r].n_tasks; ti++) {
+            int64_t E = 0;
+            for (int32_t w = 0; w < nwin[ti]; w++) E += wcount[task_first_win[ti] + w];
+            int64_t s = 0;
+            for (int32_t w = 0; w < nwin[ti]; w++) {
+                win_dst[task_first_win[ti] + w] = pos + E - 1 - s;
+                s += wcount[task_first_win[ti] + w];
+            }
+            pos += E;
+            ro.cells += cells[ti];
+            ro.n_windows += nwin[ti];
+        }
+        ro.n_pairs = pos - ro.first_pair;
+    }
+    const int64_t total_pairs = pos;
+    if ((rc = upload(ctx, b->d_win_dst, win_dst))) return rc;
+    SA_HIP(ctx, b->d_packed.ensure(sizeof(int32_t) * 3 * std::max<int64_t>(total_pairs, 1)));
+    SA_HIP(ctx, hipEventRecord(ctx->ev[5], st));
+    SA_HIP(ctx, sa::launch_collect(b->d_wins.as<WinDev>(), b->d_win_count.as<int32_t>(), b->d_win_dst.as<int64_t>(),
+                                   b->d_out.as<int32_t>(), b->d_packed.as<int32_t>(), b->d_x1.as<int32_t>(),
+                                   b->d_y1.as<int32_t>(), (int)nw, st));
+    SA_HIP(ctx, hipEventRecord(ctx->ev[6], st));
+    res->pairs.resize(total_pairs);
+    std::vector<double> wtot(nw);
+    if (total_pairs)
+        SA_HIP(ctx, hipMemcpyAsync(res->pairs.data(), b->d_packed.p, sizeof(int32_t) * 3 * total_pairs,
+                                   hipMemcpyDeviceToHost, st));
+    SA_HIP(ctx, hipMemcpyAsync(wtot.data(), b->d_win_total.p, sizeof(double) * nw, hipMemcpyDeviceToHost, st));
+    SA_HIP(ctx, hipStreamSynchronize(st));
+    for (int64_t r = 0; r < b->n_reads; r++) {
+        double ll = 0.0;
+        for (int64_t ti = b->reads[r].first_task; ti < b->reads[r].first_task + b->reads[r].n_tasks; ti++)
+            if (nwin[ti] > 0) ll += wtot[task_first_win[ti] + nwin[ti] - 1];
+        res->reads[r].log_likelihood = ll;
+    }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]); tm.prep_ms = ms;
+    (void)hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]); tm.forward_ms = ms;
+    (void)hipEventElapsedTime(&ms, ctx->ev[3], ctx->ev[4]); tm.backward_ms = ms;
+    (void)hipEventElapsedTime(&ms, ctx->ev[5], ctx->ev[6]); tm.collect_ms = ms;
+    (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[6]); tm.total_ms = ms;
+    tm.cells = total_cells;
+    return SA_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int sa_batch_run(sa_batch *b, sa_precision precision, sa_result **out) {
+    *out = nullptr;
+    if (!b) return SA_ERR_INVALID_ARGUMENT;
+    sa_context *ctx = b->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    sa_result *res = new sa_result();
+    int rc = precision == SA_FP32 ? run_batch<float>(b, res) : run_batch<double>(b, res);
+    if (rc) {
+        delete res;
+        return rc;
+    }
+    *out = res;
+    return SA_OK;
+}
+
+int sa_align_batch(sa_context *ctx, const sa_model *model, const sa_read *reads, int64_t n_reads,
+                   const sa_params *params, sa_precision precision, sa_result **out) {
+    *out = nullptr;
+    sa_batch *b = nullptr;
+    int rc = sa_batch_create(ctx, model, reads, n_reads, params, &b);
+    if (rc) return rc;
+    rc = sa_batch_run(b, precision, out);
+    sa_batch_destroy(b);
+    return rc;
+}
+
+int sa_align(sa_context *ctx, const sa_model *model, const sa_read *read, const sa_params *params,
+             sa_precision precision, sa_result **out) {
+    return sa_align_batch(ctx, model, read, 1, params, precision, out);
+}
+
+int64_t sa_result_n_reads(const sa_result *res) { return res ? (int64_t)res->reads.size() : 0; }
+
+int sa_result_read(const sa_result *res, int64_t i, sa_read_result *out) {
+    if (!res || i < 0 || i >= (int64_t)res->reads.size()) return SA_ERR_INVALID_ARGUMENT;
+    const ReadOut &r = res->reads[i];
+    out->pairs = r.n_pairs ? res->pairs.data() + r.first_pair : nullptr;
+    out->n_pairs = r.n_pairs;
+    out->log_likelihood = r.log_likelihood;
+    out->cells = r.cells;
+    out->n_windows = r.n_windows;
+    out->n_subalignments = r.n_sub;
+    return SA_OK;
+}
+
+int sa_result_timing(const sa_result *res, sa_timing *out) {
+    if (!res) return SA_ERR_INVALID_ARGUMENT;
+    *out = res->timing;
+    return SA_OK;
+}
+
+void sa_result_free(sa_result *res) { delete res; }
+
+} // extern "C"

@@ -1,0 +1,303 @@
+"""ctypes binding of libsignalalign_amd.so (include/signalalign_amd.h).
+
+This is the Python face of the drop-in: `Aligner.get_aligned_pairs` takes the
+arguments of getAlignedPairsUsingAnchors (impl/pairwiseAligner.c:1871-1899) in
+flat form and returns the reference's (floor(p*1e7), x, y) tuples in the
+reference's list order.  The library must be built (``__graft_entry__.build()``
+or ``make -C signalalign_amd/csrc``); importing this module without it raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import models as _models
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsignalalign_amd.so")
+
+SA_FP64, SA_FP32 = 0, 1
+SA_EMIT_GAUSS_DESCALED, SA_EMIT_GAUSS, SA_EMIT_HDP = 0, 1, 2
+
+_ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "out of device memory", -4: "k-mer outside alphabet",
+           -5: "invalid anchors", -6: "unsupported", -7: "no device"}
+
+_F64P = ctypes.POINTER(ctypes.c_double)
+_I64P = ctypes.POINTER(ctypes.c_int64)
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [("threshold", ctypes.c_double), ("min_diags_between_traceback", ctypes.c_int64),
+                ("traceback_diagonals", ctypes.c_int64), ("diagonal_expansion", ctypes.c_int64),
+                ("split_matrix_bigger_than_this", ctypes.c_int64)]
+
+
+class _ModelDesc(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("kmer_length", ctypes.c_int32), ("alphabet", ctypes.c_char_p),
+                ("transitions", ctypes.c_double * 9), ("match", _F64P), ("gapy", _F64P), ("gapx", _F64P),
+                ("grid_length", ctypes.c_int64), ("grid", _F64P), ("n_rows", ctypes.c_int64),
+                ("density", _F64P), ("slopes", _F64P), ("kmer_row", _I64P), ("level_mean", _F64P)]
+
+
+class _Read(ctypes.Structure):
+    _fields_ = [("ref", ctypes.c_char_p), ("ref_length", ctypes.c_int64), ("events", _F64P),
+                ("n_events", ctypes.c_int64), ("event_stride", ctypes.c_int64), ("anchors", _I64P),
+                ("n_anchors", ctypes.c_int64), ("scale", ctypes.c_double), ("shift", ctypes.c_double),
+                ("var", ctypes.c_double), ("ragged_left", ctypes.c_int32), ("ragged_right", ctypes.c_int32)]
+
+
+class _Pair(ctypes.Structure):
+    _fields_ = [("p", ctypes.c_int32), ("x", ctypes.c_int32), ("y", ctypes.c_int32)]
+
+
+class _ReadResult(ctypes.Structure):
+    _fields_ = [("pairs", ctypes.POINTER(_Pair)), ("n_pairs", ctypes.c_int64), ("log_likelihood", ctypes.c_double),
+                ("cells", ctypes.c_int64), ("n_windows", ctypes.c_int64), ("n_subalignments", ctypes.c_int64)]
+
+
+class _Timing(ctypes.Structure):
+    _fields_ = [("prep_ms", ctypes.c_double), ("forward_ms", ctypes.c_double), ("backward_ms", ctypes.c_double),
+                ("collect_ms", ctypes.c_double), ("total_ms", ctypes.c_double), ("cells", ctypes.c_int64),
+                ("forward_launches", ctypes.c_int64), ("backward_launches", ctypes.c_int64)]
+
+
+EXPORTED = ["sa_params_default", "sa_device_count", "sa_context_create", "sa_context_destroy", "sa_model_create",
+            "sa_model_destroy", "sa_align", "sa_align_batch", "sa_batch_create", "sa_batch_run",
+            "sa_batch_destroy", "sa_result_n_reads", "sa_result_read", "sa_result_timing", "sa_result_free",
+            "sa_last_error"]
+
+_lib = None
+
+
+def load():
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                           "(make -C signalalign_amd/csrc); there is no CPU fallback")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    L.sa_params_default.argtypes = [ctypes.POINTER(_Params)]
+    L.sa_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
+    L.sa_context_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+    L.sa_context_destroy.argtypes = [vp]
+    L.sa_model_create.argtypes = [vp, ctypes.POINTER(_ModelDesc), ctypes.POINTER(vp)]
+    L.sa_model_destroy.argtypes = [vp]
+    L.sa_align.argtypes = [vp, vp, ctypes.POINTER(_Read), ctypes.POINTER(_Params), ctypes.c_int, ctypes.POINTER(vp)]
+    L.sa_align_batch.argtypes = [vp, vp, ctypes.POINTER(_Read), ctypes.c_int64, ctypes.POINTER(_Params),
+                                 ctypes.c_int, ctypes.POINTER(vp)]
+    L.sa_batch_create.argtypes = [vp, vp, ctypes.POINTER(_Read), ctypes.c_int64, ctypes.POINTER(_Params),
+                                  ctypes.POINTER(vp)]
+    L.sa_batch_run.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
+    L.sa_batch_destroy.argtypes = [vp]
+    L.sa_result_n_reads.argtypes = [vp]
+    L.sa_result_n_reads.restype = ctypes.c_int64
+    L.sa_result_read.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(_ReadResult)]
+    L.sa_result_timing.argtypes = [vp, ctypes.POINTER(_Timing)]
+    L.sa_result_free.argtypes = [vp]
+    L.sa_last_error.argtypes = [vp]
+    L.sa_last_error.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    load().sa_device_count(ctypes.byref(n))
+    return n.value
+
+
+class SignalAlignError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{_ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Params:
+    """PairwiseAlignmentParameters fields used by the path (defaults of
+    pairwiseAlignmentBandingParameters_construct, impl/pairwiseAligner.c:1843-1856)."""
+
+    def __init__(self, threshold=0.01, min_diags_between_traceback=1000, traceback_diagonals=40,
+                 diagonal_expansion=20, split_matrix_bigger_than_this=3000 * 3000):
+        self.threshold = threshold
+        self.min_diags_between_traceback = min_diags_between_traceback
+        self.traceback_diagonals = traceback_diagonals
+        self.diagonal_expansion = diagonal_expansion
+        self.split_matrix_bigger_than_this = split_matrix_bigger_than_this
+
+    def _struct(self):
+        return _Params(self.threshold, self.min_diags_between_traceback, self.traceback_diagonals,
+                       self.diagonal_expansion, self.split_matrix_bigger_than_this)
+
+
+class Read:
+    """One strand of one read: reference nucleotides, event records (mean, noise, ...),
+    0-based anchors (k-mer index, event index), descaling and ragged ends."""
+
+    def __init__(self, ref, events, anchors, scale=1.0, shift=0.0, var=1.0, ragged_left=True, ragged_right=True):
+        self.ref = ref.encode() if isinstance(ref, str) else bytes(ref)
+        ev = np.asarray(events, dtype=np.float64)
+        if ev.ndim != 2 or ev.shape[1] < 2:
+            raise ValueError("events must be (n, >=2)")
+        self.events = np.ascontiguousarray(ev)
+        self.anchors = np.ascontiguousarray(np.asarray(anchors, dtype=np.int64).reshape(-1, 2))
+        self.scale, self.shift, self.var = float(scale), float(shift), float(var)
+        self.ragged_left, self.ragged_right = bool(ragged_left), bool(ragged_right)
+
+    def _struct(self):
+        return _Read(self.ref, len(self.ref), self.events.ctypes.data_as(_F64P), len(self.events),
+                     self.events.shape[1], self.anchors.ctypes.data_as(_I64P), len(self.anchors), self.scale,
+                     self.shift, self.var, int(self.ragged_left), int(self.ragged_right))
+
+
+class ReadResult:
+    def __init__(self, pairs, log_likelihood, cells, n_windows, n_subalignments):
+        self.pairs = pairs              # (n, 3) int64 [p_int, x, y]
+        self.log_likelihood = log_likelihood
+        self.cells = cells
+        self.n_windows = n_windows
+        self.n_subalignments = n_subalignments
+
+
+class Context:
+    """A device context (one HIP stream) with resident models."""
+
+    def __init__(self, device=0):
+        L = load()
+        h = ctypes.c_void_p()
+        rc = L.sa_context_create(device, ctypes.byref(h))
+        if rc != 0:
+            raise SignalAlignError(rc, "sa_context_create")
+        self._h = h
+        self._L = L
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise SignalAlignError(rc, f"{what}: {self._L.sa_last_error(self._h).decode(errors='replace')}")
+
+    def close(self):
+        if self._h:
+            self._L.sa_context_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def model(self, m, emission=None):
+        """Upload a models.GaussianModel (emission SA_EMIT_GAUSS_DESCALED by default,
+        or SA_EMIT_GAUSS) or a models.HdpModel."""
+        return DeviceModel(self, m, emission)
+
+    def align(self, model, read, params=None, precision=SA_FP64):
+        return self.align_batch(model, [read], params, precision)[0][0]
+
+    def align_batch(self, model, reads, params=None, precision=SA_FP64):
+        """Returns ([ReadResult], timing dict)."""
+        b = Batch(self, model, reads, params)
+        try:
+            return b.run(precision)
+        finally:
+            b.close()
+
+
+class DeviceModel:
+    def __init__(self, ctx, m, emission=None):
+        self.ctx = ctx
+        self.src = m
+        d = _ModelDesc()
+        keep = []
+        if isinstance(m, _models.HdpModel):
+            d.kind = SA_EMIT_HDP
+            for name in ("grid", "density", "slopes", "level_mean"):
+                a = np.ascontiguousarray(getattr(m, name), dtype=np.float64)
+                keep.append(a)
+                setattr(d, name, a.ctypes.data_as(_F64P))
+            kr = np.ascontiguousarray(m.kmer_row, dtype=np.int64)
+            keep.append(kr)
+            d.kmer_row = kr.ctypes.data_as(_I64P)
+            d.grid_length = len(m.grid)
+            d.n_rows = m.density.shape[0]
+        else:
+            d.kind = SA_EMIT_GAUSS_DESCALED if emission is None else emission
+            for name in ("match", "gapy", "gapx"):
+                a = np.ascontiguousarray(getattr(m, name), dtype=np.float64)
+                keep.append(a)
+                setattr(d, name, a.ctypes.data_as(_F64P))
+        d.kmer_length = m.kmer_length
+        d.alphabet = m.alphabet.encode()
+        for i in range(9):
+            d.transitions[i] = float(m.transitions[i])
+        h = ctypes.c_void_p()
+        ctx._check(ctx._L.sa_model_create(ctx._h, ctypes.byref(d), ctypes.byref(h)), "sa_model_create")
+        self._h = h
+        self.kind = d.kind
+        self.kmer_length = m.kmer_length
+
+    def close(self):
+        if self._h:
+            self.ctx._L.sa_model_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Reads uploaded to HBM once, runnable any number of times (sa_batch_*)."""
+
+    def __init__(self, ctx, model, reads, params=None):
+        self.ctx = ctx
+        self.model = model
+        self.reads = list(reads)
+        params = params or Params()
+        arr = (_Read * max(1, len(self.reads)))()
+        for i, r in enumerate(self.reads):
+            arr[i] = r._struct()
+        ps = params._struct()
+        h = ctypes.c_void_p()
+        ctx._check(ctx._L.sa_batch_create(ctx._h, model._h, arr, len(self.reads), ctypes.byref(ps),
+                                          ctypes.byref(h)), "sa_batch_create")
+        self._h = h
+
+    def run(self, precision=SA_FP64, fetch=True):
+        L = self.ctx._L
+        res = ctypes.c_void_p()
+        self.ctx._check(L.sa_batch_run(self._h, precision, ctypes.byref(res)), "sa_batch_run")
+        try:
+            t = _Timing()
+            L.sa_result_timing(res, ctypes.byref(t))
+            timing = {k: getattr(t, k) for k, _ in _Timing._fields_}
+            out = []
+            if fetch:
+                rr = _ReadResult()
+                for i in range(L.sa_result_n_reads(res)):
+                    L.sa_result_read(res, i, ctypes.byref(rr))
+                    if rr.n_pairs:
+                        buf = np.ctypeslib.as_array(ctypes.cast(rr.pairs, ctypes.POINTER(ctypes.c_int32)),
+                                                    shape=(rr.n_pairs * 3,))
+                        pairs = buf.reshape(-1, 3).astype(np.int64)
+                    else:
+                        pairs = np.zeros((0, 3), dtype=np.int64)
+                    out.append(ReadResult(pairs, rr.log_likelihood, rr.cells, rr.n_windows, rr.n_subalignments))
+            return out, timing
+        finally:
+            L.sa_result_free(res)
+
+    def close(self):
+        if self._h:
+            self.ctx._L.sa_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,63 @@
+"""Comparison of aligned-pair lists against the oracle (test helper).
+
+The bar, from the hot path's definition: the same aligned-pair set except pairs
+whose posterior is within 1e-3 of the threshold, posteriors within 1e-4 absolute,
+and total log-likelihood within 1e-6 relative."""
+import numpy as np
+
+import oracle
+from signalalign_amd import native
+
+P_TOL = 1e-4
+EDGE = 1e-3
+LL_REL = 1e-6
+
+
+def oracle_model(m, read, kind=None):
+    from signalalign_amd import models
+    if isinstance(m, models.HdpModel):
+        match = np.zeros((m.n_kmers, 5))
+        match[:, 0] = m.level_mean
+        return oracle.Model(oracle.HDP, m.alphabet, m.kmer_length, m.transitions, match, match,
+                            np.full(m.n_kmers, -2.3025850929940455), read["scale"], read["shift"], read["var"],
+                            m.grid, m.density, m.slopes, m.kmer_row)
+    kind = oracle.GAUSS_DESCALED if kind is None else kind
+    return oracle.Model(kind, m.alphabet, m.kmer_length, m.transitions, m.match, m.gapy, m.gapx,
+                        read["scale"], read["shift"], read["var"])
+
+
+def native_read(r, ragged_left=True, ragged_right=True):
+    return native.Read(r["ref"], r["events"], r["anchors"], r["scale"], r["shift"], r["var"], ragged_left,
+                       ragged_right)
+
+
+def run_oracle(m, r, params, ragged_left=True, ragged_right=True, kind=None):
+    op = oracle.Params(params.threshold, params.min_diags_between_traceback, params.traceback_diagonals,
+                       params.diagonal_expansion, params.split_matrix_bigger_than_this)
+    return oracle.get_aligned_pairs(oracle_model(m, r, kind), r["ref"], r["events"], r["anchors"], op,
+                                    ragged_left, ragged_right)
+
+
+def compare(gpu_pairs, ref_pairs, threshold, p_tol=P_TOL, edge=EDGE, check_order=True):
+    """Returns dict of error figures; raises AssertionError if outside the bar."""
+    g = {(int(x), int(y)): p / 1e7 for p, x, y in gpu_pairs}
+    r = {(int(x), int(y)): p / 1e7 for p, x, y in ref_pairs}
+    assert len(g) == len(gpu_pairs), "duplicate pairs in GPU output"
+    only_g = [k for k in g if k not in r and abs(g[k] - threshold) > edge]
+    only_r = [k for k in r if k not in g and abs(r[k] - threshold) > edge]
+    common = [k for k in g if k in r]
+    dmax = max((abs(g[k] - r[k]) for k in common), default=0.0)
+    out = {"n_gpu": len(g), "n_ref": len(r), "only_gpu": len(only_g), "only_ref": len(only_r),
+           "max_abs_dp": dmax}
+    assert not only_g and not only_r, f"pair sets differ beyond the threshold edge: {out}"
+    assert dmax <= p_tol, f"posterior error {dmax} > {p_tol}: {out}"
+    if check_order:
+        cs = set(common)
+        og = [(int(x), int(y)) for _, x, y in gpu_pairs if (int(x), int(y)) in cs]
+        orr = [(int(x), int(y)) for _, x, y in ref_pairs if (int(x), int(y)) in cs]
+        assert og == orr, "pair order differs from the reference list order"
+    return out
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)

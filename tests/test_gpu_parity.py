@@ -1,0 +1,146 @@
+"""HIP path vs the FP64 oracle on identical seeded inputs (GPU tests).
+
+Bar (FP64 drop-in): same aligned-pair set except pairs within 1e-3 of the
+threshold, posteriors within 1e-4 absolute, total log-likelihood within 1e-6
+relative, and the reference's list order.  FP32 results are held to the bar
+stated in each test and their errors are printed.
+"""
+import numpy as np
+import pytest
+
+from signalalign_amd import models, native, synth
+from tests import _parity as P
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def g6():
+    return synth.gaussian_model("ACGT", 6, seed=1)
+
+
+def _check(ctx, m, r, params, precision, p_tol=P.P_TOL, ll_rel=P.LL_REL, ragged=(True, True), kind=None):
+    dm = ctx.model(m, kind)
+    res, _ = ctx.align_batch(dm, [P.native_read(r, *ragged)], params, precision)
+    ref, st = P.run_oracle(m, r, params, *ragged, kind=kind)
+    out = P.compare(res[0].pairs, ref, params.threshold, p_tol=p_tol)
+    out["ll_rel"] = P.rel(res[0].log_likelihood, st["log_likelihood"])
+    out["cells"] = res[0].cells
+    assert res[0].cells == st["cells"]
+    assert res[0].n_windows == st["n_windows"]
+    assert out["ll_rel"] <= ll_rel, out
+    print(out)
+    return out
+
+
+def test_c1_fp64(gpu_ctx, g6):
+    r = synth.single_read(g6, 2000, seed=11)
+    _check(gpu_ctx, g6, r, native.Params(diagonal_expansion=20), native.SA_FP64)
+
+
+def test_c1_fp32(gpu_ctx, g6):
+    r = synth.single_read(g6, 2000, seed=11)
+    _check(gpu_ctx, g6, r, native.Params(diagonal_expansion=20), native.SA_FP32, p_tol=2e-3, ll_rel=1e-5)
+
+
+def test_reference_5mer_kat_on_gpu(gpu_ctx):
+    """tests/stateMachineTests.c:565-692 inputs through the drop-in: 7 pairs at
+    threshold 0.2 at (0,0) (1,1) (2,2) (3,3) (5,4) (6,5) (8,6)."""
+    import os
+    m = models.load_model_file(os.path.join(os.path.dirname(__file__), "golden", "testModelR9_5mer_acgt_template.model"))
+    from tests.test_oracle_kats import KAT5_EVENTS, KAT5_REF
+    dm = gpu_ctx.model(m, native.SA_EMIT_GAUSS)
+    for prec in (native.SA_FP64, native.SA_FP32):
+        res, _ = gpu_ctx.align_batch(dm, [native.Read(KAT5_REF, KAT5_EVENTS, [], 1.0, 0.0, 1.0, False, False)],
+                                     native.Params(threshold=0.2, diagonal_expansion=2), prec)
+        got = sorted((int(x), int(y)) for _, x, y in res[0].pairs)
+        assert got == [(0, 0), (1, 1), (2, 2), (3, 3), (5, 4), (6, 5), (8, 6)]
+
+
+def test_batch_mixed_lengths_fp64(gpu_ctx, g6):
+    reads = [synth.single_read(g6, n, seed=100 + i) for i, n in enumerate([60, 300, 900, 1500, 2500, 4000])]
+    params = native.Params(diagonal_expansion=20)
+    dm = gpu_ctx.model(g6)
+    res, _ = gpu_ctx.align_batch(dm, [P.native_read(r) for r in reads], params, native.SA_FP64)
+    for r, g in zip(reads, res):
+        ref, st = P.run_oracle(g6, r, params)
+        P.compare(g.pairs, ref, params.threshold)
+        assert P.rel(g.log_likelihood, st["log_likelihood"]) <= P.LL_REL
+
+
+@pytest.mark.parametrize("ragged", [(False, False), (True, False), (False, True)])
+def test_ragged_ends(gpu_ctx, g6, ragged):
+    r = synth.single_read(g6, 1200, seed=21)
+    _check(gpu_ctx, g6, r, native.Params(diagonal_expansion=20), native.SA_FP64, ragged=ragged)
+
+
+def test_no_anchors_full_band(gpu_ctx, g6):
+    r = synth.single_read(g6, 150, seed=22)
+    r["anchors"] = np.zeros((0, 2), dtype=np.int64)
+    _check(gpu_ctx, g6, r, native.Params(diagonal_expansion=20), native.SA_FP64)
+
+
+def test_split_blocks(gpu_ctx, g6):
+    """A small split_matrix_bigger_than_this makes every anchor gap a split
+    (getSplitPoints, impl/pairwiseAligner.c:1706-1757)."""
+    r = synth.single_read(g6, 1500, seed=23)
+    params = native.Params(diagonal_expansion=20, split_matrix_bigger_than_this=50 * 50)
+    out = _check(gpu_ctx, g6, r, params, native.SA_FP64)
+    assert out["n_ref"] > 0
+
+
+def test_wide_expansion_and_threshold(gpu_ctx, g6):
+    r = synth.single_read(g6, 3000, seed=24)
+    _check(gpu_ctx, g6, r, native.Params(diagonal_expansion=100, threshold=0.05), native.SA_FP64)
+
+
+def test_undescaled_emission(gpu_ctx, g6):
+    r = synth.single_read(g6, 800, seed=25)
+    r["scale"], r["shift"] = 1.0, 0.0
+    import oracle
+    _check(gpu_ctx, g6, r, native.Params(diagonal_expansion=20), native.SA_FP64, kind=native.SA_EMIT_GAUSS)
+
+
+def test_acegt_methylated(gpu_ctx):
+    m = synth.gaussian_model("ACEGT", 6, seed=5)
+    rng = synth.rng_for(31, 0, 0)
+    fwd, _ = synth.random_reference(1500, rng, methyl_cpg=0.3)
+    r = synth.simulate_read(m, fwd, synth.rng_for(31, 0, 1))
+    _check(gpu_ctx, m, r, native.Params(diagonal_expansion=50), native.SA_FP64)
+
+
+def test_hdp_fp64_and_fp32(gpu_ctx):
+    m = synth.hdp_model("ACEGT", 4, seed=7, grid_length=400)
+    rng = synth.rng_for(41, 0, 0)
+    fwd, _ = synth.random_reference(1200, rng, methyl_cpg=0.3)
+    r = synth.simulate_read(m, fwd, synth.rng_for(41, 0, 1))
+    _check(gpu_ctx, m, r, native.Params(diagonal_expansion=50), native.SA_FP64)
+    _check(gpu_ctx, m, r, native.Params(diagonal_expansion=50), native.SA_FP32, p_tol=2e-3, ll_rel=1e-5)
+
+
+def test_empty_and_short_inputs(gpu_ctx, g6):
+    dm = gpu_ctx.model(g6)
+    reads = [native.Read("ACG", np.zeros((0, 4)), []),                    # no k-mers, no events
+             native.Read("ACGTACGTAC", np.zeros((0, 4)), []),             # k-mers only
+             native.Read("AC", np.array([[80.0, 1.0, 0.01, 0.0]] * 3), [])]  # events only
+    res, _ = gpu_ctx.align_batch(dm, reads, native.Params(), native.SA_FP64)
+    for g in res:
+        assert len(g.pairs) == 0
+
+
+def test_bad_kmer_is_an_error(gpu_ctx, g6):
+    dm = gpu_ctx.model(g6)
+    r = synth.single_read(g6, 300, seed=26)
+    bad = r["ref"][:100] + b"N" + r["ref"][101:]
+    with pytest.raises(native.SignalAlignError) as e:
+        gpu_ctx.align(dm, native.Read(bad, r["events"], r["anchors"], r["scale"], r["shift"]))
+    assert e.value.code == -4
+
+
+def test_bad_anchors_is_an_error(gpu_ctx, g6):
+    dm = gpu_ctx.model(g6)
+    r = synth.single_read(g6, 300, seed=27)
+    anc = np.array([[10, 10], [5, 20]])
+    with pytest.raises(native.SignalAlignError) as e:
+        gpu_ctx.align(dm, native.Read(r["ref"], r["events"], anc, r["scale"], r["shift"]))
+    assert e.value.code == -5
