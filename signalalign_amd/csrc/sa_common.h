@@ -27,7 +27,8 @@ struct TaskDev {
     int64_t evc_off;    /* first slot in the per-event constant arrays */
     int64_t diag_off;   /* first slot in the per-diagonal arrays (N+1 entries) */
     int64_t anc_off;    /* first anchor (matrix coordinates) of this block */
-    int64_t cell_off;   /* first cell of the forward planes (set after prep) */
+    int64_t cell_off;   /* first cell of the forward match plane (set after prep) */
+    int64_t xy_off;     /* first cell of the forward gapX/gapY planes (set after prep) */
     int32_t lX, lY;
     int32_t n_anc;
     int32_t ragged_left, ragged_right;

@@ -52,13 +52,17 @@ struct Args {
     int32_t *kid;
     T *evm, *evn, *evl, *evi;
     int32_t *xlo, *xhi, *doff;
+    int8_t *dflag;          /* 1 where the forward gapX/gapY states are kept */
+    int32_t *doffxy;        /* offset of a kept diagonal in the gapX/gapY planes */
     double *foff;
     int64_t *task_cells;
+    int64_t *task_xycells;
     int32_t *task_maxw;
     int32_t *task_nwin;
     WinDev *win_slots;
     int32_t *err;
-    /* forward planes */
+    /* forward planes: match state of every cell; gapX/gapY states of the
+       diagonals the total-probability refresh reads (d and d-1 of each refresh) */
     T *fM, *fX, *fY;
     /* window outputs: 3 int32 (p, x, y) per pair */
     int32_t *out;

@@ -175,10 +175,11 @@ __global__ void __launch_bounds__(256) prep_kernel(Args<T> a) {
     const int nt = blockDim.x;
     const int k = a.m.kmer_length;
     const int A = a.m.alphabet_size;
-    __shared__ long long s_cells;
+    __shared__ long long s_cells, s_xy;
     __shared__ int s_maxw;
     __shared__ int s_err;
-    if (tid == 0) { s_cells = 0; s_maxw = 0; s_err = 0; }
+    __shared__ int s_nwin;
+    if (tid == 0) { s_cells = 0; s_xy = 0; s_maxw = 0; s_err = 0; s_nwin = 0; }
     __syncthreads();
     int err = 0;
     for (int p = tid; p < t.lX; p += nt) {
@@ -231,7 +232,28 @@ __global__ void __launch_bounds__(256) prep_kernel(Args<T> a) {
         a.task_cells[ti] = s_cells;
         a.task_maxw[ti] = s_maxw;
         if (s_err) atomicOr(a.err, s_err);
+        s_nwin = nw;
     }
+    /* the refresh at diagonal e (every 10th posterior diagonal of a window, counted
+       down from tracedBackFrom) reads the forward gapX/gapY states of e and e-1 */
+    for (int64_t d = tid; d <= N; d += nt) a.dflag[t.diag_off + d] = 0;
+    __syncthreads();
+    if (tid == 0) {
+        for (int w = 0; w < s_nwin; w++) {
+            const WinDev W = a.win_slots[t.win_off + w];
+            for (int64_t e = W.TF; e > W.T; e -= 10) {
+                a.dflag[t.diag_off + e] = 1;
+                a.dflag[t.diag_off + e - 1] = 1;
+            }
+        }
+    }
+    __syncthreads();
+    long long xy = 0;
+    for (int64_t d = tid; d <= N; d += nt)
+        if (a.dflag[t.diag_off + d]) xy += a.xhi[t.diag_off + d] - a.xlo[t.diag_off + d] + 1;
+    atomicAdd((unsigned long long *)&s_xy, (unsigned long long)xy);
+    __syncthreads();
+    if (tid == 0) a.task_xycells[ti] = s_xy;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -253,8 +275,11 @@ __global__ void __launch_bounds__(NT) forward_kernel(Args<T> a) {
     const int32_t *xlo = a.xlo + t.diag_off;
     const int32_t *xhi = a.xhi + t.diag_off;
     int32_t *doff = a.doff + t.diag_off;
+    int32_t *doffxy = a.doffxy + t.diag_off;
+    const int8_t *dflag = a.dflag + t.diag_off;
     double *foff = a.foff + t.diag_off;
     const int64_t base = t.cell_off;
+    const int64_t xybase = t.xy_off;
     const int32_t *kid = a.kid + t.pos_off;
     const int64_t evc = t.evc_off;
     const T s = (T)(m.kind == 1 ? 1.0 : t.scale / t.var);
@@ -273,12 +298,15 @@ __global__ void __launch_bounds__(NT) forward_kernel(Args<T> a) {
             ring[1 * R + (x & RM)] = v12;
             ring[2 * R + (x & RM)] = v12;
             a.fM[base + i] = v0;
-            a.fX[base + i] = v12;
-            a.fY[base + i] = v12;
+            if (dflag[0]) {
+                a.fX[xybase + i] = v12;
+                a.fY[xybase + i] = v12;
+            }
         }
-        if (tid == 0) { doff[0] = 0; foff[0] = 0.0; }
+        if (tid == 0) { doff[0] = 0; doffxy[0] = 0; foff[0] = 0.0; }
     }
     int64_t run = xhi[0] - xlo[0] + 1;
+    int64_t runxy = dflag[0] ? run : 0;
     double off1 = 0.0, off2 = 0.0;
     __syncthreads();
 
@@ -345,6 +373,7 @@ __global__ void __launch_bounds__(NT) forward_kernel(Args<T> a) {
             for (int q = 0; q < CPT; q++) { vM[q] -= sh; vX[q] -= sh; vY[q] -= sh; }
             offd = off1 + (double)sh;
         }
+        const bool keep_xy = dflag[d] != 0;
 #pragma unroll
         for (int q = 0; q < CPT; q++) {
             const int i = tid + q * NT;
@@ -354,15 +383,19 @@ __global__ void __launch_bounds__(NT) forward_kernel(Args<T> a) {
                 rc[R + (x & RM)] = vX[q];
                 rc[2 * R + (x & RM)] = vY[q];
                 a.fM[base + run + i] = vM[q];
-                a.fX[base + run + i] = vX[q];
-                a.fY[base + run + i] = vY[q];
+                if (keep_xy) {
+                    a.fX[xybase + runxy + i] = vX[q];
+                    a.fY[xybase + runxy + i] = vY[q];
+                }
             }
         }
         if (tid == 0) {
             doff[d] = (int32_t)run;
+            doffxy[d] = keep_xy ? (int32_t)runxy : -1;
             foff[d] = offd;
         }
         run += w;
+        if (keep_xy) runxy += w;
         off2 = off1;
         off1 = offd;
         __syncthreads();
@@ -406,8 +439,10 @@ __global__ void __launch_bounds__(64) backward_kernel(Args<T> a) {
     const int32_t *xlo = a.xlo + t.diag_off;
     const int32_t *xhi = a.xhi + t.diag_off;
     const int32_t *doff = a.doff + t.diag_off;
+    const int32_t *doffxy = a.doffxy + t.diag_off;
     const double *foff = a.foff + t.diag_off;
     const int64_t base = t.cell_off;
+    const int64_t xybase = t.xy_off;
     const int32_t *kid = a.kid + t.pos_off;
     const int64_t evc = t.evc_off;
     const T s = (T)(m.kind == 1 ? 1.0 : t.scale / t.var);
@@ -504,11 +539,12 @@ __global__ void __launch_bounds__(64) backward_kernel(Args<T> a) {
             const double offe = NORM ? foff[e] + obe : 0.0;
             if (posterior_count % 10 == 0) {
                 /* diagonalCalculationTotalProbability (impl/pairwiseAligner.c:1182-1200) */
+                const int64_t fxy = xybase + doffxy[e];
                 for (int i = lane; i < w; i += 64) {
                     const int x = lo + i;
                     T cell = a.fM[fb + i] + rc[x & RM];
-                    cell = logadd(cell, a.fX[fb + i] + rc[R + (x & RM)]);
-                    cell = logadd(cell, a.fY[fb + i] + rc[2 * R + (x & RM)]);
+                    cell = logadd(cell, a.fX[fxy + i] + rc[R + (x & RM)]);
+                    cell = logadd(cell, a.fY[fxy + i] + rc[2 * R + (x & RM)]);
                     termA[i] = (double)cell + offe;
                 }
                 int w1 = 0;
@@ -517,6 +553,7 @@ __global__ void __launch_bounds__(64) backward_kernel(Args<T> a) {
                     w1 = xhi[e + 1] - lo1e + 1;
                     const int lom = xlo[e - 1], him = xhi[e - 1];
                     const int64_t fbm = base + doff[e - 1];
+                    const int64_t fxym = xybase + doffxy[e - 1];
                     const double offm = NORM ? foff[e - 1] + ob1 : 0.0;
                     for (int i = lane; i < w1; i += 64) {
                         const int x = lo1e + i;        /* cell (x, y) on e+1, middle (x-1, y-1) on e-1 */
@@ -524,11 +561,11 @@ __global__ void __launch_bounds__(64) backward_kernel(Args<T> a) {
                         const bool hm = x - 1 >= lom && x - 1 <= him && x >= 1 && y >= 1;
                         T Mv = NI;
                         if (hm) {
-                            const int64_t fi = fbm + (x - 1 - lom);
+                            const int64_t fi = x - 1 - lom;
                             const T eM = Emit<T>::match(m, a, kid[x - 1], evc + (y - 1), s, inv_var);
-                            Mv = a.fM[fi] + (eM + tMC);
-                            Mv = logadd(Mv, a.fX[fi] + (eM + tMFX));
-                            Mv = logadd(Mv, a.fY[fi] + (eM + tMFY));
+                            Mv = a.fM[fbm + fi] + (eM + tMC);
+                            Mv = logadd(Mv, a.fX[fxym + fi] + (eM + tMFX));
+                            Mv = logadd(Mv, a.fY[fxym + fi] + (eM + tMFY));
                         }
                         termB[i] = (double)(Mv + r1[x & RM]) + offm;
                     }

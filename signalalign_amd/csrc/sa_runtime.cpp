@@ -109,13 +109,14 @@ struct sa_batch {
     /* raw inputs resident in HBM */
     DBuf d_ref, d_events, d_ancx, d_ancy, d_tasks, d_x1, d_y1;
     /* derived */
-    DBuf d_kid, d_evm, d_evn, d_evl, d_evi, d_xlo, d_xhi, d_doff, d_foff;
-    DBuf d_task_cells, d_task_maxw, d_task_nwin, d_win_slots, d_err;
+    DBuf d_kid, d_evm, d_evn, d_evl, d_evi, d_xlo, d_xhi, d_doff, d_foff, d_dflag, d_doffxy;
+    DBuf d_task_cells, d_task_xycells, d_task_maxw, d_task_nwin, d_win_slots, d_err;
     DBuf d_fM, d_fX, d_fY;
     DBuf d_wins, d_task_order, d_win_order, d_out, d_win_count, d_win_total, d_win_dst, d_packed;
     ~sa_batch() {
         DBuf *all[] = {&d_ref, &d_events, &d_ancx, &d_ancy, &d_tasks, &d_x1, &d_y1, &d_kid, &d_evm, &d_evn,
-                       &d_evl, &d_evi, &d_xlo, &d_xhi, &d_doff, &d_foff, &d_task_cells, &d_task_maxw,
+                       &d_evl, &d_evi, &d_xlo, &d_xhi, &d_doff, &d_foff, &d_dflag, &d_doffxy, &d_task_cells,
+                       &d_task_xycells, &d_task_maxw,
                        &d_task_nwin, &d_win_slots, &d_err, &d_fM, &d_fX, &d_fY, &d_wins, &d_task_order,
                        &d_win_order, &d_out, &d_win_count, &d_win_total, &d_win_dst, &d_packed};
         for (DBuf *b : all) b->release();
@@ -580,7 +581,10 @@ int run_batch(sa_batch *b, sa_result *res) {
     SA_HIP(ctx, b->d_xhi.ensure(sizeof(int32_t) * b->total_diag));
     SA_HIP(ctx, b->d_doff.ensure(sizeof(int32_t) * b->total_diag));
     SA_HIP(ctx, b->d_foff.ensure(sizeof(double) * b->total_diag));
+    SA_HIP(ctx, b->d_dflag.ensure(sizeof(int8_t) * b->total_diag));
+    SA_HIP(ctx, b->d_doffxy.ensure(sizeof(int32_t) * b->total_diag));
     SA_HIP(ctx, b->d_task_cells.ensure(sizeof(int64_t) * nt));
+    SA_HIP(ctx, b->d_task_xycells.ensure(sizeof(int64_t) * nt));
     SA_HIP(ctx, b->d_task_maxw.ensure(sizeof(int32_t) * nt));
     SA_HIP(ctx, b->d_task_nwin.ensure(sizeof(int32_t) * nt));
     SA_HIP(ctx, b->d_win_slots.ensure(sizeof(WinDev) * std::max<int64_t>(b->total_wslots, 1)));
@@ -602,7 +606,10 @@ int run_batch(sa_batch *b, sa_result *res) {
     a.evm = b->d_evm.as<T>(); a.evn = b->d_evn.as<T>(); a.evl = b->d_evl.as<T>(); a.evi = b->d_evi.as<T>();
     a.xlo = b->d_xlo.as<int32_t>(); a.xhi = b->d_xhi.as<int32_t>(); a.doff = b->d_doff.as<int32_t>();
     a.foff = b->d_foff.as<double>();
+    a.dflag = b->d_dflag.as<int8_t>();
+    a.doffxy = b->d_doffxy.as<int32_t>();
     a.task_cells = b->d_task_cells.as<int64_t>();
+    a.task_xycells = b->d_task_xycells.as<int64_t>();
     a.task_maxw = b->d_task_maxw.as<int32_t>();
     a.task_nwin = b->d_task_nwin.as<int32_t>();
     a.win_slots = b->d_win_slots.as<WinDev>();
@@ -615,11 +622,12 @@ int run_batch(sa_batch *b, sa_result *res) {
     SA_HIP(ctx, hipEventRecord(ctx->ev[0], st));
     SA_HIP(ctx, sa::launch_prep<T>(a, (int)nt, st));
     SA_HIP(ctx, hipEventRecord(ctx->ev[1], st));
-    std::vector<int64_t> cells(nt);
+    std::vector<int64_t> cells(nt), xycells(nt);
     std::vector<int32_t> maxw(nt), nwin(nt);
     std::vector<WinDev> slots(b->total_wslots);
     int32_t err = 0;
     SA_HIP(ctx, hipMemcpyAsync(cells.data(), b->d_task_cells.p, sizeof(int64_t) * nt, hipMemcpyDeviceToHost, st));
+    SA_HIP(ctx, hipMemcpyAsync(xycells.data(), b->d_task_xycells.p, sizeof(int64_t) * nt, hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipMemcpyAsync(maxw.data(), b->d_task_maxw.p, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipMemcpyAsync(nwin.data(), b->d_task_nwin.p, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipMemcpyAsync(slots.data(), b->d_win_slots.p, sizeof(WinDev) * b->total_wslots,
@@ -632,11 +640,13 @@ int run_batch(sa_batch *b, sa_result *res) {
 
     /* plan: forward planes, windows, outputs */
     int32_t gmaxw = 1;
-    int64_t total_cells = 0;
+    int64_t total_cells = 0, total_xy = 0;
     std::vector<TaskDev> tasks = b->tasks;
     for (int64_t i = 0; i < nt; i++) {
         tasks[i].cell_off = total_cells;
+        tasks[i].xy_off = total_xy;
         total_cells += cells[i];
+        total_xy += xycells[i];
         gmaxw = std::max(gmaxw, maxw[i]);
     }
     const int ring = next_pow2(gmaxw);
@@ -665,8 +675,8 @@ int run_batch(sa_batch *b, sa_result *res) {
         return (int64_t)(wins[x].D - wins[x].T) * maxw[wins[x].task] > (int64_t)(wins[y].D - wins[y].T) * maxw[wins[y].task];
     });
     SA_HIP(ctx, b->d_fM.ensure(sizeof(T) * total_cells));
-    SA_HIP(ctx, b->d_fX.ensure(sizeof(T) * total_cells));
-    SA_HIP(ctx, b->d_fY.ensure(sizeof(T) * total_cells));
+    SA_HIP(ctx, b->d_fX.ensure(sizeof(T) * std::max<int64_t>(total_xy, 1)));
+    SA_HIP(ctx, b->d_fY.ensure(sizeof(T) * std::max<int64_t>(total_xy, 1)));
     int rc;
     if ((rc = upload(ctx, b->d_tasks, tasks)) || (rc = upload(ctx, b->d_wins, wins)) ||
         (rc = upload(ctx, b->d_task_order, task_order)) || (rc = upload(ctx, b->d_win_order, win_order)))
@@ -683,7 +693,9 @@ int run_batch(sa_batch *b, sa_result *res) {
     a.win_count = b->d_win_count.as<int32_t>();
     a.win_total = b->d_win_total.as<double>();
     a.ring = ring;
-    const int fnt = gmaxw > 128 ? 256 : 64;
+    /* many sub-alignments: one wavefront each (occupancy); few: 4 wavefronts
+       per sub-alignment to shorten each diagonal step (latency) */
+    const int fnt = (nt >= 4 * std::max(ctx->n_cu, 1) || gmaxw <= 64) && gmaxw <= 256 ? 64 : 256;
     const int fcpt = (gmaxw + fnt - 1) / fnt;
     const int bcpt = (gmaxw + 63) / 64;
     if (fcpt > 4 || bcpt > 8) return fail(ctx, SA_ERR_UNSUPPORTED, "band too wide for the kernels");
