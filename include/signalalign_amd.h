@@ -124,13 +124,18 @@ typedef struct {
 
 typedef struct {
     double prep_ms;        /* band, schedule, k-mer ids, event constants (GPU) */
-    double forward_ms;     /* forward kernel */
-    double backward_ms;    /* backward + posterior kernel */
+    double forward_ms;     /* forward kernels, summed over sub-alignment groups */
+    double backward_ms;    /* backward + posterior kernels, summed over groups */
     double collect_ms;     /* pair compaction */
     double total_ms;       /* whole call on the device, inputs resident to outputs resident */
     int64_t cells;
     int64_t forward_launches;
     int64_t backward_launches;
+    double fb_ms;            /* forward + backward sweeps on the device timeline */
+    double main_forward_ms;  /* forward kernel of the largest group of sub-alignments */
+    double main_backward_ms; /* backward kernel of the largest group */
+    int64_t main_cells;      /* band cells of that group */
+    int32_t main_cpt;        /* its register window: 64*main_cpt positions (0: LDS sweep) */
 } sa_timing;
 
 typedef struct sa_context sa_context;
@@ -167,6 +172,22 @@ int64_t sa_result_n_reads(const sa_result *res);
 int sa_result_read(const sa_result *res, int64_t i, sa_read_result *out);
 int sa_result_timing(const sa_result *res, sa_timing *out);
 void sa_result_free(sa_result *res);
+
+/* Host-side views of the integer geometry the kernels compute on the device
+ * (same code, sa_common.h), for validation and for callers that plan work:
+ *   sa_band_geometry: band_construct (impl/pairwiseAligner.c:131-182) as per-diagonal
+ *     x ranges xlo[d]..xhi[d], d = 0..lX+lY; anchors 0-based, strictly increasing.
+ *   sa_window_schedule: the traceback windows of getPosteriorProbsWithBanding
+ *     (impl/pairwiseAligner.c:1319-1410), 4 values per window (tracedBackTo, the
+ *     diagonal the backward pass starts on, tracedBackFrom, last); returns the count.
+ *   sa_split_points: getSplitPoints (impl/pairwiseAligner.c:1706-1757), 4 values per
+ *     block (x1, y1, x2, y2); returns the count. */
+int sa_band_geometry(const int64_t *anchors, int64_t n_anchors, int64_t lX, int64_t lY, int64_t expansion,
+                     int32_t *xlo, int32_t *xhi);
+int64_t sa_window_schedule(const int32_t *xlo, const int32_t *xhi, int64_t lX, int64_t lY, const sa_params *params,
+                           int32_t *out, int64_t cap);
+int64_t sa_split_points(const int64_t *anchors, int64_t n_anchors, int64_t lX, int64_t lY, const sa_params *params,
+                        int32_t ragged_left, int32_t ragged_right, int64_t *out, int64_t cap);
 
 /* Text of the last error on this context (valid until the next call). */
 const char *sa_last_error(const sa_context *ctx);

@@ -76,6 +76,8 @@ def lib():
             build()
         L = ctypes.CDLL(_LIB_PATH)
         L.or_set_exact_logadd.argtypes = [ctypes.c_int]
+        L.or_record_windows.argtypes = [_I64P, ctypes.c_int64]
+        L.or_recorded_windows.restype = ctypes.c_int64
         L.or_lookup.restype = ctypes.c_double
         L.or_lookup.argtypes = [ctypes.c_double]
         L.or_logadd.restype = ctypes.c_double
@@ -192,6 +194,19 @@ class Params:
     def _struct(self):
         return _Params(self.threshold, self.min_diags_between_traceback, self.traceback_diagonals,
                        self.diagonal_expansion, self.split_matrix_bigger_than_this)
+
+
+def record_windows(fn, *args, cap=4096, **kw):
+    """Run fn(*args, **kw) and return (its result, the traceback windows the oracle ran
+    as rows (tracedBackTo, traceback diagonal, tracedBackFrom, atEnd))."""
+    buf = np.zeros((cap, 4), dtype=np.int64)
+    lib().or_record_windows(_pi64(buf), cap)
+    try:
+        out = fn(*args, **kw)
+        n = lib().or_recorded_windows()
+    finally:
+        lib().or_record_windows(None, 0)
+    return out, buf[:min(n, cap)].copy()
 
 
 def set_exact_logadd(on):

@@ -41,8 +41,19 @@
 enum { M_ST = 0, X_ST = 1, Y_ST = 2 };
 
 static int g_exact_logadd = 0;
+static int64_t *g_win_buf = NULL;
+static int64_t g_win_cap = 0, g_win_n = 0;
 
 void or_set_exact_logadd(int on) { g_exact_logadd = on; }
+
+/* Test hook: record (tracedBackTo, traceback diagonal, tracedBackFrom, atEnd) of
+ * every traceback window run by the next alignments; NULL stops recording. */
+void or_record_windows(int64_t *buf, int64_t cap) {
+    g_win_buf = buf;
+    g_win_cap = cap;
+    g_win_n = 0;
+}
+int64_t or_recorded_windows(void) { return g_win_n; }
 
 /* impl/pairwiseAligner.c:237-248 -- piecewise cubic approximation of log(exp(x)+1)
  * on [0, 7.5].  The coefficients are float literals promoted to double. */
@@ -574,6 +585,15 @@ static int posterior_probs_with_banding(const or_seqs *s, const int64_t *anchors
             diag_init_values(matrix_create(bw, d, bL[d], bR[d]), v);
             if (d > traced_back_to + 1) matrix_create(bw, d - 1, bL[d - 1], bR[d - 1]);
             int64_t traced_back_from = d - (at_end ? 0 : p->traceback_diagonals + 1);
+            if (g_win_buf) {
+                if (g_win_n < g_win_cap) {
+                    g_win_buf[4 * g_win_n + 0] = traced_back_to;
+                    g_win_buf[4 * g_win_n + 1] = d;
+                    g_win_buf[4 * g_win_n + 2] = traced_back_from;
+                    g_win_buf[4 * g_win_n + 3] = at_end;
+                }
+                g_win_n++;
+            }
             double total = OR_LOG_ZERO;
             int64_t this_tb = 0;
             int64_t d2 = d;

@@ -78,6 +78,8 @@ typedef struct {
 } or_stats;
 
 void or_set_exact_logadd(int on);
+void or_record_windows(int64_t *buf, int64_t cap);
+int64_t or_recorded_windows(void);
 double or_lookup(double x);
 double or_logadd(double x, double y);
 

@@ -1,0 +1,187 @@
+/*
+ * sa_device.h -- device arithmetic shared by the kernels: the reference's lookup
+ * log-add, the emission densities, wave reductions and one-lane DPP shifts.
+ */
+#ifndef SA_DEVICE_H
+#define SA_DEVICE_H
+
+#include "sa_kernels.h"
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+namespace sa {
+
+/* ------------------------------------------------------------------------ */
+/* numerics                                                                  */
+/* ------------------------------------------------------------------------ */
+
+template <typename T> __device__ __forceinline__ T ninf();
+template <> __device__ __forceinline__ float ninf<float>() { return -__builtin_huge_valf(); }
+template <> __device__ __forceinline__ double ninf<double>() { return -__builtin_huge_val(); }
+
+__device__ __forceinline__ double poly3(double d, double c0, double c1, double c2, double c3) {
+#pragma clang fp contract(off)
+    return ((c0 * d + c1) * d + c2) * d + c3;
+}
+__device__ __forceinline__ float poly3(float d, float c0, float c1, float c2, float c3) {
+    return __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(c0, d, c1), d, c2), d, c3);
+}
+
+/* logAdd (impl/pairwiseAligner.c:237-254): lo + lookup(hi - lo) while hi - lo < 7.5,
+ * else hi.  A -inf operand makes the difference inf or NaN, which selects hi. */
+template <typename T>
+__device__ __forceinline__ T logadd(T a, T b) {
+    T hi = a > b ? a : b;
+    T lo = a > b ? b : a;
+    T d = hi - lo;
+    const bool p1 = d <= (T)1.00f, p2 = d <= (T)2.50f, p3 = d <= (T)4.50f;
+    T c0 = p1 ? (T)-0.009350833524763f : p2 ? (T)-0.014532321752540f : p3 ? (T)-0.004605031767994f : (T)-0.000458661602210f;
+    T c1 = p1 ? (T)0.130659527668286f : p2 ? (T)0.139942324101744f : p3 ? (T)0.063427417320019f : (T)0.009695946122598f;
+    T c2 = p1 ? (T)0.498799810682272f : p2 ? (T)0.495635523139337f : p3 ? (T)0.695956496475118f : (T)0.930734667215156f;
+    T c3 = p1 ? (T)0.693203116424741f : p2 ? (T)0.692140569840976f : p3 ? (T)0.514272634594009f : (T)0.168037164329057f;
+    T r = poly3(d, c0, c1, c2, c3) + lo;
+    return d < (T)7.5 ? r : hi;
+}
+
+__device__ __forceinline__ double flog(double v) { return log(v); }
+__device__ __forceinline__ float flog(float v) { return logf(v); }
+__device__ __forceinline__ double fexp(double v) { return exp(v); }
+__device__ __forceinline__ float fexp(float v) { return expf(v); }
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        T u = __shfl_xor(v, o, 64);
+        v = u > v ? u : v;
+    }
+    return v;
+}
+
+/* ------------------------------------------------------------------------ */
+/* emissions                                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* emissions_signal_strawManGetKmerEventMatchProb[WithDescaling]
+ * (impl/stateMachine.c:498-589): log N(descaled mean; mu, sd) + log IG(noise; nmu, lambda),
+ * rearranged with per-k-mer and per-event constants:
+ *   a = (evm - s*mu) / sd,  evm = (mean - shift)/var, s = scale/var (1, 0, 1 undescaled)
+ *   e = ca - a^2/2 + evl - nb (noise - nmu)^2 / noise,  evl = -1.5 log(noise). */
+template <typename T>
+__device__ __forceinline__ T gauss_emit(T mu, T isd, T ca, T nmu, T nb, T evm, T evn, T evl, T evi, T s) {
+    T a = (evm - s * mu) * isd;
+    T u = evn - nmu;
+    return (ca - (T)0.5 * a * a) + evl - nb * (u * u * evi);
+}
+
+/* emissions_signal_getHdpKmerDensity (impl/stateMachine.c:468-495) ->
+ * dir_proc_density (impl/hdp.c:2583-2607) -> grid_spline_interp
+ * (impl/hdp_math_utils.c:471-496): Hermite cubic on the row's uniform grid with linear
+ * extrapolation, clamped at 0, divided by var, in log space. */
+template <typename T>
+__device__ __forceinline__ T hdp_emit_row(const Model<T> &m, int32_t row, T hmu, T evm, T s, T inv_var) {
+    if (row < 0) return ninf<T>();
+    T u = evm + hmu * ((T)1 - s);
+    const int32_t G = m.grid_length;
+    const T *ys = m.ys + (int64_t)row * 2 * G;
+    const T x0 = m.gridx[0], xn = m.gridx[G - 1];
+    T interp;
+    if (u <= x0) {
+        interp = ys[0] - ys[1] * (x0 - u);
+    } else if (u >= xn) {
+        interp = ys[2 * (G - 1)] + ys[2 * (G - 1) + 1] * (u - xn);
+    } else {
+        int32_t il = (int32_t)((u - x0) * m.inv_dx);
+        il = il > G - 2 ? G - 2 : il;
+        T y0 = ys[2 * il], s0 = ys[2 * il + 1], y1 = ys[2 * il + 2], s1 = ys[2 * il + 3];
+        T dx = m.gridx[1] - m.gridx[0];
+        T dy = y1 - y0;
+        T a = s0 * dx - dy;
+        T b = dy - s1 * dx;
+        T tl = (u - m.gridx[il]) * m.inv_dx;
+        T tr = (T)1 - tl;
+        interp = tr * y0 + tl * y1 + tl * tr * (a * tr + b * tl);
+    }
+    T dens = interp > (T)0 ? interp : (T)0;
+    return flog(inv_var * dens);
+}
+
+template <typename T>
+__device__ __forceinline__ T hdp_emit(const Model<T> &m, int32_t kid, T evm, T s, T inv_var) {
+    return hdp_emit_row(m, m.hrow[kid], m.hmu[kid], evm, s, inv_var);
+}
+
+template <typename T>
+struct Emit {
+    /* match and extra-event emissions of k-mer `kid` against event j */
+    __device__ __forceinline__ static void eval(const Model<T> &m, const Args<T> &a, int32_t kid, int64_t j, T s,
+                                                T inv_var, T &em, T &ey) {
+        T evm = a.evm[j];
+        if (m.kind == 2) {
+            em = hdp_emit(m, kid, evm, s, inv_var);
+            ey = em;
+            return;
+        }
+        T evn = a.evn[j], evl = a.evl[j], evi = a.evi[j];
+        em = gauss_emit(m.mmu[kid], m.misd[kid], m.mca[kid], m.mnmu[kid], m.mnb[kid], evm, evn, evl, evi, s);
+        ey = gauss_emit(m.ymu[kid], m.yisd[kid], m.yca[kid], m.ynmu[kid], m.ynb[kid], evm, evn, evl, evi, s);
+    }
+    __device__ __forceinline__ static T match(const Model<T> &m, const Args<T> &a, int32_t kid, int64_t j, T s,
+                                              T inv_var) {
+        T evm = a.evm[j];
+        if (m.kind == 2) return hdp_emit(m, kid, evm, s, inv_var);
+        return gauss_emit(m.mmu[kid], m.misd[kid], m.mca[kid], m.mnmu[kid], m.mnb[kid], evm, a.evn[j], a.evl[j],
+                          a.evi[j], s);
+    }
+    __device__ __forceinline__ static T gapy(const Model<T> &m, const Args<T> &a, int32_t kid, int64_t j, T s,
+                                             T inv_var) {
+        T evm = a.evm[j];
+        if (m.kind == 2) return hdp_emit(m, kid, evm, s, inv_var);
+        return gauss_emit(m.ymu[kid], m.yisd[kid], m.yca[kid], m.ynmu[kid], m.ynb[kid], evm, a.evn[j], a.evl[j],
+                          a.evi[j], s);
+    }
+    /* gapX: emissions_kmer_getGapProb (impl/stateMachine.c:155-172) or the HDP
+     * machine's constant log(0.1) (:1282) */
+    __device__ __forceinline__ static T gapx(const Model<T> &m, int32_t kid) {
+        if (m.kind == 2) return (T)-2.3025850929940455;
+        return m.gx[kid];
+    }
+};
+
+
+/* ------------------------------------------------------------------------ */
+/* one-lane wave shifts (DPP wave_shr:1 / wave_shl:1, all 64 lanes active)    */
+/* ------------------------------------------------------------------------ */
+
+/* lane i receives lane i-1's value; lane 0 receives `fill` */
+__device__ __forceinline__ int shr1(int v, int fill) {
+    return __builtin_amdgcn_update_dpp(fill, v, 0x138, 0xf, 0xf, false);
+}
+/* lane i receives lane i+1's value; lane 63 receives `fill` */
+__device__ __forceinline__ int shl1(int v, int fill) {
+    return __builtin_amdgcn_update_dpp(fill, v, 0x130, 0xf, 0xf, false);
+}
+__device__ __forceinline__ float shr1(float v, float fill) {
+    return __int_as_float(shr1(__float_as_int(v), __float_as_int(fill)));
+}
+__device__ __forceinline__ float shl1(float v, float fill) {
+    return __int_as_float(shl1(__float_as_int(v), __float_as_int(fill)));
+}
+__device__ __forceinline__ double shr1(double v, double fill) {
+    const long long b = __double_as_longlong(v), f = __double_as_longlong(fill);
+    const int lo = shr1((int)(b & 0xffffffffll), (int)(f & 0xffffffffll));
+    const int hi = shr1((int)(b >> 32), (int)(f >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double shl1(double v, double fill) {
+    const long long b = __double_as_longlong(v), f = __double_as_longlong(fill);
+    const int lo = shl1((int)(b & 0xffffffffll), (int)(f & 0xffffffffll));
+    const int hi = shl1((int)(b >> 32), (int)(f >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+} // namespace sa
+
+#endif

@@ -1,0 +1,714 @@
+/*
+ * sa_fb_reg.hip -- register-resident forward and backward sweeps (one wavefront
+ * per sub-alignment / per traceback window).
+ *
+ * Layout: lane i owns the CPT consecutive reference positions
+ * x = X0 + CPT*i + c (c < CPT) of a window [X0, X0 + 64*CPT) that slides along the
+ * band.  On anti-diagonal d a lane's cells are (x, d - x); the three states of the
+ * two previous anti-diagonals stay in registers.  The neighbours of
+ * stateMachine3_cellCalculate (impl/stateMachine.c:1194-1257) are then:
+ *   forward  lower  (x-1, d-1): own cell c-1, or lane i-1's last cell (DPP wave_shr:1)
+ *            upper  (x,   d-1): own cell c
+ *            middle (x-1, d-2): own cell c-1 of d-2, or what arrived as lower one step ago
+ *   backward lower  (x+1, e+1): own cell c+1, or lane i+1's first cell (DPP wave_shl:1)
+ *            upper  (x,   e+1): own cell c
+ *            middle (x+1, e+2): own cell c+1 of e+2, or what arrived as lower one step ago
+ * The k-mer constants of a lane's positions live in registers and are reloaded
+ * only when the window slides; event constants move one lane per anti-diagonal
+ * with the same DPP shift, the lane at the window edge loading the next event
+ * one step ahead.  Per-diagonal band bounds are read 64 diagonals at a time (one
+ * per lane) and broadcast with readlane.
+ *
+ * Arithmetic per cell is identical to the LDS kernels of sa_kernels.hip: the
+ * same terms, in the same order, through the same lookup log-add.
+ */
+#include "sa_device.h"
+
+namespace sa {
+
+/* ------------------------------------------------------------------------ */
+/* per-position and per-event constants                                       */
+/* ------------------------------------------------------------------------ */
+
+template <typename T, int EK> struct KCon;
+template <typename T> struct KCon<T, 0> {
+    T gx, mmu, misd, mca, mnmu, mnb, ymu, yisd, yca, ynmu, ynb;
+};
+template <typename T> struct KCon<T, 2> {
+    T hmu;
+    int32_t row;
+};
+
+template <typename T, int EK> struct ECon;
+template <typename T> struct ECon<T, 0> { T m, n, l, i; };
+template <typename T> struct ECon<T, 2> { T m; };
+
+/* k-mer (x-1) of cell x; an absent k-mer (x = 0 or past the end) gives -inf emissions */
+template <typename T>
+__device__ __forceinline__ void load_kcon(KCon<T, 0> &k, const Model<T> &m, const int32_t *kid, int64_t pos,
+                                          bool ok) {
+    if (ok) {
+        const int32_t id = kid[pos];
+        k.gx = m.gx[id];
+        k.mmu = m.mmu[id]; k.misd = m.misd[id]; k.mca = m.mca[id]; k.mnmu = m.mnmu[id]; k.mnb = m.mnb[id];
+        k.ymu = m.ymu[id]; k.yisd = m.yisd[id]; k.yca = m.yca[id]; k.ynmu = m.ynmu[id]; k.ynb = m.ynb[id];
+    } else {
+        const T NI = ninf<T>();
+        k.gx = NI;
+        k.mmu = 0; k.misd = 0; k.mca = NI; k.mnmu = 1; k.mnb = 0;
+        k.ymu = 0; k.yisd = 0; k.yca = NI; k.ynmu = 1; k.ynb = 0;
+    }
+}
+template <typename T>
+__device__ __forceinline__ void load_kcon(KCon<T, 2> &k, const Model<T> &m, const int32_t *kid, int64_t pos,
+                                          bool ok) {
+    if (ok) {
+        const int32_t id = kid[pos];
+        k.hmu = m.hmu[id];
+        k.row = m.hrow[id];
+    } else {
+        k.hmu = 0;
+        k.row = -1;
+    }
+}
+
+/* event j; an absent event gives finite constants (its cells have no in-band
+ * neighbour that reads it) */
+template <typename T>
+__device__ __forceinline__ void load_econ(ECon<T, 0> &e, const Args<T> &a, int64_t j, bool ok) {
+    if (ok) { e.m = a.evm[j]; e.n = a.evn[j]; e.l = a.evl[j]; e.i = a.evi[j]; }
+    else { e.m = 0; e.n = 1; e.l = 0; e.i = 1; }
+}
+template <typename T>
+__device__ __forceinline__ void load_econ(ECon<T, 2> &e, const Args<T> &a, int64_t j, bool ok) {
+    e.m = ok ? a.evm[j] : (T)0;
+}
+
+template <typename T> __device__ __forceinline__ KCon<T, 0> kcon_invalid0() {
+    KCon<T, 0> k;
+    const T NI = ninf<T>();
+    k.gx = NI; k.mmu = 0; k.misd = 0; k.mca = NI; k.mnmu = 1; k.mnb = 0;
+    k.ymu = 0; k.yisd = 0; k.yca = NI; k.ynmu = 1; k.ynb = 0;
+    return k;
+}
+
+template <typename T> __device__ __forceinline__ ECon<T, 0> shr1(const ECon<T, 0> &e, const ECon<T, 0> &f) {
+    return ECon<T, 0>{shr1(e.m, f.m), shr1(e.n, f.n), shr1(e.l, f.l), shr1(e.i, f.i)};
+}
+template <typename T> __device__ __forceinline__ ECon<T, 0> shl1(const ECon<T, 0> &e, const ECon<T, 0> &f) {
+    return ECon<T, 0>{shl1(e.m, f.m), shl1(e.n, f.n), shl1(e.l, f.l), shl1(e.i, f.i)};
+}
+template <typename T> __device__ __forceinline__ ECon<T, 2> shr1(const ECon<T, 2> &e, const ECon<T, 2> &f) {
+    return ECon<T, 2>{shr1(e.m, f.m)};
+}
+template <typename T> __device__ __forceinline__ ECon<T, 2> shl1(const ECon<T, 2> &e, const ECon<T, 2> &f) {
+    return ECon<T, 2>{shl1(e.m, f.m)};
+}
+
+/* the match/gapX constants of the next position, for the backward sweep */
+template <typename T> __device__ __forceinline__ KCon<T, 0> shl1_match(const KCon<T, 0> &k) {
+    KCon<T, 0> r = k;
+    const T NI = ninf<T>();
+    r.gx = shl1(k.gx, NI);
+    r.mmu = shl1(k.mmu, (T)0); r.misd = shl1(k.misd, (T)0); r.mca = shl1(k.mca, NI);
+    r.mnmu = shl1(k.mnmu, (T)1); r.mnb = shl1(k.mnb, (T)0);
+    return r;
+}
+template <typename T> __device__ __forceinline__ KCon<T, 2> shl1_match(const KCon<T, 2> &k) {
+    return KCon<T, 2>{shl1(k.hmu, (T)0), shl1(k.row, -1)};
+}
+
+/* emissions from registers */
+template <typename T>
+__device__ __forceinline__ T em_match(const Model<T> &, const KCon<T, 0> &k, const ECon<T, 0> &e, T s, T) {
+    return gauss_emit(k.mmu, k.misd, k.mca, k.mnmu, k.mnb, e.m, e.n, e.l, e.i, s);
+}
+template <typename T>
+__device__ __forceinline__ T em_gapy(const Model<T> &, const KCon<T, 0> &k, const ECon<T, 0> &e, T s, T) {
+    return gauss_emit(k.ymu, k.yisd, k.yca, k.ynmu, k.ynb, e.m, e.n, e.l, e.i, s);
+}
+template <typename T> __device__ __forceinline__ T em_gapx(const KCon<T, 0> &k) { return k.gx; }
+template <typename T>
+__device__ __forceinline__ T em_match(const Model<T> &m, const KCon<T, 2> &k, const ECon<T, 2> &e, T s, T iv) {
+    return hdp_emit_row(m, k.row, k.hmu, e.m, s, iv);
+}
+template <typename T>
+__device__ __forceinline__ T em_gapy(const Model<T> &m, const KCon<T, 2> &k, const ECon<T, 2> &e, T s, T iv) {
+    return hdp_emit_row(m, k.row, k.hmu, e.m, s, iv);
+}
+template <typename T> __device__ __forceinline__ T em_gapx(const KCon<T, 2> &) { return (T)-2.3025850929940455; }
+
+template <typename T> __device__ __forceinline__ T shfl_idx(T v, int src, T fill) {
+    T r = __shfl(v, src & 63, 64);
+    return (src >= 0 && src < 64) ? r : fill;
+}
+
+__device__ __forceinline__ int rdl(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+
+/* ------------------------------------------------------------------------ */
+/* forward                                                                   */
+/* ------------------------------------------------------------------------ */
+
+template <typename T, int CPT, int EK>
+__global__ void __launch_bounds__(64) forward_reg_kernel(Args<T> a, const int32_t *__restrict__ order) {
+    constexpr bool NORM = sizeof(T) == 4;
+    constexpr int W = 64 * CPT;
+    const int ti = order[blockIdx.x];
+    const TaskDev t = a.tasks[ti];
+    const Model<T> &m = a.m;
+    const int lane = threadIdx.x;
+    const int lX = t.lX, lY = t.lY, N = lX + lY;
+    const int32_t *xlo = a.xlo + t.diag_off;
+    const int32_t *xhi = a.xhi + t.diag_off;
+    const int8_t *dfl = a.dflag + t.diag_off;
+    int32_t *doff = a.doff + t.diag_off;
+    int32_t *doffxy = a.doffxy + t.diag_off;
+    double *foff = a.foff + t.diag_off;
+    const int32_t *kid = a.kid + t.pos_off;
+    const int64_t evc = t.evc_off;
+    T *fM = a.fM + t.cell_off;
+    T *fX = a.fX + t.xy_off;
+    T *fY = a.fY + t.xy_off;
+    const T s = (T)(m.kind == 1 ? 1.0 : t.scale / t.var);
+    const T iv = (T)(1.0 / t.var);
+    const T tMC = m.trans[0], tMFX = m.trans[1], tMFY = m.trans[2], tGOX = m.trans[3], tGOY = m.trans[4];
+    const T tGEX = m.trans[5], tGEY = m.trans[6], tSWX = m.trans[7];
+    const T NI = ninf<T>();
+    const bool useSWX = tSWX > NI;
+
+    /* band bounds and keep-flags, 64 diagonals per load: lane j holds diagonal dc + j */
+    int dc = 0;
+    int cl, ch, cf, nl, nh, nf;
+    {
+        int d = lane;
+        cl = d <= N ? xlo[d] : 0; ch = d <= N ? xhi[d] : -1; cf = d <= N ? dfl[d] : 0;
+        d = 64 + lane;
+        nl = d <= N ? xlo[d] : 0; nh = d <= N ? xhi[d] : -1; nf = d <= N ? dfl[d] : 0;
+    }
+    int md_doff = 0, md_dxy = -1;
+    double md_foff = 0.0;
+
+    int X0 = 0;
+    KCon<T, EK> kc[CPT];
+    ECon<T, EK> ev[CPT];
+    T pM[CPT], pX[CPT], pY[CPT], qM[CPT], qX[CPT], qY[CPT];
+    T LpM = NI, LpX = NI, LpY = NI;
+
+    /* diagonal 0: start or ragged-start values (impl/stateMachine.c:1022-1031) */
+    const int lo0 = rdl(cl, 0), hi0 = rdl(ch, 0), f0 = rdl(cf, 0);
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+        const int x = CPT * lane + c;
+        const bool v = x >= lo0 && x <= hi0;
+        pM[c] = v ? (t.ragged_left ? NI : (T)0) : NI;
+        pX[c] = v ? (t.ragged_left ? (T)0 : NI) : NI;
+        pY[c] = pX[c];
+        qM[c] = NI; qX[c] = NI; qY[c] = NI;
+        if (v) {
+            fM[x - lo0] = pM[c];
+            if (f0) { fX[x - lo0] = pX[c]; fY[x - lo0] = pY[c]; }
+        }
+        load_kcon(kc[c], m, kid, x - 1, x >= 1 && x <= lX);
+        const int j = 1 - x - 1;
+        load_econ(ev[c], a, evc + j, j >= 0 && j < lY);
+    }
+    if (lane == 0) { md_doff = 0; md_dxy = f0 ? 0 : -1; md_foff = 0.0; }
+    int64_t run = hi0 - lo0 + 1;
+    int64_t runxy = f0 ? run : 0;
+    double off1 = 0.0, off2 = 0.0;
+    int lo1 = lo0, lo2 = lo0;
+    ECon<T, EK> pre;
+    load_econ(pre, a, evc + (1 - X0), lane == 0 && 1 - X0 >= 0 && 1 - X0 < lY);
+    bool fresh = true;
+
+    for (int d = 1; d <= N; d++) {
+        int j = d - dc;
+        if (j == 64) {
+            if (dc + lane <= N) { doff[dc + lane] = md_doff; doffxy[dc + lane] = md_dxy; foff[dc + lane] = md_foff; }
+            dc += 64;
+            j = 0;
+            cl = nl; ch = nh; cf = nf;
+            const int dn = dc + 64 + lane;
+            nl = dn <= N ? xlo[dn] : 0; nh = dn <= N ? xhi[dn] : -1; nf = dn <= N ? dfl[dn] : 0;
+        }
+        const int lo = rdl(cl, j), hi = rdl(ch, j), fl = rdl(cf, j);
+        if (lo < X0) { /* band bounds are non-decreasing; anything else goes to the LDS kernels */
+            if (lane == 0) atomicOr(a.err, ERR_WINDOWS);
+            return;
+        }
+        if (hi > X0 + W - 1) {
+            /* slide the window as far as the two previous diagonals allow */
+            const int minlo = lo1 < lo2 ? lo1 : lo2;
+            const int smax = (minlo - X0) / CPT;
+            const int sreq = (hi - (X0 + W - 1) + CPT - 1) / CPT;
+            if (smax < sreq) {
+                if (lane == 0) atomicOr(a.err, ERR_WINDOWS);
+                return;
+            }
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                pM[c] = shfl_idx(pM[c], lane + smax, NI); pX[c] = shfl_idx(pX[c], lane + smax, NI);
+                pY[c] = shfl_idx(pY[c], lane + smax, NI);
+                qM[c] = shfl_idx(qM[c], lane + smax, NI); qX[c] = shfl_idx(qX[c], lane + smax, NI);
+                qY[c] = shfl_idx(qY[c], lane + smax, NI);
+            }
+            X0 += smax * CPT;
+            LpM = shr1(qM[CPT - 1], NI); LpX = shr1(qX[CPT - 1], NI); LpY = shr1(qY[CPT - 1], NI);
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                const int x = X0 + CPT * lane + c;
+                load_kcon(kc[c], m, kid, x - 1, x >= 1 && x <= lX);
+                const int jj = d - x - 1;
+                load_econ(ev[c], a, evc + jj, jj >= 0 && jj < lY);
+            }
+            fresh = true;
+        }
+        if (!fresh) {
+            /* cell x on d reads event d-x-1: its left neighbour's event of d-1 */
+            const ECon<T, EK> in = shr1(ev[CPT - 1], pre);
+#pragma unroll
+            for (int c = CPT - 1; c >= 1; c--) ev[c] = ev[c - 1];
+            ev[0] = in;
+        }
+        fresh = false;
+        {
+            const int jn = d - X0; /* event of lane 0's first cell on d+1 */
+            load_econ(pre, a, evc + jn, lane == 0 && jn >= 0 && jn < lY);
+        }
+        const T LM = shr1(pM[CPT - 1], NI), LX = shr1(pX[CPT - 1], NI), LY = shr1(pY[CPT - 1], NI);
+        const T dl2 = NORM ? (T)(off2 - off1) : (T)0;
+        T nM[CPT], nX[CPT], nY[CPT];
+        T mx = NI;
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            const int x = X0 + CPT * lane + c;
+            const bool valid = x >= lo && x <= hi;
+            const T lM = c ? pM[c - 1] : LM, lXv = c ? pX[c - 1] : LX, lYv = c ? pY[c - 1] : LY;
+            T mM = c ? qM[c - 1] : LpM, mXv = c ? qX[c - 1] : LpX, mYv = c ? qY[c - 1] : LpY;
+            if (NORM) { mM += dl2; mXv += dl2; mYv += dl2; }
+            const T eX = em_gapx<T>(kc[c]);
+            const T eM = em_match<T>(m, kc[c], ev[c], s, iv);
+            const T eY = EK == 2 ? eM : em_gapy<T>(m, kc[c], ev[c], s, iv);
+            /* lower -> gapX, middle -> match, upper -> gapY */
+            T X = lM + (eX + tGOX);
+            X = logadd(X, lXv + (eX + tGEX));
+            if (useSWX) X = logadd(X, lYv + (eX + tSWX));
+            T M = mM + (eM + tMC);
+            M = logadd(M, mXv + (eM + tMFX));
+            M = logadd(M, mYv + (eM + tMFY));
+            T Y = pM[c] + (eY + tGOY);
+            Y = logadd(Y, pY[c] + (eY + tGEY));
+            nM[c] = valid ? M : NI;
+            nX[c] = valid ? X : NI;
+            nY[c] = valid ? Y : NI;
+            if (NORM) {
+                T m3 = nM[c] > nX[c] ? nM[c] : nX[c];
+                m3 = m3 > nY[c] ? m3 : nY[c];
+                mx = m3 > mx ? m3 : mx;
+            }
+        }
+        double offd = off1;
+        if (NORM && (d & 7) == 0) {
+            mx = wave_max(mx);
+            if (mx > NI) {
+#pragma unroll
+                for (int c = 0; c < CPT; c++) { nM[c] -= mx; nX[c] -= mx; nY[c] -= mx; }
+                offd = off1 + (double)mx;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            const int x = X0 + CPT * lane + c;
+            if (x >= lo && x <= hi) {
+                fM[run + (x - lo)] = nM[c];
+                if (fl) { fX[runxy + (x - lo)] = nX[c]; fY[runxy + (x - lo)] = nY[c]; }
+            }
+        }
+        if (lane == j) { md_doff = (int32_t)run; md_dxy = fl ? (int32_t)runxy : -1; md_foff = offd; }
+        run += hi - lo + 1;
+        if (fl) runxy += hi - lo + 1;
+        LpM = LM; LpX = LX; LpY = LY;
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            qM[c] = pM[c]; qX[c] = pX[c]; qY[c] = pY[c];
+            pM[c] = nM[c]; pX[c] = nX[c]; pY[c] = nY[c];
+        }
+        off2 = off1;
+        off1 = offd;
+        lo2 = lo1;
+        lo1 = lo;
+    }
+    if (dc + lane <= N) { doff[dc + lane] = md_doff; doffxy[dc + lane] = md_dxy; foff[dc + lane] = md_foff; }
+}
+
+/* ------------------------------------------------------------------------ */
+/* backward + total refresh + posterior pairs                                 */
+/* ------------------------------------------------------------------------ */
+
+__device__ __forceinline__ double chain_terms(const double *term, int n) {
+    double acc = -__builtin_huge_val();
+    double pm = -__builtin_huge_val();
+    for (int i = 0; i < n; i++) {
+        const double v = term[i];
+        if (!(v > pm - 7.5)) continue; /* logAdd(acc, v) == acc exactly when acc - v >= 7.5 */
+        acc = logadd(acc, v);
+        pm = v > pm ? v : pm;
+    }
+    return acc;
+}
+
+template <typename T, int CPT, int EK>
+__global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32_t *__restrict__ order) {
+    constexpr bool NORM = sizeof(T) == 4;
+    constexpr int W = 64 * CPT;
+    __shared__ double termA[W];
+    __shared__ double termB[W];
+    __shared__ double bcast;
+    const int wi = order[blockIdx.x];
+    const WinDev Wd = a.wins[wi];
+    const TaskDev t = a.tasks[Wd.task];
+    const Model<T> &m = a.m;
+    const int lane = threadIdx.x;
+    const int lX = t.lX, lY = t.lY;
+    const int32_t *xlo = a.xlo + t.diag_off;
+    const int32_t *xhi = a.xhi + t.diag_off;
+    const int32_t *doff = a.doff + t.diag_off;
+    const int32_t *doffxy = a.doffxy + t.diag_off;
+    const double *foff = a.foff + t.diag_off;
+    const int32_t *kid = a.kid + t.pos_off;
+    const int64_t evc = t.evc_off;
+    const T *fM = a.fM + t.cell_off;
+    const T *fX = a.fX + t.xy_off;
+    const T *fY = a.fY + t.xy_off;
+    const T s = (T)(m.kind == 1 ? 1.0 : t.scale / t.var);
+    const T iv = (T)(1.0 / t.var);
+    const T tMC = m.trans[0], tMFX = m.trans[1], tMFY = m.trans[2], tGOX = m.trans[3], tGOY = m.trans[4];
+    const T tGEX = m.trans[5], tGEY = m.trans[6], tSWX = m.trans[7];
+    const T NI = ninf<T>();
+    const bool useSWX = tSWX > NI;
+    const T thr = (T)a.threshold;
+    const int D = Wd.D, TF = Wd.TF, TB = Wd.T;
+    const bool ragged_end = Wd.last && t.ragged_right;
+    const T e0 = ragged_end ? m.rendv[0] : m.endv[0];
+    const T e1 = ragged_end ? m.rendv[1] : m.endv[1];
+    const T e2 = ragged_end ? m.rendv[2] : m.endv[2];
+    int32_t *out = a.out + 3 * Wd.out_off;
+    int cursor = 0;
+    int posterior_count = 0;
+    double total = -__builtin_huge_val();
+
+    /* band bounds, offsets: lane j holds diagonal dc - j (descending) */
+    int dc = D;
+    int cl, ch, cdo;
+    double cfo;
+    int nl, nh, ndo;
+    double nfo;
+    {
+        int e = D - lane;
+        bool ok = e > TB;
+        cl = ok ? xlo[e] : 0; ch = ok ? xhi[e] : -1; cdo = ok ? doff[e] : 0; cfo = ok ? foff[e] : 0.0;
+        e = D - 64 - lane;
+        ok = e > TB;
+        nl = ok ? xlo[e] : 0; nh = ok ? xhi[e] : -1; ndo = ok ? doff[e] : 0; nfo = ok ? foff[e] : 0.0;
+    }
+
+    const int loD = rdl(cl, 0), hiD = rdl(ch, 0);
+    int X0 = hiD - W + 1;
+    KCon<T, EK> kc[CPT];
+    ECon<T, EK> ev[CPT];
+    T pM[CPT], pX[CPT], pY[CPT], qM[CPT];
+    T RpM = NI; /* lane i+1's first cell of e+2, match state */
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+        const int x = X0 + CPT * lane + c;
+        load_kcon(kc[c], m, kid, x - 1, x >= 1 && x <= lX);
+        const int j = D - x;
+        load_econ(ev[c], a, evc + j, j >= 0 && j < lY);
+        pM[c] = NI; pX[c] = NI; pY[c] = NI; qM[c] = NI;
+    }
+    (void)loD;
+    ECon<T, EK> pre;
+    {
+        const int jn = (D - 1) - (X0 + W - 1); /* event of lane 63's last cell on D-1 */
+        load_econ(pre, a, evc + jn, lane == 63 && jn >= 0 && jn < lY);
+    }
+    double ob1 = 0.0, ob2 = 0.0; /* offsets of B[e+1], B[e+2] */
+    int hi1 = -1, hi2 = -1;      /* xhi of e+1, e+2 (-1: outside the window of diagonals) */
+    bool fresh = true;
+
+    for (int e = D; e > TB; e--) {
+        int j = dc - e;
+        if (j == 64) {
+            dc -= 64;
+            j = 0;
+            cl = nl; ch = nh; cdo = ndo; cfo = nfo;
+            const int en = dc - 64 - lane;
+            const bool ok = en > TB;
+            nl = ok ? xlo[en] : 0; nh = ok ? xhi[en] : -1; ndo = ok ? doff[en] : 0; nfo = ok ? foff[en] : 0.0;
+        }
+        const int lo = rdl(cl, j), hi = rdl(ch, j), dof = rdl(cdo, j);
+        const double fo = __shfl(cfo, j, 64);
+        T nM[CPT], nX[CPT], nY[CPT];
+        T LowM = NI, LowX = NI; /* lane i+1's first cell of e+1 */
+        double obe = 0.0;
+        if (e == D) {
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                const int x = X0 + CPT * lane + c;
+                const bool valid = x >= lo && x <= hi;
+                nM[c] = valid ? e0 : NI; nX[c] = valid ? e1 : NI; nY[c] = valid ? e2 : NI;
+            }
+            fresh = false;
+        } else {
+            if (lo < X0) {
+                /* slide the window left as far as the two later diagonals allow */
+                const int maxhi = hi1 > hi2 ? hi1 : hi2;
+                const int smax = (X0 + W - 1 - maxhi) / CPT;
+                const int sreq = (X0 - lo + CPT - 1) / CPT;
+                if (smax < sreq) {
+                    if (lane == 0) atomicOr(a.err, ERR_WINDOWS);
+                    return;
+                }
+#pragma unroll
+                for (int c = 0; c < CPT; c++) {
+                    pM[c] = shfl_idx(pM[c], lane - smax, NI); pX[c] = shfl_idx(pX[c], lane - smax, NI);
+                    pY[c] = shfl_idx(pY[c], lane - smax, NI); qM[c] = shfl_idx(qM[c], lane - smax, NI);
+                }
+                X0 -= smax * CPT;
+                RpM = shl1(qM[0], NI);
+#pragma unroll
+                for (int c = 0; c < CPT; c++) {
+                    const int x = X0 + CPT * lane + c;
+                    load_kcon(kc[c], m, kid, x - 1, x >= 1 && x <= lX);
+                    const int jj = e - x;
+                    load_econ(ev[c], a, evc + jj, jj >= 0 && jj < lY);
+                }
+                fresh = true;
+            }
+            if (hi > X0 + W - 1) {
+                if (lane == 0) atomicOr(a.err, ERR_WINDOWS);
+                return;
+            }
+            if (!fresh) {
+                /* cell x on e reads event e-x: its right neighbour's event of e+1 */
+                const ECon<T, EK> in = shl1(ev[0], pre);
+#pragma unroll
+                for (int c = 0; c < CPT - 1; c++) ev[c] = ev[c + 1];
+                ev[CPT - 1] = in;
+            }
+            fresh = false;
+            LowM = shl1(pM[0], NI);
+            LowX = shl1(pX[0], NI);
+            const KCon<T, EK> kn = shl1_match(kc[0]); /* k-mer x of lane's last cell */
+            const T dl2 = NORM ? (T)(ob2 - ob1) : (T)0;
+            T mx = NI;
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                const int x = X0 + CPT * lane + c;
+                const bool valid = x >= lo && x <= hi;
+                const KCon<T, EK> &kx = c < CPT - 1 ? kc[c + 1] : kn;
+                const T ClowX = c < CPT - 1 ? pX[c + 1] : LowX;
+                const T CupY = pY[c];
+                T CmidM = c < CPT - 1 ? qM[c + 1] : RpM;
+                if (NORM) CmidM += dl2;
+                const T eX = em_gapx<T>(kx);
+                const T eM = em_match<T>(m, kx, ev[c], s, iv);
+                const T eY = em_gapy<T>(m, kc[c], ev[c], s, iv);
+                /* contributions arrive middle (e+2), then upper, then lower (e+1) */
+                T M = CmidM + (eM + tMC);
+                M = logadd(M, CupY + (eY + tGOY));
+                M = logadd(M, ClowX + (eX + tGOX));
+                T X = CmidM + (eM + tMFX);
+                X = logadd(X, ClowX + (eX + tGEX));
+                T Y = CmidM + (eM + tMFY);
+                Y = logadd(Y, CupY + (eY + tGEY));
+                if (useSWX) Y = logadd(Y, ClowX + (eX + tSWX));
+                nM[c] = valid ? M : NI;
+                nX[c] = valid ? X : NI;
+                nY[c] = valid ? Y : NI;
+                if (NORM) {
+                    T m3 = nM[c] > nX[c] ? nM[c] : nX[c];
+                    m3 = m3 > nY[c] ? m3 : nY[c];
+                    mx = m3 > mx ? m3 : mx;
+                }
+            }
+            obe = ob1;
+            if (NORM && ((D - e) & 7) == 0) {
+                mx = wave_max(mx);
+                if (mx > NI) {
+#pragma unroll
+                    for (int c = 0; c < CPT; c++) { nM[c] -= mx; nX[c] -= mx; nY[c] -= mx; }
+                    obe = ob1 + (double)mx;
+                }
+            }
+        }
+        {
+            const int jn = (e - 1) - (X0 + W - 1); /* event of lane 63's last cell on e-1 */
+            load_econ(pre, a, evc + jn, lane == 63 && jn >= 0 && jn < lY);
+        }
+
+        if (e <= TF) {
+            const double offe = NORM ? fo + obe : 0.0;
+            if (posterior_count % 10 == 0) {
+                /* diagonalCalculationTotalProbability (impl/pairwiseAligner.c:1182-1200) */
+                const int dxy = doffxy[e];
+#pragma unroll
+                for (int c = 0; c < CPT; c++) {
+                    const int x = X0 + CPT * lane + c;
+                    if (x >= lo && x <= hi) {
+                        const int i = x - lo;
+                        T cell = fM[dof + i] + nM[c];
+                        cell = logadd(cell, fX[dxy + i] + nX[c]);
+                        cell = logadd(cell, fY[dxy + i] + nY[c]);
+                        termA[i] = (double)cell + offe;
+                    }
+                }
+                const bool h1 = e + 1 <= D;
+                int w1 = 0;
+                if (h1) {
+                    const int lo1e = xlo[e + 1];
+                    w1 = xhi[e + 1] - lo1e + 1;
+                    const int lom = xlo[e - 1], him = xhi[e - 1];
+                    const int dm = doff[e - 1], dxym = doffxy[e - 1];
+                    const double offm = NORM ? foff[e - 1] + ob1 : 0.0;
+#pragma unroll
+                    for (int c = 0; c < CPT; c++) {
+                        const int x = X0 + CPT * lane + c; /* cell (x, e+1-x); middle (x-1, e-x) on e-1 */
+                        if (x >= lo1e && x < lo1e + w1) {
+                            const bool hm = x - 1 >= lom && x - 1 <= him && x >= 1 && e + 1 - x >= 1;
+                            T Mv = NI;
+                            if (hm) {
+                                const int fi = x - 1 - lom;
+                                const T eM = em_match<T>(m, kc[c], ev[c], s, iv);
+                                Mv = fM[dm + fi] + (eM + tMC);
+                                Mv = logadd(Mv, fX[dxym + fi] + (eM + tMFX));
+                                Mv = logadd(Mv, fY[dxym + fi] + (eM + tMFY));
+                            }
+                            termB[x - lo1e] = (double)(Mv + pM[c]) + offm;
+                        }
+                    }
+                }
+                __syncthreads();
+                if (lane == 0) {
+                    double tot = chain_terms(termA, hi - lo + 1);
+                    if (h1) tot = logadd(tot, chain_terms(termB, w1));
+                    bcast = tot;
+                }
+                __syncthreads();
+                total = bcast;
+                if (e == TF && lane == 0) a.win_total[wi] = total;
+            }
+            posterior_count++;
+            /* diagonalCalculationPosteriorMatchProbs (impl/pairwiseAligner.c:1202-1253) */
+            const T cst = NORM ? (T)(offe - total) : (T)0;
+            bool keep[CPT];
+            double pv[CPT];
+            unsigned long long bal[CPT];
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                const int x = X0 + CPT * lane + c;
+                const int y = e - x;
+                keep[c] = false;
+                pv[c] = 0.0;
+                if (x >= lo && x <= hi && x > 0 && y > 0) {
+                    T p;
+                    if (NORM) p = fexp((fM[dof + (x - lo)] + nM[c]) + cst);
+                    else p = fexp((fM[dof + (x - lo)] + nM[c]) - (T)total);
+                    keep[c] = p >= thr;
+                    pv[c] = (double)p;
+                }
+                bal[c] = __ballot(keep[c]);
+            }
+            const unsigned long long lt = (1ull << lane) - 1ull;
+            int base = cursor;
+            int ntot = 0;
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                base += __popcll(bal[c] & lt);
+                ntot += __popcll(bal[c]);
+            }
+            int own = 0;
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                if (keep[c]) {
+                    const int idx = base + own;
+                    if (idx < Wd.out_cap) {
+                        const int x = X0 + CPT * lane + c;
+                        const double p = pv[c] > 1.0 ? 1.0 : pv[c];
+                        out[3 * idx + 0] = (int32_t)floor(p * (double)kPairProb1);
+                        out[3 * idx + 1] = x - 1;
+                        out[3 * idx + 2] = e - x - 1;
+                    }
+                    own++;
+                }
+            }
+            cursor += ntot;
+        }
+        RpM = LowM;
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            qM[c] = pM[c];
+            pM[c] = nM[c]; pX[c] = nX[c]; pY[c] = nY[c];
+        }
+        ob2 = ob1;
+        ob1 = obe;
+        hi2 = hi1;
+        hi1 = hi;
+    }
+    if (lane == 0) a.win_count[wi] = cursor;
+}
+
+/* ------------------------------------------------------------------------ */
+/* launchers                                                                 */
+/* ------------------------------------------------------------------------ */
+
+template <typename T, int CPT>
+static hipError_t fwd_reg_inst(const Args<T> &a, const int32_t *order, int n, int ek, hipStream_t st) {
+    if (ek == 2) hipLaunchKernelGGL((forward_reg_kernel<T, CPT, 2>), dim3(n), dim3(64), 0, st, a, order);
+    else hipLaunchKernelGGL((forward_reg_kernel<T, CPT, 0>), dim3(n), dim3(64), 0, st, a, order);
+    return hipGetLastError();
+}
+
+template <typename T, int CPT>
+static hipError_t bwd_reg_inst(const Args<T> &a, const int32_t *order, int n, int ek, hipStream_t st) {
+    if (ek == 2) hipLaunchKernelGGL((backward_reg_kernel<T, CPT, 2>), dim3(n), dim3(64), 0, st, a, order);
+    else hipLaunchKernelGGL((backward_reg_kernel<T, CPT, 0>), dim3(n), dim3(64), 0, st, a, order);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_forward_reg(const Args<T> &a, const int32_t *order, int n, int cpt, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const int ek = a.m.kind == 2 ? 2 : 0;
+    switch (cpt) {
+    case 1: return fwd_reg_inst<T, 1>(a, order, n, ek, st);
+    case 2: return fwd_reg_inst<T, 2>(a, order, n, ek, st);
+    case 3: return fwd_reg_inst<T, 3>(a, order, n, ek, st);
+    case 4: return fwd_reg_inst<T, 4>(a, order, n, ek, st);
+    case 6: return fwd_reg_inst<T, 6>(a, order, n, ek, st);
+    case 8: return fwd_reg_inst<T, 8>(a, order, n, ek, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
+hipError_t launch_backward_reg(const Args<T> &a, const int32_t *order, int n, int cpt, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const int ek = a.m.kind == 2 ? 2 : 0;
+    switch (cpt) {
+    case 1: return bwd_reg_inst<T, 1>(a, order, n, ek, st);
+    case 2: return bwd_reg_inst<T, 2>(a, order, n, ek, st);
+    case 3: return bwd_reg_inst<T, 3>(a, order, n, ek, st);
+    case 4: return bwd_reg_inst<T, 4>(a, order, n, ek, st);
+    case 6: return bwd_reg_inst<T, 6>(a, order, n, ek, st);
+    case 8: return bwd_reg_inst<T, 8>(a, order, n, ek, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template hipError_t launch_forward_reg<float>(const Args<float> &, const int32_t *, int, int, hipStream_t);
+template hipError_t launch_forward_reg<double>(const Args<double> &, const int32_t *, int, int, hipStream_t);
+template hipError_t launch_backward_reg<float>(const Args<float> &, const int32_t *, int, int, hipStream_t);
+template hipError_t launch_backward_reg<double>(const Args<double> &, const int32_t *, int, int, hipStream_t);
+
+} // namespace sa

@@ -58,6 +58,7 @@ struct Args {
     int64_t *task_cells;
     int64_t *task_xycells;
     int32_t *task_maxw;
+    int32_t *task_span;     /* positions a register window must cover (see prep_kernel) */
     int32_t *task_nwin;
     WinDev *win_slots;
     int32_t *err;
@@ -76,6 +77,10 @@ struct Args {
 template <typename T> hipError_t launch_prep(const Args<T> &a, int n_tasks, hipStream_t st);
 template <typename T> hipError_t launch_forward(const Args<T> &a, int n_tasks, int nt, int cpt, hipStream_t st);
 template <typename T> hipError_t launch_backward(const Args<T> &a, int n_wins, int cpt, hipStream_t st);
+template <typename T>
+hipError_t launch_forward_reg(const Args<T> &a, const int32_t *order, int n, int cpt, hipStream_t st);
+template <typename T>
+hipError_t launch_backward_reg(const Args<T> &a, const int32_t *order, int n, int cpt, hipStream_t st);
 hipError_t launch_collect(const WinDev *wins, const int32_t *win_count, const int64_t *win_dst, const int32_t *src,
                           int32_t *dst, const int32_t *task_x1, const int32_t *task_y1, int n_wins, hipStream_t st);
 

@@ -18,6 +18,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <numeric>
@@ -62,9 +63,11 @@ struct DevTables {
 struct sa_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t aux[3] = {};   /* sub-alignment groups of different window sizes run side by side */
     std::mutex mu;
     std::string err;
     hipEvent_t ev[8] = {};
+    hipEvent_t gev[32] = {};   /* per-group kernel timing and joins */
     int n_cu = 0;
 };
 
@@ -110,13 +113,13 @@ struct sa_batch {
     DBuf d_ref, d_events, d_ancx, d_ancy, d_tasks, d_x1, d_y1;
     /* derived */
     DBuf d_kid, d_evm, d_evn, d_evl, d_evi, d_xlo, d_xhi, d_doff, d_foff, d_dflag, d_doffxy;
-    DBuf d_task_cells, d_task_xycells, d_task_maxw, d_task_nwin, d_win_slots, d_err;
+    DBuf d_task_cells, d_task_xycells, d_task_maxw, d_task_span, d_task_nwin, d_win_slots, d_err;
     DBuf d_fM, d_fX, d_fY;
     DBuf d_wins, d_task_order, d_win_order, d_out, d_win_count, d_win_total, d_win_dst, d_packed;
     ~sa_batch() {
         DBuf *all[] = {&d_ref, &d_events, &d_ancx, &d_ancy, &d_tasks, &d_x1, &d_y1, &d_kid, &d_evm, &d_evn,
                        &d_evl, &d_evi, &d_xlo, &d_xhi, &d_doff, &d_foff, &d_dflag, &d_doffxy, &d_task_cells,
-                       &d_task_xycells, &d_task_maxw,
+                       &d_task_xycells, &d_task_maxw, &d_task_span,
                        &d_task_nwin, &d_win_slots, &d_err, &d_fM, &d_fX, &d_fY, &d_wins, &d_task_order,
                        &d_win_order, &d_out, &d_win_count, &d_win_total, &d_win_dst, &d_packed};
         for (DBuf *b : all) b->release();
@@ -321,6 +324,8 @@ int sa_context_create(int device, sa_context **out) {
         return SA_ERR_HIP;
     }
     for (auto &e : c->ev) (void)hipEventCreate(&e);
+    for (auto &e : c->gev) (void)hipEventCreate(&e);
+    for (auto &q : c->aux) (void)hipStreamCreate(&q);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->n_cu = prop.multiProcessorCount;
     *out = c;
@@ -331,6 +336,8 @@ void sa_context_destroy(sa_context *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     for (auto &e : ctx->ev) if (e) (void)hipEventDestroy(e);
+    for (auto &e : ctx->gev) if (e) (void)hipEventDestroy(e);
+    for (auto &q : ctx->aux) if (q) (void)hipStreamDestroy(q);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -552,8 +559,10 @@ void sa_batch_destroy(sa_batch *b) {
 
 namespace {
 
+constexpr int kRetryLds = 1000;
+
 template <typename T>
-int run_batch(sa_batch *b, sa_result *res) {
+int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     sa_context *ctx = b->ctx;
     const sa_model *model = b->model;
     hipStream_t st = ctx->stream;
@@ -586,6 +595,7 @@ int run_batch(sa_batch *b, sa_result *res) {
     SA_HIP(ctx, b->d_task_cells.ensure(sizeof(int64_t) * nt));
     SA_HIP(ctx, b->d_task_xycells.ensure(sizeof(int64_t) * nt));
     SA_HIP(ctx, b->d_task_maxw.ensure(sizeof(int32_t) * nt));
+    SA_HIP(ctx, b->d_task_span.ensure(sizeof(int32_t) * nt));
     SA_HIP(ctx, b->d_task_nwin.ensure(sizeof(int32_t) * nt));
     SA_HIP(ctx, b->d_win_slots.ensure(sizeof(WinDev) * std::max<int64_t>(b->total_wslots, 1)));
     SA_HIP(ctx, b->d_err.ensure(sizeof(int32_t)));
@@ -611,6 +621,7 @@ int run_batch(sa_batch *b, sa_result *res) {
     a.task_cells = b->d_task_cells.as<int64_t>();
     a.task_xycells = b->d_task_xycells.as<int64_t>();
     a.task_maxw = b->d_task_maxw.as<int32_t>();
+    a.task_span = b->d_task_span.as<int32_t>();
     a.task_nwin = b->d_task_nwin.as<int32_t>();
     a.win_slots = b->d_win_slots.as<WinDev>();
     a.err = b->d_err.as<int32_t>();
@@ -623,12 +634,13 @@ int run_batch(sa_batch *b, sa_result *res) {
     SA_HIP(ctx, sa::launch_prep<T>(a, (int)nt, st));
     SA_HIP(ctx, hipEventRecord(ctx->ev[1], st));
     std::vector<int64_t> cells(nt), xycells(nt);
-    std::vector<int32_t> maxw(nt), nwin(nt);
+    std::vector<int32_t> maxw(nt), nwin(nt), span(nt);
     std::vector<WinDev> slots(b->total_wslots);
     int32_t err = 0;
     SA_HIP(ctx, hipMemcpyAsync(cells.data(), b->d_task_cells.p, sizeof(int64_t) * nt, hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipMemcpyAsync(xycells.data(), b->d_task_xycells.p, sizeof(int64_t) * nt, hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipMemcpyAsync(maxw.data(), b->d_task_maxw.p, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, st));
+    SA_HIP(ctx, hipMemcpyAsync(span.data(), b->d_task_span.p, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipMemcpyAsync(nwin.data(), b->d_task_nwin.p, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipMemcpyAsync(slots.data(), b->d_win_slots.p, sizeof(WinDev) * b->total_wslots,
                                hipMemcpyDeviceToHost, st));
@@ -639,7 +651,6 @@ int run_batch(sa_batch *b, sa_result *res) {
     if (err & sa::ERR_WINDOWS) return fail(ctx, SA_ERR_HIP, "window schedule overflow");
 
     /* plan: forward planes, windows, outputs */
-    int32_t gmaxw = 1;
     int64_t total_cells = 0, total_xy = 0;
     std::vector<TaskDev> tasks = b->tasks;
     for (int64_t i = 0; i < nt; i++) {
@@ -647,11 +658,7 @@ int run_batch(sa_batch *b, sa_result *res) {
         tasks[i].xy_off = total_xy;
         total_cells += cells[i];
         total_xy += xycells[i];
-        gmaxw = std::max(gmaxw, maxw[i]);
     }
-    const int ring = next_pow2(gmaxw);
-    const int ring_max = sizeof(T) == 8 ? 512 : 1024;
-    if (ring > ring_max) return fail(ctx, SA_ERR_UNSUPPORTED, "band diagonal wider than " + std::to_string(ring_max));
     std::vector<WinDev> wins;
     std::vector<int32_t> task_first_win(nt);
     int64_t out_total = 0;
@@ -667,13 +674,46 @@ int run_batch(sa_batch *b, sa_result *res) {
         }
     }
     const int64_t nw = (int64_t)wins.size();
-    std::vector<int32_t> task_order(nt), win_order(nw);
-    std::iota(task_order.begin(), task_order.end(), 0);
-    std::stable_sort(task_order.begin(), task_order.end(), [&](int32_t x, int32_t y) { return cells[x] > cells[y]; });
-    std::iota(win_order.begin(), win_order.end(), 0);
-    std::stable_sort(win_order.begin(), win_order.end(), [&](int32_t x, int32_t y) {
-        return (int64_t)(wins[x].D - wins[x].T) * maxw[wins[x].task] > (int64_t)(wins[y].D - wins[y].T) * maxw[wins[y].task];
-    });
+
+    /* kernel per sub-alignment: the register sweep with the smallest lane window
+       (64*CPT positions) that covers its band span, or the LDS sweep (group 0) */
+    static const int kCpt[] = {1, 2, 3, 4, 6, 8};
+    constexpr int kGroups = 7;
+    std::vector<int> tgroup(nt);
+    for (int64_t i = 0; i < nt; i++) {
+        int g = 0;
+        if (!force_lds)
+            for (int k = 0; k < 6; k++)
+                if (64 * kCpt[k] >= span[i] + kCpt[k] - 1) { g = k + 1; break; }
+        tgroup[i] = g;
+    }
+    std::vector<int32_t> task_order, win_order;
+    int64_t g_toff[kGroups + 1] = {0}, g_woff[kGroups + 1] = {0}, g_cells[kGroups] = {0};
+    int32_t lds_maxw = 1;
+    for (int g = 0; g < kGroups; g++) {
+        std::vector<int32_t> to, wo;
+        for (int64_t i = 0; i < nt; i++)
+            if (tgroup[i] == g) {
+                to.push_back((int32_t)i);
+                g_cells[g] += cells[i];
+                if (g == 0) lds_maxw = std::max(lds_maxw, maxw[i]);
+            }
+        std::stable_sort(to.begin(), to.end(), [&](int32_t x, int32_t y) { return cells[x] > cells[y]; });
+        for (int32_t ti : to)
+            for (int32_t w = 0; w < nwin[ti]; w++) wo.push_back(task_first_win[ti] + w);
+        std::stable_sort(wo.begin(), wo.end(), [&](int32_t x, int32_t y) {
+            return (int64_t)(wins[x].D - wins[x].T) * maxw[wins[x].task] >
+                   (int64_t)(wins[y].D - wins[y].T) * maxw[wins[y].task];
+        });
+        task_order.insert(task_order.end(), to.begin(), to.end());
+        win_order.insert(win_order.end(), wo.begin(), wo.end());
+        g_toff[g + 1] = (int64_t)task_order.size();
+        g_woff[g + 1] = (int64_t)win_order.size();
+    }
+    const int ring = next_pow2(lds_maxw);
+    const int ring_max = sizeof(T) == 8 ? 512 : 1024;
+    if (g_toff[1] > 0 && ring > ring_max)
+        return fail(ctx, SA_ERR_UNSUPPORTED, "band diagonal wider than " + std::to_string(ring_max));
     SA_HIP(ctx, b->d_fM.ensure(sizeof(T) * total_cells));
     SA_HIP(ctx, b->d_fX.ensure(sizeof(T) * std::max<int64_t>(total_xy, 1)));
     SA_HIP(ctx, b->d_fY.ensure(sizeof(T) * std::max<int64_t>(total_xy, 1)));
@@ -686,30 +726,92 @@ int run_batch(sa_batch *b, sa_result *res) {
     SA_HIP(ctx, b->d_win_total.ensure(sizeof(double) * std::max<int64_t>(nw, 1)));
     a.tasks = b->d_tasks.as<TaskDev>();
     a.wins = b->d_wins.as<WinDev>();
-    a.task_order = b->d_task_order.as<int32_t>();
-    a.win_order = b->d_win_order.as<int32_t>();
     a.fM = b->d_fM.as<T>(); a.fX = b->d_fX.as<T>(); a.fY = b->d_fY.as<T>();
     a.out = b->d_out.as<int32_t>();
     a.win_count = b->d_win_count.as<int32_t>();
     a.win_total = b->d_win_total.as<double>();
     a.ring = ring;
-    /* many sub-alignments: one wavefront each (occupancy); few: 4 wavefronts
-       per sub-alignment to shorten each diagonal step (latency) */
-    const int fnt = (nt >= 4 * std::max(ctx->n_cu, 1) || gmaxw <= 64) && gmaxw <= 256 ? 64 : 256;
-    const int fcpt = (gmaxw + fnt - 1) / fnt;
-    const int bcpt = (gmaxw + 63) / 64;
-    if (fcpt > 4 || bcpt > 8) return fail(ctx, SA_ERR_UNSUPPORTED, "band too wide for the kernels");
+    const int32_t *d_to = b->d_task_order.as<int32_t>();
+    const int32_t *d_wo = b->d_win_order.as<int32_t>();
+    const int lds_fnt = (g_toff[1] >= 4 * std::max(ctx->n_cu, 1) || lds_maxw <= 64) && lds_maxw <= 256 ? 64 : 256;
+    const int lds_fcpt = (lds_maxw + lds_fnt - 1) / lds_fnt;
+    const int lds_bcpt = (lds_maxw + 63) / 64;
+    if (g_toff[1] > 0 && (lds_fcpt > 4 || lds_bcpt > 8))
+        return fail(ctx, SA_ERR_UNSUPPORTED, "band too wide for the kernels");
 
-    SA_HIP(ctx, hipEventRecord(ctx->ev[2], st));
-    SA_HIP(ctx, sa::launch_forward<T>(a, (int)nt, fnt, fcpt, st));
-    SA_HIP(ctx, hipEventRecord(ctx->ev[3], st));
-    SA_HIP(ctx, sa::launch_backward<T>(a, (int)nw, bcpt, st));
-    SA_HIP(ctx, hipEventRecord(ctx->ev[4], st));
-    tm.forward_launches = 1;
-    tm.backward_launches = 1;
+    /* groups run on up to four streams after prep; each stream runs its groups'
+       forward then backward sweeps */
+    hipStream_t gst[4] = {st, ctx->aux[0], ctx->aux[1], ctx->aux[2]};
+    int n_active = 0, active[kGroups];
+    for (int g = 0; g < kGroups; g++)
+        if (g_toff[g + 1] > g_toff[g]) active[n_active++] = g;
+    double main_f = 0, main_b = 0;
+    int64_t main_cells = -1;
+    int main_cpt = 0;
+    auto run_groups = [&](bool fwd) -> int {
+        SA_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+        for (int q = 1; q < 4; q++) SA_HIP(ctx, hipStreamWaitEvent(gst[q], ctx->ev[2], 0));
+        for (int k = 0; k < n_active; k++) {
+            const int g = active[k];
+            hipStream_t sg = gst[k % 4];
+            const int nt_g = (int)(g_toff[g + 1] - g_toff[g]), nw_g = (int)(g_woff[g + 1] - g_woff[g]);
+            SA_HIP(ctx, hipEventRecord(ctx->gev[3 * k + 0], sg));
+            if (fwd) {
+                if (g == 0) {
+                    sa::Args<T> al = a;
+                    al.task_order = d_to + g_toff[g];
+                    SA_HIP(ctx, sa::launch_forward<T>(al, nt_g, lds_fnt, lds_fcpt, sg));
+                } else {
+                    SA_HIP(ctx, sa::launch_forward_reg<T>(a, d_to + g_toff[g], nt_g, kCpt[g - 1], sg));
+                }
+            }
+            SA_HIP(ctx, hipEventRecord(ctx->gev[3 * k + 1], sg));
+            if (g == 0) {
+                sa::Args<T> al = a;
+                al.win_order = d_wo + g_woff[g];
+                SA_HIP(ctx, sa::launch_backward<T>(al, nw_g, lds_bcpt, sg));
+            } else {
+                SA_HIP(ctx, sa::launch_backward_reg<T>(a, d_wo + g_woff[g], nw_g, kCpt[g - 1], sg));
+            }
+            SA_HIP(ctx, hipEventRecord(ctx->gev[3 * k + 2], sg));
+        }
+        for (int q = 1; q < 4; q++) {
+            SA_HIP(ctx, hipEventRecord(ctx->gev[24 + q], gst[q]));
+            SA_HIP(ctx, hipStreamWaitEvent(st, ctx->gev[24 + q], 0));
+        }
+        SA_HIP(ctx, hipEventRecord(ctx->ev[4], st));
+        return SA_OK;
+    };
+    auto group_times = [&](bool fwd) {
+        for (int k = 0; k < n_active; k++) {
+            float f = 0.f, bb = 0.f;
+            if (fwd) (void)hipEventElapsedTime(&f, ctx->gev[3 * k + 0], ctx->gev[3 * k + 1]);
+            (void)hipEventElapsedTime(&bb, ctx->gev[3 * k + 1], ctx->gev[3 * k + 2]);
+            tm.forward_ms += f;
+            tm.backward_ms += bb;
+            const int g = active[k];
+            if (g_cells[g] > main_cells) {
+                main_cells = g_cells[g];
+                main_f = f;
+                main_b = bb;
+                main_cpt = g == 0 ? 0 : kCpt[g - 1];
+            }
+        }
+    };
+    if ((rc = run_groups(true))) return rc;
+    tm.forward_launches = n_active;
+    tm.backward_launches = n_active;
     std::vector<int32_t> wcount(nw);
     SA_HIP(ctx, hipMemcpyAsync(wcount.data(), b->d_win_count.p, sizeof(int32_t) * nw, hipMemcpyDeviceToHost, st));
+    SA_HIP(ctx, hipMemcpyAsync(&err, b->d_err.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipStreamSynchronize(st));
+    if (err & sa::ERR_WINDOWS) return kRetryLds;
+    group_times(true);
+    {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[4]);
+        tm.fb_ms = ms;
+    }
     /* windows whose pairs overflowed their slot run again with the exact count */
     std::vector<int32_t> redo;
     for (int64_t w = 0; w < nw; w++)
@@ -721,20 +823,21 @@ int run_batch(sa_batch *b, sa_result *res) {
             wins[w].out_cap = wcount[w];
             extra += wcount[w];
         }
+        /* ensure() may reallocate and drop the first-pass pairs, so every window runs
+           again into the resized buffer */
         SA_HIP(ctx, b->d_out.ensure(sizeof(int32_t) * 3 * (out_total + extra)));
-        /* ensure() may have reallocated: the first-pass pairs of the other windows are gone,
-           so every window runs again into the resized buffer */
-        std::vector<int32_t> all(nw);
-        std::iota(all.begin(), all.end(), 0);
-        if ((rc = upload(ctx, b->d_wins, wins)) || (rc = upload(ctx, b->d_win_order, all))) return rc;
+        if ((rc = upload(ctx, b->d_wins, wins))) return rc;
         a.out = b->d_out.as<int32_t>();
         a.wins = b->d_wins.as<WinDev>();
-        a.win_order = b->d_win_order.as<int32_t>();
-        SA_HIP(ctx, sa::launch_backward<T>(a, (int)nw, bcpt, st));
-        tm.backward_launches++;
+        if ((rc = run_groups(false))) return rc;
+        tm.backward_launches += n_active;
         SA_HIP(ctx, hipMemcpyAsync(wcount.data(), b->d_win_count.p, sizeof(int32_t) * nw, hipMemcpyDeviceToHost, st));
         SA_HIP(ctx, hipStreamSynchronize(st));
     }
+    tm.main_forward_ms = main_f;
+    tm.main_backward_ms = main_b;
+    tm.main_cells = main_cells;
+    tm.main_cpt = main_cpt;
     /* final positions in the reference's list order: per read, sub-alignments in
        split order, each in reverse emission order */
     std::vector<int64_t> win_dst(nw);
@@ -780,8 +883,6 @@ int run_batch(sa_batch *b, sa_result *res) {
     }
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]); tm.prep_ms = ms;
-    (void)hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]); tm.forward_ms = ms;
-    (void)hipEventElapsedTime(&ms, ctx->ev[3], ctx->ev[4]); tm.backward_ms = ms;
     (void)hipEventElapsedTime(&ms, ctx->ev[5], ctx->ev[6]); tm.collect_ms = ms;
     (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[6]); tm.total_ms = ms;
     tm.cells = total_cells;
@@ -799,7 +900,14 @@ int sa_batch_run(sa_batch *b, sa_precision precision, sa_result **out) {
     std::lock_guard<std::mutex> lk(ctx->mu);
     (void)hipSetDevice(ctx->device);
     sa_result *res = new sa_result();
-    int rc = precision == SA_FP32 ? run_batch<float>(b, res) : run_batch<double>(b, res);
+    const char *kern = getenv("SA_KERNELS");
+    const bool lds = kern && std::strcmp(kern, "lds") == 0;
+    int rc = precision == SA_FP32 ? run_batch<float>(b, res, lds) : run_batch<double>(b, res, lds);
+    if (rc == kRetryLds) { /* a band escaped a register window: redo with the LDS sweeps */
+        delete res;
+        res = new sa_result();
+        rc = precision == SA_FP32 ? run_batch<float>(b, res, true) : run_batch<double>(b, res, true);
+    }
     if (rc) {
         delete res;
         return rc;
@@ -822,6 +930,52 @@ int sa_align_batch(sa_context *ctx, const sa_model *model, const sa_read *reads,
 int sa_align(sa_context *ctx, const sa_model *model, const sa_read *read, const sa_params *params,
              sa_precision precision, sa_result **out) {
     return sa_align_batch(ctx, model, read, 1, params, precision, out);
+}
+
+int sa_band_geometry(const int64_t *anchors, int64_t n_anchors, int64_t lX, int64_t lY, int64_t expansion,
+                     int32_t *xlo, int32_t *xhi) {
+    if (lX < 0 || lY < 0 || expansion < 0 || expansion % 2 != 0 || n_anchors < 0) return SA_ERR_INVALID_ARGUMENT;
+    std::vector<int32_t> ax(n_anchors), ay(n_anchors);
+    for (int64_t i = 0; i < n_anchors; i++) {
+        const int64_t x = anchors[2 * i] + 1, y = anchors[2 * i + 1] + 1;
+        if (x <= 0 || y <= 0 || x > lX || y > lY || (i > 0 && (x <= ax[i - 1] || y <= ay[i - 1])))
+            return SA_ERR_ANCHORS;
+        ax[i] = (int32_t)x;
+        ay[i] = (int32_t)y;
+    }
+    for (int64_t d = 0; d <= lX + lY; d++) {
+        int64_t L, R;
+        if (!sa::band_at(ax.data(), ay.data(), (int32_t)n_anchors, lX, lY, expansion, d, &L, &R)) return SA_ERR_ANCHORS;
+        xlo[d] = (int32_t)sa::div2(d + L);
+        xhi[d] = (int32_t)sa::div2(d + R);
+    }
+    return SA_OK;
+}
+
+int64_t sa_window_schedule(const int32_t *xlo, const int32_t *xhi, int64_t lX, int64_t lY, const sa_params *p,
+                           int32_t *out, int64_t cap) {
+    const int64_t N = lX + lY;
+    std::vector<WinDev> w(sa::window_cap(N, p->min_diags_between_traceback, p->traceback_diagonals));
+    auto width = [&](int64_t d) { return (int64_t)(xhi[d] - xlo[d] + 1); };
+    int32_t n = sa::window_schedule(N, p->min_diags_between_traceback, p->traceback_diagonals, p->diagonal_expansion,
+                                    width, w.data(), (int32_t)w.size(), 0);
+    for (int32_t i = 0; i < n && i < cap; i++) {
+        out[4 * i + 0] = w[i].T;
+        out[4 * i + 1] = w[i].D;
+        out[4 * i + 2] = w[i].TF;
+        out[4 * i + 3] = w[i].last;
+    }
+    return n;
+}
+
+int64_t sa_split_points(const int64_t *anchors, int64_t n_anchors, int64_t lX, int64_t lY, const sa_params *p,
+                        int32_t ragged_left, int32_t ragged_right, int64_t *out, int64_t cap) {
+    std::vector<int64_t> sp;
+    if (!split_points(anchors, n_anchors, lX, lY, p->split_matrix_bigger_than_this, ragged_left, ragged_right, sp))
+        return SA_ERR_ANCHORS;
+    const int64_t n = (int64_t)sp.size() / 4;
+    for (int64_t i = 0; i < 4 * std::min(n, cap); i++) out[i] = sp[i];
+    return n;
 }
 
 int64_t sa_result_n_reads(const sa_result *res) { return res ? (int64_t)res->reads.size() : 0; }

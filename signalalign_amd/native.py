@@ -58,13 +58,11 @@ class _ReadResult(ctypes.Structure):
 class _Timing(ctypes.Structure):
     _fields_ = [("prep_ms", ctypes.c_double), ("forward_ms", ctypes.c_double), ("backward_ms", ctypes.c_double),
                 ("collect_ms", ctypes.c_double), ("total_ms", ctypes.c_double), ("cells", ctypes.c_int64),
-                ("forward_launches", ctypes.c_int64), ("backward_launches", ctypes.c_int64)]
+                ("forward_launches", ctypes.c_int64), ("backward_launches", ctypes.c_int64),
+                ("fb_ms", ctypes.c_double), ("main_forward_ms", ctypes.c_double),
+                ("main_backward_ms", ctypes.c_double), ("main_cells", ctypes.c_int64), ("main_cpt", ctypes.c_int32)]
 
 
-EXPORTED = ["sa_params_default", "sa_device_count", "sa_context_create", "sa_context_destroy", "sa_model_create",
-            "sa_model_destroy", "sa_align", "sa_align_batch", "sa_batch_create", "sa_batch_run",
-            "sa_batch_destroy", "sa_result_n_reads", "sa_result_read", "sa_result_timing", "sa_result_free",
-            "sa_last_error"]
 
 _lib = None
 
@@ -99,8 +97,57 @@ def load():
     L.sa_result_free.argtypes = [vp]
     L.sa_last_error.argtypes = [vp]
     L.sa_last_error.restype = ctypes.c_char_p
+    I32P = ctypes.POINTER(ctypes.c_int32)
+    L.sa_band_geometry.argtypes = [_I64P, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, I32P, I32P]
+    L.sa_window_schedule.argtypes = [I32P, I32P, ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(_Params), I32P,
+                                     ctypes.c_int64]
+    L.sa_window_schedule.restype = ctypes.c_int64
+    L.sa_split_points.argtypes = [_I64P, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(_Params),
+                                  ctypes.c_int32, ctypes.c_int32, _I64P, ctypes.c_int64]
+    L.sa_split_points.restype = ctypes.c_int64
     _lib = L
     return L
+
+
+def band_geometry(anchors, lX, lY, expansion):
+    """Per-diagonal x ranges (xlo, xhi) of the band, computed by the library on the host."""
+    a = np.ascontiguousarray(np.asarray(anchors, dtype=np.int64).reshape(-1, 2))
+    lo = np.zeros(lX + lY + 1, dtype=np.int32)
+    hi = np.zeros(lX + lY + 1, dtype=np.int32)
+    I32P = ctypes.POINTER(ctypes.c_int32)
+    rc = load().sa_band_geometry(a.ctypes.data_as(_I64P), len(a), lX, lY, expansion, lo.ctypes.data_as(I32P),
+                                 hi.ctypes.data_as(I32P))
+    if rc != 0:
+        raise SignalAlignError(rc, "sa_band_geometry")
+    return lo, hi
+
+
+def window_schedule(xlo, xhi, lX, lY, params=None):
+    """Traceback windows (tracedBackTo, start diagonal, tracedBackFrom, last) as rows."""
+    params = params or Params()
+    I32P = ctypes.POINTER(ctypes.c_int32)
+    lo = np.ascontiguousarray(xlo, dtype=np.int32)
+    hi = np.ascontiguousarray(xhi, dtype=np.int32)
+    cap = (lX + lY) // 10 + 16
+    out = np.zeros((cap, 4), dtype=np.int32)
+    ps = params._struct()
+    n = load().sa_window_schedule(lo.ctypes.data_as(I32P), hi.ctypes.data_as(I32P), lX, lY, ctypes.byref(ps),
+                                  out.ctypes.data_as(I32P), cap)
+    return out[:n].copy()
+
+
+def split_points(anchors, lX, lY, params=None, ragged_left=True, ragged_right=True):
+    params = params or Params()
+    a = np.ascontiguousarray(np.asarray(anchors, dtype=np.int64).reshape(-1, 2))
+    ps = params._struct()
+    n = load().sa_split_points(a.ctypes.data_as(_I64P), len(a), lX, lY, ctypes.byref(ps), int(ragged_left),
+                               int(ragged_right), None, 0)
+    if n < 0:
+        raise SignalAlignError(int(n), "sa_split_points")
+    out = np.zeros((max(n, 1), 4), dtype=np.int64)
+    load().sa_split_points(a.ctypes.data_as(_I64P), len(a), lX, lY, ctypes.byref(ps), int(ragged_left),
+                           int(ragged_right), out.ctypes.data_as(_I64P), n)
+    return out[:n]
 
 
 def device_count():

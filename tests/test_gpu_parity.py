@@ -14,6 +14,14 @@ from tests import _parity as P
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(params=["reg", "lds"], autouse=True)
+def kernel_path(request, monkeypatch):
+    """Every case runs through both sweeps: the register-resident kernels (default)
+    and the LDS-ring kernels (the fallback for bands wider than a register window)."""
+    monkeypatch.setenv("SA_KERNELS", request.param)
+    return request.param
+
+
 @pytest.fixture(scope="module")
 def g6():
     return synth.gaussian_model("ACGT", 6, seed=1)
@@ -87,6 +95,20 @@ def test_split_blocks(gpu_ctx, g6):
     params = native.Params(diagonal_expansion=20, split_matrix_bigger_than_this=50 * 50)
     out = _check(gpu_ctx, g6, r, params, native.SA_FP64)
     assert out["n_ref"] > 0
+
+
+@pytest.mark.parametrize("expansion", [2, 20, 50, 100, 200, 300])
+def test_window_sizes(gpu_ctx, g6, expansion):
+    """Band expansions that need every register window size (64 to 512 positions)."""
+    r = synth.single_read(g6, 2500, seed=40 + expansion)
+    _check(gpu_ctx, g6, r, native.Params(diagonal_expansion=expansion), native.SA_FP64)
+
+
+def test_window_sizes_fp32(gpu_ctx, g6):
+    for expansion in (20, 100, 300):
+        r = synth.single_read(g6, 2500, seed=60 + expansion)
+        _check(gpu_ctx, g6, r, native.Params(diagonal_expansion=expansion), native.SA_FP32, p_tol=2e-3,
+               ll_rel=1e-5)
 
 
 def test_wide_expansion_and_threshold(gpu_ctx, g6):
