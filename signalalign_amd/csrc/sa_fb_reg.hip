@@ -30,8 +30,14 @@ namespace sa {
 /* per-position and per-event constants                                       */
 /* ------------------------------------------------------------------------ */
 
+/* EK 0: Gaussian tables as the .model loader builds them (extra-event table = match
+ * table with level_sd x 1.75, one skip emission for every k-mer): the extra-event
+ * emission follows from the match constants.  EK 1: any Gaussian tables.  EK 2: HDP. */
 template <typename T, int EK> struct KCon;
 template <typename T> struct KCon<T, 0> {
+    T mmu, misd, mca, mnmu, mnb;
+};
+template <typename T> struct KCon<T, 1> {
     T gx, mmu, misd, mca, mnmu, mnb, ymu, yisd, yca, ynmu, ynb;
 };
 template <typename T> struct KCon<T, 2> {
@@ -41,11 +47,22 @@ template <typename T> struct KCon<T, 2> {
 
 template <typename T, int EK> struct ECon;
 template <typename T> struct ECon<T, 0> { T m, n, l, i; };
+template <typename T> struct ECon<T, 1> { T m, n, l, i; };
 template <typename T> struct ECon<T, 2> { T m; };
 
 /* k-mer (x-1) of cell x; an absent k-mer (x = 0 or past the end) gives -inf emissions */
 template <typename T>
 __device__ __forceinline__ void load_kcon(KCon<T, 0> &k, const Model<T> &m, const int32_t *kid, int64_t pos,
+                                          bool ok) {
+    if (ok) {
+        const int32_t id = kid[pos];
+        k.mmu = m.mmu[id]; k.misd = m.misd[id]; k.mca = m.mca[id]; k.mnmu = m.mnmu[id]; k.mnb = m.mnb[id];
+    } else {
+        k.mmu = 0; k.misd = 0; k.mca = ninf<T>(); k.mnmu = 1; k.mnb = 0;
+    }
+}
+template <typename T>
+__device__ __forceinline__ void load_kcon(KCon<T, 1> &k, const Model<T> &m, const int32_t *kid, int64_t pos,
                                           bool ok) {
     if (ok) {
         const int32_t id = kid[pos];
@@ -74,8 +91,8 @@ __device__ __forceinline__ void load_kcon(KCon<T, 2> &k, const Model<T> &m, cons
 
 /* event j; an absent event gives finite constants (its cells have no in-band
  * neighbour that reads it) */
-template <typename T>
-__device__ __forceinline__ void load_econ(ECon<T, 0> &e, const Args<T> &a, int64_t j, bool ok) {
+template <typename T, int EK>
+__device__ __forceinline__ void load_econ(ECon<T, EK> &e, const Args<T> &a, int64_t j, bool ok) {
     if (ok) { e.m = a.evm[j]; e.n = a.evn[j]; e.l = a.evl[j]; e.i = a.evi[j]; }
     else { e.m = 0; e.n = 1; e.l = 0; e.i = 1; }
 }
@@ -84,19 +101,13 @@ __device__ __forceinline__ void load_econ(ECon<T, 2> &e, const Args<T> &a, int64
     e.m = ok ? a.evm[j] : (T)0;
 }
 
-template <typename T> __device__ __forceinline__ KCon<T, 0> kcon_invalid0() {
-    KCon<T, 0> k;
-    const T NI = ninf<T>();
-    k.gx = NI; k.mmu = 0; k.misd = 0; k.mca = NI; k.mnmu = 1; k.mnb = 0;
-    k.ymu = 0; k.yisd = 0; k.yca = NI; k.ynmu = 1; k.ynb = 0;
-    return k;
+template <typename T, int EK>
+__device__ __forceinline__ ECon<T, EK> shr1(const ECon<T, EK> &e, const ECon<T, EK> &f) {
+    return ECon<T, EK>{shr1(e.m, f.m), shr1(e.n, f.n), shr1(e.l, f.l), shr1(e.i, f.i)};
 }
-
-template <typename T> __device__ __forceinline__ ECon<T, 0> shr1(const ECon<T, 0> &e, const ECon<T, 0> &f) {
-    return ECon<T, 0>{shr1(e.m, f.m), shr1(e.n, f.n), shr1(e.l, f.l), shr1(e.i, f.i)};
-}
-template <typename T> __device__ __forceinline__ ECon<T, 0> shl1(const ECon<T, 0> &e, const ECon<T, 0> &f) {
-    return ECon<T, 0>{shl1(e.m, f.m), shl1(e.n, f.n), shl1(e.l, f.l), shl1(e.i, f.i)};
+template <typename T, int EK>
+__device__ __forceinline__ ECon<T, EK> shl1(const ECon<T, EK> &e, const ECon<T, EK> &f) {
+    return ECon<T, EK>{shl1(e.m, f.m), shl1(e.n, f.n), shl1(e.l, f.l), shl1(e.i, f.i)};
 }
 template <typename T> __device__ __forceinline__ ECon<T, 2> shr1(const ECon<T, 2> &e, const ECon<T, 2> &f) {
     return ECon<T, 2>{shr1(e.m, f.m)};
@@ -107,7 +118,11 @@ template <typename T> __device__ __forceinline__ ECon<T, 2> shl1(const ECon<T, 2
 
 /* the match/gapX constants of the next position, for the backward sweep */
 template <typename T> __device__ __forceinline__ KCon<T, 0> shl1_match(const KCon<T, 0> &k) {
-    KCon<T, 0> r = k;
+    return KCon<T, 0>{shl1(k.mmu, (T)0), shl1(k.misd, (T)0), shl1(k.mca, ninf<T>()), shl1(k.mnmu, (T)1),
+                      shl1(k.mnb, (T)0)};
+}
+template <typename T> __device__ __forceinline__ KCon<T, 1> shl1_match(const KCon<T, 1> &k) {
+    KCon<T, 1> r = k;
     const T NI = ninf<T>();
     r.gx = shl1(k.gx, NI);
     r.mmu = shl1(k.mmu, (T)0); r.misd = shl1(k.misd, (T)0); r.mca = shl1(k.mca, NI);
@@ -123,11 +138,42 @@ template <typename T>
 __device__ __forceinline__ T em_match(const Model<T> &, const KCon<T, 0> &k, const ECon<T, 0> &e, T s, T) {
     return gauss_emit(k.mmu, k.misd, k.mca, k.mnmu, k.mnb, e.m, e.n, e.l, e.i, s);
 }
+/* extra-event table = match table with sd x 1.75: a/1.75 and -log(1.75) */
 template <typename T>
 __device__ __forceinline__ T em_gapy(const Model<T> &, const KCon<T, 0> &k, const ECon<T, 0> &e, T s, T) {
+    const T a = (e.m - s * k.mmu) * k.misd;
+    const T u = e.n - k.mnmu;
+    return ((k.mca - (T)0.55961578793542266) - (T)0.16326530612244898 * (a * a)) + e.l - k.mnb * (u * u * e.i);
+}
+template <typename T>
+__device__ __forceinline__ void em_both(const Model<T> &, const KCon<T, 0> &k, const ECon<T, 0> &e, T s, T,
+                                        T &eM, T &eY) {
+    const T a = (e.m - s * k.mmu) * k.misd;
+    const T u = e.n - k.mnmu;
+    const T noise = e.l - k.mnb * (u * u * e.i);
+    const T a2 = a * a;
+    eM = (k.mca - (T)0.5 * a2) + noise;
+    eY = ((k.mca - (T)0.55961578793542266) - (T)0.16326530612244898 * a2) + noise;
+}
+template <typename T>
+__device__ __forceinline__ T em_gapx(const Model<T> &m, const KCon<T, 0> &k) {
+    return k.mca > ninf<T>() ? m.gx_const : ninf<T>();
+}
+template <typename T>
+__device__ __forceinline__ T em_match(const Model<T> &, const KCon<T, 1> &k, const ECon<T, 1> &e, T s, T) {
+    return gauss_emit(k.mmu, k.misd, k.mca, k.mnmu, k.mnb, e.m, e.n, e.l, e.i, s);
+}
+template <typename T>
+__device__ __forceinline__ T em_gapy(const Model<T> &, const KCon<T, 1> &k, const ECon<T, 1> &e, T s, T) {
     return gauss_emit(k.ymu, k.yisd, k.yca, k.ynmu, k.ynb, e.m, e.n, e.l, e.i, s);
 }
-template <typename T> __device__ __forceinline__ T em_gapx(const KCon<T, 0> &k) { return k.gx; }
+template <typename T>
+__device__ __forceinline__ void em_both(const Model<T> &m, const KCon<T, 1> &k, const ECon<T, 1> &e, T s, T iv,
+                                        T &eM, T &eY) {
+    eM = em_match<T>(m, k, e, s, iv);
+    eY = em_gapy<T>(m, k, e, s, iv);
+}
+template <typename T> __device__ __forceinline__ T em_gapx(const Model<T> &, const KCon<T, 1> &k) { return k.gx; }
 template <typename T>
 __device__ __forceinline__ T em_match(const Model<T> &m, const KCon<T, 2> &k, const ECon<T, 2> &e, T s, T iv) {
     return hdp_emit_row(m, k.row, k.hmu, e.m, s, iv);
@@ -136,7 +182,54 @@ template <typename T>
 __device__ __forceinline__ T em_gapy(const Model<T> &m, const KCon<T, 2> &k, const ECon<T, 2> &e, T s, T iv) {
     return hdp_emit_row(m, k.row, k.hmu, e.m, s, iv);
 }
-template <typename T> __device__ __forceinline__ T em_gapx(const KCon<T, 2> &) { return (T)-2.3025850929940455; }
+template <typename T>
+__device__ __forceinline__ void em_both(const Model<T> &m, const KCon<T, 2> &k, const ECon<T, 2> &e, T s, T iv,
+                                        T &eM, T &eY) {
+    eM = hdp_emit_row(m, k.row, k.hmu, e.m, s, iv);
+    eY = eM;
+}
+template <typename T>
+__device__ __forceinline__ T em_gapx(const Model<T> &, const KCon<T, 2> &) { return (T)-2.3025850929940455; }
+
+/* Uniform-address loads through the constant address space become scalar loads:
+ * they are counted by lgkmcnt, so waiting for one does not wait for the vector
+ * stores of the forward planes (vmcnt counts vector loads and stores together). */
+template <typename T> __device__ __forceinline__ T ld_scalar(const T *p, int i) {
+    typedef __attribute__((address_space(4))) const T CT;
+    return ((const CT *)p)[__builtin_amdgcn_readfirstlane(i)];
+}
+template <typename T>
+__device__ __forceinline__ void load_econ_uniform(ECon<T, 2> &e, const Args<T> &a, int64_t base, int j, bool ok) {
+    e.m = ok ? ld_scalar(a.evm + base, j) : (T)0;
+}
+template <typename T, int EK>
+__device__ __forceinline__ void load_econ_uniform(ECon<T, EK> &e, const Args<T> &a, int64_t base, int j, bool ok) {
+    if (ok) {
+        e.m = ld_scalar(a.evm + base, j); e.n = ld_scalar(a.evn + base, j);
+        e.l = ld_scalar(a.evl + base, j); e.i = ld_scalar(a.evi + base, j);
+    } else {
+        e.m = 0; e.n = 1; e.l = 0; e.i = 1;
+    }
+}
+
+/* Make the window-slide reloads complete inside the (rare) slide branch: an empty
+ * asm that reads the registers.  Otherwise the first use on the common path
+ * carries a vmcnt wait that also waits for the previous diagonal's stores. */
+__device__ __forceinline__ void sink(float v) { asm volatile("" ::"v"(v)); }
+__device__ __forceinline__ void sink(double v) { asm volatile("" ::"v"(v)); }
+__device__ __forceinline__ void sink(int v) { asm volatile("" ::"v"(v)); }
+template <typename T> __device__ __forceinline__ void sink(const KCon<T, 0> &k) {
+    sink(k.mmu); sink(k.misd); sink(k.mca); sink(k.mnmu); sink(k.mnb);
+}
+template <typename T> __device__ __forceinline__ void sink(const KCon<T, 1> &k) {
+    sink(k.gx); sink(k.mmu); sink(k.misd); sink(k.mca); sink(k.mnmu); sink(k.mnb);
+    sink(k.ymu); sink(k.yisd); sink(k.yca); sink(k.ynmu); sink(k.ynb);
+}
+template <typename T> __device__ __forceinline__ void sink(const KCon<T, 2> &k) { sink(k.hmu); sink(k.row); }
+template <typename T> __device__ __forceinline__ void sink(const ECon<T, 2> &e) { sink(e.m); }
+template <typename T, int EK> __device__ __forceinline__ void sink(const ECon<T, EK> &e) {
+    sink(e.m); sink(e.n); sink(e.l); sink(e.i);
+}
 
 template <typename T> __device__ __forceinline__ T shfl_idx(T v, int src, T fill) {
     T r = __shfl(v, src & 63, 64);
@@ -149,14 +242,22 @@ __device__ __forceinline__ int rdl(int v, int lane) { return __builtin_amdgcn_re
 /* forward                                                                   */
 /* ------------------------------------------------------------------------ */
 
-template <typename T, int CPT, int EK>
-__global__ void __launch_bounds__(64) forward_reg_kernel(Args<T> a, const int32_t *__restrict__ order) {
+template <typename T, int CPT, int EK, int NW>
+__global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const int32_t *__restrict__ order) {
     constexpr bool NORM = sizeof(T) == 4;
-    constexpr int W = 64 * CPT;
+    constexpr int WL = 64 * NW;   /* lanes of the workgroup */
+    constexpr int W = WL * CPT;   /* positions of the window */
+    /* NW > 1: the last cell of each wavefront crosses to the next through LDS,
+       double-buffered by diagonal parity (one barrier per diagonal) */
+    __shared__ T bx[2][NW][3];
+    __shared__ ECon<T, EK> bev[2][NW];
+    __shared__ T red[NW];
+    __shared__ T shbuf[NW > 1 ? 6 * W : 1];
     const int ti = order[blockIdx.x];
     const TaskDev t = a.tasks[ti];
     const Model<T> &m = a.m;
-    const int lane = threadIdx.x;
+    const int gl = threadIdx.x;
+    const int lane = gl & 63, wave = gl >> 6;
     const int lX = t.lX, lY = t.lY, N = lX + lY;
     const int32_t *xlo = a.xlo + t.diag_off;
     const int32_t *xhi = a.xhi + t.diag_off;
@@ -198,7 +299,7 @@ __global__ void __launch_bounds__(64) forward_reg_kernel(Args<T> a, const int32_
     const int lo0 = rdl(cl, 0), hi0 = rdl(ch, 0), f0 = rdl(cf, 0);
 #pragma unroll
     for (int c = 0; c < CPT; c++) {
-        const int x = CPT * lane + c;
+        const int x = CPT * gl + c;
         const bool v = x >= lo0 && x <= hi0;
         pM[c] = v ? (t.ragged_left ? NI : (T)0) : NI;
         pX[c] = v ? (t.ragged_left ? (T)0 : NI) : NI;
@@ -212,19 +313,25 @@ __global__ void __launch_bounds__(64) forward_reg_kernel(Args<T> a, const int32_
         const int j = 1 - x - 1;
         load_econ(ev[c], a, evc + j, j >= 0 && j < lY);
     }
-    if (lane == 0) { md_doff = 0; md_dxy = f0 ? 0 : -1; md_foff = 0.0; }
+    if (gl == 0) { md_doff = 0; md_dxy = f0 ? 0 : -1; md_foff = 0.0; }
+    if constexpr (NW > 1) {
+        if (lane == 63) { bx[0][wave][0] = pM[CPT - 1]; bx[0][wave][1] = pX[CPT - 1]; bx[0][wave][2] = pY[CPT - 1]; }
+        __syncthreads();
+    }
     int64_t run = hi0 - lo0 + 1;
     int64_t runxy = f0 ? run : 0;
     double off1 = 0.0, off2 = 0.0;
     int lo1 = lo0, lo2 = lo0;
     ECon<T, EK> pre;
-    load_econ(pre, a, evc + (1 - X0), lane == 0 && 1 - X0 >= 0 && 1 - X0 < lY);
+    load_econ_uniform(pre, a, evc, 1 - X0, 1 - X0 >= 0 && 1 - X0 < lY);
     bool fresh = true;
 
     for (int d = 1; d <= N; d++) {
         int j = d - dc;
         if (j == 64) {
-            if (dc + lane <= N) { doff[dc + lane] = md_doff; doffxy[dc + lane] = md_dxy; foff[dc + lane] = md_foff; }
+            if (wave == 0 && dc + lane <= N) {
+                doff[dc + lane] = md_doff; doffxy[dc + lane] = md_dxy; foff[dc + lane] = md_foff;
+            }
             dc += 64;
             j = 0;
             cl = nl; ch = nh; cf = nf;
@@ -233,8 +340,17 @@ __global__ void __launch_bounds__(64) forward_reg_kernel(Args<T> a, const int32_
         }
         const int lo = rdl(cl, j), hi = rdl(ch, j), fl = rdl(cf, j);
         if (lo < X0) { /* band bounds are non-decreasing; anything else goes to the LDS kernels */
-            if (lane == 0) atomicOr(a.err, ERR_WINDOWS);
+            if (gl == 0) atomicOr(a.err, ERR_WINDOWS);
             return;
+        }
+        /* fill values for the first lane of each wavefront: the last cell of the
+           previous wavefront on d-1 (and on d-2 for the middle state) */
+        T fLM = NI, fLX = NI, fLY = NI;
+        if constexpr (NW > 1) {
+            if (wave > 0) {
+                const int pb = (d - 1) & 1;
+                fLM = bx[pb][wave - 1][0]; fLX = bx[pb][wave - 1][1]; fLY = bx[pb][wave - 1][2];
+            }
         }
         if (hi > X0 + W - 1) {
             /* slide the window as far as the two previous diagonals allow */
@@ -242,53 +358,84 @@ __global__ void __launch_bounds__(64) forward_reg_kernel(Args<T> a, const int32_
             const int smax = (minlo - X0) / CPT;
             const int sreq = (hi - (X0 + W - 1) + CPT - 1) / CPT;
             if (smax < sreq) {
-                if (lane == 0) atomicOr(a.err, ERR_WINDOWS);
+                if (gl == 0) atomicOr(a.err, ERR_WINDOWS);
                 return;
             }
+            if constexpr (NW == 1) {
 #pragma unroll
-            for (int c = 0; c < CPT; c++) {
-                pM[c] = shfl_idx(pM[c], lane + smax, NI); pX[c] = shfl_idx(pX[c], lane + smax, NI);
-                pY[c] = shfl_idx(pY[c], lane + smax, NI);
-                qM[c] = shfl_idx(qM[c], lane + smax, NI); qX[c] = shfl_idx(qX[c], lane + smax, NI);
-                qY[c] = shfl_idx(qY[c], lane + smax, NI);
+                for (int c = 0; c < CPT; c++) {
+                    pM[c] = shfl_idx(pM[c], lane + smax, NI); pX[c] = shfl_idx(pX[c], lane + smax, NI);
+                    pY[c] = shfl_idx(pY[c], lane + smax, NI);
+                    qM[c] = shfl_idx(qM[c], lane + smax, NI); qX[c] = shfl_idx(qX[c], lane + smax, NI);
+                    qY[c] = shfl_idx(qY[c], lane + smax, NI);
+                }
+                LpM = shr1(qM[CPT - 1], NI); LpX = shr1(qX[CPT - 1], NI); LpY = shr1(qY[CPT - 1], NI);
+            } else {
+#pragma unroll
+                for (int c = 0; c < CPT; c++) {
+                    const int i = CPT * gl + c;
+                    shbuf[0 * W + i] = pM[c]; shbuf[1 * W + i] = pX[c]; shbuf[2 * W + i] = pY[c];
+                    shbuf[3 * W + i] = qM[c]; shbuf[4 * W + i] = qX[c]; shbuf[5 * W + i] = qY[c];
+                }
+                __syncthreads();
+                const bool in = gl + smax < WL;
+#pragma unroll
+                for (int c = 0; c < CPT; c++) {
+                    const int i = CPT * (gl + smax) + c;
+                    pM[c] = in ? shbuf[0 * W + i] : NI; pX[c] = in ? shbuf[1 * W + i] : NI;
+                    pY[c] = in ? shbuf[2 * W + i] : NI; qM[c] = in ? shbuf[3 * W + i] : NI;
+                    qX[c] = in ? shbuf[4 * W + i] : NI; qY[c] = in ? shbuf[5 * W + i] : NI;
+                }
+                /* the previous position's last cell: source lane gl-1+smax */
+                const int sl = gl - 1 + smax;
+                const bool ok = sl >= 0 && sl < WL;
+                const int i = CPT * sl + CPT - 1;
+                fLM = ok ? shbuf[0 * W + i] : NI; fLX = ok ? shbuf[1 * W + i] : NI; fLY = ok ? shbuf[2 * W + i] : NI;
+                LpM = ok ? shbuf[3 * W + i] : NI; LpX = ok ? shbuf[4 * W + i] : NI; LpY = ok ? shbuf[5 * W + i] : NI;
+                __syncthreads();
             }
             X0 += smax * CPT;
-            LpM = shr1(qM[CPT - 1], NI); LpX = shr1(qX[CPT - 1], NI); LpY = shr1(qY[CPT - 1], NI);
 #pragma unroll
             for (int c = 0; c < CPT; c++) {
-                const int x = X0 + CPT * lane + c;
+                const int x = X0 + CPT * gl + c;
                 load_kcon(kc[c], m, kid, x - 1, x >= 1 && x <= lX);
                 const int jj = d - x - 1;
                 load_econ(ev[c], a, evc + jj, jj >= 0 && jj < lY);
             }
+#pragma unroll
+            for (int c = 0; c < CPT; c++) { sink(kc[c]); sink(ev[c]); }
             fresh = true;
         }
         if (!fresh) {
             /* cell x on d reads event d-x-1: its left neighbour's event of d-1 */
-            const ECon<T, EK> in = shr1(ev[CPT - 1], pre);
+            ECon<T, EK> fill = pre;
+            if constexpr (NW > 1) {
+                if (wave > 0) fill = bev[(d - 1) & 1][wave - 1];
+            }
+            const ECon<T, EK> inc = shr1(ev[CPT - 1], fill);
 #pragma unroll
             for (int c = CPT - 1; c >= 1; c--) ev[c] = ev[c - 1];
-            ev[0] = in;
+            ev[0] = inc;
         }
         fresh = false;
         {
-            const int jn = d - X0; /* event of lane 0's first cell on d+1 */
-            load_econ(pre, a, evc + jn, lane == 0 && jn >= 0 && jn < lY);
+            const int jn = d - X0; /* event of the first cell of the window on d+1 */
+            load_econ_uniform(pre, a, evc, jn, jn >= 0 && jn < lY);
         }
-        const T LM = shr1(pM[CPT - 1], NI), LX = shr1(pX[CPT - 1], NI), LY = shr1(pY[CPT - 1], NI);
+        const T LM = shr1(pM[CPT - 1], fLM), LX = shr1(pX[CPT - 1], fLX), LY = shr1(pY[CPT - 1], fLY);
         const T dl2 = NORM ? (T)(off2 - off1) : (T)0;
         T nM[CPT], nX[CPT], nY[CPT];
         T mx = NI;
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
-            const int x = X0 + CPT * lane + c;
+            const int x = X0 + CPT * gl + c;
             const bool valid = x >= lo && x <= hi;
             const T lM = c ? pM[c - 1] : LM, lXv = c ? pX[c - 1] : LX, lYv = c ? pY[c - 1] : LY;
             T mM = c ? qM[c - 1] : LpM, mXv = c ? qX[c - 1] : LpX, mYv = c ? qY[c - 1] : LpY;
             if (NORM) { mM += dl2; mXv += dl2; mYv += dl2; }
-            const T eX = em_gapx<T>(kc[c]);
-            const T eM = em_match<T>(m, kc[c], ev[c], s, iv);
-            const T eY = EK == 2 ? eM : em_gapy<T>(m, kc[c], ev[c], s, iv);
+            const T eX = em_gapx<T>(m, kc[c]);
+            T eM, eY;
+            em_both<T>(m, kc[c], ev[c], s, iv, eM, eY);
             /* lower -> gapX, middle -> match, upper -> gapY */
             T X = lM + (eX + tGOX);
             X = logadd(X, lXv + (eX + tGEX));
@@ -310,6 +457,13 @@ __global__ void __launch_bounds__(64) forward_reg_kernel(Args<T> a, const int32_
         double offd = off1;
         if (NORM && (d & 7) == 0) {
             mx = wave_max(mx);
+            if constexpr (NW > 1) {
+                if (lane == 0) red[wave] = mx;
+                __syncthreads();
+                mx = red[0];
+#pragma unroll
+                for (int w = 1; w < NW; w++) mx = red[w] > mx ? red[w] : mx;
+            }
             if (mx > NI) {
 #pragma unroll
                 for (int c = 0; c < CPT; c++) { nM[c] -= mx; nX[c] -= mx; nY[c] -= mx; }
@@ -318,15 +472,23 @@ __global__ void __launch_bounds__(64) forward_reg_kernel(Args<T> a, const int32_
         }
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
-            const int x = X0 + CPT * lane + c;
+            const int x = X0 + CPT * gl + c;
             if (x >= lo && x <= hi) {
                 fM[run + (x - lo)] = nM[c];
                 if (fl) { fX[runxy + (x - lo)] = nX[c]; fY[runxy + (x - lo)] = nY[c]; }
             }
         }
-        if (lane == j) { md_doff = (int32_t)run; md_dxy = fl ? (int32_t)runxy : -1; md_foff = offd; }
+        if (gl == j) { md_doff = (int32_t)run; md_dxy = fl ? (int32_t)runxy : -1; md_foff = offd; }
         run += hi - lo + 1;
         if (fl) runxy += hi - lo + 1;
+        if constexpr (NW > 1) {
+            if (lane == 63) {
+                const int pb = d & 1;
+                bx[pb][wave][0] = nM[CPT - 1]; bx[pb][wave][1] = nX[CPT - 1]; bx[pb][wave][2] = nY[CPT - 1];
+                bev[pb][wave] = ev[CPT - 1];
+            }
+            __syncthreads();
+        }
         LpM = LM; LpX = LX; LpY = LY;
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
@@ -338,21 +500,46 @@ __global__ void __launch_bounds__(64) forward_reg_kernel(Args<T> a, const int32_
         lo2 = lo1;
         lo1 = lo;
     }
-    if (dc + lane <= N) { doff[dc + lane] = md_doff; doffxy[dc + lane] = md_dxy; foff[dc + lane] = md_foff; }
+    if (wave == 0 && dc + lane <= N) { doff[dc + lane] = md_doff; doffxy[dc + lane] = md_dxy; foff[dc + lane] = md_foff; }
 }
 
 /* ------------------------------------------------------------------------ */
 /* backward + total refresh + posterior pairs                                 */
 /* ------------------------------------------------------------------------ */
 
-__device__ __forceinline__ double chain_terms(const double *term, int n) {
-    double acc = -__builtin_huge_val();
-    double pm = -__builtin_huge_val();
-    for (int i = 0; i < n; i++) {
-        const double v = term[i];
-        if (!(v > pm - 7.5)) continue; /* logAdd(acc, v) == acc exactly when acc - v >= 7.5 */
-        acc = logadd(acc, v);
-        pm = v > pm ? v : pm;
+/* Sequential log-add chain of dpDiagonal_dotProduct (impl/pairwiseAligner.c:1014-1027)
+ * over the terms held CPT per lane in x order.  Terms at least 7.5 below the running
+ * maximum are exact no-ops of logAdd and are skipped; terms more than 40 below the
+ * largest term are skipped too (their combined weight is below 1e-15 of the total).
+ * The remaining few terms are visited in order with a ballot walk. */
+template <int CPT>
+__device__ __forceinline__ double chain_reg(const double (&t)[CPT]) {
+    double mx = -__builtin_huge_val();
+#pragma unroll
+    for (int c = 0; c < CPT; c++) mx = t[c] > mx ? t[c] : mx;
+    mx = wave_max(mx);
+    const double cut = mx - 40.0;
+    unsigned long long act[CPT];
+    unsigned long long any = 0;
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+        act[c] = __ballot(t[c] > cut);
+        any |= act[c];
+    }
+    double acc = -__builtin_huge_val(), pm = -__builtin_huge_val();
+    while (any) {
+        const int L = __builtin_ctzll(any);
+        any &= any - 1;
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            if ((act[c] >> L) & 1ull) {
+                const double v = __shfl(t[c], L, 64);
+                if (v > pm - 7.5) {
+                    acc = logadd(acc, v);
+                    pm = v > pm ? v : pm;
+                }
+            }
+        }
     }
     return acc;
 }
@@ -361,9 +548,6 @@ template <typename T, int CPT, int EK>
 __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32_t *__restrict__ order) {
     constexpr bool NORM = sizeof(T) == 4;
     constexpr int W = 64 * CPT;
-    __shared__ double termA[W];
-    __shared__ double termB[W];
-    __shared__ double bcast;
     const int wi = order[blockIdx.x];
     const WinDev Wd = a.wins[wi];
     const TaskDev t = a.tasks[Wd.task];
@@ -430,7 +614,7 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
     ECon<T, EK> pre;
     {
         const int jn = (D - 1) - (X0 + W - 1); /* event of lane 63's last cell on D-1 */
-        load_econ(pre, a, evc + jn, lane == 63 && jn >= 0 && jn < lY);
+        load_econ_uniform(pre, a, evc, jn, jn >= 0 && jn < lY);
     }
     double ob1 = 0.0, ob2 = 0.0; /* offsets of B[e+1], B[e+2] */
     int hi1 = -1, hi2 = -1;      /* xhi of e+1, e+2 (-1: outside the window of diagonals) */
@@ -483,6 +667,8 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
                     const int jj = e - x;
                     load_econ(ev[c], a, evc + jj, jj >= 0 && jj < lY);
                 }
+#pragma unroll
+                for (int c = 0; c < CPT; c++) { sink(kc[c]); sink(ev[c]); }
                 fresh = true;
             }
             if (hi > X0 + W - 1) {
@@ -511,7 +697,7 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
                 const T CupY = pY[c];
                 T CmidM = c < CPT - 1 ? qM[c + 1] : RpM;
                 if (NORM) CmidM += dl2;
-                const T eX = em_gapx<T>(kx);
+                const T eX = em_gapx<T>(m, kx);
                 const T eM = em_match<T>(m, kx, ev[c], s, iv);
                 const T eY = em_gapy<T>(m, kc[c], ev[c], s, iv);
                 /* contributions arrive middle (e+2), then upper, then lower (e+1) */
@@ -544,7 +730,7 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
         }
         {
             const int jn = (e - 1) - (X0 + W - 1); /* event of lane 63's last cell on e-1 */
-            load_econ(pre, a, evc + jn, lane == 63 && jn >= 0 && jn < lY);
+            load_econ_uniform(pre, a, evc, jn, jn >= 0 && jn < lY);
         }
 
         if (e <= TF) {
@@ -552,29 +738,31 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
             if (posterior_count % 10 == 0) {
                 /* diagonalCalculationTotalProbability (impl/pairwiseAligner.c:1182-1200) */
                 const int dxy = doffxy[e];
+                double tA[CPT], tB[CPT];
 #pragma unroll
                 for (int c = 0; c < CPT; c++) {
                     const int x = X0 + CPT * lane + c;
+                    tA[c] = -__builtin_huge_val();
+                    tB[c] = -__builtin_huge_val();
                     if (x >= lo && x <= hi) {
                         const int i = x - lo;
                         T cell = fM[dof + i] + nM[c];
                         cell = logadd(cell, fX[dxy + i] + nX[c]);
                         cell = logadd(cell, fY[dxy + i] + nY[c]);
-                        termA[i] = (double)cell + offe;
+                        tA[c] = (double)cell + offe;
                     }
                 }
                 const bool h1 = e + 1 <= D;
-                int w1 = 0;
                 if (h1) {
-                    const int lo1e = xlo[e + 1];
-                    w1 = xhi[e + 1] - lo1e + 1;
+                    /* match flux: B[e+1] against match transitions out of F[e-1] */
+                    const int lo1e = xlo[e + 1], hi1e = xhi[e + 1];
                     const int lom = xlo[e - 1], him = xhi[e - 1];
                     const int dm = doff[e - 1], dxym = doffxy[e - 1];
                     const double offm = NORM ? foff[e - 1] + ob1 : 0.0;
 #pragma unroll
                     for (int c = 0; c < CPT; c++) {
                         const int x = X0 + CPT * lane + c; /* cell (x, e+1-x); middle (x-1, e-x) on e-1 */
-                        if (x >= lo1e && x < lo1e + w1) {
+                        if (x >= lo1e && x <= hi1e) {
                             const bool hm = x - 1 >= lom && x - 1 <= him && x >= 1 && e + 1 - x >= 1;
                             T Mv = NI;
                             if (hm) {
@@ -584,18 +772,13 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
                                 Mv = logadd(Mv, fX[dxym + fi] + (eM + tMFX));
                                 Mv = logadd(Mv, fY[dxym + fi] + (eM + tMFY));
                             }
-                            termB[x - lo1e] = (double)(Mv + pM[c]) + offm;
+                            tB[c] = (double)(Mv + pM[c]) + offm;
                         }
                     }
                 }
-                __syncthreads();
-                if (lane == 0) {
-                    double tot = chain_terms(termA, hi - lo + 1);
-                    if (h1) tot = logadd(tot, chain_terms(termB, w1));
-                    bcast = tot;
-                }
-                __syncthreads();
-                total = bcast;
+                double tot = chain_reg<CPT>(tA);
+                if (h1) tot = logadd(tot, chain_reg<CPT>(tB));
+                total = tot;
                 if (e == TF && lane == 0) a.win_total[wi] = total;
             }
             posterior_count++;
@@ -662,53 +845,73 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
 /* launchers                                                                 */
 /* ------------------------------------------------------------------------ */
 
-template <typename T, int CPT>
-static hipError_t fwd_reg_inst(const Args<T> &a, const int32_t *order, int n, int ek, hipStream_t st) {
-    if (ek == 2) hipLaunchKernelGGL((forward_reg_kernel<T, CPT, 2>), dim3(n), dim3(64), 0, st, a, order);
-    else hipLaunchKernelGGL((forward_reg_kernel<T, CPT, 0>), dim3(n), dim3(64), 0, st, a, order);
+template <typename T, int CPT, int EK>
+static hipError_t fwd_reg_inst(const Args<T> &a, const int32_t *order, int n, int nw, hipStream_t st) {
+    if constexpr (CPT <= 2) {
+        if (nw == 4) {
+            hipLaunchKernelGGL((forward_reg_kernel<T, CPT, EK, 4>), dim3(n), dim3(256), 0, st, a, order);
+            return hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL((forward_reg_kernel<T, CPT, EK, 1>), dim3(n), dim3(64), 0, st, a, order);
     return hipGetLastError();
 }
 
 template <typename T, int CPT>
-static hipError_t bwd_reg_inst(const Args<T> &a, const int32_t *order, int n, int ek, hipStream_t st) {
-    if (ek == 2) hipLaunchKernelGGL((backward_reg_kernel<T, CPT, 2>), dim3(n), dim3(64), 0, st, a, order);
-    else hipLaunchKernelGGL((backward_reg_kernel<T, CPT, 0>), dim3(n), dim3(64), 0, st, a, order);
+static hipError_t fwd_reg_ek(const Args<T> &a, const int32_t *order, int n, int nw, int ek, hipStream_t st) {
+    switch (ek) {
+    case 0: return fwd_reg_inst<T, CPT, 0>(a, order, n, nw, st);
+    case 1: return fwd_reg_inst<T, CPT, 1>(a, order, n, nw, st);
+    default: return fwd_reg_inst<T, CPT, 2>(a, order, n, nw, st);
+    }
+}
+
+template <typename T, int CPT>
+static hipError_t bwd_reg_ek(const Args<T> &a, const int32_t *order, int n, int ek, hipStream_t st) {
+    switch (ek) {
+    case 0: hipLaunchKernelGGL((backward_reg_kernel<T, CPT, 0>), dim3(n), dim3(64), 0, st, a, order); break;
+    case 1: hipLaunchKernelGGL((backward_reg_kernel<T, CPT, 1>), dim3(n), dim3(64), 0, st, a, order); break;
+    default: hipLaunchKernelGGL((backward_reg_kernel<T, CPT, 2>), dim3(n), dim3(64), 0, st, a, order); break;
+    }
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_forward_reg(const Args<T> &a, const int32_t *order, int n, int cpt, hipStream_t st) {
+hipError_t launch_forward_reg(const Args<T> &a, const int32_t *order, int n, int cpt, int nw, int ek,
+                              hipStream_t st) {
     if (n == 0) return hipSuccess;
-    const int ek = a.m.kind == 2 ? 2 : 0;
+    if (nw != 1 && nw != 4) return hipErrorInvalidValue;
+    if (nw == 4 && cpt > 2) return hipErrorInvalidValue;
     switch (cpt) {
-    case 1: return fwd_reg_inst<T, 1>(a, order, n, ek, st);
-    case 2: return fwd_reg_inst<T, 2>(a, order, n, ek, st);
-    case 3: return fwd_reg_inst<T, 3>(a, order, n, ek, st);
-    case 4: return fwd_reg_inst<T, 4>(a, order, n, ek, st);
-    case 6: return fwd_reg_inst<T, 6>(a, order, n, ek, st);
-    case 8: return fwd_reg_inst<T, 8>(a, order, n, ek, st);
+    case 1: return fwd_reg_ek<T, 1>(a, order, n, nw, ek, st);
+    case 2: return fwd_reg_ek<T, 2>(a, order, n, nw, ek, st);
+    case 3: return fwd_reg_ek<T, 3>(a, order, n, 1, ek, st);
+    case 4: return fwd_reg_ek<T, 4>(a, order, n, 1, ek, st);
+    case 6: return fwd_reg_ek<T, 6>(a, order, n, 1, ek, st);
+    case 8: return fwd_reg_ek<T, 8>(a, order, n, 1, ek, st);
     default: return hipErrorInvalidValue;
     }
 }
 
 template <typename T>
-hipError_t launch_backward_reg(const Args<T> &a, const int32_t *order, int n, int cpt, hipStream_t st) {
+hipError_t launch_backward_reg(const Args<T> &a, const int32_t *order, int n, int cpt, int ek, hipStream_t st) {
     if (n == 0) return hipSuccess;
-    const int ek = a.m.kind == 2 ? 2 : 0;
     switch (cpt) {
-    case 1: return bwd_reg_inst<T, 1>(a, order, n, ek, st);
-    case 2: return bwd_reg_inst<T, 2>(a, order, n, ek, st);
-    case 3: return bwd_reg_inst<T, 3>(a, order, n, ek, st);
-    case 4: return bwd_reg_inst<T, 4>(a, order, n, ek, st);
-    case 6: return bwd_reg_inst<T, 6>(a, order, n, ek, st);
-    case 8: return bwd_reg_inst<T, 8>(a, order, n, ek, st);
+    case 1: return bwd_reg_ek<T, 1>(a, order, n, ek, st);
+    case 2: return bwd_reg_ek<T, 2>(a, order, n, ek, st);
+    case 3: return bwd_reg_ek<T, 3>(a, order, n, ek, st);
+    case 4: return bwd_reg_ek<T, 4>(a, order, n, ek, st);
+    case 6: return bwd_reg_ek<T, 6>(a, order, n, ek, st);
+    case 8: return bwd_reg_ek<T, 8>(a, order, n, ek, st);
     default: return hipErrorInvalidValue;
     }
 }
 
-template hipError_t launch_forward_reg<float>(const Args<float> &, const int32_t *, int, int, hipStream_t);
-template hipError_t launch_forward_reg<double>(const Args<double> &, const int32_t *, int, int, hipStream_t);
-template hipError_t launch_backward_reg<float>(const Args<float> &, const int32_t *, int, int, hipStream_t);
-template hipError_t launch_backward_reg<double>(const Args<double> &, const int32_t *, int, int, hipStream_t);
+template hipError_t launch_forward_reg<float>(const Args<float> &, const int32_t *, int, int, int, int,
+                                              hipStream_t);
+template hipError_t launch_forward_reg<double>(const Args<double> &, const int32_t *, int, int, int, int,
+                                               hipStream_t);
+template hipError_t launch_backward_reg<float>(const Args<float> &, const int32_t *, int, int, int, hipStream_t);
+template hipError_t launch_backward_reg<double>(const Args<double> &, const int32_t *, int, int, int, hipStream_t);
 
 } // namespace sa

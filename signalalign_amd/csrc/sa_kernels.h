@@ -29,6 +29,7 @@ struct Model {
     const T *ys;
     int32_t grid_length;
     T inv_dx;
+    T gx_const;             /* the skip emission when every k-mer has the same one */
     T trans[9];
     T endv[3];
     T rendv[3];
@@ -78,9 +79,10 @@ template <typename T> hipError_t launch_prep(const Args<T> &a, int n_tasks, hipS
 template <typename T> hipError_t launch_forward(const Args<T> &a, int n_tasks, int nt, int cpt, hipStream_t st);
 template <typename T> hipError_t launch_backward(const Args<T> &a, int n_wins, int cpt, hipStream_t st);
 template <typename T>
-hipError_t launch_forward_reg(const Args<T> &a, const int32_t *order, int n, int cpt, hipStream_t st);
+hipError_t launch_forward_reg(const Args<T> &a, const int32_t *order, int n, int cpt, int nw, int ek,
+                              hipStream_t st);
 template <typename T>
-hipError_t launch_backward_reg(const Args<T> &a, const int32_t *order, int n, int cpt, hipStream_t st);
+hipError_t launch_backward_reg(const Args<T> &a, const int32_t *order, int n, int cpt, int ek, hipStream_t st);
 hipError_t launch_collect(const WinDev *wins, const int32_t *win_count, const int64_t *win_dst, const int32_t *src,
                           int32_t *dst, const int32_t *task_x1, const int32_t *task_y1, int n_wins, hipStream_t st);
 

@@ -83,6 +83,8 @@ struct sa_model {
     double endv[3], rendv[3];
     int32_t grid_length = 0;
     double inv_dx = 0.0;
+    int32_t ek = 1;          /* register-kernel emission variant: 0 standard Gaussian, 1 any Gaussian, 2 HDP */
+    double gx_const = -2.3025850929940455;
     DBuf d_lut;
     DevTables<double> t64;
     DevTables<float> t32;
@@ -239,6 +241,7 @@ sa::Model<T> device_model(const sa_model *m, const DevTables<T> &t) {
     r.ys = (const T *)t.ys.p;
     r.grid_length = m->grid_length;
     r.inv_dx = (T)m->inv_dx;
+    r.gx_const = (T)m->gx_const;
     for (int i = 0; i < 9; i++) r.trans[i] = (T)m->trans[i];
     for (int i = 0; i < 3; i++) { r.endv[i] = (T)m->endv[i]; r.rendv[i] = (T)m->rendv[i]; }
     return r;
@@ -388,6 +391,24 @@ int sa_model_create(sa_context *ctx, const sa_model_desc *d, sa_model **out) {
                 delete m;
                 return fail(ctx, SA_ERR_INVALID_ARGUMENT, "kmer_row out of range");
             }
+    }
+    if (d->kind == SA_EMIT_HDP) {
+        m->ek = 2;
+    } else {
+        /* tables as the .model loader builds them: extra-event = match with sd x 1.75
+           (impl/stateMachine.c:1410-1412), one skip emission (:1474-1476), sd > 0 */
+        bool std_tables = true;
+        for (int64_t i = 0; i < m->n_kmers && std_tables; i++) {
+            const double *mt = d->match + 5 * i;
+            if (!(mt[1] > 0.0)) std_tables = false;
+            if (d->gapy) {
+                const double *gt = d->gapy + 5 * i;
+                if (gt[0] != mt[0] || gt[1] != mt[1] * 1.75 || gt[2] != mt[2] || gt[4] != mt[4]) std_tables = false;
+            }
+            if (d->gapx && d->gapx[i] != d->gapx[0]) std_tables = false;
+        }
+        m->gx_const = d->gapx ? d->gapx[0] : -2.3025850929940455;
+        m->ek = std_tables ? 0 : 1;
     }
     std::vector<int8_t> lut(m->lut, m->lut + 256);
     int rc = upload(ctx, m->d_lut, lut);
@@ -675,44 +696,79 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     }
     const int64_t nw = (int64_t)wins.size();
 
-    /* kernel per sub-alignment: the register sweep with the smallest lane window
-       (64*CPT positions) that covers its band span, or the LDS sweep (group 0) */
+    /* Kernels per sub-alignment.  Forward: the register sweep with the smallest lane
+       window covering its band span, one wavefront per sub-alignment; long
+       sub-alignments (and every sub-alignment of a small call) get four wavefronts
+       per sub-alignment to shorten each anti-diagonal step.  Backward: one wavefront
+       per traceback window.  Bands no register window covers use the LDS sweeps. */
     static const int kCpt[] = {1, 2, 3, 4, 6, 8};
-    constexpr int kGroups = 7;
-    std::vector<int> tgroup(nt);
+    auto cpt1 = [&](int sp) {
+        for (int k = 0; k < 6; k++)
+            if (64 * kCpt[k] >= sp + kCpt[k] - 1) return kCpt[k];
+        return 0;
+    };
+    auto cpt4 = [&](int sp) {
+        for (int k = 0; k < 2; k++)
+            if (256 * kCpt[k] >= sp + kCpt[k] - 1) return kCpt[k];
+        return 0;
+    };
+    std::vector<int64_t> nd(nt);
+    for (int64_t i = 0; i < nt; i++) nd[i] = (int64_t)b->tasks[i].lX + b->tasks[i].lY;
+    int64_t long_n = INT64_MAX;
+    {
+        std::vector<int64_t> srt(nd);
+        std::sort(srt.begin(), srt.end());
+        const int64_t med = srt[srt.size() / 2];
+        long_n = nt <= 2 * (int64_t)std::max(ctx->n_cu, 1) ? 0 : std::max<int64_t>(20000, (5 * med) / 2);
+        const char *fw = getenv("SA_FWD_WAVES"); /* "1" or "4": one forward shape for every sub-alignment */
+        if (fw && std::strcmp(fw, "1") == 0) long_n = INT64_MAX;
+        if (fw && std::strcmp(fw, "4") == 0) long_n = 0;
+    }
+    /* forward group key: 0 = LDS; 1 + k = one wavefront, kCpt[k]; 7 + k = four wavefronts, kCpt[k] */
+    constexpr int kFG = 9, kBG = 7;
+    std::vector<int> fgrp(nt), bgrp(nt);
     for (int64_t i = 0; i < nt; i++) {
-        int g = 0;
-        if (!force_lds)
-            for (int k = 0; k < 6; k++)
-                if (64 * kCpt[k] >= span[i] + kCpt[k] - 1) { g = k + 1; break; }
-        tgroup[i] = g;
+        int f = 0, bb = 0;
+        if (!force_lds) {
+            const int c1 = cpt1(span[i]), c4 = cpt4(span[i]);
+            if (c1) {
+                bb = 1 + (int)(std::find(kCpt, kCpt + 6, c1) - kCpt);
+                f = bb;
+            }
+            if (c4 && nd[i] >= long_n) f = 7 + (c4 - 1);
+        }
+        fgrp[i] = f;
+        bgrp[i] = bb;
     }
     std::vector<int32_t> task_order, win_order;
-    int64_t g_toff[kGroups + 1] = {0}, g_woff[kGroups + 1] = {0}, g_cells[kGroups] = {0};
+    int64_t g_toff[kFG + 1] = {0}, g_woff[kBG + 1] = {0}, fg_cells[kFG] = {0}, bg_cells[kBG] = {0};
     int32_t lds_maxw = 1;
-    for (int g = 0; g < kGroups; g++) {
-        std::vector<int32_t> to, wo;
+    for (int g = 0; g < kFG; g++) {
+        std::vector<int32_t> to;
         for (int64_t i = 0; i < nt; i++)
-            if (tgroup[i] == g) {
-                to.push_back((int32_t)i);
-                g_cells[g] += cells[i];
-                if (g == 0) lds_maxw = std::max(lds_maxw, maxw[i]);
-            }
+            if (fgrp[i] == g) { to.push_back((int32_t)i); fg_cells[g] += cells[i]; }
         std::stable_sort(to.begin(), to.end(), [&](int32_t x, int32_t y) { return cells[x] > cells[y]; });
-        for (int32_t ti : to)
-            for (int32_t w = 0; w < nwin[ti]; w++) wo.push_back(task_first_win[ti] + w);
+        task_order.insert(task_order.end(), to.begin(), to.end());
+        g_toff[g + 1] = (int64_t)task_order.size();
+    }
+    for (int g = 0; g < kBG; g++) {
+        std::vector<int32_t> wo;
+        for (int64_t i = 0; i < nt; i++)
+            if (bgrp[i] == g) {
+                bg_cells[g] += cells[i];
+                if (g == 0) lds_maxw = std::max(lds_maxw, maxw[i]);
+                for (int32_t w = 0; w < nwin[i]; w++) wo.push_back(task_first_win[i] + w);
+            }
         std::stable_sort(wo.begin(), wo.end(), [&](int32_t x, int32_t y) {
             return (int64_t)(wins[x].D - wins[x].T) * maxw[wins[x].task] >
                    (int64_t)(wins[y].D - wins[y].T) * maxw[wins[y].task];
         });
-        task_order.insert(task_order.end(), to.begin(), to.end());
         win_order.insert(win_order.end(), wo.begin(), wo.end());
-        g_toff[g + 1] = (int64_t)task_order.size();
         g_woff[g + 1] = (int64_t)win_order.size();
     }
     const int ring = next_pow2(lds_maxw);
     const int ring_max = sizeof(T) == 8 ? 512 : 1024;
-    if (g_toff[1] > 0 && ring > ring_max)
+    if ((g_toff[1] > 0 || g_woff[1] > 0) && ring > ring_max)
         return fail(ctx, SA_ERR_UNSUPPORTED, "band diagonal wider than " + std::to_string(ring_max));
     SA_HIP(ctx, b->d_fM.ensure(sizeof(T) * total_cells));
     SA_HIP(ctx, b->d_fX.ensure(sizeof(T) * std::max<int64_t>(total_xy, 1)));
@@ -733,84 +789,105 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     a.ring = ring;
     const int32_t *d_to = b->d_task_order.as<int32_t>();
     const int32_t *d_wo = b->d_win_order.as<int32_t>();
-    const int lds_fnt = (g_toff[1] >= 4 * std::max(ctx->n_cu, 1) || lds_maxw <= 64) && lds_maxw <= 256 ? 64 : 256;
+    const int64_t n_lds = g_toff[1];
+    const int lds_fnt = (n_lds >= 4 * std::max(ctx->n_cu, 1) || lds_maxw <= 64) && lds_maxw <= 256 ? 64 : 256;
     const int lds_fcpt = (lds_maxw + lds_fnt - 1) / lds_fnt;
     const int lds_bcpt = (lds_maxw + 63) / 64;
-    if (g_toff[1] > 0 && (lds_fcpt > 4 || lds_bcpt > 8))
+    if ((g_toff[1] > 0 || g_woff[1] > 0) && (lds_fcpt > 4 || lds_bcpt > 8))
         return fail(ctx, SA_ERR_UNSUPPORTED, "band too wide for the kernels");
+    const int ek = model->ek;
 
-    /* groups run on up to four streams after prep; each stream runs its groups'
-       forward then backward sweeps */
+    /* groups run on up to four streams; all forward sweeps finish before the
+       backward sweeps start */
     hipStream_t gst[4] = {st, ctx->aux[0], ctx->aux[1], ctx->aux[2]};
-    int n_active = 0, active[kGroups];
-    for (int g = 0; g < kGroups; g++)
-        if (g_toff[g + 1] > g_toff[g]) active[n_active++] = g;
     double main_f = 0, main_b = 0;
-    int64_t main_cells = -1;
+    int64_t main_fc = -1, main_bc = -1;
     int main_cpt = 0;
-    auto run_groups = [&](bool fwd) -> int {
-        SA_HIP(ctx, hipEventRecord(ctx->ev[2], st));
-        for (int q = 1; q < 4; q++) SA_HIP(ctx, hipStreamWaitEvent(gst[q], ctx->ev[2], 0));
-        for (int k = 0; k < n_active; k++) {
-            const int g = active[k];
+    auto join = [&](int ev_slot) -> int {
+        for (int q = 1; q < 4; q++) {
+            SA_HIP(ctx, hipEventRecord(ctx->gev[28 + q], gst[q]));
+            SA_HIP(ctx, hipStreamWaitEvent(st, ctx->gev[28 + q], 0));
+        }
+        SA_HIP(ctx, hipEventRecord(ctx->ev[ev_slot], st));
+        return SA_OK;
+    };
+    auto fork = [&](int ev_slot) -> int {
+        SA_HIP(ctx, hipEventRecord(ctx->ev[ev_slot], st));
+        for (int q = 1; q < 4; q++) SA_HIP(ctx, hipStreamWaitEvent(gst[q], ctx->ev[ev_slot], 0));
+        return SA_OK;
+    };
+    int nfa = 0, fa[kFG], nba = 0, ba[kBG];
+    for (int g = 0; g < kFG; g++) if (g_toff[g + 1] > g_toff[g]) fa[nfa++] = g;
+    for (int g = 0; g < kBG; g++) if (g_woff[g + 1] > g_woff[g]) ba[nba++] = g;
+    auto run_forward = [&]() -> int {
+        int r;
+        if ((r = fork(2))) return r;
+        for (int k = 0; k < nfa; k++) {
+            const int g = fa[k];
             hipStream_t sg = gst[k % 4];
-            const int nt_g = (int)(g_toff[g + 1] - g_toff[g]), nw_g = (int)(g_woff[g + 1] - g_woff[g]);
-            SA_HIP(ctx, hipEventRecord(ctx->gev[3 * k + 0], sg));
-            if (fwd) {
-                if (g == 0) {
-                    sa::Args<T> al = a;
-                    al.task_order = d_to + g_toff[g];
-                    SA_HIP(ctx, sa::launch_forward<T>(al, nt_g, lds_fnt, lds_fcpt, sg));
-                } else {
-                    SA_HIP(ctx, sa::launch_forward_reg<T>(a, d_to + g_toff[g], nt_g, kCpt[g - 1], sg));
-                }
+            const int n_g = (int)(g_toff[g + 1] - g_toff[g]);
+            SA_HIP(ctx, hipEventRecord(ctx->gev[2 * k], sg));
+            if (g == 0) {
+                sa::Args<T> al = a;
+                al.task_order = d_to + g_toff[g];
+                SA_HIP(ctx, sa::launch_forward<T>(al, n_g, lds_fnt, lds_fcpt, sg));
+            } else if (g < 7) {
+                SA_HIP(ctx, sa::launch_forward_reg<T>(a, d_to + g_toff[g], n_g, kCpt[g - 1], 1, ek, sg));
+            } else {
+                SA_HIP(ctx, sa::launch_forward_reg<T>(a, d_to + g_toff[g], n_g, kCpt[g - 7], 4, ek, sg));
             }
-            SA_HIP(ctx, hipEventRecord(ctx->gev[3 * k + 1], sg));
+            SA_HIP(ctx, hipEventRecord(ctx->gev[2 * k + 1], sg));
+        }
+        return join(3);
+    };
+    auto run_backward = [&]() -> int {
+        int r;
+        if ((r = fork(3))) return r;
+        for (int k = 0; k < nba; k++) {
+            const int g = ba[k];
+            hipStream_t sg = gst[k % 4];
+            const int n_g = (int)(g_woff[g + 1] - g_woff[g]);
+            SA_HIP(ctx, hipEventRecord(ctx->gev[16 + 2 * k], sg));
             if (g == 0) {
                 sa::Args<T> al = a;
                 al.win_order = d_wo + g_woff[g];
-                SA_HIP(ctx, sa::launch_backward<T>(al, nw_g, lds_bcpt, sg));
+                SA_HIP(ctx, sa::launch_backward<T>(al, n_g, lds_bcpt, sg));
             } else {
-                SA_HIP(ctx, sa::launch_backward_reg<T>(a, d_wo + g_woff[g], nw_g, kCpt[g - 1], sg));
+                SA_HIP(ctx, sa::launch_backward_reg<T>(a, d_wo + g_woff[g], n_g, kCpt[g - 1], ek, sg));
             }
-            SA_HIP(ctx, hipEventRecord(ctx->gev[3 * k + 2], sg));
+            SA_HIP(ctx, hipEventRecord(ctx->gev[16 + 2 * k + 1], sg));
         }
-        for (int q = 1; q < 4; q++) {
-            SA_HIP(ctx, hipEventRecord(ctx->gev[24 + q], gst[q]));
-            SA_HIP(ctx, hipStreamWaitEvent(st, ctx->gev[24 + q], 0));
-        }
-        SA_HIP(ctx, hipEventRecord(ctx->ev[4], st));
-        return SA_OK;
+        return join(4);
     };
-    auto group_times = [&](bool fwd) {
-        for (int k = 0; k < n_active; k++) {
-            float f = 0.f, bb = 0.f;
-            if (fwd) (void)hipEventElapsedTime(&f, ctx->gev[3 * k + 0], ctx->gev[3 * k + 1]);
-            (void)hipEventElapsedTime(&bb, ctx->gev[3 * k + 1], ctx->gev[3 * k + 2]);
-            tm.forward_ms += f;
-            tm.backward_ms += bb;
-            const int g = active[k];
-            if (g_cells[g] > main_cells) {
-                main_cells = g_cells[g];
-                main_f = f;
-                main_b = bb;
-                main_cpt = g == 0 ? 0 : kCpt[g - 1];
-            }
-        }
-    };
-    if ((rc = run_groups(true))) return rc;
-    tm.forward_launches = n_active;
-    tm.backward_launches = n_active;
+    if ((rc = run_forward()) || (rc = run_backward())) return rc;
+    tm.forward_launches = nfa;
+    tm.backward_launches = nba;
     std::vector<int32_t> wcount(nw);
     SA_HIP(ctx, hipMemcpyAsync(wcount.data(), b->d_win_count.p, sizeof(int32_t) * nw, hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipMemcpyAsync(&err, b->d_err.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipStreamSynchronize(st));
     if (err & sa::ERR_WINDOWS) return kRetryLds;
-    group_times(true);
     {
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[4]);
-        tm.fb_ms = ms;
+        float f = 0.f, bb = 0.f;
+        (void)hipEventElapsedTime(&f, ctx->ev[2], ctx->ev[3]);
+        (void)hipEventElapsedTime(&bb, ctx->ev[3], ctx->ev[4]);
+        tm.forward_ms = f;
+        tm.backward_ms = bb;
+        tm.fb_ms = f + bb;
+        for (int k = 0; k < nfa; k++) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, ctx->gev[2 * k], ctx->gev[2 * k + 1]);
+            if (fg_cells[fa[k]] > main_fc) { main_fc = fg_cells[fa[k]]; main_f = ms; }
+        }
+        for (int k = 0; k < nba; k++) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, ctx->gev[16 + 2 * k], ctx->gev[16 + 2 * k + 1]);
+            if (bg_cells[ba[k]] > main_bc) {
+                main_bc = bg_cells[ba[k]];
+                main_b = ms;
+                main_cpt = ba[k] == 0 ? 0 : kCpt[ba[k] - 1];
+            }
+        }
     }
     /* windows whose pairs overflowed their slot run again with the exact count */
     std::vector<int32_t> redo;
@@ -829,11 +906,12 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
         if ((rc = upload(ctx, b->d_wins, wins))) return rc;
         a.out = b->d_out.as<int32_t>();
         a.wins = b->d_wins.as<WinDev>();
-        if ((rc = run_groups(false))) return rc;
-        tm.backward_launches += n_active;
+        if ((rc = run_backward())) return rc;
+        tm.backward_launches += nba;
         SA_HIP(ctx, hipMemcpyAsync(wcount.data(), b->d_win_count.p, sizeof(int32_t) * nw, hipMemcpyDeviceToHost, st));
         SA_HIP(ctx, hipStreamSynchronize(st));
     }
+    const int64_t main_cells = main_bc;
     tm.main_forward_ms = main_f;
     tm.main_backward_ms = main_b;
     tm.main_cells = main_cells;

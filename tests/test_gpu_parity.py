@@ -14,11 +14,16 @@ from tests import _parity as P
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(params=["reg", "lds"], autouse=True)
+@pytest.fixture(params=["reg1", "reg4", "lds"], autouse=True)
 def kernel_path(request, monkeypatch):
-    """Every case runs through both sweeps: the register-resident kernels (default)
-    and the LDS-ring kernels (the fallback for bands wider than a register window)."""
-    monkeypatch.setenv("SA_KERNELS", request.param)
+    """Every case runs through each sweep: the register-resident kernels with one
+    wavefront per forward sweep, with four, and the LDS-ring kernels (the fallback
+    for bands wider than a register window)."""
+    if request.param == "lds":
+        monkeypatch.setenv("SA_KERNELS", "lds")
+    else:
+        monkeypatch.setenv("SA_KERNELS", "reg")
+        monkeypatch.setenv("SA_FWD_WAVES", request.param[-1])
     return request.param
 
 
@@ -114,6 +119,18 @@ def test_window_sizes_fp32(gpu_ctx, g6):
 def test_wide_expansion_and_threshold(gpu_ctx, g6):
     r = synth.single_read(g6, 3000, seed=24)
     _check(gpu_ctx, g6, r, native.Params(diagonal_expansion=100, threshold=0.05), native.SA_FP64)
+
+
+def test_general_gaussian_tables(gpu_ctx, g6):
+    """Extra-event and skip tables that are not the .model loader's (sd x 1.75, one
+    skip value) take the general emission path."""
+    m = models.GaussianModel(g6.alphabet, 6, g6.transitions, g6.match, g6.gapy.copy(), g6.gapx.copy())
+    m.gapy[:, 4] *= 1.3
+    m.gapy[:, 0] += 0.7
+    m.gapx[::7] = np.log(0.05)
+    r = synth.single_read(g6, 1500, seed=28)
+    _check(gpu_ctx, m, r, native.Params(diagonal_expansion=20), native.SA_FP64)
+    _check(gpu_ctx, m, r, native.Params(diagonal_expansion=20), native.SA_FP32, p_tol=2e-3, ll_rel=1e-5)
 
 
 def test_undescaled_emission(gpu_ctx, g6):
