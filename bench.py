@@ -89,6 +89,60 @@ def cpu_baseline(model, reads, params, target_s, threads):
                       f"thread, {wall:.1f} s wall"}
 
 
+def copy_calibration(mib):
+    """One device-to-device copy of a known size: the HBM byte counters of this
+    dispatch (FETCH_SIZE, WRITE_SIZE) calibrate those of the alignment kernels."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    log(f"copy calibration: {mib} MiB read + {mib} MiB written")
+    del x, y
+
+
+def traffic_for(kname, cells, args):
+    """HBM bytes per launch of `kname`: the calibrated rocprofv3 FETCH_SIZE +
+    WRITE_SIZE bytes per band cell kept with the profiles, times the launch's cells
+    (None if that kernel is not listed or the workload is not the profiled one)."""
+    if args.reads != 4096 or args.median != 8000 or args.expansion != 50:
+        return None
+    try:
+        with open(args.traffic_file) as f:
+            for line in f:
+                parts = line.strip().split("\t")
+                if len(parts) >= 2 and parts[0] == kname:
+                    return round(float(parts[1]) * cells)
+    except OSError:
+        pass
+    return None
+
+
+def fp32_error(r32, r64, threshold):
+    """Per-read error of the FP32 recurrence against the FP64 one (which matches the
+    FP64 oracle to 1e-4 in posteriors): max |dp| over pairs in both lists, pairs
+    present in only one list (and how many of those lie more than 1e-3 from the
+    threshold), and |dlogL|/|logL|."""
+    dps, lls, only, only_far = [], [], 0, 0
+    for a, b in zip(r32, r64):
+        ka = a.pairs[:, 1] * (1 << 24) + a.pairs[:, 2]
+        kb = b.pairs[:, 1] * (1 << 24) + b.pairs[:, 2]
+        _, ia, ib = np.intersect1d(ka, kb, assume_unique=True, return_indices=True)
+        dps.append(float(np.max(np.abs(a.pairs[ia, 0] - b.pairs[ib, 0]))) / 1e7 if len(ia) else 0.0)
+        xa = np.setdiff1d(np.arange(len(ka)), ia)
+        xb = np.setdiff1d(np.arange(len(kb)), ib)
+        only += len(xa) + len(xb)
+        pa = np.concatenate([a.pairs[xa, 0], b.pairs[xb, 0]]) / 1e7
+        only_far += int(np.sum(np.abs(pa - threshold) > 1e-3))
+        lls.append(abs(a.log_likelihood - b.log_likelihood) / max(abs(b.log_likelihood), 1e-300))
+    dps, lls = np.array(dps), np.array(lls)
+    return {"reads": len(dps), "max_abs_dp": float(dps.max()), "p99_abs_dp": float(np.percentile(dps, 99)),
+            "median_abs_dp": float(np.median(dps)), "max_ll_rel": float(lls.max()),
+            "p99_ll_rel": float(np.percentile(lls, 99)), "pairs_in_one_list": only,
+            "pairs_in_one_list_beyond_1e-3_of_threshold": only_far}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +158,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-fp64-check", action="store_true", help="skip the FP32-vs-FP64 per-read error report")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first run a device copy of this many MiB (HBM byte-counter calibration under rocprofv3)")
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_batched_4096reads.txt"))
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (reads shard across GPUs with no "
@@ -120,6 +178,8 @@ def main():
     log(f"generated {len(reads)} reads in {time.time() - t0:.1f}s: "
         f"{sum(len(r['ref']) for r in reads)} nt, {sum(len(r['events']) for r in reads)} events")
 
+    if args.copy_calibration_mib:
+        copy_calibration(args.copy_calibration_mib)
     ctx = native.Context(0)
     dm = ctx.model(model)
     params = native.Params(diagonal_expansion=args.expansion)
@@ -129,7 +189,7 @@ def main():
     for i in range(args.warmup):
         _, tm = batch.run(prec, fetch=False)
         log(f"warmup {i}: device {tm['total_ms']:.1f} ms")
-    dev_ms, wall_ms, fwd_ms, bwd_ms = [], [], [], []
+    dev_ms, wall_ms, fwd_ms, bwd_ms, tms = [], [], [], [], []
     cells = 0
     for i in range(args.steps):
         tw = time.perf_counter()
@@ -138,6 +198,7 @@ def main():
         dev_ms.append(tm["total_ms"])
         fwd_ms.append(tm["forward_ms"])
         bwd_ms.append(tm["backward_ms"])
+        tms.append(tm)
         cells = tm["cells"]
         log(f"step {i}: device {tm['total_ms']:.1f} ms (prep {tm['prep_ms']:.1f} fwd {tm['forward_ms']:.1f} "
             f"bwd {tm['backward_ms']:.1f} collect {tm['collect_ms']:.1f}), wall {wall_ms[-1]:.1f} ms, "
@@ -145,17 +206,44 @@ def main():
     n_pairs = sum(len(r.pairs) for r in res)
     ms = float(np.mean(dev_ms))
     gcups = cells / (ms * 1e-3) / 1e9
-    f_ms, b_ms = float(np.mean(fwd_ms)), float(np.mean(bwd_ms))
-    if b_ms >= f_ms:
-        dom, k_ms, flop = "backward_kernel", b_ms, FLOP_BWD
-    else:
-        dom, k_ms, flop = "forward_kernel", f_ms, FLOP_FWD
-    achieved = flop * cells / (k_ms * 1e-3) / 1e12
+    # roofline of the dominant kernel: the kernel (one instantiation) with the most
+    # launch time; its average launch duration (HIP events on the stream each launch
+    # runs on) and average cells per launch, as rocprofv3 averages per kernel name
+    tname = "float" if dtype == "f32" else "double"
+    ek = tms[-1]["emission_path"]
+    agg = {}
+    for t in tms:
+        for L in t["launches"]:
+            if L["direction"] == 0:
+                name = (f"sa::forward_reg_kernel<{tname}, {L['cpt']}, {ek}, {L['waves']}>" if L["cpt"]
+                        else f"sa::forward_kernel<{tname}, ...>")
+            else:
+                name = (f"sa::backward_reg_kernel<{tname}, {L['cpt']}, {ek}>" if L["cpt"]
+                        else f"sa::backward_kernel<{tname}, ...>")
+            a_ = agg.setdefault(name, {"n": 0, "ms": 0.0, "cells": 0, "flop": FLOP_FWD if L["direction"] == 0
+                                       else FLOP_BWD})
+            a_["n"] += 1
+            a_["ms"] += L["ms"]
+            a_["cells"] += L["cells"]
+    kname = max(agg, key=lambda k: agg[k]["ms"])
+    k = agg[kname]
+    k_ms, k_cells, flop = k["ms"] / k["n"], k["cells"] / k["n"], k["flop"]
+    achieved = flop * k_cells / (k_ms * 1e-3) / 1e12
     peak = PEAK_TFLOPS[dtype]
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(achieved / peak, 4), "traffic": None, "kernel": dom,
-                "kernel_ms": round(k_ms, 3), "flop_per_cell": flop,
-                "other_kernel_ms": round(f_ms if dom == "backward_kernel" else b_ms, 3)}
+                "frac": round(achieved / peak, 4), "traffic": traffic_for(kname, k_cells, args), "kernel": kname,
+                "kernel_avg_ms": round(k_ms, 3), "kernel_cells_per_launch": int(k_cells),
+                "flop_per_cell": flop,
+                "launches": {n: {"calls": v["n"], "avg_ms": round(v["ms"] / v["n"], 3),
+                                 "cells_per_launch": int(v["cells"] / v["n"])} for n, v in agg.items()},
+                "forward_ms": round(float(np.mean(fwd_ms)), 3), "backward_ms": round(float(np.mean(bwd_ms)), 3),
+                "sweeps_ms": round(float(np.mean([t["fb_ms"] for t in tms])), 3)}
+    fp_err = None
+    if not args.no_fp64_check and prec == native.SA_FP32:
+        res64, tm64 = batch.run(native.SA_FP64, fetch=True)
+        fp_err = fp32_error(res, res64, params.threshold)
+        fp_err["fp64_gcups"] = round(tm64["cells"] / tm64["total_ms"] / 1e6, 3)
+        log(f"fp32 vs fp64: {fp_err}")
     batch.close()
 
     plain = {}
@@ -202,6 +290,7 @@ def main():
                    "reads": args.reads, "cells": cells, "pairs": n_pairs, "precision": args.precision},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "fp32_error_vs_fp64": fp_err,
         "plain_gpu": plain,
         "wall_ms_per_step_pcie_inclusive": round(float(np.mean(wall_ms)), 3),
     }

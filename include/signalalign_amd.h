@@ -124,19 +124,29 @@ typedef struct {
 
 typedef struct {
     double prep_ms;        /* band, schedule, k-mer ids, event constants (GPU) */
-    double forward_ms;     /* forward kernels, summed over sub-alignment groups */
-    double backward_ms;    /* backward + posterior kernels, summed over groups */
+    double forward_ms;     /* start of the sweeps to the end of the last forward launch */
+    double backward_ms;    /* start of the first backward launch to the end of the sweeps */
     double collect_ms;     /* pair compaction */
     double total_ms;       /* whole call on the device, inputs resident to outputs resident */
     int64_t cells;
     int64_t forward_launches;
     int64_t backward_launches;
-    double fb_ms;            /* forward + backward sweeps on the device timeline */
-    double main_forward_ms;  /* forward kernel of the largest group of sub-alignments */
-    double main_backward_ms; /* backward kernel of the largest group */
-    int64_t main_cells;      /* band cells of that group */
-    int32_t main_cpt;        /* its register window: 64*main_cpt positions (0: LDS sweep) */
+    double fb_ms;            /* forward + backward sweeps on the device timeline (they overlap) */
+    int32_t emission_path;   /* 0: standard Gaussian tables, 1: general Gaussian, 2: HDP grid */
 } sa_timing;
+
+/* One kernel launch of a run: forward sweeps of a group of sub-alignments, or
+ * backward sweeps of a group of traceback windows. */
+typedef struct {
+    int32_t direction;  /* 0 forward, 1 backward */
+    int32_t cpt;        /* register window of 64*cpt positions per wavefront; 0 = LDS sweep */
+    int32_t waves;      /* wavefronts per sub-alignment (forward) or window (backward) */
+    int32_t stream;     /* 0..3 */
+    int64_t items;      /* sub-alignments (forward) or windows (backward) */
+    int64_t cells;      /* band cells those items cover */
+    double start_ms;    /* from the start of the sweeps (HIP events on its stream) */
+    double ms;          /* duration */
+} sa_launch;
 
 typedef struct sa_context sa_context;
 typedef struct sa_model sa_model;
@@ -171,6 +181,8 @@ void sa_batch_destroy(sa_batch *batch);
 int64_t sa_result_n_reads(const sa_result *res);
 int sa_result_read(const sa_result *res, int64_t i, sa_read_result *out);
 int sa_result_timing(const sa_result *res, sa_timing *out);
+/* Launches of the run (up to cap); returns how many there were. */
+int64_t sa_result_launches(const sa_result *res, sa_launch *out, int64_t cap);
 void sa_result_free(sa_result *res);
 
 /* Host-side views of the integer geometry the kernels compute on the device

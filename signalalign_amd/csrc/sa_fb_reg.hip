@@ -253,6 +253,9 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
     __shared__ ECon<T, EK> bev[2][NW];
     __shared__ T red[NW];
     __shared__ T shbuf[NW > 1 ? 6 * W : 1];
+    /* the four-wavefront sweeps carry the longest dependency chains of a launch
+       sequence: their instructions go first on shared SIMDs */
+    if constexpr (NW > 1) __builtin_amdgcn_s_setprio(3);
     const int ti = order[blockIdx.x];
     const TaskDev t = a.tasks[ti];
     const Model<T> &m = a.m;

@@ -66,8 +66,8 @@ struct sa_context {
     hipStream_t aux[3] = {};   /* sub-alignment groups of different window sizes run side by side */
     std::mutex mu;
     std::string err;
-    hipEvent_t ev[8] = {};
-    hipEvent_t gev[32] = {};   /* per-group kernel timing and joins */
+    hipEvent_t ev[12] = {};
+    hipEvent_t gev[160] = {};  /* start/end of each forward and backward launch */
     int n_cu = 0;
 };
 
@@ -138,6 +138,7 @@ struct sa_result {
     std::vector<sa_pair> pairs;
     std::vector<ReadOut> reads;
     sa_timing timing;
+    std::vector<sa_launch> launches;
 };
 
 namespace {
@@ -740,35 +741,66 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
         fgrp[i] = f;
         bgrp[i] = bb;
     }
+    /* Streams: sub-alignments whose forward sweep uses four wavefronts (the long
+       ones) run on stream 1, LDS sweeps on stream 2, one-wavefront forward sweeps
+       on streams 0 and 3 (the sub-alignments holding the first half of their cells,
+       longest first, on 0).  Each stream runs its forward launches and then the
+       backward launches of its own sub-alignments' windows, so backward passes of
+       short sub-alignments overlap the forward passes of long ones. */
+    constexpr int kLanes = 4;
+    std::vector<int> tlane(nt);
+    {
+        std::vector<int32_t> nw1;
+        int64_t nw1_cells = 0;
+        for (int64_t i = 0; i < nt; i++) {
+            tlane[i] = fgrp[i] == 0 ? 2 : fgrp[i] >= 7 ? 1 : 0;
+            if (tlane[i] == 0) { nw1.push_back((int32_t)i); nw1_cells += cells[i]; }
+        }
+        std::stable_sort(nw1.begin(), nw1.end(), [&](int32_t x, int32_t y) { return cells[x] > cells[y]; });
+        int64_t acc = 0;
+        for (int32_t i : nw1) {
+            if (2 * acc >= nw1_cells) tlane[i] = 3;
+            acc += cells[i];
+        }
+    }
+    struct Launch { int lane, group; int64_t off, n, cells; };
+    std::vector<Launch> fl, bl;
     std::vector<int32_t> task_order, win_order;
-    int64_t g_toff[kFG + 1] = {0}, g_woff[kBG + 1] = {0}, fg_cells[kFG] = {0}, bg_cells[kBG] = {0};
     int32_t lds_maxw = 1;
-    for (int g = 0; g < kFG; g++) {
-        std::vector<int32_t> to;
-        for (int64_t i = 0; i < nt; i++)
-            if (fgrp[i] == g) { to.push_back((int32_t)i); fg_cells[g] += cells[i]; }
-        std::stable_sort(to.begin(), to.end(), [&](int32_t x, int32_t y) { return cells[x] > cells[y]; });
-        task_order.insert(task_order.end(), to.begin(), to.end());
-        g_toff[g + 1] = (int64_t)task_order.size();
+    for (int lane = 0; lane < kLanes; lane++) {
+        for (int g = 0; g < kFG; g++) {
+            Launch L{lane, g, (int64_t)task_order.size(), 0, 0};
+            for (int64_t i = 0; i < nt; i++)
+                if (tlane[i] == lane && fgrp[i] == g) { task_order.push_back((int32_t)i); L.cells += cells[i]; }
+            L.n = (int64_t)task_order.size() - L.off;
+            std::stable_sort(task_order.begin() + L.off, task_order.end(),
+                             [&](int32_t x, int32_t y) { return cells[x] > cells[y]; });
+            if (L.n) fl.push_back(L);
+        }
+        for (int g = 0; g < kBG; g++) {
+            Launch L{lane, g, (int64_t)win_order.size(), 0, 0};
+            for (int64_t i = 0; i < nt; i++)
+                if (tlane[i] == lane && bgrp[i] == g) {
+                    L.cells += cells[i];
+                    if (g == 0) lds_maxw = std::max(lds_maxw, maxw[i]);
+                    for (int32_t w = 0; w < nwin[i]; w++) win_order.push_back(task_first_win[i] + w);
+                }
+            L.n = (int64_t)win_order.size() - L.off;
+            std::stable_sort(win_order.begin() + L.off, win_order.end(), [&](int32_t x, int32_t y) {
+                return (int64_t)(wins[x].D - wins[x].T) * maxw[wins[x].task] >
+                       (int64_t)(wins[y].D - wins[y].T) * maxw[wins[y].task];
+            });
+            if (L.n) bl.push_back(L);
+        }
     }
-    for (int g = 0; g < kBG; g++) {
-        std::vector<int32_t> wo;
-        for (int64_t i = 0; i < nt; i++)
-            if (bgrp[i] == g) {
-                bg_cells[g] += cells[i];
-                if (g == 0) lds_maxw = std::max(lds_maxw, maxw[i]);
-                for (int32_t w = 0; w < nwin[i]; w++) wo.push_back(task_first_win[i] + w);
-            }
-        std::stable_sort(wo.begin(), wo.end(), [&](int32_t x, int32_t y) {
-            return (int64_t)(wins[x].D - wins[x].T) * maxw[wins[x].task] >
-                   (int64_t)(wins[y].D - wins[y].T) * maxw[wins[y].task];
-        });
-        win_order.insert(win_order.end(), wo.begin(), wo.end());
-        g_woff[g + 1] = (int64_t)win_order.size();
-    }
+    for (int64_t i = 0; i < nt; i++)
+        if (fgrp[i] == 0) lds_maxw = std::max(lds_maxw, maxw[i]);
     const int ring = next_pow2(lds_maxw);
     const int ring_max = sizeof(T) == 8 ? 512 : 1024;
-    if ((g_toff[1] > 0 || g_woff[1] > 0) && ring > ring_max)
+    bool any_lds = false;
+    for (const Launch &L : fl) any_lds |= L.group == 0;
+    for (const Launch &L : bl) any_lds |= L.group == 0;
+    if (any_lds && ring > ring_max)
         return fail(ctx, SA_ERR_UNSUPPORTED, "band diagonal wider than " + std::to_string(ring_max));
     SA_HIP(ctx, b->d_fM.ensure(sizeof(T) * total_cells));
     SA_HIP(ctx, b->d_fX.ensure(sizeof(T) * std::max<int64_t>(total_xy, 1)));
@@ -789,105 +821,97 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     a.ring = ring;
     const int32_t *d_to = b->d_task_order.as<int32_t>();
     const int32_t *d_wo = b->d_win_order.as<int32_t>();
-    const int64_t n_lds = g_toff[1];
+    int64_t n_lds = 0;
+    for (const Launch &L : fl) if (L.group == 0) n_lds += L.n;
     const int lds_fnt = (n_lds >= 4 * std::max(ctx->n_cu, 1) || lds_maxw <= 64) && lds_maxw <= 256 ? 64 : 256;
     const int lds_fcpt = (lds_maxw + lds_fnt - 1) / lds_fnt;
     const int lds_bcpt = (lds_maxw + 63) / 64;
-    if ((g_toff[1] > 0 || g_woff[1] > 0) && (lds_fcpt > 4 || lds_bcpt > 8))
-        return fail(ctx, SA_ERR_UNSUPPORTED, "band too wide for the kernels");
+    if (any_lds && (lds_fcpt > 4 || lds_bcpt > 8)) return fail(ctx, SA_ERR_UNSUPPORTED, "band too wide for the kernels");
     const int ek = model->ek;
+    if ((int64_t)(fl.size() + bl.size()) * 2 > (int64_t)(sizeof(ctx->gev) / sizeof(ctx->gev[0])))
+        return fail(ctx, SA_ERR_HIP, "too many launches for the timing events");
 
-    /* groups run on up to four streams; all forward sweeps finish before the
-       backward sweeps start */
-    hipStream_t gst[4] = {st, ctx->aux[0], ctx->aux[1], ctx->aux[2]};
-    double main_f = 0, main_b = 0;
-    int64_t main_fc = -1, main_bc = -1;
-    int main_cpt = 0;
-    auto join = [&](int ev_slot) -> int {
-        for (int q = 1; q < 4; q++) {
-            SA_HIP(ctx, hipEventRecord(ctx->gev[28 + q], gst[q]));
-            SA_HIP(ctx, hipStreamWaitEvent(st, ctx->gev[28 + q], 0));
+    hipStream_t gst[kLanes] = {st, ctx->aux[0], ctx->aux[1], ctx->aux[2]};
+    auto launch_fwd = [&](const Launch &L, hipStream_t sg) -> hipError_t {
+        if (L.group == 0) {
+            sa::Args<T> al = a;
+            al.task_order = d_to + L.off;
+            return sa::launch_forward<T>(al, (int)L.n, lds_fnt, lds_fcpt, sg);
         }
-        SA_HIP(ctx, hipEventRecord(ctx->ev[ev_slot], st));
+        if (L.group < 7) return sa::launch_forward_reg<T>(a, d_to + L.off, (int)L.n, kCpt[L.group - 1], 1, ek, sg);
+        return sa::launch_forward_reg<T>(a, d_to + L.off, (int)L.n, kCpt[L.group - 7], 4, ek, sg);
+    };
+    auto launch_bwd = [&](const Launch &L, hipStream_t sg) -> hipError_t {
+        if (L.group == 0) {
+            sa::Args<T> al = a;
+            al.win_order = d_wo + L.off;
+            return sa::launch_backward<T>(al, (int)L.n, lds_bcpt, sg);
+        }
+        return sa::launch_backward_reg<T>(a, d_wo + L.off, (int)L.n, kCpt[L.group - 1], ek, sg);
+    };
+    /* fork on ev[2], per-launch events gev[2k], gev[2k+1] (forward launches first),
+       join on ev[4] */
+    auto run_sweeps = [&](bool forward) -> int {
+        SA_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+        for (int q = 1; q < kLanes; q++) SA_HIP(ctx, hipStreamWaitEvent(gst[q], ctx->ev[2], 0));
+        for (int lane = 0; lane < kLanes; lane++) {
+            hipStream_t sg = gst[lane];
+            if (forward)
+                for (size_t k = 0; k < fl.size(); k++) {
+                    if (fl[k].lane != lane) continue;
+                    SA_HIP(ctx, hipEventRecord(ctx->gev[2 * k], sg));
+                    SA_HIP(ctx, launch_fwd(fl[k], sg));
+                    SA_HIP(ctx, hipEventRecord(ctx->gev[2 * k + 1], sg));
+                }
+            for (size_t k = 0; k < bl.size(); k++) {
+                if (bl[k].lane != lane) continue;
+                const size_t e = 2 * (fl.size() + k);
+                SA_HIP(ctx, hipEventRecord(ctx->gev[e], sg));
+                SA_HIP(ctx, launch_bwd(bl[k], sg));
+                SA_HIP(ctx, hipEventRecord(ctx->gev[e + 1], sg));
+            }
+        }
+        for (int q = 1; q < kLanes; q++) {
+            SA_HIP(ctx, hipEventRecord(ctx->ev[8 + q], gst[q]));
+            SA_HIP(ctx, hipStreamWaitEvent(st, ctx->ev[8 + q], 0));
+        }
+        SA_HIP(ctx, hipEventRecord(ctx->ev[4], st));
         return SA_OK;
     };
-    auto fork = [&](int ev_slot) -> int {
-        SA_HIP(ctx, hipEventRecord(ctx->ev[ev_slot], st));
-        for (int q = 1; q < 4; q++) SA_HIP(ctx, hipStreamWaitEvent(gst[q], ctx->ev[ev_slot], 0));
-        return SA_OK;
-    };
-    int nfa = 0, fa[kFG], nba = 0, ba[kBG];
-    for (int g = 0; g < kFG; g++) if (g_toff[g + 1] > g_toff[g]) fa[nfa++] = g;
-    for (int g = 0; g < kBG; g++) if (g_woff[g + 1] > g_woff[g]) ba[nba++] = g;
-    auto run_forward = [&]() -> int {
-        int r;
-        if ((r = fork(2))) return r;
-        for (int k = 0; k < nfa; k++) {
-            const int g = fa[k];
-            hipStream_t sg = gst[k % 4];
-            const int n_g = (int)(g_toff[g + 1] - g_toff[g]);
-            SA_HIP(ctx, hipEventRecord(ctx->gev[2 * k], sg));
-            if (g == 0) {
-                sa::Args<T> al = a;
-                al.task_order = d_to + g_toff[g];
-                SA_HIP(ctx, sa::launch_forward<T>(al, n_g, lds_fnt, lds_fcpt, sg));
-            } else if (g < 7) {
-                SA_HIP(ctx, sa::launch_forward_reg<T>(a, d_to + g_toff[g], n_g, kCpt[g - 1], 1, ek, sg));
-            } else {
-                SA_HIP(ctx, sa::launch_forward_reg<T>(a, d_to + g_toff[g], n_g, kCpt[g - 7], 4, ek, sg));
-            }
-            SA_HIP(ctx, hipEventRecord(ctx->gev[2 * k + 1], sg));
-        }
-        return join(3);
-    };
-    auto run_backward = [&]() -> int {
-        int r;
-        if ((r = fork(3))) return r;
-        for (int k = 0; k < nba; k++) {
-            const int g = ba[k];
-            hipStream_t sg = gst[k % 4];
-            const int n_g = (int)(g_woff[g + 1] - g_woff[g]);
-            SA_HIP(ctx, hipEventRecord(ctx->gev[16 + 2 * k], sg));
-            if (g == 0) {
-                sa::Args<T> al = a;
-                al.win_order = d_wo + g_woff[g];
-                SA_HIP(ctx, sa::launch_backward<T>(al, n_g, lds_bcpt, sg));
-            } else {
-                SA_HIP(ctx, sa::launch_backward_reg<T>(a, d_wo + g_woff[g], n_g, kCpt[g - 1], ek, sg));
-            }
-            SA_HIP(ctx, hipEventRecord(ctx->gev[16 + 2 * k + 1], sg));
-        }
-        return join(4);
-    };
-    if ((rc = run_forward()) || (rc = run_backward())) return rc;
-    tm.forward_launches = nfa;
-    tm.backward_launches = nba;
+    if ((rc = run_sweeps(true))) return rc;
+    tm.forward_launches = (int64_t)fl.size();
+    tm.backward_launches = (int64_t)bl.size();
     std::vector<int32_t> wcount(nw);
     SA_HIP(ctx, hipMemcpyAsync(wcount.data(), b->d_win_count.p, sizeof(int32_t) * nw, hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipMemcpyAsync(&err, b->d_err.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipStreamSynchronize(st));
     if (err & sa::ERR_WINDOWS) return kRetryLds;
     {
-        float f = 0.f, bb = 0.f;
-        (void)hipEventElapsedTime(&f, ctx->ev[2], ctx->ev[3]);
-        (void)hipEventElapsedTime(&bb, ctx->ev[3], ctx->ev[4]);
-        tm.forward_ms = f;
-        tm.backward_ms = bb;
-        tm.fb_ms = f + bb;
-        for (int k = 0; k < nfa; k++) {
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, ctx->gev[2 * k], ctx->gev[2 * k + 1]);
-            if (fg_cells[fa[k]] > main_fc) { main_fc = fg_cells[fa[k]]; main_f = ms; }
-        }
-        for (int k = 0; k < nba; k++) {
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, ctx->gev[16 + 2 * k], ctx->gev[16 + 2 * k + 1]);
-            if (bg_cells[ba[k]] > main_bc) {
-                main_bc = bg_cells[ba[k]];
-                main_b = ms;
-                main_cpt = ba[k] == 0 ? 0 : kCpt[ba[k] - 1];
-            }
-        }
+        float fb = 0.f;
+        (void)hipEventElapsedTime(&fb, ctx->ev[2], ctx->ev[4]);
+        tm.fb_ms = fb;
+        double fwd_end = 0.0, bwd_start = fb;
+        auto record = [&](const Launch &L, int dir, size_t e) {
+            float t0 = 0.f, t1 = 0.f;
+            (void)hipEventElapsedTime(&t0, ctx->ev[2], ctx->gev[e]);
+            (void)hipEventElapsedTime(&t1, ctx->ev[2], ctx->gev[e + 1]);
+            sa_launch li;
+            li.direction = dir;
+            li.cpt = L.group == 0 ? 0 : dir == 0 && L.group >= 7 ? kCpt[L.group - 7] : kCpt[L.group - 1];
+            li.waves = dir == 0 && L.group >= 7 ? 4 : 1;
+            li.stream = L.lane;
+            li.items = L.n;
+            li.cells = L.cells;
+            li.start_ms = t0;
+            li.ms = t1 - t0;
+            res->launches.push_back(li);
+            if (dir == 0) fwd_end = std::max(fwd_end, (double)t1);
+            else bwd_start = std::min(bwd_start, (double)t0);
+        };
+        for (size_t k = 0; k < fl.size(); k++) record(fl[k], 0, 2 * k);
+        for (size_t k = 0; k < bl.size(); k++) record(bl[k], 1, 2 * (fl.size() + k));
+        tm.forward_ms = fwd_end;
+        tm.backward_ms = fb - bwd_start;
     }
     /* windows whose pairs overflowed their slot run again with the exact count */
     std::vector<int32_t> redo;
@@ -906,16 +930,12 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
         if ((rc = upload(ctx, b->d_wins, wins))) return rc;
         a.out = b->d_out.as<int32_t>();
         a.wins = b->d_wins.as<WinDev>();
-        if ((rc = run_backward())) return rc;
-        tm.backward_launches += nba;
+        if ((rc = run_sweeps(false))) return rc;
+        tm.backward_launches += (int64_t)bl.size();
         SA_HIP(ctx, hipMemcpyAsync(wcount.data(), b->d_win_count.p, sizeof(int32_t) * nw, hipMemcpyDeviceToHost, st));
         SA_HIP(ctx, hipStreamSynchronize(st));
     }
-    const int64_t main_cells = main_bc;
-    tm.main_forward_ms = main_f;
-    tm.main_backward_ms = main_b;
-    tm.main_cells = main_cells;
-    tm.main_cpt = main_cpt;
+    tm.emission_path = model->ek;
     /* final positions in the reference's list order: per read, sub-alignments in
        split order, each in reverse emission order */
     std::vector<int64_t> win_dst(nw);
@@ -1068,6 +1088,13 @@ int sa_result_read(const sa_result *res, int64_t i, sa_read_result *out) {
     out->n_windows = r.n_windows;
     out->n_subalignments = r.n_sub;
     return SA_OK;
+}
+
+int64_t sa_result_launches(const sa_result *res, sa_launch *out, int64_t cap) {
+    if (!res) return SA_ERR_INVALID_ARGUMENT;
+    const int64_t n = (int64_t)res->launches.size();
+    for (int64_t i = 0; i < n && i < cap; i++) out[i] = res->launches[i];
+    return n;
 }
 
 int sa_result_timing(const sa_result *res, sa_timing *out) {

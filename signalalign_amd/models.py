@@ -103,6 +103,20 @@ def load_model_file(path):
     return GaussianModel(alphabet, kmer_length, transitions, vals.reshape(n, 5))
 
 
+def write_model_file(model, path, transition_line=None):
+    """Write a GaussianModel in the .model format load_model_file reads (header,
+    nine transition probabilities + likelihood, the match table).  The
+    transitions written are `transition_line` (probabilities; the shipped
+    default when None), so the model must have been built from them."""
+    line = list(DEFAULT_TRANSITION_LINE if transition_line is None else transition_line)
+    if not np.array_equal(transitions_from_line(line), model.transitions):
+        raise ValueError("model transitions do not come from this transition line")
+    with open(path, "w") as fh:
+        fh.write(f"3\t{len(model.alphabet)}\t{model.alphabet}\t{model.kmer_length}\n")
+        fh.write("\t".join(repr(float(v)) for v in line) + "\t0.0\n")
+        fh.write("\t".join(repr(float(v)) for v in model.match.reshape(-1)) + "\n")
+
+
 def scale_noise(model, scale_sd, var_sd):
     """emissions_signal_scaleNoise (impl/stateMachine.c:609-629), including the
     reference's use of the match row's lambda for the gapY noise sd."""

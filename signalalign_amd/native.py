@@ -59,8 +59,13 @@ class _Timing(ctypes.Structure):
     _fields_ = [("prep_ms", ctypes.c_double), ("forward_ms", ctypes.c_double), ("backward_ms", ctypes.c_double),
                 ("collect_ms", ctypes.c_double), ("total_ms", ctypes.c_double), ("cells", ctypes.c_int64),
                 ("forward_launches", ctypes.c_int64), ("backward_launches", ctypes.c_int64),
-                ("fb_ms", ctypes.c_double), ("main_forward_ms", ctypes.c_double),
-                ("main_backward_ms", ctypes.c_double), ("main_cells", ctypes.c_int64), ("main_cpt", ctypes.c_int32)]
+                ("fb_ms", ctypes.c_double), ("emission_path", ctypes.c_int32)]
+
+
+class _Launch(ctypes.Structure):
+    _fields_ = [("direction", ctypes.c_int32), ("cpt", ctypes.c_int32), ("waves", ctypes.c_int32),
+                ("stream", ctypes.c_int32), ("items", ctypes.c_int64), ("cells", ctypes.c_int64),
+                ("start_ms", ctypes.c_double), ("ms", ctypes.c_double)]
 
 
 
@@ -94,6 +99,8 @@ def load():
     L.sa_result_n_reads.restype = ctypes.c_int64
     L.sa_result_read.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(_ReadResult)]
     L.sa_result_timing.argtypes = [vp, ctypes.POINTER(_Timing)]
+    L.sa_result_launches.argtypes = [vp, ctypes.POINTER(_Launch), ctypes.c_int64]
+    L.sa_result_launches.restype = ctypes.c_int64
     L.sa_result_free.argtypes = [vp]
     L.sa_last_error.argtypes = [vp]
     L.sa_last_error.restype = ctypes.c_char_p
@@ -322,6 +329,10 @@ class Batch:
             t = _Timing()
             L.sa_result_timing(res, ctypes.byref(t))
             timing = {k: getattr(t, k) for k, _ in _Timing._fields_}
+            n = L.sa_result_launches(res, None, 0)
+            arr = (_Launch * max(n, 1))()
+            L.sa_result_launches(res, arr, n)
+            timing["launches"] = [{k: getattr(arr[i], k) for k, _ in _Launch._fields_} for i in range(n)]
             out = []
             if fetch:
                 rr = _ReadResult()
