@@ -102,6 +102,135 @@ def copy_calibration(mib):
     del x, y
 
 
+def hdp_leg(native, synth, ctx, args):
+    """The HDP configuration: 5-mer ACEGT template and complement HDP models (3125
+    k-mers, mixture densities on a 2000-point grid), 2048 reads (median 5 kb) on
+    both strands (the complement read aligns to the reverse-complement target with
+    the complement model), band expansion 50, FP32.  One step = both strands' batches
+    (run one after the other); GCUPS = cells / summed device time."""
+    t0 = time.time()
+    mt = synth.hdp_model("ACEGT", 5, seed=3)
+    mc = synth.hdp_model("ACEGT", 5, seed=4)
+    reads = synth.batch_reads(mt, args.hdp_reads, 5000, seed=args.seed + 1, methyl_cpg=0.3, strands=2,
+                              complement_model=mc)
+    log(f"hdp: generated {len(reads)} strand-reads in {time.time() - t0:.1f}s")
+    params = native.Params(diagonal_expansion=50)
+    out = {}
+    batches = []
+    for strand, m in ((0, mt), (1, mc)):
+        dm = ctx.model(m)
+        rs = [r for r in reads if r["strand"] == strand]
+        batches.append((dm, native.Batch(ctx, dm, make_native_reads(native, rs), params)))
+    ms, cells = [], 0
+    for step in range(args.warmup + args.steps):
+        tot, c = 0.0, 0
+        for _, b in batches:
+            _, tm = b.run(native.SA_FP32, fetch=step == args.warmup + args.steps - 1)
+            tot += tm["total_ms"]
+            c += tm["cells"]
+        if step >= args.warmup:
+            ms.append(tot)
+            cells = c
+    for dm, b in batches:
+        b.close()
+        dm.close()
+    out = {"workload": f"HDP: {args.hdp_reads} reads x 2 strands, log-normal lengths median 5000 nt, ACEGT 5-mer "
+                       "template/complement HDP grids (3125 k-mers x 2000 points), band expansion 50, fp32",
+           "gcups": round(cells / float(np.mean(ms)) / 1e6, 3), "ms_per_step": round(float(np.mean(ms)), 3),
+           "cells": cells}
+    log(f"hdp: {out}")
+    return out
+
+
+def serving_leg(native, synth, ctx, gmodel, args):
+    """The serving configuration: Poisson arrivals of single-read requests at a fixed
+    rate (a fraction --serve-load of the capacity a closed burst measures first),
+    lengths from the batched configuration's distribution, a 50/50 mix of the ACEGT
+    6-mer Gaussian model and an ACEGT 5-mer HDP model (both resident), FP32, reads
+    admitted by band-matrix footprint into waves (continuous batching).  Requests
+    are drawn from a pool of synthetic reads per model.  Latency = result ready on
+    the host - submission; GCUPS = cells / (last result - first submission)."""
+    import queue
+    t0 = time.time()
+    hmodel = synth.hdp_model("ACEGT", 5, seed=3)
+    pools = []
+    for m, seed in ((gmodel, args.seed + 11), (hmodel, args.seed + 12)):
+        rs = synth.batch_reads(m, args.serve_pool, args.median, seed=seed, methyl_cpg=0.3)
+        objs = make_native_reads(native, rs)
+        pools.append((ctx.model(m), objs, [o._struct() for o in objs]))
+    log(f"serving: pools of {args.serve_pool} reads per model in {time.time() - t0:.1f}s")
+    ps = native.Params(diagonal_expansion=args.expansion)._struct()
+    srv = native.Server(0, hbm_budget_bytes=96 << 30, lanes=3)
+    rng = np.random.default_rng(args.seed + 13)
+
+    def pick(n):
+        mi = rng.integers(0, 2, n)
+        ri = rng.integers(0, args.serve_pool, n)
+        return list(zip(mi.tolist(), ri.tolist()))
+
+    def run(reqs, rate):
+        """Submit `reqs` (at `rate` per second, or all at once), collect every result."""
+        q = queue.Queue()
+        lag = [0.0]
+
+        def submitter():
+            start = time.perf_counter()
+            arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs))) if rate else np.zeros(len(reqs))
+            for (mi, ri), at in zip(reqs, arrivals):
+                now = time.perf_counter() - start
+                if at > now:
+                    time.sleep(at - now)
+                else:
+                    lag[0] = max(lag[0], now - at)
+                dm, _, structs = pools[mi]
+                q.put(srv.submit_struct(dm, structs[ri], ps, native.SA_FP32))
+            q.put(None)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        lat, cells, t_first, t_last, waves = [], 0, None, 0.0, set()
+        while True:
+            t = q.get()
+            if t is None:
+                break
+            res, tm = srv.wait(t, timeout=300.0, fetch=False)
+            srv.release(t)
+            lat.append(tm["done_s"] - tm["submit_s"])
+            cells += res.cells
+            t_first = tm["submit_s"] if t_first is None else min(t_first, tm["submit_s"])
+            t_last = max(t_last, tm["done_s"])
+            waves.add(tm["wave"])
+        th.join()
+        return np.array(lat), cells, t_last - t_first, len(waves), lag[0]
+
+    burst = pick(args.serve_burst)
+    lat, cells, span, nw, _ = run(burst, 0.0)
+    capacity = len(burst) / span
+    log(f"serving: burst of {len(burst)} reads in {span:.2f}s ({capacity:.0f} reads/s, "
+        f"{cells / span / 1e9:.1f} GCUPS, {nw} waves)")
+    rate = args.serve_load * capacity
+    reqs = pick(args.serve_requests)
+    lat, cells, span, nw, lag = run(reqs, rate)
+    st = srv.stats()
+    srv.close()
+    for dm, _, _ in pools:
+        dm.close()
+    out = {"workload": f"serving: {len(reqs)} single-read requests, Poisson arrivals at {rate:.0f} reads/s "
+                       f"({args.serve_load:.0%} of the {capacity:.0f} reads/s a {len(burst)}-read burst sustains), "
+                       f"lengths log-normal median {args.median} nt, 50/50 ACEGT 6-mer Gaussian / ACEGT 5-mer HDP "
+                       f"models resident, band expansion {args.expansion}, fp32, drawn from pools of "
+                       f"{args.serve_pool} synthetic reads per model",
+           "rate_reads_per_s": round(rate, 1), "capacity_reads_per_s": round(capacity, 1),
+           "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+           "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+           "max_ms": round(float(lat.max()) * 1e3, 3), "gcups": round(cells / span / 1e9, 3),
+           "achieved_reads_per_s": round(len(reqs) / span, 1), "waves": nw,
+           "mean_wave_reads": round(len(reqs) / max(nw, 1), 1), "submitter_max_lag_ms": round(lag * 1e3, 3),
+           "max_wave_reads": st["max_wave_reads"]}
+    log(f"serving: {out}")
+    return out
+
+
 def traffic_for(kname, cells, args):
     """HBM bytes per launch of `kname`: the calibrated rocprofv3 FETCH_SIZE +
     WRITE_SIZE bytes per band cell kept with the profiles, times the launch's cells
@@ -159,6 +288,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fp64-check", action="store_true", help="skip the FP32-vs-FP64 per-read error report")
+    ap.add_argument("--no-hdp", action="store_true", help="skip the HDP-emission configuration")
+    ap.add_argument("--hdp-reads", type=int, default=2048)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving configuration")
+    ap.add_argument("--serve-requests", type=int, default=100000)
+    ap.add_argument("--serve-pool", type=int, default=1024)
+    ap.add_argument("--serve-burst", type=int, default=8192)
+    ap.add_argument("--serve-load", type=float, default=0.7)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run a device copy of this many MiB (HBM byte-counter calibration under rocprofv3)")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_batched_4096reads.txt"))
@@ -273,6 +409,9 @@ def main():
         plain["per_read_loop_reads"] = len(sub)
         log(f"plain: {plain}")
 
+    hdp = None if args.no_hdp else hdp_leg(native, synth, ctx, args)
+    serving = None if args.no_serve else serving_leg(native, synth, ctx, model, args)
+
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -292,6 +431,8 @@ def main():
         "cpu_baseline": cpu,
         "fp32_error_vs_fp64": fp_err,
         "plain_gpu": plain,
+        "hdp": hdp,
+        "serving": serving,
         "wall_ms_per_step_pcie_inclusive": round(float(np.mean(wall_ms)), 3),
     }
     if cpu:

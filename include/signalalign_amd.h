@@ -30,6 +30,7 @@ extern "C" {
                                         (asserts at impl/pairwiseAligner.c:161-167) */
 #define SA_ERR_UNSUPPORTED (-6)      /* band wider than the kernels support, unknown emission */
 #define SA_ERR_NO_DEVICE (-7)
+#define SA_PENDING 1 /* sa_server_wait: the request has not finished */
 
 /* Arithmetic of the recurrence.  SA_FP64 follows the reference's doubles;
  * SA_FP32 runs the cell recurrence in float log space relative to a per-diagonal
@@ -153,12 +154,16 @@ typedef struct sa_model sa_model;
 typedef struct sa_result sa_result;
 
 int sa_context_create(int device, sa_context **out);
+/* n_streams 4 (as sa_context_create): sub-alignment groups run on four streams;
+ * 1: every launch on one stream (for callers that run several contexts side by side). */
+int sa_context_create_ex(int device, int n_streams, sa_context **out);
 void sa_context_destroy(sa_context *ctx);
 int sa_device_count(int *count);
 
 /* Uploads a model's tables to HBM (resident until sa_model_destroy). */
 int sa_model_create(sa_context *ctx, const sa_model_desc *desc, sa_model **out);
 void sa_model_destroy(sa_model *model);
+int sa_model_kmer_length(const sa_model *model);
 
 /* Plain per-read call: the drop-in for one getAlignedPairsUsingAnchors call. */
 int sa_align(sa_context *ctx, const sa_model *model, const sa_read *read, const sa_params *params,
@@ -176,6 +181,9 @@ typedef struct sa_batch sa_batch;
 int sa_batch_create(sa_context *ctx, const sa_model *model, const sa_read *reads, int64_t n_reads,
                     const sa_params *params, sa_batch **out);
 int sa_batch_run(sa_batch *batch, sa_precision precision, sa_result **out);
+/* Replaces the batch's reads (and model/params), reusing its device buffers. */
+int sa_batch_refill(sa_batch *batch, const sa_model *model, const sa_read *reads, int64_t n_reads,
+                    const sa_params *params);
 void sa_batch_destroy(sa_batch *batch);
 
 int64_t sa_result_n_reads(const sa_result *res);
@@ -203,6 +211,45 @@ int64_t sa_split_points(const int64_t *anchors, int64_t n_anchors, int64_t lX, i
 
 /* Text of the last error on this context (valid until the next call). */
 const char *sa_last_error(const sa_context *ctx);
+
+/* ---- serving: request loop with continuous batching --------------------------
+ * Reads are submitted one at a time (inputs copied) and grouped by a scheduler
+ * thread into waves of the same model, parameters and precision, admitted while
+ * the band-matrix bytes of the waves in flight fit the HBM budget; up to `lanes`
+ * waves run at once on the device.  Models may come from any context on the same
+ * device and must outlive the requests that use them.  A result does not depend
+ * on the wave it ran in. */
+typedef struct sa_server sa_server;
+typedef struct {
+    int64_t hbm_budget_bytes; /* band-matrix bytes of the waves in flight (0: 32 GiB) */
+    int32_t lanes;            /* waves in flight, 1-4 (0: 3) */
+    int32_t max_wave_reads;   /* reads per wave at most (0: 2048) */
+} sa_server_config;
+typedef struct {
+    double submit_s, admit_s, done_s; /* seconds on the server's clock */
+    int64_t wave;                     /* wave the request ran in */
+    int32_t wave_reads;               /* reads in that wave */
+    int32_t pad;
+} sa_request_timing;
+typedef struct {
+    int64_t waves, reads, cells, max_wave_reads;
+    double device_ms;        /* device time summed over waves (they overlap) */
+    double now_s;
+    int64_t pending;         /* submitted, not yet admitted */
+    int64_t inflight_bytes;  /* band-matrix bytes admitted and not finished */
+} sa_server_stats;
+
+int sa_server_create(int device, const sa_server_config *config, sa_server **out);
+int sa_server_submit(sa_server *server, const sa_model *model, const sa_read *read, const sa_params *params,
+                     sa_precision precision, int64_t *ticket);
+/* Waits up to timeout_s (0: poll) for a request: SA_OK with its result (valid until
+ * sa_server_release), SA_PENDING, or the request's error status. */
+int sa_server_wait(sa_server *server, int64_t ticket, double timeout_s, sa_read_result *out,
+                   sa_request_timing *timing);
+int sa_server_release(sa_server *server, int64_t ticket);
+int sa_server_get_stats(sa_server *server, sa_server_stats *out);
+/* Finishes the waves in flight, drops pending requests, frees everything. */
+void sa_server_destroy(sa_server *server);
 
 #ifdef __cplusplus
 }

@@ -36,7 +36,7 @@ struct DBuf {
     hipError_t ensure(size_t bytes) {
         if (bytes <= cap && p) return hipSuccess;
         if (p) { (void)hipFree(p); p = nullptr; cap = 0; }
-        size_t want = std::max<size_t>(bytes, 256);
+        size_t want = std::max<size_t>(bytes + bytes / 4, 256); /* headroom: refilled batches grow rarely */
         hipError_t e = hipMalloc(&p, want);
         if (e == hipSuccess) cap = want;
         return e;
@@ -64,6 +64,7 @@ struct sa_context {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t aux[3] = {};   /* sub-alignment groups of different window sizes run side by side */
+    int n_streams = 4;         /* 1: everything on `stream` (aux[] alias it) */
     std::mutex mu;
     std::string err;
     hipEvent_t ev[12] = {};
@@ -159,7 +160,12 @@ int fail(sa_context *ctx, int code, const std::string &msg) {
 template <typename T>
 int upload(sa_context *ctx, DBuf &b, const std::vector<T> &v) {
     SA_HIP(ctx, b.ensure(sizeof(T) * std::max<size_t>(v.size(), 1)));
-    if (!v.empty()) SA_HIP(ctx, hipMemcpy(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    /* stream-ordered: a blocking copy on the null stream would wait for the work of
+       every other context on the device */
+    if (!v.empty()) {
+        SA_HIP(ctx, hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, ctx->stream));
+        SA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
     return SA_OK;
 }
 
@@ -317,31 +323,39 @@ int sa_device_count(int *count) {
     return SA_OK;
 }
 
-int sa_context_create(int device, sa_context **out) {
+int sa_context_create_ex(int device, int n_streams, sa_context **out) {
     *out = nullptr;
+    if (n_streams != 1 && n_streams != 4) return SA_ERR_INVALID_ARGUMENT;
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return SA_ERR_NO_DEVICE;
     sa_context *c = new sa_context();
     c->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+    c->n_streams = n_streams;
+    /* non-blocking streams at the default priority: no implicit waits on the null stream */
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return SA_ERR_HIP;
     }
     for (auto &e : c->ev) (void)hipEventCreate(&e);
     for (auto &e : c->gev) (void)hipEventCreate(&e);
-    for (auto &q : c->aux) (void)hipStreamCreate(&q);
+    for (auto &q : c->aux) {
+        if (n_streams == 1) q = c->stream;
+        else (void)hipStreamCreateWithFlags(&q, hipStreamNonBlocking);
+    }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->n_cu = prop.multiProcessorCount;
     *out = c;
     return SA_OK;
 }
 
+int sa_context_create(int device, sa_context **out) { return sa_context_create_ex(device, 4, out); }
+
 void sa_context_destroy(sa_context *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     for (auto &e : ctx->ev) if (e) (void)hipEventDestroy(e);
     for (auto &e : ctx->gev) if (e) (void)hipEventDestroy(e);
-    for (auto &q : ctx->aux) if (q) (void)hipStreamDestroy(q);
+    for (auto &q : ctx->aux) if (q && q != ctx->stream) (void)hipStreamDestroy(q);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -424,6 +438,8 @@ int sa_model_create(sa_context *ctx, const sa_model_desc *d, sa_model **out) {
     return SA_OK;
 }
 
+int sa_model_kmer_length(const sa_model *m) { return m ? m->k : 0; }
+
 void sa_model_destroy(sa_model *m) {
     if (!m) return;
     (void)hipSetDevice(m->ctx->device);
@@ -438,12 +454,14 @@ int sa_batch_create(sa_context *ctx, const sa_model *model, const sa_read *reads
 int sa_batch_run(sa_batch *b, sa_precision precision, sa_result **out);
 void sa_batch_destroy(sa_batch *b);
 
-int sa_batch_create(sa_context *ctx, const sa_model *model, const sa_read *reads, int64_t n_reads,
-                    const sa_params *params, sa_batch **out) {
-    *out = nullptr;
-    if (!ctx || !model || (!reads && n_reads > 0) || n_reads < 0) return SA_ERR_INVALID_ARGUMENT;
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    (void)hipSetDevice(ctx->device);
+} // extern "C"
+
+namespace {
+
+/* Splits the reads into sub-alignments and uploads the raw inputs into the batch's
+   device buffers (which only grow, so a refilled batch reuses them). */
+int batch_fill(sa_context *ctx, sa_batch *b, const sa_model *model, const sa_read *reads, int64_t n_reads,
+               const sa_params *params) {
     sa_params p;
     if (params) p = *params; else sa_params_default(&p);
     /* getPosteriorProbsWithBanding prerequisites (impl/pairwiseAligner.c:1291-1295) */
@@ -451,12 +469,15 @@ int sa_batch_create(sa_context *ctx, const sa_model *model, const sa_read *reads
         !(p.min_diags_between_traceback >= 2) || !(p.traceback_diagonals + 1 < p.min_diags_between_traceback) ||
         !(p.threshold >= 0.0 && p.threshold <= 1.0))
         return fail(ctx, SA_ERR_INVALID_ARGUMENT, "invalid banding parameters");
-    sa_batch *b = new sa_batch();
     b->ctx = ctx;
     b->model = model;
     b->params = p;
     b->n_reads = n_reads;
-    b->reads.resize(n_reads);
+    b->reads.assign(n_reads, ReadPlan());
+    b->tasks.clear();
+    b->thost.clear();
+    b->total_pos = b->total_ev = b->total_diag = b->total_wslots = b->total_anc = 0;
+    b->ref_bytes = b->ev_records = 0;
     const int k = model->k;
     std::vector<int64_t> splits;
     std::vector<int32_t> ancx, ancy;
@@ -466,14 +487,12 @@ int sa_batch_create(sa_context *ctx, const sa_model *model, const sa_read *reads
         const sa_read &rd = reads[r];
         if (rd.event_stride < 2 || (stride >= 0 && rd.event_stride != stride) || rd.n_events < 0 ||
             rd.ref_length < 0 || rd.n_anchors < 0 || !(rd.var != 0.0)) {
-            delete b;
             return fail(ctx, SA_ERR_INVALID_ARGUMENT, "read " + std::to_string(r) + ": bad sizes");
         }
         stride = rd.event_stride;
         const int64_t lX = correct_length(rd.ref_length, k), lY = rd.n_events;
         if (!split_points(rd.anchors, rd.n_anchors, lX, lY, p.split_matrix_bigger_than_this, rd.ragged_left,
                           rd.ragged_right, splits)) {
-            delete b;
             return fail(ctx, SA_ERR_ANCHORS, "read " + std::to_string(r) + ": anchors out of order or range");
         }
         const int64_t nsplit = (int64_t)splits.size() / 4;
@@ -487,7 +506,6 @@ int sa_batch_create(sa_context *ctx, const sa_model *model, const sa_read *reads
             t.lX = (int32_t)(x2 - x1);
             t.lY = (int32_t)(y2 - y1);
             if (x2 - x1 > INT32_MAX / 4 || y2 - y1 > INT32_MAX / 4) {
-                delete b;
                 return fail(ctx, SA_ERR_UNSUPPORTED, "sub-alignment too long");
             }
             t.ref_off = ref_off + x1;
@@ -502,7 +520,6 @@ int sa_batch_create(sa_context *ctx, const sa_model *model, const sa_read *reads
                 const int64_t x = rd.anchors[2 * j], y = rd.anchors[2 * j + 1];
                 if (x + y >= x2 + y2) break;
                 if (!(x >= x1 && x < x2 && y >= y1 && y < y2) || x - x1 <= px || y - y1 <= py) {
-                    delete b;
                     return fail(ctx, SA_ERR_ANCHORS, "read " + std::to_string(r) + ": anchors out of order");
                 }
                 px = x - x1;
@@ -534,7 +551,6 @@ int sa_batch_create(sa_context *ctx, const sa_model *model, const sa_read *reads
             }
         }
         if (j != rd.n_anchors) {
-            delete b;
             return fail(ctx, SA_ERR_ANCHORS, "read " + std::to_string(r) + ": anchors not consumed");
         }
         ref_off += rd.ref_length;
@@ -562,13 +578,49 @@ int sa_batch_create(sa_context *ctx, const sa_model *model, const sa_read *reads
     if ((rc = upload(ctx, b->d_ref, ref)) || (rc = upload(ctx, b->d_events, ev)) ||
         (rc = upload(ctx, b->d_ancx, ancx)) || (rc = upload(ctx, b->d_ancy, ancy)) ||
         (rc = upload(ctx, b->d_tasks, b->tasks)) || (rc = upload(ctx, b->d_x1, x1)) ||
-        (rc = upload(ctx, b->d_y1, y1))) {
+        (rc = upload(ctx, b->d_y1, y1)))
+        return rc;
+    return SA_OK;
+
+}
+
+void batch_clear(sa_batch *b) {
+    b->n_reads = 0;
+    b->reads.clear();
+    b->tasks.clear();
+    b->thost.clear();
+    b->total_pos = b->total_ev = b->total_diag = b->total_wslots = b->total_anc = 0;
+}
+
+} // namespace
+
+extern "C" {
+
+int sa_batch_create(sa_context *ctx, const sa_model *model, const sa_read *reads, int64_t n_reads,
+                    const sa_params *params, sa_batch **out) {
+    *out = nullptr;
+    if (!ctx || !model || (!reads && n_reads > 0) || n_reads < 0) return SA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    sa_batch *b = new sa_batch();
+    const int rc = batch_fill(ctx, b, model, reads, n_reads, params);
+    if (rc != SA_OK) {
         delete b;
         return rc;
     }
-
     *out = b;
     return SA_OK;
+}
+
+int sa_batch_refill(sa_batch *b, const sa_model *model, const sa_read *reads, int64_t n_reads,
+                    const sa_params *params) {
+    if (!b || !model || (!reads && n_reads > 0) || n_reads < 0) return SA_ERR_INVALID_ARGUMENT;
+    sa_context *ctx = b->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    const int rc = batch_fill(ctx, b, model, reads, n_reads, params);
+    if (rc != SA_OK) batch_clear(b);
+    return rc;
 }
 
 void sa_batch_destroy(sa_batch *b) {
@@ -853,7 +905,8 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
        join on ev[4] */
     auto run_sweeps = [&](bool forward) -> int {
         SA_HIP(ctx, hipEventRecord(ctx->ev[2], st));
-        for (int q = 1; q < kLanes; q++) SA_HIP(ctx, hipStreamWaitEvent(gst[q], ctx->ev[2], 0));
+        for (int q = 1; q < kLanes; q++)
+            if (gst[q] != st) SA_HIP(ctx, hipStreamWaitEvent(gst[q], ctx->ev[2], 0));
         for (int lane = 0; lane < kLanes; lane++) {
             hipStream_t sg = gst[lane];
             if (forward)
@@ -872,6 +925,7 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
             }
         }
         for (int q = 1; q < kLanes; q++) {
+            if (gst[q] == st) continue;
             SA_HIP(ctx, hipEventRecord(ctx->ev[8 + q], gst[q]));
             SA_HIP(ctx, hipStreamWaitEvent(st, ctx->ev[8 + q], 0));
         }

@@ -1,0 +1,389 @@
+/*
+ * sa_serve.cpp -- request loop with continuous batching on one device.
+ *
+ * Clients submit single reads (inputs are copied); a scheduler thread forms
+ * "waves" from the pending requests -- same model, parameters and precision, in
+ * arrival order -- admitting each read only while the band-matrix bytes of all
+ * waves in flight stay within the HBM budget.  Each wave runs on one of several
+ * lanes (a context with one stream and a batch whose device buffers are reused),
+ * so waves overlap on the device: a request that arrives while others run is
+ * admitted as soon as a lane and memory free up.  Model tables stay resident.
+ *
+ * A read's result does not depend on the wave it ran in: every sub-alignment's
+ * arithmetic is fixed by its own inputs (kernel shapes only change which lanes of
+ * a wavefront hold which cells, and the cell arithmetic and its order are the same
+ * in every shape).
+ */
+#include "signalalign_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+namespace {
+
+struct Request {
+    int64_t ticket = 0;
+    const sa_model *model = nullptr;
+    sa_params params{};
+    sa_precision precision = SA_FP32;
+    /* owned inputs */
+    std::string ref;
+    std::vector<double> events;
+    int64_t n_events = 0, stride = 4;
+    std::vector<int64_t> anchors;
+    double scale = 1, shift = 0, var = 1;
+    int32_t rl = 1, rr = 1;
+    int64_t est_bytes = 0;
+    /* outputs */
+    int status = SA_PENDING;
+    std::vector<sa_pair> pairs;
+    double log_likelihood = 0.0;
+    int64_t cells = 0, n_windows = 0, n_sub = 0;
+    sa_request_timing timing{};
+};
+
+struct Lane {
+    sa_context *ctx = nullptr;
+    sa_batch *batch = nullptr;
+    std::thread th;
+    std::vector<Request *> wave;
+    int64_t wave_bytes = 0;
+    int64_t wave_id = 0;
+    bool busy = false;
+};
+
+bool same_params(const sa_params &a, const sa_params &b) {
+    return a.threshold == b.threshold && a.min_diags_between_traceback == b.min_diags_between_traceback &&
+           a.traceback_diagonals == b.traceback_diagonals && a.diagonal_expansion == b.diagonal_expansion &&
+           a.split_matrix_bigger_than_this == b.split_matrix_bigger_than_this;
+}
+
+} // namespace
+
+struct sa_server {
+    sa_server_config cfg{};
+    int device = 0;
+    std::chrono::steady_clock::time_point t0;
+    std::vector<std::unique_ptr<Lane>> lanes;
+    std::mutex mu;
+    std::condition_variable cv_sched, cv_lane, cv_done;
+    std::deque<Request *> pending;
+    std::unordered_map<int64_t, Request *> reqs;
+    int64_t next_ticket = 1;
+    int64_t inflight_bytes = 0;
+    int64_t next_wave = 1;
+    bool stop = false;
+    std::thread scheduler;
+    sa_server_stats stats{};
+
+    double now() const {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    void schedule_loop();
+    void lane_loop(Lane *L);
+};
+
+void sa_server::schedule_loop() {
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+        Lane *idle = nullptr;
+        cv_sched.wait(lk, [&] {
+            if (stop) return true;
+            if (pending.empty()) return false;
+            idle = nullptr;
+            for (auto &l : lanes)
+                if (!l->busy) { idle = l.get(); break; }
+            if (!idle) return false;
+            /* memory: the head request must fit beside the waves in flight (alone, it is
+               admitted once nothing else is in flight) */
+            const int64_t head = pending.front()->est_bytes;
+            return inflight_bytes == 0 || inflight_bytes + head <= cfg.hbm_budget_bytes;
+        });
+        if (stop) return;
+        Request *first = pending.front();
+        std::vector<Request *> wave;
+        int64_t bytes = 0;
+        for (auto it = pending.begin(); it != pending.end() && (int64_t)wave.size() < cfg.max_wave_reads;) {
+            Request *q = *it;
+            const bool fits = wave.empty() ? true : inflight_bytes + bytes + q->est_bytes <= cfg.hbm_budget_bytes;
+            if (q->model == first->model && q->precision == first->precision &&
+                same_params(q->params, first->params) && fits) {
+                wave.push_back(q);
+                bytes += q->est_bytes;
+                it = pending.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        const double t = now();
+        const int64_t id = next_wave++;
+        for (Request *q : wave) {
+            q->timing.admit_s = t;
+            q->timing.wave = id;
+            q->timing.wave_reads = (int32_t)wave.size();
+        }
+        idle->wave = std::move(wave);
+        idle->wave_bytes = bytes;
+        idle->wave_id = id;
+        idle->busy = true;
+        inflight_bytes += bytes;
+        cv_lane.notify_all();
+    }
+}
+
+void sa_server::lane_loop(Lane *L) {
+    (void)hipSetDevice(device);
+    for (;;) {
+        std::vector<Request *> wave;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv_lane.wait(lk, [&] { return stop || (L->busy && !L->wave.empty()); });
+            if (stop) return;
+            wave = L->wave;
+        }
+        std::vector<sa_read> rd(wave.size());
+        for (size_t i = 0; i < wave.size(); i++) {
+            const Request *q = wave[i];
+            rd[i].ref = q->ref.data();
+            rd[i].ref_length = (int64_t)q->ref.size();
+            rd[i].events = q->events.data();
+            rd[i].n_events = q->n_events;
+            rd[i].event_stride = q->stride;
+            rd[i].anchors = q->anchors.data();
+            rd[i].n_anchors = (int64_t)q->anchors.size() / 2;
+            rd[i].scale = q->scale;
+            rd[i].shift = q->shift;
+            rd[i].var = q->var;
+            rd[i].ragged_left = q->rl;
+            rd[i].ragged_right = q->rr;
+        }
+        const Request *f = wave.front();
+        int64_t cells = 0;
+        double dev_ms = 0.0;
+        /* runs reads [i0, i1) of the wave as one batch */
+        auto run = [&](size_t i0, size_t i1) -> int {
+            const int64_t n = (int64_t)(i1 - i0);
+            int rc = L->batch ? sa_batch_refill(L->batch, f->model, rd.data() + i0, n, &f->params)
+                              : sa_batch_create(L->ctx, f->model, rd.data() + i0, n, &f->params, &L->batch);
+            sa_result *res = nullptr;
+            if (rc == SA_OK) rc = sa_batch_run(L->batch, f->precision, &res);
+            if (rc != SA_OK) return rc;
+            for (size_t i = i0; i < i1; i++) {
+                Request *q = wave[i];
+                sa_read_result rr;
+                sa_result_read(res, (int64_t)(i - i0), &rr);
+                q->status = SA_OK;
+                q->pairs.assign(rr.pairs, rr.pairs + rr.n_pairs);
+                q->log_likelihood = rr.log_likelihood;
+                q->cells = rr.cells;
+                q->n_windows = rr.n_windows;
+                q->n_sub = rr.n_subalignments;
+                cells += rr.cells;
+            }
+            sa_timing tm;
+            sa_result_timing(res, &tm);
+            dev_ms += tm.total_ms;
+            sa_result_free(res);
+            return SA_OK;
+        };
+        const int rc = run(0, wave.size());
+        if (rc != SA_OK) {
+            /* one bad read fails its batch: run the reads one by one so that each gets
+               its own status */
+            if (wave.size() == 1) {
+                wave[0]->status = rc;
+            } else {
+                for (size_t i = 0; i < wave.size(); i++) {
+                    const int r1 = run(i, i + 1);
+                    if (r1 != SA_OK) wave[i]->status = r1;
+                }
+            }
+        }
+        for (Request *q : wave) { /* inputs are no longer needed */
+            std::string().swap(q->ref);
+            std::vector<double>().swap(q->events);
+            std::vector<int64_t>().swap(q->anchors);
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            const double t = now();
+            for (Request *q : wave) q->timing.done_s = t;
+            stats.waves++;
+            stats.reads += (int64_t)wave.size();
+            stats.cells += cells;
+            stats.device_ms += dev_ms;
+            if ((int64_t)wave.size() > stats.max_wave_reads) stats.max_wave_reads = (int64_t)wave.size();
+            inflight_bytes -= L->wave_bytes;
+            L->wave.clear();
+            L->wave_bytes = 0;
+            L->busy = false;
+        }
+        cv_done.notify_all();
+        cv_sched.notify_all();
+    }
+}
+
+namespace {
+
+/* HBM a read occupies while it runs: band cells (forward match plane, gap planes on
+   refresh diagonals) plus per-diagonal, per-event and output arrays.  Band cells
+   are bounded by the rectangles between consecutive anchors widened by the band
+   expansion (band_construct keeps each diagonal within the anchors around it):
+   O(anchors), cheap enough for the submitting thread. */
+int64_t estimate_bytes(const sa_read *r, int kmer_length, const sa_params *p, sa_precision prec) {
+    const int64_t lX = r->ref_length >= kmer_length ? r->ref_length - kmer_length + 1 : 0;
+    const int64_t lY = r->n_events;
+    const int64_t e = p->diagonal_expansion;
+    int64_t cells = 0, px = 0, py = 0;
+    for (int64_t i = 0; i <= r->n_anchors; i++) {
+        const int64_t x = i < r->n_anchors ? r->anchors[2 * i] + 1 : lX;
+        const int64_t y = i < r->n_anchors ? r->anchors[2 * i + 1] + 1 : lY;
+        cells += (std::min(lX, std::max<int64_t>(x - px, 0) + e) + 1) * (std::min(lY, std::max<int64_t>(y - py, 0) + e) + 1);
+        px = x;
+        py = y;
+    }
+    const int64_t w = prec == SA_FP32 ? 4 : 8;
+    return cells * w * 3 / 2 + (lX + lY + 1) * 32 + lY * (4 * w + 8 * r->event_stride) + lX * 8 + lY * 36 + 4096;
+}
+
+} // namespace
+
+extern "C" {
+
+int sa_server_create(int device, const sa_server_config *cfg, sa_server **out) {
+    *out = nullptr;
+    sa_server_config c{};
+    if (cfg) c = *cfg;
+    if (c.hbm_budget_bytes <= 0) c.hbm_budget_bytes = (int64_t)32 << 30;
+    if (c.lanes <= 0) c.lanes = 3;
+    if (c.max_wave_reads <= 0) c.max_wave_reads = 2048;
+    if (c.lanes > 4) return SA_ERR_INVALID_ARGUMENT;
+    auto *s = new sa_server();
+    s->cfg = c;
+    s->device = device;
+    s->t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < c.lanes; i++) {
+        auto L = std::make_unique<Lane>();
+        const int rc = sa_context_create_ex(device, 1, &L->ctx);
+        if (rc != SA_OK) {
+            for (auto &l : s->lanes) sa_context_destroy(l->ctx);
+            delete s;
+            return rc;
+        }
+        s->lanes.push_back(std::move(L));
+    }
+    for (auto &l : s->lanes) {
+        Lane *p = l.get();
+        l->th = std::thread([s, p] { s->lane_loop(p); });
+    }
+    s->scheduler = std::thread([s] { s->schedule_loop(); });
+    *out = s;
+    return SA_OK;
+}
+
+int sa_server_submit(sa_server *s, const sa_model *model, const sa_read *read, const sa_params *params,
+                     sa_precision precision, int64_t *ticket) {
+    if (!s || !model || !read || !ticket || (precision != SA_FP32 && precision != SA_FP64))
+        return SA_ERR_INVALID_ARGUMENT;
+    if (read->ref_length < 0 || read->n_events < 0 || read->n_anchors < 0 || read->event_stride < 2)
+        return SA_ERR_INVALID_ARGUMENT;
+    auto *q = new Request();
+    q->model = model;
+    if (params) q->params = *params; else sa_params_default(&q->params);
+    q->precision = precision;
+    q->ref.assign(read->ref, (size_t)read->ref_length);
+    q->n_events = read->n_events;
+    q->stride = read->event_stride;
+    q->events.assign(read->events, read->events + read->n_events * read->event_stride);
+    q->anchors.assign(read->anchors, read->anchors + 2 * read->n_anchors);
+    q->scale = read->scale;
+    q->shift = read->shift;
+    q->var = read->var;
+    q->rl = read->ragged_left;
+    q->rr = read->ragged_right;
+    q->est_bytes = estimate_bytes(read, sa_model_kmer_length(model), &q->params, precision);
+    std::lock_guard<std::mutex> lk(s->mu);
+    q->ticket = s->next_ticket++;
+    q->timing.submit_s = s->now();
+    s->reqs[q->ticket] = q;
+    s->pending.push_back(q);
+    *ticket = q->ticket;
+    s->cv_sched.notify_all();
+    return SA_OK;
+}
+
+int sa_server_wait(sa_server *s, int64_t ticket, double timeout_s, sa_read_result *out, sa_request_timing *timing) {
+    if (!s) return SA_ERR_INVALID_ARGUMENT;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return SA_ERR_INVALID_ARGUMENT;
+    Request *q = it->second;
+    auto done = [&] { return q->timing.done_s > 0.0; };
+    if (!done()) {
+        if (timeout_s <= 0.0) return SA_PENDING;
+        if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), done)) return SA_PENDING;
+    }
+    if (timing) *timing = q->timing;
+    if (q->status != SA_OK) return q->status;
+    if (out) {
+        out->pairs = q->pairs.empty() ? nullptr : q->pairs.data();
+        out->n_pairs = (int64_t)q->pairs.size();
+        out->log_likelihood = q->log_likelihood;
+        out->cells = q->cells;
+        out->n_windows = q->n_windows;
+        out->n_subalignments = q->n_sub;
+    }
+    return SA_OK;
+}
+
+int sa_server_release(sa_server *s, int64_t ticket) {
+    if (!s) return SA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end() || it->second->timing.done_s <= 0.0) return SA_ERR_INVALID_ARGUMENT;
+    delete it->second;
+    s->reqs.erase(it);
+    return SA_OK;
+}
+
+int sa_server_get_stats(sa_server *s, sa_server_stats *out) {
+    if (!s || !out) return SA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->stats;
+    out->now_s = s->now();
+    out->pending = (int64_t)s->pending.size();
+    out->inflight_bytes = s->inflight_bytes;
+    return SA_OK;
+}
+
+void sa_server_destroy(sa_server *s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_sched.notify_all();
+    s->cv_lane.notify_all();
+    s->cv_done.notify_all();
+    if (s->scheduler.joinable()) s->scheduler.join();
+    for (auto &l : s->lanes) {
+        if (l->th.joinable()) l->th.join();
+        if (l->batch) sa_batch_destroy(l->batch);
+        sa_context_destroy(l->ctx);
+    }
+    for (auto &kv : s->reqs) delete kv.second; /* every request, pending or done, until released */
+    delete s;
+}
+
+} // extern "C"

@@ -62,6 +62,21 @@ class _Timing(ctypes.Structure):
                 ("fb_ms", ctypes.c_double), ("emission_path", ctypes.c_int32)]
 
 
+class _ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("lanes", ctypes.c_int32), ("max_wave_reads", ctypes.c_int32)]
+
+
+class _ReqTiming(ctypes.Structure):
+    _fields_ = [("submit_s", ctypes.c_double), ("admit_s", ctypes.c_double), ("done_s", ctypes.c_double),
+                ("wave", ctypes.c_int64), ("wave_reads", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+class _ServerStats(ctypes.Structure):
+    _fields_ = [("waves", ctypes.c_int64), ("reads", ctypes.c_int64), ("cells", ctypes.c_int64),
+                ("max_wave_reads", ctypes.c_int64), ("device_ms", ctypes.c_double), ("now_s", ctypes.c_double),
+                ("pending", ctypes.c_int64), ("inflight_bytes", ctypes.c_int64)]
+
+
 class _Launch(ctypes.Structure):
     _fields_ = [("direction", ctypes.c_int32), ("cpt", ctypes.c_int32), ("waves", ctypes.c_int32),
                 ("stream", ctypes.c_int32), ("items", ctypes.c_int64), ("cells", ctypes.c_int64),
@@ -94,6 +109,16 @@ def load():
     L.sa_batch_create.argtypes = [vp, vp, ctypes.POINTER(_Read), ctypes.c_int64, ctypes.POINTER(_Params),
                                   ctypes.POINTER(vp)]
     L.sa_batch_run.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
+    L.sa_batch_refill.argtypes = [vp, vp, ctypes.POINTER(_Read), ctypes.c_int64, ctypes.POINTER(_Params)]
+    L.sa_context_create_ex.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
+    L.sa_server_create.argtypes = [ctypes.c_int, ctypes.POINTER(_ServerConfig), ctypes.POINTER(vp)]
+    L.sa_server_submit.argtypes = [vp, vp, ctypes.POINTER(_Read), ctypes.POINTER(_Params), ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_int64)]
+    L.sa_server_wait.argtypes = [vp, ctypes.c_int64, ctypes.c_double, ctypes.POINTER(_ReadResult),
+                                 ctypes.POINTER(_ReqTiming)]
+    L.sa_server_release.argtypes = [vp, ctypes.c_int64]
+    L.sa_server_get_stats.argtypes = [vp, ctypes.POINTER(_ServerStats)]
+    L.sa_server_destroy.argtypes = [vp]
     L.sa_batch_destroy.argtypes = [vp]
     L.sa_result_n_reads.argtypes = [vp]
     L.sa_result_n_reads.restype = ctypes.c_int64
@@ -352,6 +377,75 @@ class Batch:
     def close(self):
         if self._h:
             self.ctx._L.sa_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+SA_PENDING = 1
+
+
+class Server:
+    """Request loop with continuous batching (sa_server_*): submit reads one at a
+    time, collect results as waves finish."""
+
+    def __init__(self, device=0, hbm_budget_bytes=0, lanes=0, max_wave_reads=0):
+        L = load()
+        cfg = _ServerConfig(int(hbm_budget_bytes), int(lanes), int(max_wave_reads))
+        h = ctypes.c_void_p()
+        rc = L.sa_server_create(device, ctypes.byref(cfg), ctypes.byref(h))
+        if rc != 0:
+            raise SignalAlignError(rc, "sa_server_create")
+        self._h, self._L = h, L
+
+    def submit_struct(self, model, read_struct, params_struct, precision=SA_FP32):
+        t = ctypes.c_int64()
+        rc = self._L.sa_server_submit(self._h, model._h, ctypes.byref(read_struct), ctypes.byref(params_struct),
+                                      precision, ctypes.byref(t))
+        if rc != 0:
+            raise SignalAlignError(rc, "sa_server_submit")
+        return t.value
+
+    def submit(self, model, read, params=None, precision=SA_FP32):
+        return self.submit_struct(model, read._struct(), (params or Params())._struct(), precision)
+
+    def wait(self, ticket, timeout=60.0, fetch=True):
+        """(ReadResult or None, timing dict), or None while the request is pending."""
+        rr, tm = _ReadResult(), _ReqTiming()
+        rc = self._L.sa_server_wait(self._h, ticket, float(timeout), ctypes.byref(rr), ctypes.byref(tm))
+        if rc == SA_PENDING:
+            return None
+        timing = {k: getattr(tm, k) for k, _ in _ReqTiming._fields_ if k != "pad"}
+        if rc != 0:
+            raise SignalAlignError(rc, f"request {ticket}")
+        res = None
+        if fetch:
+            if rr.n_pairs:
+                buf = np.ctypeslib.as_array(ctypes.cast(rr.pairs, ctypes.POINTER(ctypes.c_int32)),
+                                            shape=(rr.n_pairs * 3,))
+                pairs = buf.reshape(-1, 3).astype(np.int64)
+            else:
+                pairs = np.zeros((0, 3), dtype=np.int64)
+            res = ReadResult(pairs, rr.log_likelihood, rr.cells, rr.n_windows, rr.n_subalignments)
+        else:
+            res = ReadResult(None, rr.log_likelihood, rr.cells, rr.n_windows, rr.n_subalignments)
+        return res, timing
+
+    def release(self, ticket):
+        self._L.sa_server_release(self._h, ticket)
+
+    def stats(self):
+        st = _ServerStats()
+        self._L.sa_server_get_stats(self._h, ctypes.byref(st))
+        return {k: getattr(st, k) for k, _ in _ServerStats._fields_}
+
+    def close(self):
+        if self._h:
+            self._L.sa_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

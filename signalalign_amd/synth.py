@@ -65,6 +65,21 @@ def anchors_from_path(path, lX, lY, rng, every=100, jitter=3):
     return np.array(out, dtype=np.int64).reshape(-1, 2)
 
 
+def _quantiles(model, n=4097):
+    """Inverse CDF of each density row on n uniform probabilities (piecewise linear
+    CDF over the grid), cached on the model: event levels are drawn from it."""
+    q = getattr(model, "_quantile_table", None)
+    if q is None:
+        g = model.grid
+        mass = 0.5 * (model.density[:, 1:] + model.density[:, :-1]) * np.diff(g)[None, :]
+        cdf = np.concatenate([np.zeros((len(mass), 1)), np.cumsum(mass, axis=1)], axis=1)
+        cdf /= cdf[:, -1:]
+        p = np.linspace(0.0, 1.0, n)
+        q = np.stack([np.interp(p, cdf[r], g) for r in range(len(cdf))])
+        model._quantile_table = q
+    return q
+
+
 def simulate_read(model, ref, rng, scale=None, shift=None, var=1.0):
     """Events and anchors for one strand of `ref` under `model` (GaussianModel or
     HdpModel).  Returns dict(ref, events (n,4), anchors (m,2), scale, shift, var, path)."""
@@ -81,14 +96,11 @@ def simulate_read(model, ref, rng, scale=None, shift=None, var=1.0):
     ev = np.zeros((lY, 4))
     if isinstance(model, _models.HdpModel):
         rows = model.kmer_row[kk]
-        g = model.grid
-        dx = np.diff(g)
-        mass = model.density[:, 1:] * dx[None, :]
-        cdf = np.cumsum(mass, axis=1)
-        cdf /= cdf[:, -1:]
-        u = rng.random(lY)
-        idx = np.array([np.searchsorted(cdf[r], q) for r, q in zip(rows, u)], dtype=np.int64)
-        level = g[np.minimum(idx + 1, len(g) - 1)]
+        q = _quantiles(model)
+        u = rng.random(lY) * (q.shape[1] - 1)
+        i0 = np.minimum(u.astype(np.int64), q.shape[1] - 2)
+        f = u - i0
+        level = q[rows, i0] * (1.0 - f) + q[rows, i0 + 1] * f
         mu = model.level_mean[kk]
         ev[:, 0] = scale * mu + shift + var * (level - mu)
         ev[:, 1] = rng.uniform(0.8, 2.0, lY)
