@@ -45,6 +45,51 @@ __device__ __forceinline__ T logadd(T a, T b) {
     return d < (T)7.5 ? r : hi;
 }
 
+/* The same log-add with the cubic's coefficients read from a 16-entry table in LDS
+ * instead of selected by three compares.  Entry k = ceil(2 * min(d, 7.5)) holds the
+ * piece containing (0.5(k-1), 0.5k]; the pieces change at 1.0, 2.5 and 4.5, all
+ * multiples of 0.5, so each entry lies inside one piece and the reference's
+ * inclusive bounds (d <= 1.0, d <= 2.5, d <= 4.5) select the same piece.  d is NaN
+ * only when both operands are -inf (min(NaN, 7.5) = 7.5 picks some entry; the
+ * result is hi either way).  Results are bit-identical to logadd(). */
+template <typename T> struct LutEntry;
+template <> struct LutEntry<float> { float c[4]; };
+template <> struct LutEntry<double> { double c[4]; };
+
+template <typename T>
+__device__ __forceinline__ void lut_fill(LutEntry<T> *lut, int tid) {
+    if (tid < 16) {
+        const int piece = tid <= 2 ? 0 : tid <= 5 ? 1 : tid <= 9 ? 2 : 3;
+        const float C[4][4] = {
+            {-0.009350833524763f, 0.130659527668286f, 0.498799810682272f, 0.693203116424741f},
+            {-0.014532321752540f, 0.139942324101744f, 0.495635523139337f, 0.692140569840976f},
+            {-0.004605031767994f, 0.063427417320019f, 0.695956496475118f, 0.514272634594009f},
+            {-0.000458661602210f, 0.009695946122598f, 0.930734667215156f, 0.168037164329057f}};
+        for (int i = 0; i < 4; i++) lut[tid].c[i] = (T)C[piece][i];
+    }
+}
+
+__device__ __forceinline__ float logadd_lut(float a, float b, const LutEntry<float> *lut) {
+    const float hi = __builtin_fmaxf(a, b), lo = __builtin_fminf(a, b);
+    const float d = hi - lo;
+    const float dd = __builtin_fminf(d, 7.5f);
+    const int k = (int)__builtin_ceilf(dd + dd);
+    const float4 c = *reinterpret_cast<const float4 *>(&lut[k]);
+    const float r = poly3(d, c.x, c.y, c.z, c.w) + lo;
+    return d < 7.5f ? r : hi;
+}
+
+__device__ __forceinline__ double logadd_lut(double a, double b, const LutEntry<double> *lut) {
+    const double hi = __builtin_fmax(a, b), lo = __builtin_fmin(a, b);
+    const double d = hi - lo;
+    const double dd = __builtin_fmin(d, 7.5);
+    const int k = (int)__builtin_ceil(dd + dd);
+    const double2 c01 = *reinterpret_cast<const double2 *>(&lut[k].c[0]);
+    const double2 c23 = *reinterpret_cast<const double2 *>(&lut[k].c[2]);
+    const double r = poly3(d, c01.x, c01.y, c23.x, c23.y) + lo;
+    return d < 7.5 ? r : hi;
+}
+
 __device__ __forceinline__ double flog(double v) { return log(v); }
 __device__ __forceinline__ float flog(float v) { return logf(v); }
 __device__ __forceinline__ double fexp(double v) { return exp(v); }

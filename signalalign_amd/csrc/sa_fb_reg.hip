@@ -253,9 +253,12 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
     __shared__ ECon<T, EK> bev[2][NW];
     __shared__ T red[NW];
     __shared__ T shbuf[NW > 1 ? 6 * W : 1];
+    __shared__ LutEntry<T> lut[16];
     /* the four-wavefront sweeps carry the longest dependency chains of a launch
        sequence: their instructions go first on shared SIMDs */
     if constexpr (NW > 1) __builtin_amdgcn_s_setprio(3);
+    lut_fill(lut, (int)threadIdx.x);
+    __syncthreads();
     const int ti = order[blockIdx.x];
     const TaskDev t = a.tasks[ti];
     const Model<T> &m = a.m;
@@ -278,7 +281,7 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
     const T tMC = m.trans[0], tMFX = m.trans[1], tMFY = m.trans[2], tGOX = m.trans[3], tGOY = m.trans[4];
     const T tGEX = m.trans[5], tGEY = m.trans[6], tSWX = m.trans[7];
     const T NI = ninf<T>();
-    const bool useSWX = tSWX > NI;
+    const bool useSWX = m.use_swx != 0;
 
     /* band bounds and keep-flags, 64 diagonals per load: lane j holds diagonal dc + j */
     int dc = 0;
@@ -441,13 +444,13 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
             em_both<T>(m, kc[c], ev[c], s, iv, eM, eY);
             /* lower -> gapX, middle -> match, upper -> gapY */
             T X = lM + (eX + tGOX);
-            X = logadd(X, lXv + (eX + tGEX));
-            if (useSWX) X = logadd(X, lYv + (eX + tSWX));
+            X = logadd_lut(X, lXv + (eX + tGEX), lut);
+            if (useSWX) X = logadd_lut(X, lYv + (eX + tSWX), lut);
             T M = mM + (eM + tMC);
-            M = logadd(M, mXv + (eM + tMFX));
-            M = logadd(M, mYv + (eM + tMFY));
+            M = logadd_lut(M, mXv + (eM + tMFX), lut);
+            M = logadd_lut(M, mYv + (eM + tMFY), lut);
             T Y = pM[c] + (eY + tGOY);
-            Y = logadd(Y, pY[c] + (eY + tGEY));
+            Y = logadd_lut(Y, pY[c] + (eY + tGEY), lut);
             nM[c] = valid ? M : NI;
             nX[c] = valid ? X : NI;
             nY[c] = valid ? Y : NI;
@@ -551,6 +554,9 @@ template <typename T, int CPT, int EK>
 __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32_t *__restrict__ order) {
     constexpr bool NORM = sizeof(T) == 4;
     constexpr int W = 64 * CPT;
+    __shared__ LutEntry<T> lut[16];
+    lut_fill(lut, (int)threadIdx.x);
+    __syncthreads();
     const int wi = order[blockIdx.x];
     const WinDev Wd = a.wins[wi];
     const TaskDev t = a.tasks[Wd.task];
@@ -572,7 +578,7 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
     const T tMC = m.trans[0], tMFX = m.trans[1], tMFY = m.trans[2], tGOX = m.trans[3], tGOY = m.trans[4];
     const T tGEX = m.trans[5], tGEY = m.trans[6], tSWX = m.trans[7];
     const T NI = ninf<T>();
-    const bool useSWX = tSWX > NI;
+    const bool useSWX = m.use_swx != 0;
     const T thr = (T)a.threshold;
     const int D = Wd.D, TF = Wd.TF, TB = Wd.T;
     const bool ragged_end = Wd.last && t.ragged_right;
@@ -705,13 +711,13 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
                 const T eY = em_gapy<T>(m, kc[c], ev[c], s, iv);
                 /* contributions arrive middle (e+2), then upper, then lower (e+1) */
                 T M = CmidM + (eM + tMC);
-                M = logadd(M, CupY + (eY + tGOY));
-                M = logadd(M, ClowX + (eX + tGOX));
+                M = logadd_lut(M, CupY + (eY + tGOY), lut);
+                M = logadd_lut(M, ClowX + (eX + tGOX), lut);
                 T X = CmidM + (eM + tMFX);
-                X = logadd(X, ClowX + (eX + tGEX));
+                X = logadd_lut(X, ClowX + (eX + tGEX), lut);
                 T Y = CmidM + (eM + tMFY);
-                Y = logadd(Y, CupY + (eY + tGEY));
-                if (useSWX) Y = logadd(Y, ClowX + (eX + tSWX));
+                Y = logadd_lut(Y, CupY + (eY + tGEY), lut);
+                if (useSWX) Y = logadd_lut(Y, ClowX + (eX + tSWX), lut);
                 nM[c] = valid ? M : NI;
                 nX[c] = valid ? X : NI;
                 nY[c] = valid ? Y : NI;
@@ -750,8 +756,8 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
                     if (x >= lo && x <= hi) {
                         const int i = x - lo;
                         T cell = fM[dof + i] + nM[c];
-                        cell = logadd(cell, fX[dxy + i] + nX[c]);
-                        cell = logadd(cell, fY[dxy + i] + nY[c]);
+                        cell = logadd_lut(cell, fX[dxy + i] + nX[c], lut);
+                        cell = logadd_lut(cell, fY[dxy + i] + nY[c], lut);
                         tA[c] = (double)cell + offe;
                     }
                 }
@@ -772,8 +778,8 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
                                 const int fi = x - 1 - lom;
                                 const T eM = em_match<T>(m, kc[c], ev[c], s, iv);
                                 Mv = fM[dm + fi] + (eM + tMC);
-                                Mv = logadd(Mv, fX[dxym + fi] + (eM + tMFX));
-                                Mv = logadd(Mv, fY[dxym + fi] + (eM + tMFY));
+                                Mv = logadd_lut(Mv, fX[dxym + fi] + (eM + tMFX), lut);
+                                Mv = logadd_lut(Mv, fY[dxym + fi] + (eM + tMFY), lut);
                             }
                             tB[c] = (double)(Mv + pM[c]) + offm;
                         }

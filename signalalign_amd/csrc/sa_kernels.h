@@ -28,6 +28,7 @@ struct Model {
     const T *gridx;
     const T *ys;
     int32_t grid_length;
+    int32_t use_swx;        /* GAP_SWITCH_TO_X above -inf: the gapY -> gapX term is summed */
     T inv_dx;
     T gx_const;             /* the skip emission when every k-mer has the same one */
     T trans[9];

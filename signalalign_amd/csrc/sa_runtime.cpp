@@ -247,6 +247,7 @@ sa::Model<T> device_model(const sa_model *m, const DevTables<T> &t) {
     r.gridx = (const T *)t.gridx.p;
     r.ys = (const T *)t.ys.p;
     r.grid_length = m->grid_length;
+    r.use_swx = m->trans[7] > -HUGE_VAL ? 1 : 0;
     r.inv_dx = (T)m->inv_dx;
     r.gx_const = (T)m->gx_const;
     for (int i = 0; i < 9; i++) r.trans[i] = (T)m->trans[i];
