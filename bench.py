@@ -289,6 +289,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fp64-check", action="store_true", help="skip the FP32-vs-FP64 per-read error report")
     ap.add_argument("--no-hdp", action="store_true", help="skip the HDP-emission configuration")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch tuning")
+    ap.add_argument("--no-graph", action="store_true", help="timed steps without graph replay")
     ap.add_argument("--hdp-reads", type=int, default=2048)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving configuration")
     ap.add_argument("--serve-requests", type=int, default=100000)
@@ -325,6 +327,15 @@ def main():
     for i in range(args.warmup):
         _, tm = batch.run(prec, fetch=False)
         log(f"warmup {i}: device {tm['total_ms']:.1f} ms")
+    tuning = None
+    if not args.no_autotune:
+        tuning, tuned_ms = batch.autotune(prec)
+        log(f"autotune: {tuning} ({tuned_ms:.1f} ms)")
+    if args.no_graph:
+        batch.set_tuning(use_graph=0)
+    for i in range(2):  # the second run records the graph the timed steps replay
+        _, tm = batch.run(prec, fetch=False)
+        log(f"prime {i}: device {tm['total_ms']:.1f} ms")
     dev_ms, wall_ms, fwd_ms, bwd_ms, tms = [], [], [], [], []
     cells = 0
     for i in range(args.steps):
@@ -336,7 +347,8 @@ def main():
         bwd_ms.append(tm["backward_ms"])
         tms.append(tm)
         cells = tm["cells"]
-        log(f"step {i}: device {tm['total_ms']:.1f} ms (prep {tm['prep_ms']:.1f} fwd {tm['forward_ms']:.1f} "
+        log(f"step {i}: device {tm['total_ms']:.1f} ms{' (graph replay)' if tm['replayed'] else ''} "
+            f"(prep {tm['prep_ms']:.1f} fwd {tm['forward_ms']:.1f} "
             f"bwd {tm['backward_ms']:.1f} collect {tm['collect_ms']:.1f}), wall {wall_ms[-1]:.1f} ms, "
             f"{cells / tm['total_ms'] / 1e6:.1f} GCUPS")
     n_pairs = sum(len(r.pairs) for r in res)
@@ -426,7 +438,8 @@ def main():
         "config": {"workload": f"batched: {args.reads} reads, log-normal lengths median {args.median} nt "
                                "(sigma 0.5, 1-50 kb), ACEGT 6-mer Gaussian table (15625 k-mers), band expansion "
                                f"{args.expansion}, {args.precision} recurrence with FP64 totals",
-                   "reads": args.reads, "cells": cells, "pairs": n_pairs, "precision": args.precision},
+                   "reads": args.reads, "cells": cells, "pairs": n_pairs, "precision": args.precision,
+                   "launch_tuning": tuning, "graph_replayed_steps": int(sum(t["replayed"] for t in tms))},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "fp32_error_vs_fp64": fp_err,

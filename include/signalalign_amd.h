@@ -134,6 +134,7 @@ typedef struct {
     int64_t backward_launches;
     double fb_ms;            /* forward + backward sweeps on the device timeline (they overlap) */
     int32_t emission_path;   /* 0: standard Gaussian tables, 1: general Gaussian, 2: HDP grid */
+    int32_t replayed;        /* 1: the run replayed the batch's recorded graph */
 } sa_timing;
 
 /* One kernel launch of a run: forward sweeps of a group of sub-alignments, or
@@ -185,6 +186,27 @@ int sa_batch_run(sa_batch *batch, sa_precision precision, sa_result **out);
 int sa_batch_refill(sa_batch *batch, const sa_model *model, const sa_read *reads, int64_t n_reads,
                     const sa_params *params);
 void sa_batch_destroy(sa_batch *batch);
+
+/* Launch tuning of a batch.  use_graph: the first run records the launch sequence
+ * (band prep, every forward and backward sweep on its stream, pair compaction) as a
+ * HIP graph and later runs of the same batch replay it.  waves_per_simd: occupancy
+ * cap of the one-wavefront sweeps (0: none).  long_factor: sub-alignments with
+ * lX + lY >= max(20000, long_factor x median) run four wavefronts per forward sweep
+ * (0: 2.5; negative: never).  stream_split: fraction of the one-wavefront forward
+ * cells (longest sub-alignments first) on stream 0, the rest on stream 3 (0: 0.5). */
+typedef struct {
+    int32_t use_graph;
+    int32_t waves_per_simd;
+    double long_factor;
+    double stream_split;
+} sa_tuning;
+void sa_tuning_default(sa_tuning *t);
+int sa_batch_set_tuning(sa_batch *batch, const sa_tuning *t);
+int sa_batch_get_tuning(const sa_batch *batch, sa_tuning *t);
+/* Times runs of the batch under candidate tunings (one knob at a time: long_factor,
+ * stream_split, waves_per_simd), keeps the fastest on the batch and returns it with
+ * its device time.  use_graph is left as set. */
+int sa_batch_autotune(sa_batch *batch, sa_precision precision, sa_tuning *best, double *best_ms);
 
 int64_t sa_result_n_reads(const sa_result *res);
 int sa_result_read(const sa_result *res, int64_t i, sa_read_result *out);

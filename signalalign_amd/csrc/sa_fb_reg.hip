@@ -198,6 +198,13 @@ template <typename T> __device__ __forceinline__ T ld_scalar(const T *p, int i) 
     typedef __attribute__((address_space(4))) const T CT;
     return ((const CT *)p)[__builtin_amdgcn_readfirstlane(i)];
 }
+/* byte i of p through the scalar cache: the aligned dword holding it (flag arrays
+ * are allocated with 4 bytes of padding) */
+__device__ __forceinline__ int ld_scalar_byte(const int8_t *p, int i) {
+    const uintptr_t ad = (uintptr_t)p + (uintptr_t)__builtin_amdgcn_readfirstlane(i);
+    const int32_t w = ld_scalar((const int32_t *)(ad & ~(uintptr_t)3), 0);
+    return (w >> (int)((ad & 3) * 8)) & 0xff;
+}
 template <typename T>
 __device__ __forceinline__ void load_econ_uniform(ECon<T, 2> &e, const Args<T> &a, int64_t base, int j, bool ok) {
     e.m = ok ? ld_scalar(a.evm + base, j) : (T)0;
@@ -283,15 +290,10 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
     const T NI = ninf<T>();
     const bool useSWX = m.use_swx != 0;
 
-    /* band bounds and keep-flags, 64 diagonals per load: lane j holds diagonal dc + j */
+    /* band bounds and keep-flag of each diagonal come through the scalar cache, one
+       diagonal ahead; the per-diagonal plane offsets are written 64 diagonals at a
+       time (lane j of wave 0 holds diagonal dc + j) */
     int dc = 0;
-    int cl, ch, cf, nl, nh, nf;
-    {
-        int d = lane;
-        cl = d <= N ? xlo[d] : 0; ch = d <= N ? xhi[d] : -1; cf = d <= N ? dfl[d] : 0;
-        d = 64 + lane;
-        nl = d <= N ? xlo[d] : 0; nh = d <= N ? xhi[d] : -1; nf = d <= N ? dfl[d] : 0;
-    }
     int md_doff = 0, md_dxy = -1;
     double md_foff = 0.0;
 
@@ -302,7 +304,9 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
     T LpM = NI, LpX = NI, LpY = NI;
 
     /* diagonal 0: start or ragged-start values (impl/stateMachine.c:1022-1031) */
-    const int lo0 = rdl(cl, 0), hi0 = rdl(ch, 0), f0 = rdl(cf, 0);
+    const int lo0 = ld_scalar(xlo, 0), hi0 = ld_scalar(xhi, 0), f0 = ld_scalar_byte(dfl, 0);
+    int nlo = 0, nhi = -1, nfl = 0;
+    if (N >= 1) { nlo = ld_scalar(xlo, 1); nhi = ld_scalar(xhi, 1); nfl = ld_scalar_byte(dfl, 1); }
 #pragma unroll
     for (int c = 0; c < CPT; c++) {
         const int x = CPT * gl + c;
@@ -340,11 +344,9 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
             }
             dc += 64;
             j = 0;
-            cl = nl; ch = nh; cf = nf;
-            const int dn = dc + 64 + lane;
-            nl = dn <= N ? xlo[dn] : 0; nh = dn <= N ? xhi[dn] : -1; nf = dn <= N ? dfl[dn] : 0;
         }
-        const int lo = rdl(cl, j), hi = rdl(ch, j), fl = rdl(cf, j);
+        const int lo = nlo, hi = nhi, fl = nfl;
+        if (d < N) { nlo = ld_scalar(xlo, d + 1); nhi = ld_scalar(xhi, d + 1); nfl = ld_scalar_byte(dfl, d + 1); }
         if (lo < X0) { /* band bounds are non-decreasing; anything else goes to the LDS kernels */
             if (gl == 0) atomicOr(a.err, ERR_WINDOWS);
             return;
@@ -590,22 +592,11 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
     int posterior_count = 0;
     double total = -__builtin_huge_val();
 
-    /* band bounds, offsets: lane j holds diagonal dc - j (descending) */
-    int dc = D;
-    int cl, ch, cdo;
-    double cfo;
-    int nl, nh, ndo;
-    double nfo;
-    {
-        int e = D - lane;
-        bool ok = e > TB;
-        cl = ok ? xlo[e] : 0; ch = ok ? xhi[e] : -1; cdo = ok ? doff[e] : 0; cfo = ok ? foff[e] : 0.0;
-        e = D - 64 - lane;
-        ok = e > TB;
-        nl = ok ? xlo[e] : 0; nh = ok ? xhi[e] : -1; ndo = ok ? doff[e] : 0; nfo = ok ? foff[e] : 0.0;
-    }
-
-    const int loD = rdl(cl, 0), hiD = rdl(ch, 0);
+    /* band bounds and plane offsets of each diagonal come through the scalar cache,
+       one diagonal ahead */
+    int nlo = ld_scalar(xlo, D), nhi = ld_scalar(xhi, D), ndo = ld_scalar(doff, D);
+    double nfo = ld_scalar(foff, D);
+    const int loD = nlo, hiD = nhi;
     int X0 = hiD - W + 1;
     KCon<T, EK> kc[CPT];
     ECon<T, EK> ev[CPT];
@@ -630,17 +621,12 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
     bool fresh = true;
 
     for (int e = D; e > TB; e--) {
-        int j = dc - e;
-        if (j == 64) {
-            dc -= 64;
-            j = 0;
-            cl = nl; ch = nh; cdo = ndo; cfo = nfo;
-            const int en = dc - 64 - lane;
-            const bool ok = en > TB;
-            nl = ok ? xlo[en] : 0; nh = ok ? xhi[en] : -1; ndo = ok ? doff[en] : 0; nfo = ok ? foff[en] : 0.0;
+        const int lo = nlo, hi = nhi, dof = ndo;
+        const double fo = nfo;
+        if (e - 1 > TB) {
+            nlo = ld_scalar(xlo, e - 1); nhi = ld_scalar(xhi, e - 1); ndo = ld_scalar(doff, e - 1);
+            nfo = ld_scalar(foff, e - 1);
         }
-        const int lo = rdl(cl, j), hi = rdl(ch, j), dof = rdl(cdo, j);
-        const double fo = __shfl(cfo, j, 64);
         T nM[CPT], nX[CPT], nY[CPT];
         T LowM = NI, LowX = NI; /* lane i+1's first cell of e+1 */
         double obe = 0.0;
@@ -855,72 +841,77 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
 /* ------------------------------------------------------------------------ */
 
 template <typename T, int CPT, int EK>
-static hipError_t fwd_reg_inst(const Args<T> &a, const int32_t *order, int n, int nw, hipStream_t st) {
+static hipError_t fwd_reg_inst(const Args<T> &a, const int32_t *order, int n, int nw, size_t shmem, hipStream_t st) {
     if constexpr (CPT <= 2) {
         if (nw == 4) {
-            hipLaunchKernelGGL((forward_reg_kernel<T, CPT, EK, 4>), dim3(n), dim3(256), 0, st, a, order);
+            hipLaunchKernelGGL((forward_reg_kernel<T, CPT, EK, 4>), dim3(n), dim3(256), shmem, st, a, order);
             return hipGetLastError();
         }
     }
-    hipLaunchKernelGGL((forward_reg_kernel<T, CPT, EK, 1>), dim3(n), dim3(64), 0, st, a, order);
+    hipLaunchKernelGGL((forward_reg_kernel<T, CPT, EK, 1>), dim3(n), dim3(64), shmem, st, a, order);
     return hipGetLastError();
 }
 
 template <typename T, int CPT>
-static hipError_t fwd_reg_ek(const Args<T> &a, const int32_t *order, int n, int nw, int ek, hipStream_t st) {
+static hipError_t fwd_reg_ek(const Args<T> &a, const int32_t *order, int n, int nw, int ek, size_t shmem,
+                             hipStream_t st) {
     switch (ek) {
-    case 0: return fwd_reg_inst<T, CPT, 0>(a, order, n, nw, st);
-    case 1: return fwd_reg_inst<T, CPT, 1>(a, order, n, nw, st);
-    default: return fwd_reg_inst<T, CPT, 2>(a, order, n, nw, st);
+    case 0: return fwd_reg_inst<T, CPT, 0>(a, order, n, nw, shmem, st);
+    case 1: return fwd_reg_inst<T, CPT, 1>(a, order, n, nw, shmem, st);
+    default: return fwd_reg_inst<T, CPT, 2>(a, order, n, nw, shmem, st);
     }
 }
 
 template <typename T, int CPT>
-static hipError_t bwd_reg_ek(const Args<T> &a, const int32_t *order, int n, int ek, hipStream_t st) {
+static hipError_t bwd_reg_ek(const Args<T> &a, const int32_t *order, int n, int ek, size_t shmem,
+                             hipStream_t st) {
     switch (ek) {
-    case 0: hipLaunchKernelGGL((backward_reg_kernel<T, CPT, 0>), dim3(n), dim3(64), 0, st, a, order); break;
-    case 1: hipLaunchKernelGGL((backward_reg_kernel<T, CPT, 1>), dim3(n), dim3(64), 0, st, a, order); break;
-    default: hipLaunchKernelGGL((backward_reg_kernel<T, CPT, 2>), dim3(n), dim3(64), 0, st, a, order); break;
+    case 0: hipLaunchKernelGGL((backward_reg_kernel<T, CPT, 0>), dim3(n), dim3(64), shmem, st, a, order); break;
+    case 1: hipLaunchKernelGGL((backward_reg_kernel<T, CPT, 1>), dim3(n), dim3(64), shmem, st, a, order); break;
+    default: hipLaunchKernelGGL((backward_reg_kernel<T, CPT, 2>), dim3(n), dim3(64), shmem, st, a, order); break;
     }
     return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_forward_reg(const Args<T> &a, const int32_t *order, int n, int cpt, int nw, int ek,
-                              hipStream_t st) {
+                              size_t shmem, hipStream_t st) {
     if (n == 0) return hipSuccess;
     if (nw != 1 && nw != 4) return hipErrorInvalidValue;
     if (nw == 4 && cpt > 2) return hipErrorInvalidValue;
     switch (cpt) {
-    case 1: return fwd_reg_ek<T, 1>(a, order, n, nw, ek, st);
-    case 2: return fwd_reg_ek<T, 2>(a, order, n, nw, ek, st);
-    case 3: return fwd_reg_ek<T, 3>(a, order, n, 1, ek, st);
-    case 4: return fwd_reg_ek<T, 4>(a, order, n, 1, ek, st);
-    case 6: return fwd_reg_ek<T, 6>(a, order, n, 1, ek, st);
-    case 8: return fwd_reg_ek<T, 8>(a, order, n, 1, ek, st);
+    case 1: return fwd_reg_ek<T, 1>(a, order, n, nw, ek, shmem, st);
+    case 2: return fwd_reg_ek<T, 2>(a, order, n, nw, ek, shmem, st);
+    case 3: return fwd_reg_ek<T, 3>(a, order, n, 1, ek, shmem, st);
+    case 4: return fwd_reg_ek<T, 4>(a, order, n, 1, ek, shmem, st);
+    case 6: return fwd_reg_ek<T, 6>(a, order, n, 1, ek, shmem, st);
+    case 8: return fwd_reg_ek<T, 8>(a, order, n, 1, ek, shmem, st);
     default: return hipErrorInvalidValue;
     }
 }
 
 template <typename T>
-hipError_t launch_backward_reg(const Args<T> &a, const int32_t *order, int n, int cpt, int ek, hipStream_t st) {
+hipError_t launch_backward_reg(const Args<T> &a, const int32_t *order, int n, int cpt, int ek, size_t shmem,
+                               hipStream_t st) {
     if (n == 0) return hipSuccess;
     switch (cpt) {
-    case 1: return bwd_reg_ek<T, 1>(a, order, n, ek, st);
-    case 2: return bwd_reg_ek<T, 2>(a, order, n, ek, st);
-    case 3: return bwd_reg_ek<T, 3>(a, order, n, ek, st);
-    case 4: return bwd_reg_ek<T, 4>(a, order, n, ek, st);
-    case 6: return bwd_reg_ek<T, 6>(a, order, n, ek, st);
-    case 8: return bwd_reg_ek<T, 8>(a, order, n, ek, st);
+    case 1: return bwd_reg_ek<T, 1>(a, order, n, ek, shmem, st);
+    case 2: return bwd_reg_ek<T, 2>(a, order, n, ek, shmem, st);
+    case 3: return bwd_reg_ek<T, 3>(a, order, n, ek, shmem, st);
+    case 4: return bwd_reg_ek<T, 4>(a, order, n, ek, shmem, st);
+    case 6: return bwd_reg_ek<T, 6>(a, order, n, ek, shmem, st);
+    case 8: return bwd_reg_ek<T, 8>(a, order, n, ek, shmem, st);
     default: return hipErrorInvalidValue;
     }
 }
 
-template hipError_t launch_forward_reg<float>(const Args<float> &, const int32_t *, int, int, int, int,
+template hipError_t launch_forward_reg<float>(const Args<float> &, const int32_t *, int, int, int, int, size_t,
                                               hipStream_t);
-template hipError_t launch_forward_reg<double>(const Args<double> &, const int32_t *, int, int, int, int,
+template hipError_t launch_forward_reg<double>(const Args<double> &, const int32_t *, int, int, int, int, size_t,
                                                hipStream_t);
-template hipError_t launch_backward_reg<float>(const Args<float> &, const int32_t *, int, int, int, hipStream_t);
-template hipError_t launch_backward_reg<double>(const Args<double> &, const int32_t *, int, int, int, hipStream_t);
+template hipError_t launch_backward_reg<float>(const Args<float> &, const int32_t *, int, int, int, size_t,
+                                              hipStream_t);
+template hipError_t launch_backward_reg<double>(const Args<double> &, const int32_t *, int, int, int, size_t,
+                                               hipStream_t);
 
 } // namespace sa

@@ -80,10 +80,13 @@ template <typename T> hipError_t launch_prep(const Args<T> &a, int n_tasks, hipS
 template <typename T> hipError_t launch_forward(const Args<T> &a, int n_tasks, int nt, int cpt, hipStream_t st);
 template <typename T> hipError_t launch_backward(const Args<T> &a, int n_wins, int cpt, hipStream_t st);
 template <typename T>
-hipError_t launch_forward_reg(const Args<T> &a, const int32_t *order, int n, int cpt, int nw, int ek,
+hipError_t launch_forward_reg(const Args<T> &a, const int32_t *order, int n, int cpt, int nw, int ek, size_t shmem,
                               hipStream_t st);
 template <typename T>
-hipError_t launch_backward_reg(const Args<T> &a, const int32_t *order, int n, int cpt, int ek, hipStream_t st);
+hipError_t launch_backward_reg(const Args<T> &a, const int32_t *order, int n, int cpt, int ek, size_t shmem,
+                               hipStream_t st);
+/* zeroes n ints (a kernel, so that it can sit in a captured graph) */
+hipError_t launch_zero(int32_t *p, int n, hipStream_t st);
 hipError_t launch_collect(const WinDev *wins, const int32_t *win_count, const int64_t *win_dst, const int32_t *src,
                           int32_t *dst, const int32_t *task_x1, const int32_t *task_y1, int n_wins, hipStream_t st);
 

@@ -589,6 +589,17 @@ hipError_t launch_backward(const Args<T> &a, int n_wins, int cpt, hipStream_t st
     }
 }
 
+__global__ void zero_kernel(int32_t *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+hipError_t launch_zero(int32_t *p, int n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_collect(const WinDev *wins, const int32_t *win_count, const int64_t *win_dst, const int32_t *src,
                           int32_t *dst, const int32_t *task_x1, const int32_t *task_y1, int n_wins, hipStream_t st) {
     if (n_wins == 0) return hipSuccess;

@@ -67,7 +67,7 @@ struct sa_context {
     int n_streams = 4;         /* 1: everything on `stream` (aux[] alias it) */
     std::mutex mu;
     std::string err;
-    hipEvent_t ev[12] = {};
+    hipEvent_t ev[16] = {};
     hipEvent_t gev[160] = {};  /* start/end of each forward and backward launch */
     int n_cu = 0;
 };
@@ -89,6 +89,12 @@ struct sa_model {
     DBuf d_lut;
     DevTables<double> t64;
     DevTables<float> t32;
+};
+
+struct ReadOut {
+    int64_t first_pair = 0, n_pairs = 0;
+    double log_likelihood = 0.0;
+    int64_t cells = 0, n_windows = 0, n_sub = 0;
 };
 
 struct ReadPlan {
@@ -119,7 +125,26 @@ struct sa_batch {
     DBuf d_task_cells, d_task_xycells, d_task_maxw, d_task_span, d_task_nwin, d_win_slots, d_err;
     DBuf d_fM, d_fX, d_fY;
     DBuf d_wins, d_task_order, d_win_order, d_out, d_win_count, d_win_total, d_win_dst, d_packed;
+    sa_tuning tune{1, 0, 2.5, 0.5};
+    int32_t nruns[2] = {0, 0}; /* runs since the batch or its tuning last changed, per precision */
+    /* a recorded run per precision (0 FP64, 1 FP32), replayed while the batch is unchanged */
+    struct GraphCache {
+        hipGraphExec_t exec = nullptr;
+        std::vector<ReadOut> reads;
+        std::vector<int32_t> last_win;  /* per sub-alignment: its final window, or -1 */
+        std::vector<sa_launch> launches; /* shapes; durations filled from the replay's events */
+        int64_t nw = 0, total_pairs = 0, cells = 0;
+        int32_t ek = 0;
+        void reset() {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            exec = nullptr;
+            reads.clear();
+            last_win.clear();
+            launches.clear();
+        }
+    } gc[2];
     ~sa_batch() {
+        for (auto &g : gc) g.reset();
         DBuf *all[] = {&d_ref, &d_events, &d_ancx, &d_ancy, &d_tasks, &d_x1, &d_y1, &d_kid, &d_evm, &d_evn,
                        &d_evl, &d_evi, &d_xlo, &d_xhi, &d_doff, &d_foff, &d_dflag, &d_doffxy, &d_task_cells,
                        &d_task_xycells, &d_task_maxw, &d_task_span,
@@ -129,11 +154,6 @@ struct sa_batch {
     }
 };
 
-struct ReadOut {
-    int64_t first_pair = 0, n_pairs = 0;
-    double log_likelihood = 0.0;
-    int64_t cells = 0, n_windows = 0, n_sub = 0;
-};
 
 struct sa_result {
     std::vector<sa_pair> pairs;
@@ -470,6 +490,8 @@ int batch_fill(sa_context *ctx, sa_batch *b, const sa_model *model, const sa_rea
         !(p.min_diags_between_traceback >= 2) || !(p.traceback_diagonals + 1 < p.min_diags_between_traceback) ||
         !(p.threshold >= 0.0 && p.threshold <= 1.0))
         return fail(ctx, SA_ERR_INVALID_ARGUMENT, "invalid banding parameters");
+    for (auto &g : b->gc) g.reset();
+    b->nruns[0] = b->nruns[1] = 0;
     b->ctx = ctx;
     b->model = model;
     b->params = p;
@@ -624,6 +646,73 @@ int sa_batch_refill(sa_batch *b, const sa_model *model, const sa_read *reads, in
     return rc;
 }
 
+void sa_tuning_default(sa_tuning *t) {
+    t->use_graph = 1;
+    t->waves_per_simd = 0;
+    t->long_factor = 2.5;
+    t->stream_split = 0.5;
+}
+
+int sa_batch_set_tuning(sa_batch *b, const sa_tuning *t) {
+    if (!b || !t || t->waves_per_simd < 0 || t->waves_per_simd > 8 || !(t->stream_split >= 0.0) ||
+        !(t->stream_split < 1.0))
+        return SA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lk(b->ctx->mu);
+    (void)hipSetDevice(b->ctx->device);
+    for (auto &g : b->gc) g.reset();
+    b->nruns[0] = b->nruns[1] = 0;
+    b->tune = *t;
+    return SA_OK;
+}
+
+int sa_batch_get_tuning(const sa_batch *b, sa_tuning *t) {
+    if (!b || !t) return SA_ERR_INVALID_ARGUMENT;
+    *t = b->tune;
+    return SA_OK;
+}
+
+int sa_batch_autotune(sa_batch *b, sa_precision precision, sa_tuning *best, double *best_ms) {
+    if (!b) return SA_ERR_INVALID_ARGUMENT;
+    sa_tuning cur = b->tune;
+    const int32_t keep_graph = cur.use_graph;
+    cur.use_graph = 0;
+    auto time_it = [&](const sa_tuning &t, double *ms) -> int {
+        int rc = sa_batch_set_tuning(b, &t);
+        if (rc) return rc;
+        sa_result *r = nullptr;
+        if ((rc = sa_batch_run(b, precision, &r))) return rc;
+        *ms = r->timing.total_ms;
+        sa_result_free(r);
+        return SA_OK;
+    };
+    double cur_ms = 0.0;
+    int rc = time_it(cur, &cur_ms);
+    if (rc) return rc;
+    const double lf[] = {2.5, 1.5, 4.0, -1.0};
+    const double sp[] = {0.5, 0.35, 0.65};
+    const int wp[] = {0, 2, 3, 4};
+    for (int knob = 0; knob < 3; knob++) {
+        const int n = knob == 0 ? 4 : knob == 1 ? 3 : 4;
+        for (int i = 0; i < n; i++) {
+            sa_tuning t = cur;
+            if (knob == 0) t.long_factor = lf[i];
+            else if (knob == 1) t.stream_split = sp[i];
+            else t.waves_per_simd = wp[i];
+            if (t.long_factor == cur.long_factor && t.stream_split == cur.stream_split &&
+                t.waves_per_simd == cur.waves_per_simd)
+                continue;
+            double ms = 0.0;
+            if ((rc = time_it(t, &ms))) return rc;
+            if (ms < cur_ms) { cur = t; cur_ms = ms; }
+        }
+    }
+    cur.use_graph = keep_graph;
+    if ((rc = sa_batch_set_tuning(b, &cur))) return rc;
+    if (best) *best = cur;
+    if (best_ms) *best_ms = cur_ms;
+    return SA_OK;
+}
+
 void sa_batch_destroy(sa_batch *b) {
     if (!b) return;
     (void)hipSetDevice(b->ctx->device);
@@ -665,7 +754,7 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     SA_HIP(ctx, b->d_xhi.ensure(sizeof(int32_t) * b->total_diag));
     SA_HIP(ctx, b->d_doff.ensure(sizeof(int32_t) * b->total_diag));
     SA_HIP(ctx, b->d_foff.ensure(sizeof(double) * b->total_diag));
-    SA_HIP(ctx, b->d_dflag.ensure(sizeof(int8_t) * b->total_diag));
+    SA_HIP(ctx, b->d_dflag.ensure(sizeof(int8_t) * b->total_diag + 4)); /* + padding for dword reads */
     SA_HIP(ctx, b->d_doffxy.ensure(sizeof(int32_t) * b->total_diag));
     SA_HIP(ctx, b->d_task_cells.ensure(sizeof(int64_t) * nt));
     SA_HIP(ctx, b->d_task_xycells.ensure(sizeof(int64_t) * nt));
@@ -773,7 +862,10 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
         std::vector<int64_t> srt(nd);
         std::sort(srt.begin(), srt.end());
         const int64_t med = srt[srt.size() / 2];
-        long_n = nt <= 2 * (int64_t)std::max(ctx->n_cu, 1) ? 0 : std::max<int64_t>(20000, (5 * med) / 2);
+        const double lf = b->tune.long_factor == 0.0 ? 2.5 : b->tune.long_factor;
+        long_n = nt <= 2 * (int64_t)std::max(ctx->n_cu, 1) ? 0
+               : lf < 0.0 ? INT64_MAX
+                          : std::max<int64_t>(20000, (int64_t)(lf * (double)med));
         const char *fw = getenv("SA_FWD_WAVES"); /* "1" or "4": one forward shape for every sub-alignment */
         if (fw && std::strcmp(fw, "1") == 0) long_n = INT64_MAX;
         if (fw && std::strcmp(fw, "4") == 0) long_n = 0;
@@ -811,8 +903,9 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
         }
         std::stable_sort(nw1.begin(), nw1.end(), [&](int32_t x, int32_t y) { return cells[x] > cells[y]; });
         int64_t acc = 0;
+        const double split = b->tune.stream_split > 0.0 && b->tune.stream_split < 1.0 ? b->tune.stream_split : 0.5;
         for (int32_t i : nw1) {
-            if (2 * acc >= nw1_cells) tlane[i] = 3;
+            if ((double)acc >= split * (double)nw1_cells) tlane[i] = 3;
             acc += cells[i];
         }
     }
@@ -885,14 +978,19 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
         return fail(ctx, SA_ERR_HIP, "too many launches for the timing events");
 
     hipStream_t gst[kLanes] = {st, ctx->aux[0], ctx->aux[1], ctx->aux[2]};
+    /* occupancy cap: dynamic LDS per one-wavefront workgroup so that at most
+       waves_per_simd of them share a SIMD (160 KiB of LDS per CU, four SIMDs) */
+    const int wps = b->tune.waves_per_simd;
+    const size_t shmem1 = wps > 0 && wps < 8 ? (size_t)std::max(0, 163840 / (4 * wps) - 4096) : 0;
     auto launch_fwd = [&](const Launch &L, hipStream_t sg) -> hipError_t {
         if (L.group == 0) {
             sa::Args<T> al = a;
             al.task_order = d_to + L.off;
             return sa::launch_forward<T>(al, (int)L.n, lds_fnt, lds_fcpt, sg);
         }
-        if (L.group < 7) return sa::launch_forward_reg<T>(a, d_to + L.off, (int)L.n, kCpt[L.group - 1], 1, ek, sg);
-        return sa::launch_forward_reg<T>(a, d_to + L.off, (int)L.n, kCpt[L.group - 7], 4, ek, sg);
+        if (L.group < 7)
+            return sa::launch_forward_reg<T>(a, d_to + L.off, (int)L.n, kCpt[L.group - 1], 1, ek, shmem1, sg);
+        return sa::launch_forward_reg<T>(a, d_to + L.off, (int)L.n, kCpt[L.group - 7], 4, ek, 0, sg);
     };
     auto launch_bwd = [&](const Launch &L, hipStream_t sg) -> hipError_t {
         if (L.group == 0) {
@@ -900,7 +998,7 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
             al.win_order = d_wo + L.off;
             return sa::launch_backward<T>(al, (int)L.n, lds_bcpt, sg);
         }
-        return sa::launch_backward_reg<T>(a, d_wo + L.off, (int)L.n, kCpt[L.group - 1], ek, sg);
+        return sa::launch_backward_reg<T>(a, d_wo + L.off, (int)L.n, kCpt[L.group - 1], ek, shmem1, sg);
     };
     /* fork on ev[2], per-launch events gev[2k], gev[2k+1] (forward launches first),
        join on ev[4] */
@@ -1039,6 +1137,112 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     (void)hipEventElapsedTime(&ms, ctx->ev[5], ctx->ev[6]); tm.collect_ms = ms;
     (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[6]); tm.total_ms = ms;
     tm.cells = total_cells;
+
+    /* record the launch sequence of this (now planned) batch as a graph: later runs
+       replay it without the host round trips between the steps */
+    auto &gc = b->gc[sizeof(T) == 4 ? 1 : 0];
+    gc.reset();
+    /* only a batch that is run again is worth recording (sa_align runs once) */
+    if (b->tune.use_graph && !force_lds && redo.empty() && b->nruns[sizeof(T) == 4 ? 1 : 0] >= 1) {
+        hipGraph_t g = nullptr;
+        bool ok = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) == hipSuccess;
+        if (ok) {
+            const bool enq =
+                sa::launch_zero(b->d_err.as<int32_t>(), 1, st) == hipSuccess &&
+                hipEventRecord(ctx->ev[0], st) == hipSuccess && sa::launch_prep<T>(a, (int)nt, st) == hipSuccess &&
+                hipEventRecord(ctx->ev[1], st) == hipSuccess && run_sweeps(true) == SA_OK &&
+                hipEventRecord(ctx->ev[5], st) == hipSuccess &&
+                sa::launch_collect(b->d_wins.as<WinDev>(), b->d_win_count.as<int32_t>(), b->d_win_dst.as<int64_t>(),
+                                   b->d_out.as<int32_t>(), b->d_packed.as<int32_t>(), b->d_x1.as<int32_t>(),
+                                   b->d_y1.as<int32_t>(), (int)nw, st) == hipSuccess &&
+                hipEventRecord(ctx->ev[6], st) == hipSuccess;
+            const hipError_t ec = hipStreamEndCapture(st, &g);
+            ok = enq && ec == hipSuccess && g != nullptr &&
+                 hipGraphInstantiate(&gc.exec, g, nullptr, nullptr, 0) == hipSuccess;
+            if (g) (void)hipGraphDestroy(g);
+        }
+        if (ok) {
+            gc.reads = res->reads;
+            gc.last_win.assign(nt, -1);
+            for (int64_t ti = 0; ti < nt; ti++)
+                if (nwin[ti] > 0) gc.last_win[ti] = task_first_win[ti] + nwin[ti] - 1;
+            gc.launches = res->launches;
+            gc.nw = nw;
+            gc.total_pairs = total_pairs;
+            gc.cells = total_cells;
+            gc.ek = model->ek;
+        } else {
+            gc.reset();
+            (void)hipGetLastError();
+            ctx->err.clear();
+        }
+    }
+    return SA_OK;
+}
+
+/* A run from the recorded graph: same launches, same buffers, same results. */
+template <typename T>
+int replay_batch(sa_batch *b, sa_result *res) {
+    sa_context *ctx = b->ctx;
+    hipStream_t st = ctx->stream;
+    auto &gc = b->gc[sizeof(T) == 4 ? 1 : 0];
+    sa_timing &tm = res->timing;
+    std::memset(&tm, 0, sizeof(tm));
+    SA_HIP(ctx, hipEventRecord(ctx->ev[12], st));
+    SA_HIP(ctx, hipGraphLaunch(gc.exec, st));
+    SA_HIP(ctx, hipEventRecord(ctx->ev[13], st));
+    res->reads = gc.reads;
+    res->pairs.resize(gc.total_pairs);
+    std::vector<double> wtot(gc.nw);
+    int32_t err = 0;
+    if (gc.total_pairs)
+        SA_HIP(ctx, hipMemcpyAsync(res->pairs.data(), b->d_packed.p, sizeof(int32_t) * 3 * gc.total_pairs,
+                                   hipMemcpyDeviceToHost, st));
+    SA_HIP(ctx, hipMemcpyAsync(wtot.data(), b->d_win_total.p, sizeof(double) * gc.nw, hipMemcpyDeviceToHost, st));
+    SA_HIP(ctx, hipMemcpyAsync(&err, b->d_err.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    SA_HIP(ctx, hipStreamSynchronize(st));
+    if (err) return fail(ctx, SA_ERR_HIP, "replayed run raised device error flags " + std::to_string(err));
+    for (int64_t r = 0; r < b->n_reads; r++) {
+        double ll = 0.0;
+        for (int64_t ti = b->reads[r].first_task; ti < b->reads[r].first_task + b->reads[r].n_tasks; ti++)
+            if (gc.last_win[ti] >= 0) ll += wtot[gc.last_win[ti]];
+        res->reads[r].log_likelihood = ll;
+    }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev[12], ctx->ev[13]); tm.total_ms = ms;
+    /* per-step times from the events the graph recorded (left 0 if unavailable) */
+    if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) == hipSuccess) tm.prep_ms = ms;
+    if (hipEventElapsedTime(&ms, ctx->ev[5], ctx->ev[6]) == hipSuccess) tm.collect_ms = ms;
+    float fb = 0.f;
+    const bool have_fb = hipEventElapsedTime(&fb, ctx->ev[2], ctx->ev[4]) == hipSuccess;
+    if (have_fb) tm.fb_ms = fb;
+    double fwd_end = 0.0, bwd_start = fb;
+    res->launches = gc.launches;
+    size_t nf = 0;
+    for (const sa_launch &L : gc.launches) nf += L.direction == 0;
+    size_t kf = 0, kb = 0;
+    for (sa_launch &L : res->launches) {
+        const size_t e = L.direction == 0 ? 2 * kf++ : 2 * (nf + kb++);
+        float t0 = 0.f, t1 = 0.f;
+        if (have_fb && hipEventElapsedTime(&t0, ctx->ev[2], ctx->gev[e]) == hipSuccess &&
+            hipEventElapsedTime(&t1, ctx->ev[2], ctx->gev[e + 1]) == hipSuccess) {
+            L.start_ms = t0;
+            L.ms = t1 - t0;
+            if (L.direction == 0) fwd_end = std::max(fwd_end, (double)t1);
+            else bwd_start = std::min(bwd_start, (double)t0);
+        } else {
+            L.start_ms = 0.0;
+            L.ms = 0.0;
+        }
+    }
+    (void)hipGetLastError();
+    tm.forward_ms = fwd_end;
+    tm.backward_ms = have_fb ? fb - bwd_start : 0.0;
+    tm.forward_launches = (int64_t)nf;
+    tm.backward_launches = (int64_t)(gc.launches.size() - nf);
+    tm.cells = gc.cells;
+    tm.emission_path = gc.ek;
+    tm.replayed = 1;
     return SA_OK;
 }
 
@@ -1055,7 +1259,11 @@ int sa_batch_run(sa_batch *b, sa_precision precision, sa_result **out) {
     sa_result *res = new sa_result();
     const char *kern = getenv("SA_KERNELS");
     const bool lds = kern && std::strcmp(kern, "lds") == 0;
-    int rc = precision == SA_FP32 ? run_batch<float>(b, res, lds) : run_batch<double>(b, res, lds);
+    int rc;
+    if (!lds && b->tune.use_graph && b->gc[precision == SA_FP32 ? 1 : 0].exec)
+        rc = precision == SA_FP32 ? replay_batch<float>(b, res) : replay_batch<double>(b, res);
+    else
+        rc = precision == SA_FP32 ? run_batch<float>(b, res, lds) : run_batch<double>(b, res, lds);
     if (rc == kRetryLds) { /* a band escaped a register window: redo with the LDS sweeps */
         delete res;
         res = new sa_result();
@@ -1065,6 +1273,7 @@ int sa_batch_run(sa_batch *b, sa_precision precision, sa_result **out) {
         delete res;
         return rc;
     }
+    b->nruns[precision == SA_FP32 ? 1 : 0]++;
     *out = res;
     return SA_OK;
 }

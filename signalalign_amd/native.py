@@ -59,7 +59,7 @@ class _Timing(ctypes.Structure):
     _fields_ = [("prep_ms", ctypes.c_double), ("forward_ms", ctypes.c_double), ("backward_ms", ctypes.c_double),
                 ("collect_ms", ctypes.c_double), ("total_ms", ctypes.c_double), ("cells", ctypes.c_int64),
                 ("forward_launches", ctypes.c_int64), ("backward_launches", ctypes.c_int64),
-                ("fb_ms", ctypes.c_double), ("emission_path", ctypes.c_int32)]
+                ("fb_ms", ctypes.c_double), ("emission_path", ctypes.c_int32), ("replayed", ctypes.c_int32)]
 
 
 class _ServerConfig(ctypes.Structure):
@@ -75,6 +75,11 @@ class _ServerStats(ctypes.Structure):
     _fields_ = [("waves", ctypes.c_int64), ("reads", ctypes.c_int64), ("cells", ctypes.c_int64),
                 ("max_wave_reads", ctypes.c_int64), ("device_ms", ctypes.c_double), ("now_s", ctypes.c_double),
                 ("pending", ctypes.c_int64), ("inflight_bytes", ctypes.c_int64)]
+
+
+class _Tuning(ctypes.Structure):
+    _fields_ = [("use_graph", ctypes.c_int32), ("waves_per_simd", ctypes.c_int32), ("long_factor", ctypes.c_double),
+                ("stream_split", ctypes.c_double)]
 
 
 class _Launch(ctypes.Structure):
@@ -109,6 +114,9 @@ def load():
     L.sa_batch_create.argtypes = [vp, vp, ctypes.POINTER(_Read), ctypes.c_int64, ctypes.POINTER(_Params),
                                   ctypes.POINTER(vp)]
     L.sa_batch_run.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
+    L.sa_batch_set_tuning.argtypes = [vp, ctypes.POINTER(_Tuning)]
+    L.sa_batch_get_tuning.argtypes = [vp, ctypes.POINTER(_Tuning)]
+    L.sa_batch_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(_Tuning), ctypes.POINTER(ctypes.c_double)]
     L.sa_batch_refill.argtypes = [vp, vp, ctypes.POINTER(_Read), ctypes.c_int64, ctypes.POINTER(_Params)]
     L.sa_context_create_ex.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
     L.sa_server_create.argtypes = [ctypes.c_int, ctypes.POINTER(_ServerConfig), ctypes.POINTER(vp)]
@@ -373,6 +381,23 @@ class Batch:
             return out, timing
         finally:
             L.sa_result_free(res)
+
+    def tuning(self):
+        t = _Tuning()
+        self.ctx._L.sa_batch_get_tuning(self._h, ctypes.byref(t))
+        return {k: getattr(t, k) for k, _ in _Tuning._fields_}
+
+    def set_tuning(self, **kw):
+        t = self.tuning()
+        t.update(kw)
+        self.ctx._check(self.ctx._L.sa_batch_set_tuning(self._h, ctypes.byref(_Tuning(**t))), "sa_batch_set_tuning")
+
+    def autotune(self, precision=SA_FP32):
+        """Times candidate tunings, keeps the fastest; returns (tuning dict, device ms)."""
+        t, ms = _Tuning(), ctypes.c_double()
+        self.ctx._check(self.ctx._L.sa_batch_autotune(self._h, precision, ctypes.byref(t), ctypes.byref(ms)),
+                        "sa_batch_autotune")
+        return {k: getattr(t, k) for k, _ in _Tuning._fields_}, ms.value
 
     def close(self):
         if self._h:
