@@ -335,6 +335,7 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
     ECon<T, EK> pre;
     load_econ_uniform(pre, a, evc, 1 - X0, 1 - X0 >= 0 && 1 - X0 < lY);
     bool fresh = true;
+    bool bad = false;
 
     for (int d = 1; d <= N; d++) {
         int j = d - dc;
@@ -347,10 +348,10 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
         }
         const int lo = nlo, hi = nhi, fl = nfl;
         if (d < N) { nlo = ld_scalar(xlo, d + 1); nhi = ld_scalar(xhi, d + 1); nfl = ld_scalar_byte(dfl, d + 1); }
-        if (lo < X0) { /* band bounds are non-decreasing; anything else goes to the LDS kernels */
-            if (gl == 0) atomicOr(a.err, ERR_WINDOWS);
-            return;
-        }
+        /* band bounds are non-decreasing; anything else goes to the LDS kernels.  The
+           sweep runs on to its end (its results are discarded) so that the loop has a
+           single exit: early returns cost the loop its register assignment. */
+        bad |= lo < X0;
         /* fill values for the first lane of each wavefront: the last cell of the
            previous wavefront on d-1 (and on d-2 for the middle state) */
         T fLM = NI, fLX = NI, fLY = NI;
@@ -363,12 +364,10 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
         if (hi > X0 + W - 1) {
             /* slide the window as far as the two previous diagonals allow */
             const int minlo = lo1 < lo2 ? lo1 : lo2;
-            const int smax = (minlo - X0) / CPT;
+            int smax = (minlo - X0) / CPT;
             const int sreq = (hi - (X0 + W - 1) + CPT - 1) / CPT;
-            if (smax < sreq) {
-                if (gl == 0) atomicOr(a.err, ERR_WINDOWS);
-                return;
-            }
+            bad |= smax < sreq;
+            smax = smax < 0 ? 0 : smax;
             if constexpr (NW == 1) {
 #pragma unroll
                 for (int c = 0; c < CPT; c++) {
@@ -432,6 +431,10 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
         }
         const T LM = shr1(pM[CPT - 1], fLM), LX = shr1(pX[CPT - 1], fLX), LY = shr1(pY[CPT - 1], fLY);
         const T dl2 = NORM ? (T)(off2 - off1) : (T)0;
+        /* the rows are renormalised every 8th diagonal, so d-2 and d-1 carry different
+           offsets only on the diagonal after one */
+        const bool shift2 = NORM && off2 != off1;
+        const bool renorm = NORM && (d & 7) == 0;
         T nM[CPT], nX[CPT], nY[CPT];
         T mx = NI;
 #pragma unroll
@@ -440,7 +443,7 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
             const bool valid = x >= lo && x <= hi;
             const T lM = c ? pM[c - 1] : LM, lXv = c ? pX[c - 1] : LX, lYv = c ? pY[c - 1] : LY;
             T mM = c ? qM[c - 1] : LpM, mXv = c ? qX[c - 1] : LpX, mYv = c ? qY[c - 1] : LpY;
-            if (NORM) { mM += dl2; mXv += dl2; mYv += dl2; }
+            if (shift2) { mM += dl2; mXv += dl2; mYv += dl2; }
             const T eX = em_gapx<T>(m, kc[c]);
             T eM, eY;
             em_both<T>(m, kc[c], ev[c], s, iv, eM, eY);
@@ -456,14 +459,14 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
             nM[c] = valid ? M : NI;
             nX[c] = valid ? X : NI;
             nY[c] = valid ? Y : NI;
-            if (NORM) {
+            if (renorm) {
                 T m3 = nM[c] > nX[c] ? nM[c] : nX[c];
                 m3 = m3 > nY[c] ? m3 : nY[c];
                 mx = m3 > mx ? m3 : mx;
             }
         }
         double offd = off1;
-        if (NORM && (d & 7) == 0) {
+        if (renorm) {
             mx = wave_max(mx);
             if constexpr (NW > 1) {
                 if (lane == 0) red[wave] = mx;
@@ -509,6 +512,7 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
         lo1 = lo;
     }
     if (wave == 0 && dc + lane <= N) { doff[dc + lane] = md_doff; doffxy[dc + lane] = md_dxy; foff[dc + lane] = md_foff; }
+    if (bad && gl == 0) atomicOr(a.err, ERR_WINDOWS);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -619,6 +623,7 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
     double ob1 = 0.0, ob2 = 0.0; /* offsets of B[e+1], B[e+2] */
     int hi1 = -1, hi2 = -1;      /* xhi of e+1, e+2 (-1: outside the window of diagonals) */
     bool fresh = true;
+    bool bad = false;
 
     for (int e = D; e > TB; e--) {
         const int lo = nlo, hi = nhi, dof = ndo;
@@ -642,12 +647,10 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
             if (lo < X0) {
                 /* slide the window left as far as the two later diagonals allow */
                 const int maxhi = hi1 > hi2 ? hi1 : hi2;
-                const int smax = (X0 + W - 1 - maxhi) / CPT;
+                int smax = (X0 + W - 1 - maxhi) / CPT;
                 const int sreq = (X0 - lo + CPT - 1) / CPT;
-                if (smax < sreq) {
-                    if (lane == 0) atomicOr(a.err, ERR_WINDOWS);
-                    return;
-                }
+                bad |= smax < sreq; /* single loop exit, as in the forward sweep */
+                smax = smax < 0 ? 0 : smax;
 #pragma unroll
                 for (int c = 0; c < CPT; c++) {
                     pM[c] = shfl_idx(pM[c], lane - smax, NI); pX[c] = shfl_idx(pX[c], lane - smax, NI);
@@ -666,10 +669,7 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
                 for (int c = 0; c < CPT; c++) { sink(kc[c]); sink(ev[c]); }
                 fresh = true;
             }
-            if (hi > X0 + W - 1) {
-                if (lane == 0) atomicOr(a.err, ERR_WINDOWS);
-                return;
-            }
+            bad |= hi > X0 + W - 1;
             if (!fresh) {
                 /* cell x on e reads event e-x: its right neighbour's event of e+1 */
                 const ECon<T, EK> in = shl1(ev[0], pre);
@@ -682,6 +682,8 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
             LowX = shl1(pX[0], NI);
             const KCon<T, EK> kn = shl1_match(kc[0]); /* k-mer x of lane's last cell */
             const T dl2 = NORM ? (T)(ob2 - ob1) : (T)0;
+            const bool shift2 = NORM && ob2 != ob1;
+            const bool renorm = NORM && ((D - e) & 7) == 0;
             T mx = NI;
 #pragma unroll
             for (int c = 0; c < CPT; c++) {
@@ -691,7 +693,7 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
                 const T ClowX = c < CPT - 1 ? pX[c + 1] : LowX;
                 const T CupY = pY[c];
                 T CmidM = c < CPT - 1 ? qM[c + 1] : RpM;
-                if (NORM) CmidM += dl2;
+                if (shift2) CmidM += dl2;
                 const T eX = em_gapx<T>(m, kx);
                 const T eM = em_match<T>(m, kx, ev[c], s, iv);
                 const T eY = em_gapy<T>(m, kc[c], ev[c], s, iv);
@@ -707,14 +709,14 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
                 nM[c] = valid ? M : NI;
                 nX[c] = valid ? X : NI;
                 nY[c] = valid ? Y : NI;
-                if (NORM) {
+                if (renorm) {
                     T m3 = nM[c] > nX[c] ? nM[c] : nX[c];
                     m3 = m3 > nY[c] ? m3 : nY[c];
                     mx = m3 > mx ? m3 : mx;
                 }
             }
             obe = ob1;
-            if (NORM && ((D - e) & 7) == 0) {
+            if (renorm) {
                 mx = wave_max(mx);
                 if (mx > NI) {
 #pragma unroll
@@ -834,6 +836,7 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
         hi1 = hi;
     }
     if (lane == 0) a.win_count[wi] = cursor;
+    if (bad && lane == 0) atomicOr(a.err, ERR_WINDOWS);
 }
 
 /* ------------------------------------------------------------------------ */
