@@ -160,7 +160,7 @@ def serving_leg(native, synth, ctx, gmodel, args):
         pools.append((ctx.model(m), objs, [o._struct() for o in objs]))
     log(f"serving: pools of {args.serve_pool} reads per model in {time.time() - t0:.1f}s")
     ps = native.Params(diagonal_expansion=args.expansion)._struct()
-    srv = native.Server(0, hbm_budget_bytes=96 << 30, lanes=3)
+    srv = native.Server(0, hbm_budget_bytes=96 << 30, lanes=args.serve_lanes, max_wave_reads=args.serve_max_wave)
     rng = np.random.default_rng(args.seed + 13)
 
     def pick(n):
@@ -226,7 +226,8 @@ def serving_leg(native, synth, ctx, gmodel, args):
            "max_ms": round(float(lat.max()) * 1e3, 3), "gcups": round(cells / span / 1e9, 3),
            "achieved_reads_per_s": round(len(reqs) / span, 1), "waves": nw,
            "mean_wave_reads": round(len(reqs) / max(nw, 1), 1), "submitter_max_lag_ms": round(lag * 1e3, 3),
-           "max_wave_reads": st["max_wave_reads"]}
+           "max_wave_reads": st["max_wave_reads"], "lanes": args.serve_lanes,
+           "wave_cap_reads": args.serve_max_wave}
     log(f"serving: {out}")
     return out
 
@@ -297,6 +298,8 @@ def main():
     ap.add_argument("--serve-pool", type=int, default=1024)
     ap.add_argument("--serve-burst", type=int, default=8192)
     ap.add_argument("--serve-load", type=float, default=0.7)
+    ap.add_argument("--serve-lanes", type=int, default=3)
+    ap.add_argument("--serve-max-wave", type=int, default=1024)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run a device copy of this many MiB (HBM byte-counter calibration under rocprofv3)")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "traffic_batched_4096reads.txt"))

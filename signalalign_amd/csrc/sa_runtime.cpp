@@ -49,6 +49,27 @@ struct DBuf {
     template <typename U> U *as() const { return reinterpret_cast<U *>(p); }
 };
 
+/* pinned host staging (DMA at full PCIe rate; grows, never shrinks) */
+struct HBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap && p) return hipSuccess;
+        if (p) { (void)hipHostFree(p); p = nullptr; cap = 0; }
+        size_t want = std::max<size_t>(bytes + bytes / 4, 1 << 20);
+        hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+        if (e == hipSuccess) cap = want;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <typename U> U *as() const { return reinterpret_cast<U *>(p); }
+};
+
+
 template <typename T>
 struct DevTables {
     DBuf mmu, misd, mca, mnmu, mnb, ymu, yisd, yca, ynmu, ynb, gx, hmu, hrow, gridx, ys;
@@ -65,6 +86,7 @@ struct sa_context {
     hipStream_t stream = nullptr;
     hipStream_t aux[3] = {};   /* sub-alignment groups of different window sizes run side by side */
     int n_streams = 4;         /* 1: everything on `stream` (aux[] alias it) */
+    HBuf stage;                /* pinned staging for inputs and pairs (calls are serialised by mu) */
     std::mutex mu;
     std::string err;
     hipEvent_t ev[16] = {};
@@ -377,6 +399,7 @@ void sa_context_destroy(sa_context *ctx) {
     for (auto &e : ctx->ev) if (e) (void)hipEventDestroy(e);
     for (auto &e : ctx->gev) if (e) (void)hipEventDestroy(e);
     for (auto &q : ctx->aux) if (q && q != ctx->stream) (void)hipStreamDestroy(q);
+    ctx->stage.release();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -584,22 +607,30 @@ int batch_fill(sa_context *ctx, sa_batch *b, const sa_model *model, const sa_rea
     b->total_anc = (int64_t)ancx.size();
     if (stride < 0) stride = 4;
     b->stride = stride;
-    /* pack raw inputs into HBM */
-    std::vector<char> ref((size_t)std::max<int64_t>(ref_off, 1));
-    std::vector<double> ev((size_t)std::max<int64_t>(ev_off * stride, 1));
+    /* pack the raw inputs straight into pinned staging, then one DMA per array */
+    const size_t ref_bytes = (size_t)std::max<int64_t>(ref_off, 1);
+    const size_t ref_pad = (ref_bytes + 255) & ~(size_t)255;
+    const size_t ev_bytes = sizeof(double) * (size_t)std::max<int64_t>(ev_off * stride, 1);
+    SA_HIP(ctx, ctx->stage.ensure(ref_pad + ev_bytes));
+    char *h_ref = ctx->stage.as<char>();
+    double *h_ev = reinterpret_cast<double *>(h_ref + ref_pad);
     int64_t ro = 0, eo = 0;
     for (int64_t r = 0; r < n_reads; r++) {
-        if (reads[r].ref_length) std::memcpy(ref.data() + ro, reads[r].ref, (size_t)reads[r].ref_length);
+        if (reads[r].ref_length) std::memcpy(h_ref + ro, reads[r].ref, (size_t)reads[r].ref_length);
         if (reads[r].n_events)
-            std::memcpy(ev.data() + eo * stride, reads[r].events, sizeof(double) * (size_t)(reads[r].n_events * stride));
+            std::memcpy(h_ev + eo * stride, reads[r].events, sizeof(double) * (size_t)(reads[r].n_events * stride));
         ro += reads[r].ref_length;
         eo += reads[r].n_events;
     }
+    SA_HIP(ctx, b->d_ref.ensure(ref_bytes));
+    SA_HIP(ctx, b->d_events.ensure(ev_bytes));
+    SA_HIP(ctx, hipMemcpyAsync(b->d_ref.p, h_ref, ref_bytes, hipMemcpyHostToDevice, ctx->stream));
+    SA_HIP(ctx, hipMemcpyAsync(b->d_events.p, h_ev, ev_bytes, hipMemcpyHostToDevice, ctx->stream));
+    SA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     std::vector<int32_t> x1(b->tasks.size()), y1(b->tasks.size());
     for (size_t i = 0; i < b->tasks.size(); i++) { x1[i] = b->thost[i].x1; y1[i] = b->thost[i].y1; }
     int rc;
-    if ((rc = upload(ctx, b->d_ref, ref)) || (rc = upload(ctx, b->d_events, ev)) ||
-        (rc = upload(ctx, b->d_ancx, ancx)) || (rc = upload(ctx, b->d_ancy, ancy)) ||
+    if ((rc = upload(ctx, b->d_ancx, ancx)) || (rc = upload(ctx, b->d_ancy, ancy)) ||
         (rc = upload(ctx, b->d_tasks, b->tasks)) || (rc = upload(ctx, b->d_x1, x1)) ||
         (rc = upload(ctx, b->d_y1, y1)))
         return rc;
@@ -1119,13 +1150,14 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
                                    b->d_out.as<int32_t>(), b->d_packed.as<int32_t>(), b->d_x1.as<int32_t>(),
                                    b->d_y1.as<int32_t>(), (int)nw, st));
     SA_HIP(ctx, hipEventRecord(ctx->ev[6], st));
-    res->pairs.resize(total_pairs);
     std::vector<double> wtot(nw);
+    SA_HIP(ctx, ctx->stage.ensure(sizeof(sa_pair) * (size_t)std::max<int64_t>(total_pairs, 1)));
     if (total_pairs)
-        SA_HIP(ctx, hipMemcpyAsync(res->pairs.data(), b->d_packed.p, sizeof(int32_t) * 3 * total_pairs,
+        SA_HIP(ctx, hipMemcpyAsync(ctx->stage.p, b->d_packed.p, sizeof(int32_t) * 3 * total_pairs,
                                    hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipMemcpyAsync(wtot.data(), b->d_win_total.p, sizeof(double) * nw, hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipStreamSynchronize(st));
+    res->pairs.assign(ctx->stage.as<sa_pair>(), ctx->stage.as<sa_pair>() + total_pairs);
     for (int64_t r = 0; r < b->n_reads; r++) {
         double ll = 0.0;
         for (int64_t ti = b->reads[r].first_task; ti < b->reads[r].first_task + b->reads[r].n_tasks; ti++)
@@ -1192,15 +1224,16 @@ int replay_batch(sa_batch *b, sa_result *res) {
     SA_HIP(ctx, hipGraphLaunch(gc.exec, st));
     SA_HIP(ctx, hipEventRecord(ctx->ev[13], st));
     res->reads = gc.reads;
-    res->pairs.resize(gc.total_pairs);
     std::vector<double> wtot(gc.nw);
     int32_t err = 0;
+    SA_HIP(ctx, ctx->stage.ensure(sizeof(sa_pair) * (size_t)std::max<int64_t>(gc.total_pairs, 1)));
     if (gc.total_pairs)
-        SA_HIP(ctx, hipMemcpyAsync(res->pairs.data(), b->d_packed.p, sizeof(int32_t) * 3 * gc.total_pairs,
+        SA_HIP(ctx, hipMemcpyAsync(ctx->stage.p, b->d_packed.p, sizeof(int32_t) * 3 * gc.total_pairs,
                                    hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipMemcpyAsync(wtot.data(), b->d_win_total.p, sizeof(double) * gc.nw, hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipMemcpyAsync(&err, b->d_err.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipStreamSynchronize(st));
+    res->pairs.assign(ctx->stage.as<sa_pair>(), ctx->stage.as<sa_pair>() + gc.total_pairs);
     if (err) return fail(ctx, SA_ERR_HIP, "replayed run raised device error flags " + std::to_string(err));
     for (int64_t r = 0; r < b->n_reads; r++) {
         double ll = 0.0;

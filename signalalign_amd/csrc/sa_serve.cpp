@@ -45,11 +45,10 @@ struct Request {
     double scale = 1, shift = 0, var = 1;
     int32_t rl = 1, rr = 1;
     int64_t est_bytes = 0;
-    /* outputs */
+    /* outputs: the wave's result, shared by its requests */
     int status = SA_PENDING;
-    std::vector<sa_pair> pairs;
-    double log_likelihood = 0.0;
-    int64_t cells = 0, n_windows = 0, n_sub = 0;
+    std::shared_ptr<sa_result> wres;
+    int64_t widx = 0;
     sa_request_timing timing{};
 };
 
@@ -179,22 +178,19 @@ void sa_server::lane_loop(Lane *L) {
             sa_result *res = nullptr;
             if (rc == SA_OK) rc = sa_batch_run(L->batch, f->precision, &res);
             if (rc != SA_OK) return rc;
+            std::shared_ptr<sa_result> sp(res, sa_result_free);
             for (size_t i = i0; i < i1; i++) {
                 Request *q = wave[i];
                 sa_read_result rr;
                 sa_result_read(res, (int64_t)(i - i0), &rr);
                 q->status = SA_OK;
-                q->pairs.assign(rr.pairs, rr.pairs + rr.n_pairs);
-                q->log_likelihood = rr.log_likelihood;
-                q->cells = rr.cells;
-                q->n_windows = rr.n_windows;
-                q->n_sub = rr.n_subalignments;
+                q->wres = sp;
+                q->widx = (int64_t)(i - i0);
                 cells += rr.cells;
             }
             sa_timing tm;
             sa_result_timing(res, &tm);
             dev_ms += tm.total_ms;
-            sa_result_free(res);
             return SA_OK;
         };
         const int rc = run(0, wave.size());
@@ -303,9 +299,14 @@ int sa_server_submit(sa_server *s, const sa_model *model, const sa_read *read, c
     if (params) q->params = *params; else sa_params_default(&q->params);
     q->precision = precision;
     q->ref.assign(read->ref, (size_t)read->ref_length);
+    /* the sweeps read an event's mean and noise only: keep those two */
     q->n_events = read->n_events;
-    q->stride = read->event_stride;
-    q->events.assign(read->events, read->events + read->n_events * read->event_stride);
+    q->stride = 2;
+    q->events.resize((size_t)(2 * read->n_events));
+    for (int64_t j = 0; j < read->n_events; j++) {
+        q->events[2 * j] = read->events[j * read->event_stride];
+        q->events[2 * j + 1] = read->events[j * read->event_stride + 1];
+    }
     q->anchors.assign(read->anchors, read->anchors + 2 * read->n_anchors);
     q->scale = read->scale;
     q->shift = read->shift;
@@ -336,14 +337,7 @@ int sa_server_wait(sa_server *s, int64_t ticket, double timeout_s, sa_read_resul
     }
     if (timing) *timing = q->timing;
     if (q->status != SA_OK) return q->status;
-    if (out) {
-        out->pairs = q->pairs.empty() ? nullptr : q->pairs.data();
-        out->n_pairs = (int64_t)q->pairs.size();
-        out->log_likelihood = q->log_likelihood;
-        out->cells = q->cells;
-        out->n_windows = q->n_windows;
-        out->n_subalignments = q->n_sub;
-    }
+    if (out) sa_result_read(q->wres.get(), q->widx, out);
     return SA_OK;
 }
 
