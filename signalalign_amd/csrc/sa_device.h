@@ -124,33 +124,60 @@ __device__ __forceinline__ T gauss_emit(T mu, T isd, T ca, T nmu, T nb, T evm, T
 /* emissions_signal_getHdpKmerDensity (impl/stateMachine.c:468-495) ->
  * dir_proc_density (impl/hdp.c:2583-2607) -> grid_spline_interp
  * (impl/hdp_math_utils.c:471-496): Hermite cubic on the row's uniform grid with linear
- * extrapolation, clamped at 0, divided by var, in log space. */
+ * extrapolation, clamped at 0, divided by var, in log space.  Split in two so that a
+ * sweep can issue the gathers (hdp_fetch) a diagonal before it needs the value
+ * (hdp_finish); hdp_emit_row is the two back to back. */
+template <typename T> struct HdpFetch {
+    T u;        /* descaled event mean */
+    T y0, s0;   /* knot il */
+    T y1, s1;   /* knot il + 1 */
+    T xl;       /* grid x of knot il */
+    int32_t row;
+};
+
+/* The knot pair below u (clamped to the grid): below the grid that is knot 0, above it
+ * knot G-1 is the right one of the last pair, so the extrapolations read the same
+ * registers without a branch around the loads. */
 template <typename T>
-__device__ __forceinline__ T hdp_emit_row(const Model<T> &m, int32_t row, T hmu, T evm, T s, T inv_var) {
-    if (row < 0) return ninf<T>();
-    T u = evm + hmu * ((T)1 - s);
+__device__ __forceinline__ HdpFetch<T> hdp_fetch(const Model<T> &m, int32_t row, T hmu, T evm, T s) {
+    HdpFetch<T> f;
+    f.u = evm + hmu * ((T)1 - s);
+    f.row = row;
     const int32_t G = m.grid_length;
-    const T *ys = m.ys + (int64_t)row * 2 * G;
-    const T x0 = m.gridx[0], xn = m.gridx[G - 1];
+    T t = (f.u - m.grid_x0) * m.inv_dx;
+    t = t > (T)0 ? t : (T)0;
+    t = t < (T)(G - 2) ? t : (T)(G - 2);
+    const int32_t il = (int32_t)t;
+    const T *ys = m.ys + ((int64_t)(row < 0 ? 0 : row) * G + il) * 2;
+    f.y0 = ys[0]; f.s0 = ys[1]; f.y1 = ys[2]; f.s1 = ys[3];
+    f.xl = m.gridx[il];
+    return f;
+}
+
+template <typename T>
+__device__ __forceinline__ T hdp_finish(const Model<T> &m, const HdpFetch<T> &f, T inv_var) {
+    if (f.row < 0) return ninf<T>();
     T interp;
-    if (u <= x0) {
-        interp = ys[0] - ys[1] * (x0 - u);
-    } else if (u >= xn) {
-        interp = ys[2 * (G - 1)] + ys[2 * (G - 1) + 1] * (u - xn);
+    if (f.u <= m.grid_x0) {
+        interp = f.y0 - f.s0 * (m.grid_x0 - f.u);
+    } else if (f.u >= m.grid_xn) {
+        interp = f.y1 + f.s1 * (f.u - m.grid_xn);
     } else {
-        int32_t il = (int32_t)((u - x0) * m.inv_dx);
-        il = il > G - 2 ? G - 2 : il;
-        T y0 = ys[2 * il], s0 = ys[2 * il + 1], y1 = ys[2 * il + 2], s1 = ys[2 * il + 3];
-        T dx = m.gridx[1] - m.gridx[0];
-        T dy = y1 - y0;
-        T a = s0 * dx - dy;
-        T b = dy - s1 * dx;
-        T tl = (u - m.gridx[il]) * m.inv_dx;
-        T tr = (T)1 - tl;
-        interp = tr * y0 + tl * y1 + tl * tr * (a * tr + b * tl);
+        const T dx = m.grid_dx;
+        const T dy = f.y1 - f.y0;
+        const T a = f.s0 * dx - dy;
+        const T b = dy - f.s1 * dx;
+        const T tl = (f.u - f.xl) * m.inv_dx;
+        const T tr = (T)1 - tl;
+        interp = tr * f.y0 + tl * f.y1 + tl * tr * (a * tr + b * tl);
     }
     T dens = interp > (T)0 ? interp : (T)0;
     return flog(inv_var * dens);
+}
+
+template <typename T>
+__device__ __forceinline__ T hdp_emit_row(const Model<T> &m, int32_t row, T hmu, T evm, T s, T inv_var) {
+    return hdp_finish(m, hdp_fetch(m, row, hmu, evm, s), inv_var);
 }
 
 template <typename T>

@@ -336,6 +336,11 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
     load_econ_uniform(pre, a, evc, 1 - X0, 1 - X0 >= 0 && 1 - X0 < lY);
     bool fresh = true;
     bool bad = false;
+    /* HDP with one wavefront: the density gathers of diagonal d+1 are issued at the end
+       of diagonal d, before its stores, and used a diagonal later */
+    constexpr bool PF = EK == 2 && NW == 1;
+    HdpFetch<T> gp[PF ? CPT : 1];
+    bool have_gp = false;
 
     for (int d = 1; d <= N; d++) {
         int j = d - dc;
@@ -412,6 +417,7 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
 #pragma unroll
             for (int c = 0; c < CPT; c++) { sink(kc[c]); sink(ev[c]); }
             fresh = true;
+            have_gp = false;
         }
         if (!fresh) {
             /* cell x on d reads event d-x-1: its left neighbour's event of d-1 */
@@ -446,7 +452,12 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
             if (shift2) { mM += dl2; mXv += dl2; mYv += dl2; }
             const T eX = em_gapx<T>(m, kc[c]);
             T eM, eY;
-            em_both<T>(m, kc[c], ev[c], s, iv, eM, eY);
+            if (PF && have_gp) {
+                eM = hdp_finish(m, gp[PF ? c : 0], iv);
+                eY = eM;
+            } else {
+                em_both<T>(m, kc[c], ev[c], s, iv, eM, eY);
+            }
             /* lower -> gapX, middle -> match, upper -> gapY */
             T X = lM + (eX + tGOX);
             X = logadd_lut(X, lXv + (eX + tGEX), lut);
@@ -480,6 +491,17 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
                 for (int c = 0; c < CPT; c++) { nM[c] -= mx; nX[c] -= mx; nY[c] -= mx; }
                 offd = off1 + (double)mx;
             }
+        }
+        if constexpr (PF) {
+            /* the events of d+1 are this diagonal's shifted one lane (the shift the next
+               iteration makes); the first lane's comes from `pre` */
+            const ECon<T, EK> inc = shr1(ev[CPT - 1], pre);
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                const T evm = c ? ev[c - 1].m : inc.m;
+                gp[c] = hdp_fetch(m, kc[c].row, kc[c].hmu, evm, s);
+            }
+            have_gp = true;
         }
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
@@ -624,6 +646,14 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
     int hi1 = -1, hi2 = -1;      /* xhi of e+1, e+2 (-1: outside the window of diagonals) */
     bool fresh = true;
     bool bad = false;
+    /* HDP: the match and extra-event emissions are the same density, and the match
+       emission of cell x on e (k-mer x+1, event e-x) is the extra-event emission its
+       right neighbour computed on e+1; so one density per cell, its gathers issued a
+       diagonal ahead */
+    constexpr bool PF = EK == 2;
+    HdpFetch<T> gp[PF ? CPT : 1];
+    T eYp[PF ? CPT : 1];
+    bool have_gp = false, have_eY = false;
 
     for (int e = D; e > TB; e--) {
         const int lo = nlo, hi = nhi, dof = ndo;
@@ -668,6 +698,8 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
 #pragma unroll
                 for (int c = 0; c < CPT; c++) { sink(kc[c]); sink(ev[c]); }
                 fresh = true;
+                have_gp = false;
+                have_eY = false;
             }
             bad |= hi > X0 + W - 1;
             if (!fresh) {
@@ -683,6 +715,8 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
             const KCon<T, EK> kn = shl1_match(kc[0]); /* k-mer x of lane's last cell */
             const T dl2 = NORM ? (T)(ob2 - ob1) : (T)0;
             const bool shift2 = NORM && ob2 != ob1;
+            T eYc[PF ? CPT : 1];
+            const T eYn = PF && have_eY ? shl1(eYp[0], NI) : NI;
             const bool renorm = NORM && ((D - e) & 7) == 0;
             T mx = NI;
 #pragma unroll
@@ -695,8 +729,15 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
                 T CmidM = c < CPT - 1 ? qM[c + 1] : RpM;
                 if (shift2) CmidM += dl2;
                 const T eX = em_gapx<T>(m, kx);
-                const T eM = em_match<T>(m, kx, ev[c], s, iv);
-                const T eY = em_gapy<T>(m, kc[c], ev[c], s, iv);
+                T eM, eY;
+                if constexpr (PF) {
+                    eY = have_gp ? hdp_finish(m, gp[c], iv) : em_gapy<T>(m, kc[c], ev[c], s, iv);
+                    eM = have_eY ? (c < CPT - 1 ? eYp[c < CPT - 1 ? c + 1 : 0] : eYn) : em_match<T>(m, kx, ev[c], s, iv);
+                    eYc[c] = eY;
+                } else {
+                    eM = em_match<T>(m, kx, ev[c], s, iv);
+                    eY = em_gapy<T>(m, kc[c], ev[c], s, iv);
+                }
                 /* contributions arrive middle (e+2), then upper, then lower (e+1) */
                 T M = CmidM + (eM + tMC);
                 M = logadd_lut(M, CupY + (eY + tGOY), lut);
@@ -715,6 +756,11 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
                     mx = m3 > mx ? m3 : mx;
                 }
             }
+            if constexpr (PF) {
+#pragma unroll
+                for (int c = 0; c < CPT; c++) eYp[c] = eYc[c];
+                have_eY = true;
+            }
             obe = ob1;
             if (renorm) {
                 mx = wave_max(mx);
@@ -728,6 +774,16 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
         {
             const int jn = (e - 1) - (X0 + W - 1); /* event of lane 63's last cell on e-1 */
             load_econ_uniform(pre, a, evc, jn, jn >= 0 && jn < lY);
+        }
+        if constexpr (PF) {
+            /* the events of e-1 are this diagonal's shifted one lane left */
+            const ECon<T, EK> in = shl1(ev[0], pre);
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                const T evm = c < CPT - 1 ? ev[c < CPT - 1 ? c + 1 : 0].m : in.m;
+                gp[c] = hdp_fetch(m, kc[c].row, kc[c].hmu, evm, s);
+            }
+            have_gp = true;
         }
 
         if (e <= TF) {

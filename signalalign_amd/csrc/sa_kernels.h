@@ -30,6 +30,7 @@ struct Model {
     int32_t grid_length;
     int32_t use_swx;        /* GAP_SWITCH_TO_X above -inf: the gapY -> gapX term is summed */
     T inv_dx;
+    T grid_x0, grid_xn, grid_dx; /* first and last knot, knot spacing (knot 1 - knot 0) */
     T gx_const;             /* the skip emission when every k-mer has the same one */
     T trans[9];
     T endv[3];

@@ -106,6 +106,7 @@ struct sa_model {
     double endv[3], rendv[3];
     int32_t grid_length = 0;
     double inv_dx = 0.0;
+    double grid0 = 0.0, grid1 = 0.0, gridn = 0.0; /* first, second and last grid knots */
     int32_t ek = 1;          /* register-kernel emission variant: 0 standard Gaussian, 1 any Gaussian, 2 HDP */
     double gx_const = -2.3025850929940455;
     DBuf d_lut;
@@ -291,6 +292,9 @@ sa::Model<T> device_model(const sa_model *m, const DevTables<T> &t) {
     r.grid_length = m->grid_length;
     r.use_swx = m->trans[7] > -HUGE_VAL ? 1 : 0;
     r.inv_dx = (T)m->inv_dx;
+    r.grid_x0 = (T)m->grid0;
+    r.grid_xn = (T)m->gridn;
+    r.grid_dx = (T)m->grid1 - (T)m->grid0; /* as the kernels computed it from the knots */
     r.gx_const = (T)m->gx_const;
     for (int i = 0; i < 9; i++) r.trans[i] = (T)m->trans[i];
     for (int i = 0; i < 3; i++) { r.endv[i] = (T)m->endv[i]; r.rendv[i] = (T)m->rendv[i]; }
@@ -445,6 +449,9 @@ int sa_model_create(sa_context *ctx, const sa_model_desc *d, sa_model **out) {
     if (d->kind == SA_EMIT_HDP) {
         m->grid_length = (int32_t)d->grid_length;
         m->inv_dx = 1.0 / (d->grid[1] - d->grid[0]);
+        m->grid0 = d->grid[0];
+        m->grid1 = d->grid[1];
+        m->gridn = d->grid[d->grid_length - 1];
         for (int64_t i = 0; i < m->n_kmers; i++)
             if (d->kmer_row[i] < 0 || d->kmer_row[i] >= d->n_rows) {
                 delete m;

@@ -45,6 +45,7 @@ struct Request {
     double scale = 1, shift = 0, var = 1;
     int32_t rl = 1, rr = 1;
     int64_t est_bytes = 0;
+    int64_t n_diag = 0; /* lX + lY: the length of the forward sweep's dependency chain */
     /* outputs: the wave's result, shared by its requests */
     int status = SA_PENDING;
     std::shared_ptr<sa_result> wres;
@@ -116,8 +117,11 @@ void sa_server::schedule_loop() {
         for (auto it = pending.begin(); it != pending.end() && (int64_t)wave.size() < cfg.max_wave_reads;) {
             Request *q = *it;
             const bool fits = wave.empty() ? true : inflight_bytes + bytes + q->est_bytes <= cfg.hbm_budget_bytes;
+            /* a wave finishes with its longest read: keep reads of one length class
+               (within a factor of two of the oldest request) together */
+            const bool same_class = 2 * q->n_diag >= first->n_diag && q->n_diag <= 2 * first->n_diag;
             if (q->model == first->model && q->precision == first->precision &&
-                same_params(q->params, first->params) && fits) {
+                same_params(q->params, first->params) && same_class && fits) {
                 wave.push_back(q);
                 bytes += q->est_bytes;
                 it = pending.erase(it);
@@ -314,6 +318,7 @@ int sa_server_submit(sa_server *s, const sa_model *model, const sa_read *read, c
     q->rl = read->ragged_left;
     q->rr = read->ragged_right;
     q->est_bytes = estimate_bytes(read, sa_model_kmer_length(model), &q->params, precision);
+    q->n_diag = std::max<int64_t>(read->ref_length - sa_model_kmer_length(model) + 1, 0) + read->n_events;
     std::lock_guard<std::mutex> lk(s->mu);
     q->ticket = s->next_ticket++;
     q->timing.submit_s = s->now();
