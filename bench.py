@@ -204,8 +204,12 @@ def serving_leg(native, synth, ctx, gmodel, args):
         return np.array(lat), cells, t_last - t_first, len(waves), lag[0]
 
     burst = pick(args.serve_burst)
+    st0 = srv.stats()
     lat, cells, span, nw, _ = run(burst, 0.0)
     capacity = len(burst) / span
+    st1 = srv.stats()
+    log(f"serving burst: lanes' fill {st1['fill_ms'] - st0['fill_ms']:.0f} ms, run {st1['run_ms'] - st0['run_ms']:.0f} ms, "
+        f"device {st1['device_ms'] - st0['device_ms']:.0f} ms over {st1['waves'] - st0['waves']} waves")
     log(f"serving: burst of {len(burst)} reads in {span:.2f}s ({capacity:.0f} reads/s, "
         f"{cells / span / 1e9:.1f} GCUPS, {nw} waves)")
     rate = args.serve_load * capacity

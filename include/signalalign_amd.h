@@ -185,6 +185,7 @@ int sa_batch_run(sa_batch *batch, sa_precision precision, sa_result **out);
 /* Replaces the batch's reads (and model/params), reusing its device buffers. */
 int sa_batch_refill(sa_batch *batch, const sa_model *model, const sa_read *reads, int64_t n_reads,
                     const sa_params *params);
+/* (a batch is destroyed before the context it was created on) */
 void sa_batch_destroy(sa_batch *batch);
 
 /* Launch tuning of a batch.  use_graph: the first run records the launch sequence
@@ -259,6 +260,8 @@ typedef struct {
     double now_s;
     int64_t pending;         /* submitted, not yet admitted */
     int64_t inflight_bytes;  /* band-matrix bytes admitted and not finished */
+    double fill_ms;          /* lane host time splitting and uploading waves, summed */
+    double run_ms;           /* lane wall time running waves (device work and host planning), summed */
 } sa_server_stats;
 
 int sa_server_create(int device, const sa_server_config *config, sa_server **out);

@@ -30,21 +30,45 @@ using sa::WinDev;
 
 namespace {
 
+/* The stream a thread's batch buffers are ordered on while it holds a context's lock:
+   growing a buffer then frees and allocates on that stream (stream-ordered pool)
+   instead of hipFree, which waits for every stream on the device -- other
+   contexts' work included. */
+thread_local hipStream_t t_alloc_stream = nullptr;
+
+struct AllocStream {
+    explicit AllocStream(hipStream_t s) { t_alloc_stream = s; }
+    ~AllocStream() { t_alloc_stream = nullptr; }
+};
+
 struct DBuf {
     void *p = nullptr;
     size_t cap = 0;
+    hipStream_t owner = nullptr; /* stream of a stream-ordered allocation */
     hipError_t ensure(size_t bytes) {
         if (bytes <= cap && p) return hipSuccess;
-        if (p) { (void)hipFree(p); p = nullptr; cap = 0; }
+        release();
         size_t want = std::max<size_t>(bytes + bytes / 4, 256); /* headroom: refilled batches grow rarely */
-        hipError_t e = hipMalloc(&p, want);
-        if (e == hipSuccess) cap = want;
+        hipStream_t s = t_alloc_stream;
+        hipError_t e = s ? hipMallocAsync(&p, want, s) : hipMalloc(&p, want);
+        if (e != hipSuccess && s) { /* no stream-ordered pool: plain allocation */
+            (void)hipGetLastError();
+            p = nullptr;
+            s = nullptr;
+            e = hipMalloc(&p, want);
+        }
+        if (e == hipSuccess) { cap = want; owner = s; }
+        else p = nullptr;
         return e;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (owner) (void)hipFreeAsync(p, t_alloc_stream ? t_alloc_stream : owner);
+            else (void)hipFree(p);
+        }
         p = nullptr;
         cap = 0;
+        owner = nullptr;
     }
     template <typename U> U *as() const { return reinterpret_cast<U *>(p); }
 };
@@ -57,7 +81,7 @@ struct HBuf {
         if (bytes <= cap && p) return hipSuccess;
         if (p) { (void)hipHostFree(p); p = nullptr; cap = 0; }
         size_t want = std::max<size_t>(bytes + bytes / 4, 1 << 20);
-        hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+        hipError_t e = hipHostMalloc(&p, want, hipHostMallocNonCoherent); /* host-cached */
         if (e == hipSuccess) cap = want;
         return e;
     }
@@ -663,6 +687,7 @@ int sa_batch_create(sa_context *ctx, const sa_model *model, const sa_read *reads
     if (!ctx || !model || (!reads && n_reads > 0) || n_reads < 0) return SA_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lk(ctx->mu);
     (void)hipSetDevice(ctx->device);
+    AllocStream as(ctx->stream);
     sa_batch *b = new sa_batch();
     const int rc = batch_fill(ctx, b, model, reads, n_reads, params);
     if (rc != SA_OK) {
@@ -679,6 +704,7 @@ int sa_batch_refill(sa_batch *b, const sa_model *model, const sa_read *reads, in
     sa_context *ctx = b->ctx;
     std::lock_guard<std::mutex> lk(ctx->mu);
     (void)hipSetDevice(ctx->device);
+    AllocStream as(ctx->stream);
     const int rc = batch_fill(ctx, b, model, reads, n_reads, params);
     if (rc != SA_OK) batch_clear(b);
     return rc;
@@ -753,7 +779,10 @@ int sa_batch_autotune(sa_batch *b, sa_precision precision, sa_tuning *best, doub
 
 void sa_batch_destroy(sa_batch *b) {
     if (!b) return;
-    (void)hipSetDevice(b->ctx->device);
+    sa_context *ctx = b->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    AllocStream as(ctx->stream); /* buffers go back to the pool on the context's stream */
     delete b;
 }
 
@@ -1296,6 +1325,7 @@ int sa_batch_run(sa_batch *b, sa_precision precision, sa_result **out) {
     sa_context *ctx = b->ctx;
     std::lock_guard<std::mutex> lk(ctx->mu);
     (void)hipSetDevice(ctx->device);
+    AllocStream as(ctx->stream);
     sa_result *res = new sa_result();
     const char *kern = getenv("SA_KERNELS");
     const bool lds = kern && std::strcmp(kern, "lds") == 0;

@@ -175,12 +175,17 @@ void sa_server::lane_loop(Lane *L) {
         int64_t cells = 0;
         double dev_ms = 0.0;
         /* runs reads [i0, i1) of the wave as one batch */
+        double fill_ms = 0.0, run_ms = 0.0;
         auto run = [&](size_t i0, size_t i1) -> int {
             const int64_t n = (int64_t)(i1 - i0);
+            const auto t0 = std::chrono::steady_clock::now();
             int rc = L->batch ? sa_batch_refill(L->batch, f->model, rd.data() + i0, n, &f->params)
                               : sa_batch_create(L->ctx, f->model, rd.data() + i0, n, &f->params, &L->batch);
+            const auto t1 = std::chrono::steady_clock::now();
             sa_result *res = nullptr;
             if (rc == SA_OK) rc = sa_batch_run(L->batch, f->precision, &res);
+            fill_ms += std::chrono::duration<double, std::milli>(t1 - t0).count();
+            run_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
             if (rc != SA_OK) return rc;
             std::shared_ptr<sa_result> sp(res, sa_result_free);
             for (size_t i = i0; i < i1; i++) {
@@ -223,6 +228,8 @@ void sa_server::lane_loop(Lane *L) {
             stats.reads += (int64_t)wave.size();
             stats.cells += cells;
             stats.device_ms += dev_ms;
+            stats.fill_ms += fill_ms;
+            stats.run_ms += run_ms;
             if ((int64_t)wave.size() > stats.max_wave_reads) stats.max_wave_reads = (int64_t)wave.size();
             inflight_bytes -= L->wave_bytes;
             L->wave.clear();

@@ -74,7 +74,8 @@ class _ReqTiming(ctypes.Structure):
 class _ServerStats(ctypes.Structure):
     _fields_ = [("waves", ctypes.c_int64), ("reads", ctypes.c_int64), ("cells", ctypes.c_int64),
                 ("max_wave_reads", ctypes.c_int64), ("device_ms", ctypes.c_double), ("now_s", ctypes.c_double),
-                ("pending", ctypes.c_int64), ("inflight_bytes", ctypes.c_int64)]
+                ("pending", ctypes.c_int64), ("inflight_bytes", ctypes.c_int64), ("fill_ms", ctypes.c_double),
+                ("run_ms", ctypes.c_double)]
 
 
 class _Tuning(ctypes.Structure):
