@@ -295,6 +295,8 @@ def main():
     ap.add_argument("--no-fp64-check", action="store_true", help="skip the FP32-vs-FP64 per-read error report")
     ap.add_argument("--no-hdp", action="store_true", help="skip the HDP-emission configuration")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch tuning")
+    ap.add_argument("--tuning", default="", help="fixed launch tuning instead of the autotuner, e.g. "
+                                                 "long_factor=1.5,stream_split=0.65,waves_per_simd=0")
     ap.add_argument("--no-graph", action="store_true", help="timed steps without graph replay")
     ap.add_argument("--hdp-reads", type=int, default=2048)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving configuration")
@@ -335,7 +337,12 @@ def main():
         _, tm = batch.run(prec, fetch=False)
         log(f"warmup {i}: device {tm['total_ms']:.1f} ms")
     tuning = None
-    if not args.no_autotune:
+    if args.tuning:
+        kv = dict(item.split("=") for item in args.tuning.split(","))
+        batch.set_tuning(**{k: (float(v) if k in ("long_factor", "stream_split") else int(v)) for k, v in kv.items()})
+        tuning = batch.tuning()
+        log(f"tuning: {tuning}")
+    elif not args.no_autotune:
         tuning, tuned_ms = batch.autotune(prec)
         log(f"autotune: {tuning} ({tuned_ms:.1f} ms)")
     if args.no_graph:
