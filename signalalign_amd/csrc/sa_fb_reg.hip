@@ -901,6 +901,16 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
 
 template <typename T, int CPT, int EK>
 static hipError_t fwd_reg_inst(const Args<T> &a, const int32_t *order, int n, int nw, size_t shmem, hipStream_t st) {
+    if constexpr (CPT == 1) {
+        if (nw == 2) {
+            hipLaunchKernelGGL((forward_reg_kernel<T, CPT, EK, 2>), dim3(n), dim3(128), shmem, st, a, order);
+            return hipGetLastError();
+        }
+        if (nw == 3) {
+            hipLaunchKernelGGL((forward_reg_kernel<T, CPT, EK, 3>), dim3(n), dim3(192), shmem, st, a, order);
+            return hipGetLastError();
+        }
+    }
     if constexpr (CPT <= 2) {
         if (nw == 4) {
             hipLaunchKernelGGL((forward_reg_kernel<T, CPT, EK, 4>), dim3(n), dim3(256), shmem, st, a, order);
@@ -936,7 +946,8 @@ template <typename T>
 hipError_t launch_forward_reg(const Args<T> &a, const int32_t *order, int n, int cpt, int nw, int ek,
                               size_t shmem, hipStream_t st) {
     if (n == 0) return hipSuccess;
-    if (nw != 1 && nw != 4) return hipErrorInvalidValue;
+    if (nw < 1 || nw > 4) return hipErrorInvalidValue;
+    if ((nw == 2 || nw == 3) && cpt != 1) return hipErrorInvalidValue;
     if (nw == 4 && cpt > 2) return hipErrorInvalidValue;
     switch (cpt) {
     case 1: return fwd_reg_ek<T, 1>(a, order, n, nw, ek, shmem, st);

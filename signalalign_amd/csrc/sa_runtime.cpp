@@ -917,10 +917,12 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
             if (64 * kCpt[k] >= sp + kCpt[k] - 1) return kCpt[k];
         return 0;
     };
-    auto cpt4 = [&](int sp) {
-        for (int k = 0; k < 2; k++)
-            if (256 * kCpt[k] >= sp + kCpt[k] - 1) return kCpt[k];
-        return 0;
+    /* multi-wavefront forward shapes, narrowest first: (waves, cpt) */
+    static const int kMW[4][2] = {{2, 1}, {3, 1}, {4, 1}, {4, 2}};
+    auto mw_shape = [&](int sp) {
+        for (int k = 0; k < 4; k++)
+            if (64 * kMW[k][0] * kMW[k][1] >= sp + kMW[k][1] - 1) return k;
+        return -1;
     };
     std::vector<int64_t> nd(nt);
     for (int64_t i = 0; i < nt; i++) nd[i] = (int64_t)b->tasks[i].lX + b->tasks[i].lY;
@@ -933,22 +935,23 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
         long_n = nt <= 2 * (int64_t)std::max(ctx->n_cu, 1) ? 0
                : lf < 0.0 ? INT64_MAX
                           : std::max<int64_t>(20000, (int64_t)(lf * (double)med));
-        const char *fw = getenv("SA_FWD_WAVES"); /* "1" or "4": one forward shape for every sub-alignment */
+        const char *fw = getenv("SA_FWD_WAVES"); /* "1": one wavefront per forward sweep; "4": several, always */
         if (fw && std::strcmp(fw, "1") == 0) long_n = INT64_MAX;
         if (fw && std::strcmp(fw, "4") == 0) long_n = 0;
     }
-    /* forward group key: 0 = LDS; 1 + k = one wavefront, kCpt[k]; 7 + k = four wavefronts, kCpt[k] */
-    constexpr int kFG = 9, kBG = 7;
+    /* forward group key: 0 = LDS; 1 + k = one wavefront, kCpt[k]; 7 + k = kMW[k] (the
+       narrowest several-wavefront window that holds the band) */
+    constexpr int kFG = 11, kBG = 7;
     std::vector<int> fgrp(nt), bgrp(nt);
     for (int64_t i = 0; i < nt; i++) {
         int f = 0, bb = 0;
         if (!force_lds) {
-            const int c1 = cpt1(span[i]), c4 = cpt4(span[i]);
+            const int c1 = cpt1(span[i]), mw = mw_shape(span[i]);
             if (c1) {
                 bb = 1 + (int)(std::find(kCpt, kCpt + 6, c1) - kCpt);
                 f = bb;
             }
-            if (c4 && nd[i] >= long_n) f = 7 + (c4 - 1);
+            if (mw >= 0 && nd[i] >= long_n) f = 7 + mw;
         }
         fgrp[i] = f;
         bgrp[i] = bb;
@@ -1057,7 +1060,8 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
         }
         if (L.group < 7)
             return sa::launch_forward_reg<T>(a, d_to + L.off, (int)L.n, kCpt[L.group - 1], 1, ek, shmem1, sg);
-        return sa::launch_forward_reg<T>(a, d_to + L.off, (int)L.n, kCpt[L.group - 7], 4, ek, 0, sg);
+        return sa::launch_forward_reg<T>(a, d_to + L.off, (int)L.n, kMW[L.group - 7][1], kMW[L.group - 7][0], ek, 0,
+                                         sg);
     };
     auto launch_bwd = [&](const Launch &L, hipStream_t sg) -> hipError_t {
         if (L.group == 0) {
@@ -1117,8 +1121,8 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
             (void)hipEventElapsedTime(&t1, ctx->ev[2], ctx->gev[e + 1]);
             sa_launch li;
             li.direction = dir;
-            li.cpt = L.group == 0 ? 0 : dir == 0 && L.group >= 7 ? kCpt[L.group - 7] : kCpt[L.group - 1];
-            li.waves = dir == 0 && L.group >= 7 ? 4 : 1;
+            li.cpt = L.group == 0 ? 0 : dir == 0 && L.group >= 7 ? kMW[L.group - 7][1] : kCpt[L.group - 1];
+            li.waves = dir == 0 && L.group >= 7 ? kMW[L.group - 7][0] : 1;
             li.stream = L.lane;
             li.items = L.n;
             li.cells = L.cells;
