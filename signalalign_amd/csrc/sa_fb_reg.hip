@@ -543,16 +543,17 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
 
 /* Sequential log-add chain of dpDiagonal_dotProduct (impl/pairwiseAligner.c:1014-1027)
  * over the terms held CPT per lane in x order.  Terms at least 7.5 below the running
- * maximum are exact no-ops of logAdd and are skipped; terms more than 40 below the
- * largest term are skipped too (their combined weight is below 1e-15 of the total).
- * The remaining few terms are visited in order with a ballot walk. */
+ * maximum are exact no-ops of logAdd and are skipped; terms more than 24 below the
+ * largest term are skipped too (each weighs under 4e-11 of the total; together they move
+ * the log total by under 1e-8).  The remaining few terms are visited in order with a
+ * ballot walk. */
 template <int CPT>
 __device__ __forceinline__ double chain_reg(const double (&t)[CPT]) {
     double mx = -__builtin_huge_val();
 #pragma unroll
     for (int c = 0; c < CPT; c++) mx = t[c] > mx ? t[c] : mx;
     mx = wave_max(mx);
-    const double cut = mx - 40.0;
+    const double cut = mx - 24.0;
     unsigned long long act[CPT];
     unsigned long long any = 0;
 #pragma unroll
