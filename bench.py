@@ -365,6 +365,9 @@ def main():
             f"(prep {tm['prep_ms']:.1f} fwd {tm['forward_ms']:.1f} "
             f"bwd {tm['backward_ms']:.1f} collect {tm['collect_ms']:.1f}), wall {wall_ms[-1]:.1f} ms, "
             f"{cells / tm['total_ms'] / 1e6:.1f} GCUPS")
+    for L in tms[-1]["launches"]:
+        log(f"  launch stream {L['stream']} {'fwd' if L['direction'] == 0 else 'bwd'} cpt {L['cpt']} waves "
+            f"{L['waves']} items {L['items']} cells {L['cells']} start {L['start_ms']:.1f} ms dur {L['ms']:.1f} ms")
     n_pairs = sum(len(r.pairs) for r in res)
     ms = float(np.mean(dev_ms))
     gcups = cells / (ms * 1e-3) / 1e9

@@ -47,7 +47,10 @@ struct WinDev {
     int32_t TF;    /* tracedBackFrom: posteriors for T < d <= TF */
     int32_t last;  /* atEnd */
     int32_t out_cap;
+    int32_t span;  /* positions a register window must cover over (T, D] (set by prep) */
+    int32_t pad;
     int64_t out_off;
+    int64_t cells; /* cells of the diagonals (T, TF] (set by prep) */
 };
 
 SA_HD int64_t div2(int64_t v) { return v / 2; } /* C truncation, as the reference */
@@ -129,7 +132,10 @@ SA_HD int32_t window_schedule(int64_t N, int64_t min_diags, int64_t traceback, i
         w.TF = (int32_t)TF;
         w.last = last;
         w.out_cap = 0;
+        w.span = 1;
+        w.pad = 0;
         w.out_off = 0;
+        w.cells = 0;
         out[n++] = w;
         if (last) break;
         T = TF;

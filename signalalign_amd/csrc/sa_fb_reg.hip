@@ -538,6 +538,345 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
 }
 
 /* ------------------------------------------------------------------------ */
+/* forward, one wavefront, window width following the band                  */
+/* ------------------------------------------------------------------------ */
+
+/* Register-window class of a block of 64 diagonals (the narrowest of 1, 2, 3 cells per
+ * lane that holds the band there), limited to the widest window of the launch; a
+ * launch wider than three cells per lane keeps its width. */
+__device__ __forceinline__ int window_class(int stored, int cmax) {
+    if (cmax > 3) return cmax;
+    return stored < 1 || stored > cmax ? cmax : stored;
+}
+
+/* What a one-wavefront forward sweep carries from one window width to the next. */
+struct FwdRun {
+    int d;        /* next diagonal to compute */
+    int X0;       /* window of the previous segment, its width */
+    int wprev;
+    int dc;       /* first diagonal of the current block of 64 (plane offsets, classes) */
+    int md_doff, md_dxy;
+    double md_foff;
+    int64_t run, runxy;
+    double off1, off2;
+    int lo1, lo2;
+    int ncls;     /* stored class byte of the next block */
+    bool bad;
+};
+
+/* Diagonals st.d.. of one sub-alignment with CPT cells per lane, until the band's class
+ * changes at a block boundary (returns the new class; the states of the last two
+ * diagonals are left in xs, indexed by position - X0) or the sweep ends (returns 0).
+ * xs holds 6 planes of stride xsw. */
+template <typename T, int CPT, int EK>
+__device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, FwdRun &st, T *xs, int xsw,
+                                           int cmax, const LutEntry<T> *lut) {
+    constexpr bool NORM = sizeof(T) == 4;
+    constexpr int W = 64 * CPT;
+    const Model<T> &m = a.m;
+    const int lane = threadIdx.x;
+    const int lX = t.lX, lY = t.lY, N = lX + lY;
+    const int32_t *xlo = a.xlo + t.diag_off;
+    const int32_t *xhi = a.xhi + t.diag_off;
+    const int8_t *dfl = a.dflag + t.diag_off;
+    const int8_t *dcl = a.dcls + t.diag_off;
+    int32_t *doff = a.doff + t.diag_off;
+    int32_t *doffxy = a.doffxy + t.diag_off;
+    double *foff = a.foff + t.diag_off;
+    const int32_t *kid = a.kid + t.pos_off;
+    const int64_t evc = t.evc_off;
+    T *fM = a.fM + t.cell_off;
+    T *fX = a.fX + t.xy_off;
+    T *fY = a.fY + t.xy_off;
+    const T s = (T)(m.kind == 1 ? 1.0 : t.scale / t.var);
+    const T iv = (T)(1.0 / t.var);
+    const T tMC = m.trans[0], tMFX = m.trans[1], tMFY = m.trans[2], tGOX = m.trans[3], tGOY = m.trans[4];
+    const T tGEX = m.trans[5], tGEY = m.trans[6], tSWX = m.trans[7];
+    const T NI = ninf<T>();
+    const bool useSWX = m.use_swx != 0;
+
+    KCon<T, EK> kc[CPT];
+    ECon<T, EK> ev[CPT];
+    T pM[CPT], pX[CPT], pY[CPT], qM[CPT], qX[CPT], qY[CPT];
+    T LpM, LpX, LpY;
+    int d = st.d;
+    int X0;
+    if (d == 0) {
+        /* diagonal 0: start or ragged-start values (impl/stateMachine.c:1022-1031) */
+        X0 = 0;
+        const int lo0 = ld_scalar(xlo, 0), hi0 = ld_scalar(xhi, 0), f0 = ld_scalar_byte(dfl, 0);
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            const int x = CPT * lane + c;
+            const bool v = x >= lo0 && x <= hi0;
+            pM[c] = v ? (t.ragged_left ? NI : (T)0) : NI;
+            pX[c] = v ? (t.ragged_left ? (T)0 : NI) : NI;
+            pY[c] = pX[c];
+            qM[c] = NI; qX[c] = NI; qY[c] = NI;
+            if (v) {
+                fM[x - lo0] = pM[c];
+                if (f0) { fX[x - lo0] = pX[c]; fY[x - lo0] = pY[c]; }
+            }
+            load_kcon(kc[c], m, kid, x - 1, x >= 1 && x <= lX);
+            const int j = 1 - x - 1;
+            load_econ(ev[c], a, evc + j, j >= 0 && j < lY);
+        }
+        if (lane == 0) { st.md_doff = 0; st.md_dxy = f0 ? 0 : -1; st.md_foff = 0.0; }
+        st.run = hi0 - lo0 + 1;
+        st.runxy = f0 ? st.run : 0;
+        st.off1 = 0.0; st.off2 = 0.0;
+        st.lo1 = lo0; st.lo2 = lo0;
+        LpM = NI; LpX = NI; LpY = NI;
+        d = 1;
+    } else {
+        /* a new width: the window starts at the leftmost cell the diagonal and the two
+           before it hold; the states of d-1 and d-2 come from xs */
+        const int lo = ld_scalar(xlo, d);
+        X0 = lo < st.lo1 ? lo : st.lo1;
+        X0 = X0 < st.lo2 ? X0 : st.lo2;
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            const int x = X0 + CPT * lane + c;
+            const int o = x - st.X0;
+            const bool in = o >= 0 && o < st.wprev;
+            const int oi = in ? o : 0;
+            pM[c] = in ? xs[0 * xsw + oi] : NI; pX[c] = in ? xs[1 * xsw + oi] : NI;
+            pY[c] = in ? xs[2 * xsw + oi] : NI; qM[c] = in ? xs[3 * xsw + oi] : NI;
+            qX[c] = in ? xs[4 * xsw + oi] : NI; qY[c] = in ? xs[5 * xsw + oi] : NI;
+            load_kcon(kc[c], m, kid, x - 1, x >= 1 && x <= lX);
+            const int jj = d - x - 1;
+            load_econ(ev[c], a, evc + jj, jj >= 0 && jj < lY);
+        }
+        {
+            const int o = X0 + CPT * lane - 1 - st.X0; /* middle of the lane's first cell */
+            const bool in = o >= 0 && o < st.wprev;
+            const int oi = in ? o : 0;
+            LpM = in ? xs[3 * xsw + oi] : NI; LpX = in ? xs[4 * xsw + oi] : NI; LpY = in ? xs[5 * xsw + oi] : NI;
+        }
+#pragma unroll
+        for (int c = 0; c < CPT; c++) { sink(kc[c]); sink(ev[c]); }
+        __syncthreads(); /* xs is read before the next segment writes it */
+    }
+    int nlo = 0, nhi = -1, nfl = 0;
+    if (d <= N) { nlo = ld_scalar(xlo, d); nhi = ld_scalar(xhi, d); nfl = ld_scalar_byte(dfl, d); }
+    ECon<T, EK> pre;
+    load_econ_uniform(pre, a, evc, d - X0, d - X0 >= 0 && d - X0 < lY);
+    bool fresh = true;
+    bool bad = st.bad;
+    constexpr bool PF = EK == 2;
+    HdpFetch<T> gp[PF ? CPT : 1];
+    bool have_gp = false;
+    int64_t run = st.run, runxy = st.runxy;
+    double off1 = st.off1, off2 = st.off2;
+    int lo1 = st.lo1, lo2 = st.lo2;
+    int dc = st.dc;
+    int md_doff = st.md_doff, md_dxy = st.md_dxy;
+    double md_foff = st.md_foff;
+    int ncls = st.ncls;
+    int next = 0;
+
+    for (; d <= N; d++) {
+        int j = d - dc;
+        const int lo = nlo, hi = nhi, fl = nfl;
+        if (j == 64) {
+            if (dc + lane <= N) { doff[dc + lane] = md_doff; doffxy[dc + lane] = md_dxy; foff[dc + lane] = md_foff; }
+            dc += 64;
+            j = 0;
+            const int want = window_class(ncls & 7, cmax);
+            ncls = d + 64 <= N ? ld_scalar_byte(dcl, d + 64) : 0;
+            if (want != CPT) {
+                next = want;
+                break;
+            }
+        }
+        if (d < N) { nlo = ld_scalar(xlo, d + 1); nhi = ld_scalar(xhi, d + 1); nfl = ld_scalar_byte(dfl, d + 1); }
+        /* band bounds are non-decreasing; anything else goes to the LDS kernels.  The
+           sweep runs on to its end (its results are discarded) so that the loop has a
+           single exit per segment: early returns cost the loop its register assignment. */
+        bad |= lo < X0;
+        if (hi > X0 + W - 1) {
+            /* slide the window as far as the two previous diagonals allow */
+            const int minlo = lo1 < lo2 ? lo1 : lo2;
+            int smax = (minlo - X0) / CPT;
+            const int sreq = (hi - (X0 + W - 1) + CPT - 1) / CPT;
+            bad |= smax < sreq;
+            smax = smax < 0 ? 0 : smax;
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                pM[c] = shfl_idx(pM[c], lane + smax, NI); pX[c] = shfl_idx(pX[c], lane + smax, NI);
+                pY[c] = shfl_idx(pY[c], lane + smax, NI);
+                qM[c] = shfl_idx(qM[c], lane + smax, NI); qX[c] = shfl_idx(qX[c], lane + smax, NI);
+                qY[c] = shfl_idx(qY[c], lane + smax, NI);
+            }
+            LpM = shr1(qM[CPT - 1], NI); LpX = shr1(qX[CPT - 1], NI); LpY = shr1(qY[CPT - 1], NI);
+            X0 += smax * CPT;
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                const int x = X0 + CPT * lane + c;
+                load_kcon(kc[c], m, kid, x - 1, x >= 1 && x <= lX);
+                const int jj = d - x - 1;
+                load_econ(ev[c], a, evc + jj, jj >= 0 && jj < lY);
+            }
+#pragma unroll
+            for (int c = 0; c < CPT; c++) { sink(kc[c]); sink(ev[c]); }
+            fresh = true;
+            have_gp = false;
+        }
+        if (!fresh) {
+            /* cell x on d reads event d-x-1: its left neighbour's event of d-1 */
+            const ECon<T, EK> inc = shr1(ev[CPT - 1], pre);
+#pragma unroll
+            for (int c = CPT - 1; c >= 1; c--) ev[c] = ev[c - 1];
+            ev[0] = inc;
+        }
+        fresh = false;
+        {
+            const int jn = d - X0; /* event of the first cell of the window on d+1 */
+            load_econ_uniform(pre, a, evc, jn, jn >= 0 && jn < lY);
+        }
+        const T LM = shr1(pM[CPT - 1], NI), LX = shr1(pX[CPT - 1], NI), LY = shr1(pY[CPT - 1], NI);
+        const T dl2 = NORM ? (T)(off2 - off1) : (T)0;
+        /* the rows are renormalised every 8th diagonal, so d-2 and d-1 carry different
+           offsets only on the diagonal after one */
+        const bool shift2 = NORM && off2 != off1;
+        const bool renorm = NORM && (d & 7) == 0;
+        T nM[CPT], nX[CPT], nY[CPT];
+        T mx = NI;
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            const int x = X0 + CPT * lane + c;
+            const bool valid = x >= lo && x <= hi;
+            const T lM = c ? pM[c - 1] : LM, lXv = c ? pX[c - 1] : LX, lYv = c ? pY[c - 1] : LY;
+            T mM = c ? qM[c - 1] : LpM, mXv = c ? qX[c - 1] : LpX, mYv = c ? qY[c - 1] : LpY;
+            if (shift2) { mM += dl2; mXv += dl2; mYv += dl2; }
+            const T eX = em_gapx<T>(m, kc[c]);
+            T eM, eY;
+            if (PF && have_gp) {
+                eM = hdp_finish(m, gp[PF ? c : 0], iv);
+                eY = eM;
+            } else {
+                em_both<T>(m, kc[c], ev[c], s, iv, eM, eY);
+            }
+            /* lower -> gapX, middle -> match, upper -> gapY */
+            T X = lM + (eX + tGOX);
+            X = logadd_lut(X, lXv + (eX + tGEX), lut);
+            if (useSWX) X = logadd_lut(X, lYv + (eX + tSWX), lut);
+            T M = mM + (eM + tMC);
+            M = logadd_lut(M, mXv + (eM + tMFX), lut);
+            M = logadd_lut(M, mYv + (eM + tMFY), lut);
+            T Y = pM[c] + (eY + tGOY);
+            Y = logadd_lut(Y, pY[c] + (eY + tGEY), lut);
+            nM[c] = valid ? M : NI;
+            nX[c] = valid ? X : NI;
+            nY[c] = valid ? Y : NI;
+            if (renorm) {
+                T m3 = nM[c] > nX[c] ? nM[c] : nX[c];
+                m3 = m3 > nY[c] ? m3 : nY[c];
+                mx = m3 > mx ? m3 : mx;
+            }
+        }
+        double offd = off1;
+        if (renorm) {
+            mx = wave_max(mx);
+            if (mx > NI) {
+#pragma unroll
+                for (int c = 0; c < CPT; c++) { nM[c] -= mx; nX[c] -= mx; nY[c] -= mx; }
+                offd = off1 + (double)mx;
+            }
+        }
+        if constexpr (PF) {
+            /* HDP: the density gathers of diagonal d+1 are issued at the end of diagonal
+               d, before its stores, and used a diagonal later.  The events of d+1 are
+               this diagonal's shifted one lane; the first lane's comes from `pre` */
+            const ECon<T, EK> inc = shr1(ev[CPT - 1], pre);
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                const T evm = c ? ev[c - 1].m : inc.m;
+                gp[c] = hdp_fetch(m, kc[c].row, kc[c].hmu, evm, s);
+            }
+            have_gp = true;
+        }
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            const int x = X0 + CPT * lane + c;
+            if (x >= lo && x <= hi) {
+                fM[run + (x - lo)] = nM[c];
+                if (fl) { fX[runxy + (x - lo)] = nX[c]; fY[runxy + (x - lo)] = nY[c]; }
+            }
+        }
+        if (lane == j) { md_doff = (int32_t)run; md_dxy = fl ? (int32_t)runxy : -1; md_foff = offd; }
+        run += hi - lo + 1;
+        if (fl) runxy += hi - lo + 1;
+        LpM = LM; LpX = LX; LpY = LY;
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            qM[c] = pM[c]; qX[c] = pX[c]; qY[c] = pY[c];
+            pM[c] = nM[c]; pX[c] = nX[c]; pY[c] = nY[c];
+        }
+        off2 = off1;
+        off1 = offd;
+        lo2 = lo1;
+        lo1 = lo;
+    }
+    if (next) {
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            const int i = CPT * lane + c;
+            xs[0 * xsw + i] = pM[c]; xs[1 * xsw + i] = pX[c]; xs[2 * xsw + i] = pY[c];
+            xs[3 * xsw + i] = qM[c]; xs[4 * xsw + i] = qX[c]; xs[5 * xsw + i] = qY[c];
+        }
+        __syncthreads();
+    }
+    st.d = d;
+    st.X0 = X0;
+    st.wprev = W;
+    st.dc = dc;
+    st.md_doff = md_doff; st.md_dxy = md_dxy; st.md_foff = md_foff;
+    st.run = run; st.runxy = runxy;
+    st.off1 = off1; st.off2 = off2;
+    st.lo1 = lo1; st.lo2 = lo2;
+    st.ncls = ncls;
+    st.bad = bad;
+    return next;
+}
+
+/* One wavefront per sub-alignment; the window runs at 1, 2 or 3 cells per lane as the
+ * band's width allows (block classes from prep_kernel), never wider than CMAX. */
+template <typename T, int CMAX, int EK>
+__global__ void __launch_bounds__(64) forward_sw_kernel(Args<T> a, const int32_t *__restrict__ order) {
+    extern __shared__ __align__(16) unsigned char dsm[];
+    T *xs = reinterpret_cast<T *>(dsm); /* [6][64 * CMAX]: states across a width change */
+    __shared__ LutEntry<T> lut[16];
+    lut_fill(lut, (int)threadIdx.x);
+    __syncthreads();
+    const TaskDev t = a.tasks[order[blockIdx.x]];
+    const int lane = threadIdx.x;
+    const int N = t.lX + t.lY;
+    const int8_t *dcl = a.dcls + t.diag_off;
+    FwdRun st;
+    st.d = 0; st.X0 = 0; st.wprev = 0; st.dc = 0;
+    st.md_doff = 0; st.md_dxy = -1; st.md_foff = 0.0;
+    st.run = 0; st.runxy = 0; st.off1 = 0.0; st.off2 = 0.0; st.lo1 = 0; st.lo2 = 0;
+    st.ncls = 64 <= N ? ld_scalar_byte(dcl, 64) : 0;
+    st.bad = false;
+    int cls = window_class(ld_scalar_byte(dcl, 0) & 7, CMAX);
+    while (cls) {
+        if constexpr (CMAX == 1 || CMAX > 3) {
+            cls = fwd_segment<T, CMAX, EK>(a, t, st, xs, 64 * CMAX, CMAX, lut);
+        } else {
+            if (cls == 1) cls = fwd_segment<T, 1, EK>(a, t, st, xs, 64 * CMAX, CMAX, lut);
+            else if (CMAX == 3 && cls == 2) cls = fwd_segment<T, 2, EK>(a, t, st, xs, 64 * CMAX, CMAX, lut);
+            else cls = fwd_segment<T, CMAX, EK>(a, t, st, xs, 64 * CMAX, CMAX, lut);
+        }
+    }
+    int32_t *doff = a.doff + t.diag_off;
+    int32_t *doffxy = a.doffxy + t.diag_off;
+    double *foff = a.foff + t.diag_off;
+    if (st.dc + lane <= N) { doff[st.dc + lane] = st.md_doff; doffxy[st.dc + lane] = st.md_dxy; foff[st.dc + lane] = st.md_foff; }
+    if (st.bad && lane == 0) atomicOr(a.err, ERR_WINDOWS);
+}
+
+/* ------------------------------------------------------------------------ */
 /* backward + total refresh + posterior pairs                                 */
 /* ------------------------------------------------------------------------ */
 
@@ -579,16 +918,27 @@ __device__ __forceinline__ double chain_reg(const double (&t)[CPT]) {
     return acc;
 }
 
+/* What a backward sweep carries from one window width to the next. */
+struct BwdRun {
+    int e;          /* next diagonal to compute */
+    int X0, wprev;  /* window of the previous segment, its width */
+    int ncls;       /* stored class byte of the next block down */
+    double ob1, ob2;
+    int hi1, hi2;
+    int cursor, posterior_count;
+    double total;
+    bool bad;
+};
+
+/* Diagonals st.e down to T+1 of one traceback window with CPT cells per lane, until the
+ * band's class changes at a block boundary (returns the new class; the states of e+1
+ * and e+2 are left in xs, indexed by position - X0) or the window ends (returns 0).
+ * xs holds 4 planes of stride xsw. */
 template <typename T, int CPT, int EK>
-__global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32_t *__restrict__ order) {
+__device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const WinDev &Wd, const TaskDev &t,
+                                           BwdRun &st, T *xs, int xsw, int cmax, const LutEntry<T> *lut) {
     constexpr bool NORM = sizeof(T) == 4;
     constexpr int W = 64 * CPT;
-    __shared__ LutEntry<T> lut[16];
-    lut_fill(lut, (int)threadIdx.x);
-    __syncthreads();
-    const int wi = order[blockIdx.x];
-    const WinDev Wd = a.wins[wi];
-    const TaskDev t = a.tasks[Wd.task];
     const Model<T> &m = a.m;
     const int lane = threadIdx.x;
     const int lX = t.lX, lY = t.lY;
@@ -597,6 +947,7 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
     const int32_t *doff = a.doff + t.diag_off;
     const int32_t *doffxy = a.doffxy + t.diag_off;
     const double *foff = a.foff + t.diag_off;
+    const int8_t *dcl = a.dcls + t.diag_off;
     const int32_t *kid = a.kid + t.pos_off;
     const int64_t evc = t.evc_off;
     const T *fM = a.fM + t.cell_off;
@@ -615,38 +966,67 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
     const T e1 = ragged_end ? m.rendv[1] : m.endv[1];
     const T e2 = ragged_end ? m.rendv[2] : m.endv[2];
     int32_t *out = a.out + 3 * Wd.out_off;
-    int cursor = 0;
-    int posterior_count = 0;
-    double total = -__builtin_huge_val();
 
+    const int es = st.e;
     /* band bounds and plane offsets of each diagonal come through the scalar cache,
        one diagonal ahead */
-    int nlo = ld_scalar(xlo, D), nhi = ld_scalar(xhi, D), ndo = ld_scalar(doff, D);
-    double nfo = ld_scalar(foff, D);
-    const int loD = nlo, hiD = nhi;
-    int X0 = hiD - W + 1;
+    int nlo = ld_scalar(xlo, es), nhi = ld_scalar(xhi, es), ndo = ld_scalar(doff, es);
+    double nfo = ld_scalar(foff, es);
+    int X0;
     KCon<T, EK> kc[CPT];
     ECon<T, EK> ev[CPT];
     T pM[CPT], pX[CPT], pY[CPT], qM[CPT];
-    T RpM = NI; /* lane i+1's first cell of e+2, match state */
+    T RpM; /* lane i+1's first cell of e+2, match state */
+    if (es == D) {
+        X0 = nhi - W + 1;
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            pM[c] = NI; pX[c] = NI; pY[c] = NI; qM[c] = NI;
+        }
+        RpM = NI;
+    } else {
+        /* a new width: the window ends at the rightmost cell of e, e+1 and e+2; their
+           states come from xs */
+        int rh = nhi > st.hi1 ? nhi : st.hi1;
+        rh = rh > st.hi2 ? rh : st.hi2;
+        X0 = rh - W + 1;
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            const int o = X0 + CPT * lane + c - st.X0;
+            const bool in = o >= 0 && o < st.wprev;
+            const int oi = in ? o : 0;
+            pM[c] = in ? xs[0 * xsw + oi] : NI; pX[c] = in ? xs[1 * xsw + oi] : NI;
+            pY[c] = in ? xs[2 * xsw + oi] : NI; qM[c] = in ? xs[3 * xsw + oi] : NI;
+        }
+        const int o = X0 + CPT * lane + CPT - st.X0;
+        const bool in = o >= 0 && o < st.wprev;
+        RpM = in ? xs[3 * xsw + (in ? o : 0)] : NI;
+        __syncthreads(); /* xs is read before the next segment writes it */
+    }
 #pragma unroll
     for (int c = 0; c < CPT; c++) {
         const int x = X0 + CPT * lane + c;
         load_kcon(kc[c], m, kid, x - 1, x >= 1 && x <= lX);
-        const int j = D - x;
+        const int j = es - x;
         load_econ(ev[c], a, evc + j, j >= 0 && j < lY);
-        pM[c] = NI; pX[c] = NI; pY[c] = NI; qM[c] = NI;
     }
-    (void)loD;
+    if (es != D) {
+#pragma unroll
+        for (int c = 0; c < CPT; c++) { sink(kc[c]); sink(ev[c]); }
+    }
     ECon<T, EK> pre;
     {
-        const int jn = (D - 1) - (X0 + W - 1); /* event of lane 63's last cell on D-1 */
+        const int jn = (es - 1) - (X0 + W - 1); /* event of lane 63's last cell on es-1 */
         load_econ_uniform(pre, a, evc, jn, jn >= 0 && jn < lY);
     }
-    double ob1 = 0.0, ob2 = 0.0; /* offsets of B[e+1], B[e+2] */
-    int hi1 = -1, hi2 = -1;      /* xhi of e+1, e+2 (-1: outside the window of diagonals) */
+    double ob1 = st.ob1, ob2 = st.ob2; /* offsets of B[e+1], B[e+2] */
+    int hi1 = st.hi1, hi2 = st.hi2;    /* xhi of e+1, e+2 (-1: outside the window of diagonals) */
+    int cursor = st.cursor, posterior_count = st.posterior_count;
+    double total = st.total;
+    int ncls = st.ncls;
     bool fresh = true;
-    bool bad = false;
+    bool bad = st.bad;
+    int next = 0;
     /* HDP: the match and extra-event emissions are the same density, and the match
        emission of cell x on e (k-mer x+1, event e-x) is the extra-event emission its
        right neighbour computed on e+1; so one density per cell, its gathers issued a
@@ -656,7 +1036,17 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
     T eYp[PF ? CPT : 1];
     bool have_gp = false, have_eY = false;
 
-    for (int e = D; e > TB; e--) {
+    int e = es;
+    for (; e > TB; e--) {
+        if ((e & 63) == 63 && e != es) {
+            /* top of block e/64: its class decides the width */
+            const int want = window_class((ncls >> 3) & 7, cmax);
+            ncls = e - 64 > TB ? ld_scalar_byte(dcl, e - 127) : 0;
+            if (want != CPT) {
+                next = want;
+                break;
+            }
+        }
         const int lo = nlo, hi = nhi, dof = ndo;
         const double fo = nfo;
         if (e - 1 > TB) {
@@ -892,8 +1282,62 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
         hi2 = hi1;
         hi1 = hi;
     }
-    if (lane == 0) a.win_count[wi] = cursor;
-    if (bad && lane == 0) atomicOr(a.err, ERR_WINDOWS);
+    if (next) {
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            const int i = CPT * lane + c;
+            xs[0 * xsw + i] = pM[c]; xs[1 * xsw + i] = pX[c]; xs[2 * xsw + i] = pY[c]; xs[3 * xsw + i] = qM[c];
+        }
+        __syncthreads();
+    }
+    st.e = e;
+    st.X0 = X0;
+    st.wprev = W;
+    st.ncls = ncls;
+    st.ob1 = ob1; st.ob2 = ob2;
+    st.hi1 = hi1; st.hi2 = hi2;
+    st.cursor = cursor; st.posterior_count = posterior_count;
+    st.total = total;
+    st.bad = bad;
+    return next;
+}
+
+/* One wavefront per traceback window; the window runs at 1, 2 or 3 cells per lane as
+ * the band's width allows (block classes from prep_kernel), never wider than CMAX. */
+template <typename T, int CMAX, int EK>
+__global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32_t *__restrict__ order) {
+    extern __shared__ __align__(16) unsigned char dsm[];
+    T *xs = reinterpret_cast<T *>(dsm); /* [4][64 * CMAX]: states across a width change */
+    __shared__ LutEntry<T> lut[16];
+    lut_fill(lut, (int)threadIdx.x);
+    __syncthreads();
+    const int wi = order[blockIdx.x];
+    const WinDev Wd = a.wins[wi];
+    const TaskDev t = a.tasks[Wd.task];
+    const int lane = threadIdx.x;
+    const int8_t *dcl = a.dcls + t.diag_off;
+    BwdRun st;
+    st.e = Wd.D;
+    st.X0 = 0; st.wprev = 0;
+    const int b0 = Wd.D >> 6;
+    st.ncls = 64 * b0 - 1 > Wd.T ? ld_scalar_byte(dcl, 64 * (b0 - 1)) : 0;
+    st.ob1 = 0.0; st.ob2 = 0.0;
+    st.hi1 = -1; st.hi2 = -1;
+    st.cursor = 0; st.posterior_count = 0;
+    st.total = -__builtin_huge_val();
+    st.bad = false;
+    int cls = window_class((ld_scalar_byte(dcl, 64 * b0) >> 3) & 7, CMAX);
+    while (cls) {
+        if constexpr (CMAX == 1 || CMAX > 3) {
+            cls = bwd_segment<T, CMAX, EK>(a, wi, Wd, t, st, xs, 64 * CMAX, CMAX, lut);
+        } else {
+            if (cls == 1) cls = bwd_segment<T, 1, EK>(a, wi, Wd, t, st, xs, 64 * CMAX, CMAX, lut);
+            else if (CMAX == 3 && cls == 2) cls = bwd_segment<T, 2, EK>(a, wi, Wd, t, st, xs, 64 * CMAX, CMAX, lut);
+            else cls = bwd_segment<T, CMAX, EK>(a, wi, Wd, t, st, xs, 64 * CMAX, CMAX, lut);
+        }
+    }
+    if (lane == 0) a.win_count[wi] = st.cursor;
+    if (st.bad && lane == 0) atomicOr(a.err, ERR_WINDOWS);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -918,7 +1362,8 @@ static hipError_t fwd_reg_inst(const Args<T> &a, const int32_t *order, int n, in
             return hipGetLastError();
         }
     }
-    hipLaunchKernelGGL((forward_reg_kernel<T, CPT, EK, 1>), dim3(n), dim3(64), shmem, st, a, order);
+    const size_t xs = sizeof(T) * 6 * 64 * CPT; /* states across a width change */
+    hipLaunchKernelGGL((forward_sw_kernel<T, CPT, EK>), dim3(n), dim3(64), shmem > xs ? shmem : xs, st, a, order);
     return hipGetLastError();
 }
 
@@ -935,6 +1380,8 @@ static hipError_t fwd_reg_ek(const Args<T> &a, const int32_t *order, int n, int 
 template <typename T, int CPT>
 static hipError_t bwd_reg_ek(const Args<T> &a, const int32_t *order, int n, int ek, size_t shmem,
                              hipStream_t st) {
+    const size_t xs = sizeof(T) * 4 * 64 * CPT; /* states across a width change */
+    shmem = shmem > xs ? shmem : xs;
     switch (ek) {
     case 0: hipLaunchKernelGGL((backward_reg_kernel<T, CPT, 0>), dim3(n), dim3(64), shmem, st, a, order); break;
     case 1: hipLaunchKernelGGL((backward_reg_kernel<T, CPT, 1>), dim3(n), dim3(64), shmem, st, a, order); break;

@@ -56,6 +56,8 @@ struct Args {
     T *evm, *evn, *evl, *evi;
     int32_t *xlo, *xhi, *doff;
     int8_t *dflag;          /* 1 where the forward gapX/gapY states are kept */
+    int8_t *dcls;           /* at d = 64b: register-window classes of diagonals 64b..64b+63
+                               (forward bits 0-2, backward bits 3-5; see prep_kernel) */
     int32_t *doffxy;        /* offset of a kept diagonal in the gapX/gapY planes */
     double *foff;
     int64_t *task_cells;

@@ -142,6 +142,49 @@ __global__ void __launch_bounds__(256) prep_kernel(Args<T> a) {
         a.task_xycells[ti] = s_xy;
         a.task_span[ti] = s_span;
     }
+    /* register-window class of each aligned block of 64 diagonals: the narrowest
+       lane window of the register sweeps (1, 2 or 3 positions per lane; 4: wider)
+       holding the forward span (bits 0-2) and the backward span (bits 3-5) of every
+       diagonal in the block.  The sweeps change window width at block boundaries. */
+    auto span_class = [](int sp) { return sp <= 64 ? 1 : sp <= 127 ? 2 : sp <= 190 ? 3 : 4; };
+    for (int64_t b = tid; b <= N / 64; b += nt) {
+        int cf = 1, cb = 1;
+        const int64_t d1 = min(N, 64 * b + 63);
+        for (int64_t d = 64 * b; d <= d1; d++) {
+            int lmin = XL[d];
+            if (d >= 1) lmin = min(lmin, XL[d - 1]);
+            if (d >= 2) lmin = min(lmin, XL[d - 2]);
+            int hmax = XH[d];
+            if (d + 1 <= N) hmax = max(hmax, XH[d + 1]);
+            if (d + 2 <= N) hmax = max(hmax, XH[d + 2]);
+            cf = max(cf, span_class(XH[d] - lmin + 1));
+            cb = max(cb, span_class(hmax - XL[d] + 1));
+        }
+        a.dcls[t.diag_off + 64 * b] = (int8_t)(cf | cb << 3);
+    }
+    /* per traceback window: the backward span over its diagonals (T, D] (each
+       window's sweep picks the narrowest register window that holds it) and the
+       cells of its posterior diagonals (T, TF] */
+    for (int w = 0; w < s_nwin; w++) {
+        WinDev *W = a.win_slots + t.win_off + w;
+        const int64_t wT = W->T, wD = W->D, wTF = W->TF;
+        __syncthreads();
+        if (tid == 0) { s_span = 1; s_cells = 0; }
+        __syncthreads();
+        int wspan = 1;
+        long long wcells = 0;
+        for (int64_t e = wT + 1 + tid; e <= wD; e += nt) {
+            int hmax = XH[e];
+            if (e + 1 <= N) hmax = max(hmax, XH[e + 1]);
+            if (e + 2 <= N) hmax = max(hmax, XH[e + 2]);
+            wspan = max(wspan, hmax - XL[e] + 1);
+            if (e <= wTF) wcells += XH[e] - XL[e] + 1;
+        }
+        atomicMax(&s_span, wspan);
+        atomicAdd((unsigned long long *)&s_cells, (unsigned long long)wcells);
+        __syncthreads();
+        if (tid == 0) { W->span = s_span; W->cells = s_cells; }
+    }
 }
 
 /* ------------------------------------------------------------------------ */

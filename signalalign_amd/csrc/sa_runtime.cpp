@@ -168,7 +168,7 @@ struct sa_batch {
     /* raw inputs resident in HBM */
     DBuf d_ref, d_events, d_ancx, d_ancy, d_tasks, d_x1, d_y1;
     /* derived */
-    DBuf d_kid, d_evm, d_evn, d_evl, d_evi, d_xlo, d_xhi, d_doff, d_foff, d_dflag, d_doffxy;
+    DBuf d_kid, d_evm, d_evn, d_evl, d_evi, d_xlo, d_xhi, d_doff, d_foff, d_dflag, d_dcls, d_doffxy;
     DBuf d_task_cells, d_task_xycells, d_task_maxw, d_task_span, d_task_nwin, d_win_slots, d_err;
     DBuf d_fM, d_fX, d_fY;
     DBuf d_wins, d_task_order, d_win_order, d_out, d_win_count, d_win_total, d_win_dst, d_packed;
@@ -193,7 +193,7 @@ struct sa_batch {
     ~sa_batch() {
         for (auto &g : gc) g.reset();
         DBuf *all[] = {&d_ref, &d_events, &d_ancx, &d_ancy, &d_tasks, &d_x1, &d_y1, &d_kid, &d_evm, &d_evn,
-                       &d_evl, &d_evi, &d_xlo, &d_xhi, &d_doff, &d_foff, &d_dflag, &d_doffxy, &d_task_cells,
+                       &d_evl, &d_evi, &d_xlo, &d_xhi, &d_doff, &d_foff, &d_dflag, &d_dcls, &d_doffxy, &d_task_cells,
                        &d_task_xycells, &d_task_maxw, &d_task_span,
                        &d_task_nwin, &d_win_slots, &d_err, &d_fM, &d_fX, &d_fY, &d_wins, &d_task_order,
                        &d_win_order, &d_out, &d_win_count, &d_win_total, &d_win_dst, &d_packed};
@@ -822,6 +822,7 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     SA_HIP(ctx, b->d_doff.ensure(sizeof(int32_t) * b->total_diag));
     SA_HIP(ctx, b->d_foff.ensure(sizeof(double) * b->total_diag));
     SA_HIP(ctx, b->d_dflag.ensure(sizeof(int8_t) * b->total_diag + 4)); /* + padding for dword reads */
+    SA_HIP(ctx, b->d_dcls.ensure(sizeof(int8_t) * b->total_diag + 4));
     SA_HIP(ctx, b->d_doffxy.ensure(sizeof(int32_t) * b->total_diag));
     SA_HIP(ctx, b->d_task_cells.ensure(sizeof(int64_t) * nt));
     SA_HIP(ctx, b->d_task_xycells.ensure(sizeof(int64_t) * nt));
@@ -848,6 +849,7 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     a.xlo = b->d_xlo.as<int32_t>(); a.xhi = b->d_xhi.as<int32_t>(); a.doff = b->d_doff.as<int32_t>();
     a.foff = b->d_foff.as<double>();
     a.dflag = b->d_dflag.as<int8_t>();
+    a.dcls = b->d_dcls.as<int8_t>();
     a.doffxy = b->d_doffxy.as<int32_t>();
     a.task_cells = b->d_task_cells.as<int64_t>();
     a.task_xycells = b->d_task_xycells.as<int64_t>();
@@ -956,6 +958,15 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
         fgrp[i] = f;
         bgrp[i] = bb;
     }
+    /* backward group per traceback window: the narrowest register window that holds
+       the band over the window's own diagonals (LDS sub-alignments stay on LDS) */
+    std::vector<int> wgrp(nw);
+    for (int64_t i = 0; i < nt; i++)
+        for (int32_t w = 0; w < nwin[i]; w++) {
+            const int32_t wi = task_first_win[i] + w;
+            const int c = bgrp[i] == 0 ? 0 : cpt1(wins[wi].span);
+            wgrp[wi] = c == 0 ? bgrp[i] : std::min(bgrp[i], 1 + (int)(std::find(kCpt, kCpt + 6, c) - kCpt));
+        }
     /* Streams: sub-alignments whose forward sweep uses four wavefronts (the long
        ones) run on stream 1, LDS sweeps on stream 2, one-wavefront forward sweeps
        on streams 0 and 3 (the sub-alignments holding the first half of their cells,
@@ -995,12 +1006,15 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
         }
         for (int g = 0; g < kBG; g++) {
             Launch L{lane, g, (int64_t)win_order.size(), 0, 0};
-            for (int64_t i = 0; i < nt; i++)
-                if (tlane[i] == lane && bgrp[i] == g) {
-                    L.cells += cells[i];
-                    if (g == 0) lds_maxw = std::max(lds_maxw, maxw[i]);
-                    for (int32_t w = 0; w < nwin[i]; w++) win_order.push_back(task_first_win[i] + w);
-                }
+            for (int64_t i = 0; i < nt; i++) {
+                if (tlane[i] != lane) continue;
+                if (g == 0 && bgrp[i] == 0) lds_maxw = std::max(lds_maxw, maxw[i]);
+                for (int32_t w = 0; w < nwin[i]; w++)
+                    if (wgrp[task_first_win[i] + w] == g) {
+                        win_order.push_back(task_first_win[i] + w);
+                        L.cells += wins[task_first_win[i] + w].cells;
+                    }
+            }
             L.n = (int64_t)win_order.size() - L.off;
             std::stable_sort(win_order.begin() + L.off, win_order.end(), [&](int32_t x, int32_t y) {
                 return (int64_t)(wins[x].D - wins[x].T) * maxw[wins[x].task] >
