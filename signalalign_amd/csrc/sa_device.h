@@ -93,8 +93,23 @@ __device__ __forceinline__ double logadd_lut(double a, double b, const LutEntry<
 __device__ __forceinline__ double flog(double v) { return log(v); }
 __device__ __forceinline__ float flog(float v) { return logf(v); }
 __device__ __forceinline__ double fexp(double v) { return exp(v); }
-__device__ __forceinline__ float fexp(float v) { return expf(v); }
+/* FP32 posteriors: the hardware exp2 of v*log2(e) (relative error about 4e-7 at the
+ * 0.01 threshold's exponent; the FP32 variant's tolerance is 2e-3) */
+__device__ __forceinline__ float fexp(float v) { return __expf(v); }
 
+/* a value every lane holds, made wave-uniform for the compiler (scalar registers,
+ * scalar branches on it) */
+__device__ __forceinline__ float uniform(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ double uniform(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+/* maximum over the wavefront, uniform */
 template <typename T>
 __device__ __forceinline__ T wave_max(T v) {
 #pragma unroll
@@ -102,7 +117,7 @@ __device__ __forceinline__ T wave_max(T v) {
         T u = __shfl_xor(v, o, 64);
         v = u > v ? u : v;
     }
-    return v;
+    return uniform(v);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -441,6 +441,12 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
            offsets only on the diagonal after one */
         const bool shift2 = NORM && off2 != off1;
         const bool renorm = NORM && (d & 7) == 0;
+        if (shift2) {
+            /* d-2 carries its own offset: bring its states to d-1's */
+#pragma unroll
+            for (int c = 0; c < CPT; c++) { qM[c] += dl2; qX[c] += dl2; qY[c] += dl2; }
+            LpM += dl2; LpX += dl2; LpY += dl2;
+        }
         T nM[CPT], nX[CPT], nY[CPT];
         T mx = NI;
 #pragma unroll
@@ -448,8 +454,7 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
             const int x = X0 + CPT * gl + c;
             const bool valid = x >= lo && x <= hi;
             const T lM = c ? pM[c - 1] : LM, lXv = c ? pX[c - 1] : LX, lYv = c ? pY[c - 1] : LY;
-            T mM = c ? qM[c - 1] : LpM, mXv = c ? qX[c - 1] : LpX, mYv = c ? qY[c - 1] : LpY;
-            if (shift2) { mM += dl2; mXv += dl2; mYv += dl2; }
+            const T mM = c ? qM[c - 1] : LpM, mXv = c ? qX[c - 1] : LpX, mYv = c ? qY[c - 1] : LpY;
             const T eX = em_gapx<T>(m, kc[c]);
             T eM, eY;
             if (PF && have_gp) {
@@ -740,6 +745,12 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
            offsets only on the diagonal after one */
         const bool shift2 = NORM && off2 != off1;
         const bool renorm = NORM && (d & 7) == 0;
+        if (shift2) {
+            /* d-2 carries its own offset: bring its states to d-1's */
+#pragma unroll
+            for (int c = 0; c < CPT; c++) { qM[c] += dl2; qX[c] += dl2; qY[c] += dl2; }
+            LpM += dl2; LpX += dl2; LpY += dl2;
+        }
         T nM[CPT], nX[CPT], nY[CPT];
         T mx = NI;
 #pragma unroll
@@ -747,8 +758,7 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
             const int x = X0 + CPT * lane + c;
             const bool valid = x >= lo && x <= hi;
             const T lM = c ? pM[c - 1] : LM, lXv = c ? pX[c - 1] : LX, lYv = c ? pY[c - 1] : LY;
-            T mM = c ? qM[c - 1] : LpM, mXv = c ? qX[c - 1] : LpX, mYv = c ? qY[c - 1] : LpY;
-            if (shift2) { mM += dl2; mXv += dl2; mYv += dl2; }
+            const T mM = c ? qM[c - 1] : LpM, mXv = c ? qX[c - 1] : LpX, mYv = c ? qY[c - 1] : LpY;
             const T eX = em_gapx<T>(m, kc[c]);
             T eM, eY;
             if (PF && have_gp) {
@@ -1106,6 +1116,12 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
             const KCon<T, EK> kn = shl1_match(kc[0]); /* k-mer x of lane's last cell */
             const T dl2 = NORM ? (T)(ob2 - ob1) : (T)0;
             const bool shift2 = NORM && ob2 != ob1;
+            if (shift2) {
+                /* e+2 carries its own offset: bring its match states to e+1's */
+#pragma unroll
+                for (int c = 0; c < CPT; c++) qM[c] += dl2;
+                RpM += dl2;
+            }
             T eYc[PF ? CPT : 1];
             const T eYn = PF && have_eY ? shl1(eYp[0], NI) : NI;
             const bool renorm = NORM && ((D - e) & 7) == 0;
@@ -1117,8 +1133,7 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                 const KCon<T, EK> &kx = c < CPT - 1 ? kc[c + 1] : kn;
                 const T ClowX = c < CPT - 1 ? pX[c + 1] : LowX;
                 const T CupY = pY[c];
-                T CmidM = c < CPT - 1 ? qM[c + 1] : RpM;
-                if (shift2) CmidM += dl2;
+                const T CmidM = c < CPT - 1 ? qM[c + 1] : RpM;
                 const T eX = em_gapx<T>(m, kx);
                 T eM, eY;
                 if constexpr (PF) {

@@ -123,44 +123,42 @@ __global__ void __launch_bounds__(256) prep_kernel(Args<T> a) {
         if (a.dflag[t.diag_off + d]) xy += a.xhi[t.diag_off + d] - a.xlo[t.diag_off + d] + 1;
     atomicAdd((unsigned long long *)&s_xy, (unsigned long long)xy);
     /* window span for the register sweeps: forward at d needs
-       [min xlo(d-2..d), xhi(d)], backward at e needs [xlo(e), max xhi(e..e+2)] */
+       [min xlo(d-2..d), xhi(d)], backward at e needs [xlo(e), max xhi(e..e+2)].
+       Per aligned block of 64 diagonals (one wavefront's diagonals of an iteration)
+       the register-window class: the narrowest lane window of the register sweeps
+       (1, 2 or 3 positions per lane; 4: wider) holding the forward span (bits 0-2)
+       and the backward span (bits 3-5) of every diagonal in the block.  The sweeps
+       change window width at block boundaries. */
     int span = 1;
     const int32_t *XL = a.xlo + t.diag_off, *XH = a.xhi + t.diag_off;
-    for (int64_t d = tid; d <= N; d += nt) {
-        int lmin = XL[d];
-        if (d >= 1) lmin = min(lmin, XL[d - 1]);
-        if (d >= 2) lmin = min(lmin, XL[d - 2]);
-        int hmax = XH[d];
-        if (d + 1 <= N) hmax = max(hmax, XH[d + 1]);
-        if (d + 2 <= N) hmax = max(hmax, XH[d + 2]);
-        span = max(span, XH[d] - lmin + 1);
-        span = max(span, hmax - XL[d] + 1);
-    }
-    atomicMax(&s_span, span);
-    __syncthreads();
-    if (tid == 0) {
-        a.task_xycells[ti] = s_xy;
-        a.task_span[ti] = s_span;
-    }
-    /* register-window class of each aligned block of 64 diagonals: the narrowest
-       lane window of the register sweeps (1, 2 or 3 positions per lane; 4: wider)
-       holding the forward span (bits 0-2) and the backward span (bits 3-5) of every
-       diagonal in the block.  The sweeps change window width at block boundaries. */
     auto span_class = [](int sp) { return sp <= 64 ? 1 : sp <= 127 ? 2 : sp <= 190 ? 3 : 4; };
-    for (int64_t b = tid; b <= N / 64; b += nt) {
+    auto wave_class = [](int c) {
+        return __ballot(c >= 4) ? 4 : __ballot(c >= 3) ? 3 : __ballot(c >= 2) ? 2 : 1;
+    };
+    for (int64_t base = 0; base <= N; base += nt) {
+        const int64_t d = base + tid;
         int cf = 1, cb = 1;
-        const int64_t d1 = min(N, 64 * b + 63);
-        for (int64_t d = 64 * b; d <= d1; d++) {
+        if (d <= N) {
             int lmin = XL[d];
             if (d >= 1) lmin = min(lmin, XL[d - 1]);
             if (d >= 2) lmin = min(lmin, XL[d - 2]);
             int hmax = XH[d];
             if (d + 1 <= N) hmax = max(hmax, XH[d + 1]);
             if (d + 2 <= N) hmax = max(hmax, XH[d + 2]);
-            cf = max(cf, span_class(XH[d] - lmin + 1));
-            cb = max(cb, span_class(hmax - XL[d] + 1));
+            const int fsp = XH[d] - lmin + 1, bsp = hmax - XL[d] + 1;
+            span = max(span, max(fsp, bsp));
+            cf = span_class(fsp);
+            cb = span_class(bsp);
         }
-        a.dcls[t.diag_off + 64 * b] = (int8_t)(cf | cb << 3);
+        cf = wave_class(cf);
+        cb = wave_class(cb);
+        if ((tid & 63) == 0 && d <= N) a.dcls[t.diag_off + d] = (int8_t)(cf | cb << 3);
+    }
+    atomicMax(&s_span, span);
+    __syncthreads();
+    if (tid == 0) {
+        a.task_xycells[ti] = s_xy;
+        a.task_span[ti] = s_span;
     }
     /* per traceback window: the backward span over its diagonals (T, D] (each
        window's sweep picks the narrowest register window that holds it) and the
