@@ -380,7 +380,8 @@ def main():
     for t in tms:
         for L in t["launches"]:
             if L["direction"] == 0:
-                name = (f"sa::forward_reg_kernel<{tname}, {L['cpt']}, {ek}, {L['waves']}>" if L["cpt"]
+                name = (f"sa::forward_sw_kernel<{tname}, {L['cpt']}, {ek}>" if L["cpt"] and L["waves"] == 1
+                        else f"sa::forward_reg_kernel<{tname}, {L['cpt']}, {ek}, {L['waves']}>" if L["cpt"]
                         else f"sa::forward_kernel<{tname}, ...>")
             else:
                 name = (f"sa::backward_reg_kernel<{tname}, {L['cpt']}, {ek}>" if L["cpt"]

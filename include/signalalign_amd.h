@@ -192,8 +192,8 @@ void sa_batch_destroy(sa_batch *batch);
  * (band prep, every forward and backward sweep on its stream, pair compaction) as a
  * HIP graph and later runs of the same batch replay it.  waves_per_simd: occupancy
  * cap of the one-wavefront sweeps (0: none).  long_factor: sub-alignments with
- * lX + lY >= max(20000, long_factor x median) run four wavefronts per forward sweep
- * (0: 2.5; negative: never).  stream_split: fraction of the one-wavefront forward
+ * lX + lY >= max(20000, long_factor x median) run their forward sweep on two to four
+ * wavefronts, the fewest whose window holds the band (0: 2.5; negative: never).  stream_split: fraction of the one-wavefront forward
  * cells (longest sub-alignments first) on stream 0, the rest on stream 3 (0: 0.5). */
 typedef struct {
     int32_t use_graph;
@@ -205,8 +205,9 @@ void sa_tuning_default(sa_tuning *t);
 int sa_batch_set_tuning(sa_batch *batch, const sa_tuning *t);
 int sa_batch_get_tuning(const sa_batch *batch, sa_tuning *t);
 /* Times runs of the batch under candidate tunings (one knob at a time: long_factor,
- * stream_split, waves_per_simd), keeps the fastest on the batch and returns it with
- * its device time.  use_graph is left as set. */
+ * stream_split, waves_per_simd; each the best of two runs of the forward-backward
+ * sweeps, a candidate kept only when at least 1% faster), keeps the fastest on the
+ * batch and returns it with its sweep time.  use_graph is left as set. */
 int sa_batch_autotune(sa_batch *batch, sa_precision precision, sa_tuning *best, double *best_ms);
 
 int64_t sa_result_n_reads(const sa_result *res);

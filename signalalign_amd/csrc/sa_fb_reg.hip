@@ -244,6 +244,13 @@ template <typename T> __device__ __forceinline__ T shfl_idx(T v, int src, T fill
 }
 
 __device__ __forceinline__ int rdl(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+/* lane `lane`'s value (uniform lane index) */
+__device__ __forceinline__ double readlane(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 
 /* ------------------------------------------------------------------------ */
 /* forward                                                                   */
@@ -739,18 +746,8 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
             const int jn = d - X0; /* event of the first cell of the window on d+1 */
             load_econ_uniform(pre, a, evc, jn, jn >= 0 && jn < lY);
         }
-        const T LM = shr1(pM[CPT - 1], NI), LX = shr1(pX[CPT - 1], NI), LY = shr1(pY[CPT - 1], NI);
-        const T dl2 = NORM ? (T)(off2 - off1) : (T)0;
-        /* the rows are renormalised every 8th diagonal, so d-2 and d-1 carry different
-           offsets only on the diagonal after one */
-        const bool shift2 = NORM && off2 != off1;
+        T LM = shr1(pM[CPT - 1], NI), LX = shr1(pX[CPT - 1], NI), LY = shr1(pY[CPT - 1], NI);
         const bool renorm = NORM && (d & 7) == 0;
-        if (shift2) {
-            /* d-2 carries its own offset: bring its states to d-1's */
-#pragma unroll
-            for (int c = 0; c < CPT; c++) { qM[c] += dl2; qX[c] += dl2; qY[c] += dl2; }
-            LpM += dl2; LpX += dl2; LpY += dl2;
-        }
         T nM[CPT], nX[CPT], nY[CPT];
         T mx = NI;
 #pragma unroll
@@ -787,10 +784,17 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
         }
         double offd = off1;
         if (renorm) {
+            /* every 8th diagonal: the largest state becomes 0, its log carried in the
+               diagonal's offset; d-1's states (d-2's on the next diagonal) move with them,
+               so that the two diagonals a cell reads always share one offset */
             mx = wave_max(mx);
             if (mx > NI) {
 #pragma unroll
-                for (int c = 0; c < CPT; c++) { nM[c] -= mx; nX[c] -= mx; nY[c] -= mx; }
+                for (int c = 0; c < CPT; c++) {
+                    nM[c] -= mx; nX[c] -= mx; nY[c] -= mx;
+                    pM[c] -= mx; pX[c] -= mx; pY[c] -= mx;
+                }
+                LM -= mx; LX -= mx; LY -= mx;
                 offd = off1 + (double)mx;
             }
         }
@@ -897,7 +901,7 @@ __global__ void __launch_bounds__(64) forward_sw_kernel(Args<T> a, const int32_t
  * the log total by under 1e-8).  The remaining few terms are visited in order with a
  * ballot walk. */
 template <int CPT>
-__device__ __forceinline__ double chain_reg(const double (&t)[CPT]) {
+__device__ __forceinline__ double chain_reg(const double (&t)[CPT], const LutEntry<double> *lutd) {
     double mx = -__builtin_huge_val();
 #pragma unroll
     for (int c = 0; c < CPT; c++) mx = t[c] > mx ? t[c] : mx;
@@ -910,6 +914,8 @@ __device__ __forceinline__ double chain_reg(const double (&t)[CPT]) {
         act[c] = __ballot(t[c] > cut);
         any |= act[c];
     }
+    /* the walk runs on wave-uniform values: each term is read from its lane into
+       scalar registers, the log-add reads its cubic from the LDS table */
     double acc = -__builtin_huge_val(), pm = -__builtin_huge_val();
     while (any) {
         const int L = __builtin_ctzll(any);
@@ -917,9 +923,9 @@ __device__ __forceinline__ double chain_reg(const double (&t)[CPT]) {
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
             if ((act[c] >> L) & 1ull) {
-                const double v = __shfl(t[c], L, 64);
+                const double v = readlane(t[c], L);
                 if (v > pm - 7.5) {
-                    acc = logadd(acc, v);
+                    acc = logadd_lut(acc, v, lutd);
                     pm = v > pm ? v : pm;
                 }
             }
@@ -946,7 +952,8 @@ struct BwdRun {
  * xs holds 4 planes of stride xsw. */
 template <typename T, int CPT, int EK>
 __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const WinDev &Wd, const TaskDev &t,
-                                           BwdRun &st, T *xs, int xsw, int cmax, const LutEntry<T> *lut) {
+                                           BwdRun &st, T *xs, int xsw, int cmax, const LutEntry<T> *lut,
+                                           const LutEntry<double> *lutd) {
     constexpr bool NORM = sizeof(T) == 4;
     constexpr int W = 64 * CPT;
     const Model<T> &m = a.m;
@@ -1045,6 +1052,10 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
     HdpFetch<T> gp[PF ? CPT : 1];
     T eYp[PF ? CPT : 1];
     bool have_gp = false, have_eY = false;
+    /* the forward match states of the next diagonal's cells for its posteriors, loaded
+       a diagonal ahead (they would otherwise stall every diagonal on HBM latency) */
+    T fpre[CPT];
+    bool have_fpre = false;
 
     int e = es;
     for (; e > TB; e--) {
@@ -1101,6 +1112,7 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                 fresh = true;
                 have_gp = false;
                 have_eY = false;
+                have_fpre = false;
             }
             bad |= hi > X0 + W - 1;
             if (!fresh) {
@@ -1114,14 +1126,6 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
             LowM = shl1(pM[0], NI);
             LowX = shl1(pX[0], NI);
             const KCon<T, EK> kn = shl1_match(kc[0]); /* k-mer x of lane's last cell */
-            const T dl2 = NORM ? (T)(ob2 - ob1) : (T)0;
-            const bool shift2 = NORM && ob2 != ob1;
-            if (shift2) {
-                /* e+2 carries its own offset: bring its match states to e+1's */
-#pragma unroll
-                for (int c = 0; c < CPT; c++) qM[c] += dl2;
-                RpM += dl2;
-            }
             T eYc[PF ? CPT : 1];
             const T eYn = PF && have_eY ? shl1(eYp[0], NI) : NI;
             const bool renorm = NORM && ((D - e) & 7) == 0;
@@ -1169,10 +1173,17 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
             }
             obe = ob1;
             if (renorm) {
+                /* every 8th diagonal: the largest state becomes 0, its log carried in the
+                   diagonal's offset; e+1's match states (e+2's on the next diagonal) move
+                   with them, so that the two diagonals a cell reads always share one offset */
                 mx = wave_max(mx);
                 if (mx > NI) {
 #pragma unroll
-                    for (int c = 0; c < CPT; c++) { nM[c] -= mx; nX[c] -= mx; nY[c] -= mx; }
+                    for (int c = 0; c < CPT; c++) {
+                        nM[c] -= mx; nX[c] -= mx; nY[c] -= mx;
+                        pM[c] -= mx;
+                    }
+                    LowM -= mx;
                     obe = ob1 + (double)mx;
                 }
             }
@@ -1217,7 +1228,7 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                     const int lo1e = xlo[e + 1], hi1e = xhi[e + 1];
                     const int lom = xlo[e - 1], him = xhi[e - 1];
                     const int dm = doff[e - 1], dxym = doffxy[e - 1];
-                    const double offm = NORM ? foff[e - 1] + ob1 : 0.0;
+                    const double offm = NORM ? foff[e - 1] + obe : 0.0; /* B[e+1] is at e's offset */
 #pragma unroll
                     for (int c = 0; c < CPT; c++) {
                         const int x = X0 + CPT * lane + c; /* cell (x, e+1-x); middle (x-1, e-x) on e-1 */
@@ -1235,8 +1246,8 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                         }
                     }
                 }
-                double tot = chain_reg<CPT>(tA);
-                if (h1) tot = logadd(tot, chain_reg<CPT>(tB));
+                double tot = chain_reg<CPT>(tA, lutd);
+                if (h1) tot = logadd(tot, chain_reg<CPT>(tB, lutd));
                 total = tot;
                 if (e == TF && lane == 0) a.win_total[wi] = total;
             }
@@ -1244,20 +1255,21 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
             /* diagonalCalculationPosteriorMatchProbs (impl/pairwiseAligner.c:1202-1253) */
             const T cst = NORM ? (T)(offe - total) : (T)0;
             bool keep[CPT];
-            double pv[CPT];
+            T pv[CPT];
             unsigned long long bal[CPT];
 #pragma unroll
             for (int c = 0; c < CPT; c++) {
                 const int x = X0 + CPT * lane + c;
                 const int y = e - x;
                 keep[c] = false;
-                pv[c] = 0.0;
+                pv[c] = 0;
                 if (x >= lo && x <= hi && x > 0 && y > 0) {
+                    const T fm = have_fpre ? fpre[c] : fM[dof + (x - lo)];
                     T p;
-                    if (NORM) p = fexp((fM[dof + (x - lo)] + nM[c]) + cst);
-                    else p = fexp((fM[dof + (x - lo)] + nM[c]) - (T)total);
+                    if (NORM) p = fexp((fm + nM[c]) + cst);
+                    else p = fexp((fm + nM[c]) - (T)total);
                     keep[c] = p >= thr;
-                    pv[c] = (double)p;
+                    pv[c] = p;
                 }
                 bal[c] = __ballot(keep[c]);
             }
@@ -1276,7 +1288,7 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                     const int idx = base + own;
                     if (idx < Wd.out_cap) {
                         const int x = X0 + CPT * lane + c;
-                        const double p = pv[c] > 1.0 ? 1.0 : pv[c];
+                        const double p = pv[c] > (T)1 ? 1.0 : (double)pv[c];
                         out[3 * idx + 0] = (int32_t)floor(p * (double)kPairProb1);
                         out[3 * idx + 1] = x - 1;
                         out[3 * idx + 2] = e - x - 1;
@@ -1285,6 +1297,15 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                 }
             }
             cursor += ntot;
+        }
+        if (e - 1 <= TF && e - 1 > TB) {
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                const int x = X0 + CPT * lane + c;
+                const bool v = x >= nlo && x <= nhi && x > 0 && e - 1 - x > 0;
+                fpre[c] = v ? fM[ndo + (x - nlo)] : NI;
+            }
+            have_fpre = true;
         }
         RpM = LowM;
 #pragma unroll
@@ -1324,8 +1345,11 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
     extern __shared__ __align__(16) unsigned char dsm[];
     T *xs = reinterpret_cast<T *>(dsm); /* [4][64 * CMAX]: states across a width change */
     __shared__ LutEntry<T> lut[16];
+    __shared__ LutEntry<double> lutd2[sizeof(T) == 8 ? 1 : 16]; /* FP64 table for the totals */
     lut_fill(lut, (int)threadIdx.x);
+    if constexpr (sizeof(T) != 8) lut_fill(lutd2, (int)threadIdx.x);
     __syncthreads();
+    const LutEntry<double> *lutd = sizeof(T) == 8 ? reinterpret_cast<const LutEntry<double> *>(lut) : lutd2;
     const int wi = order[blockIdx.x];
     const WinDev Wd = a.wins[wi];
     const TaskDev t = a.tasks[Wd.task];
@@ -1344,11 +1368,11 @@ __global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32
     int cls = window_class((ld_scalar_byte(dcl, 64 * b0) >> 3) & 7, CMAX);
     while (cls) {
         if constexpr (CMAX == 1 || CMAX > 3) {
-            cls = bwd_segment<T, CMAX, EK>(a, wi, Wd, t, st, xs, 64 * CMAX, CMAX, lut);
+            cls = bwd_segment<T, CMAX, EK>(a, wi, Wd, t, st, xs, 64 * CMAX, CMAX, lut, lutd);
         } else {
-            if (cls == 1) cls = bwd_segment<T, 1, EK>(a, wi, Wd, t, st, xs, 64 * CMAX, CMAX, lut);
-            else if (CMAX == 3 && cls == 2) cls = bwd_segment<T, 2, EK>(a, wi, Wd, t, st, xs, 64 * CMAX, CMAX, lut);
-            else cls = bwd_segment<T, CMAX, EK>(a, wi, Wd, t, st, xs, 64 * CMAX, CMAX, lut);
+            if (cls == 1) cls = bwd_segment<T, 1, EK>(a, wi, Wd, t, st, xs, 64 * CMAX, CMAX, lut, lutd);
+            else if (CMAX == 3 && cls == 2) cls = bwd_segment<T, 2, EK>(a, wi, Wd, t, st, xs, 64 * CMAX, CMAX, lut, lutd);
+            else cls = bwd_segment<T, CMAX, EK>(a, wi, Wd, t, st, xs, 64 * CMAX, CMAX, lut, lutd);
         }
     }
     if (lane == 0) a.win_count[wi] = st.cursor;

@@ -740,13 +740,17 @@ int sa_batch_autotune(sa_batch *b, sa_precision precision, sa_tuning *best, doub
     sa_tuning cur = b->tune;
     const int32_t keep_graph = cur.use_graph;
     cur.use_graph = 0;
+    /* the sweeps' device time (what the knobs change), best of two runs */
     auto time_it = [&](const sa_tuning &t, double *ms) -> int {
         int rc = sa_batch_set_tuning(b, &t);
         if (rc) return rc;
-        sa_result *r = nullptr;
-        if ((rc = sa_batch_run(b, precision, &r))) return rc;
-        *ms = r->timing.total_ms;
-        sa_result_free(r);
+        *ms = 1e30;
+        for (int k = 0; k < 2; k++) {
+            sa_result *r = nullptr;
+            if ((rc = sa_batch_run(b, precision, &r))) return rc;
+            *ms = std::min(*ms, (double)r->timing.fb_ms);
+            sa_result_free(r);
+        }
         return SA_OK;
     };
     double cur_ms = 0.0;
@@ -767,7 +771,7 @@ int sa_batch_autotune(sa_batch *b, sa_precision precision, sa_tuning *best, doub
                 continue;
             double ms = 0.0;
             if ((rc = time_it(t, &ms))) return rc;
-            if (ms < cur_ms) { cur = t; cur_ms = ms; }
+            if (ms < 0.99 * cur_ms) { cur = t; cur_ms = ms; } /* a clear gain, not noise */
         }
     }
     cur.use_graph = keep_graph;
