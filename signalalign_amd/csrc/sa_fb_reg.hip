@@ -1056,6 +1056,30 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
        a diagonal ahead (they would otherwise stall every diagonal on HBM latency) */
     T fpre[CPT];
     bool have_fpre = false;
+    /* the forward states the next total refresh reads (gapX/gapY of its diagonal r,
+       all three states of r-1), loaded a diagonal ahead as well */
+    T rfx[CPT], rfy[CPT], rgm[CPT], rgx[CPT], rgy[CPT];
+    bool have_rpre = false;
+    auto load_refresh = [&](int r) {
+        const int lor = ld_scalar(xlo, r), hir = ld_scalar(xhi, r), dxy = ld_scalar(doffxy, r);
+        const bool h1r = r + 1 <= D; /* (r = D: no later diagonal in the window) */
+        const int lo1 = h1r ? ld_scalar(xlo, r + 1) : 1, hi1r = h1r ? ld_scalar(xhi, r + 1) : 0;
+        const int lom = ld_scalar(xlo, r - 1), him = ld_scalar(xhi, r - 1);
+        const int dm = ld_scalar(doff, r - 1), dxym = ld_scalar(doffxy, r - 1);
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            const int x = X0 + CPT * lane + c;
+            const bool in = x >= lor && x <= hir;
+            rfx[c] = in ? fX[dxy + (x - lor)] : NI;
+            rfy[c] = in ? fY[dxy + (x - lor)] : NI;
+            /* cell (x, r+1-x) of r+1; its middle (x-1, r-x) on r-1 */
+            const bool hm = x >= lo1 && x <= hi1r && x - 1 >= lom && x - 1 <= him && x >= 1 && r + 1 - x >= 1;
+            const int fi = x - 1 - lom;
+            rgm[c] = hm ? fM[dm + fi] : NI;
+            rgx[c] = hm ? fX[dxym + fi] : NI;
+            rgy[c] = hm ? fY[dxym + fi] : NI;
+        }
+    };
 
     int e = es;
     for (; e > TB; e--) {
@@ -1113,6 +1137,7 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                 have_gp = false;
                 have_eY = false;
                 have_fpre = false;
+                have_rpre = false;
             }
             bad |= hi > X0 + W - 1;
             if (!fresh) {
@@ -1207,7 +1232,8 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
             const double offe = NORM ? fo + obe : 0.0;
             if (posterior_count % 10 == 0) {
                 /* diagonalCalculationTotalProbability (impl/pairwiseAligner.c:1182-1200) */
-                const int dxy = doffxy[e];
+                if (!have_rpre) load_refresh(e);
+                have_rpre = false;
                 double tA[CPT], tB[CPT];
 #pragma unroll
                 for (int c = 0; c < CPT; c++) {
@@ -1215,33 +1241,26 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                     tA[c] = -__builtin_huge_val();
                     tB[c] = -__builtin_huge_val();
                     if (x >= lo && x <= hi) {
-                        const int i = x - lo;
-                        T cell = fM[dof + i] + nM[c];
-                        cell = logadd_lut(cell, fX[dxy + i] + nX[c], lut);
-                        cell = logadd_lut(cell, fY[dxy + i] + nY[c], lut);
+                        T cell = (have_fpre ? fpre[c] : fM[dof + (x - lo)]) + nM[c];
+                        cell = logadd_lut(cell, rfx[c] + nX[c], lut);
+                        cell = logadd_lut(cell, rfy[c] + nY[c], lut);
                         tA[c] = (double)cell + offe;
                     }
                 }
                 const bool h1 = e + 1 <= D;
                 if (h1) {
-                    /* match flux: B[e+1] against match transitions out of F[e-1] */
-                    const int lo1e = xlo[e + 1], hi1e = xhi[e + 1];
-                    const int lom = xlo[e - 1], him = xhi[e - 1];
-                    const int dm = doff[e - 1], dxym = doffxy[e - 1];
+                    /* match flux: B[e+1] against match transitions out of F[e-1] (a cell
+                       whose middle lies outside F[e-1] contributes -inf, as in the reference) */
+                    const int lo1e = ld_scalar(xlo, e + 1), hi1e = ld_scalar(xhi, e + 1);
                     const double offm = NORM ? foff[e - 1] + obe : 0.0; /* B[e+1] is at e's offset */
 #pragma unroll
                     for (int c = 0; c < CPT; c++) {
                         const int x = X0 + CPT * lane + c; /* cell (x, e+1-x); middle (x-1, e-x) on e-1 */
                         if (x >= lo1e && x <= hi1e) {
-                            const bool hm = x - 1 >= lom && x - 1 <= him && x >= 1 && e + 1 - x >= 1;
-                            T Mv = NI;
-                            if (hm) {
-                                const int fi = x - 1 - lom;
-                                const T eM = em_match<T>(m, kc[c], ev[c], s, iv);
-                                Mv = fM[dm + fi] + (eM + tMC);
-                                Mv = logadd_lut(Mv, fX[dxym + fi] + (eM + tMFX), lut);
-                                Mv = logadd_lut(Mv, fY[dxym + fi] + (eM + tMFY), lut);
-                            }
+                            const T eM = em_match<T>(m, kc[c], ev[c], s, iv);
+                            T Mv = rgm[c] + (eM + tMC);
+                            Mv = logadd_lut(Mv, rgx[c] + (eM + tMFX), lut);
+                            Mv = logadd_lut(Mv, rgy[c] + (eM + tMFY), lut);
                             tB[c] = (double)(Mv + pM[c]) + offm;
                         }
                     }
@@ -1302,10 +1321,14 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
 #pragma unroll
             for (int c = 0; c < CPT; c++) {
                 const int x = X0 + CPT * lane + c;
-                const bool v = x >= nlo && x <= nhi && x > 0 && e - 1 - x > 0;
+                const bool v = x >= nlo && x <= nhi;
                 fpre[c] = v ? fM[ndo + (x - nlo)] : NI;
             }
             have_fpre = true;
+            if (posterior_count % 10 == 0) { /* e-1 refreshes the total */
+                load_refresh(e - 1);
+                have_rpre = true;
+            }
         }
         RpM = LowM;
 #pragma unroll
