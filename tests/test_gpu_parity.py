@@ -116,6 +116,42 @@ def test_window_sizes_fp32(gpu_ctx, g6):
                ll_rel=1e-5)
 
 
+def _window_classes(r, m, expansion):
+    """Register-window class (cells per lane: 1, 2, 3; 4 = wider) of each aligned block
+    of 64 diagonals, forward and backward spans, from the oracle's band."""
+    import oracle
+    lX, lY = len(r["ref"]) - m.kmer_length + 1, len(r["events"])
+    L, R = oracle.band(r["anchors"], lX, lY, expansion)
+    d = np.arange(lX + lY + 1)
+    lo = np.trunc((d + L) / 2).astype(np.int64)
+    hi = np.trunc((d + R) / 2).astype(np.int64)
+    lmin = lo.copy()
+    lmin[1:] = np.minimum(lmin[1:], lo[:-1])
+    lmin[2:] = np.minimum(lmin[2:], lo[:-2])
+    hmax = hi.copy()
+    hmax[:-1] = np.maximum(hmax[:-1], hi[1:])
+    hmax[:-2] = np.maximum(hmax[:-2], hi[2:])
+    span = np.maximum(hi - lmin + 1, hmax - lo + 1)
+    cls = np.select([span <= 64, span <= 127, span <= 190], [1, 2, 3], 4)
+    return [int(cls[b:b + 64].max()) for b in range(0, len(cls), 64)]
+
+
+@pytest.mark.parametrize("expansion,gaps", [(10, [(500, 600), (1800, 1850)]), (20, [(500, 650), (1800, 1875)]),
+                                             (36, [(500, 600), (1800, 1850)])])
+def test_window_width_switching(gpu_ctx, g6, expansion, gaps):
+    """Anchor-free stretches widen the band there, so one sweep runs its blocks of 64
+    diagonals at different window widths (1, 2 or 3 cells per lane), switching both
+    ways through LDS; FP64 against the oracle and FP32 to its bar."""
+    r = synth.single_read(g6, 2600, seed=70 + expansion)
+    keep = [a for a in r["anchors"] if not any(g0 <= a[1] < g1 for g0, g1 in gaps)]
+    r["anchors"] = np.array(keep, dtype=np.int64).reshape(-1, 2)
+    cls = _window_classes(r, g6, expansion)
+    assert max(cls) <= 3 and len(set(cls)) >= 2, cls  # the sweeps switch width (a class 4 would pin it)
+    _check(gpu_ctx, g6, r, native.Params(diagonal_expansion=expansion), native.SA_FP64)
+    _check(gpu_ctx, g6, r, native.Params(diagonal_expansion=expansion), native.SA_FP32, p_tol=2e-3,
+           ll_rel=1e-5)
+
+
 def test_wide_expansion_and_threshold(gpu_ctx, g6):
     r = synth.single_read(g6, 3000, seed=24)
     _check(gpu_ctx, g6, r, native.Params(diagonal_expansion=100, threshold=0.05), native.SA_FP64)
