@@ -109,6 +109,13 @@ __device__ __forceinline__ double uniform(double v) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+/* wave priority among the waves sharing a SIMD (higher issues first) */
+__device__ __forceinline__ void set_priority(int p) {
+    if (p >= 3) __builtin_amdgcn_s_setprio(3);
+    else if (p == 2) __builtin_amdgcn_s_setprio(2);
+    else if (p == 1) __builtin_amdgcn_s_setprio(1);
+}
+
 /* maximum over the wavefront, uniform */
 template <typename T>
 __device__ __forceinline__ T wave_max(T v) {

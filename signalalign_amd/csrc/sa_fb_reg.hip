@@ -268,9 +268,9 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
     __shared__ T red[NW];
     __shared__ T shbuf[NW > 1 ? 6 * W : 1];
     __shared__ LutEntry<T> lut[16];
-    /* the four-wavefront sweeps carry the longest dependency chains of a launch
-       sequence: their instructions go first on shared SIMDs */
-    if constexpr (NW > 1) __builtin_amdgcn_s_setprio(3);
+    /* the several-wavefront sweeps carry the longest dependency chains of a launch
+       sequence: their instructions go first on shared SIMDs (the launch's priority) */
+    set_priority(a.prio);
     lut_fill(lut, (int)threadIdx.x);
     __syncthreads();
     const int ti = order[blockIdx.x];
@@ -861,6 +861,7 @@ __global__ void __launch_bounds__(64) forward_sw_kernel(Args<T> a, const int32_t
     extern __shared__ __align__(16) unsigned char dsm[];
     T *xs = reinterpret_cast<T *>(dsm); /* [6][64 * CMAX]: states across a width change */
     __shared__ LutEntry<T> lut[16];
+    set_priority(a.prio);
     lut_fill(lut, (int)threadIdx.x);
     __syncthreads();
     const TaskDev t = a.tasks[order[blockIdx.x]];

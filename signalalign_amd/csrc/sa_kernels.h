@@ -77,6 +77,7 @@ struct Args {
     double threshold;
     int64_t min_diags, traceback, expansion;
     int32_t ring;
+    int32_t prio;           /* forward sweeps: wave priority on shared SIMDs (0-3) */
 };
 
 template <typename T> hipError_t launch_prep(const Args<T> &a, int n_tasks, hipStream_t st);

@@ -971,7 +971,7 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
             const int c = bgrp[i] == 0 ? 0 : cpt1(wins[wi].span);
             wgrp[wi] = c == 0 ? bgrp[i] : std::min(bgrp[i], 1 + (int)(std::find(kCpt, kCpt + 6, c) - kCpt));
         }
-    /* Streams: sub-alignments whose forward sweep uses four wavefronts (the long
+    /* Streams: sub-alignments whose forward sweep uses several wavefronts (the long
        ones) run on stream 1, LDS sweeps on stream 2, one-wavefront forward sweeps
        on streams 0 and 3 (the sub-alignments holding the first half of their cells,
        longest first, on 0).  Each stream runs its forward launches and then the
@@ -992,6 +992,24 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
         for (int32_t i : nw1) {
             if ((double)acc >= split * (double)nw1_cells) tlane[i] = 3;
             acc += cells[i];
+        }
+        /* With no LDS sweeps, stream 2 takes the longest several-wavefront forward
+           sweeps (within 85% of the longest): the backward passes of the other long
+           sub-alignments then start when those finish, not when the longest does. */
+        bool any_lds_task = false;
+        int64_t mw_max = 0;
+        for (int64_t i = 0; i < nt; i++) {
+            any_lds_task |= fgrp[i] == 0;
+            if (fgrp[i] >= 7) mw_max = std::max(mw_max, nd[i]);
+        }
+        if (!any_lds_task && mw_max > 0) {
+            int n_head = 0, n_mw = 0;
+            for (int64_t i = 0; i < nt; i++) n_mw += fgrp[i] >= 7;
+            for (int64_t i = 0; i < nt; i++)
+                if (fgrp[i] >= 7 && (double)nd[i] >= 0.85 * (double)mw_max && n_head < n_mw - 1) {
+                    tlane[i] = 2;
+                    n_head++;
+                }
         }
     }
     struct Launch { int lane, group; int64_t off, n, cells; };
@@ -1076,10 +1094,15 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
             al.task_order = d_to + L.off;
             return sa::launch_forward<T>(al, (int)L.n, lds_fnt, lds_fcpt, sg);
         }
+        /* wave priorities follow the dependency chains: the longest several-wavefront
+           sweeps (stream 2 without LDS work) 3, the other several-wavefront ones 2,
+           the longer one-wavefront half (stream 0) 1, the rest 0 */
+        sa::Args<T> ap = a;
+        ap.prio = L.group >= 7 ? (L.lane == 2 ? 3 : 2) : L.lane == 0 ? 1 : 0;
         if (L.group < 7)
-            return sa::launch_forward_reg<T>(a, d_to + L.off, (int)L.n, kCpt[L.group - 1], 1, ek, shmem1, sg);
-        return sa::launch_forward_reg<T>(a, d_to + L.off, (int)L.n, kMW[L.group - 7][1], kMW[L.group - 7][0], ek, 0,
-                                         sg);
+            return sa::launch_forward_reg<T>(ap, d_to + L.off, (int)L.n, kCpt[L.group - 1], 1, ek, shmem1, sg);
+        return sa::launch_forward_reg<T>(ap, d_to + L.off, (int)L.n, kMW[L.group - 7][1], kMW[L.group - 7][0], ek,
+                                         0, sg);
     };
     auto launch_bwd = [&](const Launch &L, hipStream_t sg) -> hipError_t {
         if (L.group == 0) {
