@@ -978,6 +978,8 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
     const T NI = ninf<T>();
     const bool useSWX = m.use_swx != 0;
     const T thr = (T)a.threshold;
+    /* candidates for the threshold: exponents at least log(threshold) - 1e-4 */
+    const T lthr = a.threshold > 0.0 ? (T)(log(a.threshold) - 1e-4) : -__builtin_huge_valf();
     const int D = Wd.D, TF = Wd.TF, TB = Wd.T;
     const bool ragged_end = Wd.last && t.ragged_right;
     const T e0 = ragged_end ? m.rendv[0] : m.endv[0];
@@ -1272,51 +1274,62 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                 if (e == TF && lane == 0) a.win_total[wi] = total;
             }
             posterior_count++;
-            /* diagonalCalculationPosteriorMatchProbs (impl/pairwiseAligner.c:1202-1253) */
+            /* diagonalCalculationPosteriorMatchProbs (impl/pairwiseAligner.c:1202-1253): a
+               cell is kept when exp(F + B - total) >= threshold.  The exponent is first
+               compared with log(threshold) less a margin far above the rounding of exp;
+               only diagonals with a cell past it take the exp and the exact comparison. */
             const T cst = NORM ? (T)(offe - total) : (T)0;
-            bool keep[CPT];
-            T pv[CPT];
-            unsigned long long bal[CPT];
+            T sv[CPT];
+            bool cand[CPT];
+            bool anyc = false;
 #pragma unroll
             for (int c = 0; c < CPT; c++) {
                 const int x = X0 + CPT * lane + c;
                 const int y = e - x;
-                keep[c] = false;
-                pv[c] = 0;
+                cand[c] = false;
+                sv[c] = NI;
                 if (x >= lo && x <= hi && x > 0 && y > 0) {
                     const T fm = have_fpre ? fpre[c] : fM[dof + (x - lo)];
-                    T p;
-                    if (NORM) p = fexp((fm + nM[c]) + cst);
-                    else p = fexp((fm + nM[c]) - (T)total);
-                    keep[c] = p >= thr;
-                    pv[c] = p;
+                    sv[c] = NORM ? (fm + nM[c]) + cst : (fm + nM[c]) - (T)total;
+                    cand[c] = sv[c] >= lthr;
                 }
-                bal[c] = __ballot(keep[c]);
+                anyc |= cand[c];
             }
-            const unsigned long long lt = (1ull << lane) - 1ull;
-            int base = cursor;
-            int ntot = 0;
+            if (__ballot(anyc)) {
+                bool keep[CPT];
+                T pv[CPT];
+                unsigned long long bal[CPT];
 #pragma unroll
-            for (int c = 0; c < CPT; c++) {
-                base += __popcll(bal[c] & lt);
-                ntot += __popcll(bal[c]);
-            }
-            int own = 0;
+                for (int c = 0; c < CPT; c++) {
+                    pv[c] = fexp(sv[c]);
+                    keep[c] = cand[c] && pv[c] >= thr;
+                    bal[c] = __ballot(keep[c]);
+                }
+                const unsigned long long lt = (1ull << lane) - 1ull;
+                int base = cursor;
+                int ntot = 0;
 #pragma unroll
-            for (int c = 0; c < CPT; c++) {
-                if (keep[c]) {
-                    const int idx = base + own;
-                    if (idx < Wd.out_cap) {
-                        const int x = X0 + CPT * lane + c;
-                        const double p = pv[c] > (T)1 ? 1.0 : (double)pv[c];
-                        out[3 * idx + 0] = (int32_t)floor(p * (double)kPairProb1);
-                        out[3 * idx + 1] = x - 1;
-                        out[3 * idx + 2] = e - x - 1;
+                for (int c = 0; c < CPT; c++) {
+                    base += __popcll(bal[c] & lt);
+                    ntot += __popcll(bal[c]);
+                }
+                int own = 0;
+#pragma unroll
+                for (int c = 0; c < CPT; c++) {
+                    if (keep[c]) {
+                        const int idx = base + own;
+                        if (idx < Wd.out_cap) {
+                            const int x = X0 + CPT * lane + c;
+                            const double p = pv[c] > (T)1 ? 1.0 : (double)pv[c];
+                            out[3 * idx + 0] = (int32_t)floor(p * (double)kPairProb1);
+                            out[3 * idx + 1] = x - 1;
+                            out[3 * idx + 2] = e - x - 1;
+                        }
+                        own++;
                     }
-                    own++;
                 }
+                cursor += ntot;
             }
-            cursor += ntot;
         }
         if (e - 1 <= TF && e - 1 > TB) {
 #pragma unroll
