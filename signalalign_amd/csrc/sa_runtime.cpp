@@ -987,10 +987,13 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
             if (tlane[i] == 0) { nw1.push_back((int32_t)i); nw1_cells += cells[i]; }
         }
         std::stable_sort(nw1.begin(), nw1.end(), [&](int32_t x, int32_t y) { return cells[x] > cells[y]; });
+        /* one stream (a serving lane): the lanes run one after another, so the
+           one-wavefront sweeps stay in one launch and the several-wavefront ones too */
+        const bool streams4 = ctx->n_streams >= 4;
         int64_t acc = 0;
         const double split = b->tune.stream_split > 0.0 && b->tune.stream_split < 1.0 ? b->tune.stream_split : 0.5;
         for (int32_t i : nw1) {
-            if ((double)acc >= split * (double)nw1_cells) tlane[i] = 3;
+            if (streams4 && (double)acc >= split * (double)nw1_cells) tlane[i] = 3;
             acc += cells[i];
         }
         /* With no LDS sweeps, stream 2 takes the longest several-wavefront forward
@@ -1002,7 +1005,7 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
             any_lds_task |= fgrp[i] == 0;
             if (fgrp[i] >= 7) mw_max = std::max(mw_max, nd[i]);
         }
-        if (!any_lds_task && mw_max > 0) {
+        if (streams4 && !any_lds_task && mw_max > 0) {
             int n_head = 0, n_mw = 0;
             for (int64_t i = 0; i < nt; i++) n_mw += fgrp[i] >= 7;
             for (int64_t i = 0; i < nt; i++)
