@@ -371,9 +371,11 @@ def main():
     n_pairs = sum(len(r.pairs) for r in res)
     ms = float(np.mean(dev_ms))
     gcups = cells / (ms * 1e-3) / 1e9
-    # roofline of the dominant kernel: the kernel (one instantiation) with the most
-    # launch time; its average launch duration (HIP events on the stream each launch
-    # runs on) and average cells per launch, as rocprofv3 averages per kernel name
+    # roofline of the dominant kernel: the kernel (one instantiation) that carries the
+    # most of the job's work (band cells x arithmetic per cell; launch time would pick
+    # the latency-bound sweeps of the few longest reads, which overlap everything else);
+    # its average launch duration (HIP events on the stream each launch runs on) and
+    # average cells per launch, as rocprofv3 averages per kernel name
     tname = "float" if dtype == "f32" else "double"
     ek = tms[-1]["emission_path"]
     agg = {}
@@ -391,7 +393,7 @@ def main():
             a_["n"] += 1
             a_["ms"] += L["ms"]
             a_["cells"] += L["cells"]
-    kname = max(agg, key=lambda k: agg[k]["ms"])
+    kname = max(agg, key=lambda k: (agg[k]["cells"] * agg[k]["flop"], agg[k]["ms"]))
     k = agg[kname]
     k_ms, k_cells, flop = k["ms"] / k["n"], k["cells"] / k["n"], k["flop"]
     achieved = flop * k_cells / (k_ms * 1e-3) / 1e12
