@@ -78,11 +78,21 @@ struct Args {
     int64_t min_diags, traceback, expansion;
     int32_t ring;
     int32_t prio;           /* forward sweeps: wave priority on shared SIMDs (0-3) */
+    /* LDS sweeps whose ring does not fit in LDS: per-workgroup slices of HBM */
+    unsigned char *ring_g;
+    int64_t ring_g_stride;
 };
 
 template <typename T> hipError_t launch_prep(const Args<T> &a, int n_tasks, hipStream_t st);
-template <typename T> hipError_t launch_forward(const Args<T> &a, int n_tasks, int nt, int cpt, hipStream_t st);
-template <typename T> hipError_t launch_backward(const Args<T> &a, int n_wins, int cpt, hipStream_t st);
+/* bytes of LDS (or HBM slice) one workgroup of the LDS sweeps needs for a ring of R
+   positions: three diagonals of three states (+ wave maxima); the backward sweep also
+   keeps the two refresh term rows and a broadcast slot in FP64 */
+template <typename T> constexpr size_t forward_block_bytes(int R) { return (size_t)9 * R * sizeof(T) + 16 * sizeof(T); }
+template <typename T> constexpr size_t backward_block_bytes(int R) {
+    return (size_t)2 * R * sizeof(double) + 2 * sizeof(double) + (size_t)9 * R * sizeof(T);
+}
+template <typename T> hipError_t launch_forward(const Args<T> &a, int n_tasks, int nt, hipStream_t st);
+template <typename T> hipError_t launch_backward(const Args<T> &a, int n_wins, hipStream_t st);
 template <typename T>
 hipError_t launch_forward_reg(const Args<T> &a, const int32_t *order, int n, int cpt, int nw, int ek, size_t shmem,
                               hipStream_t st);

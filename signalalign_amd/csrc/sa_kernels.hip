@@ -189,12 +189,14 @@ __global__ void __launch_bounds__(256) prep_kernel(Args<T> a) {
 /* forward                                                                   */
 /* ------------------------------------------------------------------------ */
 
-template <typename T, int NT, int CPT>
+template <typename T, int NT>
 __global__ void __launch_bounds__(NT) forward_kernel(Args<T> a) {
     constexpr bool NORM = sizeof(T) == 4;
     extern __shared__ __align__(16) unsigned char smem[];
     const int R = a.ring, RM = R - 1;
-    T *ring = reinterpret_cast<T *>(smem);               /* [3 diag][3 state][R] */
+    /* the ring in LDS, or (bands too wide for LDS) in this workgroup's HBM slice */
+    unsigned char *mem = a.ring_g ? a.ring_g + (size_t)blockIdx.x * a.ring_g_stride : smem;
+    T *ring = reinterpret_cast<T *>(mem);                /* [3 diag][3 state][R] */
     T *red = ring + 9 * R;                              /* NT/64 wave maxima */
     const int ti = a.task_order[blockIdx.x];
     const TaskDev t = a.tasks[ti];
@@ -246,44 +248,50 @@ __global__ void __launch_bounds__(NT) forward_kernel(Args<T> a) {
         const int c = (int)(d % 3), c1 = (int)((d + 2) % 3), c2 = (int)((d + 1) % 3);
         T *rc = ring + c * 3 * R, *r1 = ring + c1 * 3 * R, *r2 = ring + c2 * 3 * R;
         const T dl2 = NORM ? (T)(off2 - off1) : (T)0;
-        T vM[CPT], vX[CPT], vY[CPT];
+        const bool keep_xy = dflag[d] != 0;
         T mx = NI;
-#pragma unroll
-        for (int q = 0; q < CPT; q++) {
-            const int i = tid + q * NT;
-            vM[q] = NI; vX[q] = NI; vY[q] = NI;
-            if (i < w) {
-                const int x = lo + i;
-                const int64_t y = d - x;
-                const bool hasL = x - 1 >= lo1 && x - 1 <= hi1;
-                const bool hasU = x >= lo1 && x <= hi1;
-                const bool hasM = x - 1 >= lo2 && x - 1 <= hi2;
-                const T LM = hasL ? r1[(x - 1) & RM] : NI, LX = hasL ? r1[R + ((x - 1) & RM)] : NI,
-                        LY = hasL ? r1[2 * R + ((x - 1) & RM)] : NI;
-                const T UM = hasU ? r1[x & RM] : NI, UY = hasU ? r1[2 * R + (x & RM)] : NI;
-                T MM = hasM ? r2[(x - 1) & RM] : NI, MX = hasM ? r2[R + ((x - 1) & RM)] : NI,
-                  MY = hasM ? r2[2 * R + ((x - 1) & RM)] : NI;
-                if (NORM) { MM += dl2; MX += dl2; MY += dl2; }
-                const bool kx = x >= 1, ky = y >= 1;
-                const int32_t kk = kid[kx ? x - 1 : 0];
-                const int64_t jj = evc + (ky ? y - 1 : 0);
-                T eX = kx ? Emit<T>::gapx(m, kk) : NI;
-                T eM = NI, eY = NI;
-                if (kx && ky && (hasM || hasU)) Emit<T>::eval(m, a, kk, jj, s, inv_var, eM, eY);
-                /* lower -> gapX, middle -> match, upper -> gapY (impl/stateMachine.c:1209-1256) */
-                T X = LM + (eX + tGOX);
-                X = logadd(X, LX + (eX + tGEX));
-                X = logadd(X, LY + (eX + tSWX));
-                T M = MM + (eM + tMC);
-                M = logadd(M, MX + (eM + tMFX));
-                M = logadd(M, MY + (eM + tMFY));
-                T Y = UM + (eY + tGOY);
-                Y = logadd(Y, UY + (eY + tGEY));
-                vM[q] = M; vX[q] = X; vY[q] = Y;
-                if (NORM) {
-                    T m3 = M > X ? M : X;
-                    m3 = m3 > Y ? m3 : Y;
-                    mx = m3 > mx ? m3 : mx;
+        /* cells of d: FP64 values go straight to the ring and the planes; FP32 values
+           go to the ring unnormalised and are shifted by the diagonal's maximum in a
+           second pass over the same cells (each thread re-reads its own) */
+        for (int i = tid; i < w; i += NT) {
+            const int x = lo + i;
+            const int64_t y = d - x;
+            const bool hasL = x - 1 >= lo1 && x - 1 <= hi1;
+            const bool hasU = x >= lo1 && x <= hi1;
+            const bool hasM = x - 1 >= lo2 && x - 1 <= hi2;
+            const T LM = hasL ? r1[(x - 1) & RM] : NI, LX = hasL ? r1[R + ((x - 1) & RM)] : NI,
+                    LY = hasL ? r1[2 * R + ((x - 1) & RM)] : NI;
+            const T UM = hasU ? r1[x & RM] : NI, UY = hasU ? r1[2 * R + (x & RM)] : NI;
+            T MM = hasM ? r2[(x - 1) & RM] : NI, MX = hasM ? r2[R + ((x - 1) & RM)] : NI,
+              MY = hasM ? r2[2 * R + ((x - 1) & RM)] : NI;
+            if (NORM) { MM += dl2; MX += dl2; MY += dl2; }
+            const bool kx = x >= 1, ky = y >= 1;
+            const int32_t kk = kid[kx ? x - 1 : 0];
+            const int64_t jj = evc + (ky ? y - 1 : 0);
+            T eX = kx ? Emit<T>::gapx(m, kk) : NI;
+            T eM = NI, eY = NI;
+            if (kx && ky && (hasM || hasU)) Emit<T>::eval(m, a, kk, jj, s, inv_var, eM, eY);
+            /* lower -> gapX, middle -> match, upper -> gapY (impl/stateMachine.c:1209-1256) */
+            T X = LM + (eX + tGOX);
+            X = logadd(X, LX + (eX + tGEX));
+            X = logadd(X, LY + (eX + tSWX));
+            T M = MM + (eM + tMC);
+            M = logadd(M, MX + (eM + tMFX));
+            M = logadd(M, MY + (eM + tMFY));
+            T Y = UM + (eY + tGOY);
+            Y = logadd(Y, UY + (eY + tGEY));
+            rc[x & RM] = M;
+            rc[R + (x & RM)] = X;
+            rc[2 * R + (x & RM)] = Y;
+            if (NORM) {
+                T m3 = M > X ? M : X;
+                m3 = m3 > Y ? m3 : Y;
+                mx = m3 > mx ? m3 : mx;
+            } else {
+                a.fM[base + run + i] = M;
+                if (keep_xy) {
+                    a.fX[xybase + runxy + i] = X;
+                    a.fY[xybase + runxy + i] = Y;
                 }
             }
         }
@@ -298,23 +306,17 @@ __global__ void __launch_bounds__(NT) forward_kernel(Args<T> a) {
                 for (int q = 1; q < NT / 64; q++) mx = red[q] > mx ? red[q] : mx;
             }
             const T sh = mx > NI ? mx : (T)0;
-#pragma unroll
-            for (int q = 0; q < CPT; q++) { vM[q] -= sh; vX[q] -= sh; vY[q] -= sh; }
             offd = off1 + (double)sh;
-        }
-        const bool keep_xy = dflag[d] != 0;
-#pragma unroll
-        for (int q = 0; q < CPT; q++) {
-            const int i = tid + q * NT;
-            if (i < w) {
+            for (int i = tid; i < w; i += NT) {
                 const int x = lo + i;
-                rc[x & RM] = vM[q];
-                rc[R + (x & RM)] = vX[q];
-                rc[2 * R + (x & RM)] = vY[q];
-                a.fM[base + run + i] = vM[q];
+                const T vM = rc[x & RM] - sh, vX = rc[R + (x & RM)] - sh, vY = rc[2 * R + (x & RM)] - sh;
+                rc[x & RM] = vM;
+                rc[R + (x & RM)] = vX;
+                rc[2 * R + (x & RM)] = vY;
+                a.fM[base + run + i] = vM;
                 if (keep_xy) {
-                    a.fX[xybase + runxy + i] = vX[q];
-                    a.fY[xybase + runxy + i] = vY[q];
+                    a.fX[xybase + runxy + i] = vX;
+                    a.fY[xybase + runxy + i] = vY;
                 }
             }
         }
@@ -351,12 +353,15 @@ __device__ __forceinline__ double chain(const double *term, int n) {
     return acc;
 }
 
-template <typename T, int CPT>
+template <typename T>
 __global__ void __launch_bounds__(64) backward_kernel(Args<T> a) {
     constexpr bool NORM = sizeof(T) == 4;
     extern __shared__ __align__(16) unsigned char smem[];
     const int R = a.ring, RM = R - 1;
-    double *termA = reinterpret_cast<double *>(smem);   /* R */
+    /* the ring and the refresh terms in LDS, or (bands too wide for LDS) in this
+       workgroup's HBM slice */
+    unsigned char *mem = a.ring_g ? a.ring_g + (size_t)blockIdx.x * a.ring_g_stride : smem;
+    double *termA = reinterpret_cast<double *>(mem);    /* R */
     double *termB = termA + R;                          /* R */
     double *bcast = termB + R;                          /* 2 */
     T *ring = reinterpret_cast<T *>(bcast + 2);         /* [3 diag][3 state][R] */
@@ -398,48 +403,49 @@ __global__ void __launch_bounds__(64) backward_kernel(Args<T> a) {
         const int lo2 = h2 ? xlo[e + 2] : 1, hi2 = h2 ? xhi[e + 2] : 0;
         const int c = e % 3, c1 = (e + 1) % 3, c2 = (e + 2) % 3;
         T *rc = ring + c * 3 * R, *r1 = ring + c1 * 3 * R, *r2 = ring + c2 * 3 * R;
-        T vM[CPT], vX[CPT], vY[CPT];
         T mx = NI;
         if (e == D) {
-#pragma unroll
-            for (int q = 0; q < CPT; q++) { vM[q] = e0; vX[q] = e1; vY[q] = e2; }
+            for (int i = lane; i < w; i += 64) {
+                const int x = lo + i;
+                rc[x & RM] = e0;
+                rc[R + (x & RM)] = e1;
+                rc[2 * R + (x & RM)] = e2;
+            }
         } else {
             const T dl2 = NORM ? (T)(ob2 - ob1) : (T)0;
-#pragma unroll
-            for (int q = 0; q < CPT; q++) {
-                const int i = lane + q * 64;
-                vM[q] = NI; vX[q] = NI; vY[q] = NI;
-                if (i < w) {
-                    const int x = lo + i;
-                    const int y = e - x;
-                    const bool hasLow = x + 1 >= lo1 && x + 1 <= hi1; /* C = (x+1, y) on e+1 */
-                    const bool hasUp = x >= lo1 && x <= hi1;          /* C = (x, y+1) on e+1 */
-                    const bool hasMid = x + 1 >= lo2 && x + 1 <= hi2; /* C = (x+1, y+1) on e+2 */
-                    const T ClowX = hasLow ? r1[R + ((x + 1) & RM)] : NI;
-                    const T CupY = hasUp ? r1[2 * R + (x & RM)] : NI;
-                    T CmidM = hasMid ? r2[(x + 1) & RM] : NI;
-                    if (NORM) CmidM += dl2;
-                    const int32_t kx = kid[x < t.lX ? x : 0];
-                    const int32_t kxm1 = kid[x >= 1 ? x - 1 : 0];
-                    const int64_t jj = evc + (y < t.lY ? y : 0);
-                    const T eX = hasLow ? Emit<T>::gapx(m, kx) : NI;
-                    const T eM = hasMid ? Emit<T>::match(m, a, kx, jj, s, inv_var) : NI;
-                    const T eY = (hasUp && x >= 1) ? Emit<T>::gapy(m, a, kxm1, jj, s, inv_var) : NI;
-                    /* contributions arrive middle (from e+2), then upper, then lower */
-                    T M = CmidM + (eM + tMC);
-                    M = logadd(M, CupY + (eY + tGOY));
-                    M = logadd(M, ClowX + (eX + tGOX));
-                    T X = CmidM + (eM + tMFX);
-                    X = logadd(X, ClowX + (eX + tGEX));
-                    T Y = CmidM + (eM + tMFY);
-                    Y = logadd(Y, CupY + (eY + tGEY));
-                    Y = logadd(Y, ClowX + (eX + tSWX));
-                    vM[q] = M; vX[q] = X; vY[q] = Y;
-                    if (NORM) {
-                        T m3 = M > X ? M : X;
-                        m3 = m3 > Y ? m3 : Y;
-                        mx = m3 > mx ? m3 : mx;
-                    }
+            for (int i = lane; i < w; i += 64) {
+                const int x = lo + i;
+                const int y = e - x;
+                const bool hasLow = x + 1 >= lo1 && x + 1 <= hi1; /* C = (x+1, y) on e+1 */
+                const bool hasUp = x >= lo1 && x <= hi1;          /* C = (x, y+1) on e+1 */
+                const bool hasMid = x + 1 >= lo2 && x + 1 <= hi2; /* C = (x+1, y+1) on e+2 */
+                const T ClowX = hasLow ? r1[R + ((x + 1) & RM)] : NI;
+                const T CupY = hasUp ? r1[2 * R + (x & RM)] : NI;
+                T CmidM = hasMid ? r2[(x + 1) & RM] : NI;
+                if (NORM) CmidM += dl2;
+                const int32_t kx = kid[x < t.lX ? x : 0];
+                const int32_t kxm1 = kid[x >= 1 ? x - 1 : 0];
+                const int64_t jj = evc + (y < t.lY ? y : 0);
+                const T eX = hasLow ? Emit<T>::gapx(m, kx) : NI;
+                const T eM = hasMid ? Emit<T>::match(m, a, kx, jj, s, inv_var) : NI;
+                const T eY = (hasUp && x >= 1) ? Emit<T>::gapy(m, a, kxm1, jj, s, inv_var) : NI;
+                /* contributions arrive middle (from e+2), then upper, then lower */
+                T M = CmidM + (eM + tMC);
+                M = logadd(M, CupY + (eY + tGOY));
+                M = logadd(M, ClowX + (eX + tGOX));
+                T X = CmidM + (eM + tMFX);
+                X = logadd(X, ClowX + (eX + tGEX));
+                T Y = CmidM + (eM + tMFY);
+                Y = logadd(Y, CupY + (eY + tGEY));
+                Y = logadd(Y, ClowX + (eX + tSWX));
+                /* the ring slot of e holds e+3, which no cell of e reads */
+                rc[x & RM] = M;
+                rc[R + (x & RM)] = X;
+                rc[2 * R + (x & RM)] = Y;
+                if (NORM) {
+                    T m3 = M > X ? M : X;
+                    m3 = m3 > Y ? m3 : Y;
+                    mx = m3 > mx ? m3 : mx;
                 }
             }
         }
@@ -447,19 +453,13 @@ __global__ void __launch_bounds__(64) backward_kernel(Args<T> a) {
         if (NORM && e != D) {
             mx = wave_max(mx);
             const T sh = mx > NI ? mx : (T)0;
-#pragma unroll
-            for (int q = 0; q < CPT; q++) { vM[q] -= sh; vX[q] -= sh; vY[q] -= sh; }
-            obe = ob1 + (double)sh;
-        }
-#pragma unroll
-        for (int q = 0; q < CPT; q++) {
-            const int i = lane + q * 64;
-            if (i < w) {
+            for (int i = lane; i < w; i += 64) { /* each lane shifts the cells it wrote */
                 const int x = lo + i;
-                rc[x & RM] = vM[q];
-                rc[R + (x & RM)] = vX[q];
-                rc[2 * R + (x & RM)] = vY[q];
+                rc[x & RM] -= sh;
+                rc[R + (x & RM)] -= sh;
+                rc[2 * R + (x & RM)] -= sh;
             }
+            obe = ob1 + (double)sh;
         }
         __syncthreads();
 
@@ -579,55 +579,20 @@ hipError_t launch_prep(const Args<T> &a, int n_tasks, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <typename T, int NT, int CPT>
-static hipError_t fwd_inst(const Args<T> &a, int n, size_t lds, hipStream_t st) {
-    hipLaunchKernelGGL((forward_kernel<T, NT, CPT>), dim3(n), dim3(NT), lds, st, a);
+template <typename T>
+hipError_t launch_forward(const Args<T> &a, int n_tasks, int nt, hipStream_t st) {
+    const size_t lds = a.ring_g ? 0 : forward_block_bytes<T>(a.ring);
+    if (nt == 64) hipLaunchKernelGGL((forward_kernel<T, 64>), dim3(n_tasks), dim3(64), lds, st, a);
+    else if (nt == 256) hipLaunchKernelGGL((forward_kernel<T, 256>), dim3(n_tasks), dim3(256), lds, st, a);
+    else return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_forward(const Args<T> &a, int n_tasks, int nt, int cpt, hipStream_t st) {
-    const size_t lds = (size_t)9 * a.ring * sizeof(T) + 16 * sizeof(T);
-    if (nt == 64) {
-        switch (cpt) {
-        case 1: return fwd_inst<T, 64, 1>(a, n_tasks, lds, st);
-        case 2: return fwd_inst<T, 64, 2>(a, n_tasks, lds, st);
-        case 3: return fwd_inst<T, 64, 3>(a, n_tasks, lds, st);
-        case 4: return fwd_inst<T, 64, 4>(a, n_tasks, lds, st);
-        default: break;
-        }
-    } else if (nt == 256) {
-        switch (cpt) {
-        case 1: return fwd_inst<T, 256, 1>(a, n_tasks, lds, st);
-        case 2: return fwd_inst<T, 256, 2>(a, n_tasks, lds, st);
-        case 3: return fwd_inst<T, 256, 3>(a, n_tasks, lds, st);
-        case 4: return fwd_inst<T, 256, 4>(a, n_tasks, lds, st);
-        default: break;
-        }
-    }
-    return hipErrorInvalidValue;
-}
-
-template <typename T, int CPT>
-static hipError_t bwd_inst(const Args<T> &a, int n, size_t lds, hipStream_t st) {
-    hipLaunchKernelGGL((backward_kernel<T, CPT>), dim3(n), dim3(64), lds, st, a);
+hipError_t launch_backward(const Args<T> &a, int n_wins, hipStream_t st) {
+    const size_t lds = a.ring_g ? 0 : backward_block_bytes<T>(a.ring);
+    hipLaunchKernelGGL((backward_kernel<T>), dim3(n_wins), dim3(64), lds, st, a);
     return hipGetLastError();
-}
-
-template <typename T>
-hipError_t launch_backward(const Args<T> &a, int n_wins, int cpt, hipStream_t st) {
-    const size_t lds = (size_t)2 * a.ring * sizeof(double) + 2 * sizeof(double) + (size_t)9 * a.ring * sizeof(T);
-    switch (cpt) {
-    case 1: return bwd_inst<T, 1>(a, n_wins, lds, st);
-    case 2: return bwd_inst<T, 2>(a, n_wins, lds, st);
-    case 3: return bwd_inst<T, 3>(a, n_wins, lds, st);
-    case 4: return bwd_inst<T, 4>(a, n_wins, lds, st);
-    case 5: return bwd_inst<T, 5>(a, n_wins, lds, st);
-    case 6: return bwd_inst<T, 6>(a, n_wins, lds, st);
-    case 7: return bwd_inst<T, 7>(a, n_wins, lds, st);
-    case 8: return bwd_inst<T, 8>(a, n_wins, lds, st);
-    default: return hipErrorInvalidValue;
-    }
 }
 
 __global__ void zero_kernel(int32_t *p, int n) {
@@ -651,9 +616,9 @@ hipError_t launch_collect(const WinDev *wins, const int32_t *win_count, const in
 
 template hipError_t launch_prep<float>(const Args<float> &, int, hipStream_t);
 template hipError_t launch_prep<double>(const Args<double> &, int, hipStream_t);
-template hipError_t launch_forward<float>(const Args<float> &, int, int, int, hipStream_t);
-template hipError_t launch_forward<double>(const Args<double> &, int, int, int, hipStream_t);
-template hipError_t launch_backward<float>(const Args<float> &, int, int, hipStream_t);
-template hipError_t launch_backward<double>(const Args<double> &, int, int, hipStream_t);
+template hipError_t launch_forward<float>(const Args<float> &, int, int, hipStream_t);
+template hipError_t launch_forward<double>(const Args<double> &, int, int, hipStream_t);
+template hipError_t launch_backward<float>(const Args<float> &, int, hipStream_t);
+template hipError_t launch_backward<double>(const Args<double> &, int, hipStream_t);
 
 } // namespace sa

@@ -172,6 +172,7 @@ struct sa_batch {
     DBuf d_task_cells, d_task_xycells, d_task_maxw, d_task_span, d_task_nwin, d_win_slots, d_err;
     DBuf d_fM, d_fX, d_fY;
     DBuf d_wins, d_task_order, d_win_order, d_out, d_win_count, d_win_total, d_win_dst, d_packed;
+    DBuf d_ring_g; /* LDS sweeps' rings in HBM when a band is too wide for LDS */
     sa_tuning tune{1, 0, 2.5, 0.5};
     int32_t nruns[2] = {0, 0}; /* runs since the batch or its tuning last changed, per precision */
     /* a recorded run per precision (0 FP64, 1 FP32), replayed while the batch is unchanged */
@@ -196,7 +197,7 @@ struct sa_batch {
                        &d_evl, &d_evi, &d_xlo, &d_xhi, &d_doff, &d_foff, &d_dflag, &d_dcls, &d_doffxy, &d_task_cells,
                        &d_task_xycells, &d_task_maxw, &d_task_span,
                        &d_task_nwin, &d_win_slots, &d_err, &d_fM, &d_fX, &d_fY, &d_wins, &d_task_order,
-                       &d_win_order, &d_out, &d_win_count, &d_win_total, &d_win_dst, &d_packed};
+                       &d_win_order, &d_out, &d_win_count, &d_win_total, &d_win_dst, &d_packed, &d_ring_g};
         for (DBuf *b : all) b->release();
     }
 };
@@ -1051,12 +1052,23 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     for (int64_t i = 0; i < nt; i++)
         if (fgrp[i] == 0) lds_maxw = std::max(lds_maxw, maxw[i]);
     const int ring = next_pow2(lds_maxw);
-    const int ring_max = sizeof(T) == 8 ? 512 : 1024;
     bool any_lds = false;
-    for (const Launch &L : fl) any_lds |= L.group == 0;
-    for (const Launch &L : bl) any_lds |= L.group == 0;
-    if (any_lds && ring > ring_max)
-        return fail(ctx, SA_ERR_UNSUPPORTED, "band diagonal wider than " + std::to_string(ring_max));
+    int64_t lds_nf = 0, lds_nb = 0; /* workgroups of the largest LDS-sweep launch per direction */
+    for (const Launch &L : fl)
+        if (L.group == 0) { any_lds = true; lds_nf = std::max(lds_nf, L.n); }
+    for (const Launch &L : bl)
+        if (L.group == 0) { any_lds = true; lds_nb = std::max(lds_nb, L.n); }
+    if (any_lds && ring > (1 << 20)) return fail(ctx, SA_ERR_UNSUPPORTED, "band diagonal wider than 2^20");
+    /* LDS sweeps keep their rings in LDS up to 64 KiB per workgroup, else in a slice of
+       HBM per workgroup (all LDS launches run on one stream, one after another, so one
+       buffer serves them) */
+    constexpr size_t kLdsMax = 65536;
+    const size_t f_bytes = sa::forward_block_bytes<T>(ring), b_bytes = sa::backward_block_bytes<T>(ring);
+    const bool f_hbm = any_lds && f_bytes > kLdsMax, b_hbm = any_lds && b_bytes > kLdsMax;
+    const size_t f_stride = (f_bytes + 255) / 256 * 256, b_stride = (b_bytes + 255) / 256 * 256;
+    if (f_hbm || b_hbm)
+        SA_HIP(ctx, b->d_ring_g.ensure(std::max(f_hbm ? (size_t)lds_nf * f_stride : 0,
+                                                b_hbm ? (size_t)lds_nb * b_stride : 0)));
     SA_HIP(ctx, b->d_fM.ensure(sizeof(T) * total_cells));
     SA_HIP(ctx, b->d_fX.ensure(sizeof(T) * std::max<int64_t>(total_xy, 1)));
     SA_HIP(ctx, b->d_fY.ensure(sizeof(T) * std::max<int64_t>(total_xy, 1)));
@@ -1079,9 +1091,6 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     int64_t n_lds = 0;
     for (const Launch &L : fl) if (L.group == 0) n_lds += L.n;
     const int lds_fnt = (n_lds >= 4 * std::max(ctx->n_cu, 1) || lds_maxw <= 64) && lds_maxw <= 256 ? 64 : 256;
-    const int lds_fcpt = (lds_maxw + lds_fnt - 1) / lds_fnt;
-    const int lds_bcpt = (lds_maxw + 63) / 64;
-    if (any_lds && (lds_fcpt > 4 || lds_bcpt > 8)) return fail(ctx, SA_ERR_UNSUPPORTED, "band too wide for the kernels");
     const int ek = model->ek;
     if ((int64_t)(fl.size() + bl.size()) * 2 > (int64_t)(sizeof(ctx->gev) / sizeof(ctx->gev[0])))
         return fail(ctx, SA_ERR_HIP, "too many launches for the timing events");
@@ -1095,7 +1104,9 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
         if (L.group == 0) {
             sa::Args<T> al = a;
             al.task_order = d_to + L.off;
-            return sa::launch_forward<T>(al, (int)L.n, lds_fnt, lds_fcpt, sg);
+            al.ring_g = f_hbm ? b->d_ring_g.as<unsigned char>() : nullptr;
+            al.ring_g_stride = (int64_t)f_stride;
+            return sa::launch_forward<T>(al, (int)L.n, lds_fnt, sg);
         }
         /* wave priorities follow the dependency chains: the longest several-wavefront
            sweeps (stream 2 without LDS work) 3, the other several-wavefront ones 2,
@@ -1111,7 +1122,9 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
         if (L.group == 0) {
             sa::Args<T> al = a;
             al.win_order = d_wo + L.off;
-            return sa::launch_backward<T>(al, (int)L.n, lds_bcpt, sg);
+            al.ring_g = b_hbm ? b->d_ring_g.as<unsigned char>() : nullptr;
+            al.ring_g_stride = (int64_t)b_stride;
+            return sa::launch_backward<T>(al, (int)L.n, sg);
         }
         return sa::launch_backward_reg<T>(a, d_wo + L.off, (int)L.n, kCpt[L.group - 1], ek, shmem1, sg);
     };

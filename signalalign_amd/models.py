@@ -131,6 +131,23 @@ def scale_noise(model, scale_sd, var_sd):
     return GaussianModel(model.alphabet, model.kmer_length, model.transitions, m, g, model.gapx)
 
 
+def scale_model(model, scale, shift, var, scale_sd, var_sd):
+    """emissions_signal_scaleModel (impl/stateMachine.c:631-667): the tables moved to a
+    read's scale (level mean x scale + shift, level sd x var, noise mean x scale_sd,
+    noise lambda x var_sd, noise sd recomputed; the gapY noise sd again from the match
+    row's lambda, as the reference computes it)."""
+    m = model.match.copy()
+    g = model.gapy.copy()
+    for t in (m, g):
+        t[:, 0] = t[:, 0] * scale + shift
+        t[:, 1] = t[:, 1] * var
+        t[:, 2] = t[:, 2] * scale_sd
+        t[:, 4] = t[:, 4] * var_sd
+    m[:, 3] = np.sqrt(m[:, 2] ** 3.0 / m[:, 4])
+    g[:, 3] = np.sqrt(g[:, 2] ** 3.0 / m[:, 4])
+    return GaussianModel(model.alphabet, model.kmer_length, model.transitions, m, g, model.gapx)
+
+
 def kmer_ids(seq, alphabet, kmer_length):
     """Vectorised kmer_id (impl/nanopore_hdp.c:360-393) of every k-mer of `seq`."""
     alphabet = "".join(sorted(alphabet))

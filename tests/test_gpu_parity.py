@@ -93,6 +93,17 @@ def test_no_anchors_full_band(gpu_ctx, g6):
     _check(gpu_ctx, g6, r, native.Params(diagonal_expansion=20), native.SA_FP64)
 
 
+@pytest.mark.parametrize("n", [700, 1400])
+def test_full_band_wider_than_lds(gpu_ctx, g6, n):
+    """No anchors on a longer read: the band is the whole matrix, diagonals of up to
+    about n cells, wider than any register window; the LDS sweeps keep their rings in
+    per-workgroup HBM slices where LDS is too small."""
+    r = synth.single_read(g6, n, seed=90 + n)
+    r["anchors"] = np.zeros((0, 2), dtype=np.int64)
+    _check(gpu_ctx, g6, r, native.Params(), native.SA_FP64)
+    _check(gpu_ctx, g6, r, native.Params(), native.SA_FP32, p_tol=2e-3, ll_rel=1e-5)
+
+
 def test_split_blocks(gpu_ctx, g6):
     """A small split_matrix_bigger_than_this makes every anchor gap a split
     (getSplitPoints, impl/pairwiseAligner.c:1706-1757)."""
