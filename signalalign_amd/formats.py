@@ -93,3 +93,93 @@ def load_npread(path):
         del strands[s]["n"], strands[s]["len"]
     out.update(strands)
     return out
+
+
+def load_nhdp(path):
+    """Reader for a serialised nanopore HDP (.nhdp): deserialize_nhdp
+    (impl/nanopore_hdp.c:1071-1098) around deserialize_hdp (impl/hdp.c:3047-3317).
+
+    Keeps what the alignment's emissions read: the sampling grid, each Dirichlet
+    process's posterior predictive density and spline slopes on it, and for each
+    k-mer (DP id = kmer_id over the sorted alphabet) its first observed ancestor
+    (dir_proc_density, impl/hdp.c:2583-2607; a DP is observed when a data point is
+    assigned to it or to a descendant, mark_observed_dps, impl/hdp.c:1127-1155).
+    The Gibbs-sampling state (factor tree, gamma samples) is skipped.
+
+    Returns a dict: alphabet (sorted), kmer_length, grid, density (rows), slopes (rows),
+    kmer_row (row of each k-mer), base (mu, nu, alpha, beta), n_dps."""
+    from . import models
+    with open(path) as f:
+        lines = f.read().split("\n")
+    pos = 0
+
+    def nxt():
+        nonlocal pos
+        if pos >= len(lines):
+            raise ValueError(f"{path}: unexpected end of file")
+        line = lines[pos]
+        pos += 1
+        return line
+
+    alphabet_size = int(nxt().split()[0])
+    alphabet = nxt().split()[0][:alphabet_size]
+    kmer_length = int(nxt().split()[0])
+    splines_finalized = int(nxt().split()[0]) != 0
+    has_data = int(nxt().split()[0]) != 0
+    sample_gamma = int(nxt().split()[0]) != 0
+    num_dps = int(nxt().split()[0])
+    dp_ids = None
+    if has_data:
+        nxt()  # the data points themselves
+        dp_ids = np.array([int(t) for t in nxt().split()], dtype=np.int64)
+    base = tuple(float(t) for t in nxt().split()[:4])
+    g = nxt().split()
+    grid_start, grid_stop, grid_length = float(g[0]), float(g[1]), int(g[2])
+    nxt()  # gamma
+    if sample_gamma:
+        for _ in range(4):  # gamma alpha, gamma beta, w, s
+            nxt()
+    parent = np.full(num_dps, -1, dtype=np.int64)
+    for i in range(num_dps):
+        tok = nxt().split()
+        if tok[0] != "-":
+            parent[i] = int(tok[0])
+    observed = np.zeros(num_dps, dtype=bool)
+    if has_data:
+        for i in np.unique(dp_ids):
+            j = int(i)
+            while j >= 0 and not observed[j]:
+                observed[j] = True
+                j = int(parent[j])
+    density = np.zeros((num_dps, grid_length))
+    if has_data:
+        for i in range(num_dps):
+            tok = nxt().split()
+            if tok:
+                if len(tok) != grid_length:
+                    raise ValueError(f"{path}: DP {i} posterior predictive has {len(tok)} values")
+                density[i] = [float(t) for t in tok]
+    if not splines_finalized:
+        raise ValueError(f"{path}: spline slopes are not finalised (dir_proc_density refuses such an HDP)")
+    slopes = np.zeros((num_dps, grid_length))
+    for i in range(num_dps):
+        tok = nxt().split()
+        if tok:
+            slopes[i] = [float(t) for t in tok]
+    grid = np.array(models.linspace(grid_start, grid_stop, grid_length))
+    # rows: the observed DPs; each k-mer reads its first observed ancestor's row
+    alpha_sorted = "".join(sorted(alphabet))
+    n_kmers = len(alpha_sorted) ** kmer_length
+    rows = np.flatnonzero(observed)
+    row_of = np.full(num_dps, -1, dtype=np.int64)
+    row_of[rows] = np.arange(len(rows))
+    kmer_row = np.empty(n_kmers, dtype=np.int64)
+    for k in range(n_kmers):
+        j = k
+        while j >= 0 and not observed[j]:
+            j = int(parent[j])
+        if j < 0:
+            raise ValueError(f"{path}: k-mer {k} has no observed ancestor")
+        kmer_row[k] = row_of[j]
+    return {"alphabet": alpha_sorted, "kmer_length": kmer_length, "grid": grid, "density": density[rows],
+            "slopes": slopes[rows], "kmer_row": kmer_row, "base": base, "n_dps": num_dps}

@@ -114,3 +114,46 @@ def test_gpu_reproduces_the_reference_count(gpu_ctx, precision):
             out = P.compare(res[0].pairs, want, params.threshold, p_tol=2e-3)
             assert P.rel(res[0].log_likelihood, st["log_likelihood"]) <= 1e-5
         print(name, precision, out)
+
+
+def _hdp_model(m):
+    h = formats.load_nhdp(os.path.join(GOLDEN, "templateSingleLevelFixed.nhdp"))
+    return h, models.HdpModel(h["alphabet"], h["kmer_length"], m.transitions, h["grid"], h["density"], h["slopes"],
+                              h["kmer_row"], m.match[:, 0])
+
+
+def test_nhdp_loader_reads_the_reference_file():
+    """Structure of the serialised HDP, and its spline slopes: written by the
+    reference's finalize_distributions, they equal spline_knot_slopes of the stored
+    posterior predictive densities exactly (pins the slope restatement on reference
+    output)."""
+    h = formats.load_nhdp(os.path.join(GOLDEN, "templateSingleLevelFixed.nhdp"))
+    assert h["alphabet"] == "ACEGOT" and h["kmer_length"] == 6 and h["n_dps"] == 6 ** 6 + 1
+    assert np.array_equal(h["grid"], np.array(models.linspace(0.0, 100.0, 100)))
+    assert h["density"].shape == h["slopes"].shape == (352, 100)
+    assert (h["density"] >= 0).all()
+    assert np.array_equal(models.spline_knot_slopes_rows(h["grid"], h["density"]), h["slopes"])
+    rows, counts = np.unique(h["kmer_row"], return_counts=True)
+    assert len(rows) == 352 and counts.max() == 46305  # unobserved k-mers fall back to the root
+    assert (counts[counts != 46305] == 1).all()
+
+
+@pytest.mark.gpu
+def test_gpu_real_read_on_the_reference_hdp(gpu_ctx):
+    """The Zymo read's events against the reference's HDP density grid (descaled by
+    the read's template parameters, threshold 0.1, full band): FP64 to the oracle's
+    bar, FP32 to its stated tolerance."""
+    r, ref, m = _inputs()
+    h, hm = _hdp_model(m)
+    t = r["template"]
+    p = t["params"]
+    read = {"ref": ref.encode(), "events": t["events"], "anchors": np.zeros((0, 2), dtype=np.int64),
+            "scale": p["scale"], "shift": p["shift"], "var": p["var"]}
+    dm = gpu_ctx.model(hm)
+    params = native.Params(threshold=0.1)
+    want, st = P.run_oracle(hm, read, params)
+    for prec, p_tol, ll in ((native.SA_FP64, P.P_TOL, P.LL_REL), (native.SA_FP32, 2e-3, 1e-5)):
+        res, _ = gpu_ctx.align_batch(dm, [P.native_read(read)], params, prec)
+        out = P.compare(res[0].pairs, want, params.threshold, p_tol=p_tol)
+        assert P.rel(res[0].log_likelihood, st["log_likelihood"]) <= ll
+        print(prec, out)
