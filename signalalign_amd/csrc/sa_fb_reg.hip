@@ -1377,8 +1377,14 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
 
 /* One wavefront per traceback window; the window runs at 1, 2 or 3 cells per lane as
  * the band's width allows (block classes from prep_kernel), never wider than CMAX. */
+/* Waves per SIMD the backward sweep is compiled for: the FP64 sweeps at three cells
+   per lane would otherwise hold one wave per SIMD; at two they spill a few registers
+   to scratch and still run 10% faster.  Otherwise the compiler's choice. */
+template <typename T, int CMAX> struct BwdWaves { static constexpr int n = sizeof(T) == 8 && CMAX == 3 ? 2 : 1; };
+
 template <typename T, int CMAX, int EK>
-__global__ void __launch_bounds__(64) backward_reg_kernel(Args<T> a, const int32_t *__restrict__ order) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BwdWaves<T, CMAX>::n)))
+backward_reg_kernel(Args<T> a, const int32_t *__restrict__ order) {
     extern __shared__ __align__(16) unsigned char dsm[];
     T *xs = reinterpret_cast<T *>(dsm); /* [4][64 * CMAX]: states across a width change */
     __shared__ LutEntry<T> lut[16];
