@@ -258,15 +258,16 @@ __device__ __forceinline__ double readlane(double v, int lane) {
 
 template <typename T, int CPT, int EK, int NW>
 __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const int32_t *__restrict__ order) {
+    static_assert(NW > 1, "one-wavefront sweeps are forward_sw_kernel");
     constexpr bool NORM = sizeof(T) == 4;
     constexpr int WL = 64 * NW;   /* lanes of the workgroup */
     constexpr int W = WL * CPT;   /* positions of the window */
-    /* NW > 1: the last cell of each wavefront crosses to the next through LDS,
-       double-buffered by diagonal parity (one barrier per diagonal) */
+    /* the last cell of each wavefront crosses to the next through LDS, double-buffered
+       by diagonal parity (one barrier per diagonal) */
     __shared__ T bx[2][NW][3];
     __shared__ ECon<T, EK> bev[2][NW];
     __shared__ T red[NW];
-    __shared__ T shbuf[NW > 1 ? 6 * W : 1];
+    __shared__ T shbuf[6 * W];
     __shared__ LutEntry<T> lut[16];
     /* the several-wavefront sweeps carry the longest dependency chains of a launch
        sequence: their instructions go first on shared SIMDs (the launch's priority) */
@@ -331,10 +332,8 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
         load_econ(ev[c], a, evc + j, j >= 0 && j < lY);
     }
     if (gl == 0) { md_doff = 0; md_dxy = f0 ? 0 : -1; md_foff = 0.0; }
-    if constexpr (NW > 1) {
-        if (lane == 63) { bx[0][wave][0] = pM[CPT - 1]; bx[0][wave][1] = pX[CPT - 1]; bx[0][wave][2] = pY[CPT - 1]; }
-        __syncthreads();
-    }
+    if (lane == 63) { bx[0][wave][0] = pM[CPT - 1]; bx[0][wave][1] = pX[CPT - 1]; bx[0][wave][2] = pY[CPT - 1]; }
+    __syncthreads();
     int64_t run = hi0 - lo0 + 1;
     int64_t runxy = f0 ? run : 0;
     double off1 = 0.0, off2 = 0.0;
@@ -343,11 +342,6 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
     load_econ_uniform(pre, a, evc, 1 - X0, 1 - X0 >= 0 && 1 - X0 < lY);
     bool fresh = true;
     bool bad = false;
-    /* HDP with one wavefront: the density gathers of diagonal d+1 are issued at the end
-       of diagonal d, before its stores, and used a diagonal later */
-    constexpr bool PF = EK == 2 && NW == 1;
-    HdpFetch<T> gp[PF ? CPT : 1];
-    bool have_gp = false;
 
     for (int d = 1; d <= N; d++) {
         int j = d - dc;
@@ -367,11 +361,9 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
         /* fill values for the first lane of each wavefront: the last cell of the
            previous wavefront on d-1 (and on d-2 for the middle state) */
         T fLM = NI, fLX = NI, fLY = NI;
-        if constexpr (NW > 1) {
-            if (wave > 0) {
-                const int pb = (d - 1) & 1;
-                fLM = bx[pb][wave - 1][0]; fLX = bx[pb][wave - 1][1]; fLY = bx[pb][wave - 1][2];
-            }
+        if (wave > 0) {
+            const int pb = (d - 1) & 1;
+            fLM = bx[pb][wave - 1][0]; fLX = bx[pb][wave - 1][1]; fLY = bx[pb][wave - 1][2];
         }
         if (hi > X0 + W - 1) {
             /* slide the window as far as the two previous diagonals allow */
@@ -380,16 +372,7 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
             const int sreq = (hi - (X0 + W - 1) + CPT - 1) / CPT;
             bad |= smax < sreq;
             smax = smax < 0 ? 0 : smax;
-            if constexpr (NW == 1) {
-#pragma unroll
-                for (int c = 0; c < CPT; c++) {
-                    pM[c] = shfl_idx(pM[c], lane + smax, NI); pX[c] = shfl_idx(pX[c], lane + smax, NI);
-                    pY[c] = shfl_idx(pY[c], lane + smax, NI);
-                    qM[c] = shfl_idx(qM[c], lane + smax, NI); qX[c] = shfl_idx(qX[c], lane + smax, NI);
-                    qY[c] = shfl_idx(qY[c], lane + smax, NI);
-                }
-                LpM = shr1(qM[CPT - 1], NI); LpX = shr1(qX[CPT - 1], NI); LpY = shr1(qY[CPT - 1], NI);
-            } else {
+            {
 #pragma unroll
                 for (int c = 0; c < CPT; c++) {
                     const int i = CPT * gl + c;
@@ -424,14 +407,10 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
 #pragma unroll
             for (int c = 0; c < CPT; c++) { sink(kc[c]); sink(ev[c]); }
             fresh = true;
-            have_gp = false;
         }
         if (!fresh) {
             /* cell x on d reads event d-x-1: its left neighbour's event of d-1 */
-            ECon<T, EK> fill = pre;
-            if constexpr (NW > 1) {
-                if (wave > 0) fill = bev[(d - 1) & 1][wave - 1];
-            }
+            const ECon<T, EK> fill = wave > 0 ? bev[(d - 1) & 1][wave - 1] : pre;
             const ECon<T, EK> inc = shr1(ev[CPT - 1], fill);
 #pragma unroll
             for (int c = CPT - 1; c >= 1; c--) ev[c] = ev[c - 1];
@@ -464,12 +443,7 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
             const T mM = c ? qM[c - 1] : LpM, mXv = c ? qX[c - 1] : LpX, mYv = c ? qY[c - 1] : LpY;
             const T eX = em_gapx<T>(m, kc[c]);
             T eM, eY;
-            if (PF && have_gp) {
-                eM = hdp_finish(m, gp[PF ? c : 0], iv);
-                eY = eM;
-            } else {
-                em_both<T>(m, kc[c], ev[c], s, iv, eM, eY);
-            }
+            em_both<T>(m, kc[c], ev[c], s, iv, eM, eY);
             /* lower -> gapX, middle -> match, upper -> gapY */
             T X = lM + (eX + tGOX);
             X = logadd_lut(X, lXv + (eX + tGEX), lut);
@@ -491,29 +465,16 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
         double offd = off1;
         if (renorm) {
             mx = wave_max(mx);
-            if constexpr (NW > 1) {
-                if (lane == 0) red[wave] = mx;
-                __syncthreads();
-                mx = red[0];
+            if (lane == 0) red[wave] = mx;
+            __syncthreads();
+            mx = red[0];
 #pragma unroll
-                for (int w = 1; w < NW; w++) mx = red[w] > mx ? red[w] : mx;
-            }
+            for (int w = 1; w < NW; w++) mx = red[w] > mx ? red[w] : mx;
             if (mx > NI) {
 #pragma unroll
                 for (int c = 0; c < CPT; c++) { nM[c] -= mx; nX[c] -= mx; nY[c] -= mx; }
                 offd = off1 + (double)mx;
             }
-        }
-        if constexpr (PF) {
-            /* the events of d+1 are this diagonal's shifted one lane (the shift the next
-               iteration makes); the first lane's comes from `pre` */
-            const ECon<T, EK> inc = shr1(ev[CPT - 1], pre);
-#pragma unroll
-            for (int c = 0; c < CPT; c++) {
-                const T evm = c ? ev[c - 1].m : inc.m;
-                gp[c] = hdp_fetch(m, kc[c].row, kc[c].hmu, evm, s);
-            }
-            have_gp = true;
         }
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
@@ -526,14 +487,12 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
         if (gl == j) { md_doff = (int32_t)run; md_dxy = fl ? (int32_t)runxy : -1; md_foff = offd; }
         run += hi - lo + 1;
         if (fl) runxy += hi - lo + 1;
-        if constexpr (NW > 1) {
-            if (lane == 63) {
-                const int pb = d & 1;
-                bx[pb][wave][0] = nM[CPT - 1]; bx[pb][wave][1] = nX[CPT - 1]; bx[pb][wave][2] = nY[CPT - 1];
-                bev[pb][wave] = ev[CPT - 1];
-            }
-            __syncthreads();
+        if (lane == 63) {
+            const int pb = d & 1;
+            bx[pb][wave][0] = nM[CPT - 1]; bx[pb][wave][1] = nX[CPT - 1]; bx[pb][wave][2] = nY[CPT - 1];
+            bev[pb][wave] = ev[CPT - 1];
         }
+        __syncthreads();
         LpM = LM; LpX = LX; LpY = LY;
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
@@ -570,7 +529,7 @@ struct FwdRun {
     int md_doff, md_dxy;
     double md_foff;
     int64_t run, runxy;
-    double off1, off2;
+    double off1;  /* offset of d-1 (the two diagonals a cell reads share it) */
     int lo1, lo2;
     int ncls;     /* stored class byte of the next block */
     bool bad;
@@ -636,7 +595,7 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
         if (lane == 0) { st.md_doff = 0; st.md_dxy = f0 ? 0 : -1; st.md_foff = 0.0; }
         st.run = hi0 - lo0 + 1;
         st.runxy = f0 ? st.run : 0;
-        st.off1 = 0.0; st.off2 = 0.0;
+        st.off1 = 0.0;
         st.lo1 = lo0; st.lo2 = lo0;
         LpM = NI; LpX = NI; LpY = NI;
         d = 1;
@@ -679,7 +638,7 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
     HdpFetch<T> gp[PF ? CPT : 1];
     bool have_gp = false;
     int64_t run = st.run, runxy = st.runxy;
-    double off1 = st.off1, off2 = st.off2;
+    double off1 = st.off1;
     int lo1 = st.lo1, lo2 = st.lo2;
     int dc = st.dc;
     int md_doff = st.md_doff, md_dxy = st.md_dxy;
@@ -827,7 +786,6 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
             qM[c] = pM[c]; qX[c] = pX[c]; qY[c] = pY[c];
             pM[c] = nM[c]; pX[c] = nX[c]; pY[c] = nY[c];
         }
-        off2 = off1;
         off1 = offd;
         lo2 = lo1;
         lo1 = lo;
@@ -847,7 +805,7 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
     st.dc = dc;
     st.md_doff = md_doff; st.md_dxy = md_dxy; st.md_foff = md_foff;
     st.run = run; st.runxy = runxy;
-    st.off1 = off1; st.off2 = off2;
+    st.off1 = off1;
     st.lo1 = lo1; st.lo2 = lo2;
     st.ncls = ncls;
     st.bad = bad;
@@ -871,7 +829,7 @@ __global__ void __launch_bounds__(64) forward_sw_kernel(Args<T> a, const int32_t
     FwdRun st;
     st.d = 0; st.X0 = 0; st.wprev = 0; st.dc = 0;
     st.md_doff = 0; st.md_dxy = -1; st.md_foff = 0.0;
-    st.run = 0; st.runxy = 0; st.off1 = 0.0; st.off2 = 0.0; st.lo1 = 0; st.lo2 = 0;
+    st.run = 0; st.runxy = 0; st.off1 = 0.0; st.lo1 = 0; st.lo2 = 0;
     st.ncls = 64 <= N ? ld_scalar_byte(dcl, 64) : 0;
     st.bad = false;
     int cls = window_class(ld_scalar_byte(dcl, 0) & 7, CMAX);
@@ -940,7 +898,7 @@ struct BwdRun {
     int e;          /* next diagonal to compute */
     int X0, wprev;  /* window of the previous segment, its width */
     int ncls;       /* stored class byte of the next block down */
-    double ob1, ob2;
+    double ob1;     /* offset of B[e+1] (B[e+2] is moved to it at each renormalisation) */
     int hi1, hi2;
     int cursor, posterior_count;
     double total;
@@ -1039,7 +997,7 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
         const int jn = (es - 1) - (X0 + W - 1); /* event of lane 63's last cell on es-1 */
         load_econ_uniform(pre, a, evc, jn, jn >= 0 && jn < lY);
     }
-    double ob1 = st.ob1, ob2 = st.ob2; /* offsets of B[e+1], B[e+2] */
+    double ob1 = st.ob1; /* offset of B[e+1] */
     int hi1 = st.hi1, hi2 = st.hi2;    /* xhi of e+1, e+2 (-1: outside the window of diagonals) */
     int cursor = st.cursor, posterior_count = st.posterior_count;
     double total = st.total;
@@ -1350,7 +1308,6 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
             qM[c] = pM[c];
             pM[c] = nM[c]; pX[c] = nX[c]; pY[c] = nY[c];
         }
-        ob2 = ob1;
         ob1 = obe;
         hi2 = hi1;
         hi1 = hi;
@@ -1367,7 +1324,7 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
     st.X0 = X0;
     st.wprev = W;
     st.ncls = ncls;
-    st.ob1 = ob1; st.ob2 = ob2;
+    st.ob1 = ob1;
     st.hi1 = hi1; st.hi2 = hi2;
     st.cursor = cursor; st.posterior_count = posterior_count;
     st.total = total;
@@ -1403,7 +1360,7 @@ backward_reg_kernel(Args<T> a, const int32_t *__restrict__ order) {
     st.X0 = 0; st.wprev = 0;
     const int b0 = Wd.D >> 6;
     st.ncls = 64 * b0 - 1 > Wd.T ? ld_scalar_byte(dcl, 64 * (b0 - 1)) : 0;
-    st.ob1 = 0.0; st.ob2 = 0.0;
+    st.ob1 = 0.0;
     st.hi1 = -1; st.hi2 = -1;
     st.cursor = 0; st.posterior_count = 0;
     st.total = -__builtin_huge_val();
