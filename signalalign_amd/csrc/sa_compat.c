@@ -8,6 +8,7 @@
  * call; its pairs become the reference's stList of stIntTuple.
  */
 #include "signalalign_compat.h"
+#include "signalalign_machine.h"
 #include "signalalign_amd.h"
 
 #include <math.h>
@@ -101,6 +102,35 @@ int64_t stIntTuple_get(stIntTuple *t, int64_t i) {
 }
 int64_t stIntTuple_length(stIntTuple *t) { return t->length; }
 void stIntTuple_destruct(stIntTuple *t) { free(t); }
+
+int stIntTuple_cmpFn(stIntTuple *a, stIntTuple *b) {
+    const int64_t n = a->length < b->length ? a->length : b->length;
+    for (int64_t i = 0; i < n; i++)
+        if (a->v[i] != b->v[i]) return a->v[i] < b->v[i] ? -1 : 1;
+    return a->length < b->length ? -1 : (a->length > b->length ? 1 : 0);
+}
+
+/* stable merge sort of the item pointers */
+void stList_sort(stList *l, int (*cmpFn)(const void *a, const void *b)) {
+    const int64_t n = l->length;
+    if (n < 2) return;
+    void **tmp = (void **)xmalloc(sizeof(void *) * (size_t)n);
+    void **src = l->items, **dst = tmp;
+    for (int64_t w = 1; w < n; w *= 2) {
+        for (int64_t lo = 0; lo < n; lo += 2 * w) {
+            const int64_t mid = lo + w < n ? lo + w : n, hi = lo + 2 * w < n ? lo + 2 * w : n;
+            int64_t i = lo, j = mid, o = lo;
+            while (i < mid && j < hi) dst[o++] = cmpFn(src[j], src[i]) < 0 ? src[j++] : src[i++];
+            while (i < mid) dst[o++] = src[i++];
+            while (j < hi) dst[o++] = src[j++];
+        }
+        void **t = src;
+        src = dst;
+        dst = t;
+    }
+    if (src != l->items) memcpy(l->items, src, sizeof(void *) * (size_t)n);
+    free(tmp);
+}
 
 static void attach_arena(stList *l, void *arena) { l->arena = arena; }
 #else
@@ -361,7 +391,7 @@ static CompatMachine *load_machine(const char *modelFile, StateMachineType type)
 
 /* emissions_signal_scaleNoise (impl/stateMachine.c:609-629); the extra-event
  * noise sd uses the match table's lambda, as the reference does. */
-static void scale_noise(StateMachine *sM, NanoporeReadAdjustmentParameters npp) {
+void emissions_signal_scaleNoise(StateMachine *sM, NanoporeReadAdjustmentParameters npp) {
     double *M = sM->EMISSION_MATCH_MATRIX, *Y = sM->EMISSION_GAP_Y_MATRIX;
     for (int64_t i = 0; i < sM->parameterSetSize * MODEL_PARAMS; i += MODEL_PARAMS) {
         M[i + 2] *= npp.scale_sd;
@@ -380,7 +410,7 @@ StateMachine *getStateMachine3_descaled(const char *modelFile, NanoporeReadAdjus
     m->sm3.model.scale = npp.scale;
     m->sm3.model.shift = npp.shift;
     m->sm3.model.var = npp.var;
-    if (scaleNoise) scale_noise(&m->sm3.model, npp);
+    if (scaleNoise) emissions_signal_scaleNoise(&m->sm3.model, npp);
     return &m->sm3.model;
 }
 
