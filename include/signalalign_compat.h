@@ -172,8 +172,10 @@ StateMachine *getStateMachine3(const char *modelFile);
 /* HDP machine (impl/stateMachine.c:1655-1669) built from the density grid the
  * reference's NanoporeHDP evaluates: `grid` (grid_length uniform points),
  * `density` and `slopes` (n_rows x grid_length), `kmer_row` (row of the first
- * observed ancestor DP per k-mer, -1 for none) and the level means used to descale.
- * Takes the .model file for its transitions, alphabet and k-mer length. */
+ * observed ancestor DP per k-mer, -1 for none) and the level means used to descale
+ * (written into the match table's mean column, where the reference's HDP machine
+ * reads them; NULL keeps the .model file's).  Takes the .model file for its
+ * transitions, alphabet, k-mer length and Gaussian table (read scaling uses it). */
 StateMachine *getHdpStateMachineFromGrid(const char *modelFile, const double *grid, int64_t grid_length,
                                          const double *density, const double *slopes, int64_t n_rows,
                                          const int64_t *kmer_row, const double *level_mean,

@@ -18,6 +18,7 @@
  *                             signalUtils_guideAlignmentToRebasedAnchorPairs (impl/signalMachineUtils.c:130-163)
  *   event remapping           nanopore_remapAnchorPairsWithOffset impl/nanopore.c:438-450
  *   reference slices          signalUtils_ReferenceSequenceConstructFull impl/signalMachineUtils.c:20-86
+ *   HDP models (.nhdp)        deserialize_nhdp                    impl/nanopore_hdp.c:1071-1098
  *   TSV writers               writePosteriorProbsFull / VC / writeAssignments  signalMachine.c:68-260
  *   output order              sortByXPlusYCoordinate2             impl/pairwiseAligner.c:1434-1438
  *
@@ -187,6 +188,39 @@ char *signalUtils_getSubSequence(char *seq, int64_t start, int64_t end, bool str
 ReferenceSequence *signalUtils_ReferenceSequenceConstructFull(char *forwardReferencePath, char *backwardReferencePath,
                                                               struct PairwiseAlignment *pA);
 void signalUtils_ReferenceSequenceDestruct(ReferenceSequence *self);
+
+/* ---- HDP models --------------------------------------------------------------- */
+/* A serialised nanopore HDP (.nhdp), as far as alignment reads it: the sampling
+ * grid, each observed Dirichlet process's posterior predictive density and spline
+ * slopes on it, and the DP tree (deserialize_nhdp, impl/nanopore_hdp.c:1071-1098,
+ * around deserialize_hdp, impl/hdp.c:3047-3317; the Gibbs-sampling state is
+ * skipped).  DP ids of the leaves are kmer_id over the sorted alphabet. */
+typedef struct _nanoporeHDP {
+    char *alphabet; /* sorted */
+    int64_t alphabet_size;
+    int64_t kmer_length;
+    int64_t num_dps;
+    int64_t grid_length;
+    double *grid;
+    int64_t *parent;    /* -1 at the root */
+    bool *observed;     /* data assigned to the DP or a descendant */
+    int64_t *row;       /* row of an observed DP in density/slopes, else -1 */
+    int64_t n_rows;
+    double *density;    /* n_rows x grid_length */
+    double *slopes;
+} NanoporeHDP;
+
+NanoporeHDP *deserialize_nhdp(const char *filepath);
+void destroy_nanopore_hdp(NanoporeHDP *nhdp);
+/* dir_proc_expected_val / dir_proc_variance (impl/hdp.c:2772-2801) of a DP. */
+double nhdp_dp_expected_value(NanoporeHDP *nhdp, int64_t dp);
+double nhdp_dp_variance(NanoporeHDP *nhdp, int64_t dp);
+/* impl/stateMachine.c:1655-1669: the .model file's machine with the HDP's
+ * densities for emissions (k-mers read their first observed ancestor's). */
+StateMachine *getHdpStateMachine(NanoporeHDP *hdp, const char *modelFile, NanoporeReadAdjustmentParameters npp);
+/* impl/stateMachine.c:1163-1192: level mean and sd of every observed k-mer set
+ * to its DP's expected value and standard deviation. */
+void stateMachine3_setModelToHdpExpectedValues(StateMachine *sM, NanoporeHDP *nhdp);
 
 /* ---- alignment and output (signalMachine.c) ----------------------------------- */
 /* inc/stateMachine.h:43-46 ("template" is a C++ keyword, hence the C++ names) */

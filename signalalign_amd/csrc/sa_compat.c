@@ -443,7 +443,12 @@ StateMachine *getHdpStateMachineFromGrid(const char *modelFile, const double *gr
     m->density = (double *)dup_array(density, sizeof(double) * (size_t)(grid_length * n_rows));
     m->slopes = (double *)dup_array(slopes, sizeof(double) * (size_t)(grid_length * n_rows));
     m->kmer_row = (int64_t *)dup_array(kmer_row, sizeof(int64_t) * (size_t)nk);
-    m->level_mean = (double *)dup_array(level_mean, sizeof(double) * (size_t)nk);
+    /* the descaling level means live in the match table's mean column, as in the
+       reference's HDP machine (emissions_signal_getModelLevelMean); NULL keeps the
+       .model file's */
+    if (level_mean)
+        for (int64_t i = 0; i < nk; i++) m->sm3.model.EMISSION_MATCH_MATRIX[i * MODEL_PARAMS] = level_mean[i];
+    m->level_mean = (double *)xmalloc(sizeof(double) * (size_t)nk);
     for (int64_t i = 0; i < nk; i++) m->sm3.model.EMISSION_GAP_X_PROBS[i] = LOG_GAP_X_C;
     return &m->sm3.model;
 }
@@ -543,6 +548,8 @@ static sa_model *device_model(sa_context *ctx, CompatMachine *m) {
         d.density = m->density;
         d.slopes = m->slopes;
         d.kmer_row = m->kmer_row;
+        for (size_t i = 0; i < (size_t)sM->parameterSetSize; i++)
+            m->level_mean[i] = sM->EMISSION_MATCH_MATRIX[i * MODEL_PARAMS];
         d.level_mean = m->level_mean;
     } else {
         d.match = sM->EMISSION_MATCH_MATRIX;

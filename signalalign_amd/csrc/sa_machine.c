@@ -73,6 +73,12 @@ static void split_fields(char *s, Fields *fl) {
     }
 }
 
+static void fields_free(Fields *fl) {
+    free(fl->tok);
+    fl->tok = NULL;
+    fl->n = fl->cap = 0;
+}
+
 static int parse_i64(const char *s, int64_t *out) {
     char *end;
     long long v = strtoll(s, &end, 10);
@@ -731,6 +737,210 @@ void signalUtils_ReferenceSequenceDestruct(ReferenceSequence *R) {
     free(R->trimmedForwardSequence);
     free(R->trimmedBackwardSequence);
     free(R);
+}
+
+/* ------------------------------------------------------------------------ */
+/* HDP models (impl/nanopore_hdp.c:1071-1098, impl/hdp.c:3047-3317)           */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    char **lines;
+    int64_t n, pos;
+    const char *path;
+} LineCursor;
+
+static char *next_line(LineCursor *c) {
+    if (c->pos >= c->n) fail("deserialize_nhdp", "%s: unexpected end of file", c->path, 0, 0);
+    return c->lines[c->pos++];
+}
+
+static int64_t first_i64(LineCursor *c, const char *what) {
+    int64_t v;
+    if (!parse_i64(next_line(c), &v)) fail("deserialize_nhdp", "bad %s", what, 0, 0);
+    return v;
+}
+
+/* the values of a line into dst (n expected; an empty line leaves dst alone and
+   returns 0) */
+static int row_of_doubles(char *line, double *dst, int64_t n, const char *path) {
+    Fields fl = {0};
+    split_fields(line, &fl);
+    if (fl.n == 0) {
+        free(fl.tok);
+        return 0;
+    }
+    if (fl.n != n) fail("deserialize_nhdp", "%s: a grid row has %lld values, should have %lld", path, fl.n, n);
+    for (int64_t i = 0; i < n; i++)
+        if (!parse_f64(fl.tok[i], &dst[i])) fail("deserialize_nhdp", "%s: bad grid value", path, 0, 0);
+    free(fl.tok);
+    return 1;
+}
+
+NanoporeHDP *deserialize_nhdp(const char *path) {
+    static const char *W = "deserialize_nhdp";
+    FILE *f = fopen(path, "r");
+    if (!f) fail(W, "cannot open %s", path, 0, 0);
+    LineCursor c = {NULL, 0, 0, path};
+    int64_t cap = 0;
+    for (char *l; (l = line_of(f));) {
+        if (c.n == cap) {
+            cap = cap ? 2 * cap : 1024;
+            c.lines = (char **)realloc(c.lines, sizeof(char *) * (size_t)cap);
+            if (!c.lines) FAIL(W, "out of host memory");
+        }
+        c.lines[c.n++] = l;
+    }
+    fclose(f);
+
+    NanoporeHDP *h = (NanoporeHDP *)xalloc(sizeof(NanoporeHDP));
+    h->alphabet_size = first_i64(&c, "alphabet size");
+    Fields fl = {0};
+    char *al = next_line(&c);
+    split_fields(al, &fl);
+    if (fl.n < 1 || (int64_t)strlen(fl.tok[0]) < h->alphabet_size || h->alphabet_size < 1)
+        fail(W, "%s: bad alphabet", path, 0, 0);
+    h->alphabet = (char *)xalloc((size_t)h->alphabet_size + 1);
+    memcpy(h->alphabet, fl.tok[0], (size_t)h->alphabet_size);
+    fields_free(&fl);
+    for (int64_t i = 1; i < h->alphabet_size; i++) /* sorted, as the machine's */
+        for (int64_t j = i; j > 0 && h->alphabet[j - 1] > h->alphabet[j]; j--) {
+            char t = h->alphabet[j];
+            h->alphabet[j] = h->alphabet[j - 1];
+            h->alphabet[j - 1] = t;
+        }
+    h->kmer_length = first_i64(&c, "k-mer length");
+    const int64_t finalized = first_i64(&c, "splines-finalized flag");
+    const int64_t has_data = first_i64(&c, "has-data flag");
+    const int64_t sample_gamma = first_i64(&c, "sample-gamma flag");
+    h->num_dps = first_i64(&c, "number of DPs");
+    if (h->num_dps < 1 || h->kmer_length < 1) fail(W, "%s: bad header", path, 0, 0);
+    if (!finalized) fail(W, "%s: spline slopes are not finalised", path, 0, 0);
+    int64_t *dp_ids = NULL, n_data = 0;
+    if (has_data) {
+        next_line(&c); /* the data points */
+        split_fields(next_line(&c), &fl);
+        n_data = fl.n;
+        dp_ids = (int64_t *)xalloc(sizeof(int64_t) * (size_t)(n_data ? n_data : 1));
+        for (int64_t i = 0; i < n_data; i++)
+            if (!parse_i64(fl.tok[i], &dp_ids[i]) || dp_ids[i] < 0 || dp_ids[i] >= h->num_dps)
+                fail(W, "%s: bad data-point DP id", path, 0, 0);
+        fields_free(&fl);
+    }
+    next_line(&c); /* base distribution parameters */
+    split_fields(next_line(&c), &fl);
+    double g0, g1;
+    int64_t gl;
+    if (fl.n < 3 || !parse_f64(fl.tok[0], &g0) || !parse_f64(fl.tok[1], &g1) || !parse_i64(fl.tok[2], &gl) || gl < 2)
+        fail(W, "%s: bad grid line", path, 0, 0);
+    fields_free(&fl);
+    h->grid_length = gl;
+    h->grid = (double *)xalloc(sizeof(double) * (size_t)gl);
+    const double dx = (g1 - g0) / (double)(gl - 1); /* linspace, impl/hdp_math_utils.c:498-510 */
+    for (int64_t i = 0; i < gl; i++) h->grid[i] = g0 + (double)i * dx;
+    h->grid[gl - 1] = g1;
+    next_line(&c); /* gamma */
+    if (sample_gamma)
+        for (int i = 0; i < 4; i++) next_line(&c);
+    h->parent = (int64_t *)xalloc(sizeof(int64_t) * (size_t)h->num_dps);
+    for (int64_t i = 0; i < h->num_dps; i++) {
+        split_fields(next_line(&c), &fl);
+        if (fl.n < 1) fail(W, "%s: missing parent of a DP", path, 0, 0);
+        if (strcmp(fl.tok[0], "-") == 0)
+            h->parent[i] = -1;
+        else if (!parse_i64(fl.tok[0], &h->parent[i]) || h->parent[i] < 0 || h->parent[i] >= h->num_dps)
+            fail(W, "%s: bad parent id", path, 0, 0);
+        fields_free(&fl);
+    }
+    /* observed: a data point sits in the DP or below it (mark_observed_dps,
+       impl/hdp.c:1127-1155) */
+    h->observed = (bool *)xalloc(sizeof(bool) * (size_t)h->num_dps);
+    for (int64_t i = 0; i < n_data; i++)
+        for (int64_t j = dp_ids[i]; j >= 0 && !h->observed[j]; j = h->parent[j]) h->observed[j] = true;
+    free(dp_ids);
+    h->row = (int64_t *)xalloc(sizeof(int64_t) * (size_t)h->num_dps);
+    h->n_rows = 0;
+    for (int64_t i = 0; i < h->num_dps; i++) h->row[i] = h->observed[i] ? h->n_rows++ : -1;
+    if (h->n_rows == 0) fail(W, "%s: no observed DP", path, 0, 0);
+    h->density = (double *)xalloc(sizeof(double) * (size_t)(h->n_rows * gl));
+    h->slopes = (double *)xalloc(sizeof(double) * (size_t)(h->n_rows * gl));
+    double *scratch = (double *)xalloc(sizeof(double) * (size_t)gl);
+    if (has_data)
+        for (int64_t i = 0; i < h->num_dps; i++) {
+            double *dst = h->row[i] >= 0 ? h->density + h->row[i] * gl : scratch;
+            if (!row_of_doubles(next_line(&c), dst, gl, path) && h->row[i] >= 0)
+                fail(W, "%s: an observed DP has no posterior predictive", path, 0, 0);
+        }
+    for (int64_t i = 0; i < h->num_dps; i++) {
+        double *dst = h->row[i] >= 0 ? h->slopes + h->row[i] * gl : scratch;
+        row_of_doubles(next_line(&c), dst, gl, path);
+    }
+    free(scratch);
+    for (int64_t i = 0; i < c.n; i++) free(c.lines[i]);
+    free(c.lines);
+    return h;
+}
+
+void destroy_nanopore_hdp(NanoporeHDP *h) {
+    if (!h) return;
+    free(h->alphabet);
+    free(h->grid);
+    free(h->parent);
+    free(h->observed);
+    free(h->row);
+    free(h->density);
+    free(h->slopes);
+    free(h);
+}
+
+double nhdp_dp_expected_value(NanoporeHDP *h, int64_t dp) {
+    if (dp < 0 || dp >= h->num_dps || h->row[dp] < 0) FAIL("nhdp_dp_expected_value", "DP is not observed");
+    const double *g = h->grid, *p = h->density + h->row[dp] * h->grid_length;
+    double e = 0.0;
+    for (int64_t i = 1; i < h->grid_length; i++) e += g[i] * p[i] * (g[i] - g[i - 1]);
+    return e;
+}
+
+double nhdp_dp_variance(NanoporeHDP *h, int64_t dp) {
+    const double e = nhdp_dp_expected_value(h, dp);
+    const double *g = h->grid, *p = h->density + h->row[dp] * h->grid_length;
+    double v = 0.0;
+    for (int64_t i = 1; i < h->grid_length; i++) {
+        const double d = g[i] - e;
+        v += d * d * p[i] * (g[i] - g[i - 1]);
+    }
+    return v;
+}
+
+StateMachine *getHdpStateMachine(NanoporeHDP *h, const char *modelFile, NanoporeReadAdjustmentParameters npp) {
+    int64_t nk = 1;
+    for (int64_t i = 0; i < h->kmer_length; i++) nk *= h->alphabet_size;
+    if (h->num_dps < nk) FAIL("getHdpStateMachine", "the HDP has fewer DPs than k-mers");
+    /* every k-mer reads its first observed ancestor (dir_proc_density,
+       impl/hdp.c:2583-2607) */
+    int64_t *kmer_row = (int64_t *)xalloc(sizeof(int64_t) * (size_t)nk);
+    for (int64_t k = 0; k < nk; k++) {
+        int64_t j = k;
+        while (j >= 0 && !h->observed[j]) j = h->parent[j];
+        if (j < 0) FAIL("getHdpStateMachine", "a k-mer has no observed ancestor DP");
+        kmer_row[k] = h->row[j];
+    }
+    StateMachine *sM = getHdpStateMachineFromGrid(modelFile, h->grid, h->grid_length, h->density, h->slopes,
+                                                  h->n_rows, kmer_row, NULL, npp);
+    free(kmer_row);
+    if (sM->kmerLength != h->kmer_length || sM->alphabetSize != h->alphabet_size ||
+        strcmp(sM->alphabet, h->alphabet) != 0)
+        FAIL("getHdpStateMachine", "the HDP and the model file have different alphabets or k-mer lengths");
+    return sM;
+}
+
+void stateMachine3_setModelToHdpExpectedValues(StateMachine *sM, NanoporeHDP *h) {
+    if (strcmp(sM->alphabet, h->alphabet) != 0 || sM->alphabetSize != h->alphabet_size ||
+        sM->kmerLength != h->kmer_length)
+        FAIL("stateMachine3_setModelToHdpExpectedValues", "the HDP and the machine have different alphabets");
+    for (int64_t k = 0; k < sM->parameterSetSize; k++) {
+        if (!h->observed[k]) continue;
+        sM->EMISSION_MATCH_MATRIX[k * MODEL_PARAMS] = nhdp_dp_expected_value(h, k);
+        sM->EMISSION_MATCH_MATRIX[k * MODEL_PARAMS + 1] = sqrt(nhdp_dp_variance(h, k));
+    }
 }
 
 /* ------------------------------------------------------------------------ */

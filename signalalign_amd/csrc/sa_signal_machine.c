@@ -5,11 +5,11 @@
  * pairs appended to the posteriors file as TSV.  Same options and outputs as the
  * reference, so its pipeline (runSignalAlign) can run this executable in its place.
  *
- * Served: three-state Gaussian machines (--templateModel / --complementModel),
- * template-only and 2D reads, output formats full / variantCaller / assignments,
- * read scaling re-estimated as the reference does.  Refused with a message:
- * the HDP machine (--sm3Hdp), expectations (--templateExpectations ...) and
- * error-correction (--error_correct_path) runs.
+ * Served: three-state Gaussian machines (--templateModel / --complementModel) and
+ * HDP machines (--sm3Hdp with --templateHdp / --complementHdp), template-only and
+ * 2D reads, output formats full / variantCaller / assignments, read scaling
+ * re-estimated as the reference does.  Refused with a message: expectations
+ * (--templateExpectations ...) and error-correction (--error_correct_path) runs.
  */
 #define _GNU_SOURCE
 #include "signalalign_machine.h"
@@ -41,7 +41,7 @@ static int64_t arg_i64(const char *s, const char *name, int64_t min) {
     return (int64_t)v;
 }
 
-static StateMachine *build_machine(const char *modelFile, NanoporeReadAdjustmentParameters npp) {
+static StateMachine *build_machine(const char *modelFile, NanoporeReadAdjustmentParameters npp, NanoporeHDP *nHdp) {
     /* buildStateMachine (signalMachine.c:262-283): descaling machine, noise not
        scaled until the parameters are re-estimated */
     FILE *f = fopen(modelFile, "r");
@@ -50,7 +50,7 @@ static StateMachine *build_machine(const char *modelFile, NanoporeReadAdjustment
         exit(1);
     }
     fclose(f);
-    return getStateMachine3_descaled(modelFile, npp, false);
+    return nHdp ? getHdpStateMachine(nHdp, modelFile, npp) : getStateMachine3_descaled(modelFile, npp, false);
 }
 
 static Sequence *event_slice(NanoporeRead *r, double *events, int64_t nEvents, int64_t *map, int64_t mapLength,
@@ -142,7 +142,16 @@ int main(int argc, char *argv[]) {
         fprintf(stderr, "Missing model files, exiting\n");
         return 1;
     }
-    if (hdp || templateHdp || complementHdp) refuse("the HDP machine (--sm3Hdp, --templateHdp)");
+    if (templateHdp || complementHdp) {
+        if (!templateHdp || (twoD && !complementHdp)) {
+            fprintf(stderr, "Need to have template and complement HDPs\n");
+            return 1;
+        }
+        if (!hdp) fprintf(stderr, "[signalAlign] - Warning: this kind of stateMachine does not use the HDPs you gave\n");
+    } else if (hdp) {
+        fprintf(stderr, "signalMachine: --sm3Hdp needs --templateHdp (and --complementHdp with --twoD)\n");
+        return 1;
+    }
     if (templateExpectationsFile || complementExpectationsFile) refuse("the expectations routine");
     if (errorCorrectPath) refuse("error correction (--error_correct_path)");
     if (!forwardReference || !backwardReference) {
@@ -171,6 +180,8 @@ int main(int argc, char *argv[]) {
     p->constraintDiagonalTrim = constraintTrim;
     p->diagonalExpansion = diagExpansion;
 
+    NanoporeHDP *nHdpT = hdp ? deserialize_nhdp(templateHdp) : NULL;
+    NanoporeHDP *nHdpC = hdp && twoD ? deserialize_nhdp(complementHdp) : NULL;
     ReferenceSequence *R = signalUtils_ReferenceSequenceConstructFull(forwardReference, backwardReference, pA);
     NanoporeRead *npRead = nanopore_loadNanoporeReadFromFile(npReadFile);
 
@@ -192,10 +203,11 @@ int main(int argc, char *argv[]) {
     stList *anchorPairs = signalUtils_guideAlignmentToRebasedAnchorPairs(pA, p); /* rebases pA */
 
     fprintf(stderr, "signalAlign - starting template alignment\n");
-    StateMachine *sMt = build_machine(templateModelFile, npRead->templateParams);
+    StateMachine *sMt = build_machine(templateModelFile, npRead->templateParams, nHdpT);
     signalUtils_estimateNanoporeParams(sMt, npRead, &npRead->templateParams, ASSIGNMENT_THRESHOLD,
                                        signalUtils_templateOneDAssignmentsFromRead,
                                        nanopore_adjustTemplateEventsForDrift);
+    if (nHdpT) stateMachine3_setModelToHdpExpectedValues(sMt, nHdpT);
     stList *tPairs = performSignalAlignment(sMt, tEvents, tMap, start2, R->getTemplateTargetSequence(R), p,
                                             anchorPairs, (DegenerateType)degenerate);
     const double tScore = scoreByPosteriorProbabilityIgnoringGaps(tPairs);
@@ -210,10 +222,11 @@ int main(int argc, char *argv[]) {
     StateMachine *sMc = NULL;
     if (twoD) {
         fprintf(stderr, "signalAlign - starting complement alignment\n");
-        sMc = build_machine(complementModelFile, npRead->complementParams);
+        sMc = build_machine(complementModelFile, npRead->complementParams, nHdpC);
         signalUtils_estimateNanoporeParams(sMc, npRead, &npRead->complementParams, ASSIGNMENT_THRESHOLD,
                                            signalUtils_complementOneDAssignmentsFromRead,
                                            nanopore_adjustComplementEventsForDrift);
+        if (nHdpC) stateMachine3_setModelToHdpExpectedValues(sMc, nHdpC);
         cPairs = performSignalAlignment(sMc, cEvents, npRead->complementEventMap, start2,
                                         R->getComplementTargetSequence(R), p, anchorPairs,
                                         (DegenerateType)degenerate);
@@ -243,6 +256,8 @@ int main(int argc, char *argv[]) {
     nanopore_nanoporeReadDestruct(npRead);
     pairwiseAlignmentBandingParameters_destruct(p);
     destructPairwiseAlignment(pA);
+    destroy_nanopore_hdp(nHdpT);
+    destroy_nanopore_hdp(nHdpC);
     fprintf(stderr, "signalAlign - SUCCESS: finished alignment of query %s, exiting\n", readLabel);
     return 0;
 }

@@ -16,6 +16,11 @@
  *       Reads one exonerate cigar line from stdin, prints the rebased alignment
  *       ("pa start1 end1 strand1 start2 end2 strand2"), the anchors ("a x y") and,
  *       with a read, the anchors remapped to that strand's events ("r x y").
+ *   machine_check nhdp <.nhdp> <.model> <k-mer id>...
+ *       deserialize_nhdp; prints "h num_dps n_rows grid_length grid[0] grid[last]",
+ *       then per k-mer id "k id observed row E sd" (E, sd of observed DPs from
+ *       nhdp_dp_expected_value / nhdp_dp_variance), then "m id mean sd" of the
+ *       .model machine after stateMachine3_setModelToHdpExpectedValues.
  *   machine_check solve <n> <A row-major...> <b...>
  *       nanopore_lineq_solve; prints x.
  *   machine_check filter <x y>...
@@ -162,6 +167,26 @@ static int anchors(int argc, char **argv) {
     return 0;
 }
 
+static int nhdp(int argc, char **argv) {
+    NanoporeHDP *h = deserialize_nhdp(argv[2]);
+    printf("h %lld %lld %lld %.17g %.17g\n", (long long)h->num_dps, (long long)h->n_rows, (long long)h->grid_length,
+           h->grid[0], h->grid[h->grid_length - 1]);
+    NanoporeReadAdjustmentParameters npp = {1.0, 0.0, 1.0, 0.0, 1.0, 1.0, 0.0};
+    StateMachine *sM = getHdpStateMachine(h, argv[3], npp);
+    stateMachine3_setModelToHdpExpectedValues(sM, h);
+    for (int i = 4; i < argc; i++) {
+        const int64_t k = atoll(argv[i]);
+        const int obs = h->observed[k];
+        printf("k %lld %d %lld %.17g %.17g\n", (long long)k, obs, (long long)h->row[k],
+               obs ? nhdp_dp_expected_value(h, k) : 0.0, obs ? sqrt(nhdp_dp_variance(h, k)) : 0.0);
+        printf("m %lld %.17g %.17g\n", (long long)k, sM->EMISSION_MATCH_MATRIX[k * MODEL_PARAMS],
+               sM->EMISSION_MATCH_MATRIX[k * MODEL_PARAMS + 1]);
+    }
+    stateMachine_destruct(sM);
+    destroy_nanopore_hdp(h);
+    return 0;
+}
+
 static int solve(int argc, char **argv) {
     const int64_t n = atoll(argv[2]);
     if (argc != 3 + n * n + n) return 2;
@@ -217,8 +242,9 @@ int main(int argc, char **argv) {
     if (argc == 8 && strcmp(argv[1], "rewrite") == 0) return rewrite(argv);
     if (argc >= 3 && strcmp(argv[1], "anchors") == 0) return anchors(argc, argv);
     if (argc >= 3 && strcmp(argv[1], "solve") == 0) return solve(argc, argv);
+    if (argc >= 4 && strcmp(argv[1], "nhdp") == 0) return nhdp(argc, argv);
     if (argc >= 2 && strcmp(argv[1], "filter") == 0) return filter(argc, argv);
     if (argc >= 2 && strcmp(argv[1], "sort") == 0) return sort(argc, argv);
-    fprintf(stderr, "usage: machine_check params|rewrite|anchors|solve|filter|sort ...\n");
+    fprintf(stderr, "usage: machine_check params|rewrite|anchors|nhdp|solve|filter|sort ...\n");
     return 1;
 }

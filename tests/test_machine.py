@@ -179,13 +179,62 @@ def test_npread_c_reader_rejects_bad_files(tmp_path):
 
 
 @pytest.mark.parametrize("extra,msg", [
-    (["--sm3Hdp"], "HDP"),
+    (["--sm3Hdp"], "needs --templateHdp"),
+    (["--sm3Hdp", "--twoD", "-C", TMODEL, "-v", "t.nhdp"], "Need to have template and complement HDPs"),
     (["-t", "x.expectations"], "expectations"),
     (["-p", "somewhere"], "error correction"),
 ])
 def test_signal_machine_refuses_unserved_runs(extra, msg):
     r = _run([MACHINE, "-T", TMODEL, "-f", REF, "-b", REF, "-q", NPREAD] + extra, stdin="", check=False)
     assert r.returncode == 1 and msg in r.stderr
+
+
+NHDP = os.path.join(GOLDEN, "templateSingleLevelFixed.nhdp")
+
+
+def _hdp_expectations(h, dp):
+    """dir_proc_expected_val / dir_proc_variance (impl/hdp.c:2772-2801), sequential sums."""
+    g, d = h["grid"], h["density"][h["row_of"][dp]]
+    e = models.expected_value(g, d)
+    v = 0.0
+    for i in range(1, len(g)):
+        v += (g[i] - e) * (g[i] - e) * d[i] * (g[i] - g[i - 1])
+    return e, np.sqrt(v)
+
+
+def _nhdp_python():
+    h = formats.load_nhdp(NHDP)
+    # observed leaves are the k-mers whose row is their own (their first observed ancestor is themselves)
+    rows = h["kmer_row"]
+    own = [k for k in range(len(rows)) if np.sum(rows == rows[k]) == 1]
+    h["row_of"] = {k: int(rows[k]) for k in own}
+    return h, own
+
+
+def test_nhdp_c_reader_matches_the_python_reader():
+    """The C .nhdp reader (grid, observed DPs, densities) and
+    stateMachine3_setModelToHdpExpectedValues against the Python reader: observed
+    k-mers take their DP's expected value and standard deviation exactly, the
+    others keep the .model file's."""
+    h, own = _nhdp_python()
+    m = models.load_model_file(TMODEL)
+    pick = own[:6] + [0, 1, 46655]
+    out = _run([CHECK, "nhdp", NHDP, TMODEL] + [str(k) for k in pick]).stdout.split("\n")
+    head = out[0].split()
+    assert head[1:4] == [str(h["n_dps"]), str(len(h["density"])), str(len(h["grid"]))]
+    assert float(head[4]) == h["grid"][0] and float(head[5]) == h["grid"][-1]
+    ks = [l.split() for l in out if l.startswith("k ")]
+    ms = [l.split() for l in out if l.startswith("m ")]
+    for k, kl, ml in zip(pick, ks, ms):
+        if k in h["row_of"]:
+            e, sd = _hdp_expectations(h, k)
+            assert kl[2] == "1" and int(kl[3]) == h["row_of"][k]
+            assert float(kl[4]) == e and float(kl[5]) == sd
+            assert float(ml[2]) == e and float(ml[3]) == sd
+        else:
+            assert kl[2] == "0"
+            assert float(ml[2]) == m.match[k][0] and float(ml[3]) == m.match[k][1]
+    assert len(own) > 100
 
 
 def test_signal_machine_needs_models():
@@ -196,14 +245,15 @@ def test_signal_machine_needs_models():
 # ---------------------------------------------------------------------------
 # end to end on the GPU
 # ---------------------------------------------------------------------------
-def _guide_cigar():
+def _guide_cigar(event_map="event_map"):
     """A guide alignment of the 2D read to the reference from the reference TSV's
     confident template pairs (posterior >= 0.9): reference position against the
-    first 2D base mapped to that event, thinned to increasing points at least 40
+    first read base (2D read, or template read with event_map="strand_event_map")
+    mapped to that event, thinned to increasing points at least 40
     reference bases apart, and the last confident pair."""
     rd = formats.load_npread(NPREAD)
     first = {}
-    for b, e in enumerate(rd["template"]["event_map"]):
+    for b, e in enumerate(rd["template"][event_map]):
         first.setdefault(int(e), b)
     pts = sorted({(int(t[1]), first[int(t[5])]) for t in (l.split("\t") for l in open(TSV))
                   if t[4] == "t" and float(t[12]) >= 0.9 and int(t[5]) in first})
@@ -241,23 +291,30 @@ def _complement(s):
     return s.translate(str.maketrans("ACGT", "TGCA"))
 
 
-def _expected(strand, pa, prm, rd, ref, k=6):
+def _expected(strand, pa, prm, rd, ref, k=6, event_map="event_map", hdp=None):
     """Oracle pairs in reference coordinates for one strand of the signalMachine run."""
     name = "template" if strand == "t" else "complement"
     ev = rd[name]["events"].copy()
     ev[:, 0] = ev[:, 0] - ev[:, 3] * prm["drift"]
-    emap = rd[name]["event_map"]
+    emap = rd[name][event_map]
     e0, e1 = int(emap[pa["start2"]]), int(emap[pa["end2"]])
     span = ref[pa["start1"]:pa["end1"]]
     target = span if strand == "t" else _complement(span)[::-1]
     _, anchors = D.guide_alignment_to_anchors(pa, 14)
     m = models.load_model_file(TMODEL if strand == "t" else CMODEL)
     m = models.scale_noise(m, prm["scale_sd"], prm["var_sd"])
+    if hdp is not None:
+        # descaling level means: the .model file's, observed k-mers' from the HDP
+        level = m.match[:, 0].copy()
+        for kk in hdp["row_of"]:
+            level[kk] = _hdp_expectations(hdp, kk)[0]
+        m = models.HdpModel(hdp["alphabet"], hdp["kmer_length"], m.transitions, hdp["grid"], hdp["density"],
+                            hdp["slopes"], hdp["kmer_row"], level)
     read = {"ref": target.encode(), "events": ev[e0:e1], "anchors": np.array(D.remap(anchors, emap, pa["start2"]),
                                                                              dtype=np.int64).reshape(-1, 2),
             "scale": prm["scale"], "shift": prm["shift"], "var": prm["var"]}
     pairs, _ = P.run_oracle(m, read, native.Params(threshold=0.01, diagonal_expansion=50),
-                            kind=oracle.GAUSS_DESCALED)
+                            kind=None if hdp is not None else oracle.GAUSS_DESCALED)
     L = len(target)
     out = []
     for p, x, y in pairs:
@@ -310,3 +367,31 @@ def test_signal_machine_end_to_end(tmp_path, oracle_lib):
     assert outs[1] == []
     # assignments: path k-mer, strand, descaled mean, posterior of every full row
     assert [l.split("\t") for l in outs[2]] == [[t[15], t[4], t[13], t[12]] for t in rows]
+
+
+@pytest.mark.gpu
+def test_signal_machine_hdp_end_to_end(tmp_path, oracle_lib):
+    """signalMachine --sm3Hdp on the Zymo read's template strand (1D: the guide
+    alignment in template-read coordinates) with the reference's HDP: read scaling
+    from the .model table, level means of observed k-mers set to the HDP's expected
+    values, emissions from the density grid; pairs to the oracle's bar (FP64)."""
+    line = _guide_cigar("strand_event_map")
+    ref = open(REF).read().split()[0]
+    bwd = tmp_path / "bwd.txt"
+    bwd.write_text(_complement(ref) + "\n")
+    rd = formats.load_npread(NPREAD)
+    prm = _params()
+    pa = D.parse_cigar(line)
+    h, _ = _nhdp_python()
+    out = tmp_path / "hdp.tsv"
+    print("signalMachine --sm3Hdp", flush=True)
+    _run([MACHINE, "--sm3Hdp", "-v", NHDP, "-s", "0", "-T", TMODEL, "-f", REF, "-b", str(bwd), "-q", NPREAD,
+          "-u", str(out), "-L", "zymo_read"], stdin=line + "\n")
+    rows = [l.split("\t") for l in out.read_text().split("\n")[:-1]]
+    got = [(int(round(float(t[12]) * 1e7)), int(t[1]), int(t[5])) for t in rows]
+    want = _expected("t", pa, prm["t"], rd, ref, event_map="strand_event_map", hdp=h)
+    cmp = P.compare(got, want, 0.01, p_tol=1e-4 + 1e-6, check_order=False)
+    print("hdp", cmp)
+    assert cmp["n_gpu"] > 500 and all(t[4] == "t" for t in rows)
+    for t in rows:
+        assert t[2] == ref[int(t[1]):int(t[1]) + 6]
