@@ -56,6 +56,22 @@ class _Pair(ctypes.Structure):
     _fields_ = [("p", ctypes.c_int64), ("x", ctypes.c_int64), ("y", ctypes.c_int64)]
 
 
+class _Expect(ctypes.Structure):
+    _fields_ = [
+        ("transitions", _F64P),
+        ("likelihood", ctypes.c_double),
+        ("event_exp", _F64P),
+        ("posteriors", _F64P),
+        ("observed", ctypes.POINTER(ctypes.c_uint8)),
+        ("rec", _F64P),
+        ("rec_cap", ctypes.c_int64),
+        ("n_rec", ctypes.c_int64),
+        ("rec_threshold", ctypes.c_double),
+        ("x1", ctypes.c_int64),
+        ("y1", ctypes.c_int64),
+    ]
+
+
 class _Stats(ctypes.Structure):
     _fields_ = [("n_windows", ctypes.c_int64), ("cells", ctypes.c_int64), ("log_likelihood", ctypes.c_double)]
 
@@ -111,6 +127,10 @@ def lib():
                                                ctypes.c_int64, _I64P, ctypes.c_int64, ctypes.c_int64,
                                                ctypes.c_double, ctypes.POINTER(_Pair), ctypes.c_int64, _I64P,
                                                _F64P, _F64P, _F64P]
+        L.or_get_expectations.restype = ctypes.c_int
+        L.or_get_expectations.argtypes = [ctypes.POINTER(_Model), ctypes.c_char_p, ctypes.c_int64, _F64P,
+                                          ctypes.c_int64, _I64P, ctypes.c_int64, ctypes.POINTER(_Params),
+                                          ctypes.c_int, ctypes.c_int, ctypes.POINTER(_Expect)]
         _lib = L
     return _lib
 
@@ -316,6 +336,47 @@ def get_aligned_pairs(model, ref, events, anchors, params=None, ragged_left=True
         return get_aligned_pairs(model, ref, events, anchors, params, ragged_left, ragged_right, n.value)
     arr = np.frombuffer(out, dtype=np.int64, count=3 * n.value).reshape(-1, 3).copy()
     return arr, {"n_windows": st.n_windows, "cells": st.cells, "log_likelihood": st.log_likelihood}
+
+
+def get_expectations(model, ref, events, anchors, params=None, ragged_left=True, ragged_right=True,
+                     transition_pseudocount=0.001, emission_pseudocount=0.001, rec_threshold=None, rec_cap=None):
+    """Oracle getExpectationsUsingAnchors with the ContinuousPairHmm accumulators
+    (pseudocounts as hmmContinuous_getExpectationsHmm sets them).  Returns a dict:
+    transitions (3, 3) [from, to], likelihood, event_exp (n_kmers, 2), posteriors,
+    observed (bool), and with rec_threshold the match-transition records (n, 5):
+    k-mer position, event, p(M->M), p(X->M), p(Y->M) in call order."""
+    params = params or Params()
+    refb = ref.encode() if isinstance(ref, str) else bytes(ref)
+    lX = len(refb) - (model.kmer_length - 1) if len(refb) > 0 else 0
+    ev = np.zeros((len(events), 4), dtype=np.float64)
+    e = np.asarray(events, dtype=np.float64)
+    if len(e):
+        ev[:, : min(4, e.shape[1])] = e[:, :4]
+    a = _i64(np.asarray(anchors, dtype=np.int64).reshape(-1, 2))
+    nk = len(model.alphabet) ** model.kmer_length
+    trans = np.full(9, transition_pseudocount)
+    eexp = np.zeros(2 * nk)
+    post = np.full(nk, emission_pseudocount)
+    obs = np.zeros(nk, dtype=np.uint8)
+    if rec_cap is None:
+        rec_cap = 0 if rec_threshold is None else 8 * (lX + len(ev)) + 1024
+    rec = np.zeros(5 * max(rec_cap, 1))
+    ex = _Expect(_p64(trans), 0.0, _p64(eexp), _p64(post), obs.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                 _p64(rec), rec_cap, 0, 0.0 if rec_threshold is None else rec_threshold, 0, 0)
+    ms = model._struct()
+    ps = params._struct()
+    rc = lib().or_get_expectations(ctypes.byref(ms), refb, lX, _p64(ev), len(ev), _pi64(a), len(a),
+                                   ctypes.byref(ps), int(ragged_left), int(ragged_right), ctypes.byref(ex))
+    if rc != 0:
+        raise RuntimeError(f"oracle getExpectations failed: {rc}")
+    if rec_threshold is not None and ex.n_rec > rec_cap:
+        return get_expectations(model, ref, events, anchors, params, ragged_left, ragged_right,
+                                transition_pseudocount, emission_pseudocount, rec_threshold, ex.n_rec)
+    out = {"transitions": trans.reshape(3, 3), "likelihood": ex.likelihood, "event_exp": eexp.reshape(-1, 2),
+           "posteriors": post, "observed": obs.astype(bool)}
+    if rec_threshold is not None:
+        out["records"] = rec[:5 * ex.n_rec].reshape(-1, 5).copy()
+    return out
 
 
 def full_forward_backward(model, ref, events, anchors, expansion, threshold):

@@ -550,10 +550,89 @@ static void posterior_match_probs(int64_t xay, const or_matrix *fw, const or_mat
     }
 }
 
-/* getPosteriorProbsWithBanding (impl/pairwiseAligner.c:1282-1420) */
+/* cell_signal_updateExpectations (impl/pairwiseAligner.c:763-790): the posterior
+ * of one transition, added to the transition expectations; for transitions into
+ * the match state also to the k-mer's event expectations
+ * (continuousPairHmm_addToEmissionExpectation, impl/continuousHmm.c:160-169). */
+static void expect_transition(or_expect *ex, const double *from, const double *to, int f, int t, double eP,
+                              double tP, double total, int64_t kidx, double xd) {
+    double p = exp(from[f] + to[t] + (eP + tP) - total);
+    ex->transitions[f * 3 + t] += p;
+    if (t == M_ST) {
+        ex->event_exp[2 * kidx] += p * xd;
+        ex->posteriors[kidx] += p;
+        double u = ex->event_exp[2 * kidx] / ex->posteriors[kidx];
+        ex->event_exp[2 * kidx + 1] += p * (xd - u) * (xd - u);
+        ex->observed[kidx] = 1;
+    }
+}
+
+/* diagonalCalculation_Expectations (impl/pairwiseAligner.c:1255-1275): the cells
+ * of the backward diagonal xay against the forward diagonals xay-1 and xay-2
+ * (NULL once deleted: the first posterior diagonal of every window after the
+ * first sees no xay-2), in stateMachine3_cellCalculate's transition order. */
+static void expectations_diagonal(const or_seqs *s, int64_t xay, const or_matrix *fw, const or_matrix *bw,
+                                  double total, or_expect *ex) {
+    const or_model *m = s->m;
+    const double *t = m->trans;
+    ex->likelihood += total;
+    const or_diag *cur = matrix_get(bw, xay);
+    const or_diag *m1 = xay >= 1 ? matrix_get(fw, xay - 1) : NULL;
+    const or_diag *m2 = xay >= 2 ? matrix_get(fw, xay - 2) : NULL;
+    for (int64_t xmy = cur->xmyL; xmy <= cur->xmyR; xmy += 2) {
+        int64_t x = x_coord(xay, xmy), y = y_coord(xay, xmy);
+        const char *kmer = (x - 1 >= 0 && x - 1 < s->lX) ? s->ref + (x - 1) : NULL;
+        const double *event = (y - 1 >= 0) ? s->ev + 4 * (y - 1) : k_null_event;
+        double *c = diag_cell(cur, xmy);
+        double *lo = diag_cell(m1, xmy - 1);
+        double *mi = diag_cell(m2, xmy);
+        double *up = diag_cell(m1, xmy + 1);
+        if (lo) {
+            double eP = gapx_emission(m, kmer);
+            expect_transition(ex, lo, c, M_ST, X_ST, eP, t[OR_T_GAP_OPEN_X], total, 0, 0);
+            expect_transition(ex, lo, c, X_ST, X_ST, eP, t[OR_T_GAP_EXTEND_X], total, 0, 0);
+            expect_transition(ex, lo, c, Y_ST, X_ST, eP, t[OR_T_GAP_SWITCH_TO_X], total, 0, 0);
+        }
+        if (mi) {
+            /* a middle neighbour means x >= 1 and y >= 1 */
+            double eP = or_emission(m, kmer, event, 1);
+            int64_t kidx = or_kmer_id(kmer, m->alphabet, m->alphabet_size, m->kmer_length);
+            double mu = m->match[5 * kidx];
+            /* emissions_signal_descaleEventMean_JordanStyle (impl/stateMachine.c:285-289) */
+            double xd = (event[0] + m->var * mu - m->scale * mu - m->shift) / m->var;
+            double pr[3];
+            for (int f = 0; f < 3; f++) pr[f] = exp(mi[f] + c[M_ST] + (eP + t[f == 0 ? OR_T_MATCH_CONTINUE
+                                                                         : f == 1 ? OR_T_MATCH_FROM_GAP_X
+                                                                                  : OR_T_MATCH_FROM_GAP_Y]) - total);
+            expect_transition(ex, mi, c, M_ST, M_ST, eP, t[OR_T_MATCH_CONTINUE], total, kidx, xd);
+            expect_transition(ex, mi, c, X_ST, M_ST, eP, t[OR_T_MATCH_FROM_GAP_X], total, kidx, xd);
+            expect_transition(ex, mi, c, Y_ST, M_ST, eP, t[OR_T_MATCH_FROM_GAP_Y], total, kidx, xd);
+            double mx = pr[0] > pr[1] ? pr[0] : pr[1];
+            mx = mx > pr[2] ? mx : pr[2];
+            if (ex->rec_cap > 0 && mx >= ex->rec_threshold) {
+                if (ex->n_rec < ex->rec_cap) {
+                    double *r = ex->rec + 5 * ex->n_rec;
+                    r[0] = (double)(x - 1 + ex->x1);
+                    r[1] = (double)(y - 1 + ex->y1);
+                    r[2] = pr[0]; r[3] = pr[1]; r[4] = pr[2];
+                }
+                ex->n_rec++;
+            }
+        }
+        if (up) {
+            double eP = or_emission(m, kmer, event, 0);
+            expect_transition(ex, up, c, M_ST, Y_ST, eP, t[OR_T_GAP_OPEN_Y], total, 0, 0);
+            expect_transition(ex, up, c, Y_ST, Y_ST, eP, t[OR_T_GAP_EXTEND_Y], total, 0, 0);
+        }
+    }
+}
+
+/* getPosteriorProbsWithBanding (impl/pairwiseAligner.c:1282-1420); with `ex` the
+ * per-diagonal callback is diagonalCalculation_Expectations instead of
+ * diagonalCalculationPosteriorMatchProbs. */
 static int posterior_probs_with_banding(const or_seqs *s, const int64_t *anchors, int64_t n_anchors,
                                         const or_params *p, int ragged_left, int ragged_right,
-                                        or_pairlist *out, or_stats *stats) {
+                                        or_pairlist *out, or_stats *stats, or_expect *ex) {
     if (!(p->traceback_diagonals >= 1) || !(p->diagonal_expansion >= 0) || p->diagonal_expansion % 2 != 0 ||
         !(p->min_diags_between_traceback >= 2) ||
         !(p->traceback_diagonals + 1 < p->min_diags_between_traceback))
@@ -606,7 +685,8 @@ static int posterior_probs_with_banding(const or_seqs *s, const int64_t *anchors
                         total = total_probability(s, d2, fw, bw);
                         if (d2 == N) stats->log_likelihood += total; /* summed over sub-alignments */
                     }
-                    posterior_match_probs(d2, fw, bw, total, p->threshold, out);
+                    if (ex) expectations_diagonal(s, d2, fw, bw, total, ex);
+                    else posterior_match_probs(d2, fw, bw, total, p->threshold, out);
                     if (d2 < traced_back_from || at_end) matrix_delete(fw, d2);
                 }
                 if (d2 + 1 <= N) matrix_delete(bw, d2 + 1);
@@ -669,7 +749,7 @@ int or_get_aligned_pairs(const or_model *m, const char *ref, int64_t lX,
         or_seqs s = {m, ref + x1, x2 - x1, events + 4 * y1, y2 - y1};
         pl.n = 0;
         rc = posterior_probs_with_banding(&s, sub, ns, p, ragged_left || i > 0,
-                                          ragged_right || i < nsplit - 1, &pl, &st);
+                                          ragged_right || i < nsplit - 1, &pl, &st, NULL);
         for (int64_t k = pl.n - 1; k >= 0 && rc == 0; k--) {
             if (n < cap) {
                 out[n].p = pl.buf[k].p;
@@ -685,6 +765,48 @@ int or_get_aligned_pairs(const or_model *m, const char *ref, int64_t lX,
     free(splits);
     *n_out = n;
     if (stats) *stats = st;
+    return rc;
+}
+
+/* getExpectationsUsingAnchors (impl/pairwiseAligner.c:1983-2003) through
+ * getPosteriorProbsWithBandingSplittingAlignmentsByLargeGaps (:1773-1837): the
+ * sub-alignments in order, each accumulating into the same expectations. */
+int or_get_expectations(const or_model *m, const char *ref, int64_t lX, const double *events, int64_t lY,
+                        const int64_t *anchors, int64_t n_anchors, const or_params *p, int ragged_left,
+                        int ragged_right, or_expect *ex) {
+    or_stats st = {0, 0, 0.0};
+    int64_t nsplit = or_split_points(anchors, n_anchors, lX, lY, p->split_matrix_bigger_than_this,
+                                     ragged_left, ragged_right, NULL, 0);
+    if (nsplit < 0) return -1;
+    int64_t *splits = (int64_t *)malloc(sizeof(int64_t) * 4 * (size_t)(nsplit + 1));
+    or_split_points(anchors, n_anchors, lX, lY, p->split_matrix_bigger_than_this, ragged_left, ragged_right,
+                    splits, nsplit);
+    int64_t *sub = (int64_t *)malloc(sizeof(int64_t) * 2 * (size_t)(n_anchors + 1));
+    int64_t j = 0;
+    int rc = 0;
+    ex->n_rec = 0;
+    for (int64_t i = 0; i < nsplit && rc == 0; i++) {
+        int64_t x1 = splits[4 * i], y1 = splits[4 * i + 1], x2 = splits[4 * i + 2], y2 = splits[4 * i + 3];
+        int64_t ns = 0;
+        while (j < n_anchors) {
+            int64_t x = anchors[2 * j], y = anchors[2 * j + 1];
+            if (x + y >= x2 + y2) break;
+            if (!(x >= x1 && x < x2 && y >= y1 && y < y2)) { rc = -2; break; }
+            sub[2 * ns] = x - x1;
+            sub[2 * ns + 1] = y - y1;
+            ns++;
+            j++;
+        }
+        if (rc) break;
+        or_seqs s = {m, ref + x1, x2 - x1, events + 4 * y1, y2 - y1};
+        ex->x1 = x1;
+        ex->y1 = y1;
+        rc = posterior_probs_with_banding(&s, sub, ns, p, ragged_left || i > 0, ragged_right || i < nsplit - 1,
+                                          NULL, &st, ex);
+    }
+    if (rc == 0 && j != n_anchors) rc = -3;
+    free(sub);
+    free(splits);
     return rc;
 }
 

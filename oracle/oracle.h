@@ -77,6 +77,30 @@ typedef struct {
     double log_likelihood;    /* sum over sub-alignments of the total at their last diagonal */
 } or_stats;
 
+/* EM expectations of one read: getExpectationsUsingAnchors (impl/pairwiseAligner.c:1983-2003)
+ * with diagonalCalculation_Expectations (:1255-1275) and cell_signal_updateExpectations
+ * (:763-790), filling the accumulators of a ContinuousPairHmm (impl/continuousHmm.c:84-170).
+ * The caller sizes the per-k-mer arrays (alphabet_size^kmer_length) and sets the
+ * reference's starting values: transitions to the transition pseudocount,
+ * posteriors to the emission pseudocount, the rest to zero. */
+typedef struct {
+    double *transitions;       /* 9: from * 3 + to (match 0, gapX 1, gapY 2) */
+    double likelihood;         /* += the window total at every posterior diagonal */
+    double *event_exp;         /* 2 per k-mer: sum p x~, running sum p (x~ - u)^2 */
+    double *posteriors;        /* per k-mer: pseudocount + sum p */
+    unsigned char *observed;   /* per k-mer */
+    /* match-transition posteriors in call order: 5 doubles per cell (k-mer position
+       x, event y, p of M->M, X->M, Y->M), cells whose largest p >= rec_threshold */
+    double *rec;
+    int64_t rec_cap, n_rec;
+    double rec_threshold;
+    int64_t x1, y1;            /* offset of the current sub-alignment (set by the driver) */
+} or_expect;
+
+int or_get_expectations(const or_model *m, const char *ref, int64_t lX, const double *events, int64_t lY,
+                        const int64_t *anchors, int64_t n_anchors, const or_params *p, int ragged_left,
+                        int ragged_right, or_expect *ex);
+
 void or_set_exact_logadd(int on);
 void or_record_windows(int64_t *buf, int64_t cap);
 int64_t or_recorded_windows(void);
