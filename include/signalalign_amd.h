@@ -217,6 +217,49 @@ int sa_result_timing(const sa_result *res, sa_timing *out);
 int64_t sa_result_launches(const sa_result *res, sa_launch *out, int64_t cap);
 void sa_result_free(sa_result *res);
 
+/* ---- EM expectations: getExpectationsUsingAnchors -----------------------------
+ * Replaces getExpectationsUsingAnchors (impl/pairwiseAligner.c:1983-2003) driven by
+ * diagonalCalculation_Expectations (:1255-1275) and cell_signal_updateExpectations /
+ * cell_signal_updateExpectationsAndAssignments (:763-811): the same banded,
+ * windowed forward-backward (FP64), and at every posterior diagonal the posterior
+ * p = exp(F_from + B_to + eP + tP - total) of each transition into each cell.
+ *   transitions[from * 3 + to]  summed p (match 0, gapX 1, gapY 2), no pseudocount
+ *   likelihood                  the window total summed over the posterior diagonals
+ *   match_posterior[k]          summed p of the transitions into the match state of
+ *                               cells of k-mer k
+ *   match_event_sum[k]          summed p x descaled event mean, (e + var mu_k -
+ *                               scale mu_k - shift) / var with the read's parameters
+ *                               and mu_k the model's level mean
+ *   observed[k]                 1 where a match transition into k-mer k was evaluated
+ *   records                     the cells whose largest match-transition posterior is
+ *                               >= record_threshold and above zero (threshold 0: every
+ *                               cell that adds to the sums), in the reference's call order
+ *                               (sub-alignments in order, traceback windows in
+ *                               order, each from its top posterior diagonal down,
+ *                               cells by ascending x): k-mer x, event y, p of M->M,
+ *                               X->M, Y->M.  They carry what the order-dependent
+ *                               accumulators (running variance, HDP assignments) need.
+ * The result stays valid until sa_expect_free. */
+typedef struct {
+    int32_t x, y;
+    double p[3];
+} sa_match_record;
+typedef struct sa_expect_result sa_expect_result;
+typedef struct {
+    double transitions[9];
+    double likelihood;
+    int64_t n_kmers;
+    const double *match_posterior;
+    const double *match_event_sum;
+    const int8_t *observed;
+    const sa_match_record *records;
+    int64_t n_records;
+} sa_expectations;
+int sa_expect(sa_context *ctx, const sa_model *model, const sa_read *read, const sa_params *params,
+              double record_threshold, sa_expect_result **out);
+int sa_expect_get(const sa_expect_result *res, sa_expectations *out);
+void sa_expect_free(sa_expect_result *res);
+
 /* Host-side views of the integer geometry the kernels compute on the device
  * (same code, sa_common.h), for validation and for callers that plan work:
  *   sa_band_geometry: band_construct (impl/pairwiseAligner.c:131-182) as per-diagonal

@@ -609,7 +609,7 @@ static void expectations_diagonal(const or_seqs *s, int64_t xay, const or_matrix
             expect_transition(ex, mi, c, Y_ST, M_ST, eP, t[OR_T_MATCH_FROM_GAP_Y], total, kidx, xd);
             double mx = pr[0] > pr[1] ? pr[0] : pr[1];
             mx = mx > pr[2] ? mx : pr[2];
-            if (ex->rec_cap > 0 && mx >= ex->rec_threshold) {
+            if (ex->rec_cap > 0 && mx >= ex->rec_threshold && mx > 0.0) {
                 if (ex->n_rec < ex->rec_cap) {
                     double *r = ex->rec + 5 * ex->n_rec;
                     r[0] = (double)(x - 1 + ex->x1);

@@ -90,7 +90,8 @@ typedef struct {
     double *posteriors;        /* per k-mer: pseudocount + sum p */
     unsigned char *observed;   /* per k-mer */
     /* match-transition posteriors in call order: 5 doubles per cell (k-mer position
-       x, event y, p of M->M, X->M, Y->M), cells whose largest p >= rec_threshold */
+       x, event y, p of M->M, X->M, Y->M), cells whose largest p >= rec_threshold
+       and > 0 */
     double *rec;
     int64_t rec_cap, n_rec;
     double rec_threshold;

@@ -51,6 +51,16 @@ struct WinDev {
     int32_t pad;
     int64_t out_off;
     int64_t cells; /* cells of the diagonals (T, TF] (set by prep) */
+    int64_t rec_off; /* first expectation record of this window (EM expectations runs) */
+};
+
+/* One cell's match-transition posteriors of an EM expectations run: k-mer position
+ * x, event y (0-based, sub-alignment coordinates on the device) and the posteriors
+ * of M->M, X->M, Y->M into the cell (cell_signal_updateExpectations,
+ * impl/pairwiseAligner.c:763-790). */
+struct ExRec {
+    int32_t x, y;
+    double p[3];
 };
 
 SA_HD int64_t div2(int64_t v) { return v / 2; } /* C truncation, as the reference */

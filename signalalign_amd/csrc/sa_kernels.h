@@ -81,6 +81,20 @@ struct Args {
     /* LDS sweeps whose ring does not fit in LDS: per-workgroup slices of HBM */
     unsigned char *ring_g;
     int64_t ring_g_stride;
+    /* EM expectations (FP64 LDS sweeps only; diagonalCalculation_Expectations,
+       impl/pairwiseAligner.c:1255-1275): every diagonal keeps its gapX/gapY states;
+       per window 9 transition posterior sums (from * 3 + to) and the likelihood
+       (sum of the window total over its posterior diagonals); per task and k-mer the
+       sum of match-transition posteriors and of posterior x descaled event mean, and
+       the k-mers a match transition was evaluated for; per window the cells whose
+       largest match-transition posterior reaches ex_rec_threshold, in call order */
+    int32_t expect;
+    double ex_rec_threshold;
+    double *ex_trans;       /* 10 per window */
+    double *ex_s0, *ex_s1;  /* n_kmers per task */
+    int8_t *ex_obs;         /* n_kmers per task */
+    ExRec *ex_rec;          /* W.rec_off.. (capacity W.cells) */
+    int32_t *ex_count;      /* records per window */
 };
 
 template <typename T> hipError_t launch_prep(const Args<T> &a, int n_tasks, hipStream_t st);

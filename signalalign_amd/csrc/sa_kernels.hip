@@ -106,7 +106,8 @@ __global__ void __launch_bounds__(256) prep_kernel(Args<T> a) {
     }
     /* the refresh at diagonal e (every 10th posterior diagonal of a window, counted
        down from tracedBackFrom) reads the forward gapX/gapY states of e and e-1 */
-    for (int64_t d = tid; d <= N; d += nt) a.dflag[t.diag_off + d] = 0;
+    /* (an expectations run keeps every diagonal's gapX/gapY states) */
+    for (int64_t d = tid; d <= N; d += nt) a.dflag[t.diag_off + d] = a.expect ? 1 : 0;
     __syncthreads();
     if (tid == 0) {
         for (int w = 0; w < s_nwin; w++) {
@@ -353,7 +354,103 @@ __device__ __forceinline__ double chain(const double *term, int n) {
     return acc;
 }
 
-template <typename T>
+/* diagonalCalculation_Expectations (impl/pairwiseAligner.c:1255-1275) for posterior
+ * diagonal e of window W, FP64: each cell's transitions from the forward cells of
+ * e-1 (lower, upper) and e-2 (middle) into its backward states (rc, the ring),
+ * p = exp(F_from + B_to + (eP + tP) - total), in stateMachine3_cellCalculate's
+ * order (cell_signal_updateExpectations, :763-790).  The reference has deleted
+ * forward diagonal T-1 when window (T, D] runs, so diagonal T+1 sees no middle
+ * cells.  Match transitions also add to the k-mer's posterior and descaled-event
+ * sums (continuousPairHmm_addToEmissionExpectation, impl/continuousHmm.c:160-169)
+ * and, above the record threshold, to the window's records in call order. */
+__device__ __forceinline__ void expect_diagonal(const Args<double> &a, const TaskDev &t, const WinDev &W, int wi,
+                                             int e, int lo, int w, const double *rc, int R, double total,
+                                             double *ex, double &ex_like, int &ex_cursor) {
+    const Model<double> &m = a.m;
+    const int lane = threadIdx.x;
+    const int RM = R - 1;
+    const int32_t *xlo = a.xlo + t.diag_off;
+    const int32_t *xhi = a.xhi + t.diag_off;
+    const int32_t *doff = a.doff + t.diag_off;
+    const int32_t *doffxy = a.doffxy + t.diag_off;
+    const int32_t *kid = a.kid + t.pos_off;
+    const int64_t evc = t.evc_off;
+    const double s = m.kind == 1 ? 1.0 : t.scale / t.var;
+    const double inv_var = 1.0 / t.var;
+    const double tMC = m.trans[0], tMFX = m.trans[1], tMFY = m.trans[2], tGOX = m.trans[3], tGOY = m.trans[4];
+    const double tGEX = m.trans[5], tGEY = m.trans[6], tSWX = m.trans[7];
+    ex_like += total;
+    const int lo1 = xlo[e - 1], hi1 = xhi[e - 1];
+    const bool mid = e - 2 >= W.T && e >= 2;
+    const int lo2 = mid ? xlo[e - 2] : 1, hi2 = mid ? xhi[e - 2] : 0;
+    const int64_t f1 = t.cell_off + doff[e - 1], fxy1 = t.xy_off + doffxy[e - 1];
+    const int64_t f2 = mid ? t.cell_off + doff[e - 2] : 0, fxy2 = mid ? t.xy_off + doffxy[e - 2] : 0;
+    double *s0 = a.ex_s0 + (int64_t)W.task * m.n_kmers;
+    double *s1 = a.ex_s1 + (int64_t)W.task * m.n_kmers;
+    int8_t *obs = a.ex_obs + (int64_t)W.task * m.n_kmers;
+    const double *mu = m.kind == 2 ? m.hmu : m.mmu;
+    for (int i0 = 0; i0 < w; i0 += 64) {
+        const int i = i0 + lane;
+        bool keep = false;
+        double q0 = 0.0, q1 = 0.0, q2 = 0.0;
+        int x = 0, y = 0;
+        if (i < w) {
+            x = lo + i;
+            y = e - x;
+            const double BM = rc[x & RM], BX = rc[R + (x & RM)], BY = rc[2 * R + (x & RM)];
+            if (x - 1 >= lo1 && x - 1 <= hi1) { /* lower (x-1, y) */
+                const int64_t j = x - 1 - lo1;
+                const double eX = Emit<double>::gapx(m, kid[x - 1]);
+                ex[0 * 3 + 1] += exp(a.fM[f1 + j] + BX + (eX + tGOX) - total);
+                ex[1 * 3 + 1] += exp(a.fX[fxy1 + j] + BX + (eX + tGEX) - total);
+                ex[2 * 3 + 1] += exp(a.fY[fxy1 + j] + BX + (eX + tSWX) - total);
+            }
+            if (x - 1 >= lo2 && x - 1 <= hi2) { /* middle (x-1, y-1): x >= 1, y >= 1 */
+                const int64_t j = x - 1 - lo2;
+                const int32_t kk = kid[x - 1];
+                const double eM = Emit<double>::match(m, a, kk, evc + (y - 1), s, inv_var);
+                q0 = exp(a.fM[f2 + j] + BM + (eM + tMC) - total);
+                q1 = exp(a.fX[fxy2 + j] + BM + (eM + tMFX) - total);
+                q2 = exp(a.fY[fxy2 + j] + BM + (eM + tMFY) - total);
+                ex[0] += q0;
+                ex[1 * 3 + 0] += q1;
+                ex[2 * 3 + 0] += q2;
+                /* emissions_signal_descaleEventMean_JordanStyle (impl/stateMachine.c:285-289) */
+                const double xm = a.events[(t.ev_off + y - 1) * a.event_stride];
+                const double u = mu[kk];
+                const double xd = (xm + t.var * u - t.scale * u - t.shift) / t.var;
+                const double qs = q0 + q1 + q2;
+                atomicAdd(&s0[kk], qs);
+                atomicAdd(&s1[kk], qs * xd);
+                obs[kk] = 1;
+                const double qm = fmax(q0, fmax(q1, q2));
+                keep = qm >= a.ex_rec_threshold && qm > 0.0; /* a zero posterior adds nothing */
+            }
+            if (x >= lo1 && x <= hi1 && x >= 1) { /* upper (x, y-1); k-mer x-1 (none at x = 0) */
+                const int64_t j = x - lo1;
+                const double eY = Emit<double>::gapy(m, a, kid[x - 1], evc + (y - 1), s, inv_var);
+                ex[0 * 3 + 2] += exp(a.fM[f1 + j] + BY + (eY + tGOY) - total);
+                ex[2 * 3 + 2] += exp(a.fY[fxy1 + j] + BY + (eY + tGEY) - total);
+            }
+        }
+        const unsigned long long bal = __ballot(keep);
+        if (keep) {
+            const int idx = ex_cursor + __popcll(bal & ((1ull << lane) - 1ull));
+            if (idx < W.cells) {
+                ExRec r;
+                r.x = x - 1;
+                r.y = y - 1;
+                r.p[0] = q0;
+                r.p[1] = q1;
+                r.p[2] = q2;
+                a.ex_rec[W.rec_off + idx] = r;
+            }
+        }
+        ex_cursor += __popcll(bal);
+    }
+}
+
+template <typename T, bool EX>
 __global__ void __launch_bounds__(64) backward_kernel(Args<T> a) {
     constexpr bool NORM = sizeof(T) == 4;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -393,6 +490,9 @@ __global__ void __launch_bounds__(64) backward_kernel(Args<T> a) {
     int32_t *out = a.out + 3 * W.out_off;
     int cursor = 0;
     int posterior_count = 0;
+    double ex[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}; /* this lane's transition posterior sums */
+    double ex_like = 0.0;
+    int ex_cursor = 0;
     double total = -__builtin_huge_val();
     double ob1 = 0.0, ob2 = 0.0; /* offsets of B[e+1], B[e+2] */
 
@@ -540,11 +640,23 @@ __global__ void __launch_bounds__(64) backward_kernel(Args<T> a) {
                 }
                 cursor += __popcll(bal);
             }
+            if constexpr (EX) expect_diagonal(a, t, W, wi, e, lo, w, rc, R, total, ex, ex_like, ex_cursor);
         }
         ob2 = ob1;
         ob1 = obe;
     }
     if (lane == 0) a.win_count[wi] = cursor;
+    if constexpr (EX) {
+        for (int k = 0; k < 9; k++) {
+            double v = ex[k];
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+            if (lane == 0) a.ex_trans[10 * (int64_t)wi + k] = v;
+        }
+        if (lane == 0) {
+            a.ex_trans[10 * (int64_t)wi + 9] = ex_like;
+            a.ex_count[wi] = ex_cursor;
+        }
+    }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -591,7 +703,14 @@ hipError_t launch_forward(const Args<T> &a, int n_tasks, int nt, hipStream_t st)
 template <typename T>
 hipError_t launch_backward(const Args<T> &a, int n_wins, hipStream_t st) {
     const size_t lds = a.ring_g ? 0 : backward_block_bytes<T>(a.ring);
-    hipLaunchKernelGGL((backward_kernel<T>), dim3(n_wins), dim3(64), lds, st, a);
+    /* EM expectations: FP64 only (the runtime refuses FP32 expectations runs) */
+    if constexpr (sizeof(T) == 8) {
+        if (a.expect) {
+            hipLaunchKernelGGL((backward_kernel<T, true>), dim3(n_wins), dim3(64), lds, st, a);
+            return hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL((backward_kernel<T, false>), dim3(n_wins), dim3(64), lds, st, a);
     return hipGetLastError();
 }
 

@@ -173,6 +173,10 @@ struct sa_batch {
     DBuf d_fM, d_fX, d_fY;
     DBuf d_wins, d_task_order, d_win_order, d_out, d_win_count, d_win_total, d_win_dst, d_packed;
     DBuf d_ring_g; /* LDS sweeps' rings in HBM when a band is too wide for LDS */
+    /* EM expectations run (sa_expect): FP64 LDS sweeps with the accumulators */
+    int32_t expect = 0;
+    double ex_rec_threshold = 1.0;
+    DBuf d_ex_trans, d_ex_s0, d_ex_s1, d_ex_obs, d_ex_rec, d_ex_count;
     sa_tuning tune{1, 0, 2.5, 0.5};
     int32_t nruns[2] = {0, 0}; /* runs since the batch or its tuning last changed, per precision */
     /* a recorded run per precision (0 FP64, 1 FP32), replayed while the batch is unchanged */
@@ -197,17 +201,32 @@ struct sa_batch {
                        &d_evl, &d_evi, &d_xlo, &d_xhi, &d_doff, &d_foff, &d_dflag, &d_dcls, &d_doffxy, &d_task_cells,
                        &d_task_xycells, &d_task_maxw, &d_task_span,
                        &d_task_nwin, &d_win_slots, &d_err, &d_fM, &d_fX, &d_fY, &d_wins, &d_task_order,
-                       &d_win_order, &d_out, &d_win_count, &d_win_total, &d_win_dst, &d_packed, &d_ring_g};
+                       &d_win_order, &d_out, &d_win_count, &d_win_total, &d_win_dst, &d_packed, &d_ring_g,
+                       &d_ex_trans, &d_ex_s0, &d_ex_s1, &d_ex_obs, &d_ex_rec, &d_ex_count};
         for (DBuf *b : all) b->release();
     }
 };
 
+
+/* EM expectations of one read (summed over its sub-alignments) */
+struct ReadExpect {
+    double trans[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double likelihood = 0.0;
+    std::vector<double> s0, s1;
+    std::vector<int8_t> obs;
+    std::vector<sa_match_record> recs;
+};
 
 struct sa_result {
     std::vector<sa_pair> pairs;
     std::vector<ReadOut> reads;
     sa_timing timing;
     std::vector<sa_launch> launches;
+    std::vector<ReadExpect> expect; /* expectations runs only */
+};
+
+struct sa_expect_result {
+    ReadExpect e;
 };
 
 namespace {
@@ -801,6 +820,9 @@ template <typename T>
 int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     sa_context *ctx = b->ctx;
     const sa_model *model = b->model;
+    const bool expect = b->expect != 0;
+    if (expect && sizeof(T) != 8) return fail(ctx, SA_ERR_UNSUPPORTED, "expectations run in FP64");
+    force_lds = force_lds || expect;
     hipStream_t st = ctx->stream;
     const int64_t nt = (int64_t)b->tasks.size();
     sa_timing &tm = res->timing;
@@ -867,6 +889,8 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     a.min_diags = b->params.min_diags_between_traceback;
     a.traceback = b->params.traceback_diagonals;
     a.expansion = b->params.diagonal_expansion;
+    a.expect = expect ? 1 : 0;
+    a.ex_rec_threshold = b->ex_rec_threshold;
 
     SA_HIP(ctx, hipEventRecord(ctx->ev[0], st));
     SA_HIP(ctx, sa::launch_prep<T>(a, (int)nt, st));
@@ -899,7 +923,7 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     }
     std::vector<WinDev> wins;
     std::vector<int32_t> task_first_win(nt);
-    int64_t out_total = 0;
+    int64_t out_total = 0, rec_total = 0;
     for (int64_t i = 0; i < nt; i++) {
         task_first_win[i] = (int32_t)wins.size();
         for (int32_t w = 0; w < nwin[i]; w++) {
@@ -908,6 +932,8 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
             W.out_off = out_total;
             W.out_cap = 2 * (W.TF - W.T) + 64;
             out_total += W.out_cap;
+            W.rec_off = rec_total;
+            if (expect) rec_total += W.cells;
             wins.push_back(W);
         }
     }
@@ -1086,6 +1112,30 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     a.win_count = b->d_win_count.as<int32_t>();
     a.win_total = b->d_win_total.as<double>();
     a.ring = ring;
+    const int64_t nk = model->n_kmers;
+    auto zero_expect = [&]() -> int {
+        SA_HIP(ctx, hipMemsetAsync(b->d_ex_trans.p, 0, sizeof(double) * 10 * std::max<int64_t>(nw, 1), st));
+        SA_HIP(ctx, hipMemsetAsync(b->d_ex_s0.p, 0, sizeof(double) * nt * nk, st));
+        SA_HIP(ctx, hipMemsetAsync(b->d_ex_s1.p, 0, sizeof(double) * nt * nk, st));
+        SA_HIP(ctx, hipMemsetAsync(b->d_ex_obs.p, 0, sizeof(int8_t) * nt * nk, st));
+        SA_HIP(ctx, hipMemsetAsync(b->d_ex_count.p, 0, sizeof(int32_t) * std::max<int64_t>(nw, 1), st));
+        return SA_OK;
+    };
+    if (expect) {
+        SA_HIP(ctx, b->d_ex_trans.ensure(sizeof(double) * 10 * std::max<int64_t>(nw, 1)));
+        SA_HIP(ctx, b->d_ex_s0.ensure(sizeof(double) * nt * nk));
+        SA_HIP(ctx, b->d_ex_s1.ensure(sizeof(double) * nt * nk));
+        SA_HIP(ctx, b->d_ex_obs.ensure(sizeof(int8_t) * nt * nk));
+        SA_HIP(ctx, b->d_ex_rec.ensure(sizeof(sa::ExRec) * std::max<int64_t>(rec_total, 1)));
+        SA_HIP(ctx, b->d_ex_count.ensure(sizeof(int32_t) * std::max<int64_t>(nw, 1)));
+        if ((rc = zero_expect())) return rc;
+        a.ex_trans = b->d_ex_trans.as<double>();
+        a.ex_s0 = b->d_ex_s0.as<double>();
+        a.ex_s1 = b->d_ex_s1.as<double>();
+        a.ex_obs = b->d_ex_obs.as<int8_t>();
+        a.ex_rec = b->d_ex_rec.as<sa::ExRec>();
+        a.ex_count = b->d_ex_count.as<int32_t>();
+    }
     const int32_t *d_to = b->d_task_order.as<int32_t>();
     const int32_t *d_wo = b->d_win_order.as<int32_t>();
     int64_t n_lds = 0;
@@ -1211,6 +1261,7 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
         if ((rc = upload(ctx, b->d_wins, wins))) return rc;
         a.out = b->d_out.as<int32_t>();
         a.wins = b->d_wins.as<WinDev>();
+        if (expect && (rc = zero_expect())) return rc; /* the sums start again */
         if ((rc = run_sweeps(false))) return rc;
         tm.backward_launches += (int64_t)bl.size();
         SA_HIP(ctx, hipMemcpyAsync(wcount.data(), b->d_win_count.p, sizeof(int32_t) * nw, hipMemcpyDeviceToHost, st));
@@ -1255,6 +1306,52 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     SA_HIP(ctx, hipMemcpyAsync(wtot.data(), b->d_win_total.p, sizeof(double) * nw, hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipStreamSynchronize(st));
     res->pairs.assign(ctx->stage.as<sa_pair>(), ctx->stage.as<sa_pair>() + total_pairs);
+    if (expect) {
+        /* per read: sub-alignments in split order, each window's records in call order */
+        std::vector<double> etr(10 * nw), es0(nt * nk), es1(nt * nk);
+        std::vector<int8_t> eobs(nt * nk);
+        std::vector<int32_t> ecnt(nw);
+        std::vector<sa::ExRec> erec(std::max<int64_t>(rec_total, 1));
+        SA_HIP(ctx, hipMemcpyAsync(etr.data(), b->d_ex_trans.p, sizeof(double) * 10 * nw, hipMemcpyDeviceToHost, st));
+        SA_HIP(ctx, hipMemcpyAsync(es0.data(), b->d_ex_s0.p, sizeof(double) * nt * nk, hipMemcpyDeviceToHost, st));
+        SA_HIP(ctx, hipMemcpyAsync(es1.data(), b->d_ex_s1.p, sizeof(double) * nt * nk, hipMemcpyDeviceToHost, st));
+        SA_HIP(ctx, hipMemcpyAsync(eobs.data(), b->d_ex_obs.p, nt * nk, hipMemcpyDeviceToHost, st));
+        SA_HIP(ctx, hipMemcpyAsync(ecnt.data(), b->d_ex_count.p, sizeof(int32_t) * nw, hipMemcpyDeviceToHost, st));
+        if (rec_total)
+            SA_HIP(ctx, hipMemcpyAsync(erec.data(), b->d_ex_rec.p, sizeof(sa::ExRec) * rec_total,
+                                       hipMemcpyDeviceToHost, st));
+        SA_HIP(ctx, hipStreamSynchronize(st));
+        res->expect.assign(b->n_reads, ReadExpect());
+        for (int64_t r = 0; r < b->n_reads; r++) {
+            ReadExpect &re = res->expect[r];
+            re.s0.assign(nk, 0.0);
+            re.s1.assign(nk, 0.0);
+            re.obs.assign(nk, 0);
+            for (int64_t ti = b->reads[r].first_task; ti < b->reads[r].first_task + b->reads[r].n_tasks; ti++) {
+                for (int64_t k = 0; k < nk; k++) {
+                    re.s0[k] += es0[ti * nk + k];
+                    re.s1[k] += es1[ti * nk + k];
+                    re.obs[k] |= eobs[ti * nk + k];
+                }
+                for (int32_t w = 0; w < nwin[ti]; w++) {
+                    const int64_t wi = task_first_win[ti] + w;
+                    for (int k = 0; k < 9; k++) re.trans[k] += etr[10 * wi + k];
+                    re.likelihood += etr[10 * wi + 9];
+                    const int64_t n = std::min<int64_t>(ecnt[wi], wins[wi].cells);
+                    for (int64_t q = 0; q < n; q++) {
+                        const sa::ExRec &x = erec[wins[wi].rec_off + q];
+                        sa_match_record m;
+                        m.x = x.x + b->thost[ti].x1;
+                        m.y = x.y + b->thost[ti].y1;
+                        m.p[0] = x.p[0];
+                        m.p[1] = x.p[1];
+                        m.p[2] = x.p[2];
+                        re.recs.push_back(m);
+                    }
+                }
+            }
+        }
+    }
     for (int64_t r = 0; r < b->n_reads; r++) {
         double ll = 0.0;
         for (int64_t ti = b->reads[r].first_task; ti < b->reads[r].first_task + b->reads[r].n_tasks; ti++)
@@ -1424,6 +1521,47 @@ int sa_align(sa_context *ctx, const sa_model *model, const sa_read *read, const 
              sa_precision precision, sa_result **out) {
     return sa_align_batch(ctx, model, read, 1, params, precision, out);
 }
+
+int sa_expect(sa_context *ctx, const sa_model *model, const sa_read *read, const sa_params *params,
+              double record_threshold, sa_expect_result **out) {
+    if (!out) return SA_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (!(record_threshold >= 0.0)) return SA_ERR_INVALID_ARGUMENT;
+    sa_batch *b = nullptr;
+    int rc = sa_batch_create(ctx, model, read, 1, params, &b);
+    if (rc) return rc;
+    b->expect = 1;
+    b->ex_rec_threshold = record_threshold;
+    sa_result *res = nullptr;
+    rc = sa_batch_run(b, SA_FP64, &res);
+    sa_batch_destroy(b);
+    if (rc) return rc;
+    sa_expect_result *er = new sa_expect_result();
+    if (!res->expect.empty()) er->e = std::move(res->expect[0]);
+    else {
+        er->e.s0.assign(model->n_kmers, 0.0);
+        er->e.s1.assign(model->n_kmers, 0.0);
+        er->e.obs.assign(model->n_kmers, 0);
+    }
+    delete res;
+    *out = er;
+    return SA_OK;
+}
+
+int sa_expect_get(const sa_expect_result *res, sa_expectations *out) {
+    if (!res || !out) return SA_ERR_INVALID_ARGUMENT;
+    for (int k = 0; k < 9; k++) out->transitions[k] = res->e.trans[k];
+    out->likelihood = res->e.likelihood;
+    out->n_kmers = (int64_t)res->e.s0.size();
+    out->match_posterior = res->e.s0.data();
+    out->match_event_sum = res->e.s1.data();
+    out->observed = res->e.obs.data();
+    out->records = res->e.recs.data();
+    out->n_records = (int64_t)res->e.recs.size();
+    return SA_OK;
+}
+
+void sa_expect_free(sa_expect_result *res) { delete res; }
 
 int sa_band_geometry(const int64_t *anchors, int64_t n_anchors, int64_t lX, int64_t lY, int64_t expansion,
                      int32_t *xlo, int32_t *xhi) {

@@ -50,6 +50,18 @@ class _Pair(ctypes.Structure):
     _fields_ = [("p", ctypes.c_int32), ("x", ctypes.c_int32), ("y", ctypes.c_int32)]
 
 
+class _MatchRecord(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_int32), ("y", ctypes.c_int32), ("p", ctypes.c_double * 3)]
+
+
+class _Expectations(ctypes.Structure):
+    _fields_ = [("transitions", ctypes.c_double * 9), ("likelihood", ctypes.c_double), ("n_kmers", ctypes.c_int64),
+                ("match_posterior", ctypes.POINTER(ctypes.c_double)),
+                ("match_event_sum", ctypes.POINTER(ctypes.c_double)),
+                ("observed", ctypes.POINTER(ctypes.c_int8)), ("records", ctypes.POINTER(_MatchRecord)),
+                ("n_records", ctypes.c_int64)]
+
+
 class _ReadResult(ctypes.Structure):
     _fields_ = [("pairs", ctypes.POINTER(_Pair)), ("n_pairs", ctypes.c_int64), ("log_likelihood", ctypes.c_double),
                 ("cells", ctypes.c_int64), ("n_windows", ctypes.c_int64), ("n_subalignments", ctypes.c_int64)]
@@ -136,6 +148,10 @@ def load():
     L.sa_result_launches.argtypes = [vp, ctypes.POINTER(_Launch), ctypes.c_int64]
     L.sa_result_launches.restype = ctypes.c_int64
     L.sa_result_free.argtypes = [vp]
+    L.sa_expect.argtypes = [vp, vp, ctypes.POINTER(_Read), ctypes.POINTER(_Params), ctypes.c_double,
+                            ctypes.POINTER(vp)]
+    L.sa_expect_get.argtypes = [vp, ctypes.POINTER(_Expectations)]
+    L.sa_expect_free.argtypes = [vp]
     L.sa_last_error.argtypes = [vp]
     L.sa_last_error.restype = ctypes.c_char_p
     I32P = ctypes.POINTER(ctypes.c_int32)
@@ -283,6 +299,39 @@ class Context:
 
     def align(self, model, read, params=None, precision=SA_FP64):
         return self.align_batch(model, [read], params, precision)[0][0]
+
+    def expect(self, model, read, params=None, record_threshold=1.0):
+        """EM expectations of one read (sa_expect, FP64).  Returns a dict:
+        transitions (3, 3) [from, to] without pseudocounts, likelihood,
+        match_posterior / match_event_sum (n_kmers,), observed (bool), records (n, 5)
+        float64: k-mer position, event, p(M->M), p(X->M), p(Y->M) in call order."""
+        params = params or Params()
+        rs = read._struct()
+        ps = params._struct()
+        res = ctypes.c_void_p()
+        L = self._L
+        self._check(L.sa_expect(self._h, model._h, ctypes.byref(rs), ctypes.byref(ps), float(record_threshold),
+                                ctypes.byref(res)), "sa_expect")
+        try:
+            e = _Expectations()
+            L.sa_expect_get(res, ctypes.byref(e))
+            nk = e.n_kmers
+            out = {"transitions": np.array(list(e.transitions)).reshape(3, 3), "likelihood": e.likelihood,
+                   "match_posterior": np.ctypeslib.as_array(e.match_posterior, shape=(nk,)).copy(),
+                   "match_event_sum": np.ctypeslib.as_array(e.match_event_sum, shape=(nk,)).copy(),
+                   "observed": np.ctypeslib.as_array(e.observed, shape=(nk,)).astype(bool)}
+            rec = np.zeros((e.n_records, 5))
+            if e.n_records:
+                raw = np.ctypeslib.as_array(ctypes.cast(e.records, ctypes.POINTER(ctypes.c_uint8)),
+                                            shape=(e.n_records * ctypes.sizeof(_MatchRecord),))
+                st = np.frombuffer(raw.tobytes(), dtype=np.dtype([("x", "<i4"), ("y", "<i4"), ("p", "<f8", 3)]))
+                rec[:, 0] = st["x"]
+                rec[:, 1] = st["y"]
+                rec[:, 2:] = st["p"]
+            out["records"] = rec
+            return out
+        finally:
+            L.sa_expect_free(res)
 
     def align_batch(self, model, reads, params=None, precision=SA_FP64):
         """Returns ([ReadResult], timing dict)."""

@@ -155,3 +155,78 @@ def test_expectation_records_follow_the_call_order(oracle_lib):
     assert np.allclose(s0, ex["posteriors"], rtol=1e-12, atol=0)
     assert np.allclose(s1, ex["event_exp"][:, 0], rtol=1e-12, atol=1e-12)
     assert np.allclose(s2, ex["event_exp"][:, 1], rtol=1e-12, atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# the GPU path (sa_expect) against the oracle
+# ---------------------------------------------------------------------------
+REC_T = 0.0  # every cell whose match transitions add to the sums
+
+
+def _gpu_vs_oracle(gpu_ctx, m, read, params, kind=None):
+    from signalalign_amd import native
+    from tests import _parity as P
+    om = P.oracle_model(m, read, kind)
+    want = oracle.get_expectations(om, read["ref"], read["events"], read["anchors"],
+                                   oracle.Params(params.threshold, params.min_diags_between_traceback,
+                                                 params.traceback_diagonals, params.diagonal_expansion,
+                                                 params.split_matrix_bigger_than_this),
+                                   rec_threshold=REC_T)
+    dm = gpu_ctx.model(m) if kind is None else gpu_ctx.model(m, kind)
+    got = gpu_ctx.expect(dm, P.native_read(read), params, record_threshold=REC_T)
+    assert np.allclose(got["transitions"] + 0.001, want["transitions"], rtol=1e-9, atol=1e-12)
+    assert abs(got["likelihood"] - want["likelihood"]) <= 1e-12 * abs(want["likelihood"])
+    assert np.array_equal(got["observed"], want["observed"])
+    assert np.allclose(got["match_posterior"] + 0.001, want["posteriors"], rtol=1e-9, atol=1e-12)
+    assert np.allclose(got["match_event_sum"], want["event_exp"][:, 0], rtol=1e-9, atol=1e-9)
+    gr, wr = got["records"], want["records"]
+    assert gr.shape == wr.shape and len(gr) > 0
+    assert np.array_equal(gr[:, :2], wr[:, :2])
+    assert np.allclose(gr[:, 2:], wr[:, 2:], rtol=1e-9, atol=1e-15)
+    # the running variance term replayed from the records (order-dependent), against
+    # the oracle's
+    kid = models.kmer_ids(read["ref"], m.alphabet, m.kmer_length)
+    nk = len(got["observed"])
+    mu = m.level_mean if isinstance(m, models.HdpModel) else m.match[:, 0]
+    s0, s1, s2 = np.full(nk, 0.001), np.zeros(nk), np.zeros(nk)
+    for x, y, p1, p2, p3 in gr:
+        kk = int(kid[int(x)])
+        e = read["events"][int(y)][0]
+        xd = (e + read["var"] * mu[kk] - read["scale"] * mu[kk] - read["shift"]) / read["var"]
+        for p in (p1, p2, p3):
+            s1[kk] += p * xd
+            s0[kk] += p
+            u = s1[kk] / s0[kk]
+            s2[kk] += p * (xd - u) * (xd - u)
+    assert np.allclose(s2, want["event_exp"][:, 1], rtol=1e-9, atol=1e-12)
+    return got, want
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,expansion,seed", [(300, 20, 1), (2500, 50, 2), (4000, 20, 3)])
+def test_gpu_expectations_match_oracle(gpu_ctx, oracle_lib, n, expansion, seed):
+    """Synthetic reads on a 6-mer Gaussian model (descaled emissions), one to several
+    traceback windows: transition and k-mer expectations, likelihood, observed
+    k-mers and the ordered match records equal the oracle's (FP64)."""
+    from signalalign_amd import native, synth
+    m = models.synthetic_gaussian_model("ACGT", 6, seed=7)
+    r = synth.single_read(m, n, seed=seed)
+    print(f"expect n={n} e={expansion}", flush=True)
+    got, want = _gpu_vs_oracle(gpu_ctx, m, r, native.Params(diagonal_expansion=expansion))
+    print(len(got["records"]), got["likelihood"])
+
+
+@pytest.mark.gpu
+def test_gpu_expectations_split_and_hdp(gpu_ctx, oracle_lib):
+    """A read cut into sub-alignments (split matrix 100^2 with sparse anchors) and a
+    read on an HDP density grid (level means from the grid)."""
+    from signalalign_amd import native, synth
+    m = models.synthetic_gaussian_model("ACGT", 6, seed=7)
+    r = synth.single_read(m, 1200, seed=5)
+    r = dict(r, anchors=r["anchors"][::6])
+    print("expect split", flush=True)
+    _gpu_vs_oracle(gpu_ctx, m, r, native.Params(diagonal_expansion=20, split_matrix_bigger_than_this=100 * 100))
+    h = models.synthetic_hdp_model("ACGT", 5, seed=2)
+    r2 = synth.single_read(h, 800, seed=6)
+    print("expect hdp", flush=True)
+    _gpu_vs_oracle(gpu_ctx, h, r2, native.Params(diagonal_expansion=20))
