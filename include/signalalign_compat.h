@@ -220,6 +220,49 @@ stList *getAlignedPairsUsingAnchors(
                                     PairwiseAlignmentParameters *, void *),
     bool alignmentHasRaggedLeftEnd, bool alignmentHasRaggedRightEnd);
 
+/* ---- EM expectations (inc/stateMachine.h:49-74, inc/continuousHmm.h) ------- */
+/* The reference's Hmm header; the expectations HMMs below extend it. */
+typedef struct _hmm Hmm;
+struct _hmm {
+    double likelihood;
+    StateMachineType type;
+    int64_t stateNumber;
+    int64_t parameterSetSize;
+    int64_t matrixSize;
+    char *alphabet;
+    int64_t alphabetSize;
+    int64_t kmerLength;
+    double *transitions;  /* stateNumber^2: from * stateNumber + to */
+    void (*addToTransitionExpectationFcn)(Hmm *hmm, int64_t from, int64_t to, double p);
+    void (*setTransitionFcn)(Hmm *hmm, int64_t from, int64_t to, double p);
+    double (*getTransitionsExpFcn)(Hmm *hmm, int64_t from, int64_t to);
+};
+
+/* impl/continuousHmm.c:842-855: a ContinuousPairHmm (threeState: transition and
+ * per-k-mer event expectations, pseudocounts as given) or an HdpHmm
+ * (threeStateHdp: transition expectations and event assignments whose match
+ * posterior reaches `threshold`), with the machine's scale, shift, var and match
+ * table. */
+Hmm *hmmContinuous_getExpectationsHmm(StateMachine *sM, double threshold, double transitionsPseudocount,
+                                      double emissionsPseudocount);
+/* Names the expectations callback of getPosteriorProbsWithBanding; the
+ * expectations run on the device, so calling it directly aborts. */
+void diagonalCalculation_Expectations(StateMachine *sM, int64_t xay, DpMatrix *forwardDpMatrix,
+                                      DpMatrix *backwardDpMatrix, Sequence *sX, Sequence *sY, double totalProbability,
+                                      PairwiseAlignmentParameters *p, void *extraArgs);
+/* Replaces impl/pairwiseAligner.c:1983-2003 (with diagonalCalculation_Expectations):
+ * adds this read's expectations to hmmExpectations, in the reference's order. */
+void getExpectationsUsingAnchors(StateMachine *sM, Hmm *hmmExpectations, Sequence *SsX, Sequence *SsY,
+                                 stList *anchorPairs, PairwiseAlignmentParameters *p,
+                                 void (*diagonalCalcExpectationFcn)(StateMachine *, int64_t, DpMatrix *, DpMatrix *,
+                                                                    Sequence *, Sequence *, double,
+                                                                    PairwiseAlignmentParameters *, void *),
+                                 bool alignmentHasRaggedLeftEnd, bool alignmentHasRaggedRightEnd);
+int64_t hmmContinuous_howManyAssignments(Hmm *hmm);
+/* impl/continuousHmm.c:872-883 (continuousPairHmm_writeToFile, hdpHmm_writeToFile) */
+void hmmContinuous_writeToFile(const char *outFile, Hmm *hmm, StateMachineType type);
+void hmmContinuous_destruct(Hmm *hmm, StateMachineType type);
+
 /* ---- additions with no counterpart in the reference ------------------------- */
 /* Device for the process-wide context (before the first alignment; default 0). */
 int sa_compat_set_device(int device);

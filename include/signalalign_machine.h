@@ -236,6 +236,11 @@ Sequence *makeEventSequenceFromPairwiseAlignment(double *events, int64_t querySt
 stList *performSignalAlignment(StateMachine *sM, Sequence *eventSequence, int64_t *eventMap, int64_t mapOffset,
                                char *target, PairwiseAlignmentParameters *p, stList *unmappedAnchors,
                                DegenerateType degenerate);
+/* signalMachine.c:371-394: the read's expectations added to hmmExpectations
+ * (anchors remapped to the events, the target as a k-mer sequence). */
+void getSignalExpectations(StateMachine *sM, Hmm *hmmExpectations, Sequence *eventSequence, int64_t *eventMap,
+                           int64_t mapOffset, char *trainingTarget, PairwiseAlignmentParameters *p,
+                           stList *unmappedAnchors, DegenerateType degenerate);
 double scoreByPosteriorProbabilityIgnoringGaps(stList *alignedPairs);
 int sortByXPlusYCoordinate2(const void *i, const void *j);
 /* Positions of degenerate ('X') bases in a k-mer (impl/pairwiseAligner.c:510-524),

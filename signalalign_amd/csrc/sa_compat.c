@@ -11,6 +11,7 @@
 #include "signalalign_machine.h"
 #include "signalalign_amd.h"
 
+#include <inttypes.h>
 #include <math.h>
 #include <pthread.h>
 #include <stdio.h>
@@ -575,22 +576,17 @@ void diagonalCalculationPosteriorMatchProbs(StateMachine *sM, int64_t xay, DpMat
                                                    "getAlignedPairsUsingAnchors instead of calling it");
 }
 
-stList *getAlignedPairsUsingAnchors(
-    StateMachine *sM, Sequence *SsX, Sequence *SsY, stList *anchorPairs, PairwiseAlignmentParameters *p,
-    void (*diagonalPosteriorProbFn)(StateMachine *, int64_t, DpMatrix *, DpMatrix *, Sequence *, Sequence *, double,
-                                    PairwiseAlignmentParameters *, void *),
-    bool alignmentHasRaggedLeftEnd, bool alignmentHasRaggedRightEnd) {
-    static const char *where = "getAlignedPairsUsingAnchors";
+/* the C-ABI read and parameters of one call (anchors copied into *anc, freed by
+   the caller) */
+static void make_read(const char *where, StateMachine *sM, Sequence *SsX, Sequence *SsY, stList *anchorPairs,
+                      PairwiseAlignmentParameters *p, bool raggedLeft, bool raggedRight, sa_read *r,
+                      sa_params *prm, int64_t **anc_out) {
     if (!sM || !SsX || !SsY || !p) die(where, "null argument");
     if (sM->type != threeState && sM->type != threeStateHdp) die(where, "needs a three-state machine");
-    if (diagonalPosteriorProbFn != diagonalCalculationPosteriorMatchProbs)
-        die(where, "only diagonalCalculationPosteriorMatchProbs is supported");
     if (SsX->type != kmer) die(where, "SsX must be a k-mer sequence");
     if (p->diagonalExpansion % 2 != 0) die(where, "diagonalExpansion must be even");
-    CompatMachine *m = (CompatMachine *)sM;
     const int64_t k = sM->kmerLength;
     const int64_t lX = SsX->length, lY = SsY->length;
-
     const int64_t na = anchorPairs ? stList_length(anchorPairs) : 0;
     int64_t *anc = (int64_t *)xmalloc(sizeof(int64_t) * (size_t)(2 * na));
     for (int64_t i = 0; i < na; i++) {
@@ -598,26 +594,43 @@ stList *getAlignedPairsUsingAnchors(
         anc[2 * i] = stIntTuple_get(t, 0);
         anc[2 * i + 1] = stIntTuple_get(t, 1);
     }
+    memset(r, 0, sizeof(*r));
+    r->ref = (const char *)SsX->elements;
+    r->ref_length = lX > 0 ? lX + k - 1 : 0;
+    r->events = (const double *)SsY->elements;
+    r->n_events = lY;
+    r->event_stride = NB_EVENT_PARAMS;
+    r->anchors = anc;
+    r->n_anchors = na;
+    r->scale = sM->scale;
+    r->shift = sM->shift;
+    r->var = sM->var;
+    r->ragged_left = raggedLeft;
+    r->ragged_right = raggedRight;
+    prm->threshold = p->threshold;
+    prm->min_diags_between_traceback = p->minDiagsBetweenTraceBack;
+    prm->traceback_diagonals = p->traceBackDiagonals;
+    prm->diagonal_expansion = p->diagonalExpansion;
+    prm->split_matrix_bigger_than_this = p->splitMatrixBiggerThanThis;
+    *anc_out = anc;
+}
+
+stList *getAlignedPairsUsingAnchors(
+    StateMachine *sM, Sequence *SsX, Sequence *SsY, stList *anchorPairs, PairwiseAlignmentParameters *p,
+    void (*diagonalPosteriorProbFn)(StateMachine *, int64_t, DpMatrix *, DpMatrix *, Sequence *, Sequence *, double,
+                                    PairwiseAlignmentParameters *, void *),
+    bool alignmentHasRaggedLeftEnd, bool alignmentHasRaggedRightEnd) {
+    static const char *where = "getAlignedPairsUsingAnchors";
+    if (diagonalPosteriorProbFn != diagonalCalculationPosteriorMatchProbs)
+        die(where, "only diagonalCalculationPosteriorMatchProbs is supported");
     sa_read r;
-    memset(&r, 0, sizeof(r));
-    r.ref = (const char *)SsX->elements;
-    r.ref_length = lX > 0 ? lX + k - 1 : 0;
-    r.events = (const double *)SsY->elements;
-    r.n_events = lY;
-    r.event_stride = NB_EVENT_PARAMS;
-    r.anchors = anc;
-    r.n_anchors = na;
-    r.scale = sM->scale;
-    r.shift = sM->shift;
-    r.var = sM->var;
-    r.ragged_left = alignmentHasRaggedLeftEnd;
-    r.ragged_right = alignmentHasRaggedRightEnd;
     sa_params prm;
-    prm.threshold = p->threshold;
-    prm.min_diags_between_traceback = p->minDiagsBetweenTraceBack;
-    prm.traceback_diagonals = p->traceBackDiagonals;
-    prm.diagonal_expansion = p->diagonalExpansion;
-    prm.split_matrix_bigger_than_this = p->splitMatrixBiggerThanThis;
+    int64_t *anc = NULL;
+    make_read(where, sM, SsX, SsY, anchorPairs, p, alignmentHasRaggedLeftEnd, alignmentHasRaggedRightEnd, &r, &prm,
+              &anc);
+    CompatMachine *m = (CompatMachine *)sM;
+    const int64_t k = sM->kmerLength;
+    const int64_t lX = SsX->length;
 
     pthread_mutex_lock(&g_mu);
     sa_context *ctx = context();
@@ -652,4 +665,222 @@ stList *getAlignedPairsUsingAnchors(
     attach_arena(out, arena);
     sa_result_free(res);
     return out;
+}
+
+/* ------------------------------------------------------------------------ */
+/* EM expectations (impl/pairwiseAligner.c:1983-2003, impl/continuousHmm.c)  */
+/* ------------------------------------------------------------------------ */
+/* The public prefix is the reference's Hmm; the rest holds a ContinuousPairHmm's
+ * accumulators (threeState) or an HdpHmm's assignments (threeStateHdp). */
+typedef struct {
+    Hmm base;
+    double scale, shift, var;
+    double *eventModel;          /* MODEL_PARAMS per k-mer: the machine's match table */
+    double *eventExpectations;   /* 2 per k-mer: sum p x~, running sum p (x~ - u)^2 */
+    double *posteriors;          /* per k-mer */
+    bool *observed;
+    double threshold;            /* HDP: assignment threshold */
+    char *assignKmers;           /* HDP: kmerLength characters per assignment */
+    double *assignEvents;
+    int64_t nAssign, capAssign;
+} ExpectHmm;
+
+static void add_transition_expectation(Hmm *hmm, int64_t from, int64_t to, double p) {
+    hmm->transitions[from * hmm->stateNumber + to] += p;
+}
+static void set_transition_expectation(Hmm *hmm, int64_t from, int64_t to, double p) {
+    hmm->transitions[from * hmm->stateNumber + to] = p;
+}
+static double get_transition_expectation(Hmm *hmm, int64_t from, int64_t to) {
+    return hmm->transitions[from * hmm->stateNumber + to];
+}
+
+Hmm *hmmContinuous_getExpectationsHmm(StateMachine *sM, double threshold, double transitionsPseudocount,
+                                      double emissionsPseudocount) {
+    static const char *where = "hmmContinuous_getExpectationsHmm";
+    if (!sM || (sM->type != threeState && sM->type != threeStateHdp)) die(where, "needs a three-state machine");
+    ExpectHmm *h = (ExpectHmm *)calloc(1, sizeof(ExpectHmm));
+    if (!h) die(where, "out of host memory");
+    const int64_t nk = sM->parameterSetSize;
+    h->base.type = sM->type;
+    h->base.stateNumber = sM->stateNumber;
+    h->base.parameterSetSize = nk;
+    h->base.matrixSize = MODEL_PARAMS;
+    h->base.likelihood = 0.0;
+    h->base.alphabet = (char *)dup_array(sM->alphabet, (size_t)sM->alphabetSize + 1);
+    h->base.alphabetSize = sM->alphabetSize;
+    h->base.kmerLength = sM->kmerLength;
+    h->base.transitions = (double *)xmalloc(sizeof(double) * (size_t)(sM->stateNumber * sM->stateNumber));
+    for (int64_t i = 0; i < sM->stateNumber * sM->stateNumber; i++) h->base.transitions[i] = transitionsPseudocount;
+    h->base.addToTransitionExpectationFcn = add_transition_expectation;
+    h->base.setTransitionFcn = set_transition_expectation;
+    h->base.getTransitionsExpFcn = get_transition_expectation;
+    h->scale = sM->scale;
+    h->shift = sM->shift;
+    h->var = sM->var;
+    h->eventModel = (double *)dup_array(sM->EMISSION_MATCH_MATRIX, sizeof(double) * (size_t)(nk * MODEL_PARAMS));
+    h->threshold = threshold;
+    if (sM->type == threeState) {
+        h->eventExpectations = (double *)calloc((size_t)(2 * nk), sizeof(double));
+        h->posteriors = (double *)xmalloc(sizeof(double) * (size_t)nk);
+        h->observed = (bool *)calloc((size_t)nk, sizeof(bool));
+        if (!h->eventExpectations || !h->observed) die(where, "out of host memory");
+        for (int64_t i = 0; i < nk; i++) h->posteriors[i] = emissionsPseudocount;
+    }
+    return &h->base;
+}
+
+void diagonalCalculation_Expectations(StateMachine *sM, int64_t xay, DpMatrix *forwardDpMatrix,
+                                      DpMatrix *backwardDpMatrix, Sequence *sX, Sequence *sY, double totalProbability,
+                                      PairwiseAlignmentParameters *p, void *extraArgs) {
+    (void)sM; (void)xay; (void)forwardDpMatrix; (void)backwardDpMatrix; (void)sX; (void)sY;
+    (void)totalProbability; (void)p; (void)extraArgs;
+    die("diagonalCalculation_Expectations", "expectations are computed on the device; pass this function to "
+                                             "getExpectationsUsingAnchors instead of calling it");
+}
+
+static int64_t kmer_index(const char *kmer, const char *alphabet, int64_t A, int64_t k) {
+    int64_t id = 0;
+    for (int64_t i = 0; i < k; i++) {
+        const char *hit = memchr(alphabet, kmer[i], (size_t)A);
+        if (!hit) die("getExpectationsUsingAnchors", "k-mer character outside the alphabet");
+        id = id * A + (int64_t)(hit - alphabet);
+    }
+    return id;
+}
+
+void getExpectationsUsingAnchors(StateMachine *sM, Hmm *hmmExpectations, Sequence *SsX, Sequence *SsY,
+                                 stList *anchorPairs, PairwiseAlignmentParameters *p,
+                                 void (*diagonalCalcExpectationFcn)(StateMachine *, int64_t, DpMatrix *, DpMatrix *,
+                                                                    Sequence *, Sequence *, double,
+                                                                    PairwiseAlignmentParameters *, void *),
+                                 bool alignmentHasRaggedLeftEnd, bool alignmentHasRaggedRightEnd) {
+    static const char *where = "getExpectationsUsingAnchors";
+    if (diagonalCalcExpectationFcn != diagonalCalculation_Expectations)
+        die(where, "only diagonalCalculation_Expectations is supported");
+    if (!hmmExpectations || hmmExpectations->type != sM->type) die(where, "the HMM does not fit the machine");
+    ExpectHmm *h = (ExpectHmm *)hmmExpectations;
+    const bool hdp = sM->type == threeStateHdp;
+    sa_read r;
+    sa_params prm;
+    int64_t *anc = NULL;
+    make_read(where, sM, SsX, SsY, anchorPairs, p, alignmentHasRaggedLeftEnd, alignmentHasRaggedRightEnd, &r, &prm,
+              &anc);
+    pthread_mutex_lock(&g_mu);
+    sa_context *ctx = context();
+    sa_model *dm = device_model(ctx, (CompatMachine *)sM);
+    sa_expect_result *res = NULL;
+    /* the order-dependent accumulators replay every cell that adds to them; HDP
+       assignments need the cells with a transition at or above the threshold */
+    int rc = sa_expect(ctx, dm, &r, &prm, hdp ? h->threshold : 0.0, &res);
+    if (rc != SA_OK) {
+        fprintf(stderr, "signalalign_amd: %s: status %d: %s\n", where, rc, sa_last_error(ctx));
+        abort();
+    }
+    pthread_mutex_unlock(&g_mu);
+    free(anc);
+    sa_expectations e;
+    sa_expect_get(res, &e);
+    h->base.likelihood += e.likelihood;
+    for (int i = 0; i < 9; i++) h->base.transitions[i] += e.transitions[i];
+    const int64_t k = sM->kmerLength, A = sM->alphabetSize;
+    const char *ref = (const char *)SsX->elements;
+    const double *ev = (const double *)SsY->elements;
+    for (int64_t q = 0; q < e.n_records; q++) {
+        const sa_match_record *rec = &e.records[q];
+        const char *kmer = ref + rec->x;
+        const double mean = ev[(int64_t)rec->y * NB_EVENT_PARAMS];
+        if (hdp) {
+            /* cell_signal_updateExpectationsAndAssignments (impl/pairwiseAligner.c:792-811) */
+            for (int j = 0; j < 3; j++) {
+                if (!(rec->p[j] >= h->threshold)) continue;
+                if (h->nAssign == h->capAssign) {
+                    h->capAssign = h->capAssign ? 2 * h->capAssign : 1024;
+                    h->assignKmers = (char *)realloc(h->assignKmers, (size_t)(h->capAssign * k));
+                    h->assignEvents = (double *)realloc(h->assignEvents, sizeof(double) * (size_t)h->capAssign);
+                    if (!h->assignKmers || !h->assignEvents) die(where, "out of host memory");
+                }
+                memcpy(h->assignKmers + h->nAssign * k, kmer, (size_t)k);
+                h->assignEvents[h->nAssign++] = mean;
+            }
+        } else {
+            /* cell_signal_updateExpectations (:763-790) with
+               continuousPairHmm_addToEmissionExpectation (impl/continuousHmm.c:160-169) */
+            const int64_t kk = kmer_index(kmer, sM->alphabet, A, k);
+            const double mu = h->eventModel[kk * MODEL_PARAMS];
+            const double xd = (mean + h->var * mu - h->scale * mu - h->shift) / h->var;
+            for (int j = 0; j < 3; j++) {
+                const double pj = rec->p[j];
+                h->eventExpectations[2 * kk] += pj * xd;
+                h->posteriors[kk] += pj;
+                const double u = h->eventExpectations[2 * kk] / h->posteriors[kk];
+                h->eventExpectations[2 * kk + 1] += pj * (xd - u) * (xd - u);
+                h->observed[kk] = true;
+            }
+        }
+    }
+    if (!hdp)
+        for (int64_t kk = 0; kk < e.n_kmers; kk++)
+            if (e.observed[kk]) h->observed[kk] = true;
+    sa_expect_free(res);
+}
+
+int64_t hmmContinuous_howManyAssignments(Hmm *hmm) {
+    if (!hmm || hmm->type != threeStateHdp) die("hmmContinuous_howManyAssignments", "not an HDP HMM");
+    return ((ExpectHmm *)hmm)->nAssign;
+}
+
+/* continuousPairHmm_writeToFile / hdpHmm_writeToFile (impl/continuousHmm.c:354-408,
+ * :572-629): header, transitions and likelihood, event model, then the
+ * expectations (threeState) or the assignments (threeStateHdp). */
+void hmmContinuous_writeToFile(const char *outFile, Hmm *hmm, StateMachineType type) {
+    static const char *where = "hmmContinuous_writeToFile";
+    if (!hmm || (type != threeState && type != threeStateHdp) || hmm->type != type) die(where, "unsupported HMM type");
+    ExpectHmm *h = (ExpectHmm *)hmm;
+    const int64_t nt = hmm->stateNumber * hmm->stateNumber, nk = hmm->parameterSetSize;
+    for (int64_t i = 0; i < nt; i++)
+        if (isnan(hmm->transitions[i])) {
+            fprintf(stderr, "GOT NaN TRANS\n");
+            return;
+        }
+    FILE *f = fopen(outFile, "w");
+    if (!f) die(where, "cannot open the output file");
+    fprintf(f, "%" PRId64 "\t%" PRId64 "\t%s\t%" PRId64 "\t\n", hmm->stateNumber, hmm->alphabetSize, hmm->alphabet,
+            hmm->kmerLength);
+    for (int64_t i = 0; i < nt; i++) fprintf(f, "%f\t", hmm->transitions[i]);
+    fprintf(f, "%f\n", hmm->likelihood);
+    for (int64_t i = 0; i < nk * MODEL_PARAMS; i++) fprintf(f, "%lf\t", h->eventModel[i]);
+    fprintf(f, "\n");
+    if (type == threeState) {
+        for (int64_t i = 0; i < 2 * nk; i++) fprintf(f, "%lf\t", h->eventExpectations[i]);
+        fprintf(f, "\n");
+        for (int64_t i = 0; i < nk; i++) fprintf(f, "%lf\t", h->posteriors[i]);
+        fprintf(f, "\n");
+        for (int64_t i = 0; i < nk; i++) fprintf(f, "%d\t", h->observed[i] ? 1 : 0);
+        fprintf(f, "\n");
+    } else {
+        for (int64_t i = 0; i < h->nAssign; i++) fprintf(f, "%lf\t", h->assignEvents[i]);
+        fprintf(f, "\n");
+        for (int64_t i = 0; i < h->nAssign; i++) {
+            fwrite(h->assignKmers + i * hmm->kmerLength, 1, (size_t)hmm->kmerLength, f);
+            fprintf(f, "\t");
+        }
+        fprintf(f, "\n");
+    }
+    fclose(f);
+}
+
+void hmmContinuous_destruct(Hmm *hmm, StateMachineType type) {
+    (void)type;
+    if (!hmm) return;
+    ExpectHmm *h = (ExpectHmm *)hmm;
+    free(h->base.alphabet);
+    free(h->base.transitions);
+    free(h->eventModel);
+    free(h->eventExpectations);
+    free(h->posteriors);
+    free(h->observed);
+    free(h->assignKmers);
+    free(h->assignEvents);
+    free(h);
 }

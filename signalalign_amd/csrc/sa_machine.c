@@ -969,6 +969,19 @@ stList *performSignalAlignment(StateMachine *sM, Sequence *eventSequence, int64_
     return pairs;
 }
 
+void getSignalExpectations(StateMachine *sM, Hmm *hmmExpectations, Sequence *eventSequence, int64_t *eventMap,
+                           int64_t mapOffset, char *trainingTarget, PairwiseAlignmentParameters *p,
+                           stList *unmappedAnchors, DegenerateType degenerate) {
+    const int64_t lX = sequence_correctSeqLength((int64_t)strlen(trainingTarget), event, sM->kmerLength);
+    stList *anchors = signalUtils_getRemappedAnchorPairs(unmappedAnchors, eventMap, mapOffset);
+    Sequence *target = sequence_constructReferenceKmerSequence(lX, trainingTarget, sequence_getKmer,
+                                                               sequence_sliceNucleotideSequence, degenerate, kmer);
+    getExpectationsUsingAnchors(sM, hmmExpectations, target, eventSequence, anchors, p,
+                                diagonalCalculation_Expectations, 1, 1);
+    sequence_destruct(target);
+    stList_destruct(anchors);
+}
+
 double scoreByPosteriorProbabilityIgnoringGaps(stList *pairs) {
     double s = 0.0;
     for (int64_t i = 0; i < stList_length(pairs); i++) s += (double)stIntTuple_get((stIntTuple *)stList_get(pairs, i), 0);

@@ -7,9 +7,10 @@
  *
  * Served: three-state Gaussian machines (--templateModel / --complementModel) and
  * HDP machines (--sm3Hdp with --templateHdp / --complementHdp), template-only and
- * 2D reads, output formats full / variantCaller / assignments, read scaling
- * re-estimated as the reference does.  Refused with a message: expectations
- * (--templateExpectations ...) and error-correction (--error_correct_path) runs.
+ * 2D reads, output formats full / variantCaller / assignments, the expectations
+ * routine (--templateExpectations / --complementExpectations: the EM expectations
+ * files instead of the alignment), read scaling re-estimated as the reference
+ * does.  Refused with a message: error-correction runs (--error_correct_path).
  */
 #define _GNU_SOURCE
 #include "signalalign_machine.h"
@@ -152,7 +153,6 @@ int main(int argc, char *argv[]) {
         fprintf(stderr, "signalMachine: --sm3Hdp needs --templateHdp (and --complementHdp with --twoD)\n");
         return 1;
     }
-    if (templateExpectationsFile || complementExpectationsFile) refuse("the expectations routine");
     if (errorCorrectPath) refuse("error correction (--error_correct_path)");
     if (!forwardReference || !backwardReference) {
         fprintf(stderr, "[signalAlign] - ERROR: did not get reference files %s %s\n",
@@ -201,6 +201,59 @@ int main(int argc, char *argv[]) {
     const int64_t start2 = pA->start2;
 
     stList *anchorPairs = signalUtils_guideAlignmentToRebasedAnchorPairs(pA, p); /* rebases pA */
+
+    if (templateExpectationsFile || complementExpectationsFile) {
+        /* expectations routine (signalMachine.c:618-700) */
+        const StateMachineType type = hdp ? threeStateHdp : threeState;
+        StateMachine *sMt = build_machine(templateModelFile, npRead->templateParams, nHdpT);
+        signalUtils_estimateNanoporeParams(sMt, npRead, &npRead->templateParams, ASSIGNMENT_THRESHOLD,
+                                           signalUtils_templateOneDAssignmentsFromRead,
+                                           nanopore_adjustTemplateEventsForDrift);
+        Hmm *tExp = hmmContinuous_getExpectationsHmm(sMt, p->threshold, 0.001, 0.001);
+        fprintf(stderr, "signalAlign - getting expectations for template\n");
+        getSignalExpectations(sMt, tExp, tEvents, tMap, start2, R->getTemplateTargetSequence(R), p, anchorPairs,
+                              (DegenerateType)degenerate);
+        if (hdp)
+            fprintf(stderr, "signalAlign - got %" PRId64 "template HDP assignments\n",
+                    hmmContinuous_howManyAssignments(tExp));
+        if (templateExpectationsFile) {
+            fprintf(stderr, "signalAlign - writing expectations to file: %s\n", templateExpectationsFile);
+            hmmContinuous_writeToFile(templateExpectationsFile, tExp, type);
+        }
+        StateMachine *sMc = NULL;
+        Hmm *cExp = NULL;
+        if (twoD) {
+            fprintf(stderr, "signalAlign - getting expectations for complement\n");
+            sMc = build_machine(complementModelFile, npRead->complementParams, nHdpC);
+            signalUtils_estimateNanoporeParams(sMc, npRead, &npRead->complementParams, ASSIGNMENT_THRESHOLD,
+                                               signalUtils_complementOneDAssignmentsFromRead,
+                                               nanopore_adjustComplementEventsForDrift);
+            cExp = hmmContinuous_getExpectationsHmm(sMc, p->threshold, 0.001, 0.001);
+            getSignalExpectations(sMc, cExp, cEvents, npRead->complementEventMap, start2,
+                                  R->getComplementTargetSequence(R), p, anchorPairs, (DegenerateType)degenerate);
+            if (hdp)
+                fprintf(stderr, "signalAlign - got %" PRId64 "complement HDP assignments\n",
+                        hmmContinuous_howManyAssignments(cExp));
+            if (complementExpectationsFile) {
+                fprintf(stderr, "signalAlign - writing expectations to file: %s\n", complementExpectationsFile);
+                hmmContinuous_writeToFile(complementExpectationsFile, cExp, type);
+            }
+        }
+        hmmContinuous_destruct(tExp, type);
+        hmmContinuous_destruct(cExp, type);
+        stateMachine_destruct(sMt);
+        stateMachine_destruct(sMc);
+        stList_destruct(anchorPairs);
+        sequence_destruct(tEvents);
+        sequence_destruct(cEvents);
+        signalUtils_ReferenceSequenceDestruct(R);
+        nanopore_nanoporeReadDestruct(npRead);
+        pairwiseAlignmentBandingParameters_destruct(p);
+        destructPairwiseAlignment(pA);
+        destroy_nanopore_hdp(nHdpT);
+        destroy_nanopore_hdp(nHdpC);
+        return 0;
+    }
 
     fprintf(stderr, "signalAlign - starting template alignment\n");
     StateMachine *sMt = build_machine(templateModelFile, npRead->templateParams, nHdpT);

@@ -20,18 +20,21 @@ HEADER = os.path.join(ROOT, "include", "signalalign_compat.h")
 MODEL5 = os.path.join(ROOT, "tests", "golden", "testModelR9_5mer_acgt_template.model")
 
 
-def _declared_functions():
-    text = open(HEADER).read()
+def _declared_functions(header=HEADER):
+    text = open(header).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     names = re.findall(r"^[A-Za-z_][\w \*]*?\b([A-Za-z_]\w*)\s*\(", text, flags=re.M)
     return sorted({n for n in names if n not in ("double", "void", "int64_t")})
 
 
 def test_compat_exports_every_declared_function():
+    """Every function signalalign_compat.h and signalalign_machine.h declare."""
     lib = ctypes.CDLL(COMPAT_LIB)
     names = _declared_functions()
     assert len(names) >= 30, names
-    missing = [n for n in names if not hasattr(lib, n)]
+    machine = _declared_functions(os.path.join(ROOT, "include", "signalalign_machine.h"))
+    assert len(machine) >= 40, machine
+    missing = [n for n in names + machine if not hasattr(lib, n)]
     assert not missing, missing
 
 

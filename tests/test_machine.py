@@ -181,7 +181,6 @@ def test_npread_c_reader_rejects_bad_files(tmp_path):
 @pytest.mark.parametrize("extra,msg", [
     (["--sm3Hdp"], "needs --templateHdp"),
     (["--sm3Hdp", "--twoD", "-C", TMODEL, "-v", "t.nhdp"], "Need to have template and complement HDPs"),
-    (["-t", "x.expectations"], "expectations"),
     (["-p", "somewhere"], "error correction"),
 ])
 def test_signal_machine_refuses_unserved_runs(extra, msg):
@@ -291,8 +290,9 @@ def _complement(s):
     return s.translate(str.maketrans("ACGT", "TGCA"))
 
 
-def _expected(strand, pa, prm, rd, ref, k=6, event_map="event_map", hdp=None):
-    """Oracle pairs in reference coordinates for one strand of the signalMachine run."""
+def _oracle_inputs(strand, pa, prm, rd, ref, event_map="event_map", hdp=None):
+    """The model, read (target, drift-corrected event slice, remapped anchors, read
+    parameters), first event and target of one strand of a signalMachine run."""
     name = "template" if strand == "t" else "complement"
     ev = rd[name]["events"].copy()
     ev[:, 0] = ev[:, 0] - ev[:, 3] * prm["drift"]
@@ -313,6 +313,12 @@ def _expected(strand, pa, prm, rd, ref, k=6, event_map="event_map", hdp=None):
     read = {"ref": target.encode(), "events": ev[e0:e1], "anchors": np.array(D.remap(anchors, emap, pa["start2"]),
                                                                              dtype=np.int64).reshape(-1, 2),
             "scale": prm["scale"], "shift": prm["shift"], "var": prm["var"]}
+    return m, read, e0, target
+
+
+def _expected(strand, pa, prm, rd, ref, k=6, event_map="event_map", hdp=None):
+    """Oracle pairs in reference coordinates for one strand of the signalMachine run."""
+    m, read, e0, target = _oracle_inputs(strand, pa, prm, rd, ref, event_map, hdp)
     pairs, _ = P.run_oracle(m, read, native.Params(threshold=0.01, diagonal_expansion=50),
                             kind=None if hdp is not None else oracle.GAUSS_DESCALED)
     L = len(target)
@@ -395,3 +401,88 @@ def test_signal_machine_hdp_end_to_end(tmp_path, oracle_lib):
     assert cmp["n_gpu"] > 500 and all(t[4] == "t" for t in rows)
     for t in rows:
         assert t[2] == ref[int(t[1]):int(t[1]) + 6]
+
+
+def _read_expectations(path):
+    return [l.split("\t") for l in open(path).read().split("\n")]
+
+
+@pytest.mark.gpu
+def test_signal_machine_expectations(tmp_path, oracle_lib):
+    """signalMachine -t/-c (the expectations routine) on the Zymo read (2D): each
+    strand's expectations file in the reference's layout, its values equal to the
+    oracle's getExpectationsUsingAnchors on the same inputs (1e-9 relative; the file
+    prints six decimals), the event model the noise-scaled table."""
+    line = _guide_cigar()
+    ref = open(REF).read().split()[0]
+    bwd = tmp_path / "bwd.txt"
+    bwd.write_text(_complement(ref) + "\n")
+    rd = formats.load_npread(NPREAD)
+    prm = _params()
+    pa = D.parse_cigar(line)
+    tf, cf = tmp_path / "t.expectations", tmp_path / "c.expectations"
+    print("signalMachine expectations", flush=True)
+    _run([MACHINE, "--twoD", "-T", TMODEL, "-C", CMODEL, "-f", REF, "-b", str(bwd), "-q", NPREAD, "-L", "zymo",
+          "-t", str(tf), "-c", str(cf)], stdin=line + "\n")
+    for strand, f in (("t", tf), ("c", cf)):
+        rows = _read_expectations(f)
+        assert rows[0][:4] == ["3", "4", "ACGT", "6"] or rows[0][:4] == ["3", "6", "ACEGOT", "6"], rows[0][:4]
+        m, read, e0, target = _oracle_inputs(strand, pa, prm[strand], rd, ref)
+        om = P.oracle_model(m, read, oracle.GAUSS_DESCALED)
+        want = oracle.get_expectations(om, read["ref"], read["events"], read["anchors"],
+                                       oracle.Params(threshold=0.01, diagonal_expansion=50))
+        vals = [float(v) for v in rows[1] if v]
+        assert len(vals) == 10
+        assert np.allclose(vals[:9], want["transitions"].ravel(), rtol=1e-9, atol=1e-6)
+        assert abs(vals[9] - want["likelihood"]) <= 1e-9 * abs(want["likelihood"]) + 1e-6
+        model = np.array([float(v) for v in rows[2] if v])
+        assert np.allclose(model, m.match.ravel(), rtol=0, atol=1e-6)
+        eexp = np.array([float(v) for v in rows[3] if v]).reshape(-1, 2)
+        assert np.allclose(eexp, want["event_exp"], rtol=1e-9, atol=1e-6)
+        post = np.array([float(v) for v in rows[4] if v])
+        assert np.allclose(post, want["posteriors"], rtol=1e-9, atol=1e-6)
+        obs = np.array([int(v) for v in rows[5] if v]).astype(bool)
+        assert np.array_equal(obs, want["observed"])
+        print(strand, "likelihood", vals[9], "observed", int(obs.sum()))
+
+
+@pytest.mark.gpu
+def test_signal_machine_hdp_expectations(tmp_path, oracle_lib):
+    """signalMachine --sm3Hdp -t on the template strand (1D): the HDP expectations
+    file's transitions and its assignments (event mean, k-mer of every match
+    transition with posterior >= the threshold, in the reference's call order) equal
+    the oracle's."""
+    line = _guide_cigar("strand_event_map")
+    ref = open(REF).read().split()[0]
+    bwd = tmp_path / "bwd.txt"
+    bwd.write_text(_complement(ref) + "\n")
+    rd = formats.load_npread(NPREAD)
+    prm = _params()
+    pa = D.parse_cigar(line)
+    h, _ = _nhdp_python()
+    tf = tmp_path / "t.expectations"
+    print("signalMachine --sm3Hdp expectations", flush=True)
+    _run([MACHINE, "--sm3Hdp", "-v", NHDP, "-T", TMODEL, "-f", REF, "-b", str(bwd), "-q", NPREAD, "-L", "zymo",
+          "-t", str(tf)], stdin=line + "\n")
+    rows = _read_expectations(tf)
+    # the expectations routine does not move the level means to the HDP's (the
+    # alignment routine does): the oracle model keeps the .model file's
+    m, read, e0, target = _oracle_inputs("t", pa, prm["t"], rd, ref, "strand_event_map")
+    hm = models.HdpModel(h["alphabet"], h["kmer_length"], m.transitions, h["grid"], h["density"], h["slopes"],
+                         h["kmer_row"], m.match[:, 0].copy())
+    om = P.oracle_model(hm, read)
+    want = oracle.get_expectations(om, read["ref"], read["events"], read["anchors"],
+                                   oracle.Params(threshold=0.01, diagonal_expansion=50), rec_threshold=0.01)
+    vals = [float(v) for v in rows[1] if v]
+    assert np.allclose(vals[:9], want["transitions"].ravel(), rtol=1e-9, atol=1e-6)
+    ev = [float(v) for v in rows[3] if v]
+    km = [v for v in rows[4] if v]
+    wev, wkm = [], []
+    for x, y, p1, p2, p3 in want["records"]:
+        for pj in (p1, p2, p3):
+            if pj >= 0.01:
+                wev.append(read["events"][int(y)][0])
+                wkm.append(target[int(x):int(x) + 6])
+    assert km == wkm and len(km) > 100
+    assert np.allclose(ev, wev, rtol=0, atol=1e-6)
+    print("assignments", len(km))
