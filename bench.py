@@ -142,6 +142,45 @@ def hdp_leg(native, synth, ctx, args):
     return out
 
 
+def expectations_leg(native, ctx, dm, model, reads, params, args):
+    """EM expectations (sa_expect_batch, FP64): the first --expect-reads reads of the
+    batched workload in one run, transition and k-mer expectations without records;
+    GCUPS = band cells / device time.  Beside it the FP64 oracle's expectations on a
+    bounded sample (one core)."""
+    sub = make_native_reads(native, reads[: args.expect_reads])
+    tm = {}
+    ctx.expect_batch(dm, sub, params, record_threshold=1.0, timing=tm)  # warm-up
+    runs = []
+    for _ in range(2):
+        tm = {}
+        ctx.expect_batch(dm, sub, params, record_threshold=1.0, timing=tm)
+        runs.append(tm)
+    ms = float(np.mean([t["total_ms"] for t in runs]))
+    cells = runs[-1]["cells"]
+    out = {"workload": f"EM expectations: the first {len(sub)} reads of the batched workload, FP64, band expansion "
+                       f"{args.expansion}, sums only", "reads": len(sub), "cells": cells,
+           "gcups": round(cells / ms / 1e6, 3), "ms_per_step": round(ms, 3)}
+    if not args.no_cpu:
+        import oracle
+        from tests import _parity as P
+        oracle.build()
+        t0, ccells, n = time.time(), 0, 0
+        for r in reads:
+            if time.time() - t0 > 5.0:
+                break
+            om = P.oracle_model(model, r)
+            oracle.get_expectations(om, r["ref"], r["events"], r["anchors"],
+                                    oracle.Params(diagonal_expansion=args.expansion))
+            lo, hi = oracle.band(r["anchors"], len(r["ref"]) - (model.kmer_length - 1), len(r["events"]),
+                                 args.expansion)
+            ccells += int(np.sum((hi - lo) // 2 + 1))
+            n += 1
+        out["cpu_oracle_gcups_1core"] = round(ccells / (time.time() - t0) / 1e9, 4)
+        out["cpu_oracle_sample_reads"] = n
+    log(f"expectations: {out}")
+    return out
+
+
 def serving_leg(native, synth, ctx, gmodel, args):
     """The serving configuration: Poisson arrivals of single-read requests at a fixed
     rate (a fraction --serve-load of the capacity a closed burst measures first),
@@ -300,6 +339,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="timed steps without graph replay")
     ap.add_argument("--hdp-reads", type=int, default=2048)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving configuration")
+    ap.add_argument("--no-expect", action="store_true", help="skip the EM expectations configuration")
+    ap.add_argument("--expect-reads", type=int, default=512)
     ap.add_argument("--serve-requests", type=int, default=100000)
     ap.add_argument("--serve-pool", type=int, default=1024)
     ap.add_argument("--serve-burst", type=int, default=8192)
@@ -441,6 +482,7 @@ def main():
         plain["per_read_loop_reads"] = len(sub)
         log(f"plain: {plain}")
 
+    expect = None if args.no_expect else expectations_leg(native, ctx, dm, model, reads, params, args)
     hdp = None if args.no_hdp else hdp_leg(native, synth, ctx, args)
     serving = None if args.no_serve else serving_leg(native, synth, ctx, model, args)
 
@@ -466,6 +508,7 @@ def main():
         "plain_gpu": plain,
         "hdp": hdp,
         "serving": serving,
+        "expectations": expect,
         "wall_ms_per_step_pcie_inclusive": round(float(np.mean(wall_ms)), 3),
     }
     if cpu:

@@ -257,7 +257,14 @@ typedef struct {
 } sa_expectations;
 int sa_expect(sa_context *ctx, const sa_model *model, const sa_read *read, const sa_params *params,
               double record_threshold, sa_expect_result **out);
-int sa_expect_get(const sa_expect_result *res, sa_expectations *out);
+/* Many reads in one launch sequence (the EM training pass over a read set); each
+ * read's expectations separately.  Record storage is one record per posterior cell
+ * at most: large batches pass record_threshold 1 (no records) and use the sums. */
+int sa_expect_batch(sa_context *ctx, const sa_model *model, const sa_read *reads, int64_t n_reads,
+                    const sa_params *params, double record_threshold, sa_expect_result **out);
+int sa_expect_read(const sa_expect_result *res, int64_t i, sa_expectations *out);
+int sa_expect_get(const sa_expect_result *res, sa_expectations *out); /* read 0 */
+int sa_expect_timing(const sa_expect_result *res, sa_timing *out);
 void sa_expect_free(sa_expect_result *res);
 
 /* Host-side views of the integer geometry the kernels compute on the device

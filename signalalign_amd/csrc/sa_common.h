@@ -48,7 +48,7 @@ struct WinDev {
     int32_t last;  /* atEnd */
     int32_t out_cap;
     int32_t span;  /* positions a register window must cover over (T, D] (set by prep) */
-    int32_t pad;
+    int32_t rec_cap; /* expectation records this window's slot holds */
     int64_t out_off;
     int64_t cells; /* cells of the diagonals (T, TF] (set by prep) */
     int64_t rec_off; /* first expectation record of this window (EM expectations runs) */
@@ -143,7 +143,8 @@ SA_HD int32_t window_schedule(int64_t N, int64_t min_diags, int64_t traceback, i
         w.last = last;
         w.out_cap = 0;
         w.span = 1;
-        w.pad = 0;
+        w.rec_cap = 0;
+        w.rec_off = 0;
         w.out_off = 0;
         w.cells = 0;
         out[n++] = w;

@@ -93,7 +93,7 @@ struct Args {
     double *ex_trans;       /* 10 per window */
     double *ex_s0, *ex_s1;  /* n_kmers per task */
     int8_t *ex_obs;         /* n_kmers per task */
-    ExRec *ex_rec;          /* W.rec_off.. (capacity W.cells) */
+    ExRec *ex_rec;          /* W.rec_off.. (capacity W.rec_cap) */
     int32_t *ex_count;      /* records per window */
 };
 

@@ -436,7 +436,7 @@ __device__ __forceinline__ void expect_diagonal(const Args<double> &a, const Tas
         const unsigned long long bal = __ballot(keep);
         if (keep) {
             const int idx = ex_cursor + __popcll(bal & ((1ull << lane) - 1ull));
-            if (idx < W.cells) {
+            if (idx < W.rec_cap) {
                 ExRec r;
                 r.x = x - 1;
                 r.y = y - 1;

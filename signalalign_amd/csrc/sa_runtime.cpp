@@ -226,7 +226,8 @@ struct sa_result {
 };
 
 struct sa_expect_result {
-    ReadExpect e;
+    std::vector<ReadExpect> reads;
+    sa_timing timing;
 };
 
 namespace {
@@ -932,8 +933,13 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
             W.out_off = out_total;
             W.out_cap = 2 * (W.TF - W.T) + 64;
             out_total += W.out_cap;
+            /* every posterior cell may hold a record at threshold 0; above it a few
+               per diagonal (a window that overflows runs again with its count) */
             W.rec_off = rec_total;
-            if (expect) rec_total += W.cells;
+            W.rec_cap = !expect ? 0
+                      : b->ex_rec_threshold > 0.0 ? (int32_t)std::min<int64_t>(W.cells, 2 * (W.TF - W.T) + 64)
+                                                  : (int32_t)W.cells;
+            rec_total += W.rec_cap;
             wins.push_back(W);
         }
     }
@@ -1244,11 +1250,30 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
         tm.forward_ms = fwd_end;
         tm.backward_ms = fb - bwd_start;
     }
-    /* windows whose pairs overflowed their slot run again with the exact count */
+    /* windows whose pairs (or expectation records) overflowed their slot run again
+       with the exact count */
     std::vector<int32_t> redo;
     for (int64_t w = 0; w < nw; w++)
         if (wcount[w] > wins[w].out_cap) redo.push_back((int32_t)w);
-    if (!redo.empty()) {
+    bool ex_redo = false;
+    if (expect) {
+        std::vector<int32_t> ecnt(nw);
+        SA_HIP(ctx, hipMemcpy(ecnt.data(), b->d_ex_count.p, sizeof(int32_t) * nw, hipMemcpyDeviceToHost));
+        rec_total = 0;
+        for (int64_t w = 0; w < nw; w++) {
+            if (ecnt[w] > wins[w].rec_cap) {
+                wins[w].rec_cap = ecnt[w];
+                ex_redo = true;
+            }
+            wins[w].rec_off = rec_total;
+            rec_total += wins[w].rec_cap;
+        }
+        if (ex_redo) {
+            SA_HIP(ctx, b->d_ex_rec.ensure(sizeof(sa::ExRec) * std::max<int64_t>(rec_total, 1)));
+            a.ex_rec = b->d_ex_rec.as<sa::ExRec>();
+        }
+    }
+    if (!redo.empty() || ex_redo) {
         int64_t extra = 0;
         for (int32_t w : redo) {
             wins[w].out_off = out_total + extra;
@@ -1337,7 +1362,7 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
                     const int64_t wi = task_first_win[ti] + w;
                     for (int k = 0; k < 9; k++) re.trans[k] += etr[10 * wi + k];
                     re.likelihood += etr[10 * wi + 9];
-                    const int64_t n = std::min<int64_t>(ecnt[wi], wins[wi].cells);
+                    const int64_t n = std::min<int64_t>(ecnt[wi], wins[wi].rec_cap);
                     for (int64_t q = 0; q < n; q++) {
                         const sa::ExRec &x = erec[wins[wi].rec_off + q];
                         sa_match_record m;
@@ -1522,13 +1547,13 @@ int sa_align(sa_context *ctx, const sa_model *model, const sa_read *read, const 
     return sa_align_batch(ctx, model, read, 1, params, precision, out);
 }
 
-int sa_expect(sa_context *ctx, const sa_model *model, const sa_read *read, const sa_params *params,
-              double record_threshold, sa_expect_result **out) {
+int sa_expect_batch(sa_context *ctx, const sa_model *model, const sa_read *reads, int64_t n_reads,
+                    const sa_params *params, double record_threshold, sa_expect_result **out) {
     if (!out) return SA_ERR_INVALID_ARGUMENT;
     *out = nullptr;
-    if (!(record_threshold >= 0.0)) return SA_ERR_INVALID_ARGUMENT;
+    if (!(record_threshold >= 0.0) || n_reads < 0) return SA_ERR_INVALID_ARGUMENT;
     sa_batch *b = nullptr;
-    int rc = sa_batch_create(ctx, model, read, 1, params, &b);
+    int rc = sa_batch_create(ctx, model, reads, n_reads, params, &b);
     if (rc) return rc;
     b->expect = 1;
     b->ex_rec_threshold = record_threshold;
@@ -1537,27 +1562,44 @@ int sa_expect(sa_context *ctx, const sa_model *model, const sa_read *read, const
     sa_batch_destroy(b);
     if (rc) return rc;
     sa_expect_result *er = new sa_expect_result();
-    if (!res->expect.empty()) er->e = std::move(res->expect[0]);
-    else {
-        er->e.s0.assign(model->n_kmers, 0.0);
-        er->e.s1.assign(model->n_kmers, 0.0);
-        er->e.obs.assign(model->n_kmers, 0);
-    }
+    er->reads = std::move(res->expect);
+    er->timing = res->timing;
+    er->reads.resize(n_reads);
+    for (ReadExpect &e : er->reads)
+        if (e.s0.empty()) { /* a read with no cells */
+            e.s0.assign(model->n_kmers, 0.0);
+            e.s1.assign(model->n_kmers, 0.0);
+            e.obs.assign(model->n_kmers, 0);
+        }
     delete res;
     *out = er;
     return SA_OK;
 }
 
-int sa_expect_get(const sa_expect_result *res, sa_expectations *out) {
+int sa_expect(sa_context *ctx, const sa_model *model, const sa_read *read, const sa_params *params,
+              double record_threshold, sa_expect_result **out) {
+    return sa_expect_batch(ctx, model, read, 1, params, record_threshold, out);
+}
+
+int sa_expect_read(const sa_expect_result *res, int64_t i, sa_expectations *out) {
+    if (!res || !out || i < 0 || i >= (int64_t)res->reads.size()) return SA_ERR_INVALID_ARGUMENT;
+    const ReadExpect &e = res->reads[i];
+    for (int k = 0; k < 9; k++) out->transitions[k] = e.trans[k];
+    out->likelihood = e.likelihood;
+    out->n_kmers = (int64_t)e.s0.size();
+    out->match_posterior = e.s0.data();
+    out->match_event_sum = e.s1.data();
+    out->observed = e.obs.data();
+    out->records = e.recs.data();
+    out->n_records = (int64_t)e.recs.size();
+    return SA_OK;
+}
+
+int sa_expect_get(const sa_expect_result *res, sa_expectations *out) { return sa_expect_read(res, 0, out); }
+
+int sa_expect_timing(const sa_expect_result *res, sa_timing *out) {
     if (!res || !out) return SA_ERR_INVALID_ARGUMENT;
-    for (int k = 0; k < 9; k++) out->transitions[k] = res->e.trans[k];
-    out->likelihood = res->e.likelihood;
-    out->n_kmers = (int64_t)res->e.s0.size();
-    out->match_posterior = res->e.s0.data();
-    out->match_event_sum = res->e.s1.data();
-    out->observed = res->e.obs.data();
-    out->records = res->e.recs.data();
-    out->n_records = (int64_t)res->e.recs.size();
+    *out = res->timing;
     return SA_OK;
 }
 

@@ -151,6 +151,10 @@ def load():
     L.sa_expect.argtypes = [vp, vp, ctypes.POINTER(_Read), ctypes.POINTER(_Params), ctypes.c_double,
                             ctypes.POINTER(vp)]
     L.sa_expect_get.argtypes = [vp, ctypes.POINTER(_Expectations)]
+    L.sa_expect_batch.argtypes = [vp, vp, ctypes.POINTER(_Read), ctypes.c_int64, ctypes.POINTER(_Params),
+                                  ctypes.c_double, ctypes.POINTER(vp)]
+    L.sa_expect_read.argtypes = [vp, ctypes.c_int64, ctypes.POINTER(_Expectations)]
+    L.sa_expect_timing.argtypes = [vp, ctypes.POINTER(_Timing)]
     L.sa_expect_free.argtypes = [vp]
     L.sa_last_error.argtypes = [vp]
     L.sa_last_error.restype = ctypes.c_char_p
@@ -305,30 +309,45 @@ class Context:
         transitions (3, 3) [from, to] without pseudocounts, likelihood,
         match_posterior / match_event_sum (n_kmers,), observed (bool), records (n, 5)
         float64: k-mer position, event, p(M->M), p(X->M), p(Y->M) in call order."""
+        return self.expect_batch(model, [read], params, record_threshold)[0]
+
+    def expect_batch(self, model, reads, params=None, record_threshold=1.0, timing=None):
+        """sa_expect_batch: one dict (as expect) per read; `timing`, a dict, receives
+        the run's device timing."""
         params = params or Params()
-        rs = read._struct()
+        arr = (_Read * max(1, len(reads)))()
+        for i, r in enumerate(reads):
+            arr[i] = r._struct()
         ps = params._struct()
         res = ctypes.c_void_p()
         L = self._L
-        self._check(L.sa_expect(self._h, model._h, ctypes.byref(rs), ctypes.byref(ps), float(record_threshold),
-                                ctypes.byref(res)), "sa_expect")
+        self._check(L.sa_expect_batch(self._h, model._h, arr, len(reads), ctypes.byref(ps), float(record_threshold),
+                                      ctypes.byref(res)), "sa_expect_batch")
         try:
+            if timing is not None:
+                t = _Timing()
+                L.sa_expect_timing(res, ctypes.byref(t))
+                timing.update({k: getattr(t, k) for k, _ in _Timing._fields_})
+            out = []
             e = _Expectations()
-            L.sa_expect_get(res, ctypes.byref(e))
-            nk = e.n_kmers
-            out = {"transitions": np.array(list(e.transitions)).reshape(3, 3), "likelihood": e.likelihood,
-                   "match_posterior": np.ctypeslib.as_array(e.match_posterior, shape=(nk,)).copy(),
-                   "match_event_sum": np.ctypeslib.as_array(e.match_event_sum, shape=(nk,)).copy(),
-                   "observed": np.ctypeslib.as_array(e.observed, shape=(nk,)).astype(bool)}
-            rec = np.zeros((e.n_records, 5))
-            if e.n_records:
-                raw = np.ctypeslib.as_array(ctypes.cast(e.records, ctypes.POINTER(ctypes.c_uint8)),
-                                            shape=(e.n_records * ctypes.sizeof(_MatchRecord),))
-                st = np.frombuffer(raw.tobytes(), dtype=np.dtype([("x", "<i4"), ("y", "<i4"), ("p", "<f8", 3)]))
-                rec[:, 0] = st["x"]
-                rec[:, 1] = st["y"]
-                rec[:, 2:] = st["p"]
-            out["records"] = rec
+            for i in range(len(reads)):
+                L.sa_expect_read(res, i, ctypes.byref(e))
+                nk = e.n_kmers
+                d = {"transitions": np.array(list(e.transitions)).reshape(3, 3), "likelihood": e.likelihood,
+                     "match_posterior": np.ctypeslib.as_array(e.match_posterior, shape=(nk,)).copy(),
+                     "match_event_sum": np.ctypeslib.as_array(e.match_event_sum, shape=(nk,)).copy(),
+                     "observed": np.ctypeslib.as_array(e.observed, shape=(nk,)).astype(bool)}
+                rec = np.zeros((e.n_records, 5))
+                if e.n_records:
+                    raw = np.ctypeslib.as_array(ctypes.cast(e.records, ctypes.POINTER(ctypes.c_uint8)),
+                                                shape=(e.n_records * ctypes.sizeof(_MatchRecord),))
+                    st = np.frombuffer(raw.tobytes(),
+                                       dtype=np.dtype([("x", "<i4"), ("y", "<i4"), ("p", "<f8", 3)]))
+                    rec[:, 0] = st["x"]
+                    rec[:, 1] = st["y"]
+                    rec[:, 2:] = st["p"]
+                d["records"] = rec
+                out.append(d)
             return out
         finally:
             L.sa_expect_free(res)

@@ -230,3 +230,29 @@ def test_gpu_expectations_split_and_hdp(gpu_ctx, oracle_lib):
     r2 = synth.single_read(h, 800, seed=6)
     print("expect hdp", flush=True)
     _gpu_vs_oracle(gpu_ctx, h, r2, native.Params(diagonal_expansion=20))
+
+
+@pytest.mark.gpu
+def test_gpu_expectations_batch(gpu_ctx, oracle_lib):
+    """Several reads in one sa_expect_batch equal the oracle read by read; records at
+    threshold 0.5 (window slots sized for a few per diagonal, refilled when they
+    overflow) equal the oracle's records at that threshold."""
+    from signalalign_amd import native, synth
+    from tests import _parity as P
+    m = models.synthetic_gaussian_model("ACGT", 6, seed=7)
+    reads = [synth.single_read(m, n, seed=40 + i) for i, n in enumerate([200, 900, 1700, 3100])]
+    params = native.Params(diagonal_expansion=30)
+    dm = gpu_ctx.model(m)
+    print("expect batch", flush=True)
+    for thr in (0.5, 0.0):
+        got = gpu_ctx.expect_batch(dm, [P.native_read(r) for r in reads], params, record_threshold=thr)
+        for r, g in zip(reads, got):
+            om = P.oracle_model(m, r)
+            w = oracle.get_expectations(om, r["ref"], r["events"], r["anchors"],
+                                        oracle.Params(diagonal_expansion=30), rec_threshold=thr)
+            assert np.allclose(g["transitions"] + 0.001, w["transitions"], rtol=1e-9, atol=1e-12)
+            assert abs(g["likelihood"] - w["likelihood"]) <= 1e-12 * abs(w["likelihood"])
+            assert np.allclose(g["match_posterior"] + 0.001, w["posteriors"], rtol=1e-9, atol=1e-12)
+            assert np.array_equal(g["records"][:, :2], w["records"][:, :2])
+            assert np.allclose(g["records"][:, 2:], w["records"][:, 2:], rtol=1e-9, atol=1e-15)
+        print(thr, [len(g["records"]) for g in got])
