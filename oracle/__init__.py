@@ -53,7 +53,7 @@ class _Params(ctypes.Structure):
 
 
 class _Pair(ctypes.Structure):
-    _fields_ = [("p", ctypes.c_int64), ("x", ctypes.c_int64), ("y", ctypes.c_int64)]
+    _fields_ = [("p", ctypes.c_int64), ("x", ctypes.c_int64), ("y", ctypes.c_int64), ("path", ctypes.c_int64)]
 
 
 class _Expect(ctypes.Structure):
@@ -122,6 +122,16 @@ def lib():
                                            ctypes.c_int64, _I64P, ctypes.c_int64, ctypes.POINTER(_Params),
                                            ctypes.c_int, ctypes.c_int, ctypes.POINTER(_Pair), ctypes.c_int64,
                                            _I64P, ctypes.POINTER(_Stats)]
+        L.or_get_aligned_pairs_degenerate.restype = ctypes.c_int
+        L.or_get_aligned_pairs_degenerate.argtypes = [ctypes.POINTER(_Model), ctypes.c_char_p, ctypes.c_int64,
+                                                      _F64P, ctypes.c_int64, _I64P, ctypes.c_int64,
+                                                      ctypes.POINTER(_Params), ctypes.c_int, ctypes.c_int,
+                                                      ctypes.c_char_p, ctypes.POINTER(_Pair), ctypes.c_int64, _I64P,
+                                                      ctypes.POINTER(_Stats)]
+        L.or_path_kmers.restype = ctypes.c_int64
+        L.or_path_kmers.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int64]
+        L.or_path_legal.restype = ctypes.c_int
+        L.or_path_legal.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
         L.or_full_forward_backward.restype = ctypes.c_int
         L.or_full_forward_backward.argtypes = [ctypes.POINTER(_Model), ctypes.c_char_p, ctypes.c_int64, _F64P,
                                                ctypes.c_int64, _I64P, ctypes.c_int64, ctypes.c_int64,
@@ -307,11 +317,30 @@ def split_points(anchors, lX, lY, split_matrix, ragged_left, ragged_right):
     return out[:n]
 
 
+def path_kmers(kmer, options):
+    """The path k-mers of one k-mer with degenerate 'X' bases, in path order."""
+    k = len(kmer)
+    n = lib().or_path_kmers(kmer.encode(), k, options.encode(), None, 0)
+    if n < 0:
+        raise ValueError("an X without options")
+    buf = ctypes.create_string_buffer(int(n) * k)
+    lib().or_path_kmers(kmer.encode(), k, options.encode(), buf, n)
+    raw = buf.raw.decode()
+    return [raw[i * k:(i + 1) * k] for i in range(int(n))]
+
+
+def path_legal(from_kmer, to_kmer, k=6):
+    return bool(lib().or_path_legal(from_kmer.encode() if from_kmer else None,
+                                    to_kmer.encode() if to_kmer else None, k))
+
+
 def get_aligned_pairs(model, ref, events, anchors, params=None, ragged_left=True, ragged_right=True,
-                      cap=None):
+                      cap=None, options=None):
     """Oracle getAlignedPairsUsingAnchors.  ref: str/bytes of nucleotides (k-mer i is
     ref[i:i+k]); events: (lY, >=2) float array of (mean, noise[, duration, start]).
-    Returns (pairs (n,3) int64 [p_int, x, y] in the reference's list order, stats dict)."""
+    Returns (pairs (n,3) int64 [p_int, x, y] in the reference's list order, stats dict).
+    With `options` (the degenerate base options of 'X', e.g. "CEO") the pairs are
+    (n,4) [p_int, x, y, path]."""
     params = params or Params()
     refb = ref.encode() if isinstance(ref, str) else bytes(ref)
     lX = len(refb) - (model.kmer_length - 1) if len(refb) > 0 else 0
@@ -327,14 +356,17 @@ def get_aligned_pairs(model, ref, events, anchors, params=None, ragged_left=True
     st = _Stats()
     ms = model._struct()
     ps = params._struct()
-    rc = lib().or_get_aligned_pairs(ctypes.byref(ms), refb, lX, _p64(ev), len(ev), _pi64(a), len(a),
-                                    ctypes.byref(ps), int(ragged_left), int(ragged_right), out, cap,
-                                    ctypes.byref(n), ctypes.byref(st))
+    rc = lib().or_get_aligned_pairs_degenerate(ctypes.byref(ms), refb, lX, _p64(ev), len(ev), _pi64(a), len(a),
+                                               ctypes.byref(ps), int(ragged_left), int(ragged_right),
+                                               options.encode() if options else None, out, cap,
+                                               ctypes.byref(n), ctypes.byref(st))
     if rc != 0:
         raise RuntimeError(f"oracle getAlignedPairs failed: {rc}")
     if n.value > cap:
-        return get_aligned_pairs(model, ref, events, anchors, params, ragged_left, ragged_right, n.value)
-    arr = np.frombuffer(out, dtype=np.int64, count=3 * n.value).reshape(-1, 3).copy()
+        return get_aligned_pairs(model, ref, events, anchors, params, ragged_left, ragged_right, n.value, options)
+    arr = np.frombuffer(out, dtype=np.int64, count=4 * n.value).reshape(-1, 4).copy()
+    if not options:
+        arr = arr[:, :3].copy()
     return arr, {"n_windows": st.n_windows, "cells": st.cells, "log_likelihood": st.log_likelihood}
 
 
@@ -398,5 +430,5 @@ def full_forward_backward(model, ref, events, anchors, expansion, threshold):
                                         ctypes.byref(tb), _p64(dt))
     if rc != 0:
         raise RuntimeError("oracle full forward-backward failed")
-    arr = np.frombuffer(out, dtype=np.int64, count=3 * n.value).reshape(-1, 3).copy()
+    arr = np.frombuffer(out, dtype=np.int64, count=4 * n.value).reshape(-1, 4)[:, :3].copy()
     return arr, tf.value, tb.value, dt

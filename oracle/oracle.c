@@ -24,8 +24,10 @@
  * order of stateMachine3_cellCalculate, push-form backward accumulation, xmy-ascending
  * diagonal walks, the lookup logAdd with float-literal coefficients, the every-10th
  * diagonal total refresh and the floor(p*1e7) output.  It replaces the per-cell heap
- * objects (HDCell/Path) by flat per-diagonal arrays; only single-path cells (no 'X'
- * degenerate bases) are supported.
+ * objects (HDCell/Path) by flat per-diagonal arrays of path entries: column x holds
+ * one entry per path of its k-mer (the 'X' positions substituted by the degenerate
+ * base options, hdCell_construct, impl/pairwiseAligner.c:599-656), one entry when the
+ * k-mer has no 'X'.
  */
 #include "oracle.h"
 
@@ -350,21 +352,26 @@ int64_t or_split_points(const int64_t *anchors, int64_t n_anchors, int64_t lX, i
 
 typedef struct {
     int64_t xay, xmyL, xmyR;
-    double *cells; /* 3 per cell, cell i at xmy = xmyL + 2i */
+    const int64_t *pp; /* first path entry of column x; column x has pp[x+1] - pp[x] paths */
+    double *cells;     /* 3 per path entry; column x from 3 * (pp[x] - pp[x of xmyL]) */
 } or_diag;
 
 typedef struct {
     or_diag **diags;
     int64_t n;       /* diagonalNumber: valid xay in [0, n] */
     int64_t active;
+    const int64_t *pp;
 } or_matrix;
 
-static int64_t diag_width(const or_diag *d) { return (d->xmyR - d->xmyL) / 2 + 1; }
+static int64_t diag_entries(const or_diag *d) {
+    return d->pp[x_coord(d->xay, d->xmyR) + 1] - d->pp[x_coord(d->xay, d->xmyL)];
+}
 
-static or_matrix *matrix_new(int64_t n) {
+static or_matrix *matrix_new(int64_t n, const int64_t *pp) {
     or_matrix *m = (or_matrix *)calloc(1, sizeof(or_matrix));
     m->diags = (or_diag **)calloc((size_t)(n + 1), sizeof(or_diag *));
     m->n = n;
+    m->pp = pp;
     return m;
 }
 
@@ -384,7 +391,8 @@ static or_diag *matrix_get(const or_matrix *m, int64_t xay) {
 static or_diag *matrix_create(or_matrix *m, int64_t xay, int64_t xmyL, int64_t xmyR) {
     or_diag *d = (or_diag *)malloc(sizeof(or_diag));
     d->xay = xay; d->xmyL = xmyL; d->xmyR = xmyR;
-    int64_t w = diag_width(d);
+    d->pp = m->pp;
+    int64_t w = diag_entries(d);
     d->cells = (double *)malloc(sizeof(double) * 3 * (size_t)w);
     for (int64_t i = 0; i < 3 * w; i++) d->cells[i] = OR_LOG_ZERO;
     m->diags[xay] = d;
@@ -402,10 +410,10 @@ static void matrix_delete(or_matrix *m, int64_t xay) {
     }
 }
 
-/* dpDiagonal_getCell: NULL outside [xmyL, xmyR] */
+/* dpDiagonal_getCell: NULL outside [xmyL, xmyR]; the cell's first path entry */
 static double *diag_cell(const or_diag *d, int64_t xmy) {
     if (d == NULL || xmy < d->xmyL || xmy > d->xmyR) return NULL;
-    return d->cells + 3 * ((xmy - d->xmyL) / 2);
+    return d->cells + 3 * (d->pp[x_coord(d->xay, xmy)] - d->pp[x_coord(d->xay, d->xmyL)]);
 }
 
 /* stateMachine3_startStateProb / raggedStart / endStateProb / raggedEnd
@@ -423,7 +431,7 @@ static void state_values(const or_model *m, int which, double out[3]) {
 }
 
 static void diag_init_values(or_diag *d, const double v[3]) {
-    int64_t w = diag_width(d);
+    int64_t w = diag_entries(d);
     for (int64_t i = 0; i < w; i++)
         for (int s = 0; s < 3; s++) d->cells[3 * i + s] = v[s];
 }
@@ -434,30 +442,6 @@ static inline void do_transition(int backward, double *from, double *to, int f, 
     else from[f] = or_logadd(from[f], to[t] + (eP + tP));
 }
 
-/* stateMachine3_cellCalculate (impl/stateMachine.c:1194-1257) and the HDP variant
- * (:1259-1325) for single-path cells: the legality checks are always true. */
-static void cell_calculate(const or_model *m, double *cur, double *lower, double *middle, double *upper,
-                           const char *kmer, const double *event, int backward) {
-    const double *t = m->trans;
-    if (lower) {
-        double eP = gapx_emission(m, kmer);
-        do_transition(backward, lower, cur, M_ST, X_ST, eP, t[OR_T_GAP_OPEN_X]);
-        do_transition(backward, lower, cur, X_ST, X_ST, eP, t[OR_T_GAP_EXTEND_X]);
-        do_transition(backward, lower, cur, Y_ST, X_ST, eP, t[OR_T_GAP_SWITCH_TO_X]);
-    }
-    if (middle) {
-        double eP = or_emission(m, kmer, event, 1);
-        do_transition(backward, middle, cur, M_ST, M_ST, eP, t[OR_T_MATCH_CONTINUE]);
-        do_transition(backward, middle, cur, X_ST, M_ST, eP, t[OR_T_MATCH_FROM_GAP_X]);
-        do_transition(backward, middle, cur, Y_ST, M_ST, eP, t[OR_T_MATCH_FROM_GAP_Y]);
-    }
-    if (upper) {
-        double eP = or_emission(m, kmer, event, 0);
-        do_transition(backward, upper, cur, M_ST, Y_ST, eP, t[OR_T_GAP_OPEN_Y]);
-        do_transition(backward, upper, cur, Y_ST, Y_ST, eP, t[OR_T_GAP_EXTEND_Y]);
-    }
-}
-
 static const double k_null_event[2] = {-INFINITY, 0};
 
 typedef struct {
@@ -466,7 +450,81 @@ typedef struct {
     int64_t lX;
     const double *ev;  /* 4 doubles per event */
     int64_t lY;
+    const int64_t *pp; /* lX + 2 entries: first path entry of column x (column x, x >= 1, is k-mer x-1) */
+    const char *pk;    /* kmer_length characters per path entry: the substituted k-mer */
 } or_seqs;
+
+static int64_t n_paths(const or_seqs *s, int64_t x) { return s->pp[x + 1] - s->pp[x]; }
+
+/* Path->kmer: NULL in column 0 (no k-mer) */
+static const char *path_kmer(const or_seqs *s, int64_t x, int64_t p) {
+    return x >= 1 ? s->pk + (s->pp[x] + p) * s->m->kmer_length : NULL;
+}
+
+/* path_checkLegal (impl/pairwiseAligner.c:540-552): consecutive k-mers agree on
+ * their overlap; a NULL k-mer (column 0) joins anything */
+static int path_legal(const char *from, const char *to, int k) {
+    if (from == NULL || to == NULL) return 1;
+    for (int i = 0; i < k - 1; i++)
+        if (from[i + 1] != to[i]) return 0;
+    return 1;
+}
+
+/* stString_eq on path k-mers (two NULL k-mers are equal) */
+static int kmer_eq(const char *a, const char *b, int k) {
+    if (a == NULL || b == NULL) return a == b;
+    return memcmp(a, b, (size_t)k) == 0;
+}
+
+/* stateMachine3_cellCalculate (impl/stateMachine.c:1194-1257) and the HDP variant
+ * (:1259-1325) over the cells' paths: lower and middle transitions between legal
+ * path pairs, upper transitions within the same path, each block looping over the
+ * current cell's paths and then the neighbour's, as the reference does. */
+static void cell_calculate(const or_seqs *s, int64_t x, double *cur, double *lower, double *middle,
+                           double *upper, const double *event, int backward) {
+    const or_model *m = s->m;
+    const double *t = m->trans;
+    const int k = m->kmer_length;
+    const int64_t pc = n_paths(s, x), pn = x >= 1 ? n_paths(s, x - 1) : 0;
+    if (lower) {
+        for (int64_t p = 0; p < pc; p++) {
+            const char *kc = path_kmer(s, x, p);
+            double eP = gapx_emission(m, kc);
+            for (int64_t q = 0; q < pn; q++) {
+                if (!path_legal(path_kmer(s, x - 1, q), kc, k)) continue;
+                double *lo = lower + 3 * q, *c = cur + 3 * p;
+                do_transition(backward, lo, c, M_ST, X_ST, eP, t[OR_T_GAP_OPEN_X]);
+                do_transition(backward, lo, c, X_ST, X_ST, eP, t[OR_T_GAP_EXTEND_X]);
+                do_transition(backward, lo, c, Y_ST, X_ST, eP, t[OR_T_GAP_SWITCH_TO_X]);
+            }
+        }
+    }
+    if (middle) {
+        for (int64_t p = 0; p < pc; p++) {
+            const char *kc = path_kmer(s, x, p);
+            double eP = or_emission(m, kc, event, 1);
+            for (int64_t q = 0; q < pn; q++) {
+                if (!path_legal(path_kmer(s, x - 1, q), kc, k)) continue;
+                double *mi = middle + 3 * q, *c = cur + 3 * p;
+                do_transition(backward, mi, c, M_ST, M_ST, eP, t[OR_T_MATCH_CONTINUE]);
+                do_transition(backward, mi, c, X_ST, M_ST, eP, t[OR_T_MATCH_FROM_GAP_X]);
+                do_transition(backward, mi, c, Y_ST, M_ST, eP, t[OR_T_MATCH_FROM_GAP_Y]);
+            }
+        }
+    }
+    if (upper) {
+        for (int64_t p = 0; p < pc; p++) {
+            const char *kc = path_kmer(s, x, p);
+            double eP = or_emission(m, kc, event, 0);
+            for (int64_t q = 0; q < pc; q++) {
+                if (!kmer_eq(kc, path_kmer(s, x, q), k)) continue;
+                double *up = upper + 3 * q, *c = cur + 3 * p;
+                do_transition(backward, up, c, M_ST, Y_ST, eP, t[OR_T_GAP_OPEN_Y]);
+                do_transition(backward, up, c, Y_ST, Y_ST, eP, t[OR_T_GAP_EXTEND_Y]);
+            }
+        }
+    }
+}
 
 /* diagonalCalculation (impl/pairwiseAligner.c:1127-1158): cells in ascending xmy,
  * lower = (d-1, xmy-1), middle = (d-2, xmy), upper = (d-1, xmy+1). */
@@ -475,77 +533,142 @@ static void diagonal_calculation(const or_seqs *s, or_diag *cur, const or_diag *
     for (int64_t xmy = cur->xmyL; xmy <= cur->xmyR; xmy += 2) {
         int64_t x = x_coord(cur->xay, xmy);
         int64_t y = y_coord(cur->xay, xmy);
-        const char *kmer = (x - 1 >= 0 && x - 1 < s->lX) ? s->ref + (x - 1) : NULL;
         const double *event = (y - 1 >= 0) ? s->ev + 4 * (y - 1) : k_null_event;
         double *c = diag_cell(cur, xmy);
         double *lo = diag_cell(m1, xmy - 1);
         double *mi = diag_cell(m2, xmy);
         double *up = diag_cell(m1, xmy + 1);
-        cell_calculate(s->m, c, lo, mi, up, kmer, event, backward);
+        cell_calculate(s, x, c, lo, mi, up, event, backward);
     }
 }
 
 /* cell_dotProduct + hdCell_totalProbability + dpDiagonal_dotProduct
- * (impl/pairwiseAligner.c:728-734, :677-692, :1014-1027) */
-static double diag_dot(const or_diag *a, const or_diag *b) {
+ * (impl/pairwiseAligner.c:728-734, :677-692, :1014-1027): per cell the paths with
+ * equal k-mers, per path the three states */
+static double diag_dot(const or_seqs *s, const or_diag *a, const or_diag *b) {
+    const int k = s->m->kmer_length;
     double total = OR_LOG_ZERO;
     for (int64_t xmy = a->xmyL; xmy <= a->xmyR; xmy += 2) {
+        const int64_t x = x_coord(a->xay, xmy), np = n_paths(s, x);
         const double *c1 = diag_cell(a, xmy), *c2 = diag_cell(b, xmy);
-        double cell = c1[0] + c2[0];
-        cell = or_logadd(cell, c1[1] + c2[1]);
-        cell = or_logadd(cell, c1[2] + c2[2]);
-        total = or_logadd(total, or_logadd(OR_LOG_ZERO, cell));
+        double hd = OR_LOG_ZERO;
+        for (int64_t p = 0; p < np; p++)
+            for (int64_t q = 0; q < np; q++) {
+                if (!kmer_eq(path_kmer(s, x, p), path_kmer(s, x, q), k)) continue;
+                const double *u = c1 + 3 * p, *v = c2 + 3 * q;
+                double cell = u[0] + v[0];
+                cell = or_logadd(cell, u[1] + v[1]);
+                cell = or_logadd(cell, u[2] + v[2]);
+                hd = or_logadd(hd, cell);
+            }
+        total = or_logadd(total, hd);
     }
     return total;
 }
 
 /* diagonalCalculationTotalProbability (impl/pairwiseAligner.c:1182-1200) */
 static double total_probability(const or_seqs *s, int64_t xay, const or_matrix *fw, const or_matrix *bw) {
-    double total = diag_dot(matrix_get(fw, xay), matrix_get(bw, xay));
+    double total = diag_dot(s, matrix_get(fw, xay), matrix_get(bw, xay));
     const or_diag *f1 = matrix_get(fw, xay - 1);
     const or_diag *b1 = matrix_get(bw, xay + 1);
     if (b1 != NULL && f1 != NULL) {
         or_diag md;
         md.xay = b1->xay; md.xmyL = b1->xmyL; md.xmyR = b1->xmyR;
-        int64_t w = diag_width(b1);
+        md.pp = b1->pp;
+        int64_t w = diag_entries(b1);
         md.cells = (double *)malloc(sizeof(double) * 3 * (size_t)w);
         for (int64_t i = 0; i < 3 * w; i++) md.cells[i] = OR_LOG_ZERO;
         diagonal_calculation(s, &md, NULL, f1, 0);
-        total = or_logadd(total, diag_dot(&md, b1));
+        total = or_logadd(total, diag_dot(s, &md, b1));
         free(md.cells);
     }
     return total;
 }
+
+/* hdCell_construct (impl/pairwiseAligner.c:599-656) for every column: column 0 has
+ * one path with no k-mer; column x >= 1 one path per combination of the degenerate
+ * base options at the 'X' positions of k-mer x-1, the first 'X' varying slowest
+ * (path_listPotentialKmers, :569-585; hdCell_getSubstitutedKmer, :667-676).
+ * Returns 0, or -30 for an 'X' without options. */
+static int build_paths(const char *ref, int64_t lX, int k, const char *options, int64_t **pp_out, char **pk_out) {
+    const int64_t nopt = options ? (int64_t)strlen(options) : 0;
+    int64_t *pp = (int64_t *)malloc(sizeof(int64_t) * (size_t)(lX + 2));
+    pp[0] = 0;
+    pp[1] = 1;
+    for (int64_t x = 1; x <= lX; x++) {
+        int64_t n = 0;
+        for (int j = 0; j < k; j++) n += ref[x - 1 + j] == 'X';
+        if (n > 0 && nopt == 0) { free(pp); return -30; }
+        int64_t np = 1;
+        for (int64_t j = 0; j < n; j++) np *= nopt;
+        pp[x + 1] = pp[x] + np;
+    }
+    char *pk = (char *)malloc((size_t)(pp[lX + 1] + 1) * (size_t)k);
+    for (int64_t x = 1; x <= lX; x++) {
+        for (int64_t p = 0; p < pp[x + 1] - pp[x]; p++) {
+            char *dst = pk + (pp[x] + p) * k;
+            memcpy(dst, ref + x - 1, (size_t)k);
+            int64_t rem = p;
+            for (int j = k - 1; j >= 0; j--)
+                if (dst[j] == 'X') {
+                    dst[j] = options[rem % nopt];
+                    rem /= nopt;
+                }
+        }
+    }
+    *pp_out = pp;
+    *pk_out = pk;
+    return 0;
+}
+
+int64_t or_path_kmers(const char *kmer, int k, const char *options, char *out, int64_t cap) {
+    int64_t *pp = NULL;
+    char *pk = NULL;
+    if (build_paths(kmer, 1, k, options, &pp, &pk) != 0) return -1;
+    const int64_t n = pp[2] - pp[1];
+    if (out && n <= cap) memcpy(out, pk + pp[1] * k, (size_t)(n * k));
+    free(pp);
+    free(pk);
+    return n;
+}
+
+int or_path_legal(const char *from, const char *to, int k) { return path_legal(from, to, k); }
 
 typedef struct {
     or_pair *buf;
     int64_t n, cap;
 } or_pairlist;
 
-static void pairlist_push(or_pairlist *l, int64_t p, int64_t x, int64_t y) {
+static void pairlist_push(or_pairlist *l, int64_t p, int64_t x, int64_t y, int64_t path) {
     if (l->n >= l->cap) {
         int64_t nc = l->cap ? 2 * l->cap : 1024;
         l->buf = (or_pair *)realloc(l->buf, sizeof(or_pair) * (size_t)nc);
         l->cap = nc;
     }
-    l->buf[l->n].p = p; l->buf[l->n].x = x; l->buf[l->n].y = y;
+    l->buf[l->n].p = p; l->buf[l->n].x = x; l->buf[l->n].y = y; l->buf[l->n].path = path;
     l->n++;
 }
 
 /* diagonalCalculationPosteriorMatchProbs (impl/pairwiseAligner.c:1202-1253) */
-static void posterior_match_probs(int64_t xay, const or_matrix *fw, const or_matrix *bw, double total,
-                                  double threshold, or_pairlist *out) {
+static void posterior_match_probs(const or_seqs *s, int64_t xay, const or_matrix *fw, const or_matrix *bw,
+                                  double total, double threshold, or_pairlist *out) {
+    const int k = s->m->kmer_length;
     const or_diag *fd = matrix_get(fw, xay), *bd = matrix_get(bw, xay);
     for (int64_t xmy = fd->xmyL; xmy <= fd->xmyR; xmy += 2) {
         int64_t x = x_coord(xay, xmy), y = y_coord(xay, xmy);
         if (x > 0 && y > 0) {
             const double *cf = diag_cell(fd, xmy), *cb = diag_cell(bd, xmy);
-            double p = exp((cf[M_ST] + cb[M_ST]) - total);
-            if (p >= threshold) {
-                if (p > 1.0) p = 1.0;
-                p = floor(p * OR_PAIR_PROB_1);
-                pairlist_push(out, (int64_t)p, x - 1, y - 1);
-            }
+            const int64_t np = n_paths(s, x);
+            for (int64_t fp = 0; fp < np; fp++)
+                for (int64_t bp = 0; bp < np; bp++) {
+                    if (!kmer_eq(path_kmer(s, x, fp), path_kmer(s, x, bp), k)) continue;
+                    double p = exp((cf[3 * fp + M_ST] + cb[3 * bp + M_ST]) - total);
+                    if (p >= threshold) {
+                        if (p > 1.0) p = 1.0;
+                        p = floor(p * OR_PAIR_PROB_1);
+                        pairlist_push(out, (int64_t)p, x - 1, y - 1, fp);
+                    }
+                }
         }
     }
 }
@@ -581,7 +704,7 @@ static void expectations_diagonal(const or_seqs *s, int64_t xay, const or_matrix
     const or_diag *m2 = xay >= 2 ? matrix_get(fw, xay - 2) : NULL;
     for (int64_t xmy = cur->xmyL; xmy <= cur->xmyR; xmy += 2) {
         int64_t x = x_coord(xay, xmy), y = y_coord(xay, xmy);
-        const char *kmer = (x - 1 >= 0 && x - 1 < s->lX) ? s->ref + (x - 1) : NULL;
+        const char *kmer = path_kmer(s, x, 0); /* single-path cells (the driver checks) */
         const double *event = (y - 1 >= 0) ? s->ev + 4 * (y - 1) : k_null_event;
         double *c = diag_cell(cur, xmy);
         double *lo = diag_cell(m1, xmy - 1);
@@ -645,7 +768,7 @@ static int posterior_probs_with_banding(const or_seqs *s, const int64_t *anchors
     if (rc != 0) { free(bL); free(bR); return -11; }
     for (int64_t d = 0; d <= N; d++) stats->cells += (bR[d] - bL[d]) / 2 + 1;
 
-    or_matrix *fw = matrix_new(N), *bw = matrix_new(N);
+    or_matrix *fw = matrix_new(N, s->pp), *bw = matrix_new(N, s->pp);
     double v[3];
     state_values(s->m, ragged_left ? 1 : 0, v);
     diag_init_values(matrix_create(fw, 0, bL[0], bR[0]), v);
@@ -686,7 +809,7 @@ static int posterior_probs_with_banding(const or_seqs *s, const int64_t *anchors
                         if (d2 == N) stats->log_likelihood += total; /* summed over sub-alignments */
                     }
                     if (ex) expectations_diagonal(s, d2, fw, bw, total, ex);
-                    else posterior_match_probs(d2, fw, bw, total, p->threshold, out);
+                    else posterior_match_probs(s, d2, fw, bw, total, p->threshold, out);
                     if (d2 < traced_back_from || at_end) matrix_delete(fw, d2);
                 }
                 if (d2 + 1 <= N) matrix_delete(bw, d2 + 1);
@@ -722,6 +845,15 @@ int or_get_aligned_pairs(const or_model *m, const char *ref, int64_t lX,
                          const int64_t *anchors, int64_t n_anchors,
                          const or_params *p, int ragged_left, int ragged_right,
                          or_pair *out, int64_t cap, int64_t *n_out, or_stats *stats) {
+    return or_get_aligned_pairs_degenerate(m, ref, lX, events, lY, anchors, n_anchors, p, ragged_left, ragged_right,
+                                           NULL, out, cap, n_out, stats);
+}
+
+int or_get_aligned_pairs_degenerate(const or_model *m, const char *ref, int64_t lX,
+                                    const double *events, int64_t lY,
+                                    const int64_t *anchors, int64_t n_anchors,
+                                    const or_params *p, int ragged_left, int ragged_right, const char *options,
+                                    or_pair *out, int64_t cap, int64_t *n_out, or_stats *stats) {
     or_stats st = {0, 0, 0.0};
     int64_t nsplit = or_split_points(anchors, n_anchors, lX, lY, p->split_matrix_bigger_than_this,
                                      ragged_left, ragged_right, NULL, 0);
@@ -746,15 +878,22 @@ int or_get_aligned_pairs(const or_model *m, const char *ref, int64_t lX,
             j++;
         }
         if (rc) break;
-        or_seqs s = {m, ref + x1, x2 - x1, events + 4 * y1, y2 - y1};
+        int64_t *pp = NULL;
+        char *pk = NULL;
+        rc = build_paths(ref + x1, x2 - x1, m->kmer_length, options, &pp, &pk);
+        if (rc) break;
+        or_seqs s = {m, ref + x1, x2 - x1, events + 4 * y1, y2 - y1, pp, pk};
         pl.n = 0;
         rc = posterior_probs_with_banding(&s, sub, ns, p, ragged_left || i > 0,
                                           ragged_right || i < nsplit - 1, &pl, &st, NULL);
+        free(pp);
+        free(pk);
         for (int64_t k = pl.n - 1; k >= 0 && rc == 0; k--) {
             if (n < cap) {
                 out[n].p = pl.buf[k].p;
                 out[n].x = pl.buf[k].x + x1;
                 out[n].y = pl.buf[k].y + y1;
+                out[n].path = pl.buf[k].path;
             }
             n++;
         }
@@ -798,11 +937,17 @@ int or_get_expectations(const or_model *m, const char *ref, int64_t lX, const do
             j++;
         }
         if (rc) break;
-        or_seqs s = {m, ref + x1, x2 - x1, events + 4 * y1, y2 - y1};
+        int64_t *pp = NULL;
+        char *pk = NULL;
+        rc = build_paths(ref + x1, x2 - x1, m->kmer_length, NULL, &pp, &pk);
+        if (rc) break;
+        or_seqs s = {m, ref + x1, x2 - x1, events + 4 * y1, y2 - y1, pp, pk};
         ex->x1 = x1;
         ex->y1 = y1;
         rc = posterior_probs_with_banding(&s, sub, ns, p, ragged_left || i > 0, ragged_right || i < nsplit - 1,
                                           NULL, &st, ex);
+        free(pp);
+        free(pk);
     }
     if (rc == 0 && j != n_anchors) rc = -3;
     free(sub);
@@ -826,8 +971,11 @@ int or_full_forward_backward(const or_model *m, const char *ref, int64_t lX,
     int64_t *bL = (int64_t *)malloc(sizeof(int64_t) * (size_t)(N + 1));
     int64_t *bR = (int64_t *)malloc(sizeof(int64_t) * (size_t)(N + 1));
     if (or_band(anchors, n_anchors, lX, lY, expansion, bL, bR) != 0) { free(bL); free(bR); return -1; }
-    or_seqs s = {m, ref, lX, events, lY};
-    or_matrix *fw = matrix_new(N), *bw = matrix_new(N);
+    int64_t *pp = NULL;
+    char *pk = NULL;
+    if (build_paths(ref, lX, m->kmer_length, NULL, &pp, &pk) != 0) { free(bL); free(bR); return -1; }
+    or_seqs s = {m, ref, lX, events, lY, pp, pk};
+    or_matrix *fw = matrix_new(N, pp), *bw = matrix_new(N, pp);
     for (int64_t d = 0; d <= N; d++) {
         matrix_create(fw, d, bL[d], bR[d]);
         matrix_create(bw, d, bL[d], bR[d]);
@@ -860,7 +1008,7 @@ int or_full_forward_backward(const or_model *m, const char *ref, int64_t lX,
     if (diagonal_totals)
         for (int64_t d = 0; d <= N; d++) diagonal_totals[d] = total_probability(&s, d, fw, bw);
     or_pairlist pl = {NULL, 0, 0};
-    for (int64_t d = 1; d <= N; d++) posterior_match_probs(d, fw, bw, tf, threshold, &pl);
+    for (int64_t d = 1; d <= N; d++) posterior_match_probs(&s, d, fw, bw, tf, threshold, &pl);
     for (int64_t k = 0; k < pl.n; k++)
         if (k < cap) out[k] = pl.buf[k];
     *n_out = pl.n;
@@ -869,5 +1017,7 @@ int or_full_forward_backward(const or_model *m, const char *ref, int64_t lX,
     matrix_free(bw);
     free(bL);
     free(bR);
+    free(pp);
+    free(pk);
     return 0;
 }

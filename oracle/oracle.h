@@ -68,6 +68,7 @@ typedef struct {
     int64_t p;
     int64_t x;
     int64_t y;
+    int64_t path;   /* path of the cell (0 unless the k-mer holds degenerate 'X' bases) */
 } or_pair;
 
 /* Per-window diagnostics of one aligned sub-problem. */
@@ -131,6 +132,23 @@ int or_get_aligned_pairs(const or_model *m, const char *ref, int64_t lX,
                          const int64_t *anchors, int64_t n_anchors,
                          const or_params *p, int ragged_left, int ragged_right,
                          or_pair *out, int64_t cap, int64_t *n_out, or_stats *stats);
+
+/* The same with degenerate 'X' bases in ref: each X position takes every base of
+ * `options` (the Sequence's degenerateBases, e.g. "CEO"), a cell one path per
+ * combination (hdCell_construct, impl/pairwiseAligner.c:599-656); each pair
+ * carries the index of its path in that order (the first X varying slowest). */
+int or_get_aligned_pairs_degenerate(const or_model *m, const char *ref, int64_t lX,
+                                    const double *events, int64_t lY,
+                                    const int64_t *anchors, int64_t n_anchors,
+                                    const or_params *p, int ragged_left, int ragged_right, const char *options,
+                                    or_pair *out, int64_t cap, int64_t *n_out, or_stats *stats);
+
+/* The path k-mers of one k-mer (its 'X' positions substituted by every combination
+ * of options, the first X varying slowest): returns how many; writes k characters
+ * per path to out when cap allows.  -1 for an 'X' without options. */
+int64_t or_path_kmers(const char *kmer, int k, const char *options, char *out, int64_t cap);
+/* path_checkLegal (impl/pairwiseAligner.c:540-552) on two path k-mers (NULL: column 0). */
+int or_path_legal(const char *from, const char *to, int k);
 
 int or_full_forward_backward(const or_model *m, const char *ref, int64_t lX,
                              const double *events, int64_t lY,
