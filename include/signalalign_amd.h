@@ -104,6 +104,12 @@ typedef struct {
     int64_t n_anchors;
     double scale, shift, var; /* per-read descaling (StateMachine scale/shift/var) */
     int32_t ragged_left, ragged_right;
+    /* The bases a degenerate reference base 'X' stands for (the Sequence's
+     * degenerateBases, e.g. "CEO"), or NULL for a reference without 'X'.  A k-mer
+     * with 'X' is a multi-path cell: one path per combination of these options
+     * (hdCell_construct, impl/pairwiseAligner.c:599-656).  Batches holding such
+     * reads run in FP64; every read with 'X' in a batch takes the same options. */
+    const char *degenerate_options;
 } sa_read;
 
 /* Aligned pair: the reference tuple (floor(p*1e7), x, y, pathKmer) with the
@@ -121,6 +127,9 @@ typedef struct {
     int64_t cells;          /* band cells = work units */
     int64_t n_windows;
     int64_t n_subalignments;
+    /* per pair, the index of its path in the cell (the first 'X' varying slowest);
+     * NULL unless the batch held degenerate references */
+    const int32_t *paths;
 } sa_read_result;
 
 typedef struct {

@@ -607,6 +607,11 @@ static void make_read(const char *where, StateMachine *sM, Sequence *SsX, Sequen
     r->var = sM->var;
     r->ragged_left = raggedLeft;
     r->ragged_right = raggedRight;
+    /* a reference with 'X' takes the Sequence's degenerate base options */
+    if (r->ref_length > 0 && memchr(r->ref, 'X', (size_t)r->ref_length)) {
+        if (!SsX->degenerateBases || !SsX->degenerateBases[0]) die(where, "'X' in a sequence without degenerate bases");
+        r->degenerate_options = SsX->degenerateBases;
+    }
     prm->threshold = p->threshold;
     prm->min_diags_between_traceback = p->minDiagsBetweenTraceBack;
     prm->traceback_diagonals = p->traceBackDiagonals;
@@ -636,7 +641,8 @@ stList *getAlignedPairsUsingAnchors(
     sa_context *ctx = context();
     sa_model *dm = device_model(ctx, m);
     sa_result *res = NULL;
-    int rc = sa_align(ctx, dm, &r, &prm, g_precision, &res);
+    /* degenerate references run in FP64 */
+    int rc = sa_align(ctx, dm, &r, &prm, r.degenerate_options ? SA_FP64 : g_precision, &res);
     if (rc != SA_OK) {
         fprintf(stderr, "signalalign_amd: %s: status %d: %s\n", where, rc, sa_last_error(ctx));
         abort();
@@ -649,18 +655,34 @@ stList *getAlignedPairsUsingAnchors(
     g_last_ll = rr.log_likelihood;
     stList *out = stList_construct3(0, (void (*)(void *))stIntTuple_destruct);
     /* one NUL-terminated string per reference k-mer: the path k-mer of a
-       single-path cell is the reference k-mer at x */
+       single-path cell is the reference k-mer at x; a degenerate reference's pairs
+       each get their path's k-mer (the first 'X' varying slowest) */
     char *arena = NULL;
-    if (rr.n_pairs > 0) {
+    if (rr.n_pairs > 0 && !rr.paths) {
         arena = (char *)xmalloc((size_t)(lX * (k + 1)));
         for (int64_t x = 0; x < lX; x++) {
             memcpy(arena + x * (k + 1), (const char *)SsX->elements + x, (size_t)k);
             arena[x * (k + 1) + k] = '\0';
         }
+    } else if (rr.n_pairs > 0) {
+        arena = (char *)xmalloc((size_t)(rr.n_pairs * (k + 1)));
+        const int64_t nopt = (int64_t)strlen(SsX->degenerateBases);
+        for (int64_t i = 0; i < rr.n_pairs; i++) {
+            char *dst = arena + i * (k + 1);
+            memcpy(dst, (const char *)SsX->elements + rr.pairs[i].x, (size_t)k);
+            dst[k] = '\0';
+            int64_t rem = rr.paths[i];
+            for (int64_t j = k - 1; j >= 0; j--)
+                if (dst[j] == 'X') {
+                    dst[j] = SsX->degenerateBases[rem % nopt];
+                    rem /= nopt;
+                }
+        }
     }
     for (int64_t i = 0; i < rr.n_pairs; i++) {
         const sa_pair *q = &rr.pairs[i];
-        stList_append(out, stIntTuple_construct4(q->p, q->x, q->y, (int64_t)(intptr_t)(arena + (int64_t)q->x * (k + 1))));
+        const char *kmer = rr.paths ? arena + i * (k + 1) : arena + (int64_t)q->x * (k + 1);
+        stList_append(out, stIntTuple_construct4(q->p, q->x, q->y, (int64_t)(intptr_t)kmer));
     }
     attach_arena(out, arena);
     sa_result_free(res);

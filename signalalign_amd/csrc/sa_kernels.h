@@ -95,7 +95,30 @@ struct Args {
     int8_t *ex_obs;         /* n_kmers per task */
     ExRec *ex_rec;          /* W.rec_off.. (capacity W.rec_cap) */
     int32_t *ex_count;      /* records per window */
+    /* degenerate references: 'X' is a valid reference character (prep) */
+    int32_t paths;
 };
+
+/* Degenerate references ('X' bases, multi-path cells): per sub-alignment, column x
+ * (0..lX; column x >= 1 is k-mer x-1) holds path entries pp[x] .. pp[x+1]-1 (numbered
+ * per sub-alignment from 0), its k-mer covers the 'X' sites sa[x] .. sb[x]-1 (site
+ * index order = position order); per entry its column and the k-mer id of its
+ * substituted k-mer; per diagonal the first cell of its entries in the forward
+ * planes (all three states of every diagonal are kept). */
+struct PathTab {
+    const int64_t *col_off;   /* per task: first slot of its column arrays (lX + 2) */
+    const int64_t *ent_off;   /* per task: first slot of its entry arrays */
+    const int64_t *plane_off; /* per task: first cell of its forward planes */
+    const int32_t *pp, *sa, *sb;
+    const int32_t *ecol, *ekid;
+    const int64_t *doffE;     /* diag_off + d */
+    int32_t nopt;             /* options per 'X' site */
+    int32_t ring;             /* entries per ring row (power of two, >= the entries of any diagonal) */
+};
+inline size_t paths_forward_bytes(int R) { return (size_t)9 * R * sizeof(double); }
+inline size_t paths_backward_bytes(int R) { return (size_t)11 * R * sizeof(double) + 2 * sizeof(double); }
+hipError_t launch_paths_forward(const Args<double> &a, const PathTab &P, int n_tasks, hipStream_t st);
+hipError_t launch_paths_backward(const Args<double> &a, const PathTab &P, int n_wins, hipStream_t st);
 
 template <typename T> hipError_t launch_prep(const Args<T> &a, int n_tasks, hipStream_t st);
 /* bytes of LDS (or HBM slice) one workgroup of the LDS sweeps needs for a ring of R

@@ -56,8 +56,9 @@ __global__ void __launch_bounds__(256) prep_kernel(Args<T> a) {
         int32_t id = 0;
         bool bad = false;
         for (int j = 0; j < k; j++) {
-            int c = a.lut[(uint8_t)a.ref[t.ref_off + p + j]];
-            bad |= c < 0;
+            const char ch = a.ref[t.ref_off + p + j];
+            int c = a.lut[(uint8_t)ch];
+            bad |= c < 0 && !(a.paths && ch == 'X'); /* an 'X' k-mer's ids come per path */
             id = id * A + (c < 0 ? 0 : c);
         }
         a.kid[t.pos_off + p] = bad ? 0 : id;

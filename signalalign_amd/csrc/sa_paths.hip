@@ -1,0 +1,361 @@
+/*
+ * sa_paths.hip -- the banded forward-backward over degenerate references ('X'
+ * bases, multi-path cells; impl/pairwiseAligner.c:501-692), FP64.
+ *
+ * Column x of a sub-alignment (k-mer x-1; column 0 has no k-mer) holds one path
+ * per combination of the degenerate base options at the 'X' positions of its
+ * k-mer (hdCell_construct, :599-656), numbered with the first 'X' varying slowest
+ * (path_listPotentialKmers, :569-585).  A cell keeps three states per path.  A
+ * match or gapX transition joins path q of column x-1 and path p of column x when
+ * their k-mers agree on the overlap (path_checkLegal, :540-552): the shared 'X'
+ * sites take the same option.  GapY transitions stay on a path.
+ *
+ * The work unit is a path entry (cell, path): entries of a column are consecutive,
+ * columns in order, so a diagonal's entries are one contiguous range and the
+ * rings and forward planes are indexed by entry.  Recurrence order per state is the
+ * reference's: stateMachine3_cellCalculate visits the current cell's paths and,
+ * for each, the neighbour's paths in order (push form forward; the pull form
+ * backward takes middle, then upper, then lower contributions, each over the
+ * neighbour's paths in order).  Every diagonal's three states go to the forward
+ * planes.
+ */
+#include "sa_device.h"
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+namespace sa {
+
+namespace {
+
+__device__ __forceinline__ int ipow(int b, int e) {
+    int r = 1;
+    for (int i = 0; i < e; i++) r *= b;
+    return r;
+}
+
+/* path q of column x-1 -> path p of column x (x >= 1) */
+__device__ __forceinline__ bool legal(const int32_t *SA, const int32_t *SB, int x, int q, int p, int B) {
+    if (x - 1 == 0) return true; /* column 0 has no k-mer */
+    const int lo = max(SA[x - 1], SA[x]), hi = min(SB[x - 1], SB[x]);
+    for (int s = lo; s < hi; s++) {
+        const int dq = (q / ipow(B, SB[x - 1] - 1 - s)) % B;
+        const int dp = (p / ipow(B, SB[x] - 1 - s)) % B;
+        if (dq != dp) return false;
+    }
+    return true;
+}
+
+struct PathView {
+    const int32_t *pp, *SA, *SB, *ecol, *ekid;
+    const int64_t *doffE;
+    int64_t plane;
+    int B;
+};
+
+__device__ __forceinline__ PathView view(const PathTab &P, const TaskDev &t, int ti) {
+    PathView v;
+    v.pp = P.pp + P.col_off[ti];
+    v.SA = P.sa + P.col_off[ti];
+    v.SB = P.sb + P.col_off[ti];
+    v.ecol = P.ecol + P.ent_off[ti];
+    v.ekid = P.ekid + P.ent_off[ti];
+    v.doffE = P.doffE + t.diag_off;
+    v.plane = P.plane_off[ti];
+    v.B = P.nopt;
+    return v;
+}
+
+} // namespace
+
+__global__ void __launch_bounds__(256) paths_forward_kernel(Args<double> a, PathTab P) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int R = P.ring, RM = R - 1;
+    unsigned char *mem = a.ring_g ? a.ring_g + (size_t)blockIdx.x * a.ring_g_stride : smem;
+    double *ring = reinterpret_cast<double *>(mem); /* [3 diag][3 state][R] */
+    const int ti = blockIdx.x;
+    const TaskDev t = a.tasks[ti];
+    const PathView v = view(P, t, ti);
+    const Model<double> &m = a.m;
+    const int tid = threadIdx.x, NT = blockDim.x;
+    const int64_t N = (int64_t)t.lX + t.lY;
+    const int32_t *xlo = a.xlo + t.diag_off, *xhi = a.xhi + t.diag_off;
+    const int64_t evc = t.evc_off;
+    const double s = m.kind == 1 ? 1.0 : t.scale / t.var;
+    const double inv_var = 1.0 / t.var;
+    const double tMC = m.trans[0], tMFX = m.trans[1], tMFY = m.trans[2], tGOX = m.trans[3], tGOY = m.trans[4];
+    const double tGEX = m.trans[5], tGEY = m.trans[6], tSWX = m.trans[7];
+    const double NI = -__builtin_huge_val();
+
+    for (int64_t d = 0; d <= N; d++) {
+        const int lo = xlo[d], hi = xhi[d];
+        const int e0 = v.pp[lo], e1 = v.pp[hi + 1];
+        const bool h2 = d >= 2;
+        const int lo1 = d >= 1 ? xlo[d - 1] : 1, hi1 = d >= 1 ? xhi[d - 1] : 0;
+        const int lo2 = h2 ? xlo[d - 2] : 1, hi2 = h2 ? xhi[d - 2] : 0;
+        double *rc = ring + (d % 3) * 3 * R, *r1 = ring + ((d + 2) % 3) * 3 * R, *r2 = ring + ((d + 1) % 3) * 3 * R;
+        const int64_t pb = v.plane + v.doffE[d];
+        for (int en = e0 + tid; en < e1; en += NT) {
+            const int x = v.ecol[en];
+            const int p = en - v.pp[x];
+            const int64_t y = d - x;
+            double M = NI, X = NI, Y = NI;
+            if (d == 0) {
+                /* start or ragged-start values (impl/stateMachine.c:1022-1031) */
+                M = t.ragged_left ? NI : 0.0;
+                X = Y = t.ragged_left ? 0.0 : NI;
+            } else {
+                const int32_t kk = v.ekid[en];
+                if (x - 1 >= lo1 && x - 1 <= hi1) { /* lower -> gapX */
+                    const double eX = Emit<double>::gapx(m, kk);
+                    for (int q = 0; q < v.pp[x] - v.pp[x - 1]; q++) {
+                        if (!legal(v.SA, v.SB, x, q, p, v.B)) continue;
+                        const int eq = (v.pp[x - 1] + q) & RM;
+                        X = logadd(X, r1[eq] + (eX + tGOX));
+                        X = logadd(X, r1[R + eq] + (eX + tGEX));
+                        X = logadd(X, r1[2 * R + eq] + (eX + tSWX));
+                    }
+                }
+                if (x - 1 >= lo2 && x - 1 <= hi2) { /* middle -> match (x >= 1, y >= 1) */
+                    const double eM = Emit<double>::match(m, a, kk, evc + (y - 1), s, inv_var);
+                    for (int q = 0; q < v.pp[x] - v.pp[x - 1]; q++) {
+                        if (!legal(v.SA, v.SB, x, q, p, v.B)) continue;
+                        const int eq = (v.pp[x - 1] + q) & RM;
+                        M = logadd(M, r2[eq] + (eM + tMC));
+                        M = logadd(M, r2[R + eq] + (eM + tMFX));
+                        M = logadd(M, r2[2 * R + eq] + (eM + tMFY));
+                    }
+                }
+                if (x >= lo1 && x <= hi1 && x >= 1) { /* upper -> gapY, the same path */
+                    const double eY = Emit<double>::gapy(m, a, kk, evc + (y - 1), s, inv_var);
+                    const int eu = en & RM;
+                    Y = logadd(Y, r1[eu] + (eY + tGOY));
+                    Y = logadd(Y, r1[2 * R + eu] + (eY + tGEY));
+                }
+            }
+            const int es = en & RM;
+            rc[es] = M;
+            rc[R + es] = X;
+            rc[2 * R + es] = Y;
+            a.fM[pb + (en - e0)] = M;
+            a.fX[pb + (en - e0)] = X;
+            a.fY[pb + (en - e0)] = Y;
+        }
+        __syncthreads();
+    }
+}
+
+/* dpDiagonal_dotProduct's sequential log-add chain (impl/pairwiseAligner.c:1014-1027) */
+__device__ __forceinline__ double chain_paths(const double *term, int n) {
+    double acc = -__builtin_huge_val();
+    for (int i = 0; i < n; i++) acc = logadd(acc, term[i]);
+    return acc;
+}
+
+__global__ void __launch_bounds__(256) paths_backward_kernel(Args<double> a, PathTab P) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int R = P.ring, RM = R - 1;
+    unsigned char *mem = a.ring_g ? a.ring_g + (size_t)blockIdx.x * a.ring_g_stride : smem;
+    double *termA = reinterpret_cast<double *>(mem); /* R (cells of a diagonal) */
+    double *termB = termA + R;                       /* R */
+    double *bcast = termB + R;                       /* 2 */
+    double *ring = bcast + 2;                        /* [3 diag][3 state][R] */
+    const int wi = a.win_order[blockIdx.x];
+    const WinDev W = a.wins[wi];
+    const TaskDev t = a.tasks[W.task];
+    const PathView v = view(P, t, W.task);
+    const Model<double> &m = a.m;
+    const int tid = threadIdx.x, NT = blockDim.x;
+    const int32_t *xlo = a.xlo + t.diag_off, *xhi = a.xhi + t.diag_off;
+    const int64_t evc = t.evc_off;
+    const double s = m.kind == 1 ? 1.0 : t.scale / t.var;
+    const double inv_var = 1.0 / t.var;
+    const double tMC = m.trans[0], tMFX = m.trans[1], tMFY = m.trans[2], tGOX = m.trans[3], tGOY = m.trans[4];
+    const double tGEX = m.trans[5], tGEY = m.trans[6], tSWX = m.trans[7];
+    const double NI = -__builtin_huge_val();
+    const int D = W.D, TF = W.TF, TB = W.T;
+    const bool ragged_end = W.last && t.ragged_right;
+    const double e0v = ragged_end ? m.rendv[0] : m.endv[0];
+    const double e1v = ragged_end ? m.rendv[1] : m.endv[1];
+    const double e2v = ragged_end ? m.rendv[2] : m.endv[2];
+    const double thr = a.threshold;
+    int32_t *out = a.out + 4 * W.out_off;
+    __shared__ int s_cursor;
+    if (tid == 0) s_cursor = 0;
+    int posterior_count = 0;
+    double total = NI;
+    __shared__ int s_wave_count[4];
+
+    for (int e = D; e > TB; e--) {
+        const int lo = xlo[e], hi = xhi[e];
+        const int en0 = v.pp[lo], en1 = v.pp[hi + 1];
+        const bool h1 = e + 1 <= D, h2 = e + 2 <= D;
+        const int lo1 = h1 ? xlo[e + 1] : 1, hi1 = h1 ? xhi[e + 1] : 0;
+        const int lo2 = h2 ? xlo[e + 2] : 1, hi2 = h2 ? xhi[e + 2] : 0;
+        double *rc = ring + (e % 3) * 3 * R, *r1 = ring + ((e + 1) % 3) * 3 * R, *r2 = ring + ((e + 2) % 3) * 3 * R;
+        for (int en = en0 + tid; en < en1; en += NT) {
+            const int x = v.ecol[en];
+            const int q = en - v.pp[x];
+            const int y = e - x;
+            double M, X, Y;
+            if (e == D) {
+                M = e0v;
+                X = e1v;
+                Y = e2v;
+            } else {
+                M = X = Y = NI;
+                if (x + 1 >= lo2 && x + 1 <= hi2) { /* middle: cell (x+1, y+1) on e+2 */
+                    for (int p = 0; p < v.pp[x + 2] - v.pp[x + 1]; p++) {
+                        if (!legal(v.SA, v.SB, x + 1, q, p, v.B)) continue;
+                        const int ep = v.pp[x + 1] + p;
+                        const double eM = Emit<double>::match(m, a, v.ekid[ep], evc + y, s, inv_var);
+                        const double BM = r2[ep & RM];
+                        M = logadd(M, BM + (eM + tMC));
+                        X = logadd(X, BM + (eM + tMFX));
+                        Y = logadd(Y, BM + (eM + tMFY));
+                    }
+                }
+                if (x >= lo1 && x <= hi1 && x >= 1) { /* upper: cell (x, y+1) on e+1, the same path */
+                    const double eY = Emit<double>::gapy(m, a, v.ekid[en], evc + y, s, inv_var);
+                    const double BY = r1[2 * R + (en & RM)];
+                    M = logadd(M, BY + (eY + tGOY));
+                    Y = logadd(Y, BY + (eY + tGEY));
+                }
+                if (x + 1 >= lo1 && x + 1 <= hi1) { /* lower: cell (x+1, y) on e+1 */
+                    for (int p = 0; p < v.pp[x + 2] - v.pp[x + 1]; p++) {
+                        if (!legal(v.SA, v.SB, x + 1, q, p, v.B)) continue;
+                        const int ep = v.pp[x + 1] + p;
+                        const double eX = Emit<double>::gapx(m, v.ekid[ep]);
+                        const double BX = r1[R + (ep & RM)];
+                        M = logadd(M, BX + (eX + tGOX));
+                        X = logadd(X, BX + (eX + tGEX));
+                        Y = logadd(Y, BX + (eX + tSWX));
+                    }
+                }
+            }
+            const int es = en & RM;
+            rc[es] = M;
+            rc[R + es] = X;
+            rc[2 * R + es] = Y;
+        }
+        __syncthreads();
+
+        if (e <= TF) {
+            const int64_t fb = v.plane + v.doffE[e];
+            if (posterior_count % 10 == 0) {
+                /* diagonalCalculationTotalProbability (impl/pairwiseAligner.c:1182-1200): per
+                   cell the paths' dot products (hdCell_totalProbability, :677-692) */
+                const int w = hi - lo + 1;
+                for (int i = tid; i < w; i += NT) {
+                    const int x = lo + i;
+                    double hd = NI;
+                    for (int r = v.pp[x]; r < v.pp[x + 1]; r++) {
+                        const int64_t fi = fb + (r - en0);
+                        double c = a.fM[fi] + rc[r & RM];
+                        c = logadd(c, a.fX[fi] + rc[R + (r & RM)]);
+                        c = logadd(c, a.fY[fi] + rc[2 * R + (r & RM)]);
+                        hd = logadd(hd, c);
+                    }
+                    termA[i] = hd;
+                }
+                int w1 = 0;
+                if (h1) {
+                    /* the match flux into e+1 from e-1 (dotted with B[e+1]) */
+                    const int lo1e = xlo[e + 1];
+                    w1 = xhi[e + 1] - lo1e + 1;
+                    const int lom = xlo[e - 1], him = xhi[e - 1];
+                    const int64_t fbm = v.plane + v.doffE[e - 1];
+                    const int enm0 = v.pp[lom];
+                    for (int i = tid; i < w1; i += NT) {
+                        const int x = lo1e + i;
+                        const int y = e + 1 - x;
+                        const bool hm = x - 1 >= lom && x - 1 <= him && x >= 1 && y >= 1;
+                        double hd = NI;
+                        for (int r = 0; r < v.pp[x + 1] - v.pp[x]; r++) {
+                            const int er = v.pp[x] + r;
+                            double Mv = NI;
+                            if (hm) {
+                                const double eM = Emit<double>::match(m, a, v.ekid[er], evc + (y - 1), s, inv_var);
+                                for (int q = 0; q < v.pp[x] - v.pp[x - 1]; q++) {
+                                    if (!legal(v.SA, v.SB, x, q, r, v.B)) continue;
+                                    const int64_t fi = fbm + (v.pp[x - 1] + q - enm0);
+                                    Mv = logadd(Mv, a.fM[fi] + (eM + tMC));
+                                    Mv = logadd(Mv, a.fX[fi] + (eM + tMFX));
+                                    Mv = logadd(Mv, a.fY[fi] + (eM + tMFY));
+                                }
+                            }
+                            hd = logadd(hd, Mv + r1[er & RM]);
+                        }
+                        termB[i] = hd;
+                    }
+                }
+                __syncthreads();
+                if (tid == 0) {
+                    double tot = chain_paths(termA, w);
+                    if (h1) tot = logadd(tot, chain_paths(termB, w1));
+                    bcast[0] = tot;
+                }
+                __syncthreads();
+                total = bcast[0];
+                if (e == TF && tid == 0) a.win_total[wi] = total;
+            }
+            posterior_count++;
+            /* diagonalCalculationPosteriorMatchProbs (impl/pairwiseAligner.c:1202-1253):
+               every path of every cell, entries in order */
+            for (int b0 = en0; b0 < en1; b0 += NT) {
+                const int en = b0 + tid;
+                bool keep = false;
+                double pv = 0.0;
+                int x = 0, y = 0;
+                if (en < en1) {
+                    x = v.ecol[en];
+                    y = e - x;
+                    if (x > 0 && y > 0) {
+                        pv = exp((a.fM[fb + (en - en0)] + rc[en & RM]) - total);
+                        keep = pv >= thr;
+                    }
+                }
+                /* compaction in entry order: per wavefront ballots, then the wavefronts in order */
+                const unsigned long long bal = __ballot(keep);
+                const int wv = tid >> 6, nwv = (NT + 63) >> 6;
+                if ((tid & 63) == 0) s_wave_count[wv] = __popcll(bal);
+                __syncthreads();
+                int base = s_cursor;
+                for (int k = 0; k < wv; k++) base += s_wave_count[k];
+                if (keep) {
+                    const int idx = base + __popcll(bal & ((1ull << (tid & 63)) - 1ull));
+                    if (idx < W.out_cap) {
+                        if (pv > 1.0) pv = 1.0;
+                        out[4 * idx + 0] = (int32_t)floor(pv * (double)kPairProb1);
+                        out[4 * idx + 1] = x - 1;
+                        out[4 * idx + 2] = y - 1;
+                        out[4 * idx + 3] = en - v.pp[x];
+                    }
+                }
+                __syncthreads();
+                if (tid == 0) {
+                    int add = 0;
+                    for (int k = 0; k < nwv; k++) add += s_wave_count[k];
+                    s_cursor += add;
+                }
+                __syncthreads();
+            }
+        }
+    }
+    if (tid == 0) a.win_count[wi] = s_cursor;
+}
+
+hipError_t launch_paths_forward(const Args<double> &a, const PathTab &P, int n_tasks, hipStream_t st) {
+    const size_t lds = a.ring_g ? 0 : paths_forward_bytes(P.ring);
+    hipLaunchKernelGGL(paths_forward_kernel, dim3(n_tasks), dim3(256), lds, st, a, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_paths_backward(const Args<double> &a, const PathTab &P, int n_wins, hipStream_t st) {
+    const size_t lds = a.ring_g ? 0 : paths_backward_bytes(P.ring);
+    hipLaunchKernelGGL(paths_backward_kernel, dim3(n_wins), dim3(256), lds, st, a, P);
+    return hipGetLastError();
+}
+
+} // namespace sa

@@ -173,6 +173,11 @@ struct sa_batch {
     DBuf d_fM, d_fX, d_fY;
     DBuf d_wins, d_task_order, d_win_order, d_out, d_win_count, d_win_total, d_win_dst, d_packed;
     DBuf d_ring_g; /* LDS sweeps' rings in HBM when a band is too wide for LDS */
+    /* degenerate references ('X' with options): multi-path sweeps, FP64 */
+    int32_t paths = 0;
+    std::string options;      /* the bases an 'X' stands for */
+    std::string href;         /* the packed reference bytes (host copy, path tables) */
+    DBuf d_pcol, d_pent, d_pplane, d_pp, d_psa, d_psb, d_pecol, d_pekid, d_pdoff;
     /* EM expectations run (sa_expect): FP64 LDS sweeps with the accumulators */
     int32_t expect = 0;
     double ex_rec_threshold = 1.0;
@@ -202,7 +207,8 @@ struct sa_batch {
                        &d_task_xycells, &d_task_maxw, &d_task_span,
                        &d_task_nwin, &d_win_slots, &d_err, &d_fM, &d_fX, &d_fY, &d_wins, &d_task_order,
                        &d_win_order, &d_out, &d_win_count, &d_win_total, &d_win_dst, &d_packed, &d_ring_g,
-                       &d_ex_trans, &d_ex_s0, &d_ex_s1, &d_ex_obs, &d_ex_rec, &d_ex_count};
+                       &d_ex_trans, &d_ex_s0, &d_ex_s1, &d_ex_obs, &d_ex_rec, &d_ex_count,
+                       &d_pcol, &d_pent, &d_pplane, &d_pp, &d_psa, &d_psb, &d_pecol, &d_pekid, &d_pdoff};
         for (DBuf *b : all) b->release();
     }
 };
@@ -219,6 +225,7 @@ struct ReadExpect {
 
 struct sa_result {
     std::vector<sa_pair> pairs;
+    std::vector<int32_t> paths; /* per pair, degenerate batches only */
     std::vector<ReadOut> reads;
     sa_timing timing;
     std::vector<sa_launch> launches;
@@ -581,6 +588,20 @@ int batch_fill(sa_context *ctx, sa_batch *b, const sa_model *model, const sa_rea
     std::vector<int32_t> ancx, ancy;
     int64_t ref_off = 0, ev_off = 0;
     int64_t stride = -1;
+    b->paths = 0;
+    b->options.clear();
+    b->href.clear();
+    for (int64_t r = 0; r < n_reads; r++) {
+        const sa_read &rd = reads[r];
+        if (rd.ref_length > 0 && rd.ref && memchr(rd.ref, 'X', (size_t)rd.ref_length)) {
+            if (!rd.degenerate_options || !rd.degenerate_options[0])
+                return fail(ctx, SA_ERR_BAD_KMER, "read " + std::to_string(r) + ": 'X' with no degenerate options");
+            if (b->paths && b->options != rd.degenerate_options)
+                return fail(ctx, SA_ERR_UNSUPPORTED, "reads with 'X' in one batch take the same degenerate options");
+            b->options = rd.degenerate_options;
+            b->paths = 1;
+        }
+    }
     for (int64_t r = 0; r < n_reads; r++) {
         const sa_read &rd = reads[r];
         if (rd.event_stride < 2 || (stride >= 0 && rd.event_stride != stride) || rd.n_events < 0 ||
@@ -674,6 +695,7 @@ int batch_fill(sa_context *ctx, sa_batch *b, const sa_model *model, const sa_rea
         ro += reads[r].ref_length;
         eo += reads[r].n_events;
     }
+    if (b->paths) b->href.assign(h_ref, (size_t)ref_off);
     SA_HIP(ctx, b->d_ref.ensure(ref_bytes));
     SA_HIP(ctx, b->d_events.ensure(ev_bytes));
     SA_HIP(ctx, hipMemcpyAsync(b->d_ref.p, h_ref, ref_bytes, hipMemcpyHostToDevice, ctx->stream));
@@ -892,6 +914,7 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     a.expansion = b->params.diagonal_expansion;
     a.expect = expect ? 1 : 0;
     a.ex_rec_threshold = b->ex_rec_threshold;
+    a.paths = b->paths;
 
     SA_HIP(ctx, hipEventRecord(ctx->ev[0], st));
     SA_HIP(ctx, sa::launch_prep<T>(a, (int)nt, st));
@@ -1431,6 +1454,297 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     return SA_OK;
 }
 
+/* A batch with degenerate references: the multi-path sweeps (sa_paths.hip), FP64,
+ * every sub-alignment's forward sweep in one launch, then every window's backward
+ * pass; pairs carry their path.  The path tables are built here from the reference
+ * bytes: per column its 'X' sites and paths, per entry its substituted k-mer's id. */
+int run_paths(sa_batch *b, sa_result *res) {
+    sa_context *ctx = b->ctx;
+    const sa_model *model = b->model;
+    hipStream_t st = ctx->stream;
+    const int64_t nt = (int64_t)b->tasks.size();
+    sa_timing &tm = res->timing;
+    std::memset(&tm, 0, sizeof(tm));
+    res->reads.assign(b->n_reads, ReadOut());
+    if (b->expect) return fail(ctx, SA_ERR_UNSUPPORTED, "expectations of degenerate references");
+    if (nt == 0) {
+        for (int64_t r = 0; r < b->n_reads; r++) res->reads[r].n_sub = 0;
+        return SA_OK;
+    }
+    const int k = model->k, A = model->A;
+    const int B = (int)b->options.size();
+    SA_HIP(ctx, b->d_kid.ensure(sizeof(int32_t) * (b->total_pos + 64)));
+    SA_HIP(ctx, b->d_evm.ensure(sizeof(double) * (b->total_ev + 64)));
+    SA_HIP(ctx, b->d_evn.ensure(sizeof(double) * (b->total_ev + 64)));
+    SA_HIP(ctx, b->d_evl.ensure(sizeof(double) * (b->total_ev + 64)));
+    SA_HIP(ctx, b->d_evi.ensure(sizeof(double) * (b->total_ev + 64)));
+    SA_HIP(ctx, hipMemsetAsync(b->d_evm.p, 0, sizeof(double) * (b->total_ev + 64), st));
+    SA_HIP(ctx, hipMemsetAsync(b->d_evn.p, 0, sizeof(double) * (b->total_ev + 64), st));
+    SA_HIP(ctx, hipMemsetAsync(b->d_evl.p, 0, sizeof(double) * (b->total_ev + 64), st));
+    SA_HIP(ctx, hipMemsetAsync(b->d_evi.p, 0, sizeof(double) * (b->total_ev + 64), st));
+    SA_HIP(ctx, b->d_xlo.ensure(sizeof(int32_t) * b->total_diag));
+    SA_HIP(ctx, b->d_xhi.ensure(sizeof(int32_t) * b->total_diag));
+    SA_HIP(ctx, b->d_doff.ensure(sizeof(int32_t) * b->total_diag));
+    SA_HIP(ctx, b->d_foff.ensure(sizeof(double) * b->total_diag));
+    SA_HIP(ctx, b->d_dflag.ensure(sizeof(int8_t) * b->total_diag + 4));
+    SA_HIP(ctx, b->d_dcls.ensure(sizeof(int8_t) * b->total_diag + 4));
+    SA_HIP(ctx, b->d_doffxy.ensure(sizeof(int32_t) * b->total_diag));
+    SA_HIP(ctx, b->d_task_cells.ensure(sizeof(int64_t) * nt));
+    SA_HIP(ctx, b->d_task_xycells.ensure(sizeof(int64_t) * nt));
+    SA_HIP(ctx, b->d_task_maxw.ensure(sizeof(int32_t) * nt));
+    SA_HIP(ctx, b->d_task_span.ensure(sizeof(int32_t) * nt));
+    SA_HIP(ctx, b->d_task_nwin.ensure(sizeof(int32_t) * nt));
+    SA_HIP(ctx, b->d_win_slots.ensure(sizeof(WinDev) * std::max<int64_t>(b->total_wslots, 1)));
+    SA_HIP(ctx, b->d_err.ensure(sizeof(int32_t)));
+    SA_HIP(ctx, hipMemsetAsync(b->d_err.p, 0, sizeof(int32_t), st));
+
+    const DevTables<double> &tab = tables<double>(model);
+    sa::Args<double> a;
+    std::memset(&a, 0, sizeof(a));
+    a.m = device_model<double>(model, tab);
+    a.tasks = b->d_tasks.as<TaskDev>();
+    a.ref = b->d_ref.as<char>();
+    a.events = b->d_events.as<double>();
+    a.event_stride = b->stride;
+    a.anc_x = b->d_ancx.as<int32_t>();
+    a.anc_y = b->d_ancy.as<int32_t>();
+    a.lut = model->d_lut.as<int8_t>();
+    a.kid = b->d_kid.as<int32_t>();
+    a.evm = b->d_evm.as<double>(); a.evn = b->d_evn.as<double>(); a.evl = b->d_evl.as<double>();
+    a.evi = b->d_evi.as<double>();
+    a.xlo = b->d_xlo.as<int32_t>(); a.xhi = b->d_xhi.as<int32_t>(); a.doff = b->d_doff.as<int32_t>();
+    a.foff = b->d_foff.as<double>();
+    a.dflag = b->d_dflag.as<int8_t>();
+    a.dcls = b->d_dcls.as<int8_t>();
+    a.doffxy = b->d_doffxy.as<int32_t>();
+    a.task_cells = b->d_task_cells.as<int64_t>();
+    a.task_xycells = b->d_task_xycells.as<int64_t>();
+    a.task_maxw = b->d_task_maxw.as<int32_t>();
+    a.task_span = b->d_task_span.as<int32_t>();
+    a.task_nwin = b->d_task_nwin.as<int32_t>();
+    a.win_slots = b->d_win_slots.as<WinDev>();
+    a.err = b->d_err.as<int32_t>();
+    a.threshold = b->params.threshold;
+    a.min_diags = b->params.min_diags_between_traceback;
+    a.traceback = b->params.traceback_diagonals;
+    a.expansion = b->params.diagonal_expansion;
+    a.paths = 1;
+
+    SA_HIP(ctx, hipEventRecord(ctx->ev[0], st));
+    SA_HIP(ctx, sa::launch_prep<double>(a, (int)nt, st));
+    SA_HIP(ctx, hipEventRecord(ctx->ev[1], st));
+    std::vector<int64_t> cells(nt);
+    std::vector<int32_t> nwin(nt), xlo(b->total_diag), xhi(b->total_diag);
+    std::vector<WinDev> slots(b->total_wslots);
+    int32_t err = 0;
+    SA_HIP(ctx, hipMemcpyAsync(cells.data(), b->d_task_cells.p, sizeof(int64_t) * nt, hipMemcpyDeviceToHost, st));
+    SA_HIP(ctx, hipMemcpyAsync(nwin.data(), b->d_task_nwin.p, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, st));
+    SA_HIP(ctx, hipMemcpyAsync(slots.data(), b->d_win_slots.p, sizeof(WinDev) * b->total_wslots,
+                               hipMemcpyDeviceToHost, st));
+    SA_HIP(ctx, hipMemcpyAsync(xlo.data(), b->d_xlo.p, sizeof(int32_t) * b->total_diag, hipMemcpyDeviceToHost, st));
+    SA_HIP(ctx, hipMemcpyAsync(xhi.data(), b->d_xhi.p, sizeof(int32_t) * b->total_diag, hipMemcpyDeviceToHost, st));
+    SA_HIP(ctx, hipMemcpyAsync(&err, b->d_err.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    SA_HIP(ctx, hipStreamSynchronize(st));
+    if (err & sa::ERR_BAD_KMER) return fail(ctx, SA_ERR_BAD_KMER, "reference k-mer with a character outside the alphabet");
+    if (err & sa::ERR_BAND) return fail(ctx, SA_ERR_ANCHORS, "invalid band (anchors outside the matrix or out of order)");
+    if (err & sa::ERR_WINDOWS) return fail(ctx, SA_ERR_HIP, "window schedule overflow");
+
+    /* path tables */
+    std::vector<int64_t> col_off(nt), ent_off(nt), plane_off(nt);
+    std::vector<int32_t> pp, psa, psb, ecol, ekid;
+    std::vector<int64_t> doffE(b->total_diag);
+    int64_t planes = 0, max_ent = 1;
+    for (int64_t ti = 0; ti < nt; ti++) {
+        const TaskDev &t = b->tasks[ti];
+        const char *ref = b->href.data() + t.ref_off;
+        const int64_t lX = t.lX;
+        col_off[ti] = (int64_t)pp.size();
+        ent_off[ti] = (int64_t)ecol.size();
+        std::vector<int32_t> site; /* positions of 'X' in this sub-alignment's bases */
+        for (int64_t i = 0; i < lX + k - 1; i++)
+            if (ref[i] == 'X') site.push_back((int32_t)i);
+        int32_t P = 0;
+        size_t s0 = 0, s1 = 0;
+        for (int64_t x = 0; x <= lX; x++) {
+            int64_t np = 1;
+            if (x >= 1) {
+                while (s0 < site.size() && site[s0] < x - 1) s0++;
+                while (s1 < site.size() && site[s1] < x - 1 + k) s1++;
+                for (size_t j = s0; j < s1; j++) np *= B;
+            }
+            if (np > 729 || P + np > INT32_MAX / 2)
+                return fail(ctx, SA_ERR_UNSUPPORTED, "too many paths in a degenerate reference");
+            pp.push_back(P);
+            psa.push_back(x >= 1 ? (int32_t)s0 : 0);
+            psb.push_back(x >= 1 ? (int32_t)s1 : 0);
+            for (int64_t q = 0; q < np; q++) {
+                ecol.push_back((int32_t)x);
+                int32_t id = 0;
+                if (x >= 1) {
+                    /* the path's k-mer: the first 'X' varies slowest */
+                    char km[16];
+                    std::memcpy(km, ref + x - 1, (size_t)k);
+                    int64_t rem = q;
+                    for (int j = k - 1; j >= 0; j--)
+                        if (km[j] == 'X') {
+                            km[j] = b->options[(size_t)(rem % B)];
+                            rem /= B;
+                        }
+                    for (int j = 0; j < k; j++) {
+                        const int c = model->lut[(uint8_t)km[j]];
+                        if (c < 0) return fail(ctx, SA_ERR_BAD_KMER, "degenerate option outside the alphabet");
+                        id = id * A + c;
+                    }
+                }
+                ekid.push_back(id);
+            }
+            P += (int32_t)np;
+        }
+        pp.push_back(P);
+        psa.push_back(0);
+        psb.push_back(0);
+        plane_off[ti] = planes;
+        const int64_t N = (int64_t)t.lX + t.lY;
+        const int32_t *l = xlo.data() + t.diag_off, *h = xhi.data() + t.diag_off;
+        const int32_t *ppt = pp.data() + col_off[ti];
+        int64_t run = 0;
+        for (int64_t d = 0; d <= N; d++) {
+            doffE[t.diag_off + d] = run;
+            const int64_t ne = ppt[h[d] + 1] - ppt[l[d]];
+            run += ne;
+            max_ent = std::max(max_ent, ne);
+        }
+        planes += run;
+    }
+    const int ring = next_pow2((int)std::min<int64_t>(max_ent, 1 << 24));
+    /* windows and their pair slots: at most one pair per entry of a posterior diagonal */
+    std::vector<WinDev> wins;
+    std::vector<int32_t> task_first_win(nt);
+    int64_t out_total = 0;
+    for (int64_t ti = 0; ti < nt; ti++) {
+        const TaskDev &t = b->tasks[ti];
+        task_first_win[ti] = (int32_t)wins.size();
+        const int32_t *l = xlo.data() + t.diag_off, *h = xhi.data() + t.diag_off;
+        const int32_t *ppt = pp.data() + col_off[ti];
+        for (int32_t w = 0; w < nwin[ti]; w++) {
+            WinDev W = slots[t.win_off + w];
+            W.task = (int32_t)ti;
+            int64_t cap = 0;
+            for (int64_t d = W.T + 1; d <= W.TF; d++) cap += ppt[h[d] + 1] - ppt[l[d]];
+            if (cap > INT32_MAX) return fail(ctx, SA_ERR_UNSUPPORTED, "window too large");
+            W.out_off = out_total;
+            W.out_cap = (int32_t)cap;
+            out_total += cap;
+            wins.push_back(W);
+        }
+    }
+    const int64_t nw = (int64_t)wins.size();
+    std::vector<int32_t> win_order(nw);
+    for (int64_t w = 0; w < nw; w++) win_order[w] = (int32_t)w;
+    int rc;
+    if ((rc = upload(ctx, b->d_pcol, col_off)) || (rc = upload(ctx, b->d_pent, ent_off)) ||
+        (rc = upload(ctx, b->d_pplane, plane_off)) || (rc = upload(ctx, b->d_pp, pp)) ||
+        (rc = upload(ctx, b->d_psa, psa)) || (rc = upload(ctx, b->d_psb, psb)) ||
+        (rc = upload(ctx, b->d_pecol, ecol)) || (rc = upload(ctx, b->d_pekid, ekid)) ||
+        (rc = upload(ctx, b->d_pdoff, doffE)) || (rc = upload(ctx, b->d_wins, wins)) ||
+        (rc = upload(ctx, b->d_win_order, win_order)))
+        return rc;
+    SA_HIP(ctx, b->d_fM.ensure(sizeof(double) * std::max<int64_t>(planes, 1)));
+    SA_HIP(ctx, b->d_fX.ensure(sizeof(double) * std::max<int64_t>(planes, 1)));
+    SA_HIP(ctx, b->d_fY.ensure(sizeof(double) * std::max<int64_t>(planes, 1)));
+    SA_HIP(ctx, b->d_out.ensure(sizeof(int32_t) * 4 * std::max<int64_t>(out_total, 1)));
+    SA_HIP(ctx, b->d_win_count.ensure(sizeof(int32_t) * std::max<int64_t>(nw, 1)));
+    SA_HIP(ctx, b->d_win_total.ensure(sizeof(double) * std::max<int64_t>(nw, 1)));
+    sa::PathTab P;
+    P.col_off = b->d_pcol.as<int64_t>();
+    P.ent_off = b->d_pent.as<int64_t>();
+    P.plane_off = b->d_pplane.as<int64_t>();
+    P.pp = b->d_pp.as<int32_t>();
+    P.sa = b->d_psa.as<int32_t>();
+    P.sb = b->d_psb.as<int32_t>();
+    P.ecol = b->d_pecol.as<int32_t>();
+    P.ekid = b->d_pekid.as<int32_t>();
+    P.doffE = b->d_pdoff.as<int64_t>();
+    P.nopt = B;
+    P.ring = ring;
+    a.wins = b->d_wins.as<WinDev>();
+    a.win_order = b->d_win_order.as<int32_t>();
+    a.fM = b->d_fM.as<double>(); a.fX = b->d_fX.as<double>(); a.fY = b->d_fY.as<double>();
+    a.out = b->d_out.as<int32_t>();
+    a.win_count = b->d_win_count.as<int32_t>();
+    a.win_total = b->d_win_total.as<double>();
+    /* rings in LDS up to 64 KiB per workgroup, else per-workgroup slices of HBM */
+    constexpr size_t kLdsMax = 65536;
+    const size_t fb = sa::paths_forward_bytes(ring), bb = sa::paths_backward_bytes(ring);
+    const size_t fs = (fb + 255) / 256 * 256, bs = (bb + 255) / 256 * 256;
+    if (fb > kLdsMax || bb > kLdsMax)
+        SA_HIP(ctx, b->d_ring_g.ensure(std::max(fb > kLdsMax ? (size_t)nt * fs : 0, bb > kLdsMax ? (size_t)nw * bs : 0)));
+    SA_HIP(ctx, hipEventRecord(ctx->ev[2], st));
+    {
+        sa::Args<double> af = a;
+        af.ring_g = fb > kLdsMax ? b->d_ring_g.as<unsigned char>() : nullptr;
+        af.ring_g_stride = (int64_t)fs;
+        SA_HIP(ctx, sa::launch_paths_forward(af, P, (int)nt, st));
+    }
+    SA_HIP(ctx, hipEventRecord(ctx->ev[3], st));
+    if (nw) {
+        sa::Args<double> ab = a;
+        ab.ring_g = bb > kLdsMax ? b->d_ring_g.as<unsigned char>() : nullptr;
+        ab.ring_g_stride = (int64_t)bs;
+        SA_HIP(ctx, sa::launch_paths_backward(ab, P, (int)nw, st));
+    }
+    SA_HIP(ctx, hipEventRecord(ctx->ev[4], st));
+    std::vector<int32_t> wcount(nw);
+    std::vector<double> wtot(nw);
+    std::vector<int32_t> outv(4 * std::max<int64_t>(out_total, 1));
+    SA_HIP(ctx, hipMemcpyAsync(wcount.data(), b->d_win_count.p, sizeof(int32_t) * nw, hipMemcpyDeviceToHost, st));
+    SA_HIP(ctx, hipMemcpyAsync(wtot.data(), b->d_win_total.p, sizeof(double) * nw, hipMemcpyDeviceToHost, st));
+    if (out_total)
+        SA_HIP(ctx, hipMemcpyAsync(outv.data(), b->d_out.p, sizeof(int32_t) * 4 * out_total, hipMemcpyDeviceToHost,
+                                   st));
+    SA_HIP(ctx, hipStreamSynchronize(st));
+    /* the reference's list order: per read, sub-alignments in split order, each
+       sub-alignment's pairs in reverse emission order, shifted by (x1, y1) */
+    for (int64_t r = 0; r < b->n_reads; r++) {
+        ReadOut &ro = res->reads[r];
+        ro.first_pair = (int64_t)res->pairs.size();
+        ro.n_sub = b->reads[r].n_tasks;
+        for (int64_t ti = b->reads[r].first_task; ti < b->reads[r].first_task + b->reads[r].n_tasks; ti++) {
+            std::vector<int64_t> idx;
+            for (int32_t w = 0; w < nwin[ti]; w++) {
+                const WinDev &W = wins[task_first_win[ti] + w];
+                const int64_t n = std::min<int64_t>(wcount[task_first_win[ti] + w], W.out_cap);
+                for (int64_t q = 0; q < n; q++) idx.push_back(W.out_off + q);
+            }
+            for (auto it = idx.rbegin(); it != idx.rend(); ++it) {
+                const int32_t *o = outv.data() + 4 * *it;
+                sa_pair pr;
+                pr.p = o[0];
+                pr.x = o[1] + b->thost[ti].x1;
+                pr.y = o[2] + b->thost[ti].y1;
+                res->pairs.push_back(pr);
+                res->paths.push_back(o[3]);
+            }
+            if (nwin[ti] > 0) ro.log_likelihood += wtot[task_first_win[ti] + nwin[ti] - 1];
+            ro.cells += cells[ti];
+            ro.n_windows += nwin[ti];
+        }
+        ro.n_pairs = (int64_t)res->pairs.size() - ro.first_pair;
+    }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]); tm.prep_ms = ms;
+    (void)hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]); tm.forward_ms = ms;
+    (void)hipEventElapsedTime(&ms, ctx->ev[3], ctx->ev[4]); tm.backward_ms = ms;
+    (void)hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[4]); tm.fb_ms = ms;
+    (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[4]); tm.total_ms = ms;
+    int64_t tc = 0;
+    for (int64_t c : cells) tc += c;
+    tm.cells = tc;
+    tm.forward_launches = 1;
+    tm.backward_launches = nw ? 1 : 0;
+    return SA_OK;
+}
+
 /* A run from the recorded graph: same launches, same buffers, same results. */
 template <typename T>
 int replay_batch(sa_batch *b, sa_result *res) {
@@ -1513,6 +1827,18 @@ int sa_batch_run(sa_batch *b, sa_precision precision, sa_result **out) {
     const char *kern = getenv("SA_KERNELS");
     const bool lds = kern && std::strcmp(kern, "lds") == 0;
     int rc;
+    if (b->paths) {
+        /* degenerate references run in FP64 */
+        rc = precision == SA_FP64 ? run_paths(b, res)
+                                  : fail(ctx, SA_ERR_UNSUPPORTED, "degenerate references run in FP64");
+        if (rc) {
+            delete res;
+            return rc;
+        }
+        b->nruns[0]++;
+        *out = res;
+        return SA_OK;
+    }
     if (!lds && b->tune.use_graph && b->gc[precision == SA_FP32 ? 1 : 0].exec)
         rc = precision == SA_FP32 ? replay_batch<float>(b, res) : replay_batch<double>(b, res);
     else
@@ -1662,6 +1988,7 @@ int sa_result_read(const sa_result *res, int64_t i, sa_read_result *out) {
     out->cells = r.cells;
     out->n_windows = r.n_windows;
     out->n_subalignments = r.n_sub;
+    out->paths = res->paths.empty() || !r.n_pairs ? nullptr : res->paths.data() + r.first_pair;
     return SA_OK;
 }
 

@@ -170,6 +170,7 @@ void sa_server::lane_loop(Lane *L) {
             rd[i].var = q->var;
             rd[i].ragged_left = q->rl;
             rd[i].ragged_right = q->rr;
+            rd[i].degenerate_options = nullptr;
         }
         const Request *f = wave.front();
         int64_t cells = 0;
@@ -305,6 +306,9 @@ int sa_server_submit(sa_server *s, const sa_model *model, const sa_read *read, c
         return SA_ERR_INVALID_ARGUMENT;
     if (read->ref_length < 0 || read->n_events < 0 || read->n_anchors < 0 || read->event_stride < 2)
         return SA_ERR_INVALID_ARGUMENT;
+    /* the serving waves take plain references (degenerate ones run through sa_align) */
+    if (read->degenerate_options && read->ref_length > 0 && memchr(read->ref, 'X', (size_t)read->ref_length))
+        return SA_ERR_UNSUPPORTED;
     auto *q = new Request();
     q->model = model;
     if (params) q->params = *params; else sa_params_default(&q->params);

@@ -43,7 +43,8 @@ class _Read(ctypes.Structure):
     _fields_ = [("ref", ctypes.c_char_p), ("ref_length", ctypes.c_int64), ("events", _F64P),
                 ("n_events", ctypes.c_int64), ("event_stride", ctypes.c_int64), ("anchors", _I64P),
                 ("n_anchors", ctypes.c_int64), ("scale", ctypes.c_double), ("shift", ctypes.c_double),
-                ("var", ctypes.c_double), ("ragged_left", ctypes.c_int32), ("ragged_right", ctypes.c_int32)]
+                ("var", ctypes.c_double), ("ragged_left", ctypes.c_int32), ("ragged_right", ctypes.c_int32),
+                ("degenerate_options", ctypes.c_char_p)]
 
 
 class _Pair(ctypes.Structure):
@@ -64,7 +65,8 @@ class _Expectations(ctypes.Structure):
 
 class _ReadResult(ctypes.Structure):
     _fields_ = [("pairs", ctypes.POINTER(_Pair)), ("n_pairs", ctypes.c_int64), ("log_likelihood", ctypes.c_double),
-                ("cells", ctypes.c_int64), ("n_windows", ctypes.c_int64), ("n_subalignments", ctypes.c_int64)]
+                ("cells", ctypes.c_int64), ("n_windows", ctypes.c_int64), ("n_subalignments", ctypes.c_int64),
+                ("paths", ctypes.POINTER(ctypes.c_int32))]
 
 
 class _Timing(ctypes.Structure):
@@ -244,7 +246,8 @@ class Read:
     """One strand of one read: reference nucleotides, event records (mean, noise, ...),
     0-based anchors (k-mer index, event index), descaling and ragged ends."""
 
-    def __init__(self, ref, events, anchors, scale=1.0, shift=0.0, var=1.0, ragged_left=True, ragged_right=True):
+    def __init__(self, ref, events, anchors, scale=1.0, shift=0.0, var=1.0, ragged_left=True, ragged_right=True,
+                 degenerate_options=None):
         self.ref = ref.encode() if isinstance(ref, str) else bytes(ref)
         ev = np.asarray(events, dtype=np.float64)
         if ev.ndim != 2 or ev.shape[1] < 2:
@@ -253,20 +256,22 @@ class Read:
         self.anchors = np.ascontiguousarray(np.asarray(anchors, dtype=np.int64).reshape(-1, 2))
         self.scale, self.shift, self.var = float(scale), float(shift), float(var)
         self.ragged_left, self.ragged_right = bool(ragged_left), bool(ragged_right)
+        self.degenerate_options = degenerate_options.encode() if degenerate_options else None
 
     def _struct(self):
         return _Read(self.ref, len(self.ref), self.events.ctypes.data_as(_F64P), len(self.events),
                      self.events.shape[1], self.anchors.ctypes.data_as(_I64P), len(self.anchors), self.scale,
-                     self.shift, self.var, int(self.ragged_left), int(self.ragged_right))
+                     self.shift, self.var, int(self.ragged_left), int(self.ragged_right), self.degenerate_options)
 
 
 class ReadResult:
-    def __init__(self, pairs, log_likelihood, cells, n_windows, n_subalignments):
+    def __init__(self, pairs, log_likelihood, cells, n_windows, n_subalignments, paths=None):
         self.pairs = pairs              # (n, 3) int64 [p_int, x, y]
         self.log_likelihood = log_likelihood
         self.cells = cells
         self.n_windows = n_windows
         self.n_subalignments = n_subalignments
+        self.paths = paths              # (n,) int32 path per pair, degenerate references only
 
 
 class Context:
@@ -446,7 +451,10 @@ class Batch:
                         pairs = buf.reshape(-1, 3).astype(np.int64)
                     else:
                         pairs = np.zeros((0, 3), dtype=np.int64)
-                    out.append(ReadResult(pairs, rr.log_likelihood, rr.cells, rr.n_windows, rr.n_subalignments))
+                    paths = (np.ctypeslib.as_array(rr.paths, shape=(rr.n_pairs,)).copy()
+                             if rr.n_pairs and bool(rr.paths) else None)
+                    out.append(ReadResult(pairs, rr.log_likelihood, rr.cells, rr.n_windows, rr.n_subalignments,
+                                          paths))
             return out, timing
         finally:
             L.sa_result_free(res)

@@ -107,3 +107,97 @@ def test_degenerate_without_x_is_the_plain_run(oracle_lib):
                                       options="CEO")
     assert np.array_equal(p1, p2[:, :3]) and np.all(p2[:, 3] == 0)
     assert s1["log_likelihood"] == s2["log_likelihood"]
+
+
+# ---------------------------------------------------------------------------
+# the GPU path (multi-path sweeps) against the oracle
+# ---------------------------------------------------------------------------
+def _with_x(ref, every=1):
+    """The C of every CpG (every `every`-th one) becomes 'X'."""
+    r = list(ref)
+    n = 0
+    for i in range(len(r) - 1):
+        if r[i] == "C" and r[i + 1] == "G":
+            if n % every == 0:
+                r[i] = "X"
+            n += 1
+    return "".join(r)
+
+
+def _compare_paths(got, gpaths, want, threshold, p_tol=1e-4, edge=1e-3):
+    g = {(int(x), int(y), int(pa)): p / 1e7 for (p, x, y), pa in zip(got, gpaths)}
+    w = {(int(x), int(y), int(pa)): p / 1e7 for p, x, y, pa in want}
+    assert len(g) == len(got)
+    only_g = [k for k in g if k not in w and abs(g[k] - threshold) > edge]
+    only_w = [k for k in w if k not in g and abs(w[k] - threshold) > edge]
+    common = [k for k in g if k in w]
+    dmax = max((abs(g[k] - w[k]) for k in common), default=0.0)
+    out = {"n_gpu": len(g), "n_ref": len(w), "only_gpu": len(only_g), "only_ref": len(only_w), "max_abs_dp": dmax}
+    assert not only_g and not only_w, out
+    assert dmax <= p_tol, out
+    cs = set(common)
+    og = [(int(x), int(y), int(pa)) for (p, x, y), pa in zip(got, gpaths) if (int(x), int(y), int(pa)) in cs]
+    ow = [(int(x), int(y), int(pa)) for p, x, y, pa in want if (int(x), int(y), int(pa)) in cs]
+    assert og == ow, "pair order differs from the reference list order"
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,options,every,expansion,split", [
+    (300, "ECO", 1, 20, None), (2500, "OEC", 1, 50, None), (1800, "EC", 2, 20, None), (1200, "CEO", 1, 20, 100)])
+def test_gpu_degenerate_reads_match_oracle(gpu_ctx, oracle_lib, n, options, every, expansion, split):
+    """Reads whose CpG cytosines are 'X' (2 or 3 options per site), one to several
+    windows, with anchors, and cut into sub-alignments: the pairs, their paths,
+    posteriors and order, and the log-likelihood equal the oracle's (FP64)."""
+    from signalalign_amd import native, synth
+    from tests import _parity as P
+    m = models.synthetic_gaussian_model("ACEGOT", 6, seed=3)
+    r = synth.single_read(m, n, seed=11 + n)
+    r = dict(r, ref=_with_x(r["ref"].decode() if isinstance(r["ref"], bytes) else r["ref"], every))
+    if split:
+        r["anchors"] = r["anchors"][::6]
+    params = native.Params(diagonal_expansion=expansion,
+                           split_matrix_bigger_than_this=(split * split if split else 3000 * 3000))
+    om = P.oracle_model(m, r)
+    want, st = oracle.get_aligned_pairs(om, r["ref"], r["events"], r["anchors"],
+                                        oracle.Params(params.threshold, params.min_diags_between_traceback,
+                                                      params.traceback_diagonals, params.diagonal_expansion,
+                                                      params.split_matrix_bigger_than_this), options=options)
+    dm = gpu_ctx.model(m)
+    read = native.Read(r["ref"], r["events"], r["anchors"], r["scale"], r["shift"], r["var"],
+                       degenerate_options=options)
+    print(f"degenerate n={n} {options} every={every} split={split}", flush=True)
+    res, _ = gpu_ctx.align_batch(dm, [read], params, native.SA_FP64)
+    out = _compare_paths(res[0].pairs, res[0].paths, want, params.threshold)
+    assert P.rel(res[0].log_likelihood, st["log_likelihood"]) <= P.LL_REL
+    assert int(np.max(want[:, 3])) > 0
+    print(out, res[0].n_subalignments)
+
+
+@pytest.mark.gpu
+def test_gpu_degenerate_batch_mixed_and_refusals(gpu_ctx, oracle_lib):
+    """A batch mixing a degenerate read and a plain one (the plain one's pairs all on
+    path 0, equal to its single-path run); FP32 is refused for degenerate batches;
+    an 'X' without options is a bad k-mer."""
+    from signalalign_amd import native, synth
+    from tests import _parity as P
+    m = models.synthetic_gaussian_model("ACEGOT", 6, seed=3)
+    r1 = synth.single_read(m, 700, seed=21)
+    r2 = synth.single_read(m, 900, seed=22)
+    ref1 = _with_x(r1["ref"].decode() if isinstance(r1["ref"], bytes) else r1["ref"])
+    params = native.Params(diagonal_expansion=20)
+    dm = gpu_ctx.model(m)
+    reads = [native.Read(ref1, r1["events"], r1["anchors"], r1["scale"], r1["shift"], r1["var"],
+                         degenerate_options="CEO"),
+             native.Read(r2["ref"], r2["events"], r2["anchors"], r2["scale"], r2["shift"], r2["var"],
+                         degenerate_options="CEO")]
+    print("degenerate batch", flush=True)
+    res, _ = gpu_ctx.align_batch(dm, reads, params, native.SA_FP64)
+    plain, _ = gpu_ctx.align_batch(dm, [P.native_read(r2)], params, native.SA_FP64)
+    assert np.array_equal(res[1].pairs, plain[0].pairs) and np.all(res[1].paths == 0)
+    assert res[1].log_likelihood == plain[0].log_likelihood
+    with pytest.raises(native.SignalAlignError):
+        gpu_ctx.align_batch(dm, reads, params, native.SA_FP32)
+    with pytest.raises(native.SignalAlignError):
+        gpu_ctx.align_batch(dm, [native.Read(ref1, r1["events"], r1["anchors"], r1["scale"], r1["shift"],
+                                             r1["var"])], params, native.SA_FP64)
