@@ -181,6 +181,67 @@ def expectations_leg(native, ctx, dm, model, reads, params, args):
     return out
 
 
+def degenerate_leg(native, ctx, dm, model, reads, params, args):
+    """Degenerate references (multi-path cells, FP64): the first --degenerate-reads
+    reads of the batched workload with the cytosine of every CpG (C or E) made 'X'
+    over the options "CE", in one batch.  GCUPS = band cells / device time, beside
+    the same reads without 'X' (FP64) and the FP64 oracle on a bounded sample (one
+    core)."""
+    import re
+
+    def xref(ref):
+        return re.sub(b"[CE](?=G)", b"X", ref if isinstance(ref, bytes) else ref.encode())
+
+    sub = reads[: args.degenerate_reads]
+    xreads = [native.Read(xref(r["ref"]), r["events"], r["anchors"], r["scale"], r["shift"], r["var"],
+                          degenerate_options="CE") for r in sub]
+    k = model.kmer_length
+    sites = sum(xref(r["ref"]).count(b"X") for r in sub)
+    paths, cols = 0, 0
+    for r in sub:
+        x = np.frombuffer(xref(r["ref"]), dtype=np.uint8) == ord("X")
+        c = np.convolve(x.astype(np.int64), np.ones(k, dtype=np.int64), "valid")
+        paths += int(np.sum(2 ** c))
+        cols += len(c)
+    b = native.Batch(ctx, dm, xreads, params)
+    b.run(native.SA_FP64, fetch=False)  # warm-up
+    tms = [b.run(native.SA_FP64, fetch=False)[1] for _ in range(2)]
+    b.close()
+    ms = float(np.mean([t["total_ms"] for t in tms]))
+    cells = tms[-1]["cells"]
+    bp = native.Batch(ctx, dm, make_native_reads(native, sub), params)
+    bp.run(native.SA_FP64, fetch=False)
+    tmp = [bp.run(native.SA_FP64, fetch=False)[1] for _ in range(2)]
+    bp.close()
+    pms = float(np.mean([t["total_ms"] for t in tmp]))
+    out = {"workload": f"degenerate references: the first {len(sub)} reads of the batched workload, every CpG "
+                       f"cytosine 'X' over \"CE\", FP64, band expansion {args.expansion}",
+           "reads": len(sub), "cells": cells, "x_sites": sites, "mean_paths_per_column": round(paths / cols, 3),
+           "gcups": round(cells / ms / 1e6, 3), "ms_per_step": round(ms, 3),
+           "forward_ms": round(float(np.mean([t["forward_ms"] for t in tms])), 3),
+           "backward_ms": round(float(np.mean([t["backward_ms"] for t in tms])), 3),
+           "backward_launches": int(tms[-1]["backward_launches"]),
+           "same_reads_without_x_fp64_gcups": round(tmp[-1]["cells"] / pms / 1e6, 3)}
+    if not args.no_cpu:
+        import oracle
+        from tests import _parity as P
+        oracle.build()
+        t0, ccells, n = time.time(), 0, 0
+        for r in sub:
+            if time.time() - t0 > 5.0:
+                break
+            om = P.oracle_model(model, r)
+            oracle.get_aligned_pairs(om, xref(r["ref"]), r["events"], r["anchors"],
+                                     oracle.Params(diagonal_expansion=args.expansion), options="CE")
+            lo, hi = oracle.band(r["anchors"], len(r["ref"]) - (k - 1), len(r["events"]), args.expansion)
+            ccells += int(np.sum((hi - lo) // 2 + 1))
+            n += 1
+        out["cpu_oracle_gcups_1core"] = round(ccells / (time.time() - t0) / 1e9, 4)
+        out["cpu_oracle_sample_reads"] = n
+    log(f"degenerate: {out}")
+    return out
+
+
 def serving_leg(native, synth, ctx, gmodel, args):
     """The serving configuration: Poisson arrivals of single-read requests at a fixed
     rate (a fraction --serve-load of the capacity a closed burst measures first),
@@ -341,6 +402,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving configuration")
     ap.add_argument("--no-expect", action="store_true", help="skip the EM expectations configuration")
     ap.add_argument("--expect-reads", type=int, default=512)
+    ap.add_argument("--no-degenerate", action="store_true", help="skip the degenerate-reference configuration")
+    ap.add_argument("--degenerate-reads", type=int, default=256)
     ap.add_argument("--serve-requests", type=int, default=100000)
     ap.add_argument("--serve-pool", type=int, default=1024)
     ap.add_argument("--serve-burst", type=int, default=8192)
@@ -483,6 +546,7 @@ def main():
         log(f"plain: {plain}")
 
     expect = None if args.no_expect else expectations_leg(native, ctx, dm, model, reads, params, args)
+    degenerate = None if args.no_degenerate else degenerate_leg(native, ctx, dm, model, reads, params, args)
     hdp = None if args.no_hdp else hdp_leg(native, synth, ctx, args)
     serving = None if args.no_serve else serving_leg(native, synth, ctx, model, args)
 
@@ -509,6 +573,7 @@ def main():
         "hdp": hdp,
         "serving": serving,
         "expectations": expect,
+        "degenerate": degenerate,
         "wall_ms_per_step_pcie_inclusive": round(float(np.mean(wall_ms)), 3),
     }
     if cpu:

@@ -35,16 +35,34 @@ __device__ __forceinline__ int ipow(int b, int e) {
     return r;
 }
 
-/* path q of column x-1 -> path p of column x (x >= 1) */
-__device__ __forceinline__ bool legal(const int32_t *SA, const int32_t *SB, int x, int q, int p, int B) {
-    if (x - 1 == 0) return true; /* column 0 has no k-mer */
-    const int lo = max(SA[x - 1], SA[x]), hi = min(SB[x - 1], SB[x]);
-    for (int s = lo; s < hi; s++) {
-        const int dq = (q / ipow(B, SB[x - 1] - 1 - s)) % B;
-        const int dp = (p / ipow(B, SB[x] - 1 - s)) % B;
-        if (dq != dp) return false;
-    }
-    return true;
+/* The legal predecessors of path p of column x (x >= 1) in column x-1, in
+ * increasing order: path_checkLegal (impl/pairwiseAligner.c:540-552) accepts q
+ * when the 'X' sites the two k-mers share take the same options.  Column x-1
+ * holds sites [sa1, sb1), column x sites [sa0, sb0), sa1 <= sa0 <= sb1 <= sb0; a
+ * path's digits are its sites' options, the first site most significant.  The
+ * shared sites [sa0, sb1) are p's leading digits, so the legal q are
+ * lead * B^(sb1 - sa0) + p / B^(sb0 - sb1) for every lead over the sites that leave,
+ * [sa1, sa0).  Column 0 has one path and every transition out of it is legal. */
+struct Range {
+    int first, step, n;
+};
+
+__device__ __forceinline__ Range preds(const int32_t *SA, const int32_t *SB, int x, int p, int B) {
+    if (x - 1 == 0) return Range{0, 1, 1};
+    const int sa1 = SA[x - 1], sb1 = SB[x - 1], sa0 = SA[x], sb0 = SB[x];
+    return Range{p / ipow(B, sb0 - sb1), ipow(B, sb1 - sa0), ipow(B, sa0 - sa1)};
+}
+
+/* The legal successors of path q of column x in column x+1, in increasing order:
+ * column x+1 holds sites [sa2, sb2); the shared sites [sa2, sb0) are q's trailing
+ * digits, so the legal p are (q mod B^(sb0 - sa2)) * B^(sb2 - sb0) + trail for every
+ * trail over the sites that enter, [sb0, sb2).  From column 0 every path of column 1. */
+__device__ __forceinline__ Range succs(const int32_t *SA, const int32_t *SB, const int32_t *pp, int x, int q,
+                                       int B) {
+    if (x == 0) return Range{0, 1, pp[2] - pp[1]};
+    const int sa2 = SA[x + 1], sb2 = SB[x + 1], sb0 = SB[x];
+    const int n = ipow(B, sb2 - sb0);
+    return Range{(q % ipow(B, sb0 - sa2)) * n, 1, n};
 }
 
 struct PathView {
@@ -107,11 +125,11 @@ __global__ void __launch_bounds__(256) paths_forward_kernel(Args<double> a, Path
                 X = Y = t.ragged_left ? 0.0 : NI;
             } else {
                 const int32_t kk = v.ekid[en];
+                const Range pr = x >= 1 ? preds(v.SA, v.SB, x, p, v.B) : Range{0, 1, 0};
                 if (x - 1 >= lo1 && x - 1 <= hi1) { /* lower -> gapX */
                     const double eX = Emit<double>::gapx(m, kk);
-                    for (int q = 0; q < v.pp[x] - v.pp[x - 1]; q++) {
-                        if (!legal(v.SA, v.SB, x, q, p, v.B)) continue;
-                        const int eq = (v.pp[x - 1] + q) & RM;
+                    for (int i = 0; i < pr.n; i++) {
+                        const int eq = (v.pp[x - 1] + pr.first + i * pr.step) & RM;
                         X = logadd(X, r1[eq] + (eX + tGOX));
                         X = logadd(X, r1[R + eq] + (eX + tGEX));
                         X = logadd(X, r1[2 * R + eq] + (eX + tSWX));
@@ -119,9 +137,8 @@ __global__ void __launch_bounds__(256) paths_forward_kernel(Args<double> a, Path
                 }
                 if (x - 1 >= lo2 && x - 1 <= hi2) { /* middle -> match (x >= 1, y >= 1) */
                     const double eM = Emit<double>::match(m, a, kk, evc + (y - 1), s, inv_var);
-                    for (int q = 0; q < v.pp[x] - v.pp[x - 1]; q++) {
-                        if (!legal(v.SA, v.SB, x, q, p, v.B)) continue;
-                        const int eq = (v.pp[x - 1] + q) & RM;
+                    for (int i = 0; i < pr.n; i++) {
+                        const int eq = (v.pp[x - 1] + pr.first + i * pr.step) & RM;
                         M = logadd(M, r2[eq] + (eM + tMC));
                         M = logadd(M, r2[R + eq] + (eM + tMFX));
                         M = logadd(M, r2[2 * R + eq] + (eM + tMFY));
@@ -205,10 +222,11 @@ __global__ void __launch_bounds__(256) paths_backward_kernel(Args<double> a, Pat
                 Y = e2v;
             } else {
                 M = X = Y = NI;
-                if (x + 1 >= lo2 && x + 1 <= hi2) { /* middle: cell (x+1, y+1) on e+2 */
-                    for (int p = 0; p < v.pp[x + 2] - v.pp[x + 1]; p++) {
-                        if (!legal(v.SA, v.SB, x + 1, q, p, v.B)) continue;
-                        const int ep = v.pp[x + 1] + p;
+                const bool hmid = x + 1 >= lo2 && x + 1 <= hi2, hlow = x + 1 >= lo1 && x + 1 <= hi1;
+                const Range sr = hmid || hlow ? succs(v.SA, v.SB, v.pp, x, q, v.B) : Range{0, 1, 0};
+                if (hmid) { /* middle: cell (x+1, y+1) on e+2 */
+                    for (int i = 0; i < sr.n; i++) {
+                        const int ep = v.pp[x + 1] + sr.first + i;
                         const double eM = Emit<double>::match(m, a, v.ekid[ep], evc + y, s, inv_var);
                         const double BM = r2[ep & RM];
                         M = logadd(M, BM + (eM + tMC));
@@ -222,10 +240,9 @@ __global__ void __launch_bounds__(256) paths_backward_kernel(Args<double> a, Pat
                     M = logadd(M, BY + (eY + tGOY));
                     Y = logadd(Y, BY + (eY + tGEY));
                 }
-                if (x + 1 >= lo1 && x + 1 <= hi1) { /* lower: cell (x+1, y) on e+1 */
-                    for (int p = 0; p < v.pp[x + 2] - v.pp[x + 1]; p++) {
-                        if (!legal(v.SA, v.SB, x + 1, q, p, v.B)) continue;
-                        const int ep = v.pp[x + 1] + p;
+                if (hlow) { /* lower: cell (x+1, y) on e+1 */
+                    for (int i = 0; i < sr.n; i++) {
+                        const int ep = v.pp[x + 1] + sr.first + i;
                         const double eX = Emit<double>::gapx(m, v.ekid[ep]);
                         const double BX = r1[R + (ep & RM)];
                         M = logadd(M, BX + (eX + tGOX));
@@ -277,9 +294,9 @@ __global__ void __launch_bounds__(256) paths_backward_kernel(Args<double> a, Pat
                             double Mv = NI;
                             if (hm) {
                                 const double eM = Emit<double>::match(m, a, v.ekid[er], evc + (y - 1), s, inv_var);
-                                for (int q = 0; q < v.pp[x] - v.pp[x - 1]; q++) {
-                                    if (!legal(v.SA, v.SB, x, q, r, v.B)) continue;
-                                    const int64_t fi = fbm + (v.pp[x - 1] + q - enm0);
+                                const Range pr = preds(v.SA, v.SB, x, r, v.B);
+                                for (int i = 0; i < pr.n; i++) {
+                                    const int64_t fi = fbm + (v.pp[x - 1] + pr.first + i * pr.step - enm0);
                                     Mv = logadd(Mv, a.fM[fi] + (eM + tMC));
                                     Mv = logadd(Mv, a.fX[fi] + (eM + tMFX));
                                     Mv = logadd(Mv, a.fY[fi] + (eM + tMFY));

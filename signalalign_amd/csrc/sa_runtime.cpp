@@ -1617,24 +1617,21 @@ int run_paths(sa_batch *b, sa_result *res) {
         planes += run;
     }
     const int ring = next_pow2((int)std::min<int64_t>(max_ent, 1 << 24));
-    /* windows and their pair slots: at most one pair per entry of a posterior diagonal */
+    /* windows and their pair slots: a first guess of a few pairs per posterior
+       diagonal (a diagonal's match posteriors over cells and paths sum to at most 1);
+       windows that overflow run again with their exact count */
     std::vector<WinDev> wins;
     std::vector<int32_t> task_first_win(nt);
     int64_t out_total = 0;
     for (int64_t ti = 0; ti < nt; ti++) {
         const TaskDev &t = b->tasks[ti];
         task_first_win[ti] = (int32_t)wins.size();
-        const int32_t *l = xlo.data() + t.diag_off, *h = xhi.data() + t.diag_off;
-        const int32_t *ppt = pp.data() + col_off[ti];
         for (int32_t w = 0; w < nwin[ti]; w++) {
             WinDev W = slots[t.win_off + w];
             W.task = (int32_t)ti;
-            int64_t cap = 0;
-            for (int64_t d = W.T + 1; d <= W.TF; d++) cap += ppt[h[d] + 1] - ppt[l[d]];
-            if (cap > INT32_MAX) return fail(ctx, SA_ERR_UNSUPPORTED, "window too large");
             W.out_off = out_total;
-            W.out_cap = (int32_t)cap;
-            out_total += cap;
+            W.out_cap = 3 * (W.TF - W.T) + 64;
+            out_total += W.out_cap;
             wins.push_back(W);
         }
     }
@@ -1703,6 +1700,39 @@ int run_paths(sa_batch *b, sa_result *res) {
         SA_HIP(ctx, hipMemcpyAsync(outv.data(), b->d_out.p, sizeof(int32_t) * 4 * out_total, hipMemcpyDeviceToHost,
                                    st));
     SA_HIP(ctx, hipStreamSynchronize(st));
+    /* overflowing windows: the backward sweep again for them alone (the forward
+       planes are kept), into slots of their exact size after the first pass's */
+    std::vector<int32_t> redo;
+    for (int64_t w = 0; w < nw; w++)
+        if (wcount[w] > wins[w].out_cap) redo.push_back((int32_t)w);
+    if (!redo.empty()) {
+        int64_t extra = 0;
+        for (int32_t w : redo) {
+            wins[w].out_off = extra;
+            wins[w].out_cap = wcount[w];
+            extra += wcount[w];
+        }
+        SA_HIP(ctx, b->d_packed.ensure(sizeof(int32_t) * 4 * extra));
+        if ((rc = upload(ctx, b->d_wins, wins)) || (rc = upload(ctx, b->d_win_order, redo))) return rc;
+        sa::Args<double> ab = a;
+        ab.wins = b->d_wins.as<WinDev>();
+        ab.win_order = b->d_win_order.as<int32_t>();
+        ab.out = b->d_packed.as<int32_t>();
+        ab.ring_g = bb > kLdsMax ? b->d_ring_g.as<unsigned char>() : nullptr;
+        ab.ring_g_stride = (int64_t)bs;
+        SA_HIP(ctx, sa::launch_paths_backward(ab, P, (int)redo.size(), st));
+        /* the timed span ends after the second pass (the host round trip included) */
+        SA_HIP(ctx, hipEventRecord(ctx->ev[4], st));
+        std::vector<int32_t> extra_v(4 * extra);
+        SA_HIP(ctx, hipMemcpyAsync(extra_v.data(), b->d_packed.p, sizeof(int32_t) * 4 * extra, hipMemcpyDeviceToHost,
+                                   st));
+        SA_HIP(ctx, hipStreamSynchronize(st));
+        /* the redone windows' pairs go to the end of the host copy */
+        for (int32_t w : redo) wins[w].out_off += out_total;
+        outv.resize(4 * (out_total + extra));
+        std::memcpy(outv.data() + 4 * out_total, extra_v.data(), sizeof(int32_t) * 4 * extra);
+        tm.backward_launches++;
+    }
     /* the reference's list order: per read, sub-alignments in split order, each
        sub-alignment's pairs in reverse emission order, shifted by (x1, y1) */
     for (int64_t r = 0; r < b->n_reads; r++) {
@@ -1741,7 +1771,7 @@ int run_paths(sa_batch *b, sa_result *res) {
     for (int64_t c : cells) tc += c;
     tm.cells = tc;
     tm.forward_launches = 1;
-    tm.backward_launches = nw ? 1 : 0;
+    tm.backward_launches += nw ? 1 : 0;
     return SA_OK;
 }
 

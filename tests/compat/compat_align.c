@@ -7,9 +7,11 @@
  *       getStateMachine3 + getAlignedPairsUsingAnchors (no anchors, expansion 2,
  *       threshold 0.2, not ragged): exactly the 7 pairs (0,0) (1,1) (2,2) (3,3)
  *       (5,4) (6,5) (8,6), each path k-mer equal to the reference k-mer.
- *   compat_align align <.model> <input.txt> <output.tsv>
+ *   compat_align align <.model> <input.txt> <output.tsv> [degenerate type]
  *       One read through getStateMachine3_descaled; writes the total
- *       log-likelihood and the pairs (p, x, y, path k-mer) in list order.
+ *       log-likelihood and the pairs (p, x, y, path k-mer) in list order.  The
+ *       reference Sequence takes the DegenerateType given (its numeric value,
+ *       default cytosineMethylation2), whose options an 'X' base stands for.
  *
  * Exit status 0 on success; the compat layer aborts on library errors.
  */
@@ -60,7 +62,7 @@ static int kat5(const char *model) {
     return ok ? 0 : 1;
 }
 
-static int align(const char *model, const char *in, const char *out) {
+static int align(const char *model, const char *in, const char *out, DegenerateType dtype) {
     FILE *f = fopen(in, "r");
     if (!f) return 2;
     long long nref = 0, nev = 0, nanc = 0, rl = 0, rr = 0, exp_ = 0, mind = 0, tb = 0, split = 0;
@@ -89,7 +91,7 @@ static int align(const char *model, const char *in, const char *out) {
     StateMachine *sM = getStateMachine3_descaled(model, npp, 0);
     int64_t lX = sequence_correctSeqLength(nref, kmer, sM->kmerLength);
     Sequence *SsX = sequence_constructReferenceKmerSequence(lX, ref, sequence_getKmer, sequence_sliceNucleotideSequence,
-                                                            cytosineMethylation2, kmer);
+                                                            dtype, kmer);
     Sequence *SsY = sequence_constructEventSequence(nev, ev);
     PairwiseAlignmentParameters *p = pairwiseAlignmentBandingParameters_construct();
     p->threshold = thr;
@@ -122,7 +124,9 @@ static int align(const char *model, const char *in, const char *out) {
 
 int main(int argc, char **argv) {
     if (argc == 3 && strcmp(argv[1], "kat5") == 0) return kat5(argv[2]);
-    if (argc == 5 && strcmp(argv[1], "align") == 0) return align(argv[2], argv[3], argv[4]);
-    fprintf(stderr, "usage: compat_align kat5 <model> | align <model> <input> <output>\n");
+    if (argc == 5 && strcmp(argv[1], "align") == 0) return align(argv[2], argv[3], argv[4], cytosineMethylation2);
+    if (argc == 6 && strcmp(argv[1], "align") == 0) /* the degenerate type of an 'X' reference */
+        return align(argv[2], argv[3], argv[4], (DegenerateType)atoi(argv[5]));
+    fprintf(stderr, "usage: compat_align kat5 <model> | align <model> <input> <output> [degenerate type]\n");
     return 2;
 }

@@ -201,3 +201,94 @@ def test_gpu_degenerate_batch_mixed_and_refusals(gpu_ctx, oracle_lib):
     with pytest.raises(native.SignalAlignError):
         gpu_ctx.align_batch(dm, [native.Read(ref1, r1["events"], r1["anchors"], r1["scale"], r1["shift"],
                                              r1["var"])], params, native.SA_FP64)
+
+
+# ---------------------------------------------------------------------------
+# the reference's known answer on a real read
+# ---------------------------------------------------------------------------
+DEGENERATE_PAIRS = 7349
+
+
+def _zymo_descaled(sub):
+    """tests/stateMachineTests.c:896-947 (test_DegenerateNucleotides): the Zymo read's
+    template events, the descaling machine, threshold 0.01, neither end ragged, and
+    ZymoRef with every C replaced by `sub` (degenerate bases "CEO")."""
+    from tests.test_real_read import _inputs, _machines
+    r, ref, m = _inputs()
+    name, mm, kind, read = [t for t in _machines(r, ref, m) if t[0] == "descaled"][0]
+    return mm, kind, dict(read, ref=ref.replace("C", sub).encode())
+
+
+def test_oracle_reproduces_the_reference_degenerate_count(oracle_lib):
+    """The reference expects 1076 pairs with C->E and C->O, and 7349 (four-tuples,
+    several paths per cell) with C->X.  The reference's run has lastz anchors, absent
+    here: without anchors the band is the whole matrix, which gives the same counts."""
+    from tests import _parity as P
+    for sub, want in [("E", 1076), ("O", 1076), ("X", DEGENERATE_PAIRS)]:
+        mm, kind, read = _zymo_descaled(sub)
+        pairs, _ = oracle.get_aligned_pairs(P.oracle_model(mm, read, oracle.GAUSS_DESCALED), read["ref"],
+                                            read["events"], read["anchors"], oracle.Params(0.01), False, False,
+                                            options=("CEO" if sub == "X" else None))
+        assert len(pairs) == want, (sub, len(pairs))
+        if sub == "X":
+            assert pairs.shape[1] == 4 and len({(x, y) for _, x, y, _ in pairs}) < len(pairs)
+
+
+@pytest.mark.gpu
+def test_gpu_reproduces_the_reference_degenerate_count(gpu_ctx, oracle_lib):
+    """The same run on the GPU: 7349 pairs, equal to the oracle's (pairs, paths,
+    order, posteriors) and its log-likelihood.  The full-matrix band with up to 3^6
+    paths per column puts the rings in HBM."""
+    from signalalign_amd import native
+    from tests import _parity as P
+    mm, kind, read = _zymo_descaled("X")
+    params = native.Params(threshold=0.01)
+    want, st = oracle.get_aligned_pairs(P.oracle_model(mm, read, oracle.GAUSS_DESCALED), read["ref"],
+                                        read["events"], read["anchors"], oracle.Params(0.01), False, False,
+                                        options="CEO")
+    dm = gpu_ctx.model(mm, kind)
+    rd = native.Read(read["ref"], read["events"], read["anchors"], read["scale"], read["shift"], read["var"],
+                     False, False, degenerate_options="CEO")
+    print("zymo degenerate", flush=True)
+    res, tm = gpu_ctx.align_batch(dm, [rd], params, native.SA_FP64)
+    out = _compare_paths(res[0].pairs, res[0].paths, want, params.threshold)
+    assert len(res[0].pairs) == DEGENERATE_PAIRS, out
+    assert P.rel(res[0].log_likelihood, st["log_likelihood"]) <= P.LL_REL
+    print(out, tm)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype,options", [(0, "CE"), (1, "CEO")])
+def test_compat_degenerate_alignment_matches_oracle(tmp_path, oracle_lib, dtype, options):
+    """A reference with 'X' through sequence_constructReferenceKmerSequence (the
+    DegenerateType's options) and getAlignedPairsUsingAnchors (C): the pairs, list
+    order, log-likelihood and the path k-mer of every pair (the 'X' resolved to its
+    path's option) equal the oracle's."""
+    import subprocess
+    from signalalign_amd import synth
+    from tests import _parity as P
+    from tests.test_compat import COMPAT_BIN, _write_input
+    from signalalign_amd import native
+    m = synth.gaussian_model("ACEGOT", 6, seed=4)
+    mpath = str(tmp_path / "g6.model")
+    models.write_model_file(m, mpath)
+    r = synth.single_read(m, 1500, seed=91)
+    r = dict(r, ref=_with_x(r["ref"].decode() if isinstance(r["ref"], bytes) else r["ref"]))
+    params = native.Params(diagonal_expansion=20, split_matrix_bigger_than_this=60 * 60)
+    inp, out = str(tmp_path / "in.txt"), str(tmp_path / "out.tsv")
+    _write_input(inp, r, params, (True, True))
+    print(f"compat degenerate {options}", flush=True)
+    res = subprocess.run([COMPAT_BIN, "align", mpath, inp, out, str(dtype)], capture_output=True, text=True,
+                         timeout=120)
+    assert res.returncode == 0, res.stderr
+    lines = open(out).read().splitlines()
+    ll = float(lines[0])
+    rows = [ln.split("\t") for ln in lines[1:]]
+    want, st = oracle.get_aligned_pairs(P.oracle_model(m, r), r["ref"], r["events"], r["anchors"],
+                                        oracle.Params(params.threshold, params.min_diags_between_traceback,
+                                                      params.traceback_diagonals, params.diagonal_expansion,
+                                                      params.split_matrix_bigger_than_this), options=options)
+    got_paths = [oracle.path_kmers(r["ref"][int(b):int(b) + 6], options).index(km) for _, b, _, km in rows]
+    _compare_paths([(int(a), int(b), int(c)) for a, b, c, _ in rows], got_paths, want, params.threshold)
+    assert P.rel(ll, st["log_likelihood"]) <= P.LL_REL
+    assert max(got_paths) > 0
