@@ -416,7 +416,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (reads shard across GPUs with no "
-              "collective; the multi-GPU schedule is described in DESIGN.md, not built)", file=sys.stderr)
+              "collective: each GPU would take whole reads, balanced by band cells; that schedule is not built)", file=sys.stderr)
         sys.exit(2)
 
     from signalalign_amd import native, synth

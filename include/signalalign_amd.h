@@ -121,7 +121,8 @@ typedef struct {
 } sa_pair;
 
 typedef struct {
-    const sa_pair *pairs;   /* in the reference's list order (see DESIGN/INTEGRATION docs) */
+    const sa_pair *pairs;   /* in the reference's list order: sub-alignments in split order, each
+                               sub-alignment's pairs in reverse emission order */
     int64_t n_pairs;
     double log_likelihood;  /* sum over sub-alignments of the total at the final diagonal */
     int64_t cells;          /* band cells = work units */
