@@ -105,7 +105,22 @@ struct Args {
  * index order = position order); per entry its column and the k-mer id of its
  * substituted k-mer; per diagonal the first cell of its entries in the forward
  * planes (all three states of every diagonal are kept). */
+/* Per diagonal of a degenerate task: its band [lo, hi], its entries [e0, e1)
+ * (task-relative), and the absolute offset of its entries in the forward planes. */
+struct PathDiag {
+    int32_t lo, hi, e0, e1;
+    int64_t foff;
+};
+/* Per entry (column x, path p) of a degenerate task: the path's k-mer id and its
+ * legal predecessors in column x-1, entries pfirst + i * pstep for i < pn
+ * (task-relative; pn = 0 in column 0). */
+struct PathEnt {
+    int32_t x, p, pfirst, pstep, pn, kk;
+};
 struct PathTab {
+    const PathDiag *diag;     /* diag_off + d */
+    const PathEnt *ent;       /* ent_off[task] + entry */
+    const int32_t *maxent;    /* per task: the most entries on one diagonal */
     const int64_t *col_off;   /* per task: first slot of its column arrays (lX + 2) */
     const int64_t *ent_off;   /* per task: first slot of its entry arrays */
     const int64_t *plane_off; /* per task: first cell of its forward planes */

@@ -87,6 +87,171 @@ __device__ __forceinline__ PathView view(const PathTab &P, const TaskDev &t, int
 
 } // namespace
 
+/* The inputs of an entry's emissions that depend only on its k-mer and event
+ * (Gaussian tables): loaded a diagonal before they are used. */
+struct RawEmit {
+    double gx, mmu, misd, mca, mnmu, mnb, ymu, yisd, yca, ynmu, ynb, evm, evn, evl, evi;
+};
+
+__device__ __forceinline__ PathEnt load_ent(const PathEnt *et, const PathDiag &D, int tid) {
+    if (tid < D.e1 - D.e0) return et[D.e0 + tid];
+    return PathEnt{0, 0, 0, 1, 0, 0};
+}
+
+__device__ __forceinline__ void load_raw(const Model<double> &m, const Args<double> &a, const PathEnt &E, bool act,
+                                         int64_t d, int64_t evc, RawEmit &r) {
+    const int kk = act ? E.kk : 0;
+    const int64_t y = d - E.x;
+    const int64_t j = evc + ((act && y >= 1) ? y - 1 : 0);
+    r.gx = m.gx[kk];
+    r.mmu = m.mmu[kk]; r.misd = m.misd[kk]; r.mca = m.mca[kk]; r.mnmu = m.mnmu[kk]; r.mnb = m.mnb[kk];
+    r.ymu = m.ymu[kk]; r.yisd = m.yisd[kk]; r.yca = m.yca[kk]; r.ynmu = m.ynmu[kk]; r.ynb = m.ynb[kk];
+    r.evm = a.evm[j]; r.evn = a.evn[j]; r.evl = a.evl[j]; r.evi = a.evi[j];
+}
+
+
+/* Diagonal d of the pipelined forward sweep.  Ec, Rc: this thread's entry on d and
+ * its emission inputs; En: its entry on d+1, whose emission inputs go to Rn; En3
+ * receives its entry on d+3 (the entry on d+2 is in flight in a fourth slot).  The
+ * diagonal records slide by one. */
+template <bool LDS>
+__device__ __forceinline__ void fwd_step(const Args<double> &a, const PathEnt *et, const PathDiag *dg, double *ring,
+                                         int R, int64_t N, int64_t evc, bool gauss, bool ragged_left, double s,
+                                         double inv_var, int64_t d, PathDiag &Dm2, PathDiag &Dm1, PathDiag &D0,
+                                         PathDiag &D1, PathDiag &D2, PathDiag &D3, const PathEnt &Ec,
+                                         const PathEnt &En, PathEnt &En3, const RawEmit &Rc, RawEmit &Rn) {
+    const Model<double> &m = a.m;
+    const int RM = R - 1, tid = threadIdx.x;
+    const double NI = -__builtin_huge_val();
+    const PathDiag EMPTY = {1, 0, 0, 0, 0};
+    const bool act = tid < D0.e1 - D0.e0;
+    /* next diagonals' inputs */
+    if (gauss) load_raw(m, a, En, tid < D1.e1 - D1.e0, d + 1, evc, Rn);
+    En3 = load_ent(et, D3, tid);
+    const PathDiag D4 = d + 4 <= N ? dg[d + 4] : EMPTY;
+    double *rc = ring + (d % 3) * 3 * R, *r1 = ring + ((d + 2) % 3) * 3 * R, *r2 = ring + ((d + 1) % 3) * 3 * R;
+    if (act) {
+        const double tMC = m.trans[0], tMFX = m.trans[1], tMFY = m.trans[2], tGOX = m.trans[3], tGOY = m.trans[4];
+        const double tGEX = m.trans[5], tGEY = m.trans[6], tSWX = m.trans[7];
+        const int x = Ec.x;
+        const int64_t y = d - x;
+        const int es = (D0.e0 + tid) & RM;
+        double M = NI, X = NI, Y = NI;
+        if (d == 0) {
+            /* start or ragged-start values (impl/stateMachine.c:1022-1031) */
+            M = ragged_left ? NI : 0.0;
+            X = Y = ragged_left ? 0.0 : NI;
+        } else {
+            double eX, eM = 0.0, eY = 0.0;
+            if (gauss) {
+                eX = Rc.gx;
+                if (y >= 1) {
+                    eM = gauss_emit(Rc.mmu, Rc.misd, Rc.mca, Rc.mnmu, Rc.mnb, Rc.evm, Rc.evn, Rc.evl, Rc.evi, s);
+                    eY = gauss_emit(Rc.ymu, Rc.yisd, Rc.yca, Rc.ynmu, Rc.ynb, Rc.evm, Rc.evn, Rc.evl, Rc.evi, s);
+                }
+            } else {
+                eX = Emit<double>::gapx(m, Ec.kk);
+                if (y >= 1) {
+                    eM = Emit<double>::match(m, a, Ec.kk, evc + (y - 1), s, inv_var);
+                    eY = eM;
+                }
+            }
+            /* lower -> gapX from d-1 and middle -> match from d-2 read the same
+               predecessor entries: one loop, the two chains independent (each keeps its
+               own order of log-adds), a chain whose neighbour is outside the band left
+               as it is */
+            const bool hL = x - 1 >= Dm1.lo && x - 1 <= Dm1.hi, hM = x - 1 >= Dm2.lo && x - 1 <= Dm2.hi;
+            const bool hU = x >= Dm1.lo && x <= Dm1.hi && x >= 1;
+            const double aX0 = eX + tGOX, aX1 = eX + tGEX, aX2 = eX + tSWX;
+            const double aM0 = eM + tMC, aM1 = eM + tMFX, aM2 = eM + tMFY;
+            { /* upper -> gapY, the same path (a third independent chain) */
+                double Yn = logadd(Y, r1[es] + (eY + tGOY));
+                Yn = logadd(Yn, r1[2 * R + es] + (eY + tGEY));
+                Y = hU ? Yn : Y;
+            }
+            if (hL || hM) {
+                for (int i = 0; i < Ec.pn; i++) {
+                    const int eq = (Ec.pfirst + i * Ec.pstep) & RM;
+                    double Xn = logadd(X, r1[eq] + aX0);
+                    double Mn = logadd(M, r2[eq] + aM0);
+                    Xn = logadd(Xn, r1[R + eq] + aX1);
+                    Mn = logadd(Mn, r2[R + eq] + aM1);
+                    Xn = logadd(Xn, r1[2 * R + eq] + aX2);
+                    Mn = logadd(Mn, r2[2 * R + eq] + aM2);
+                    X = hL ? Xn : X;
+                    M = hM ? Mn : M;
+                }
+            }
+        }
+        rc[es] = M;
+        rc[R + es] = X;
+        rc[2 * R + es] = Y;
+        a.fM[D0.foff + tid] = M;
+        a.fX[D0.foff + tid] = X;
+        a.fY[D0.foff + tid] = Y;
+    }
+    if constexpr (LDS) {
+        /* only the LDS ring is exchanged: the plane stores are read by the next
+           kernel, and the loads issued above stay in flight across the barrier */
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    } else {
+        __syncthreads();
+    }
+    Dm2 = Dm1;
+    Dm1 = D0;
+    D0 = D1;
+    D1 = D2;
+    D2 = D3;
+    D3 = D4;
+}
+
+/* The forward sweep of a task whose diagonals all fit the workgroup (one entry per
+ * thread): each step loads the emission inputs of the next diagonal and the entry
+ * records of the third one on, so a step waits on LDS and the barrier rather than on
+ * a chain of dependent loads.  Unrolled by four so that the four entry slots and the
+ * two emission-input slots trade roles without register moves (a move would wait for
+ * the loads in flight).  Same arithmetic, in the same order, as the strided loop. */
+template <bool LDS>
+__device__ __forceinline__ void forward_pipelined(const Args<double> &a, const PathTab &P, const TaskDev &t, int ti,
+                                                  double *ring, int R) {
+    const PathEnt *et = P.ent + P.ent_off[ti];
+    const PathDiag *dg = P.diag + t.diag_off;
+    const int tid = threadIdx.x;
+    const int64_t N = (int64_t)t.lX + t.lY;
+    const int64_t evc = t.evc_off;
+    const bool gauss = a.m.kind != 2;
+    const bool ragged_left = t.ragged_left != 0;
+    const double s = a.m.kind == 1 ? 1.0 : t.scale / t.var;
+    const double inv_var = 1.0 / t.var;
+    const PathDiag EMPTY = {1, 0, 0, 0, 0};
+    PathDiag Dm2 = EMPTY, Dm1 = EMPTY, D0 = dg[0];
+    PathDiag D1 = N >= 1 ? dg[1] : EMPTY, D2 = N >= 2 ? dg[2] : EMPTY, D3 = N >= 3 ? dg[3] : EMPTY;
+    PathEnt EA = load_ent(et, D0, tid), EB = load_ent(et, D1, tid), EC = load_ent(et, D2, tid);
+    PathEnt ED = {0, 0, 0, 1, 0, 0};
+    RawEmit RA = {}, RB = {};
+    if (gauss) load_raw(a.m, a, EA, tid < D0.e1 - D0.e0, 0, evc, RA);
+#define SA_FWD_STEP(dd, E0, E1, E3, R0, R1)                                                                         \
+    fwd_step<LDS>(a, et, dg, ring, R, N, evc, gauss, ragged_left, s, inv_var, dd, Dm2, Dm1, D0, D1, D2, D3, E0, E1, \
+                  E3, R0, R1)
+    int64_t d = 0;
+    for (; d + 3 <= N; d += 4) {
+        SA_FWD_STEP(d, EA, EB, ED, RA, RB);
+        SA_FWD_STEP(d + 1, EB, EC, EA, RB, RA);
+        SA_FWD_STEP(d + 2, EC, ED, EB, RA, RB);
+        SA_FWD_STEP(d + 3, ED, EA, EC, RB, RA);
+    }
+    for (; d <= N; d++) {
+        SA_FWD_STEP(d, EA, EB, ED, RA, RB);
+        EA = EB;
+        EB = EC;
+        EC = ED;
+        RA = RB;
+    }
+#undef SA_FWD_STEP
+}
+
 __global__ void __launch_bounds__(256) paths_forward_kernel(Args<double> a, PathTab P) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int R = P.ring, RM = R - 1;
@@ -106,6 +271,13 @@ __global__ void __launch_bounds__(256) paths_forward_kernel(Args<double> a, Path
     const double tGEX = m.trans[5], tGEY = m.trans[6], tSWX = m.trans[7];
     const double NI = -__builtin_huge_val();
 
+    if (P.maxent[ti] <= NT) {
+        /* an LDS ring is addressed as LDS (a pointer that may be either is a flat one) */
+        if (a.ring_g) forward_pipelined<false>(a, P, t, ti, ring, R);
+        else forward_pipelined<true>(a, P, t, ti, reinterpret_cast<double *>(smem), R);
+        return;
+    }
+    /* diagonals wider than the workgroup: entries strided over the threads */
     for (int64_t d = 0; d <= N; d++) {
         const int lo = xlo[d], hi = xhi[d];
         const int e0 = v.pp[lo], e1 = v.pp[hi + 1];

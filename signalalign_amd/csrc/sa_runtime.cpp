@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -23,6 +24,7 @@
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 using sa::TaskDev;
@@ -177,7 +179,7 @@ struct sa_batch {
     int32_t paths = 0;
     std::string options;      /* the bases an 'X' stands for */
     std::string href;         /* the packed reference bytes (host copy, path tables) */
-    DBuf d_pcol, d_pent, d_pplane, d_pp, d_psa, d_psb, d_pecol, d_pekid, d_pdoff;
+    DBuf d_pcol, d_pent, d_pplane, d_pp, d_psa, d_psb, d_pecol, d_pekid, d_pdoff, d_pdiag, d_pentr, d_pmaxent;
     /* EM expectations run (sa_expect): FP64 LDS sweeps with the accumulators */
     int32_t expect = 0;
     double ex_rec_threshold = 1.0;
@@ -208,7 +210,8 @@ struct sa_batch {
                        &d_task_nwin, &d_win_slots, &d_err, &d_fM, &d_fX, &d_fY, &d_wins, &d_task_order,
                        &d_win_order, &d_out, &d_win_count, &d_win_total, &d_win_dst, &d_packed, &d_ring_g,
                        &d_ex_trans, &d_ex_s0, &d_ex_s1, &d_ex_obs, &d_ex_rec, &d_ex_count,
-                       &d_pcol, &d_pent, &d_pplane, &d_pp, &d_psa, &d_psb, &d_pecol, &d_pekid, &d_pdoff};
+                       &d_pcol, &d_pent, &d_pplane, &d_pp, &d_psa, &d_psb, &d_pecol, &d_pekid, &d_pdoff,
+                       &d_pdiag, &d_pentr, &d_pmaxent};
         for (DBuf *b : all) b->release();
     }
 };
@@ -1549,20 +1552,35 @@ int run_paths(sa_batch *b, sa_result *res) {
     if (err & sa::ERR_BAND) return fail(ctx, SA_ERR_ANCHORS, "invalid band (anchors outside the matrix or out of order)");
     if (err & sa::ERR_WINDOWS) return fail(ctx, SA_ERR_HIP, "window schedule overflow");
 
-    /* path tables */
-    std::vector<int64_t> col_off(nt), ent_off(nt), plane_off(nt);
-    std::vector<int32_t> pp, psa, psb, ecol, ekid;
+    /* path tables, one task per worker thread: per column its first entry and site
+       range, per entry its column, k-mer id and legal predecessors, per diagonal its
+       band, entries and plane offset */
+    struct TaskTab {
+        std::vector<int32_t> pp, sa, sb, ecol, ekid;
+        std::vector<sa::PathEnt> ent;
+        int64_t planes = 0;
+        int32_t max_ent = 0;
+        int err = 0;
+    };
+    std::vector<TaskTab> tt(nt);
     std::vector<int64_t> doffE(b->total_diag);
-    int64_t planes = 0, max_ent = 1;
-    for (int64_t ti = 0; ti < nt; ti++) {
+    std::vector<sa::PathDiag> pdiag(b->total_diag);
+    auto ipow = [](int64_t base, int e) {
+        int64_t r = 1;
+        for (int i = 0; i < e; i++) r *= base;
+        return r;
+    };
+    auto build_task = [&](int64_t ti) {
+        TaskTab &T = tt[ti];
         const TaskDev &t = b->tasks[ti];
         const char *ref = b->href.data() + t.ref_off;
         const int64_t lX = t.lX;
-        col_off[ti] = (int64_t)pp.size();
-        ent_off[ti] = (int64_t)ecol.size();
         std::vector<int32_t> site; /* positions of 'X' in this sub-alignment's bases */
         for (int64_t i = 0; i < lX + k - 1; i++)
             if (ref[i] == 'X') site.push_back((int32_t)i);
+        T.pp.reserve(lX + 2);
+        T.sa.reserve(lX + 2);
+        T.sb.reserve(lX + 2);
         int32_t P = 0;
         size_t s0 = 0, s1 = 0;
         for (int64_t x = 0; x <= lX; x++) {
@@ -1570,16 +1588,19 @@ int run_paths(sa_batch *b, sa_result *res) {
             if (x >= 1) {
                 while (s0 < site.size() && site[s0] < x - 1) s0++;
                 while (s1 < site.size() && site[s1] < x - 1 + k) s1++;
-                for (size_t j = s0; j < s1; j++) np *= B;
+                np = ipow(B, (int)(s1 - s0));
             }
-            if (np > 729 || P + np > INT32_MAX / 2)
-                return fail(ctx, SA_ERR_UNSUPPORTED, "too many paths in a degenerate reference");
-            pp.push_back(P);
-            psa.push_back(x >= 1 ? (int32_t)s0 : 0);
-            psb.push_back(x >= 1 ? (int32_t)s1 : 0);
+            if (np > 729 || P + np > INT32_MAX / 2) {
+                T.err = 1;
+                return;
+            }
+            T.pp.push_back(P);
+            T.sa.push_back(x >= 1 ? (int32_t)s0 : 0);
+            T.sb.push_back(x >= 1 ? (int32_t)s1 : 0);
             for (int64_t q = 0; q < np; q++) {
-                ecol.push_back((int32_t)x);
+                T.ecol.push_back((int32_t)x);
                 int32_t id = 0;
+                sa::PathEnt e{(int32_t)x, (int32_t)q, 0, 1, 0, 0};
                 if (x >= 1) {
                     /* the path's k-mer: the first 'X' varies slowest */
                     char km[16];
@@ -1592,30 +1613,94 @@ int run_paths(sa_batch *b, sa_result *res) {
                         }
                     for (int j = 0; j < k; j++) {
                         const int c = model->lut[(uint8_t)km[j]];
-                        if (c < 0) return fail(ctx, SA_ERR_BAD_KMER, "degenerate option outside the alphabet");
+                        if (c < 0) {
+                            T.err = 2;
+                            return;
+                        }
                         id = id * A + c;
                     }
+                    /* legal predecessors in column x-1: the shared sites [sa0, sb1) are
+                       q's leading digits; every option of the sites leaving, [sa1, sa0) */
+                    if (x == 1) {
+                        e.pfirst = 0;
+                        e.pstep = 1;
+                        e.pn = 1;
+                    } else {
+                        const int sa1 = T.sa[x - 1], sb1 = T.sb[x - 1], sa0 = (int)s0, sb0 = (int)s1;
+                        e.pfirst = T.pp[x - 1] + (int32_t)(q / ipow(B, sb0 - sb1));
+                        e.pstep = (int32_t)ipow(B, sb1 - sa0);
+                        e.pn = (int32_t)ipow(B, sa0 - sa1);
+                    }
                 }
-                ekid.push_back(id);
+                e.kk = id;
+                T.ekid.push_back(id);
+                T.ent.push_back(e);
             }
             P += (int32_t)np;
         }
-        pp.push_back(P);
-        psa.push_back(0);
-        psb.push_back(0);
-        plane_off[ti] = planes;
+        T.pp.push_back(P);
+        T.sa.push_back(0);
+        T.sb.push_back(0);
         const int64_t N = (int64_t)t.lX + t.lY;
         const int32_t *l = xlo.data() + t.diag_off, *h = xhi.data() + t.diag_off;
-        const int32_t *ppt = pp.data() + col_off[ti];
         int64_t run = 0;
         for (int64_t d = 0; d <= N; d++) {
+            const int32_t e0 = T.pp[l[d]], e1 = T.pp[h[d] + 1];
             doffE[t.diag_off + d] = run;
-            const int64_t ne = ppt[h[d] + 1] - ppt[l[d]];
-            run += ne;
-            max_ent = std::max(max_ent, ne);
+            pdiag[t.diag_off + d] = sa::PathDiag{l[d], h[d], e0, e1, run}; /* + the task's plane offset below */
+            run += e1 - e0;
+            T.max_ent = std::max<int32_t>(T.max_ent, e1 - e0);
         }
-        planes += run;
+        T.planes = run;
+    };
+    const int nthreads = (int)std::max<int64_t>(1, std::min<int64_t>(nt, 8));
+    auto parallel = [&](auto &&fn) {
+        std::atomic<int64_t> next{0};
+        auto worker = [&]() {
+            for (int64_t ti; (ti = next.fetch_add(1)) < nt;) fn(ti);
+        };
+        std::vector<std::thread> th;
+        for (int i = 1; i < nthreads; i++) th.emplace_back(worker);
+        worker();
+        for (auto &x : th) x.join();
+    };
+    parallel(build_task);
+    for (int64_t ti = 0; ti < nt; ti++) {
+        if (tt[ti].err == 1) return fail(ctx, SA_ERR_UNSUPPORTED, "too many paths in a degenerate reference");
+        if (tt[ti].err == 2) return fail(ctx, SA_ERR_BAD_KMER, "degenerate option outside the alphabet");
     }
+    std::vector<int64_t> col_off(nt), ent_off(nt), plane_off(nt);
+    std::vector<int32_t> maxent(nt);
+    int64_t ncol = 0, nent = 0, planes = 0, max_ent = 1;
+    for (int64_t ti = 0; ti < nt; ti++) {
+        col_off[ti] = ncol;
+        ent_off[ti] = nent;
+        plane_off[ti] = planes;
+        ncol += (int64_t)tt[ti].pp.size();
+        nent += (int64_t)tt[ti].ecol.size();
+        planes += tt[ti].planes;
+        maxent[ti] = tt[ti].max_ent;
+        max_ent = std::max<int64_t>(max_ent, tt[ti].max_ent);
+    }
+    std::vector<int32_t> pp(ncol), psa(ncol), psb(ncol), ecol(nent), ekid(nent);
+    std::vector<sa::PathEnt> pent(nent);
+    parallel([&](int64_t ti) {
+        TaskTab &T = tt[ti];
+        const TaskDev &t = b->tasks[ti];
+        std::copy(T.pp.begin(), T.pp.end(), pp.begin() + col_off[ti]);
+        std::copy(T.sa.begin(), T.sa.end(), psa.begin() + col_off[ti]);
+        std::copy(T.sb.begin(), T.sb.end(), psb.begin() + col_off[ti]);
+        std::copy(T.ecol.begin(), T.ecol.end(), ecol.begin() + ent_off[ti]);
+        std::copy(T.ekid.begin(), T.ekid.end(), ekid.begin() + ent_off[ti]);
+        std::copy(T.ent.begin(), T.ent.end(), pent.begin() + ent_off[ti]);
+        for (int64_t d = 0; d <= (int64_t)t.lX + t.lY; d++) pdiag[t.diag_off + d].foff += plane_off[ti];
+        std::vector<int32_t>().swap(T.pp); /* release the task's copy */
+        std::vector<int32_t>().swap(T.sa);
+        std::vector<int32_t>().swap(T.sb);
+        std::vector<int32_t>().swap(T.ecol);
+        std::vector<int32_t>().swap(T.ekid);
+        std::vector<sa::PathEnt>().swap(T.ent);
+    });
     const int ring = next_pow2((int)std::min<int64_t>(max_ent, 1 << 24));
     /* windows and their pair slots: a first guess of a few pairs per posterior
        diagonal (a diagonal's match posteriors over cells and paths sum to at most 1);
@@ -1643,7 +1728,9 @@ int run_paths(sa_batch *b, sa_result *res) {
         (rc = upload(ctx, b->d_pplane, plane_off)) || (rc = upload(ctx, b->d_pp, pp)) ||
         (rc = upload(ctx, b->d_psa, psa)) || (rc = upload(ctx, b->d_psb, psb)) ||
         (rc = upload(ctx, b->d_pecol, ecol)) || (rc = upload(ctx, b->d_pekid, ekid)) ||
-        (rc = upload(ctx, b->d_pdoff, doffE)) || (rc = upload(ctx, b->d_wins, wins)) ||
+        (rc = upload(ctx, b->d_pdoff, doffE)) || (rc = upload(ctx, b->d_pdiag, pdiag)) ||
+        (rc = upload(ctx, b->d_pentr, pent)) || (rc = upload(ctx, b->d_pmaxent, maxent)) ||
+        (rc = upload(ctx, b->d_wins, wins)) ||
         (rc = upload(ctx, b->d_win_order, win_order)))
         return rc;
     SA_HIP(ctx, b->d_fM.ensure(sizeof(double) * std::max<int64_t>(planes, 1)));
@@ -1653,6 +1740,9 @@ int run_paths(sa_batch *b, sa_result *res) {
     SA_HIP(ctx, b->d_win_count.ensure(sizeof(int32_t) * std::max<int64_t>(nw, 1)));
     SA_HIP(ctx, b->d_win_total.ensure(sizeof(double) * std::max<int64_t>(nw, 1)));
     sa::PathTab P;
+    P.diag = b->d_pdiag.as<sa::PathDiag>();
+    P.ent = b->d_pentr.as<sa::PathEnt>();
+    P.maxent = b->d_pmaxent.as<int32_t>();
     P.col_off = b->d_pcol.as<int64_t>();
     P.ent_off = b->d_pent.as<int64_t>();
     P.plane_off = b->d_pplane.as<int64_t>();
