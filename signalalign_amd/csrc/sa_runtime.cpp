@@ -180,7 +180,7 @@ struct sa_batch {
     std::string options;      /* the bases an 'X' stands for */
     std::string href;         /* the packed reference bytes (host copy, path tables) */
     DBuf d_pcol, d_pent, d_pplane, d_pp, d_psa, d_psb, d_pecol, d_pekid, d_pdoff, d_pdiag, d_pentr, d_pmaxent;
-    /* EM expectations run (sa_expect): FP64 LDS sweeps with the accumulators */
+    /* EM expectations run (sa_expect): FP64, register forward sweeps and LDS backward sweeps with the accumulators */
     int32_t expect = 0;
     double ex_rec_threshold = 1.0;
     DBuf d_ex_trans, d_ex_s0, d_ex_s1, d_ex_obs, d_ex_rec, d_ex_count;
@@ -848,7 +848,9 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     const sa_model *model = b->model;
     const bool expect = b->expect != 0;
     if (expect && sizeof(T) != 8) return fail(ctx, SA_ERR_UNSUPPORTED, "expectations run in FP64");
-    force_lds = force_lds || expect;
+    /* an expectations run takes the register forward sweeps (every diagonal's three
+       states stored) and the LDS backward sweeps, which carry the accumulators */
+    const bool lds_backward = force_lds || expect;
     hipStream_t st = ctx->stream;
     const int64_t nt = (int64_t)b->tasks.size();
     sa_timing &tm = res->timing;
@@ -1019,7 +1021,7 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
             if (mw >= 0 && nd[i] >= long_n) f = 7 + mw;
         }
         fgrp[i] = f;
-        bgrp[i] = bb;
+        bgrp[i] = lds_backward ? 0 : bb;
     }
     /* backward group per traceback window: the narrowest register window that holds
        the band over the window's own diagonals (LDS sub-alignments stay on LDS) */
@@ -1419,8 +1421,9 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
        replay it without the host round trips between the steps */
     auto &gc = b->gc[sizeof(T) == 4 ? 1 : 0];
     gc.reset();
-    /* only a batch that is run again is worth recording (sa_align runs once) */
-    if (b->tune.use_graph && !force_lds && redo.empty() && b->nruns[sizeof(T) == 4 ? 1 : 0] >= 1) {
+    /* only a batch that is run again is worth recording (sa_align runs once); an
+       expectations run reads its accumulators back after each run, so it is not */
+    if (b->tune.use_graph && !force_lds && !expect && redo.empty() && b->nruns[sizeof(T) == 4 ? 1 : 0] >= 1) {
         hipGraph_t g = nullptr;
         bool ok = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) == hipSuccess;
         if (ok) {

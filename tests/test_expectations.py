@@ -203,15 +203,19 @@ def _gpu_vs_oracle(gpu_ctx, m, read, params, kind=None):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("kernels", ["register", "lds"])
 @pytest.mark.parametrize("n,expansion,seed", [(300, 20, 1), (2500, 50, 2), (4000, 20, 3)])
-def test_gpu_expectations_match_oracle(gpu_ctx, oracle_lib, n, expansion, seed):
+def test_gpu_expectations_match_oracle(gpu_ctx, oracle_lib, monkeypatch, n, expansion, seed, kernels):
     """Synthetic reads on a 6-mer Gaussian model (descaled emissions), one to several
     traceback windows: transition and k-mer expectations, likelihood, observed
-    k-mers and the ordered match records equal the oracle's (FP64)."""
+    k-mers and the ordered match records equal the oracle's (FP64), with the forward
+    pass on the register sweeps (the default) and on the LDS sweeps."""
     from signalalign_amd import native, synth
+    if kernels == "lds":
+        monkeypatch.setenv("SA_KERNELS", "lds")
     m = models.synthetic_gaussian_model("ACGT", 6, seed=7)
     r = synth.single_read(m, n, seed=seed)
-    print(f"expect n={n} e={expansion}", flush=True)
+    print(f"expect n={n} e={expansion} {kernels}", flush=True)
     got, want = _gpu_vs_oracle(gpu_ctx, m, r, native.Params(diagonal_expansion=expansion))
     print(len(got["records"]), got["likelihood"])
 
