@@ -486,3 +486,71 @@ def test_signal_machine_hdp_expectations(tmp_path, oracle_lib):
     assert km == wkm and len(km) > 100
     assert np.allclose(ev, wev, rtol=0, atol=1e-6)
     print("assignments", len(km))
+
+
+@pytest.mark.gpu
+def test_signal_machine_degenerate_variant_calls(tmp_path, oracle_lib):
+    """signalMachine on the Zymo read (2D) against ZymoRef with the cytosine of every
+    CpG made 'X' (forward and backward reference files), --degenerate 1 (C, E, O):
+    the full output's pairs, posteriors and path k-mers equal the oracle's degenerate
+    run on the same anchors and parameters, in the stable x+y order of its list; the
+    variantCaller output is the reference writer's rows for the 'X' positions of those
+    pairs (writePosteriorProbsVC, signalMachine.c:135-199: the called base is the path
+    k-mer's option at the 'X', at the reference position of the 'X')."""
+    import re
+    line = _guide_cigar()
+    ref = re.sub("C(?=G)", "X", open(REF).read().split()[0])
+    fwd = tmp_path / "fwd.txt"
+    fwd.write_text(ref + "\n")
+    bwd = tmp_path / "bwd.txt"
+    bwd.write_text(_complement(ref) + "\n")
+    rd = formats.load_npread(NPREAD)
+    prm = _params()
+    pa = D.parse_cigar(line)
+    outs = {}
+    for fmt in (0, 1):
+        out = tmp_path / f"out{fmt}.tsv"
+        print(f"signalMachine degenerate -s {fmt}", flush=True)
+        _run([MACHINE, "--twoD", "-s", str(fmt), "-o", "1", "-T", TMODEL, "-C", CMODEL, "-f", str(fwd), "-b",
+              str(bwd), "-q", NPREAD, "-u", str(out), "-L", "zymo_read"], stdin=line + "\n")
+        outs[fmt] = [l.split("\t") for l in out.read_text().split("\n")[:-1]]
+    rows = outs[0]
+    k = 6
+    n_x = 0
+    for strand in ("t", "c"):
+        m, read, e0, target = _oracle_inputs(strand, pa, prm[strand], rd, ref)
+        want, _ = oracle.get_aligned_pairs(P.oracle_model(m, read, oracle.GAUSS_DESCALED), read["ref"],
+                                           read["events"], read["anchors"],
+                                           oracle.Params(threshold=0.01, diagonal_expansion=50), options="CEO")
+        L = len(target)
+        wl = []
+        for p, x, y, path in want:
+            xa = x + pa["start1"] if strand == "t" else (L - k) - (x + (L - pa["end1"]))
+            wl.append((int(xa), int(y) + e0, oracle.path_kmers(target[int(x):int(x) + k], "CEO")[int(path)],
+                       int(p) / 1e7))
+        gl = [(int(t[1]), int(t[5]), t[15], float(t[12])) for t in rows if t[4] == strand]
+        g = {q[:3]: q[3] for q in gl}
+        w = {q[:3]: q[3] for q in wl}
+        assert len(g) == len(gl)
+        only_g = [q for q in g if q not in w and abs(g[q] - 0.01) > 1e-3]
+        only_w = [q for q in w if q not in g and abs(w[q] - 0.01) > 1e-3]
+        assert not only_g and not only_w, (strand, only_g[:5], only_w[:5])
+        common = [q for q in g if q in w]
+        dmax = max(abs(g[q] - w[q]) for q in common)
+        assert dmax <= 1e-4 + 1e-6, dmax  # the TSV prints six decimals
+        keyf = (lambda q: q[0] + q[1]) if strand == "t" else (lambda q: -q[0] + q[1])
+        cs = set(common)
+        assert [q[:3] for q in gl if q[:3] in cs] == [q[:3] for q in sorted(wl, key=keyf) if q[:3] in cs]
+        n_x += sum("X" in t[2] for t in rows if t[4] == strand)
+        print(strand, len(gl), len(wl), dmax)
+    assert n_x > 0
+    # variantCaller rows from the full rows, in order
+    vc = []
+    for t in rows:
+        refk, pathk, strand = t[2], t[15], t[4]
+        for q in range(k):
+            if refk[q] == "X":
+                base = pathk[q] if strand == "t" else pathk[k - 1 - q]
+                vc.append([t[5], str(int(t[1]) + q), base, t[12], strand, "forward", "zymo_read"])
+    assert outs[1] == vc
+    assert {v[2] for v in vc} <= set("CEO") and len({v[2] for v in vc}) > 1
