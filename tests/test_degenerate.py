@@ -292,3 +292,28 @@ def test_compat_degenerate_alignment_matches_oracle(tmp_path, oracle_lib, dtype,
     _compare_paths([(int(a), int(b), int(c)) for a, b, c, _ in rows], got_paths, want, params.threshold)
     assert P.rel(ll, st["log_likelihood"]) <= P.LL_REL
     assert max(got_paths) > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,options,expansion", [(900, "CE", 20), (2000, "EC", 20)])
+def test_gpu_degenerate_hdp_reads_match_oracle(gpu_ctx, oracle_lib, n, options, expansion):
+    """Degenerate references on an HDP density grid (the methylation-calling setup:
+    ACEGT 5-mers, every CpG cytosine 'X'): pairs, paths, posteriors, order and
+    log-likelihood equal the oracle's (FP64)."""
+    from signalalign_amd import native, synth
+    from tests import _parity as P
+    h = models.synthetic_hdp_model("ACEGT", 5, seed=2)
+    r = synth.single_read(h, n, seed=31 + n)
+    r = dict(r, ref=_with_x(r["ref"].decode() if isinstance(r["ref"], bytes) else r["ref"]))
+    params = native.Params(diagonal_expansion=expansion)
+    want, st = oracle.get_aligned_pairs(P.oracle_model(h, r), r["ref"], r["events"], r["anchors"],
+                                        oracle.Params(diagonal_expansion=expansion), options=options)
+    dm = gpu_ctx.model(h)
+    read = native.Read(r["ref"], r["events"], r["anchors"], r["scale"], r["shift"], r["var"],
+                       degenerate_options=options)
+    print(f"degenerate hdp n={n} {options}", flush=True)
+    res, _ = gpu_ctx.align_batch(dm, [read], params, native.SA_FP64)
+    out = _compare_paths(res[0].pairs, res[0].paths, want, params.threshold)
+    assert P.rel(res[0].log_likelihood, st["log_likelihood"]) <= P.LL_REL
+    assert int(np.max(want[:, 3])) > 0
+    print(out)
