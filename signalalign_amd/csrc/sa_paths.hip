@@ -342,10 +342,13 @@ __device__ __forceinline__ double chain_paths(const double *term, int n) {
     return acc;
 }
 
+/* LDS: the ring and term rows are in LDS (addressed as LDS; a pointer that may be
+ * either is a flat one), else in this workgroup's slice of HBM. */
+template <bool LDS>
 __global__ void __launch_bounds__(256) paths_backward_kernel(Args<double> a, PathTab P) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int R = P.ring, RM = R - 1;
-    unsigned char *mem = a.ring_g ? a.ring_g + (size_t)blockIdx.x * a.ring_g_stride : smem;
+    unsigned char *mem = LDS ? smem : a.ring_g + (size_t)blockIdx.x * a.ring_g_stride;
     double *termA = reinterpret_cast<double *>(mem); /* R (cells of a diagonal) */
     double *termB = termA + R;                       /* R */
     double *bcast = termB + R;                       /* 2 */
@@ -543,7 +546,8 @@ hipError_t launch_paths_forward(const Args<double> &a, const PathTab &P, int n_t
 
 hipError_t launch_paths_backward(const Args<double> &a, const PathTab &P, int n_wins, hipStream_t st) {
     const size_t lds = a.ring_g ? 0 : paths_backward_bytes(P.ring);
-    hipLaunchKernelGGL(paths_backward_kernel, dim3(n_wins), dim3(256), lds, st, a, P);
+    if (a.ring_g) hipLaunchKernelGGL(paths_backward_kernel<false>, dim3(n_wins), dim3(256), 0, st, a, P);
+    else hipLaunchKernelGGL(paths_backward_kernel<true>, dim3(n_wins), dim3(256), lds, st, a, P);
     return hipGetLastError();
 }
 
