@@ -191,13 +191,14 @@ __global__ void __launch_bounds__(256) prep_kernel(Args<T> a) {
 /* forward                                                                   */
 /* ------------------------------------------------------------------------ */
 
-template <typename T, int NT>
+template <typename T, int NT, bool LDSR>
 __global__ void __launch_bounds__(NT) forward_kernel(Args<T> a) {
     constexpr bool NORM = sizeof(T) == 4;
     extern __shared__ __align__(16) unsigned char smem[];
     const int R = a.ring, RM = R - 1;
-    /* the ring in LDS, or (bands too wide for LDS) in this workgroup's HBM slice */
-    unsigned char *mem = a.ring_g ? a.ring_g + (size_t)blockIdx.x * a.ring_g_stride : smem;
+    /* the ring in LDS (LDSR: addressed as LDS, which a pointer that may be either is
+       not), or (bands too wide for LDS) in this workgroup's HBM slice */
+    unsigned char *mem = LDSR ? smem : a.ring_g + (size_t)blockIdx.x * a.ring_g_stride;
     T *ring = reinterpret_cast<T *>(mem);                /* [3 diag][3 state][R] */
     T *red = ring + 9 * R;                              /* NT/64 wave maxima */
     const int ti = a.task_order[blockIdx.x];
@@ -451,14 +452,14 @@ __device__ __forceinline__ void expect_diagonal(const Args<double> &a, const Tas
     }
 }
 
-template <typename T, bool EX>
+template <typename T, bool EX, bool LDSR>
 __global__ void __launch_bounds__(64) backward_kernel(Args<T> a) {
     constexpr bool NORM = sizeof(T) == 4;
     extern __shared__ __align__(16) unsigned char smem[];
     const int R = a.ring, RM = R - 1;
-    /* the ring and the refresh terms in LDS, or (bands too wide for LDS) in this
-       workgroup's HBM slice */
-    unsigned char *mem = a.ring_g ? a.ring_g + (size_t)blockIdx.x * a.ring_g_stride : smem;
+    /* the ring and the refresh terms in LDS (LDSR: addressed as LDS), or (bands too
+       wide for LDS) in this workgroup's HBM slice */
+    unsigned char *mem = LDSR ? smem : a.ring_g + (size_t)blockIdx.x * a.ring_g_stride;
     double *termA = reinterpret_cast<double *>(mem);    /* R */
     double *termB = termA + R;                          /* R */
     double *bcast = termB + R;                          /* 2 */
@@ -695,8 +696,11 @@ hipError_t launch_prep(const Args<T> &a, int n_tasks, hipStream_t st) {
 template <typename T>
 hipError_t launch_forward(const Args<T> &a, int n_tasks, int nt, hipStream_t st) {
     const size_t lds = a.ring_g ? 0 : forward_block_bytes<T>(a.ring);
-    if (nt == 64) hipLaunchKernelGGL((forward_kernel<T, 64>), dim3(n_tasks), dim3(64), lds, st, a);
-    else if (nt == 256) hipLaunchKernelGGL((forward_kernel<T, 256>), dim3(n_tasks), dim3(256), lds, st, a);
+    const bool g = a.ring_g != nullptr;
+    if (nt == 64 && !g) hipLaunchKernelGGL((forward_kernel<T, 64, true>), dim3(n_tasks), dim3(64), lds, st, a);
+    else if (nt == 64) hipLaunchKernelGGL((forward_kernel<T, 64, false>), dim3(n_tasks), dim3(64), 0, st, a);
+    else if (nt == 256 && !g) hipLaunchKernelGGL((forward_kernel<T, 256, true>), dim3(n_tasks), dim3(256), lds, st, a);
+    else if (nt == 256) hipLaunchKernelGGL((forward_kernel<T, 256, false>), dim3(n_tasks), dim3(256), 0, st, a);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }
@@ -705,13 +709,16 @@ template <typename T>
 hipError_t launch_backward(const Args<T> &a, int n_wins, hipStream_t st) {
     const size_t lds = a.ring_g ? 0 : backward_block_bytes<T>(a.ring);
     /* EM expectations: FP64 only (the runtime refuses FP32 expectations runs) */
+    const bool g = a.ring_g != nullptr;
     if constexpr (sizeof(T) == 8) {
         if (a.expect) {
-            hipLaunchKernelGGL((backward_kernel<T, true>), dim3(n_wins), dim3(64), lds, st, a);
+            if (g) hipLaunchKernelGGL((backward_kernel<T, true, false>), dim3(n_wins), dim3(64), 0, st, a);
+            else hipLaunchKernelGGL((backward_kernel<T, true, true>), dim3(n_wins), dim3(64), lds, st, a);
             return hipGetLastError();
         }
     }
-    hipLaunchKernelGGL((backward_kernel<T, false>), dim3(n_wins), dim3(64), lds, st, a);
+    if (g) hipLaunchKernelGGL((backward_kernel<T, false, false>), dim3(n_wins), dim3(64), 0, st, a);
+    else hipLaunchKernelGGL((backward_kernel<T, false, true>), dim3(n_wins), dim3(64), lds, st, a);
     return hipGetLastError();
 }
 
