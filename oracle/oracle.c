@@ -18,7 +18,14 @@
  * pdfs against scipy closed forms, k-mer id KATs, the 5-mer tiny forward/backward KAT
  * (tests/stateMachineTests.c:565-692: 7 pairs, forward==backward total, per-diagonal
  * totals), brute-force path enumeration with exact log-sum-exp, spline invariants, and
- * window-schedule invariants (every diagonal gets exactly one posterior).
+ * window-schedule invariants (every diagonal gets exactly one posterior); the real-read
+ * counts of tests/stateMachineTests.c:823-851 and :896-947 (1076 pairs; 7349 with every
+ * C a degenerate 'X'); for degenerate cells the path KATs
+ * (tests/signalPairwiseAlignerTest.c:556-581, tests/variableOrderPairwiseAlignerTests.c:107-123)
+ * and the mixture identity over site options (tests/test_degenerate.py); for the
+ * expectations, explicit enumeration of the 3-state paths (tests/test_expectations.py).
+ * Parity unpinned: the values of posteriors and window totals on multi-window reads
+ * (the windowed backward approximation), which only a run of the reference could fix.
  *
  * The restatement keeps the reference's operation order: the per-cell transition
  * order of stateMachine3_cellCalculate, push-form backward accumulation, xmy-ascending

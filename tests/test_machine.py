@@ -496,7 +496,8 @@ def test_signal_machine_degenerate_variant_calls(tmp_path, oracle_lib):
     run on the same anchors and parameters, in the stable x+y order of its list; the
     variantCaller output is the reference writer's rows for the 'X' positions of those
     pairs (writePosteriorProbsVC, signalMachine.c:135-199: the called base is the path
-    k-mer's option at the 'X', at the reference position of the 'X')."""
+    k-mer's option at the 'X', at the reference position of the 'X'); the assignments
+    output carries each pair's path k-mer."""
     import re
     line = _guide_cigar()
     ref = re.sub("C(?=G)", "X", open(REF).read().split()[0])
@@ -508,7 +509,7 @@ def test_signal_machine_degenerate_variant_calls(tmp_path, oracle_lib):
     prm = _params()
     pa = D.parse_cigar(line)
     outs = {}
-    for fmt in (0, 1):
+    for fmt in (0, 1, 2):
         out = tmp_path / f"out{fmt}.tsv"
         print(f"signalMachine degenerate -s {fmt}", flush=True)
         _run([MACHINE, "--twoD", "-s", str(fmt), "-o", "1", "-T", TMODEL, "-C", CMODEL, "-f", str(fwd), "-b",
@@ -554,3 +555,7 @@ def test_signal_machine_degenerate_variant_calls(tmp_path, oracle_lib):
                 vc.append([t[5], str(int(t[1]) + q), base, t[12], strand, "forward", "zymo_read"])
     assert outs[1] == vc
     assert {v[2] for v in vc} <= set("CEO") and len({v[2] for v in vc}) > 1
+    # assignments: the path k-mer (with the called option), strand, descaled mean and
+    # posterior of every full row
+    assert outs[2] == [[t[15], t[4], t[13], t[12]] for t in rows]
+    assert any("E" in t[15] or "O" in t[15] for t in rows)
