@@ -115,7 +115,8 @@ __device__ __forceinline__ void load_raw(const Model<double> &m, const Args<doub
  * receives its entry on d+3 (the entry on d+2 is in flight in a fourth slot).  The
  * diagonal records slide by one. */
 template <bool LDS>
-__device__ __forceinline__ void fwd_step(const Args<double> &a, const PathEnt *et, const PathDiag *dg, double *ring,
+__device__ __forceinline__ void fwd_step(const Args<double> &a, const LutEntry<double> *lut, const PathEnt *et,
+                                         const PathDiag *dg, double *ring,
                                          int R, int64_t N, int64_t evc, bool gauss, bool ragged_left, double s,
                                          double inv_var, int64_t d, PathDiag &Dm2, PathDiag &Dm1, PathDiag &D0,
                                          PathDiag &D1, PathDiag &D2, PathDiag &D3, const PathEnt &Ec,
@@ -165,19 +166,19 @@ __device__ __forceinline__ void fwd_step(const Args<double> &a, const PathEnt *e
             const double aX0 = eX + tGOX, aX1 = eX + tGEX, aX2 = eX + tSWX;
             const double aM0 = eM + tMC, aM1 = eM + tMFX, aM2 = eM + tMFY;
             { /* upper -> gapY, the same path (a third independent chain) */
-                double Yn = logadd(Y, r1[es] + (eY + tGOY));
-                Yn = logadd(Yn, r1[2 * R + es] + (eY + tGEY));
+                double Yn = logadd_lut(Y, r1[es] + (eY + tGOY), lut);
+                Yn = logadd_lut(Yn, r1[2 * R + es] + (eY + tGEY), lut);
                 Y = hU ? Yn : Y;
             }
             if (hL || hM) {
                 for (int i = 0; i < Ec.pn; i++) {
                     const int eq = (Ec.pfirst + i * Ec.pstep) & RM;
-                    double Xn = logadd(X, r1[eq] + aX0);
-                    double Mn = logadd(M, r2[eq] + aM0);
-                    Xn = logadd(Xn, r1[R + eq] + aX1);
-                    Mn = logadd(Mn, r2[R + eq] + aM1);
-                    Xn = logadd(Xn, r1[2 * R + eq] + aX2);
-                    Mn = logadd(Mn, r2[2 * R + eq] + aM2);
+                    double Xn = logadd_lut(X, r1[eq] + aX0, lut);
+                    double Mn = logadd_lut(M, r2[eq] + aM0, lut);
+                    Xn = logadd_lut(Xn, r1[R + eq] + aX1, lut);
+                    Mn = logadd_lut(Mn, r2[R + eq] + aM1, lut);
+                    Xn = logadd_lut(Xn, r1[2 * R + eq] + aX2, lut);
+                    Mn = logadd_lut(Mn, r2[2 * R + eq] + aM2, lut);
                     X = hL ? Xn : X;
                     M = hM ? Mn : M;
                 }
@@ -214,8 +215,8 @@ __device__ __forceinline__ void fwd_step(const Args<double> &a, const PathEnt *e
  * two emission-input slots trade roles without register moves (a move would wait for
  * the loads in flight).  Same arithmetic, in the same order, as the strided loop. */
 template <bool LDS>
-__device__ __forceinline__ void forward_pipelined(const Args<double> &a, const PathTab &P, const TaskDev &t, int ti,
-                                                  double *ring, int R) {
+__device__ __forceinline__ void forward_pipelined(const Args<double> &a, const LutEntry<double> *lut, const PathTab &P,
+                                                  const TaskDev &t, int ti, double *ring, int R) {
     const PathEnt *et = P.ent + P.ent_off[ti];
     const PathDiag *dg = P.diag + t.diag_off;
     const int tid = threadIdx.x;
@@ -233,8 +234,8 @@ __device__ __forceinline__ void forward_pipelined(const Args<double> &a, const P
     RawEmit RA = {}, RB = {};
     if (gauss) load_raw(a.m, a, EA, tid < D0.e1 - D0.e0, 0, evc, RA);
 #define SA_FWD_STEP(dd, E0, E1, E3, R0, R1)                                                                         \
-    fwd_step<LDS>(a, et, dg, ring, R, N, evc, gauss, ragged_left, s, inv_var, dd, Dm2, Dm1, D0, D1, D2, D3, E0, E1, \
-                  E3, R0, R1)
+    fwd_step<LDS>(a, lut, et, dg, ring, R, N, evc, gauss, ragged_left, s, inv_var, dd, Dm2, Dm1, D0, D1, D2, D3, E0, \
+                  E1, E3, R0, R1)
     int64_t d = 0;
     for (; d + 3 <= N; d += 4) {
         SA_FWD_STEP(d, EA, EB, ED, RA, RB);
@@ -272,9 +273,14 @@ __global__ void __launch_bounds__(256) paths_forward_kernel(Args<double> a, Path
     const double NI = -__builtin_huge_val();
 
     if (P.maxent[ti] <= NT) {
-        /* an LDS ring is addressed as LDS (a pointer that may be either is a flat one) */
-        if (a.ring_g) forward_pipelined<false>(a, P, t, ti, ring, R);
-        else forward_pipelined<true>(a, P, t, ti, reinterpret_cast<double *>(smem), R);
+        /* the log-add's cubic coefficients from an LDS table (bit-identical to the
+           compare-and-select form); an LDS ring is addressed as LDS (a pointer that
+           may be either is a flat one) */
+        __shared__ LutEntry<double> lut[16];
+        lut_fill(lut, tid);
+        __syncthreads();
+        if (a.ring_g) forward_pipelined<false>(a, lut, P, t, ti, ring, R);
+        else forward_pipelined<true>(a, lut, P, t, ti, reinterpret_cast<double *>(smem), R);
         return;
     }
     /* diagonals wider than the workgroup: entries strided over the threads */
