@@ -381,6 +381,11 @@ __global__ void __launch_bounds__(256) paths_backward_kernel(Args<double> a, Pat
     int32_t *out = a.out + 4 * W.out_off;
     __shared__ int s_cursor;
     if (tid == 0) s_cursor = 0;
+    /* the recurrence's log-add takes its cubic coefficients from an LDS table
+       (bit-identical to the compare-and-select form) */
+    __shared__ LutEntry<double> lut[16];
+    lut_fill(lut, tid);
+    __syncthreads();
     int posterior_count = 0;
     double total = NI;
     __shared__ int s_wave_count[4];
@@ -410,25 +415,25 @@ __global__ void __launch_bounds__(256) paths_backward_kernel(Args<double> a, Pat
                         const int ep = v.pp[x + 1] + sr.first + i;
                         const double eM = Emit<double>::match(m, a, v.ekid[ep], evc + y, s, inv_var);
                         const double BM = r2[ep & RM];
-                        M = logadd(M, BM + (eM + tMC));
-                        X = logadd(X, BM + (eM + tMFX));
-                        Y = logadd(Y, BM + (eM + tMFY));
+                        M = logadd_lut(M, BM + (eM + tMC), lut);
+                        X = logadd_lut(X, BM + (eM + tMFX), lut);
+                        Y = logadd_lut(Y, BM + (eM + tMFY), lut);
                     }
                 }
                 if (x >= lo1 && x <= hi1 && x >= 1) { /* upper: cell (x, y+1) on e+1, the same path */
                     const double eY = Emit<double>::gapy(m, a, v.ekid[en], evc + y, s, inv_var);
                     const double BY = r1[2 * R + (en & RM)];
-                    M = logadd(M, BY + (eY + tGOY));
-                    Y = logadd(Y, BY + (eY + tGEY));
+                    M = logadd_lut(M, BY + (eY + tGOY), lut);
+                    Y = logadd_lut(Y, BY + (eY + tGEY), lut);
                 }
                 if (hlow) { /* lower: cell (x+1, y) on e+1 */
                     for (int i = 0; i < sr.n; i++) {
                         const int ep = v.pp[x + 1] + sr.first + i;
                         const double eX = Emit<double>::gapx(m, v.ekid[ep]);
                         const double BX = r1[R + (ep & RM)];
-                        M = logadd(M, BX + (eX + tGOX));
-                        X = logadd(X, BX + (eX + tGEX));
-                        Y = logadd(Y, BX + (eX + tSWX));
+                        M = logadd_lut(M, BX + (eX + tGOX), lut);
+                        X = logadd_lut(X, BX + (eX + tGEX), lut);
+                        Y = logadd_lut(Y, BX + (eX + tSWX), lut);
                     }
                 }
             }
