@@ -180,6 +180,7 @@ struct sa_batch {
     std::string options;      /* the bases an 'X' stands for */
     std::string href;         /* the packed reference bytes (host copy, path tables) */
     DBuf d_pcol, d_pent, d_pplane, d_pp, d_psa, d_psb, d_pecol, d_pekid, d_pdoff, d_pdiag, d_pentr, d_pmaxent;
+    HBuf h_pentr, h_pdiag; /* the two largest path tables, built in pinned memory */
     /* EM expectations run (sa_expect): FP64, register forward sweeps and LDS backward sweeps with the accumulators */
     int32_t expect = 0;
     double ex_rec_threshold = 1.0;
@@ -213,6 +214,8 @@ struct sa_batch {
                        &d_pcol, &d_pent, &d_pplane, &d_pp, &d_psa, &d_psb, &d_pecol, &d_pekid, &d_pdoff,
                        &d_pdiag, &d_pentr, &d_pmaxent};
         for (DBuf *b : all) b->release();
+        h_pentr.release();
+        h_pdiag.release();
     }
 };
 
@@ -254,6 +257,17 @@ int fail(sa_context *ctx, int code, const std::string &msg) {
             return fail((ctx), _e == hipErrorOutOfMemory ? SA_ERR_OUT_OF_MEMORY : SA_ERR_HIP,                  \
                         std::string(#call) + ": " + hipGetErrorString(_e));                                   \
     } while (0)
+
+/* n elements from host memory (pinned: one DMA at the PCIe rate) */
+template <typename T>
+int upload_raw(sa_context *ctx, DBuf &b, const T *v, int64_t n) {
+    SA_HIP(ctx, b.ensure(sizeof(T) * (size_t)std::max<int64_t>(n, 1)));
+    if (n > 0) {
+        SA_HIP(ctx, hipMemcpyAsync(b.p, v, sizeof(T) * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
+        SA_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return SA_OK;
+}
 
 template <typename T>
 int upload(sa_context *ctx, DBuf &b, const std::vector<T> &v) {
@@ -1555,35 +1569,80 @@ int run_paths(sa_batch *b, sa_result *res) {
     if (err & sa::ERR_BAND) return fail(ctx, SA_ERR_ANCHORS, "invalid band (anchors outside the matrix or out of order)");
     if (err & sa::ERR_WINDOWS) return fail(ctx, SA_ERR_HIP, "window schedule overflow");
 
-    /* path tables, one task per worker thread: per column its first entry and site
-       range, per entry its column, k-mer id and legal predecessors, per diagonal its
-       band, entries and plane offset */
-    struct TaskTab {
-        std::vector<int32_t> pp, sa, sb, ecol, ekid;
-        std::vector<sa::PathEnt> ent;
-        int64_t planes = 0;
-        int32_t max_ent = 0;
-        int err = 0;
-    };
-    std::vector<TaskTab> tt(nt);
-    std::vector<int64_t> doffE(b->total_diag);
-    std::vector<sa::PathDiag> pdiag(b->total_diag);
+    /* path tables, tasks spread over worker threads, in two passes: the entries of
+       each task (per column, one path per combination of its k-mer's 'X' options),
+       then every table written in place: per column its first entry and site range,
+       per entry its column, k-mer id and legal predecessors, per diagonal its band,
+       entries and plane offset */
     auto ipow = [](int64_t base, int e) {
         int64_t r = 1;
         for (int i = 0; i < e; i++) r *= base;
         return r;
     };
-    auto build_task = [&](int64_t ti) {
-        TaskTab &T = tt[ti];
+    const int nthreads = (int)std::max<int64_t>(1, std::min<int64_t>(nt, 16));
+    auto parallel = [&](auto &&fn) {
+        std::atomic<int64_t> next{0};
+        auto worker = [&]() {
+            for (int64_t ti; (ti = next.fetch_add(1)) < nt;) fn(ti);
+        };
+        std::vector<std::thread> th;
+        for (int i = 1; i < nthreads; i++) th.emplace_back(worker);
+        worker();
+        for (auto &x : th) x.join();
+    };
+    /* the 'X' sites of a task's bases and, per column x >= 1, the sites [s0, s1) of
+       its k-mer ref[x-1 .. x-1+k) */
+    auto sites_of = [&](int64_t ti, std::vector<int32_t> &site) {
+        const TaskDev &t = b->tasks[ti];
+        const char *ref = b->href.data() + t.ref_off;
+        site.clear();
+        for (int64_t i = 0; i < (int64_t)t.lX + k - 1; i++)
+            if (ref[i] == 'X') site.push_back((int32_t)i);
+    };
+    std::vector<int64_t> n_ent(nt), planes_t(nt);
+    std::vector<int32_t> maxent(nt), terr(nt);
+    parallel([&](int64_t ti) {
+        std::vector<int32_t> site;
+        sites_of(ti, site);
+        int64_t n = 1; /* column 0 */
+        size_t s0 = 0, s1 = 0;
+        for (int64_t x = 1; x <= b->tasks[ti].lX; x++) {
+            while (s0 < site.size() && site[s0] < x - 1) s0++;
+            while (s1 < site.size() && site[s1] < x - 1 + k) s1++;
+            const int64_t np = ipow(B, (int)(s1 - s0));
+            if (np > 729 || n + np > INT32_MAX / 2) {
+                terr[ti] = 1;
+                return;
+            }
+            n += np;
+        }
+        n_ent[ti] = n;
+    });
+    for (int64_t ti = 0; ti < nt; ti++)
+        if (terr[ti] == 1) return fail(ctx, SA_ERR_UNSUPPORTED, "too many paths in a degenerate reference");
+    std::vector<int64_t> col_off(nt), ent_off(nt), plane_off(nt);
+    int64_t ncol = 0, nent = 0;
+    for (int64_t ti = 0; ti < nt; ti++) {
+        col_off[ti] = ncol;
+        ent_off[ti] = nent;
+        ncol += (int64_t)b->tasks[ti].lX + 2;
+        nent += n_ent[ti];
+    }
+    std::vector<int32_t> pp(ncol), psa(ncol), psb(ncol), ecol(nent), ekid(nent);
+    std::vector<int64_t> doffE(b->total_diag);
+    SA_HIP(ctx, b->h_pentr.ensure(sizeof(sa::PathEnt) * (size_t)std::max<int64_t>(nent, 1)));
+    SA_HIP(ctx, b->h_pdiag.ensure(sizeof(sa::PathDiag) * (size_t)std::max<int64_t>(b->total_diag, 1)));
+    sa::PathEnt *pent = b->h_pentr.as<sa::PathEnt>();
+    sa::PathDiag *pdiag = b->h_pdiag.as<sa::PathDiag>();
+    parallel([&](int64_t ti) {
         const TaskDev &t = b->tasks[ti];
         const char *ref = b->href.data() + t.ref_off;
         const int64_t lX = t.lX;
-        std::vector<int32_t> site; /* positions of 'X' in this sub-alignment's bases */
-        for (int64_t i = 0; i < lX + k - 1; i++)
-            if (ref[i] == 'X') site.push_back((int32_t)i);
-        T.pp.reserve(lX + 2);
-        T.sa.reserve(lX + 2);
-        T.sb.reserve(lX + 2);
+        std::vector<int32_t> site;
+        sites_of(ti, site);
+        int32_t *tpp = pp.data() + col_off[ti], *tsa = psa.data() + col_off[ti], *tsb = psb.data() + col_off[ti];
+        int32_t *tcol = ecol.data() + ent_off[ti], *tkid = ekid.data() + ent_off[ti];
+        sa::PathEnt *tent = pent + ent_off[ti];
         int32_t P = 0;
         size_t s0 = 0, s1 = 0;
         for (int64_t x = 0; x <= lX; x++) {
@@ -1593,17 +1652,12 @@ int run_paths(sa_batch *b, sa_result *res) {
                 while (s1 < site.size() && site[s1] < x - 1 + k) s1++;
                 np = ipow(B, (int)(s1 - s0));
             }
-            if (np > 729 || P + np > INT32_MAX / 2) {
-                T.err = 1;
-                return;
-            }
-            T.pp.push_back(P);
-            T.sa.push_back(x >= 1 ? (int32_t)s0 : 0);
-            T.sb.push_back(x >= 1 ? (int32_t)s1 : 0);
+            tpp[x] = P;
+            tsa[x] = x >= 1 ? (int32_t)s0 : 0;
+            tsb[x] = x >= 1 ? (int32_t)s1 : 0;
             for (int64_t q = 0; q < np; q++) {
-                T.ecol.push_back((int32_t)x);
-                int32_t id = 0;
                 sa::PathEnt e{(int32_t)x, (int32_t)q, 0, 1, 0, 0};
+                int32_t id = 0;
                 if (x >= 1) {
                     /* the path's k-mer: the first 'X' varies slowest */
                     char km[16];
@@ -1617,7 +1671,7 @@ int run_paths(sa_batch *b, sa_result *res) {
                     for (int j = 0; j < k; j++) {
                         const int c = model->lut[(uint8_t)km[j]];
                         if (c < 0) {
-                            T.err = 2;
+                            terr[ti] = 2;
                             return;
                         }
                         id = id * A + c;
@@ -1625,84 +1679,49 @@ int run_paths(sa_batch *b, sa_result *res) {
                     /* legal predecessors in column x-1: the shared sites [sa0, sb1) are
                        q's leading digits; every option of the sites leaving, [sa1, sa0) */
                     if (x == 1) {
-                        e.pfirst = 0;
-                        e.pstep = 1;
                         e.pn = 1;
                     } else {
-                        const int sa1 = T.sa[x - 1], sb1 = T.sb[x - 1], sa0 = (int)s0, sb0 = (int)s1;
-                        e.pfirst = T.pp[x - 1] + (int32_t)(q / ipow(B, sb0 - sb1));
+                        const int sa1 = tsa[x - 1], sb1 = tsb[x - 1], sa0 = (int)s0, sb0 = (int)s1;
+                        e.pfirst = tpp[x - 1] + (int32_t)(q / ipow(B, sb0 - sb1));
                         e.pstep = (int32_t)ipow(B, sb1 - sa0);
                         e.pn = (int32_t)ipow(B, sa0 - sa1);
                     }
                 }
                 e.kk = id;
-                T.ekid.push_back(id);
-                T.ent.push_back(e);
+                tcol[P + q] = (int32_t)x;
+                tkid[P + q] = id;
+                tent[P + q] = e;
             }
             P += (int32_t)np;
         }
-        T.pp.push_back(P);
-        T.sa.push_back(0);
-        T.sb.push_back(0);
+        tpp[lX + 1] = P;
+        tsa[lX + 1] = 0;
+        tsb[lX + 1] = 0;
         const int64_t N = (int64_t)t.lX + t.lY;
         const int32_t *l = xlo.data() + t.diag_off, *h = xhi.data() + t.diag_off;
         int64_t run = 0;
+        int32_t mx = 0;
         for (int64_t d = 0; d <= N; d++) {
-            const int32_t e0 = T.pp[l[d]], e1 = T.pp[h[d] + 1];
+            const int32_t e0 = tpp[l[d]], e1 = tpp[h[d] + 1];
             doffE[t.diag_off + d] = run;
             pdiag[t.diag_off + d] = sa::PathDiag{l[d], h[d], e0, e1, run}; /* + the task's plane offset below */
             run += e1 - e0;
-            T.max_ent = std::max<int32_t>(T.max_ent, e1 - e0);
+            mx = std::max<int32_t>(mx, e1 - e0);
         }
-        T.planes = run;
-    };
-    const int nthreads = (int)std::max<int64_t>(1, std::min<int64_t>(nt, 8));
-    auto parallel = [&](auto &&fn) {
-        std::atomic<int64_t> next{0};
-        auto worker = [&]() {
-            for (int64_t ti; (ti = next.fetch_add(1)) < nt;) fn(ti);
-        };
-        std::vector<std::thread> th;
-        for (int i = 1; i < nthreads; i++) th.emplace_back(worker);
-        worker();
-        for (auto &x : th) x.join();
-    };
-    parallel(build_task);
+        planes_t[ti] = run;
+        maxent[ti] = mx;
+    });
+    for (int64_t ti = 0; ti < nt; ti++)
+        if (terr[ti] == 2) return fail(ctx, SA_ERR_BAD_KMER, "degenerate option outside the alphabet");
+    int64_t planes = 0, max_ent = 1;
     for (int64_t ti = 0; ti < nt; ti++) {
-        if (tt[ti].err == 1) return fail(ctx, SA_ERR_UNSUPPORTED, "too many paths in a degenerate reference");
-        if (tt[ti].err == 2) return fail(ctx, SA_ERR_BAD_KMER, "degenerate option outside the alphabet");
-    }
-    std::vector<int64_t> col_off(nt), ent_off(nt), plane_off(nt);
-    std::vector<int32_t> maxent(nt);
-    int64_t ncol = 0, nent = 0, planes = 0, max_ent = 1;
-    for (int64_t ti = 0; ti < nt; ti++) {
-        col_off[ti] = ncol;
-        ent_off[ti] = nent;
         plane_off[ti] = planes;
-        ncol += (int64_t)tt[ti].pp.size();
-        nent += (int64_t)tt[ti].ecol.size();
-        planes += tt[ti].planes;
-        maxent[ti] = tt[ti].max_ent;
-        max_ent = std::max<int64_t>(max_ent, tt[ti].max_ent);
+        planes += planes_t[ti];
+        max_ent = std::max<int64_t>(max_ent, maxent[ti]);
     }
-    std::vector<int32_t> pp(ncol), psa(ncol), psb(ncol), ecol(nent), ekid(nent);
-    std::vector<sa::PathEnt> pent(nent);
     parallel([&](int64_t ti) {
-        TaskTab &T = tt[ti];
         const TaskDev &t = b->tasks[ti];
-        std::copy(T.pp.begin(), T.pp.end(), pp.begin() + col_off[ti]);
-        std::copy(T.sa.begin(), T.sa.end(), psa.begin() + col_off[ti]);
-        std::copy(T.sb.begin(), T.sb.end(), psb.begin() + col_off[ti]);
-        std::copy(T.ecol.begin(), T.ecol.end(), ecol.begin() + ent_off[ti]);
-        std::copy(T.ekid.begin(), T.ekid.end(), ekid.begin() + ent_off[ti]);
-        std::copy(T.ent.begin(), T.ent.end(), pent.begin() + ent_off[ti]);
         for (int64_t d = 0; d <= (int64_t)t.lX + t.lY; d++) pdiag[t.diag_off + d].foff += plane_off[ti];
-        std::vector<int32_t>().swap(T.pp); /* release the task's copy */
-        std::vector<int32_t>().swap(T.sa);
-        std::vector<int32_t>().swap(T.sb);
-        std::vector<int32_t>().swap(T.ecol);
-        std::vector<int32_t>().swap(T.ekid);
-        std::vector<sa::PathEnt>().swap(T.ent);
     });
     const int ring = next_pow2((int)std::min<int64_t>(max_ent, 1 << 24));
     /* windows and their pair slots: a first guess of a few pairs per posterior
@@ -1731,8 +1750,8 @@ int run_paths(sa_batch *b, sa_result *res) {
         (rc = upload(ctx, b->d_pplane, plane_off)) || (rc = upload(ctx, b->d_pp, pp)) ||
         (rc = upload(ctx, b->d_psa, psa)) || (rc = upload(ctx, b->d_psb, psb)) ||
         (rc = upload(ctx, b->d_pecol, ecol)) || (rc = upload(ctx, b->d_pekid, ekid)) ||
-        (rc = upload(ctx, b->d_pdoff, doffE)) || (rc = upload(ctx, b->d_pdiag, pdiag)) ||
-        (rc = upload(ctx, b->d_pentr, pent)) || (rc = upload(ctx, b->d_pmaxent, maxent)) ||
+        (rc = upload(ctx, b->d_pdoff, doffE)) || (rc = upload_raw(ctx, b->d_pdiag, pdiag, b->total_diag)) ||
+        (rc = upload_raw(ctx, b->d_pentr, pent, nent)) || (rc = upload(ctx, b->d_pmaxent, maxent)) ||
         (rc = upload(ctx, b->d_wins, wins)) ||
         (rc = upload(ctx, b->d_win_order, win_order)))
         return rc;
