@@ -320,3 +320,26 @@ def test_gpu_degenerate_hdp_reads_match_oracle(gpu_ctx, oracle_lib, n, options, 
     assert P.rel(res[0].log_likelihood, st["log_likelihood"]) <= P.LL_REL
     assert int(np.max(want[:, 3])) > 0
     print(out)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("options,ragged", [("ACGT", (True, True)), ("CEO", (False, False)), ("CE", (False, True))])
+def test_gpu_degenerate_options_and_ends(gpu_ctx, oracle_lib, options, ragged):
+    """canonicalVariants (four options per site) and non-ragged ends: pairs, paths,
+    posteriors, order and log-likelihood equal the oracle's (FP64)."""
+    from signalalign_amd import native, synth
+    from tests import _parity as P
+    m = models.synthetic_gaussian_model("ACEGOT", 6, seed=3)
+    r = synth.single_read(m, 1500, seed=77)
+    r = dict(r, ref=_with_x(r["ref"].decode() if isinstance(r["ref"], bytes) else r["ref"]))
+    params = native.Params(diagonal_expansion=30)
+    want, st = oracle.get_aligned_pairs(P.oracle_model(m, r), r["ref"], r["events"], r["anchors"],
+                                        oracle.Params(diagonal_expansion=30), *ragged, options=options)
+    dm = gpu_ctx.model(m)
+    read = native.Read(r["ref"], r["events"], r["anchors"], r["scale"], r["shift"], r["var"], *ragged,
+                       degenerate_options=options)
+    print(f"degenerate {options} ragged={ragged}", flush=True)
+    res, _ = gpu_ctx.align_batch(dm, [read], params, native.SA_FP64)
+    out = _compare_paths(res[0].pairs, res[0].paths, want, params.threshold)
+    assert P.rel(res[0].log_likelihood, st["log_likelihood"]) <= P.LL_REL
+    print(out)
