@@ -28,7 +28,8 @@ extern "C" {
                                         (the reference exits, impl/nanopore_hdp.c:376-381) */
 #define SA_ERR_ANCHORS (-5)          /* anchors not strictly increasing / outside the matrix
                                         (asserts at impl/pairwiseAligner.c:161-167) */
-#define SA_ERR_UNSUPPORTED (-6)      /* band wider than the kernels support, unknown emission */
+#define SA_ERR_UNSUPPORTED (-6)      /* band wider than the kernels support, unknown emission; a degenerate
+                                        reference in FP32 or in an expectations run */
 #define SA_ERR_NO_DEVICE (-7)
 #define SA_PENDING 1 /* sa_server_wait: the request has not finished */
 
@@ -326,6 +327,9 @@ typedef struct {
 } sa_server_stats;
 
 int sa_server_create(int device, const sa_server_config *config, sa_server **out);
+/* Copies the read and queues it.  A degenerate reference (degenerate_options set and
+ * an 'X' in ref) runs in FP64 (SA_ERR_UNSUPPORTED for SA_FP32), in waves of reads with
+ * the same options. */
 int sa_server_submit(sa_server *server, const sa_model *model, const sa_read *read, const sa_params *params,
                      sa_precision precision, int64_t *ticket);
 /* Waits up to timeout_s (0: poll) for a request: SA_OK with its result (valid until

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -44,6 +45,7 @@ struct Request {
     std::vector<int64_t> anchors;
     double scale = 1, shift = 0, var = 1;
     int32_t rl = 1, rr = 1;
+    std::string options; /* degenerate base options of an 'X' reference, else empty */
     int64_t est_bytes = 0;
     int64_t n_diag = 0; /* lX + lY: the length of the forward sweep's dependency chain */
     /* outputs: the wave's result, shared by its requests */
@@ -120,7 +122,7 @@ void sa_server::schedule_loop() {
             /* a wave finishes with its longest read: keep reads of one length class
                (within a factor of two of the oldest request) together */
             const bool same_class = 2 * q->n_diag >= first->n_diag && q->n_diag <= 2 * first->n_diag;
-            if (q->model == first->model && q->precision == first->precision &&
+            if (q->model == first->model && q->precision == first->precision && q->options == first->options &&
                 same_params(q->params, first->params) && same_class && fits) {
                 wave.push_back(q);
                 bytes += q->est_bytes;
@@ -170,7 +172,7 @@ void sa_server::lane_loop(Lane *L) {
             rd[i].var = q->var;
             rd[i].ragged_left = q->rl;
             rd[i].ragged_right = q->rr;
-            rd[i].degenerate_options = nullptr;
+            rd[i].degenerate_options = q->options.empty() ? nullptr : q->options.c_str();
         }
         const Request *f = wave.front();
         int64_t cells = 0;
@@ -249,7 +251,8 @@ namespace {
    are bounded by the rectangles between consecutive anchors widened by the band
    expansion (band_construct keeps each diagonal within the anchors around it):
    O(anchors), cheap enough for the submitting thread. */
-int64_t estimate_bytes(const sa_read *r, int kmer_length, const sa_params *p, sa_precision prec) {
+int64_t estimate_bytes(const sa_read *r, int kmer_length, const sa_params *p, sa_precision prec,
+                       double paths_per_column) {
     const int64_t lX = r->ref_length >= kmer_length ? r->ref_length - kmer_length + 1 : 0;
     const int64_t lY = r->n_events;
     const int64_t e = p->diagonal_expansion;
@@ -262,7 +265,27 @@ int64_t estimate_bytes(const sa_read *r, int kmer_length, const sa_params *p, sa
         py = y;
     }
     const int64_t w = prec == SA_FP32 ? 4 : 8;
-    return cells * w * 3 / 2 + (lX + lY + 1) * 32 + lY * (4 * w + 8 * r->event_stride) + lX * 8 + lY * 36 + 4096;
+    /* a degenerate reference keeps three FP64 states of every (cell, path) entry and
+       per-entry and per-diagonal path tables */
+    const int64_t cell_bytes = paths_per_column > 0.0 ? (int64_t)(paths_per_column * 48.0) + 1 : w * 3 / 2;
+    const int64_t diag_bytes = paths_per_column > 0.0 ? 64 : 32;
+    return cells * cell_bytes + (lX + lY + 1) * diag_bytes + lY * (4 * w + 8 * r->event_stride) + lX * 8 + lY * 36 +
+           4096;
+}
+
+/* the mean paths per column of a reference with 'X' (options per site raised to the
+   'X's in each k-mer) */
+double mean_paths(const sa_read *r, int k, int64_t n_opt) {
+    const int64_t lX = r->ref_length >= k ? r->ref_length - k + 1 : 0;
+    if (lX == 0) return 1.0;
+    double sum = 0.0;
+    int nx = 0;
+    for (int64_t i = 0; i < r->ref_length; i++) {
+        nx += r->ref[i] == 'X';
+        if (i >= k) nx -= r->ref[i - k] == 'X';
+        if (i >= k - 1) sum += std::pow((double)n_opt, nx);
+    }
+    return sum / (double)lX;
 }
 
 } // namespace
@@ -306,9 +329,10 @@ int sa_server_submit(sa_server *s, const sa_model *model, const sa_read *read, c
         return SA_ERR_INVALID_ARGUMENT;
     if (read->ref_length < 0 || read->n_events < 0 || read->n_anchors < 0 || read->event_stride < 2)
         return SA_ERR_INVALID_ARGUMENT;
-    /* the serving waves take plain references (degenerate ones run through sa_align) */
-    if (read->degenerate_options && read->ref_length > 0 && memchr(read->ref, 'X', (size_t)read->ref_length))
-        return SA_ERR_UNSUPPORTED;
+    /* degenerate references (FP64 only) run in waves of their own options */
+    const bool degen = read->degenerate_options && read->degenerate_options[0] && read->ref_length > 0 &&
+                       memchr(read->ref, 'X', (size_t)read->ref_length);
+    if (degen && precision != SA_FP64) return SA_ERR_UNSUPPORTED;
     auto *q = new Request();
     q->model = model;
     if (params) q->params = *params; else sa_params_default(&q->params);
@@ -328,7 +352,10 @@ int sa_server_submit(sa_server *s, const sa_model *model, const sa_read *read, c
     q->var = read->var;
     q->rl = read->ragged_left;
     q->rr = read->ragged_right;
-    q->est_bytes = estimate_bytes(read, sa_model_kmer_length(model), &q->params, precision);
+    if (degen) q->options = read->degenerate_options;
+    const int k = sa_model_kmer_length(model);
+    q->est_bytes = estimate_bytes(read, k, &q->params, precision,
+                                  degen ? mean_paths(read, k, (int64_t)q->options.size()) : 0.0);
     q->n_diag = std::max<int64_t>(read->ref_length - sa_model_kmer_length(model) + 1, 0) + read->n_events;
     std::lock_guard<std::mutex> lk(s->mu);
     q->ticket = s->next_ticket++;

@@ -17,6 +17,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsignalalign_amd.so")
 
 SA_FP64, SA_FP32 = 0, 1
+# status codes (include/signalalign_amd.h)
+SA_OK, SA_ERR_INVALID_ARGUMENT, SA_ERR_HIP, SA_ERR_OUT_OF_MEMORY = 0, -1, -2, -3
+SA_ERR_BAD_KMER, SA_ERR_ANCHORS, SA_ERR_UNSUPPORTED, SA_ERR_NO_DEVICE = -4, -5, -6, -7
 SA_EMIT_GAUSS_DESCALED, SA_EMIT_GAUSS, SA_EMIT_HDP = 0, 1, 2
 
 _ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "out of device memory", -4: "k-mer outside alphabet",
@@ -532,7 +535,9 @@ class Server:
                 pairs = buf.reshape(-1, 3).astype(np.int64)
             else:
                 pairs = np.zeros((0, 3), dtype=np.int64)
-            res = ReadResult(pairs, rr.log_likelihood, rr.cells, rr.n_windows, rr.n_subalignments)
+            paths = (np.ctypeslib.as_array(rr.paths, shape=(rr.n_pairs,)).copy()
+                     if rr.n_pairs and bool(rr.paths) else None)
+            res = ReadResult(pairs, rr.log_likelihood, rr.cells, rr.n_windows, rr.n_subalignments, paths)
         else:
             res = ReadResult(None, rr.log_likelihood, rr.cells, rr.n_windows, rr.n_subalignments)
         return res, timing

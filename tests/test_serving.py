@@ -130,3 +130,41 @@ def test_bad_read_fails_alone(models_):
         assert srv.wait(tickets[2], 120.0)[0].cells > 0
     finally:
         srv.close()
+
+
+@pytest.mark.gpu
+def test_served_degenerate_reads(gpu_ctx, models_):
+    """Degenerate references ('X' at CpG cytosines over "CE", FP64) are served in
+    waves of their own beside plain requests: each result (pairs, paths,
+    log-likelihood) equals the read's own batched run bit for bit; an FP32 request
+    with 'X' is refused at submission."""
+    import re
+    g6, hd, dg, dh = models_
+    params = native.Params(diagonal_expansion=20)
+    rs = _reads(g6, [900, 1400, 700, 1100], seed=411)
+    xr = [dict(r, ref=re.sub(b"[CE](?=G)", b"X", r["ref"])) for r in rs[:3]]
+
+    def dread(r):
+        return native.Read(r["ref"], r["events"], r["anchors"], r["scale"], r["shift"], r["var"],
+                           degenerate_options="CE")
+
+    srv = native.Server(0, lanes=2)
+    try:
+        print("serve degenerate", flush=True)
+        tickets = [srv.submit(dg, dread(r), params, native.SA_FP64) for r in xr]
+        plain = [srv.submit(dg, P.native_read(r), params, native.SA_FP64) for r in rs[2:]]
+        got = _collect(srv, tickets)
+        got_plain = _collect(srv, plain)
+        with pytest.raises(native.SignalAlignError) as e:
+            srv.submit(dg, dread(xr[0]), params, native.SA_FP32)
+        assert e.value.code == native.SA_ERR_UNSUPPORTED
+    finally:
+        srv.close()
+    for r, (res, _) in zip(xr, got):
+        want, _ = gpu_ctx.align_batch(dg, [dread(r)], params, native.SA_FP64)
+        assert np.array_equal(res.pairs, want[0].pairs) and np.array_equal(res.paths, want[0].paths)
+        assert res.log_likelihood == want[0].log_likelihood
+        assert res.paths is not None and res.paths.max() > 0
+    for r, (res, _) in zip(rs[2:], got_plain):
+        want, _ = gpu_ctx.align_batch(dg, [P.native_read(r)], params, native.SA_FP64)
+        assert np.array_equal(res.pairs, want[0].pairs) and res.paths is None
