@@ -258,7 +258,7 @@ __global__ void __launch_bounds__(256) paths_forward_kernel(Args<double> a, Path
     const int R = P.ring, RM = R - 1;
     unsigned char *mem = a.ring_g ? a.ring_g + (size_t)blockIdx.x * a.ring_g_stride : smem;
     double *ring = reinterpret_cast<double *>(mem); /* [3 diag][3 state][R] */
-    const int ti = blockIdx.x;
+    const int ti = a.task_order ? a.task_order[blockIdx.x] : blockIdx.x;
     const TaskDev t = a.tasks[ti];
     const PathView v = view(P, t, ti);
     const Model<double> &m = a.m;

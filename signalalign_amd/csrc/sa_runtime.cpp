@@ -1743,8 +1743,26 @@ int run_paths(sa_batch *b, sa_result *res) {
         }
     }
     const int64_t nw = (int64_t)wins.size();
-    std::vector<int32_t> win_order(nw);
-    for (int64_t w = 0; w < nw; w++) win_order[w] = (int32_t)w;
+    /* two groups: the tasks at least half as long as the longest (their forward sweeps
+       are the critical path) on the context's stream, the rest on a second stream,
+       whose backward windows then run while the long forward sweeps continue */
+    int64_t n_max = 0;
+    for (int64_t ti = 0; ti < nt; ti++) n_max = std::max<int64_t>(n_max, (int64_t)b->tasks[ti].lX + b->tasks[ti].lY);
+    std::vector<int32_t> task_order, win_order;
+    for (int g = 0; g < 2; g++)
+        for (int64_t ti = 0; ti < nt; ti++) {
+            const bool longt = 2 * ((int64_t)b->tasks[ti].lX + b->tasks[ti].lY) >= n_max;
+            if (longt != (g == 0)) continue;
+            task_order.push_back((int32_t)ti);
+        }
+    int64_t n_long = 0;
+    for (int64_t ti = 0; ti < nt; ti++) n_long += 2 * ((int64_t)b->tasks[ti].lX + b->tasks[ti].lY) >= n_max;
+    int64_t nw_long = 0;
+    for (int64_t i = 0; i < nt; i++) {
+        const int32_t ti = task_order[i];
+        for (int32_t w = 0; w < nwin[ti]; w++) win_order.push_back(task_first_win[ti] + w);
+        if (i + 1 == n_long) nw_long = (int64_t)win_order.size();
+    }
     int rc;
     if ((rc = upload(ctx, b->d_pcol, col_off)) || (rc = upload(ctx, b->d_pent, ent_off)) ||
         (rc = upload(ctx, b->d_pplane, plane_off)) || (rc = upload(ctx, b->d_pp, pp)) ||
@@ -1753,7 +1771,7 @@ int run_paths(sa_batch *b, sa_result *res) {
         (rc = upload(ctx, b->d_pdoff, doffE)) || (rc = upload_raw(ctx, b->d_pdiag, pdiag, b->total_diag)) ||
         (rc = upload_raw(ctx, b->d_pentr, pent, nent)) || (rc = upload(ctx, b->d_pmaxent, maxent)) ||
         (rc = upload(ctx, b->d_wins, wins)) ||
-        (rc = upload(ctx, b->d_win_order, win_order)))
+        (rc = upload(ctx, b->d_win_order, win_order)) || (rc = upload(ctx, b->d_task_order, task_order)))
         return rc;
     SA_HIP(ctx, b->d_fM.ensure(sizeof(double) * std::max<int64_t>(planes, 1)));
     SA_HIP(ctx, b->d_fX.ensure(sizeof(double) * std::max<int64_t>(planes, 1)));
@@ -1786,21 +1804,39 @@ int run_paths(sa_batch *b, sa_result *res) {
     constexpr size_t kLdsMax = 65536;
     const size_t fb = sa::paths_forward_bytes(ring), bb = sa::paths_backward_bytes(ring);
     const size_t fs = (fb + 255) / 256 * 256, bs = (bb + 255) / 256 * 256;
-    if (fb > kLdsMax || bb > kLdsMax)
-        SA_HIP(ctx, b->d_ring_g.ensure(std::max(fb > kLdsMax ? (size_t)nt * fs : 0, bb > kLdsMax ? (size_t)nw * bs : 0)));
+    /* HBM rings: one slice per forward workgroup, then one per backward workgroup (the
+       two streams run forward and backward sweeps at the same time) */
+    const size_t f_region = fb > kLdsMax ? (size_t)nt * fs : 0, b_region = bb > kLdsMax ? (size_t)nw * bs : 0;
+    if (f_region + b_region) SA_HIP(ctx, b->d_ring_g.ensure(f_region + b_region));
+    unsigned char *ring_f = fb > kLdsMax ? b->d_ring_g.as<unsigned char>() : nullptr;
+    unsigned char *ring_b = bb > kLdsMax ? b->d_ring_g.as<unsigned char>() + f_region : nullptr;
+    const int32_t *d_to = b->d_task_order.as<int32_t>(), *d_wo = b->d_win_order.as<int32_t>();
+    hipStream_t st2 = ctx->aux[0]; /* the context's stream itself in a one-stream context */
     SA_HIP(ctx, hipEventRecord(ctx->ev[2], st));
-    {
-        sa::Args<double> af = a;
-        af.ring_g = fb > kLdsMax ? b->d_ring_g.as<unsigned char>() : nullptr;
-        af.ring_g_stride = (int64_t)fs;
-        SA_HIP(ctx, sa::launch_paths_forward(af, P, (int)nt, st));
-    }
-    SA_HIP(ctx, hipEventRecord(ctx->ev[3], st));
-    if (nw) {
-        sa::Args<double> ab = a;
-        ab.ring_g = bb > kLdsMax ? b->d_ring_g.as<unsigned char>() : nullptr;
-        ab.ring_g_stride = (int64_t)bs;
-        SA_HIP(ctx, sa::launch_paths_backward(ab, P, (int)nw, st));
+    if (st2 != st) SA_HIP(ctx, hipStreamWaitEvent(st2, ctx->ev[2], 0));
+    auto sweeps = [&](hipStream_t sg, int64_t t0, int64_t t1, int64_t w0, int64_t w1, bool mark) -> int {
+        if (t1 > t0) {
+            sa::Args<double> af = a;
+            af.task_order = d_to + t0;
+            af.ring_g = ring_f ? ring_f + (size_t)t0 * fs : nullptr;
+            af.ring_g_stride = (int64_t)fs;
+            SA_HIP(ctx, sa::launch_paths_forward(af, P, (int)(t1 - t0), sg));
+        }
+        if (mark) SA_HIP(ctx, hipEventRecord(ctx->ev[3], sg));
+        if (w1 > w0) {
+            sa::Args<double> ab = a;
+            ab.win_order = d_wo + w0;
+            ab.ring_g = ring_b ? ring_b + (size_t)w0 * bs : nullptr;
+            ab.ring_g_stride = (int64_t)bs;
+            SA_HIP(ctx, sa::launch_paths_backward(ab, P, (int)(w1 - w0), sg));
+        }
+        return SA_OK;
+    };
+    if ((rc = sweeps(st, 0, n_long, 0, nw_long, true)) || (rc = sweeps(st2, n_long, nt, nw_long, nw, false)))
+        return rc;
+    if (st2 != st) {
+        SA_HIP(ctx, hipEventRecord(ctx->ev[9], st2));
+        SA_HIP(ctx, hipStreamWaitEvent(st, ctx->ev[9], 0));
     }
     SA_HIP(ctx, hipEventRecord(ctx->ev[4], st));
     std::vector<int32_t> wcount(nw);
@@ -1830,7 +1866,7 @@ int run_paths(sa_batch *b, sa_result *res) {
         ab.wins = b->d_wins.as<WinDev>();
         ab.win_order = b->d_win_order.as<int32_t>();
         ab.out = b->d_packed.as<int32_t>();
-        ab.ring_g = bb > kLdsMax ? b->d_ring_g.as<unsigned char>() : nullptr;
+        ab.ring_g = ring_b;
         ab.ring_g_stride = (int64_t)bs;
         SA_HIP(ctx, sa::launch_paths_backward(ab, P, (int)redo.size(), st));
         /* the timed span ends after the second pass (the host round trip included) */
@@ -1882,8 +1918,8 @@ int run_paths(sa_batch *b, sa_result *res) {
     int64_t tc = 0;
     for (int64_t c : cells) tc += c;
     tm.cells = tc;
-    tm.forward_launches = 1;
-    tm.backward_launches += nw ? 1 : 0;
+    tm.forward_launches = (n_long > 0) + (nt > n_long);
+    tm.backward_launches += (nw_long > 0) + (nw > nw_long);
     return SA_OK;
 }
 
