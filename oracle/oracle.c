@@ -24,8 +24,13 @@
  * (tests/signalPairwiseAlignerTest.c:556-581, tests/variableOrderPairwiseAlignerTests.c:107-123)
  * and the mixture identity over site options (tests/test_degenerate.py); for the
  * expectations, explicit enumeration of the 3-state paths (tests/test_expectations.py).
- * Parity unpinned: the values of posteriors and window totals on multi-window reads
- * (the windowed backward approximation), which only a run of the reference could fix.
+ * The windowed traceback (window schedule, the end-prior seed of each window's top
+ * diagonal, the ragged-end prior only at the end of the matrix, the posterior range
+ * of each window) is pinned by explicit enumeration of the band paths that end on each
+ * window's top diagonal (tests/test_oracle_kats.py, test_windowed_traceback_brute_force).
+ * Parity unpinned: the every-10th-diagonal total refresh (impl/pairwiseAligner.c:1370):
+ * with exact arithmetic every posterior diagonal of a window has the same total, so
+ * only a run of the reference could fix which diagonal's lookup-logAdd total is used.
  *
  * The restatement keeps the reference's operation order: the per-cell transition
  * order of stateMachine3_cellCalculate, push-form backward accumulation, xmy-ascending

@@ -316,6 +316,161 @@ def test_brute_force_enumeration(ragged, oracle_lib):
     assert abs(st2["log_likelihood"] - total) < 1e-2
 
 
+def _window_rule(L, R, N, expansion, min_diags, traceback):
+    """The traceback schedule of getPosteriorProbsWithBanding (impl/pairwiseAligner.c:1326-1331,
+    1350): a window ends at the first diagonal d >= tracedBackTo + minDiagsBetweenTraceBack
+    whose band width is at most 2 * expansion + 1, or at the last diagonal; its posterior
+    diagonals are (tracedBackTo, d - traceBackDiagonals - 1], or up to d at the end."""
+    wins, T = [], 0
+    for d in range(1, N + 1):
+        at_end = d == N
+        width = (R[d] - L[d]) // 2 + 1
+        if at_end or (d >= T + min_diags and width <= 2 * expansion + 1):
+            TF = d if at_end else d - traceback - 1
+            wins.append((T, d, TF, int(at_end)))
+            T = TF
+    return wins
+
+
+def _windowed_brute_force(m, ref, ev, anchors, expansion, min_diags, traceback, ragged_left, ragged_right,
+                          exact_emission):
+    """Explicit path enumeration of the windowed traceback.  Window (T, D, TF) seeds its
+    backward pass on diagonal D with the end prior (the ragged-end prior only on the
+    last diagonal of a ragged-right alignment), so with exact arithmetic the posterior
+    of a match cell on a diagonal in (T, TF] is
+
+        sum over band paths from the start that end on diagonal D and enter the cell
+        as a match, each weighted by start x transitions x emissions x end(state on D)
+        / the same sum over every band path ending on D,
+
+    and the total probability of every posterior diagonal of the window is that
+    denominator.  Returns (windows, {(x-1, y-1): posterior}, log of the last
+    window's denominator)."""
+    k = m.kmer_length
+    lX, lY = len(ref) - k + 1, len(ev)
+    N = lX + lY
+    L, R = oracle.band(anchors, lX, lY, expansion)
+    wins = _window_rule(L, R, N, expansion, min_diags, traceback)
+    t = m.transitions
+    trans = {(0, 0): t[0], (1, 0): t[1], (2, 0): t[2], (0, 1): t[3], (0, 2): t[4], (1, 1): t[5], (2, 2): t[6],
+             (2, 1): t[7], (1, 2): -math.inf}
+    start = [-math.inf, 0.0, 0.0] if ragged_left else [0.0, -math.inf, -math.inf]
+    end = [t[0], t[1], t[2]]
+    rend = [(t[3] + t[4]) / 2, t[5], t[6]]
+    em = {}
+    for x in range(lX):
+        for y in range(lY):
+            em[(x, y, 1)] = exact_emission(ref[x:x + k], ev[y], True)
+            em[(x, y, 0)] = exact_emission(ref[x:x + k], ev[y], False)
+
+    def in_band(x, y):
+        return L[x + y] <= x - y <= R[x + y]
+
+    top = {D: i for i, (_, D, _, _) in enumerate(wins)}
+    recs = [[] for _ in wins]
+
+    def rec(x, y, s, score, matches):
+        d = x + y
+        if d in top:
+            w = top[d]
+            e = rend if (wins[w][3] and ragged_right) else end
+            if e[s] > -math.inf:
+                recs[w].append((score + e[s], matches))
+        if d == N:
+            return
+        moves = []
+        if x < lX and y < lY:
+            moves.append((x + 1, y + 1, 0, em[(x, y, 1)]))
+        if x < lX:
+            moves.append((x + 1, y, 1, -2.3025850929940455))
+        if y < lY and x >= 1:
+            moves.append((x, y + 1, 2, em[(x - 1, y, 0)]))
+        for nx, ny, ns, e in moves:
+            tp = trans[(s, ns)]
+            if tp == -math.inf or not in_band(nx, ny):
+                continue
+            rec(nx, ny, ns, score + e + tp, matches + ((x, y),) if ns == 0 else matches)
+
+    for s0 in range(3):
+        if start[s0] > -math.inf:
+            rec(0, 0, s0, start[s0], ())
+    post, logz = {}, None
+    for (T, D, TF, _), rs in zip(wins, recs):
+        sc = np.array([s for s, _ in rs])
+        mx = sc.max()
+        logz = mx + math.log(np.exp(sc - mx).sum())
+        for s, matches in rs:
+            w = math.exp(s - logz)
+            for (px, py) in matches:
+                if T < px + py + 2 <= TF:
+                    post[(px, py)] = post.get((px, py), 0.0) + w
+        # every band match cell of the window's posterior diagonals, reached or not
+        for d in range(T + 1, TF + 1):
+            for xmy in range(L[d], R[d] + 1, 2):
+                x, y = (d + xmy) // 2, (d - xmy) // 2
+                if x >= 1 and y >= 1:
+                    post.setdefault((x - 1, y - 1), 0.0)
+    return wins, post, logz
+
+
+_WINDOW_CASES = [
+    # (min_diags, traceback, anchors, expansion, ragged)
+    (3, 1, [], 20, (False, False)),
+    (4, 2, [], 20, (True, True)),
+    (5, 1, [], 20, (False, True)),
+    (5, 3, [], 20, (True, False)),
+    (3, 1, [(2, 1), (4, 3)], 2, (False, False)),
+    (4, 1, [(1, 1), (3, 2), (5, 4)], 2, (True, True)),
+    (4, 1, [], 2, (False, True)),   # widths 6, 7, 7 defer a window from diagonal 6 to 9
+    (3, 1, [], 2, (True, False)),   # and from 5 to 9
+]
+
+
+@pytest.mark.parametrize("case", _WINDOW_CASES)
+def test_windowed_traceback_brute_force(case, oracle_lib):
+    """The windowed traceback of getPosteriorProbsWithBanding (impl/pairwiseAligner.c:1319-1410)
+    against explicit path enumeration, with exact log-sum-exp: the oracle's windows
+    follow the schedule rule (:1326-1331, 1350); every band match cell of every
+    posterior diagonal is reported once, with the posterior of the paths that end on
+    its window's top diagonal weighted by the end prior (:1335-1337) -- the
+    ragged-end prior only at the end of the matrix -- over their sum (1e-9 before
+    the reference's floor(p * 1e7)); the total is the last window's sum.  Small
+    minDiagsBetweenTraceBack / traceBackDiagonals put several windows in a
+    13-diagonal matrix, full-band and banded (where the width rule defers a window)."""
+    min_diags, tb, anchors, expansion, ragged = case
+    m = models.synthetic_gaussian_model("ACGT", 3, seed=6)
+    rng = np.random.default_rng(31 + min_diags * 7 + tb)
+    ref = "".join(rng.choice(list("ACGT"), 9))
+    kids = models.kmer_ids(ref, "ACGT", 3)
+    ev = np.zeros((6, 4))
+    for j in range(6):
+        tt = m.match[kids[min(j, len(kids) - 1)]]
+        ev[j, 0] = tt[0] + rng.normal(0, tt[1])
+        ev[j, 1] = rng.wald(tt[2], tt[4])
+    om = oracle.Model(oracle.GAUSS, m.alphabet, 3, m.transitions, m.match, m.gapy, m.gapx)
+    exact_em = lambda kmer, e, match: oracle.emission(om, kmer, e, match)
+    wins, post, logz = _windowed_brute_force(m, ref, ev, anchors, expansion, min_diags, tb, ragged[0], ragged[1],
+                                             exact_em)
+    assert len(wins) >= 4
+    if expansion == 2 and not anchors:
+        assert any(D - T > min_diags for T, D, _, _ in wins)  # the width rule deferred a window
+    params = oracle.Params(threshold=0.0, diagonal_expansion=expansion, min_diags_between_traceback=min_diags,
+                           traceback_diagonals=tb)
+    oracle.set_exact_logadd(True)
+    try:
+        (pairs, st), rec = oracle.record_windows(oracle.get_aligned_pairs, om, ref, ev, anchors, params, *ragged)
+    finally:
+        oracle.set_exact_logadd(False)
+    assert [tuple(int(v) for v in w) for w in rec] == wins
+    assert st["n_windows"] == len(wins)
+    assert abs(st["log_likelihood"] - logz) < 1e-9
+    got = [((int(x), int(y)), p / 1e7) for p, x, y in pairs]
+    assert len(got) == len(set(c for c, _ in got)) == len(post)
+    for c, p in got:
+        want = min(post[c], 1.0)
+        assert want - 1e-7 - 1e-9 <= p <= want + 1e-9, (c, p, want)
+
+
 def test_window_schedule_invariants(oracle_lib):
     """Multi-window read: the oracle's schedule runs (asserting every diagonal gets
     exactly one posterior, impl/pairwiseAligner.c:1411), windows follow the
