@@ -11,6 +11,16 @@ from signalalign_amd import native
 P_TOL = 1e-4
 EDGE = 1e-3
 LL_REL = 1e-6
+# FP32 recurrence: the log-likelihood keeps the FP64 bar; a posterior can move by
+# the reference lookup logAdd's largest jump (4.47e-4 at |x - y| = 7.5, where
+# lookup(7.5) = 7.5 + 4.47e-4 but logAdd returns the maximum for a difference of
+# 7.5 or more) plus rounding drift: an FP32 rounding of ~1e-5 that moves a term of a
+# total refresh across that knot rescales every posterior of up to ten diagonals.
+# The FP64 oracle itself moves by 4.2e-4 when only its event inputs are rounded to
+# FP32 (a 1e-8 relative change), so no FP32 recurrence can promise the 1e-4 bar on
+# every pair; the fraction of pairs beyond 1e-4 is bounded instead.
+P_TOL_FP32 = 5e-4
+LL_REL_FP32 = 1e-6
 
 
 def oracle_model(m, read, kind=None):
@@ -46,11 +56,15 @@ def compare(gpu_pairs, ref_pairs, threshold, p_tol=P_TOL, edge=EDGE, check_order
     only_g = [k for k in g if k not in r and abs(g[k] - threshold) > edge]
     only_r = [k for k in r if k not in g and abs(r[k] - threshold) > edge]
     common = [k for k in g if k in r]
-    dmax = max((abs(g[k] - r[k]) for k in common), default=0.0)
+    dps = np.array([abs(g[k] - r[k]) for k in common]) if common else np.zeros(0)
+    dmax = float(dps.max()) if len(dps) else 0.0
     out = {"n_gpu": len(g), "n_ref": len(r), "only_gpu": len(only_g), "only_ref": len(only_r),
-           "max_abs_dp": dmax}
+           "max_abs_dp": dmax, "n_beyond_1e-4": int(np.sum(dps > P_TOL)),
+           "p99_abs_dp": float(np.percentile(dps, 99)) if len(dps) else 0.0}
     assert not only_g and not only_r, f"pair sets differ beyond the threshold edge: {out}"
     assert dmax <= p_tol, f"posterior error {dmax} > {p_tol}: {out}"
+    if p_tol > P_TOL:  # the FP32 bar: at most 1 in 500 pairs beyond 1e-4
+        assert out["n_beyond_1e-4"] <= max(2, len(common) // 500), f"too many posteriors beyond 1e-4: {out}"
     if check_order:
         cs = set(common)
         og = [(int(x), int(y)) for _, x, y in gpu_pairs if (int(x), int(y)) in cs]

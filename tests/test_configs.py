@@ -1,0 +1,118 @@
+"""The benchmark configurations at their own shapes, against the FP64 oracle (GPU tests).
+
+  * one long read as the plain drop-in: 50 kb ACGT reference, 6-mer Gaussian table,
+    band expansion 100, FP64 and FP32;
+  * HDP emissions: ACEGT 5-mer template and complement HDP models (3125 k-mers,
+    mixture densities on a 2000-point grid over 30-130 pA), reads aligned on both
+    strands -- the complement read against the reverse-complement target with the
+    complement model -- band expansion 50, FP32 and FP64;
+  * batched: ACEGT 6-mer Gaussian table (15625 k-mers), reads of uneven length (one of
+    24 kb) in one batch, band expansion 50, FP32 and FP64.
+
+FP64: the hot path's bar (same pair set except pairs within 1e-3 of the threshold,
+posteriors within 1e-4, log-likelihood within 1e-6 relative, the reference's list
+order).  FP32: log-likelihood within 1e-6 relative, posteriors within P_TOL_FP32 and
+at most 1 in 500 pairs beyond 1e-4 (tests/_parity.py states why).
+"""
+import numpy as np
+import pytest
+
+from signalalign_amd import native, synth
+from tests import _parity as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _fp32_ok(out, ll_rel):
+    assert ll_rel <= P.LL_REL_FP32, (ll_rel, out)
+
+
+def _check_batch(ctx, m, reads, params, label, ragged=(True, True)):
+    dm = ctx.model(m)
+    b = native.Batch(ctx, dm, [P.native_read(r, *ragged) for r in reads], params)
+    try:
+        refs = [P.run_oracle(m, r, params, *ragged) for r in reads]
+        for prec, name in ((native.SA_FP64, "fp64"), (native.SA_FP32, "fp32")):
+            print(f"{label}: {len(reads)} reads {name}", flush=True)
+            res, tm = b.run(prec)
+            for i, (g, (ref, st)) in enumerate(zip(res, refs)):
+                assert g.cells == st["cells"] and g.n_windows == st["n_windows"]
+                ll = P.rel(g.log_likelihood, st["log_likelihood"])
+                if prec == native.SA_FP64:
+                    out = P.compare(g.pairs, ref, params.threshold)
+                    assert ll <= P.LL_REL, (i, ll)
+                else:
+                    out = P.compare(g.pairs, ref, params.threshold, p_tol=P.P_TOL_FP32)
+                    _fp32_ok(out, ll)
+                out["ll_rel"] = ll
+                print(f"  read {i} ({len(reads[i]['ref'])} nt, {len(reads[i]['events'])} events, {g.cells} cells): "
+                      f"{out}", flush=True)
+    finally:
+        b.close()
+        dm.close()
+
+
+def test_long_read_50kb_e100():
+    """The plain drop-in on one 50 kb read (about 67k events, e=100, about 17M cells,
+    over 100 traceback windows), with one and with several wavefronts per forward
+    sweep."""
+    ctx = native.Context(0)
+    try:
+        m = synth.gaussian_model("ACGT", 6, seed=1)
+        r = synth.single_read(m, 50000, seed=2074)
+        params = native.Params(diagonal_expansion=100)
+        _check_batch(ctx, m, [r], params, "50 kb read")
+    finally:
+        ctx.close()
+
+
+def test_long_read_50kb_one_wavefront(monkeypatch):
+    monkeypatch.setenv("SA_FWD_WAVES", "1")
+    ctx = native.Context(0)
+    try:
+        m = synth.gaussian_model("ACGT", 6, seed=1)
+        r = synth.single_read(m, 50000, seed=2075)
+        _check_batch(ctx, m, [r], native.Params(diagonal_expansion=100), "50 kb read, one wavefront")
+    finally:
+        ctx.close()
+
+
+def test_hdp_5mer_2000_point_grid_both_strands():
+    """HDP configuration shapes: template reads with the template model, complement
+    reads (reverse-complement target) with the complement model."""
+    ctx = native.Context(0)
+    try:
+        mt = synth.hdp_model("ACEGT", 5, seed=3)
+        mc = synth.hdp_model("ACEGT", 5, seed=4)
+        assert mt.density.shape[1] == 2000 and mt.n_kmers == 3125
+        reads = synth.batch_reads(mt, 4, 5000, seed=77, methyl_cpg=0.3, strands=2, complement_model=mc)
+        params = native.Params(diagonal_expansion=50)
+        for strand, m in ((0, mt), (1, mc)):
+            rs = [r for r in reads if r["strand"] == strand]
+            if strand == 1:
+                # the complement read's target is the reverse complement of its template's
+                t = [r for r in reads if r["strand"] == 0]
+                comp = bytes.maketrans(b"ACGTE", b"TGCAE")
+                for a, c in zip(t, rs):
+                    assert a["ref"].replace(b"E", b"C")[::-1].translate(comp) == c["ref"].replace(b"E", b"C")
+            _check_batch(ctx, m, rs, params, f"HDP strand {strand}")
+    finally:
+        ctx.close()
+
+
+def test_batched_acegt_6mer_uneven_lengths():
+    """Batched configuration shapes: the 15625-k-mer ACEGT table, 5mC at 30% of CpG
+    sites, lengths 1-24 kb in one batch (the long one on the several-wavefront sweep,
+    the rest on one-wavefront sweeps)."""
+    ctx = native.Context(0)
+    try:
+        m = synth.gaussian_model("ACEGT", 6, seed=1)
+        assert m.n_kmers == 15625
+        reads = []
+        for i, n in enumerate([24000, 1000, 3000, 8000, 5500, 12000, 2200, 9000]):
+            rng = synth.rng_for(313, i, 0)
+            fwd, _ = synth.random_reference(n, rng, methyl_cpg=0.3)
+            reads.append(synth.simulate_read(m, fwd, synth.rng_for(313, i, 1)))
+        _check_batch(ctx, m, reads, native.Params(diagonal_expansion=50), "ACEGT batch")
+    finally:
+        ctx.close()

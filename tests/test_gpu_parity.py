@@ -2,8 +2,9 @@
 
 Bar (FP64 drop-in): same aligned-pair set except pairs within 1e-3 of the
 threshold, posteriors within 1e-4 absolute, total log-likelihood within 1e-6
-relative, and the reference's list order.  FP32 results are held to the bar
-stated in each test and their errors are printed.
+relative, and the reference's list order.  FP32 results: log-likelihood within
+1e-6 relative, posteriors within P_TOL_FP32 with at most 1 in 500 beyond 1e-4
+(tests/_parity.py states why); their errors are printed.
 """
 import numpy as np
 import pytest
@@ -53,7 +54,7 @@ def test_c1_fp64(gpu_ctx, g6):
 
 def test_c1_fp32(gpu_ctx, g6):
     r = synth.single_read(g6, 2000, seed=11)
-    _check(gpu_ctx, g6, r, native.Params(diagonal_expansion=20), native.SA_FP32, p_tol=2e-3, ll_rel=1e-5)
+    _check(gpu_ctx, g6, r, native.Params(diagonal_expansion=20), native.SA_FP32, p_tol=P.P_TOL_FP32, ll_rel=P.LL_REL_FP32)
 
 
 def test_reference_5mer_kat_on_gpu(gpu_ctx):
@@ -101,7 +102,7 @@ def test_full_band_wider_than_lds(gpu_ctx, g6, n):
     r = synth.single_read(g6, n, seed=90 + n)
     r["anchors"] = np.zeros((0, 2), dtype=np.int64)
     _check(gpu_ctx, g6, r, native.Params(), native.SA_FP64)
-    _check(gpu_ctx, g6, r, native.Params(), native.SA_FP32, p_tol=2e-3, ll_rel=1e-5)
+    _check(gpu_ctx, g6, r, native.Params(), native.SA_FP32, p_tol=P.P_TOL_FP32, ll_rel=P.LL_REL_FP32)
 
 
 def test_split_blocks(gpu_ctx, g6):
@@ -123,8 +124,7 @@ def test_window_sizes(gpu_ctx, g6, expansion):
 def test_window_sizes_fp32(gpu_ctx, g6):
     for expansion in (20, 100, 300):
         r = synth.single_read(g6, 2500, seed=60 + expansion)
-        _check(gpu_ctx, g6, r, native.Params(diagonal_expansion=expansion), native.SA_FP32, p_tol=2e-3,
-               ll_rel=1e-5)
+        _check(gpu_ctx, g6, r, native.Params(diagonal_expansion=expansion), native.SA_FP32, p_tol=P.P_TOL_FP32, ll_rel=P.LL_REL_FP32)
 
 
 def _window_classes(r, m, expansion):
@@ -159,8 +159,7 @@ def test_window_width_switching(gpu_ctx, g6, expansion, gaps):
     cls = _window_classes(r, g6, expansion)
     assert max(cls) <= 3 and len(set(cls)) >= 2, cls  # the sweeps switch width (a class 4 would pin it)
     _check(gpu_ctx, g6, r, native.Params(diagonal_expansion=expansion), native.SA_FP64)
-    _check(gpu_ctx, g6, r, native.Params(diagonal_expansion=expansion), native.SA_FP32, p_tol=2e-3,
-           ll_rel=1e-5)
+    _check(gpu_ctx, g6, r, native.Params(diagonal_expansion=expansion), native.SA_FP32, p_tol=P.P_TOL_FP32, ll_rel=P.LL_REL_FP32)
 
 
 def test_wide_expansion_and_threshold(gpu_ctx, g6):
@@ -177,7 +176,7 @@ def test_general_gaussian_tables(gpu_ctx, g6):
     m.gapx[::7] = np.log(0.05)
     r = synth.single_read(g6, 1500, seed=28)
     _check(gpu_ctx, m, r, native.Params(diagonal_expansion=20), native.SA_FP64)
-    _check(gpu_ctx, m, r, native.Params(diagonal_expansion=20), native.SA_FP32, p_tol=2e-3, ll_rel=1e-5)
+    _check(gpu_ctx, m, r, native.Params(diagonal_expansion=20), native.SA_FP32, p_tol=P.P_TOL_FP32, ll_rel=P.LL_REL_FP32)
 
 
 def test_undescaled_emission(gpu_ctx, g6):
@@ -201,7 +200,7 @@ def test_hdp_fp64_and_fp32(gpu_ctx):
     fwd, _ = synth.random_reference(1200, rng, methyl_cpg=0.3)
     r = synth.simulate_read(m, fwd, synth.rng_for(41, 0, 1))
     _check(gpu_ctx, m, r, native.Params(diagonal_expansion=50), native.SA_FP64)
-    _check(gpu_ctx, m, r, native.Params(diagonal_expansion=50), native.SA_FP32, p_tol=2e-3, ll_rel=1e-5)
+    _check(gpu_ctx, m, r, native.Params(diagonal_expansion=50), native.SA_FP32, p_tol=P.P_TOL_FP32, ll_rel=P.LL_REL_FP32)
 
 
 def test_empty_and_short_inputs(gpu_ctx, g6):

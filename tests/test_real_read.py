@@ -111,8 +111,8 @@ def test_gpu_reproduces_the_reference_count(gpu_ctx, precision):
             assert P.rel(res[0].log_likelihood, st["log_likelihood"]) <= P.LL_REL
             assert len(res[0].pairs) == EXPECTED_PAIRS, (name, out)
         else:
-            out = P.compare(res[0].pairs, want, params.threshold, p_tol=2e-3)
-            assert P.rel(res[0].log_likelihood, st["log_likelihood"]) <= 1e-5
+            out = P.compare(res[0].pairs, want, params.threshold, p_tol=P.P_TOL_FP32)
+            assert P.rel(res[0].log_likelihood, st["log_likelihood"]) <= P.LL_REL_FP32
         print(name, precision, out)
 
 
@@ -152,7 +152,7 @@ def test_gpu_real_read_on_the_reference_hdp(gpu_ctx):
     dm = gpu_ctx.model(hm)
     params = native.Params(threshold=0.1)
     want, st = P.run_oracle(hm, read, params)
-    for prec, p_tol, ll in ((native.SA_FP64, P.P_TOL, P.LL_REL), (native.SA_FP32, 2e-3, 1e-5)):
+    for prec, p_tol, ll in ((native.SA_FP64, P.P_TOL, P.LL_REL), (native.SA_FP32, P.P_TOL_FP32, P.LL_REL_FP32)):
         res, _ = gpu_ctx.align_batch(dm, [P.native_read(read)], params, prec)
         out = P.compare(res[0].pairs, want, params.threshold, p_tol=p_tol)
         assert P.rel(res[0].log_likelihood, st["log_likelihood"]) <= ll
