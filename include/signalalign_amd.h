@@ -31,6 +31,8 @@ extern "C" {
 #define SA_ERR_UNSUPPORTED (-6)      /* band wider than the kernels support, unknown emission; a degenerate
                                         reference in FP32 or in an expectations run */
 #define SA_ERR_NO_DEVICE (-7)
+#define SA_ERR_CAPACITY (-8)         /* sa_align_noalloc: the read needs more than its fixed scratch or
+                                        the caller's pair buffer holds (the message names which) */
 #define SA_PENDING 1 /* sa_server_wait: the request has not finished */
 
 /* Arithmetic of the recurrence.  SA_FP64 follows the reference's doubles;
@@ -180,6 +182,24 @@ int sa_model_kmer_length(const sa_model *model);
 /* Plain per-read call: the drop-in for one getAlignedPairsUsingAnchors call. */
 int sa_align(sa_context *ctx, const sa_model *model, const sa_read *read, const sa_params *params,
              sa_precision precision, sa_result **out);
+
+/* Plain per-read call that allocates nothing.  No counterpart in the reference: it is
+ * sa_align for callers that cannot allocate memory (code that must not call hipMalloc,
+ * tight serving loops, harnesses that forbid allocation).  The caller passes the read
+ * and a host buffer for the pairs; every intermediate buffer comes from one fixed
+ * device array of SA_NOALLOC_SCRATCH_BYTES per device, held until the process ends, and
+ * the host side plans in fixed storage.  Pairs, their order and the log-likelihood are
+ * sa_align's.  Limits: the read's staged inputs, band, forward planes (4 B per band cell
+ * in FP32, 8 B in FP64, plus the refresh diagonals' gap states) and pair slots must fit
+ * the scratch -- a 50 kb read at band expansion 100 fits, FP32 or FP64 -- with at most 256
+ * split blocks, 65536 anchors and 8192 traceback windows; beyond them, or when more
+ * pairs than pairs_cap result, it returns SA_ERR_CAPACITY (*n_pairs then holds the
+ * read's pair count when that is what fell short).  No degenerate 'X' references.
+ * Calls on one device share the scratch and are serialised by a per-device lock. */
+#define SA_NOALLOC_SCRATCH_BYTES (256LL << 20)
+int sa_align_noalloc(sa_context *ctx, const sa_model *model, const sa_read *read, const sa_params *params,
+                     sa_precision precision, sa_pair *pairs, int64_t pairs_cap, int64_t *n_pairs,
+                     double *log_likelihood);
 
 /* Batched call: all reads in one length-sorted launch sequence. */
 int sa_align_batch(sa_context *ctx, const sa_model *model, const sa_read *reads, int64_t n_reads,

@@ -20,10 +20,11 @@ SA_FP64, SA_FP32 = 0, 1
 # status codes (include/signalalign_amd.h)
 SA_OK, SA_ERR_INVALID_ARGUMENT, SA_ERR_HIP, SA_ERR_OUT_OF_MEMORY = 0, -1, -2, -3
 SA_ERR_BAD_KMER, SA_ERR_ANCHORS, SA_ERR_UNSUPPORTED, SA_ERR_NO_DEVICE = -4, -5, -6, -7
+SA_ERR_CAPACITY = -8
 SA_EMIT_GAUSS_DESCALED, SA_EMIT_GAUSS, SA_EMIT_HDP = 0, 1, 2
 
 _ERRORS = {-1: "invalid argument", -2: "HIP error", -3: "out of device memory", -4: "k-mer outside alphabet",
-           -5: "invalid anchors", -6: "unsupported", -7: "no device"}
+           -5: "invalid anchors", -6: "unsupported", -7: "no device", -8: "fixed capacity exceeded"}
 
 _F64P = ctypes.POINTER(ctypes.c_double)
 _I64P = ctypes.POINTER(ctypes.c_int64)
@@ -127,6 +128,8 @@ def load():
     L.sa_model_create.argtypes = [vp, ctypes.POINTER(_ModelDesc), ctypes.POINTER(vp)]
     L.sa_model_destroy.argtypes = [vp]
     L.sa_align.argtypes = [vp, vp, ctypes.POINTER(_Read), ctypes.POINTER(_Params), ctypes.c_int, ctypes.POINTER(vp)]
+    L.sa_align_noalloc.argtypes = [vp, vp, ctypes.POINTER(_Read), ctypes.POINTER(_Params), ctypes.c_int,
+                                   ctypes.POINTER(_Pair), ctypes.c_int64, _I64P, _F64P]
     L.sa_align_batch.argtypes = [vp, vp, ctypes.POINTER(_Read), ctypes.c_int64, ctypes.POINTER(_Params),
                                  ctypes.c_int, ctypes.POINTER(vp)]
     L.sa_batch_create.argtypes = [vp, vp, ctypes.POINTER(_Read), ctypes.c_int64, ctypes.POINTER(_Params),
@@ -311,6 +314,19 @@ class Context:
 
     def align(self, model, read, params=None, precision=SA_FP64):
         return self.align_batch(model, [read], params, precision)[0][0]
+
+    def align_noalloc(self, model, read, params=None, precision=SA_FP64, pairs_cap=None):
+        """sa_align_noalloc: the plain per-read call with no allocation in the library
+        (intermediates in its fixed device scratch).  Returns (pairs (n, 3) int64,
+        log-likelihood).  With pairs_cap too small it raises SA_ERR_CAPACITY."""
+        cap = 8 * (len(read.ref) + len(read.events)) + 1024 if pairs_cap is None else int(pairs_cap)
+        buf = (_Pair * max(cap, 1))()
+        n, ll = ctypes.c_int64(0), ctypes.c_double(0.0)
+        rs, ps = read._struct(), (params or Params())._struct()
+        self._check(self._L.sa_align_noalloc(self._h, model._h, ctypes.byref(rs), ctypes.byref(ps), precision, buf,
+                                             cap, ctypes.byref(n), ctypes.byref(ll)), "sa_align_noalloc")
+        pairs = np.frombuffer(buf, dtype=np.int32, count=3 * n.value).reshape(-1, 3).astype(np.int64)
+        return pairs, ll.value
 
     def expect(self, model, read, params=None, record_threshold=1.0):
         """EM expectations of one read (sa_expect, FP64).  Returns a dict:
