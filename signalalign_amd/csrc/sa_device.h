@@ -116,12 +116,38 @@ __device__ __forceinline__ void set_priority(int p) {
     else if (p == 1) __builtin_amdgcn_s_setprio(1);
 }
 
-/* maximum over the wavefront, uniform */
-template <typename T>
-__device__ __forceinline__ T wave_max(T v) {
+/* maximum over the wavefront, uniform: within each row of 16 lanes by DPP swaps (pairs,
+   quads, half-row and row mirrors; no LDS round trip), then the four row maxima read
+   lane by lane.  Exactly the largest value, as any order of comparisons gives. */
+__device__ __forceinline__ float dpp_max_step(float v, int ctrl_sel) {
+    int u;
+    const int b = __float_as_int(v);
+    switch (ctrl_sel) {
+    case 0: u = __builtin_amdgcn_update_dpp(b, b, 0xb1, 0xf, 0xf, false); break;  /* quad_perm [1,0,3,2] */
+    case 1: u = __builtin_amdgcn_update_dpp(b, b, 0x4e, 0xf, 0xf, false); break;  /* quad_perm [2,3,0,1] */
+    case 2: u = __builtin_amdgcn_update_dpp(b, b, 0x141, 0xf, 0xf, false); break; /* row_half_mirror */
+    default: u = __builtin_amdgcn_update_dpp(b, b, 0x140, 0xf, 0xf, false); break; /* row_mirror */
+    }
+    const float w = __int_as_float(u);
+    return w > v ? w : v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+    v = dpp_max_step(v, 0);
+    v = dpp_max_step(v, 1);
+    v = dpp_max_step(v, 2);
+    v = dpp_max_step(v, 3);
+    const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+    const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
+    const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+    const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
+    const float a = r0 > r1 ? r0 : r1, b = r2 > r3 ? r2 : r3;
+    return uniform(a > b ? a : b);
+}
+/* (FP64 sweeps do not renormalise; kept for completeness) */
+__device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
-        T u = __shfl_xor(v, o, 64);
+        double u = __shfl_xor(v, o, 64);
         v = u > v ? u : v;
     }
     return uniform(v);

@@ -263,9 +263,9 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
     constexpr int WL = 64 * NW;   /* lanes of the workgroup */
     constexpr int W = WL * CPT;   /* positions of the window */
     /* the last cell of each wavefront crosses to the next through LDS, double-buffered
-       by diagonal parity (one barrier per diagonal) */
+       by diagonal parity (one barrier per diagonal); the wavefront maxima of a
+       renormalised diagonal cross at the same barrier */
     __shared__ T bx[2][NW][3];
-    __shared__ ECon<T, EK> bev[2][NW];
     __shared__ T red[NW];
     __shared__ T shbuf[6 * W];
     __shared__ LutEntry<T> lut[16];
@@ -278,7 +278,9 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
     const TaskDev t = a.tasks[ti];
     const Model<T> &m = a.m;
     const int gl = threadIdx.x;
-    const int lane = gl & 63, wave = gl >> 6;
+    const int lane = gl & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(gl >> 6); /* uniform: scalar branches on it */
+    const int woff = 64 * CPT * wave; /* the wavefront's first position in the window */
     const int lX = t.lX, lY = t.lY, N = lX + lY;
     const int32_t *xlo = a.xlo + t.diag_off;
     const int32_t *xhi = a.xhi + t.diag_off;
@@ -339,8 +341,11 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
     double off1 = 0.0, off2 = 0.0;
     int lo1 = lo0, lo2 = lo0;
     ECon<T, EK> pre;
-    load_econ_uniform(pre, a, evc, 1 - X0, 1 - X0 >= 0 && 1 - X0 < lY);
+    /* the event the wavefront's first cell reads on the next diagonal (each wavefront
+       loads its own through the scalar cache; cell x on d reads event d - x - 1) */
+    load_econ_uniform(pre, a, evc, 1 - X0 - woff, 1 - X0 - woff >= 0 && 1 - X0 - woff < lY);
     bool fresh = true;
+    T mxp = 0; /* the renormalisation shift of d-1 (its edge cells crossed unshifted) */
     bool bad = false;
 
     for (int d = 1; d <= N; d++) {
@@ -363,7 +368,7 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
         T fLM = NI, fLX = NI, fLY = NI;
         if (wave > 0) {
             const int pb = (d - 1) & 1;
-            fLM = bx[pb][wave - 1][0]; fLX = bx[pb][wave - 1][1]; fLY = bx[pb][wave - 1][2];
+            fLM = bx[pb][wave - 1][0] - mxp; fLX = bx[pb][wave - 1][1] - mxp; fLY = bx[pb][wave - 1][2] - mxp;
         }
         if (hi > X0 + W - 1) {
             /* slide the window as far as the two previous diagonals allow */
@@ -410,15 +415,14 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
         }
         if (!fresh) {
             /* cell x on d reads event d-x-1: its left neighbour's event of d-1 */
-            const ECon<T, EK> fill = wave > 0 ? bev[(d - 1) & 1][wave - 1] : pre;
-            const ECon<T, EK> inc = shr1(ev[CPT - 1], fill);
+            const ECon<T, EK> inc = shr1(ev[CPT - 1], pre);
 #pragma unroll
             for (int c = CPT - 1; c >= 1; c--) ev[c] = ev[c - 1];
             ev[0] = inc;
         }
         fresh = false;
         {
-            const int jn = d - X0; /* event of the first cell of the window on d+1 */
+            const int jn = d - X0 - woff; /* event of the wavefront's first cell on d+1 */
             load_econ_uniform(pre, a, evc, jn, jn >= 0 && jn < lY);
         }
         const T LM = shr1(pM[CPT - 1], fLM), LX = shr1(pX[CPT - 1], fLX), LY = shr1(pY[CPT - 1], fLY);
@@ -462,11 +466,20 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
                 mx = m3 > mx ? m3 : mx;
             }
         }
-        double offd = off1;
+        /* renormalisation: the wavefront maxima cross at the diagonal's barrier with
+           the edge cells (unshifted; the next diagonal's reader applies the shift) */
         if (renorm) {
             mx = wave_max(mx);
             if (lane == 0) red[wave] = mx;
-            __syncthreads();
+        }
+        if (lane == 63) {
+            const int pb = d & 1;
+            bx[pb][wave][0] = nM[CPT - 1]; bx[pb][wave][1] = nX[CPT - 1]; bx[pb][wave][2] = nY[CPT - 1];
+        }
+        __syncthreads();
+        double offd = off1;
+        mxp = 0;
+        if (renorm) {
             mx = red[0];
 #pragma unroll
             for (int w = 1; w < NW; w++) mx = red[w] > mx ? red[w] : mx;
@@ -474,6 +487,7 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
 #pragma unroll
                 for (int c = 0; c < CPT; c++) { nM[c] -= mx; nX[c] -= mx; nY[c] -= mx; }
                 offd = off1 + (double)mx;
+                mxp = mx;
             }
         }
 #pragma unroll
@@ -487,12 +501,6 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
         if (gl == j) { md_doff = (int32_t)run; md_dxy = fl ? (int32_t)runxy : -1; md_foff = offd; }
         run += hi - lo + 1;
         if (fl) runxy += hi - lo + 1;
-        if (lane == 63) {
-            const int pb = d & 1;
-            bx[pb][wave][0] = nM[CPT - 1]; bx[pb][wave][1] = nX[CPT - 1]; bx[pb][wave][2] = nY[CPT - 1];
-            bev[pb][wave] = ev[CPT - 1];
-        }
-        __syncthreads();
         LpM = LM; LpX = LX; LpY = LY;
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
