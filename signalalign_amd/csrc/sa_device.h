@@ -51,10 +51,23 @@ __device__ __forceinline__ T logadd(T a, T b) {
  * multiples of 0.5, so each entry lies inside one piece and the reference's
  * inclusive bounds (d <= 1.0, d <= 2.5, d <= 4.5) select the same piece.  d is NaN
  * only when both operands are -inf (min(NaN, 7.5) = 7.5 picks some entry; the
- * result is hi either way).  Results are bit-identical to logadd(). */
+ * result is hi either way).  Results are bit-identical to logadd().
+ *
+ * FP32 (the recurrence of the FP32 mode, not the FP64 parity path) uses a shorter
+ * form of the same function: with u = min(2d, 15.5) and g(d) = lookup(d) - d, the
+ * result is hi + g, where g is the piece's cubic with the linear coefficient less
+ * one, its coefficients scaled so that the cubic takes u (c0/8, c1/4, (c2-1)/2, c3:
+ * powers of two, so g(u) rounds as g(d) would).  A 17th entry of zeros serves
+ * d > 7.5 and the NaN of two -inf operands (min(NaN, 15.5) = 15.5), so no final
+ * compare is needed; the only difference from the reference's piecewise function is
+ * a difference of exactly 7.5, which takes the last piece instead of the maximum.
+ * The entries are stored in reverse order, entry 16 - ceil(u), so that the index is
+ * 16 + floor(-u) (one conversion with a negated operand). */
 template <typename T> struct LutEntry;
 template <> struct LutEntry<float> { float c[4]; };
 template <> struct LutEntry<double> { double c[4]; };
+template <typename T> struct LutSize { static constexpr int n = 16; };
+template <> struct LutSize<float> { static constexpr int n = 17; };
 
 template <typename T>
 __device__ __forceinline__ void lut_fill(LutEntry<T> *lut, int tid) {
@@ -69,14 +82,52 @@ __device__ __forceinline__ void lut_fill(LutEntry<T> *lut, int tid) {
     }
 }
 
+/* the FP32 form's coefficients of piece 0..3 (4: zeros) */
+__device__ __forceinline__ void lut_coeffs_f32(int piece, float *c) {
+    const float C[4][4] = {
+        {-0.009350833524763f, 0.130659527668286f, 0.498799810682272f, 0.693203116424741f},
+        {-0.014532321752540f, 0.139942324101744f, 0.495635523139337f, 0.692140569840976f},
+        {-0.004605031767994f, 0.063427417320019f, 0.695956496475118f, 0.514272634594009f},
+        {-0.000458661602210f, 0.009695946122598f, 0.930734667215156f, 0.168037164329057f}};
+    if (piece > 3) { c[0] = c[1] = c[2] = c[3] = 0.0f; return; }
+    c[0] = C[piece][0] * 0.125f;
+    c[1] = C[piece][1] * 0.25f;
+    c[2] = (C[piece][2] - 1.0f) * 0.5f;
+    c[3] = C[piece][3];
+}
+template <>
+__device__ __forceinline__ void lut_fill<float>(LutEntry<float> *lut, int tid) {
+    if (tid < 17) {
+        const int k = 16 - tid; /* entry tid serves ceil(u) = k */
+        lut_coeffs_f32(k <= 2 ? 0 : k <= 5 ? 1 : k <= 9 ? 2 : k <= 15 ? 3 : 4, lut[tid].c);
+    }
+}
+
+/* u = min(2 |a - b|, 15.5) (the doubling is a separate multiply: the output modifier
+   x2 is not applied while FP32 denormals are enabled) and the table index
+   16 + floor(-u) (one conversion with a negated operand) */
+__device__ __forceinline__ float la_u(float t) { return __builtin_fminf(__builtin_fabsf(t) * 2.0f, 15.5f); }
+__device__ __forceinline__ int la_index(float u) {
+    int m;
+    asm("v_cvt_flr_i32_f32_e64 %0, -%1" : "=v"(m) : "v"(u));
+    return m + 16;
+}
+
 __device__ __forceinline__ float logadd_lut(float a, float b, const LutEntry<float> *lut) {
-    const float hi = __builtin_fmaxf(a, b), lo = __builtin_fminf(a, b);
-    const float d = hi - lo;
-    const float dd = __builtin_fminf(d, 7.5f);
-    const int k = (int)__builtin_ceilf(dd + dd);
-    const float4 c = *reinterpret_cast<const float4 *>(&lut[k]);
-    const float r = poly3(d, c.x, c.y, c.z, c.w) + lo;
-    return d < 7.5f ? r : hi;
+    const float hi = __builtin_fmaxf(a, b);
+    const float u = la_u(a - b);
+    const float4 c = *reinterpret_cast<const float4 *>(&lut[la_index(u)]);
+    return hi + __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(c.x, u, c.y), u, c.z), u, c.w);
+}
+
+/* The FP32 form with its coefficients selected by compares (no table): bit-identical
+   to logadd_lut(float); the LDS sweeps use it. */
+__device__ __forceinline__ float logadd(float a, float b) {
+    const float hi = __builtin_fmaxf(a, b);
+    const float u = la_u(a - b);
+    float c[4];
+    lut_coeffs_f32(u <= 2.0f ? 0 : u <= 5.0f ? 1 : u <= 9.0f ? 2 : u <= 15.0f ? 3 : 4, c);
+    return hi + __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(c[0], u, c[1]), u, c[2]), u, c[3]);
 }
 
 __device__ __forceinline__ double logadd_lut(double a, double b, const LutEntry<double> *lut) {

@@ -268,7 +268,7 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
     __shared__ T bx[2][NW][3];
     __shared__ T red[NW];
     __shared__ T shbuf[6 * W];
-    __shared__ LutEntry<T> lut[16];
+    __shared__ LutEntry<T> lut[LutSize<T>::n];
     /* the several-wavefront sweeps carry the longest dependency chains of a launch
        sequence: their instructions go first on shared SIMDs (the launch's priority) */
     set_priority(a.prio);
@@ -826,7 +826,7 @@ template <typename T, int CMAX, int EK>
 __global__ void __launch_bounds__(64) forward_sw_kernel(Args<T> a, const int32_t *__restrict__ order) {
     extern __shared__ __align__(16) unsigned char dsm[];
     T *xs = reinterpret_cast<T *>(dsm); /* [6][64 * CMAX]: states across a width change */
-    __shared__ LutEntry<T> lut[16];
+    __shared__ LutEntry<T> lut[LutSize<T>::n];
     set_priority(a.prio);
     lut_fill(lut, (int)threadIdx.x);
     __syncthreads();
@@ -1352,7 +1352,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BwdWave
 backward_reg_kernel(Args<T> a, const int32_t *__restrict__ order) {
     extern __shared__ __align__(16) unsigned char dsm[];
     T *xs = reinterpret_cast<T *>(dsm); /* [4][64 * CMAX]: states across a width change */
-    __shared__ LutEntry<T> lut[16];
+    __shared__ LutEntry<T> lut[LutSize<T>::n];
     __shared__ LutEntry<double> lutd2[sizeof(T) == 8 ? 1 : 16]; /* FP64 table for the totals */
     lut_fill(lut, (int)threadIdx.x);
     if constexpr (sizeof(T) != 8) lut_fill(lutd2, (int)threadIdx.x);

@@ -68,3 +68,4 @@ def test_autotune_keeps_results(gpu_ctx, setup):
         _same(base, after)
     finally:
         b.close()
+
