@@ -138,8 +138,17 @@ typedef struct {
 
 typedef struct {
     double prep_ms;        /* band, schedule, k-mer ids, event constants (GPU) */
-    double forward_ms;     /* start of the sweeps to the end of the last forward launch */
-    double backward_ms;    /* start of the first backward launch to the end of the sweeps */
+    /* The forward and backward launches of different sub-alignment groups overlap on
+     * the streams, so these two are spans on the device timeline, not sums:
+     *   forward_ms   start of the sweeps to the end of the last forward launch (in a
+     *                batch with degenerate references: to the end of the long group's
+     *                forward launch, the shorter group's sweeps running beside it);
+     *   backward_ms  start of the first backward launch to the end of the sweeps (on a
+     *                one-stream context this span includes forward launches queued after
+     *                that first backward launch; with degenerate references: from the end
+     *                of the long group's forward launch to the end of the sweeps). */
+    double forward_ms;
+    double backward_ms;
     double collect_ms;     /* pair compaction */
     double total_ms;       /* whole call on the device, inputs resident to outputs resident */
     int64_t cells;

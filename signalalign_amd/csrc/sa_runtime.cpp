@@ -1538,7 +1538,10 @@ int run_paths(sa_batch *b, sa_result *res) {
         const TaskDev &t = b->tasks[ti];
         for (int64_t d = 0; d <= (int64_t)t.lX + t.lY; d++) pdiag[t.diag_off + d].foff += plane_off[ti];
     });
-    const int ring = next_pow2((int)std::min<int64_t>(max_ent, 1 << 24));
+    /* the ring index wraps with & (ring - 1): a diagonal with more (cell, path) entries
+       than the ring holds would alias its own entries */
+    if (max_ent > (1 << 24)) return fail(ctx, SA_ERR_UNSUPPORTED, "a diagonal holds more than 2^24 (cell, path) entries");
+    const int ring = next_pow2((int)max_ent);
     /* windows and their pair slots: a first guess of a few pairs per posterior
        diagonal (a diagonal's match posteriors over cells and paths sum to at most 1);
        windows that overflow run again with their exact count */
@@ -1647,12 +1650,19 @@ int run_paths(sa_batch *b, sa_result *res) {
         }
         return SA_OK;
     };
-    if ((rc = sweeps(st, 0, n_long, 0, nw_long, true)) || (rc = sweeps(st2, n_long, nt, nw_long, nw, false)))
-        return rc;
+    rc = sweeps(st, 0, n_long, 0, nw_long, true);
+    if (!rc) rc = sweeps(st2, n_long, nt, nw_long, nw, false);
     if (st2 != st) {
-        SA_HIP(ctx, hipEventRecord(ctx->ev[9], st2));
-        SA_HIP(ctx, hipStreamWaitEvent(st, ctx->ev[9], 0));
+        /* join st2 on every exit: buffers are freed stream-ordered on st, and a free
+           must not overtake the sweeps still queued on st2 */
+        const hipError_t e1 = hipEventRecord(ctx->ev[9], st2);
+        const hipError_t e2 = e1 == hipSuccess ? hipStreamWaitEvent(st, ctx->ev[9], 0) : e1;
+        if (e2 != hipSuccess) {
+            (void)hipStreamSynchronize(st2);
+            if (!rc) rc = sa_rt::fail_hip(ctx, e2, "joining the second stream");
+        }
     }
+    if (rc) return rc;
     SA_HIP(ctx, hipEventRecord(ctx->ev[4], st));
     std::vector<int32_t> wcount(nw);
     std::vector<double> wtot(nw);

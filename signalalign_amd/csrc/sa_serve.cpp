@@ -269,8 +269,35 @@ int64_t estimate_bytes(const sa_read *r, int kmer_length, const sa_params *p, sa
        per-entry and per-diagonal path tables */
     const int64_t cell_bytes = paths_per_column > 0.0 ? (int64_t)(paths_per_column * 48.0) + 1 : w * 3 / 2;
     const int64_t diag_bytes = paths_per_column > 0.0 ? 64 : 32;
+    /* a band too wide for a register window runs on the LDS sweeps, whose rings move to
+       per-workgroup slices of HBM when they outgrow LDS: one forward slice for the read
+       and one backward slice per traceback window */
+    int64_t wmax = 1;
+    px = 0;
+    py = 0;
+    for (int64_t i = 0; i <= r->n_anchors; i++) {
+        const int64_t x = i < r->n_anchors ? r->anchors[2 * i] + 1 : lX;
+        const int64_t y = i < r->n_anchors ? r->anchors[2 * i + 1] + 1 : lY;
+        wmax = std::max(wmax, std::min(std::min(lX, std::max<int64_t>(x - px, 0) + e),
+                                       std::min(lY, std::max<int64_t>(y - py, 0) + e)) + 1);
+        px = x;
+        py = y;
+    }
+    int64_t ring_bytes = 0;
+    const double ppc = paths_per_column > 0.0 ? paths_per_column : 1.0;
+    const int64_t ents = (int64_t)((double)wmax * ppc) + 1;
+    if (paths_per_column > 0.0 || wmax > 512) {
+        int64_t R = 1;
+        while (R < ents) R <<= 1;
+        const int64_t fwd = paths_per_column > 0.0 ? 72 * R : 9 * R * w + 16 * w;
+        const int64_t bwd = paths_per_column > 0.0 ? 88 * R + 16 : 16 * R + 16 + 9 * R * w;
+        const int64_t step = std::max<int64_t>(p->min_diags_between_traceback - p->traceback_diagonals - 1, 1);
+        const int64_t nwin = (lX + lY) / step + 2;
+        if (fwd > 65536) ring_bytes += fwd;
+        if (bwd > 65536) ring_bytes += nwin * bwd;
+    }
     return cells * cell_bytes + (lX + lY + 1) * diag_bytes + lY * (4 * w + 8 * r->event_stride) + lX * 8 + lY * 36 +
-           4096;
+           ring_bytes + 4096;
 }
 
 /* the mean paths per column of a reference with 'X' (options per site raised to the

@@ -261,6 +261,36 @@ def test_gpu_reproduces_the_reference_degenerate_count(gpu_ctx, oracle_lib):
 
 
 @pytest.mark.gpu
+def test_gpu_degenerate_two_groups_with_hbm_rings(gpu_ctx, oracle_lib):
+    """The Zymo degenerate read (whole-matrix band, up to 729 paths per column: rings
+    in HBM) batched with short degenerate reads: the short ones, under half the
+    longest sweep, run forward and backward on the second stream while the long one's
+    forward runs, with their ring slices after the long read's.  Every read equals the
+    oracle (pairs, paths, order, log-likelihood)."""
+    from signalalign_amd import native, synth
+    from tests import _parity as P
+    mm, kind, read = _zymo_descaled("X")
+    dm = gpu_ctx.model(mm, kind)
+    params = native.Params(threshold=0.01)
+    reads = [dict(read, ragged=(False, False))]
+    for i in range(3):
+        r = synth.single_read(mm, 160 + 40 * i, seed=600 + i)
+        r["ref"] = _with_x(r["ref"].decode() if isinstance(r["ref"], bytes) else r["ref"]).encode()
+        reads.append(dict(r, ragged=(True, True)))
+    nreads = [native.Read(r["ref"], r["events"], r["anchors"], r["scale"], r["shift"], r["var"], *r["ragged"],
+                          degenerate_options="CEO") for r in reads]
+    print("zymo degenerate with short reads", flush=True)
+    res, tm = gpu_ctx.align_batch(dm, nreads, params, native.SA_FP64)
+    assert tm["forward_launches"] == 2, tm
+    for r, g in zip(reads, res):
+        want, st = oracle.get_aligned_pairs(P.oracle_model(mm, r, oracle.GAUSS_DESCALED), r["ref"], r["events"],
+                                            r["anchors"], oracle.Params(0.01), *r["ragged"], options="CEO")
+        _compare_paths(g.pairs, g.paths, want, params.threshold)
+        assert P.rel(g.log_likelihood, st["log_likelihood"]) <= P.LL_REL
+    assert len(res[0].pairs) == DEGENERATE_PAIRS
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("dtype,options", [(0, "CE"), (1, "CEO")])
 def test_compat_degenerate_alignment_matches_oracle(tmp_path, oracle_lib, dtype, options):
     """A reference with 'X' through sequence_constructReferenceKmerSequence (the
