@@ -240,8 +240,10 @@ def hdp_leg(native, synth, ctx, args):
     """The HDP configuration: 5-mer ACEGT template and complement HDP models (3125
     k-mers, mixture densities on a 2000-point grid), 2048 reads (median 5 kb) on
     both strands (the complement read aligns to the reverse-complement target with
-    the complement model), band expansion 50, FP32.  One step = both strands' batches
-    (run one after the other); GCUPS = cells / summed device time."""
+    the complement model), band expansion 50, FP32.  One step = both strands'
+    batches, one after the other (the device is busy with either alone: running them
+    side by side on two contexts, or in one batch under a table of both models,
+    measured slower); GCUPS = cells / summed device time."""
     t0 = time.time()
     mt = synth.hdp_model("ACEGT", 5, seed=3)
     mc = synth.hdp_model("ACEGT", 5, seed=4)
@@ -249,7 +251,6 @@ def hdp_leg(native, synth, ctx, args):
                               complement_model=mc)
     log(f"hdp: generated {len(reads)} strand-reads in {time.time() - t0:.1f}s")
     params = native.Params(diagonal_expansion=50)
-    out = {}
     batches = []
     for strand, m in ((0, mt), (1, mc)):
         dm = ctx.model(m)

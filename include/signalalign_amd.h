@@ -157,6 +157,12 @@ typedef struct {
     double fb_ms;            /* forward + backward sweeps on the device timeline (they overlap) */
     int32_t emission_path;   /* 0: standard Gaussian tables, 1: general Gaussian, 2: HDP grid */
     int32_t replayed;        /* 1: the run replayed the batch's recorded graph */
+    /* the run's first and last device event on the device's timeline, in ms after an
+       event the library records once per device (at its first context): runs of
+       different contexts on one device are placed on one timeline (their overlap,
+       the span of several concurrent runs) */
+    double device_start_ms;
+    double device_end_ms;
 } sa_timing;
 
 /* One kernel launch of a run: forward sweeps of a group of sub-alignments, or

@@ -346,6 +346,9 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
     load_econ_uniform(pre, a, evc, 1 - X0 - woff, 1 - X0 - woff >= 0 && 1 - X0 - woff < lY);
     bool fresh = true;
     T mxp = 0; /* the renormalisation shift of d-1 (its edge cells crossed unshifted) */
+    constexpr bool PF = EK == 2;
+    HdpFetch<T> gp[PF ? CPT : 1];
+    bool have_gp = false;
     bool bad = false;
 
     for (int d = 1; d <= N; d++) {
@@ -412,6 +415,7 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
 #pragma unroll
             for (int c = 0; c < CPT; c++) { sink(kc[c]); sink(ev[c]); }
             fresh = true;
+            have_gp = false;
         }
         if (!fresh) {
             /* cell x on d reads event d-x-1: its left neighbour's event of d-1 */
@@ -447,7 +451,12 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
             const T mM = c ? qM[c - 1] : LpM, mXv = c ? qX[c - 1] : LpX, mYv = c ? qY[c - 1] : LpY;
             const T eX = em_gapx<T>(m, kc[c]);
             T eM, eY;
-            em_both<T>(m, kc[c], ev[c], s, iv, eM, eY);
+            if (PF && have_gp) {
+                eM = hdp_finish(m, gp[PF ? c : 0], iv);
+                eY = eM;
+            } else {
+                em_both<T>(m, kc[c], ev[c], s, iv, eM, eY);
+            }
             /* lower -> gapX, middle -> match, upper -> gapY */
             T X = lM + (eX + tGOX);
             X = logadd_lut(X, lXv + (eX + tGEX), lut);
@@ -465,6 +474,20 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
                 m3 = m3 > nY[c] ? m3 : nY[c];
                 mx = m3 > mx ? m3 : mx;
             }
+        }
+        if constexpr (PF) {
+            /* HDP: the density gathers of diagonal d+1 are issued here, before the
+               barrier and the stores, and used a diagonal later (two diagonals ahead
+               gained nothing: the wait before their use also covers every later
+               store and gather).  The events of d+1 are this diagonal's shifted one
+               lane; each wavefront's first lane takes `pre` */
+            const ECon<T, EK> inc = shr1(ev[CPT - 1], pre);
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                const T evm = c ? ev[c - 1].m : inc.m;
+                gp[c] = hdp_fetch(m, kc[c].row, kc[c].hmu, evm, s);
+            }
+            have_gp = true;
         }
         /* renormalisation: the wavefront maxima cross at the diagonal's barrier with
            the edge cells (unshifted; the next diagonal's reader applies the shift) */

@@ -235,6 +235,13 @@ inline bool split_points(const int64_t *anc, int64_t n, int64_t lX, int64_t lY, 
     });
 }
 
+/* The device's timeline anchor: an event recorded once per device, at its first
+   context (on that context's stream); nullptr before. */
+hipEvent_t timeline_anchor(int device);
+void timeline_init(int device, hipStream_t st);
+/* ms from the device's anchor to `ev` (a completed event on that device), 0 if none */
+double timeline_ms(int device, hipEvent_t ev);
+
 /* Resolves the no-allocation entry point's scratch array on the current device
    (sa_noalloc.cpp): its device address, or nullptr with the HIP error in *err. */
 unsigned char *scratch_setup(int device, hipError_t *err);

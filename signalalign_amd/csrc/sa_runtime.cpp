@@ -221,6 +221,9 @@ int upload_tables(sa_context *ctx, sa_model *m, DevTables<T> &t, const sa_model_
     return SA_OK;
 }
 
+std::mutex g_anchor_mu;
+hipEvent_t g_anchor[64] = {};
+
 int next_pow2(int v) {
     int r = 1;
     while (r < v) r <<= 1;
@@ -228,6 +231,36 @@ int next_pow2(int v) {
 }
 
 } // namespace
+
+namespace sa_rt {
+
+hipEvent_t timeline_anchor(int device) { return device >= 0 && device < 64 ? g_anchor[device] : nullptr; }
+
+void timeline_init(int device, hipStream_t st) {
+    if (device < 0 || device >= 64) return;
+    std::lock_guard<std::mutex> lk(g_anchor_mu);
+    if (g_anchor[device]) return;
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    if (hipEventRecord(e, st) != hipSuccess || hipEventSynchronize(e) != hipSuccess) {
+        (void)hipEventDestroy(e);
+        (void)hipGetLastError();
+        return;
+    }
+    g_anchor[device] = e;
+}
+
+double timeline_ms(int device, hipEvent_t ev) {
+    const hipEvent_t a = timeline_anchor(device);
+    float ms = 0.f;
+    if (!a || hipEventElapsedTime(&ms, a, ev) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0.0;
+    }
+    return ms;
+}
+
+} // namespace sa_rt
 
 extern "C" {
 
@@ -268,6 +301,7 @@ int sa_context_create_ex(int device, int n_streams, sa_context **out) {
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->n_cu = prop.multiProcessorCount;
+    sa_rt::timeline_init(device, c->stream);
     /* the no-allocation entry point's scratch array on this device (once per device) */
     hipError_t se = hipSuccess;
     c->scratch = sa_rt::scratch_setup(device, &se);
@@ -1244,6 +1278,8 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]); tm.prep_ms = ms;
     (void)hipEventElapsedTime(&ms, ctx->ev[5], ctx->ev[6]); tm.collect_ms = ms;
     (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[6]); tm.total_ms = ms;
+    tm.device_start_ms = sa_rt::timeline_ms(ctx->device, ctx->ev[0]);
+    tm.device_end_ms = sa_rt::timeline_ms(ctx->device, ctx->ev[6]);
     tm.cells = total_cells;
 
     /* record the launch sequence of this (now planned) batch as a graph: later runs
@@ -1740,6 +1776,8 @@ int run_paths(sa_batch *b, sa_result *res) {
     (void)hipEventElapsedTime(&ms, ctx->ev[3], ctx->ev[4]); tm.backward_ms = ms;
     (void)hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[4]); tm.fb_ms = ms;
     (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[4]); tm.total_ms = ms;
+    tm.device_start_ms = sa_rt::timeline_ms(ctx->device, ctx->ev[0]);
+    tm.device_end_ms = sa_rt::timeline_ms(ctx->device, ctx->ev[4]);
     int64_t tc = 0;
     for (int64_t c : cells) tc += c;
     tm.cells = tc;
@@ -1779,6 +1817,8 @@ int replay_batch(sa_batch *b, sa_result *res) {
     }
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, ctx->ev[12], ctx->ev[13]); tm.total_ms = ms;
+    tm.device_start_ms = sa_rt::timeline_ms(ctx->device, ctx->ev[12]);
+    tm.device_end_ms = sa_rt::timeline_ms(ctx->device, ctx->ev[13]);
     /* per-step times from the events the graph recorded (left 0 if unavailable) */
     if (hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]) == hipSuccess) tm.prep_ms = ms;
     if (hipEventElapsedTime(&ms, ctx->ev[5], ctx->ev[6]) == hipSuccess) tm.collect_ms = ms;

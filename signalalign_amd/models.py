@@ -81,6 +81,44 @@ class HdpModel:
         self.level_mean = np.asarray(level_mean, dtype=np.float64).copy()
 
 
+def merge_strand_models(mt, mc):
+    """One HDP table holding two models of the same k-mer length, grid and transitions
+    (a read's template and complement models): the alphabet doubles, the second
+    model's k-mers spelled in lower case (a complement strand's reference lower-cased
+    reads them), so a batch can hold reads of both strands under one resident table.
+    Each k-mer of one case keeps its model's density row and level mean; mixed-case
+    k-mers never occur and point at row 0.  Results equal those of each strand's
+    reads run with its own model."""
+    if mt.kmer_length != mc.kmer_length or mt.alphabet != mc.alphabet or not np.array_equal(mt.grid, mc.grid) \
+            or not np.array_equal(mt.transitions, mc.transitions):
+        raise ValueError("the models differ in k-mer length, alphabet, grid or transitions")
+    lower = mt.alphabet.lower()
+    if set(lower) & set(mt.alphabet):
+        raise ValueError("the alphabet has lower-case letters")
+    alpha = "".join(sorted(mt.alphabet + lower))
+    A, k = len(alpha), mt.kmer_length
+    a0 = len(mt.alphabet)
+    # digit of each sorted letter in its own model's sorted alphabet, and its case
+    pos = {ch: i for i, ch in enumerate(alpha)}
+    own = np.array([sorted(mt.alphabet).index(ch) if ch in mt.alphabet else sorted(lower).index(ch) for ch in alpha])
+    upper = np.array([ch in mt.alphabet for ch in alpha])
+    ids = np.arange(A ** k)
+    digits = np.stack([(ids // A ** (k - 1 - j)) % A for j in range(k)], axis=1)
+    all_up = upper[digits].all(axis=1)
+    all_lo = (~upper[digits]).all(axis=1)
+    sub = (own[digits] * (a0 ** np.arange(k - 1, -1, -1))[None, :]).sum(axis=1)
+    nrt = mt.density.shape[0]
+    kmer_row = np.zeros(A ** k, dtype=np.int64)
+    level_mean = np.zeros(A ** k)
+    kmer_row[all_up] = mt.kmer_row[sub[all_up]]
+    kmer_row[all_lo] = nrt + mc.kmer_row[sub[all_lo]]
+    level_mean[all_up] = mt.level_mean[sub[all_up]]
+    level_mean[all_lo] = mc.level_mean[sub[all_lo]]
+    del pos
+    return HdpModel(alpha, k, mt.transitions, mt.grid, np.concatenate([mt.density, mc.density]),
+                    np.concatenate([mt.slopes, mc.slopes]), kmer_row, level_mean)
+
+
 def load_model_file(path):
     """Parse a signalAlign .model file (impl/stateMachine.c:1328-1426)."""
     with open(path) as fh:

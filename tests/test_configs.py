@@ -100,6 +100,32 @@ def test_hdp_5mer_2000_point_grid_both_strands():
         ctx.close()
 
 
+def test_hdp_strands_in_one_batch_under_a_merged_table():
+    """Template and complement reads in one batch under models.merge_strand_models
+    (complement references lower-cased): every read's pairs and log-likelihood equal
+    its run with its own strand's model, bit for bit, FP32 and FP64."""
+    from signalalign_amd import models
+    ctx = native.Context(0)
+    try:
+        mt = synth.hdp_model("ACEGT", 5, seed=3)
+        mc = synth.hdp_model("ACEGT", 5, seed=4)
+        reads = synth.batch_reads(mt, 3, 3000, seed=78, methyl_cpg=0.3, strands=2, complement_model=mc)
+        params = native.Params(diagonal_expansion=50)
+        dmm = ctx.model(models.merge_strand_models(mt, mc))
+        dmt, dmc = ctx.model(mt), ctx.model(mc)
+        merged = [native.Read(r["ref"] if r["strand"] == 0 else r["ref"].lower(), r["events"], r["anchors"],
+                              r["scale"], r["shift"], r["var"]) for r in reads]
+        for prec in (native.SA_FP32, native.SA_FP64):
+            print(f"merged strands, precision {prec}", flush=True)
+            got, _ = ctx.align_batch(dmm, merged, params, prec)
+            for r, g in zip(reads, got):
+                want, _ = ctx.align_batch(dmt if r["strand"] == 0 else dmc, [P.native_read(r, True, True)],
+                                          params, prec)
+                assert np.array_equal(g.pairs, want[0].pairs) and g.log_likelihood == want[0].log_likelihood
+    finally:
+        ctx.close()
+
+
 def test_batched_acegt_6mer_uneven_lengths():
     """Batched configuration shapes: the 15625-k-mer ACEGT table, 5mC at 30% of CpG
     sites, lengths 1-24 kb in one batch (the long one on the several-wavefront sweep,
