@@ -715,6 +715,7 @@ def main():
         "expectations": expect,
         "degenerate": degenerate,
         "wall_ms_per_step_pcie_inclusive": round(float(np.mean(wall_ms)), 3),
+        "wall_gcups_pcie_inclusive": round(cells / float(np.mean(wall_ms)) / 1e6, 3),
     }
     if cpu:
         line["batched_vs_cpu"] = round(gcups / cpu["value"], 1)

@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -86,6 +87,40 @@ struct HBuf {
     template <typename U> U *as() const { return reinterpret_cast<U *>(p); }
 };
 
+/* Pinned buffers that large pair lists are copied into.  A result keeps the buffer
+   its pairs arrived in and hands it back when it is freed, so a batch run again and
+   again neither copies its pairs a second time on the host nor pins new memory.
+   Results hold the pool by shared_ptr: one may outlive its context. */
+struct PinnedPool {
+    static constexpr size_t kKeep = 2; /* free buffers kept; more are unpinned */
+    std::mutex mu;
+    std::vector<HBuf> idle;
+    HBuf take() {
+        std::lock_guard<std::mutex> lk(mu);
+        if (idle.empty()) return HBuf();
+        /* the largest: a repeated run needs the size it needed before */
+        auto it = std::max_element(idle.begin(), idle.end(),
+                                   [](const HBuf &a, const HBuf &b) { return a.cap < b.cap; });
+        HBuf h = *it;
+        idle.erase(it);
+        return h;
+    }
+    void give(HBuf h) {
+        if (!h.p) return;
+        std::lock_guard<std::mutex> lk(mu);
+        idle.push_back(h);
+        if (idle.size() > kKeep) {
+            auto it = std::min_element(idle.begin(), idle.end(),
+                                       [](const HBuf &a, const HBuf &b) { return a.cap < b.cap; });
+            it->release();
+            idle.erase(it);
+        }
+    }
+    ~PinnedPool() {
+        for (HBuf &h : idle) h.release();
+    }
+};
+
 
 template <typename T>
 struct DevTables {
@@ -108,6 +143,7 @@ struct sa_context {
     hipStream_t aux[3] = {};   /* sub-alignment groups of different window sizes run side by side */
     int n_streams = 4;         /* 1: everything on `stream` (aux[] alias it) */
     HBuf stage;                /* pinned staging for inputs and pairs (calls are serialised by mu) */
+    std::shared_ptr<sa_rt::PinnedPool> pair_pool = std::make_shared<sa_rt::PinnedPool>();
     std::mutex mu;
     char err[512] = {};        /* text of the last error (fixed storage: setting it allocates nothing) */
     hipEvent_t ev[16] = {};

@@ -122,13 +122,50 @@ struct ReadExpect {
 };
 
 struct sa_result {
-    std::vector<sa_pair> pairs;
+    std::vector<sa_pair> pairs; /* small lists and degenerate batches */
+    /* a large list stays in the pinned buffer it was copied into (pool-owned) */
+    HBuf pinned;
+    std::shared_ptr<sa_rt::PinnedPool> pool;
     std::vector<int32_t> paths; /* per pair, degenerate batches only */
     std::vector<ReadOut> reads;
     sa_timing timing;
     std::vector<sa_launch> launches;
     std::vector<ReadExpect> expect; /* expectations runs only */
+    const sa_pair *pair_data() const { return pinned.p ? pinned.as<sa_pair>() : pairs.data(); }
+    sa_result() = default;
+    sa_result(const sa_result &) = delete; /* the pinned buffer has one owner */
+    sa_result &operator=(const sa_result &) = delete;
+    ~sa_result() {
+        if (pool) pool->give(pinned);
+        else pinned.release();
+    }
 };
+
+/* Lists of at least this many pairs are handed over in pinned memory rather than
+   copied into the result (3 MiB; below it the copy costs less than a pinned
+   buffer held per live result). */
+constexpr int64_t kPinnedPairs = 1 << 18;
+
+/* Queues the device-to-host copy of n packed pairs on st into the buffer the
+   result will hold: a pinned buffer from the context's pool for large lists, else
+   the context's staging buffer (copied into res->pairs by finish_pairs after the
+   stream synchronises). */
+static int fetch_pairs(sa_context *ctx, sa_result *res, const void *d_packed, int64_t n, hipStream_t st) {
+    const size_t bytes = sizeof(sa_pair) * (size_t)std::max<int64_t>(n, 1);
+    HBuf *dst = &ctx->stage;
+    if (n >= kPinnedPairs) {
+        res->pool = ctx->pair_pool;
+        res->pinned = res->pool->take();
+        dst = &res->pinned;
+    }
+    SA_HIP(ctx, dst->ensure(bytes));
+    if (n) SA_HIP(ctx, hipMemcpyAsync(dst->p, d_packed, sizeof(sa_pair) * (size_t)n, hipMemcpyDeviceToHost, st));
+    return SA_OK;
+}
+
+static void finish_pairs(sa_context *ctx, sa_result *res, int64_t n) {
+    if (!res->pinned.p) res->pairs.assign(ctx->stage.as<sa_pair>(), ctx->stage.as<sa_pair>() + n);
+}
 
 struct sa_expect_result {
     std::vector<ReadExpect> reads;
@@ -1215,13 +1252,10 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
                                    b->d_y1.as<int32_t>(), (int)nw, st));
     SA_HIP(ctx, hipEventRecord(ctx->ev[6], st));
     std::vector<double> wtot(nw);
-    SA_HIP(ctx, ctx->stage.ensure(sizeof(sa_pair) * (size_t)std::max<int64_t>(total_pairs, 1)));
-    if (total_pairs)
-        SA_HIP(ctx, hipMemcpyAsync(ctx->stage.p, b->d_packed.p, sizeof(int32_t) * 3 * total_pairs,
-                                   hipMemcpyDeviceToHost, st));
+    if ((rc = fetch_pairs(ctx, res, b->d_packed.p, total_pairs, st))) return rc;
     SA_HIP(ctx, hipMemcpyAsync(wtot.data(), b->d_win_total.p, sizeof(double) * nw, hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipStreamSynchronize(st));
-    res->pairs.assign(ctx->stage.as<sa_pair>(), ctx->stage.as<sa_pair>() + total_pairs);
+    finish_pairs(ctx, res, total_pairs);
     if (expect) {
         /* per read: sub-alignments in split order, each window's records in call order */
         std::vector<double> etr(10 * nw), es0(nt * nk), es1(nt * nk);
@@ -1800,14 +1834,11 @@ int replay_batch(sa_batch *b, sa_result *res) {
     res->reads = gc.reads;
     std::vector<double> wtot(gc.nw);
     int32_t err = 0;
-    SA_HIP(ctx, ctx->stage.ensure(sizeof(sa_pair) * (size_t)std::max<int64_t>(gc.total_pairs, 1)));
-    if (gc.total_pairs)
-        SA_HIP(ctx, hipMemcpyAsync(ctx->stage.p, b->d_packed.p, sizeof(int32_t) * 3 * gc.total_pairs,
-                                   hipMemcpyDeviceToHost, st));
+    if (int rc = fetch_pairs(ctx, res, b->d_packed.p, gc.total_pairs, st)) return rc;
     SA_HIP(ctx, hipMemcpyAsync(wtot.data(), b->d_win_total.p, sizeof(double) * gc.nw, hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipMemcpyAsync(&err, b->d_err.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipStreamSynchronize(st));
-    res->pairs.assign(ctx->stage.as<sa_pair>(), ctx->stage.as<sa_pair>() + gc.total_pairs);
+    finish_pairs(ctx, res, gc.total_pairs);
     if (err) return fail(ctx, SA_ERR_HIP, "replayed run raised device error flags " + std::to_string(err));
     for (int64_t r = 0; r < b->n_reads; r++) {
         double ll = 0.0;
@@ -2025,7 +2056,7 @@ int64_t sa_result_n_reads(const sa_result *res) { return res ? (int64_t)res->rea
 int sa_result_read(const sa_result *res, int64_t i, sa_read_result *out) {
     if (!res || i < 0 || i >= (int64_t)res->reads.size()) return SA_ERR_INVALID_ARGUMENT;
     const ReadOut &r = res->reads[i];
-    out->pairs = r.n_pairs ? res->pairs.data() + r.first_pair : nullptr;
+    out->pairs = r.n_pairs ? res->pair_data() + r.first_pair : nullptr;
     out->n_pairs = r.n_pairs;
     out->log_likelihood = r.log_likelihood;
     out->cells = r.cells;

@@ -1,5 +1,7 @@
 """Batched runs replayed from a recorded HIP graph, and the launch autotuner: both
 change how the work is launched, never the results."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -69,3 +71,56 @@ def test_autotune_keeps_results(gpu_ctx, setup):
     finally:
         b.close()
 
+
+
+def _pairs_of(L, res):
+    out = []
+    rr = native._ReadResult()
+    for i in range(L.sa_result_n_reads(res)):
+        assert L.sa_result_read(res, i, ctypes.byref(rr)) == 0
+        if rr.n_pairs:
+            buf = np.ctypeslib.as_array(ctypes.cast(rr.pairs, ctypes.POINTER(ctypes.c_int32)),
+                                        shape=(rr.n_pairs * 3,))
+            out.append(buf.reshape(-1, 3).copy())
+        else:
+            out.append(np.zeros((0, 3), dtype=np.int32))
+    return out
+
+
+def test_large_results_stay_valid():
+    """Large pair lists (at least 2^18 pairs) arrive in pinned buffers that results
+    hand back to their context when freed: a result stays valid while its batch runs
+    again, while a later result's buffer is reused, and after its context is
+    destroyed."""
+    m = synth.gaussian_model("ACEGT", 6, seed=6)
+    reads = [P.native_read(r) for r in synth.batch_reads(m, 64, 6000, seed=402, methyl_cpg=0.3)]
+    params = native.Params(diagonal_expansion=50)
+    ctx = native.Context(0)
+    L = ctx._L
+    dm = ctx.model(m)
+    b = native.Batch(ctx, dm, reads, params)
+    res = [ctypes.c_void_p() for _ in range(3)]
+    try:
+        ctx._check(L.sa_batch_run(b._h, native.SA_FP32, ctypes.byref(res[0])), "run 0")
+        first = _pairs_of(L, res[0])
+        assert sum(len(p) for p in first) >= 1 << 18, "the batch must take the pinned path"
+        ctx._check(L.sa_batch_run(b._h, native.SA_FP32, ctypes.byref(res[1])), "run 1")
+        L.sa_result_free(res[1])  # its buffer goes back to the context
+        res[1] = ctypes.c_void_p()
+        ctx._check(L.sa_batch_run(b._h, native.SA_FP32, ctypes.byref(res[2])), "run 2")  # reuses it
+        for a, c in zip(first, _pairs_of(L, res[2])):
+            assert np.array_equal(a, c)
+        for a, c in zip(first, _pairs_of(L, res[0])):  # untouched by the later runs
+            assert np.array_equal(a, c)
+        b.close()
+        dm.close()
+        ctx.close()
+        for a, c in zip(first, _pairs_of(L, res[2])):  # outlives its context
+            assert np.array_equal(a, c)
+    finally:
+        for r in res:
+            if r:
+                L.sa_result_free(r)
+        b.close()
+        dm.close()
+        ctx.close()
