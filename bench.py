@@ -259,15 +259,21 @@ def hdp_leg(native, synth, ctx, args):
     ms, cells, tms = [], 0, []
     for step in range(args.warmup + args.steps + 1):
         tot, c = 0.0, 0
-        for _, b in batches:
+        for si, (_, b) in enumerate(batches):
             _, tm = b.run(native.SA_FP32, fetch=False)
             tot += tm["total_ms"]
             c += tm["cells"]
+            log(f"hdp step {step} strand {si}: device {tm['total_ms']:.1f} ms (prep {tm['prep_ms']:.1f} emission "
+                f"plane {tm['emission_ms']:.1f} fwd {tm['forward_ms']:.1f} bwd {tm['backward_ms']:.1f}), "
+                f"emission path {tm['emission_path']}")
             if step > args.warmup:
                 tms.append(tm)
         if step > args.warmup:  # the first runs plan and record each batch's graph
             ms.append(tot)
             cells = c
+    for L in tms[-1]["launches"]:
+        log(f"  hdp launch stream {L['stream']} {'fwd' if L['direction'] == 0 else 'bwd'} cpt {L['cpt']} waves "
+            f"{L['waves']} items {L['items']} cells {L['cells']} start {L['start_ms']:.1f} ms dur {L['ms']:.1f} ms")
     for dm, b in batches:
         b.close()
         dm.close()

@@ -155,7 +155,8 @@ typedef struct {
     int64_t forward_launches;
     int64_t backward_launches;
     double fb_ms;            /* forward + backward sweeps on the device timeline (they overlap) */
-    int32_t emission_path;   /* 0: standard Gaussian tables, 1: general Gaussian, 2: HDP grid */
+    int32_t emission_path;   /* 0: standard Gaussian tables, 1: general Gaussian, 2: HDP grid (gathered by
+                                the sweeps), 3: HDP grid through the emission plane */
     int32_t replayed;        /* 1: the run replayed the batch's recorded graph */
     /* the run's first and last device event on the device's timeline, in ms after an
        event the library records once per device (at its first context): runs of
@@ -163,6 +164,7 @@ typedef struct {
        the span of several concurrent runs) */
     double device_start_ms;
     double device_end_ms;
+    double emission_ms;      /* HDP emission plane, filled before the sweeps (part of total_ms) */
 } sa_timing;
 
 /* One kernel launch of a run: forward sweeps of a group of sub-alignments, or

@@ -32,7 +32,10 @@ namespace sa {
 
 /* EK 0: Gaussian tables as the .model loader builds them (extra-event table = match
  * table with level_sd x 1.75, one skip emission for every k-mer): the extra-event
- * emission follows from the match constants.  EK 1: any Gaussian tables.  EK 2: HDP. */
+ * emission follows from the match constants.  EK 1: any Gaussian tables.  EK 2: HDP,
+ * densities gathered by the sweep.  EK 3: HDP, each cell's density emission read
+ * from the emission plane hdp_plane_kernel filled (no per-position or per-event
+ * constants in registers). */
 template <typename T, int EK> struct KCon;
 template <typename T> struct KCon<T, 0> {
     T mmu, misd, mca, mnmu, mnb;
@@ -45,10 +48,17 @@ template <typename T> struct KCon<T, 2> {
     int32_t row;
 };
 
+template <typename T> struct KCon<T, 3> {};
+
 template <typename T, int EK> struct ECon;
 template <typename T> struct ECon<T, 0> { T m, n, l, i; };
 template <typename T> struct ECon<T, 1> { T m, n, l, i; };
 template <typename T> struct ECon<T, 2> { T m; };
+template <typename T> struct ECon<T, 3> {};
+
+template <typename T>
+__device__ __forceinline__ void load_kcon(KCon<T, 3> &, const Model<T> &, const int32_t *, int64_t, bool) {}
+template <typename T> __device__ __forceinline__ void load_econ(ECon<T, 3> &, const Args<T> &, int64_t, bool) {}
 
 /* k-mer (x-1) of cell x; an absent k-mer (x = 0 or past the end) gives -inf emissions */
 template <typename T>
@@ -115,6 +125,12 @@ template <typename T> __device__ __forceinline__ ECon<T, 2> shr1(const ECon<T, 2
 template <typename T> __device__ __forceinline__ ECon<T, 2> shl1(const ECon<T, 2> &e, const ECon<T, 2> &f) {
     return ECon<T, 2>{shl1(e.m, f.m)};
 }
+template <typename T> __device__ __forceinline__ ECon<T, 3> shr1(const ECon<T, 3> &, const ECon<T, 3> &) {
+    return ECon<T, 3>{};
+}
+template <typename T> __device__ __forceinline__ ECon<T, 3> shl1(const ECon<T, 3> &, const ECon<T, 3> &) {
+    return ECon<T, 3>{};
+}
 
 /* the match/gapX constants of the next position, for the backward sweep */
 template <typename T> __device__ __forceinline__ KCon<T, 0> shl1_match(const KCon<T, 0> &k) {
@@ -132,6 +148,7 @@ template <typename T> __device__ __forceinline__ KCon<T, 1> shl1_match(const KCo
 template <typename T> __device__ __forceinline__ KCon<T, 2> shl1_match(const KCon<T, 2> &k) {
     return KCon<T, 2>{shl1(k.hmu, (T)0), shl1(k.row, -1)};
 }
+template <typename T> __device__ __forceinline__ KCon<T, 3> shl1_match(const KCon<T, 3> &) { return KCon<T, 3>{}; }
 
 /* emissions from registers */
 template <typename T>
@@ -190,6 +207,8 @@ __device__ __forceinline__ void em_both(const Model<T> &m, const KCon<T, 2> &k, 
 }
 template <typename T>
 __device__ __forceinline__ T em_gapx(const Model<T> &, const KCon<T, 2> &) { return (T)-2.3025850929940455; }
+template <typename T>
+__device__ __forceinline__ T em_gapx(const Model<T> &, const KCon<T, 3> &) { return (T)-2.3025850929940455; }
 
 /* Uniform-address loads through the constant address space become scalar loads:
  * they are counted by lgkmcnt, so waiting for one does not wait for the vector
@@ -209,6 +228,8 @@ template <typename T>
 __device__ __forceinline__ void load_econ_uniform(ECon<T, 2> &e, const Args<T> &a, int64_t base, int j, bool ok) {
     e.m = ok ? ld_scalar(a.evm + base, j) : (T)0;
 }
+template <typename T>
+__device__ __forceinline__ void load_econ_uniform(ECon<T, 3> &, const Args<T> &, int64_t, int, bool) {}
 template <typename T, int EK>
 __device__ __forceinline__ void load_econ_uniform(ECon<T, EK> &e, const Args<T> &a, int64_t base, int j, bool ok) {
     if (ok) {
@@ -233,7 +254,9 @@ template <typename T> __device__ __forceinline__ void sink(const KCon<T, 1> &k) 
     sink(k.ymu); sink(k.yisd); sink(k.yca); sink(k.ynmu); sink(k.ynb);
 }
 template <typename T> __device__ __forceinline__ void sink(const KCon<T, 2> &k) { sink(k.hmu); sink(k.row); }
+template <typename T> __device__ __forceinline__ void sink(const KCon<T, 3> &) {}
 template <typename T> __device__ __forceinline__ void sink(const ECon<T, 2> &e) { sink(e.m); }
+template <typename T> __device__ __forceinline__ void sink(const ECon<T, 3> &) {}
 template <typename T, int EK> __device__ __forceinline__ void sink(const ECon<T, EK> &e) {
     sink(e.m); sink(e.n); sink(e.l); sink(e.i);
 }
@@ -349,6 +372,11 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
     constexpr bool PF = EK == 2;
     HdpFetch<T> gp[PF ? CPT : 1];
     bool have_gp = false;
+    /* emission plane: the lane's cells of the next diagonal, loaded a diagonal ahead */
+    constexpr bool PL = EK == 3;
+    const T *em = PL ? a.em + t.cell_off : nullptr;
+    T pe[PL ? CPT : 1];
+    bool have_pe = false;
     bool bad = false;
 
     for (int d = 1; d <= N; d++) {
@@ -416,6 +444,7 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
             for (int c = 0; c < CPT; c++) { sink(kc[c]); sink(ev[c]); }
             fresh = true;
             have_gp = false;
+            have_pe = false;
         }
         if (!fresh) {
             /* cell x on d reads event d-x-1: its left neighbour's event of d-1 */
@@ -451,7 +480,10 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
             const T mM = c ? qM[c - 1] : LpM, mXv = c ? qX[c - 1] : LpX, mYv = c ? qY[c - 1] : LpY;
             const T eX = em_gapx<T>(m, kc[c]);
             T eM, eY;
-            if (PF && have_gp) {
+            if constexpr (PL) {
+                eM = have_pe ? pe[PL ? c : 0] : valid ? em[run + (x - lo)] : (T)0;
+                eY = eM;
+            } else if (PF && have_gp) {
                 eM = hdp_finish(m, gp[PF ? c : 0], iv);
                 eY = eM;
             } else {
@@ -488,6 +520,16 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
                 gp[c] = hdp_fetch(m, kc[c].row, kc[c].hmu, evm, s);
             }
             have_gp = true;
+        }
+        if constexpr (PL) {
+            /* the emissions of d+1, read before the barrier and the stores */
+            const int64_t nrun = run + (hi - lo + 1);
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                const int x = X0 + CPT * gl + c;
+                pe[c] = d < N && x >= nlo && x <= nhi ? em[nrun + (x - nlo)] : (T)0;
+            }
+            have_pe = true;
         }
         /* renormalisation: the wavefront maxima cross at the diagonal's barrier with
            the edge cells (unshifted; the next diagonal's reader applies the shift) */
@@ -668,6 +710,11 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
     constexpr bool PF = EK == 2;
     HdpFetch<T> gp[PF ? CPT : 1];
     bool have_gp = false;
+    /* emission plane: the lane's cells of the next diagonal, loaded a diagonal ahead */
+    constexpr bool PL = EK == 3;
+    const T *em = PL ? a.em + t.cell_off : nullptr;
+    T pe[PL ? CPT : 1];
+    bool have_pe = false;
     int64_t run = st.run, runxy = st.runxy;
     double off1 = st.off1;
     int lo1 = st.lo1, lo2 = st.lo2;
@@ -723,6 +770,7 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
             for (int c = 0; c < CPT; c++) { sink(kc[c]); sink(ev[c]); }
             fresh = true;
             have_gp = false;
+            have_pe = false;
         }
         if (!fresh) {
             /* cell x on d reads event d-x-1: its left neighbour's event of d-1 */
@@ -748,7 +796,10 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
             const T mM = c ? qM[c - 1] : LpM, mXv = c ? qX[c - 1] : LpX, mYv = c ? qY[c - 1] : LpY;
             const T eX = em_gapx<T>(m, kc[c]);
             T eM, eY;
-            if (PF && have_gp) {
+            if constexpr (PL) {
+                eM = have_pe ? pe[PL ? c : 0] : valid ? em[run + (x - lo)] : (T)0;
+                eY = eM;
+            } else if (PF && have_gp) {
                 eM = hdp_finish(m, gp[PF ? c : 0], iv);
                 eY = eM;
             } else {
@@ -799,6 +850,16 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
                 gp[c] = hdp_fetch(m, kc[c].row, kc[c].hmu, evm, s);
             }
             have_gp = true;
+        }
+        if constexpr (PL) {
+            /* the emissions of d+1, read before this diagonal's stores */
+            const int64_t nrun = run + (hi - lo + 1);
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                const int x = X0 + CPT * lane + c;
+                pe[c] = d < N && x >= nlo && x <= nhi ? em[nrun + (x - nlo)] : (T)0;
+            }
+            have_pe = true;
         }
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
@@ -1042,8 +1103,21 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
        diagonal ahead */
     constexpr bool PF = EK == 2;
     HdpFetch<T> gp[PF ? CPT : 1];
-    T eYp[PF ? CPT : 1];
+    /* emission plane (EK 3): the same reuse, the extra-event emissions of e-1's cells
+       (E(e, x)) loaded a diagonal ahead */
+    constexpr bool PL = EK == 3;
+    constexpr bool PE = PF || PL;
+    const T *em = PL ? a.em + t.cell_off : nullptr;
+    T pe[PL ? CPT : 1];
+    bool have_pe = false;
+    T eYp[PE ? CPT : 1];
     bool have_gp = false, have_eY = false;
+    /* E(dd, xx) straight from the plane (window tops and slides) */
+    auto plane_at = [&](int dd, int xx) -> T {
+        if (dd > lX + lY) return (T)0;
+        const int l = ld_scalar(xlo, dd), h = ld_scalar(xhi, dd), o = ld_scalar(doff, dd);
+        return xx >= l && xx <= h ? em[o + (xx - l)] : (T)0;
+    };
     /* the forward match states of the next diagonal's cells for its posteriors, loaded
        a diagonal ahead (they would otherwise stall every diagonal on HBM latency) */
     T fpre[CPT];
@@ -1093,6 +1167,7 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
         T nM[CPT], nX[CPT], nY[CPT];
         T LowM = NI, LowX = NI; /* lane i+1's first cell of e+1 */
         double obe = 0.0;
+        T eYc[PE ? CPT : 1];
         if (e == D) {
 #pragma unroll
             for (int c = 0; c < CPT; c++) {
@@ -1127,6 +1202,7 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                 for (int c = 0; c < CPT; c++) { sink(kc[c]); sink(ev[c]); }
                 fresh = true;
                 have_gp = false;
+                have_pe = false;
                 have_eY = false;
                 have_fpre = false;
                 have_rpre = false;
@@ -1143,8 +1219,7 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
             LowM = shl1(pM[0], NI);
             LowX = shl1(pX[0], NI);
             const KCon<T, EK> kn = shl1_match(kc[0]); /* k-mer x of lane's last cell */
-            T eYc[PF ? CPT : 1];
-            const T eYn = PF && have_eY ? shl1(eYp[0], NI) : NI;
+            const T eYn = PE && have_eY ? shl1(eYp[0], NI) : NI;
             const bool renorm = NORM && ((D - e) & 7) == 0;
             T mx = NI;
 #pragma unroll
@@ -1157,7 +1232,11 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                 const T CmidM = c < CPT - 1 ? qM[c + 1] : RpM;
                 const T eX = em_gapx<T>(m, kx);
                 T eM, eY;
-                if constexpr (PF) {
+                if constexpr (PL) {
+                    eY = have_pe ? pe[PL ? c : 0] : plane_at(e + 1, x);
+                    eM = have_eY ? (c < CPT - 1 ? eYp[c < CPT - 1 ? c + 1 : 0] : eYn) : plane_at(e + 2, x + 1);
+                    eYc[c] = eY;
+                } else if constexpr (PF) {
                     eY = have_gp ? hdp_finish(m, gp[c], iv) : em_gapy<T>(m, kc[c], ev[c], s, iv);
                     eM = have_eY ? (c < CPT - 1 ? eYp[c < CPT - 1 ? c + 1 : 0] : eYn) : em_match<T>(m, kx, ev[c], s, iv);
                     eYc[c] = eY;
@@ -1183,7 +1262,7 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                     mx = m3 > mx ? m3 : mx;
                 }
             }
-            if constexpr (PF) {
+            if constexpr (PE) {
 #pragma unroll
                 for (int c = 0; c < CPT; c++) eYp[c] = eYc[c];
                 have_eY = true;
@@ -1219,6 +1298,15 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
             }
             have_gp = true;
         }
+        if constexpr (PL) {
+            /* E(e, x): the extra-event emissions of e-1's cells */
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                const int x = X0 + CPT * lane + c;
+                pe[c] = x >= lo && x <= hi ? em[dof + (x - lo)] : (T)0;
+            }
+            have_pe = true;
+        }
 
         if (e <= TF) {
             const double offe = NORM ? fo + obe : 0.0;
@@ -1249,7 +1337,9 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                     for (int c = 0; c < CPT; c++) {
                         const int x = X0 + CPT * lane + c; /* cell (x, e+1-x); middle (x-1, e-x) on e-1 */
                         if (x >= lo1e && x <= hi1e) {
-                            const T eM = em_match<T>(m, kc[c], ev[c], s, iv);
+                            T eM;
+                            if constexpr (PL) eM = eYc[PL ? c : 0]; /* E(e+1, x) */
+                            else eM = em_match<T>(m, kc[c], ev[c], s, iv);
                             T Mv = rgm[c] + (eM + tMC);
                             Mv = logadd_lut(Mv, rgx[c] + (eM + tMFX), lut);
                             Mv = logadd_lut(Mv, rgy[c] + (eM + tMFY), lut);
@@ -1411,6 +1501,79 @@ backward_reg_kernel(Args<T> a, const int32_t *__restrict__ order) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* HDP emission plane                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* The density emission of every band cell of the listed sub-alignments, in the
+ * layout of the forward match plane (cell x of diagonal d at cell_off + doff[d] +
+ * x - xlo[d]).  Cell x on d: k-mer x-1 against event d-x-1, the value the sweeps'
+ * own gathers give (hdp_emit_row: emissions_signal_getHdpKmerDensity,
+ * impl/stateMachine.c:468-495; the match and extra-event emissions are the same
+ * density); -inf outside the k-mers, an absent event reads mean 0, as in the
+ * sweeps.  The register sweeps then read a cell's emission with one coalesced load
+ * a diagonal ahead instead of gathering from the density grid on their dependency
+ * chain, and the backward pass does not gather again.  One workgroup per chunk of
+ * kPlaneDiags diagonals, one diagonal per wavefront at a time, two cells per lane
+ * in flight. */
+template <typename T>
+__global__ void __launch_bounds__(256) hdp_plane_kernel(Args<T> a, const int32_t *__restrict__ tasks,
+                                                        const int32_t *__restrict__ chunk0, int n) {
+    const int b = blockIdx.x;
+    int lo = 0, hi = n - 1; /* the listed task whose chunks hold b */
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (chunk0[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const TaskDev t = a.tasks[tasks[lo]];
+    const Model<T> &m = a.m;
+    const int N = t.lX + t.lY;
+    const int d0 = (b - chunk0[lo]) * kPlaneDiags;
+    const int32_t *xlo = a.xlo + t.diag_off;
+    const int32_t *xhi = a.xhi + t.diag_off;
+    const int32_t *doff = a.doff + t.diag_off;
+    const int32_t *kid = a.kid + t.pos_off;
+    const T *evm = a.evm + t.evc_off;
+    T *em = a.em + t.cell_off;
+    const T s = (T)(t.scale / t.var);
+    const T iv = (T)(1.0 / t.var);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    auto emit = [&](int d, int x) -> T {
+        if (x < 1 || x > t.lX) return ninf<T>();
+        const int j = d - x - 1;
+        const T e = j >= 0 && j < t.lY ? evm[j] : (T)0;
+        const int32_t id = kid[x - 1];
+        return hdp_emit_row(m, m.hrow[id], m.hmu[id], e, s, iv);
+    };
+    for (int dd = wave; dd < kPlaneDiags; dd += 4) {
+        const int d = d0 + dd;
+        if (d > N) break;
+        const int l = ld_scalar(xlo, d), h = ld_scalar(xhi, d);
+        T *row = em + ld_scalar(doff, d) - l;
+        for (int x = l + lane; x <= h; x += 128) {
+            const int x2 = x + 64;
+            const T v1 = emit(d, x);
+            const T v2 = x2 <= h ? emit(d, x2) : (T)0;
+            row[x] = v1;
+            if (x2 <= h) row[x2] = v2;
+        }
+    }
+}
+
+template <typename T>
+hipError_t launch_hdp_plane(const Args<T> &a, const int32_t *tasks, const int32_t *chunk0, int n, int n_chunks,
+                            hipStream_t st) {
+    if (n == 0 || n_chunks == 0) return hipSuccess;
+    hipLaunchKernelGGL(hdp_plane_kernel<T>, dim3(n_chunks), dim3(256), 0, st, a, tasks, chunk0, n);
+    return hipGetLastError();
+}
+template hipError_t launch_hdp_plane<float>(const Args<float> &, const int32_t *, const int32_t *, int, int,
+                                            hipStream_t);
+template hipError_t launch_hdp_plane<double>(const Args<double> &, const int32_t *, const int32_t *, int, int,
+                                             hipStream_t);
+
+/* ------------------------------------------------------------------------ */
 /* launchers                                                                 */
 /* ------------------------------------------------------------------------ */
 
@@ -1443,7 +1606,8 @@ static hipError_t fwd_reg_ek(const Args<T> &a, const int32_t *order, int n, int 
     switch (ek) {
     case 0: return fwd_reg_inst<T, CPT, 0>(a, order, n, nw, shmem, st);
     case 1: return fwd_reg_inst<T, CPT, 1>(a, order, n, nw, shmem, st);
-    default: return fwd_reg_inst<T, CPT, 2>(a, order, n, nw, shmem, st);
+    case 2: return fwd_reg_inst<T, CPT, 2>(a, order, n, nw, shmem, st);
+    default: return fwd_reg_inst<T, CPT, 3>(a, order, n, nw, shmem, st);
     }
 }
 
@@ -1455,7 +1619,8 @@ static hipError_t bwd_reg_ek(const Args<T> &a, const int32_t *order, int n, int 
     switch (ek) {
     case 0: hipLaunchKernelGGL((backward_reg_kernel<T, CPT, 0>), dim3(n), dim3(64), shmem, st, a, order); break;
     case 1: hipLaunchKernelGGL((backward_reg_kernel<T, CPT, 1>), dim3(n), dim3(64), shmem, st, a, order); break;
-    default: hipLaunchKernelGGL((backward_reg_kernel<T, CPT, 2>), dim3(n), dim3(64), shmem, st, a, order); break;
+    case 2: hipLaunchKernelGGL((backward_reg_kernel<T, CPT, 2>), dim3(n), dim3(64), shmem, st, a, order); break;
+    default: hipLaunchKernelGGL((backward_reg_kernel<T, CPT, 3>), dim3(n), dim3(64), shmem, st, a, order); break;
     }
     return hipGetLastError();
 }

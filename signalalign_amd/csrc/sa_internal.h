@@ -156,6 +156,7 @@ struct sa_context {
 
 struct sa_model {
     sa_context *ctx = nullptr;
+    int device = 0;
     int32_t kind = 0;
     int32_t k = 0;
     int32_t A = 0;
@@ -175,6 +176,13 @@ struct sa_model {
 };
 
 namespace sa_rt {
+
+/* An entry point's first step: the context's device current, and no error left over
+   from an earlier call that a launch check of this one would pick up */
+inline void enter(const sa_context *ctx) {
+    (void)hipSetDevice(ctx->device);
+    (void)hipGetLastError();
+}
 
 inline int fail(sa_context *ctx, int code, const char *msg) {
     if (ctx) std::snprintf(ctx->err, sizeof(ctx->err), "%s", msg);

@@ -70,6 +70,11 @@ struct Args {
     /* forward planes: match state of every cell; gapX/gapY states of the
        diagonals the total-probability refresh reads (d and d-1 of each refresh) */
     T *fM, *fX, *fY;
+    /* HDP emission plane (register sweeps with EK 3): the density emission of each
+       cell, laid out like fM; em_plane asks prep_kernel for the per-diagonal plane
+       offsets (doff) the plane kernel needs before any forward sweep runs */
+    T *em;
+    int32_t em_plane;
     /* window outputs: 3 int32 (p, x, y) per pair */
     int32_t *out;
     int32_t *win_count;
@@ -151,6 +156,12 @@ hipError_t launch_forward_reg(const Args<T> &a, const int32_t *order, int n, int
 template <typename T>
 hipError_t launch_backward_reg(const Args<T> &a, const int32_t *order, int n, int cpt, int ek, size_t shmem,
                                hipStream_t st);
+/* the HDP emission plane of n listed sub-alignments (tasks[i]); their chunks of
+   kPlaneDiags diagonals are numbered from chunk0[i] (chunk0[n] in all) */
+constexpr int kPlaneDiags = 32;
+template <typename T>
+hipError_t launch_hdp_plane(const Args<T> &a, const int32_t *tasks, const int32_t *chunk0, int n, int n_chunks,
+                            hipStream_t st);
 /* zeroes n ints (a kernel, so that it can sit in a captured graph) */
 hipError_t launch_zero(int32_t *p, int n, hipStream_t st);
 hipError_t launch_collect(const WinDev *wins, const int32_t *win_count, const int64_t *win_dst, const int32_t *src,

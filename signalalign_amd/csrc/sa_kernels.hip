@@ -92,6 +92,33 @@ __global__ void __launch_bounds__(256) prep_kernel(Args<T> a) {
     atomicMax(&s_maxw, maxw);
     if (err) atomicOr(&s_err, err);
     __syncthreads();
+    if (a.em_plane) {
+        /* the forward match plane's per-diagonal offsets (exclusive prefix sums of the
+           band widths), which the forward sweeps also write as they go: the HDP
+           emission plane shares the layout and is filled before they run */
+        __shared__ int s_wsum[4];
+        __shared__ int s_carry;
+        if (tid == 0) s_carry = 0;
+        __syncthreads();
+        const int32_t *bl = a.xlo + t.diag_off, *bh = a.xhi + t.diag_off;
+        for (int64_t base = 0; base <= N; base += nt) {
+            const int64_t d = base + tid;
+            const int w = d <= N ? bh[d] - bl[d] + 1 : 0;
+            int v = w; /* inclusive scan over the wavefront */
+            for (int o = 1; o < 64; o <<= 1) {
+                const int u = __shfl_up(v, o, 64);
+                if ((tid & 63) >= o) v += u;
+            }
+            if ((tid & 63) == 63) s_wsum[tid >> 6] = v;
+            __syncthreads();
+            int pre = s_carry;
+            for (int k = 0; k < (tid >> 6); k++) pre += s_wsum[k];
+            if (d <= N) a.doff[t.diag_off + d] = pre + v - w;
+            __syncthreads();
+            if (tid == nt - 1) s_carry = pre + v;
+            __syncthreads();
+        }
+    }
     if (tid == 0) {
         const int32_t *xlo = a.xlo + t.diag_off;
         const int32_t *xhi = a.xhi + t.diag_off;

@@ -484,6 +484,7 @@ extern "C" int sa_align_noalloc(sa_context *ctx, const sa_model *model, const sa
     HostPlan &H = g_plan[ctx->device];
     std::lock_guard<std::mutex> dk(H.mu);
     SA_HIP(ctx, hipSetDevice(ctx->device));
+    (void)hipGetLastError(); /* no earlier call's error reaches the launch checks */
     sa_params p;
     if (params) p = *params; else sa_params_default(&p);
     /* getPosteriorProbsWithBanding prerequisites (impl/pairwiseAligner.c:1291-1295) */

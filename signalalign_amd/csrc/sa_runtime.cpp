@@ -67,6 +67,7 @@ struct sa_batch {
     DBuf d_fM, d_fX, d_fY;
     DBuf d_wins, d_task_order, d_win_order, d_out, d_win_count, d_win_total, d_win_dst, d_packed;
     DBuf d_ring_g; /* LDS sweeps' rings in HBM when a band is too wide for LDS */
+    DBuf d_em, d_em_tasks, d_em_chunk0; /* HDP emission plane, its sub-alignments and chunk numbering */
     /* degenerate references ('X' with options): multi-path sweeps, FP64 */
     int32_t paths = 0;
     std::string options;      /* the bases an 'X' stands for */
@@ -102,6 +103,7 @@ struct sa_batch {
                        &d_task_xycells, &d_task_maxw, &d_task_span,
                        &d_task_nwin, &d_win_slots, &d_err, &d_fM, &d_fX, &d_fY, &d_wins, &d_task_order,
                        &d_win_order, &d_out, &d_win_count, &d_win_total, &d_win_dst, &d_packed, &d_ring_g,
+                       &d_em, &d_em_tasks, &d_em_chunk0,
                        &d_ex_trans, &d_ex_s0, &d_ex_s1, &d_ex_obs, &d_ex_rec, &d_ex_count,
                        &d_pcol, &d_pent, &d_pplane, &d_pp, &d_psa, &d_psb, &d_pecol, &d_pekid, &d_pdoff,
                        &d_pdiag, &d_pentr, &d_pmaxent};
@@ -354,7 +356,7 @@ int sa_context_create(int device, sa_context **out) { return sa_context_create_e
 
 void sa_context_destroy(sa_context *ctx) {
     if (!ctx) return;
-    (void)hipSetDevice(ctx->device);
+    sa_rt::enter(ctx);
     for (auto &e : ctx->ev) if (e) (void)hipEventDestroy(e);
     for (auto &e : ctx->gev) if (e) (void)hipEventDestroy(e);
     for (auto &q : ctx->aux) if (q && q != ctx->stream) (void)hipStreamDestroy(q);
@@ -369,7 +371,7 @@ int sa_model_create(sa_context *ctx, const sa_model_desc *d, sa_model **out) {
     *out = nullptr;
     if (!ctx || !d || !d->alphabet) return SA_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lk(ctx->mu);
-    (void)hipSetDevice(ctx->device);
+    sa_rt::enter(ctx);
     if (d->kind < 0 || d->kind > 2) return fail(ctx, SA_ERR_UNSUPPORTED, "unknown emission kind");
     std::string alpha(d->alphabet);
     std::sort(alpha.begin(), alpha.end()); /* sequence_prepareAlphabet (impl/pairwiseAligner.c:294-323) */
@@ -386,6 +388,7 @@ int sa_model_create(sa_context *ctx, const sa_model_desc *d, sa_model **out) {
         return fail(ctx, SA_ERR_INVALID_ARGUMENT, "HDP grid incomplete");
     sa_model *m = new sa_model();
     m->ctx = ctx;
+    m->device = ctx->device;
     m->kind = d->kind;
     m->k = d->kmer_length;
     m->A = (int32_t)alpha.size();
@@ -448,7 +451,7 @@ int sa_model_kmer_length(const sa_model *m) { return m ? m->k : 0; }
 
 void sa_model_destroy(sa_model *m) {
     if (!m) return;
-    (void)hipSetDevice(m->ctx->device);
+    (void)hipSetDevice(m->device); /* (its context may be gone already) */
     m->t64.release();
     m->t32.release();
     m->d_lut.release();
@@ -632,7 +635,7 @@ int sa_batch_create(sa_context *ctx, const sa_model *model, const sa_read *reads
     *out = nullptr;
     if (!ctx || !model || (!reads && n_reads > 0) || n_reads < 0) return SA_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> lk(ctx->mu);
-    (void)hipSetDevice(ctx->device);
+    sa_rt::enter(ctx);
     AllocStream as(ctx->stream);
     sa_batch *b = new sa_batch();
     const int rc = batch_fill(ctx, b, model, reads, n_reads, params);
@@ -649,7 +652,7 @@ int sa_batch_refill(sa_batch *b, const sa_model *model, const sa_read *reads, in
     if (!b || !model || (!reads && n_reads > 0) || n_reads < 0) return SA_ERR_INVALID_ARGUMENT;
     sa_context *ctx = b->ctx;
     std::lock_guard<std::mutex> lk(ctx->mu);
-    (void)hipSetDevice(ctx->device);
+    sa_rt::enter(ctx);
     AllocStream as(ctx->stream);
     const int rc = batch_fill(ctx, b, model, reads, n_reads, params);
     if (rc != SA_OK) batch_clear(b);
@@ -731,7 +734,7 @@ void sa_batch_destroy(sa_batch *b) {
     if (!b) return;
     sa_context *ctx = b->ctx;
     std::lock_guard<std::mutex> lk(ctx->mu);
-    (void)hipSetDevice(ctx->device);
+    sa_rt::enter(ctx);
     AllocStream as(ctx->stream); /* buffers go back to the pool on the context's stream */
     delete b;
 }
@@ -820,6 +823,12 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     a.expect = expect ? 1 : 0;
     a.ex_rec_threshold = b->ex_rec_threshold;
     a.paths = b->paths;
+    /* HDP: the register sweeps read each cell's density emission from a plane filled
+       before them (SA_HDP_PLANE=0: they gather the densities themselves) */
+    {
+        const char *hp = getenv("SA_HDP_PLANE");
+        a.em_plane = model->ek == 2 && !force_lds && !(hp && std::strcmp(hp, "0") == 0) ? 1 : 0;
+    }
 
     SA_HIP(ctx, hipEventRecord(ctx->ev[0], st));
     SA_HIP(ctx, sa::launch_prep<T>(a, (int)nt, st));
@@ -1033,6 +1042,28 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     SA_HIP(ctx, b->d_fX.ensure(sizeof(T) * std::max<int64_t>(total_xy, 1)));
     SA_HIP(ctx, b->d_fY.ensure(sizeof(T) * std::max<int64_t>(total_xy, 1)));
     int rc;
+    /* HDP emission plane over the sub-alignments of the register sweeps, numbered in
+       chunks of kPlaneDiags diagonals */
+    std::vector<int32_t> em_tasks, em_chunk0;
+    if (a.em_plane) {
+        int64_t nch = 0;
+        for (int64_t i = 0; i < nt; i++)
+            if (fgrp[i] != 0) {
+                em_tasks.push_back((int32_t)i);
+                em_chunk0.push_back((int32_t)nch);
+                nch += (nd[i] + 1 + sa::kPlaneDiags - 1) / sa::kPlaneDiags;
+            }
+        if (nch > INT32_MAX) return fail(ctx, SA_ERR_UNSUPPORTED, "emission plane: too many diagonals");
+        em_chunk0.push_back((int32_t)nch);
+        if (!em_tasks.empty()) {
+            SA_HIP(ctx, b->d_em.ensure(sizeof(T) * std::max<int64_t>(total_cells, 1)));
+            if ((rc = upload(ctx, b->d_em_tasks, em_tasks)) || (rc = upload(ctx, b->d_em_chunk0, em_chunk0)))
+                return rc;
+            a.em = b->d_em.as<T>();
+        }
+    }
+    const int n_em = (int)em_tasks.size();
+    const int em_chunks = n_em ? em_chunk0.back() : 0;
     if ((rc = upload(ctx, b->d_tasks, tasks)) || (rc = upload(ctx, b->d_wins, wins)) ||
         (rc = upload(ctx, b->d_task_order, task_order)) || (rc = upload(ctx, b->d_win_order, win_order)))
         return rc;
@@ -1075,7 +1106,7 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     int64_t n_lds = 0;
     for (const Launch &L : fl) if (L.group == 0) n_lds += L.n;
     const int lds_fnt = (n_lds >= 4 * std::max(ctx->n_cu, 1) || lds_maxw <= 64) && lds_maxw <= 256 ? 64 : 256;
-    const int ek = model->ek;
+    const int ek = n_em ? 3 : model->ek;
     if ((int64_t)(fl.size() + bl.size()) * 2 > (int64_t)(sizeof(ctx->gev) / sizeof(ctx->gev[0])))
         return fail(ctx, SA_ERR_HIP, "too many launches for the timing events");
 
@@ -1115,6 +1146,12 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     /* fork on ev[2], per-launch events gev[2k], gev[2k+1] (forward launches first),
        join on ev[4] */
     auto run_sweeps = [&](bool forward) -> int {
+        if (forward && n_em) {
+            SA_HIP(ctx, hipEventRecord(ctx->ev[14], st));
+            SA_HIP(ctx, sa::launch_hdp_plane<T>(a, b->d_em_tasks.as<int32_t>(), b->d_em_chunk0.as<int32_t>(), n_em,
+                                                em_chunks, st));
+            SA_HIP(ctx, hipEventRecord(ctx->ev[15], st));
+        }
         SA_HIP(ctx, hipEventRecord(ctx->ev[2], st));
         for (int q = 1; q < kLanes; q++)
             if (gst[q] != st) SA_HIP(ctx, hipStreamWaitEvent(gst[q], ctx->ev[2], 0));
@@ -1220,7 +1257,12 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
         SA_HIP(ctx, hipMemcpyAsync(wcount.data(), b->d_win_count.p, sizeof(int32_t) * nw, hipMemcpyDeviceToHost, st));
         SA_HIP(ctx, hipStreamSynchronize(st));
     }
-    tm.emission_path = model->ek;
+    tm.emission_path = ek;
+    if (n_em) {
+        float ems = 0.f;
+        (void)hipEventElapsedTime(&ems, ctx->ev[14], ctx->ev[15]);
+        tm.emission_ms = ems;
+    }
     /* final positions in the reference's list order: per read, sub-alignments in
        split order, each in reverse emission order */
     std::vector<int64_t> win_dst(nw);
@@ -1349,7 +1391,7 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
             gc.nw = nw;
             gc.total_pairs = total_pairs;
             gc.cells = total_cells;
-            gc.ek = model->ek;
+            gc.ek = ek;
         } else {
             gc.reset();
             (void)hipGetLastError();
@@ -1882,6 +1924,7 @@ int replay_batch(sa_batch *b, sa_result *res) {
     tm.backward_launches = (int64_t)(gc.launches.size() - nf);
     tm.cells = gc.cells;
     tm.emission_path = gc.ek;
+    if (gc.ek == 3 && hipEventElapsedTime(&ms, ctx->ev[14], ctx->ev[15]) == hipSuccess) tm.emission_ms = ms;
     tm.replayed = 1;
     return SA_OK;
 }
@@ -1895,7 +1938,7 @@ int sa_batch_run(sa_batch *b, sa_precision precision, sa_result **out) {
     if (!b) return SA_ERR_INVALID_ARGUMENT;
     sa_context *ctx = b->ctx;
     std::lock_guard<std::mutex> lk(ctx->mu);
-    (void)hipSetDevice(ctx->device);
+    sa_rt::enter(ctx);
     AllocStream as(ctx->stream);
     sa_result *res = new sa_result();
     const char *kern = getenv("SA_KERNELS");

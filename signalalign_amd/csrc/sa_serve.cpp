@@ -15,6 +15,7 @@
  * in every shape).
  */
 #include "signalalign_amd.h"
+#include "sa_internal.h"
 
 #include <hip/hip_runtime.h>
 
@@ -252,7 +253,7 @@ namespace {
    expansion (band_construct keeps each diagonal within the anchors around it):
    O(anchors), cheap enough for the submitting thread. */
 int64_t estimate_bytes(const sa_read *r, int kmer_length, const sa_params *p, sa_precision prec,
-                       double paths_per_column) {
+                       double paths_per_column, bool emission_plane) {
     const int64_t lX = r->ref_length >= kmer_length ? r->ref_length - kmer_length + 1 : 0;
     const int64_t lY = r->n_events;
     const int64_t e = p->diagonal_expansion;
@@ -267,7 +268,9 @@ int64_t estimate_bytes(const sa_read *r, int kmer_length, const sa_params *p, sa
     const int64_t w = prec == SA_FP32 ? 4 : 8;
     /* a degenerate reference keeps three FP64 states of every (cell, path) entry and
        per-entry and per-diagonal path tables */
-    const int64_t cell_bytes = paths_per_column > 0.0 ? (int64_t)(paths_per_column * 48.0) + 1 : w * 3 / 2;
+    /* (an HDP model's register sweeps add one emission per cell) */
+    const int64_t cell_bytes = paths_per_column > 0.0 ? (int64_t)(paths_per_column * 48.0) + 1
+                             : w * 3 / 2 + (emission_plane ? w : 0);
     const int64_t diag_bytes = paths_per_column > 0.0 ? 64 : 32;
     /* a band too wide for a register window runs on the LDS sweeps, whose rings move to
        per-workgroup slices of HBM when they outgrow LDS: one forward slice for the read
@@ -382,7 +385,7 @@ int sa_server_submit(sa_server *s, const sa_model *model, const sa_read *read, c
     if (degen) q->options = read->degenerate_options;
     const int k = sa_model_kmer_length(model);
     q->est_bytes = estimate_bytes(read, k, &q->params, precision,
-                                  degen ? mean_paths(read, k, (int64_t)q->options.size()) : 0.0);
+                                  degen ? mean_paths(read, k, (int64_t)q->options.size()) : 0.0, model->ek == 2);
     q->n_diag = std::max<int64_t>(read->ref_length - sa_model_kmer_length(model) + 1, 0) + read->n_events;
     std::lock_guard<std::mutex> lk(s->mu);
     q->ticket = s->next_ticket++;

@@ -78,7 +78,8 @@ class _Timing(ctypes.Structure):
                 ("collect_ms", ctypes.c_double), ("total_ms", ctypes.c_double), ("cells", ctypes.c_int64),
                 ("forward_launches", ctypes.c_int64), ("backward_launches", ctypes.c_int64),
                 ("fb_ms", ctypes.c_double), ("emission_path", ctypes.c_int32), ("replayed", ctypes.c_int32),
-                ("device_start_ms", ctypes.c_double), ("device_end_ms", ctypes.c_double)]
+                ("device_start_ms", ctypes.c_double), ("device_end_ms", ctypes.c_double),
+                ("emission_ms", ctypes.c_double)]
 
 
 class _ServerConfig(ctypes.Structure):

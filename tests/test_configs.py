@@ -122,6 +122,47 @@ def test_hdp_strands_in_one_batch_under_a_merged_table():
                 want, _ = ctx.align_batch(dmt if r["strand"] == 0 else dmc, [P.native_read(r, True, True)],
                                           params, prec)
                 assert np.array_equal(g.pairs, want[0].pairs) and g.log_likelihood == want[0].log_likelihood
+        for dm in (dmm, dmt, dmc):
+            dm.close()
+    finally:
+        ctx.close()
+
+
+@pytest.mark.parametrize("waves", ["1", "4"])
+def test_hdp_emission_plane_matches_gathers(monkeypatch, waves):
+    """HDP batches read each cell's density emission from the emission plane filled
+    before the sweeps (emission path 3); SA_HDP_PLANE=0 keeps the sweeps' own density
+    gathers (path 2).  Both give the same pairs and log-likelihoods: FP64 within one
+    step of the pair encoding (1e-7) and 1e-12 relative, FP32 within its parity
+    tolerances, with one and with several wavefronts per forward sweep."""
+    monkeypatch.setenv("SA_FWD_WAVES", waves)
+    ctx = native.Context(0)
+    try:
+        m = synth.hdp_model("ACEGT", 5, seed=3)
+        reads = synth.batch_reads(m, 6, 4000, seed=79, methyl_cpg=0.3)
+        params = native.Params(diagonal_expansion=50)
+        dm = ctx.model(m)
+        nr = [P.native_read(r) for r in reads]
+        for prec in (native.SA_FP64, native.SA_FP32):
+            got = {}
+            for plane in ("1", "0"):
+                monkeypatch.setenv("SA_HDP_PLANE", plane)
+                res, tm = ctx.align_batch(dm, nr, params, prec)
+                assert tm["emission_path"] == (3 if plane == "1" else 2)
+                got[plane] = res
+            for i, (a, b) in enumerate(zip(got["1"], got["0"])):
+                assert a.cells == b.cells
+                ll = P.rel(a.log_likelihood, b.log_likelihood)
+                ref = [tuple(int(v) for v in row) for row in b.pairs]
+                if prec == native.SA_FP64:
+                    out = P.compare(a.pairs, ref, params.threshold, p_tol=2e-7)
+                    assert ll <= 1e-12, (i, ll)
+                else:
+                    out = P.compare(a.pairs, ref, params.threshold, p_tol=P.P_TOL_FP32)
+                    _fp32_ok(out, ll)
+                print(f"precision {prec} read {i}: {out} ll_rel {ll:.2e} "
+                      f"identical {np.array_equal(a.pairs, b.pairs)}", flush=True)
+        dm.close()
     finally:
         ctx.close()
 
