@@ -237,8 +237,17 @@ template <typename T> struct HdpFetch {
 /* The knot pair below u (clamped to the grid): below the grid that is knot 0, above it
  * knot G-1 is the right one of the last pair, so the extrapolations read the same
  * registers without a branch around the loads. */
+/* (yrow: the row's interleaved (density, slope) pairs, in HBM or staged in LDS;
+ * gridx: the knots, likewise) */
 template <typename T>
-__device__ __forceinline__ HdpFetch<T> hdp_fetch(const Model<T> &m, int32_t row, T hmu, T evm, T s) {
+__device__ __forceinline__ HdpFetch<T> hdp_fetch_r(const Model<T> &m, int32_t row, const T *yrow, T hmu, T evm, T s,
+                                                   const T *gridx) {
+    /* No multiply-add fusion in the HDP emission (here and in hdp_finish): several
+       kernels compute it (the sweeps' own gathers, the emission plane's column pass,
+       the LDS sweeps), and fusion is decided per kernel, so only unfused arithmetic
+       gives every kernel the same bits for the same cell (it is also the reference's
+       operation order) */
+#pragma clang fp contract(off)
     HdpFetch<T> f;
     f.u = evm + hmu * ((T)1 - s);
     f.row = row;
@@ -247,14 +256,25 @@ __device__ __forceinline__ HdpFetch<T> hdp_fetch(const Model<T> &m, int32_t row,
     t = t > (T)0 ? t : (T)0;
     t = t < (T)(G - 2) ? t : (T)(G - 2);
     const int32_t il = (int32_t)t;
-    const T *ys = m.ys + ((int64_t)(row < 0 ? 0 : row) * G + il) * 2;
+    const T *ys = yrow + (int64_t)il * 2;
     f.y0 = ys[0]; f.s0 = ys[1]; f.y1 = ys[2]; f.s1 = ys[3];
-    f.xl = m.gridx[il];
+    f.xl = gridx[il];
     return f;
+}
+template <typename T>
+__device__ __forceinline__ HdpFetch<T> hdp_fetch_g(const Model<T> &m, int32_t row, T hmu, T evm, T s,
+                                                   const T *gridx) {
+    return hdp_fetch_r(m, row, m.ys + (int64_t)(row < 0 ? 0 : row) * m.grid_length * 2, hmu, evm, s, gridx);
+}
+/* (gridx: the model's knots, or a copy of them in LDS) */
+template <typename T>
+__device__ __forceinline__ HdpFetch<T> hdp_fetch(const Model<T> &m, int32_t row, T hmu, T evm, T s) {
+    return hdp_fetch_g(m, row, hmu, evm, s, m.gridx);
 }
 
 template <typename T>
 __device__ __forceinline__ T hdp_finish(const Model<T> &m, const HdpFetch<T> &f, T inv_var) {
+#pragma clang fp contract(off)
     if (f.row < 0) return ninf<T>();
     T interp;
     if (f.u <= m.grid_x0) {

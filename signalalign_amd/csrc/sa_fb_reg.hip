@@ -1507,71 +1507,205 @@ backward_reg_kernel(Args<T> a, const int32_t *__restrict__ order) {
 /* The density emission of every band cell of the listed sub-alignments, in the
  * layout of the forward match plane (cell x of diagonal d at cell_off + doff[d] +
  * x - xlo[d]).  Cell x on d: k-mer x-1 against event d-x-1, the value the sweeps'
- * own gathers give (hdp_emit_row: emissions_signal_getHdpKmerDensity,
+ * own gathers give (hdp_fetch/hdp_finish: emissions_signal_getHdpKmerDensity,
  * impl/stateMachine.c:468-495; the match and extra-event emissions are the same
- * density); -inf outside the k-mers, an absent event reads mean 0, as in the
+ * density); -inf where x has no k-mer, an absent event reads mean 0, as in the
  * sweeps.  The register sweeps then read a cell's emission with one coalesced load
  * a diagonal ahead instead of gathering from the density grid on their dependency
- * chain, and the backward pass does not gather again.  One workgroup per chunk of
- * kPlaneDiags diagonals, one diagonal per wavefront at a time, two cells per lane
- * in flight. */
+ * chain, and the backward pass does not gather again.
+ *
+ * Gathered cell by cell, the grid (3125 rows of 2000 knots: 50 MB in FP32) does not
+ * stay in L2 and every cell costs a cache line from beyond it.  So the plane is
+ * built in two passes.  The column pass takes the band's columns grouped by density
+ * row (the host sorts them): a workgroup stages one row (16 KB) and the knots in LDS
+ * and computes whole columns x (diagonals col_a[x] .. col_b[x], consecutive events)
+ * into the column plane.  The transpose pass then moves each chunk of kPlaneDiags
+ * diagonals from the column plane to the diagonal layout through an LDS tile. */
 template <typename T>
-__global__ void __launch_bounds__(256) hdp_plane_kernel(Args<T> a, const int32_t *__restrict__ tasks,
-                                                        const int32_t *__restrict__ chunk0, int n) {
-    const int b = blockIdx.x;
-    int lo = 0, hi = n - 1; /* the listed task whose chunks hold b */
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (chunk0[mid] <= b) lo = mid;
-        else hi = mid - 1;
-    }
-    const TaskDev t = a.tasks[tasks[lo]];
+__global__ void __launch_bounds__(256) hdp_columns_kernel(Args<T> a, const int32_t *__restrict__ ent,
+                                                          const int32_t *__restrict__ groups) {
+    constexpr int NG = 2048, NQ = kColumnGroup;
+    __shared__ T s_ys[2 * NG], s_gx[NG];
+    /* per column of the group: where its cells go, its first diagonal, its cells'
+       first flat index, and what its emissions need */
+    __shared__ int64_t s_dst[NQ];
+    __shared__ int32_t s_ca[NQ], s_x[NQ], s_f[NQ + 1], s_lY[NQ];
+    __shared__ int64_t s_evc[NQ];
+    __shared__ T s_hmu[NQ], s_s[NQ], s_iv[NQ];
     const Model<T> &m = a.m;
-    const int N = t.lX + t.lY;
-    const int d0 = (b - chunk0[lo]) * kPlaneDiags;
-    const int32_t *xlo = a.xlo + t.diag_off;
-    const int32_t *xhi = a.xhi + t.diag_off;
-    const int32_t *doff = a.doff + t.diag_off;
-    const int32_t *kid = a.kid + t.pos_off;
-    const T *evm = a.evm + t.evc_off;
-    T *em = a.em + t.cell_off;
-    const T s = (T)(t.scale / t.var);
-    const T iv = (T)(1.0 / t.var);
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    auto emit = [&](int d, int x) -> T {
-        if (x < 1 || x > t.lX) return ninf<T>();
-        const int j = d - x - 1;
-        const T e = j >= 0 && j < t.lY ? evm[j] : (T)0;
-        const int32_t id = kid[x - 1];
-        return hdp_emit_row(m, m.hrow[id], m.hmu[id], e, s, iv);
+    const int row = groups[3 * blockIdx.x], e0 = groups[3 * blockIdx.x + 1], ne = groups[3 * blockIdx.x + 2];
+    const int tid = threadIdx.x;
+    const int G = m.grid_length;
+    const bool lg = G <= NG;
+    if (row >= 0 && lg) {
+        const T *yr = m.ys + (int64_t)row * G * 2;
+        for (int i = tid; i < 2 * G; i += 256) s_ys[i] = yr[i];
+        for (int i = tid; i < G; i += 256) s_gx[i] = m.gridx[i];
+    }
+    int h = 0;
+    if (tid < NQ) {
+        if (tid < ne) {
+            const int ti = ent[2 * (e0 + tid)], x = ent[2 * (e0 + tid) + 1];
+            const TaskDev t = a.tasks[ti];
+            const int64_t cs = t.pos_off + ti + x;
+            const int ca = a.col_a[cs], cb = a.col_b[cs], co = a.col_off[cs];
+            /* (a band whose bounds are not non-decreasing fails the sweeps, which then
+               run again on the LDS kernels; its columns are only kept inside its cells) */
+            h = (int)max<int64_t>(0, min<int64_t>((int64_t)cb - ca + 1, a.task_cells[ti] - co));
+            s_dst[tid] = t.cell_off + co;
+            s_ca[tid] = ca;
+            s_x[tid] = x;
+            s_lY[tid] = t.lY;
+            s_evc[tid] = t.evc_off;
+            s_s[tid] = (T)(t.scale / t.var);
+            s_iv[tid] = (T)(1.0 / t.var);
+            s_hmu[tid] = row >= 0 ? m.hmu[a.kid[t.pos_off + x - 1]] : (T)0;
+        }
+        /* exclusive prefix of the column heights (the group fits one wavefront) */
+        int v = h;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int u = __shfl_up(v, o, 64);
+            if (tid >= o) v += u;
+        }
+        s_f[tid + 1] = v;
+        if (tid == 0) s_f[0] = 0;
+    }
+    __syncthreads();
+    /* one column per wavefront at a time, two columns in flight; the knots come from
+       LDS (ds_read) when they fit, else from HBM */
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    auto run = [&](const T *yrow, const T *gx) {
+        for (int q0 = wave; q0 < ne; q0 += 8) {
+#pragma unroll
+            for (int off = 0; off < 128; off += 64) {
+                T e[2];
+                int d[2];
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    const int q = q0 + 4 * u;
+                    const int h = q < ne ? s_f[q + 1] - s_f[q] : 0;
+                    const int i = off + lane;
+                    d[u] = q < ne ? s_ca[q] + i : 0;
+                    const int j = q < ne ? d[u] - s_x[q] - 1 : -1;
+                    e[u] = i < h && row >= 0 && j >= 0 && j < s_lY[q] ? a.evm[s_evc[q] + j] : (T)0;
+                }
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    const int q = q0 + 4 * u;
+                    const int h = q < ne ? s_f[q + 1] - s_f[q] : 0;
+                    if (off + lane >= h) continue;
+                    const T v = row < 0 ? ninf<T>()
+                                        : hdp_finish(m, hdp_fetch_r(m, row, yrow, s_hmu[q], e[u], s_s[q], gx), s_iv[q]);
+                    a.emc[s_dst[q] + (d[u] - s_ca[q])] = v;
+                }
+            }
+            /* columns taller than 128 cells: the rest one by one */
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                const int q = q0 + 4 * u;
+                if (q >= ne) continue;
+                const int h = s_f[q + 1] - s_f[q];
+                for (int i = 128 + lane; i < h; i += 64) {
+                    const int dd = s_ca[q] + i, j = dd - s_x[q] - 1;
+                    const T e = row >= 0 && j >= 0 && j < s_lY[q] ? a.evm[s_evc[q] + j] : (T)0;
+                    const T v = row < 0 ? ninf<T>()
+                                        : hdp_finish(m, hdp_fetch_r(m, row, yrow, s_hmu[q], e, s_s[q], gx), s_iv[q]);
+                    a.emc[s_dst[q] + i] = v;
+                }
+            }
+        }
     };
-    for (int dd = wave; dd < kPlaneDiags; dd += 4) {
-        const int d = d0 + dd;
-        if (d > N) break;
-        const int l = ld_scalar(xlo, d), h = ld_scalar(xhi, d);
-        T *row = em + ld_scalar(doff, d) - l;
-        for (int x = l + lane; x <= h; x += 128) {
-            const int x2 = x + 64;
-            const T v1 = emit(d, x);
-            const T v2 = x2 <= h ? emit(d, x2) : (T)0;
-            row[x] = v1;
-            if (x2 <= h) row[x2] = v2;
+    if (lg) run(s_ys, s_gx);
+    else run(m.ys + (int64_t)(row < 0 ? 0 : row) * G * 2, m.gridx);
+}
+
+/* Chunks of kPlaneDiags diagonals, grid-stride (a chunk is a few thousand cells:
+   one workgroup per chunk would spend its time being launched) */
+template <typename T>
+__global__ void __launch_bounds__(256) hdp_plane_kernel(Args<T> a, const int32_t *__restrict__ ctask,
+                                                        const int32_t *__restrict__ chunk0, int c0, int c1) {
+    constexpr int NP = 192, TS = kPlaneDiags + 1; /* tile: NP columns x kPlaneDiags diagonals (+1: banks) */
+    __shared__ int32_t s_off[kPlaneDiags + 1], s_lo[kPlaneDiags], s_hi[kPlaneDiags];
+    __shared__ int32_t s_ca[NP], s_cb[NP], s_co[NP];
+    __shared__ T tile[NP * TS];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    for (int b = c0 + blockIdx.x; b < c1; b += gridDim.x) {
+        const int ti = ctask[b]; /* the chunk's sub-alignment, its first chunk */
+        const TaskDev t = a.tasks[ti];
+        const int N = t.lX + t.lY;
+        const int d0 = (b - chunk0[ti]) * kPlaneDiags;
+        const int nd = min(kPlaneDiags, N + 1 - d0);
+        const int32_t *xlo = a.xlo + t.diag_off;
+        const int32_t *xhi = a.xhi + t.diag_off;
+        const int32_t *doff = a.doff + t.diag_off;
+        const int32_t *ca = a.col_a + t.pos_off + ti, *cb = a.col_b + t.pos_off + ti;
+        const int32_t *co = a.col_off + t.pos_off + ti;
+        T *em = a.em + t.cell_off;
+        const T *emc = a.emc + t.cell_off;
+        const int64_t cells = a.task_cells[ti];
+        __syncthreads(); /* the previous chunk is done with the shared arrays */
+        if (tid < nd) {
+            const int d = d0 + tid;
+            s_lo[tid] = xlo[d];
+            s_hi[tid] = xhi[d];
+            s_off[tid] = doff[d];
+        }
+        __syncthreads();
+        int pmin = s_lo[0], pmax = s_hi[0]; /* the chunk's columns */
+        for (int k = 1; k < nd; k++) {
+            pmin = min(pmin, s_lo[k]);
+            pmax = max(pmax, s_hi[k]);
+        }
+        const int np = pmax - pmin + 1;
+        const bool lt = np <= NP;
+        if (lt) {
+            for (int i = tid; i < np; i += 256) {
+                s_ca[i] = ca[pmin + i];
+                s_cb[i] = cb[pmin + i];
+                s_co[i] = co[pmin + i];
+            }
+            __syncthreads();
+            /* each column's segment of the chunk's diagonals: consecutive values */
+            for (int i = tid; i < np * kPlaneDiags; i += 256) {
+                const int c = i / kPlaneDiags, k = i % kPlaneDiags, d = d0 + k;
+                const int64_t ic = (int64_t)s_co[c] + (d - s_ca[c]);
+                if (k < nd && d >= s_ca[c] && d <= s_cb[c] && ic < cells) tile[c * TS + k] = emc[ic];
+            }
+            __syncthreads();
+        }
+        /* out along the diagonals, one wavefront per diagonal */
+        for (int k = wave; k < nd; k += 4) {
+            const int l = s_lo[k], hh = s_hi[k], d = d0 + k;
+            T *row = em + s_off[k] - l;
+            for (int x = l + lane; x <= hh; x += 64) {
+                T v;
+                if (lt) v = tile[(x - pmin) * TS + k];
+                else {
+                    const int64_t ic = (int64_t)co[x] + (d - ca[x]);
+                    v = ic >= 0 && ic < cells ? emc[ic] : (T)0;
+                }
+                row[x] = v;
+            }
         }
     }
 }
 
 template <typename T>
-hipError_t launch_hdp_plane(const Args<T> &a, const int32_t *tasks, const int32_t *chunk0, int n, int n_chunks,
-                            hipStream_t st) {
-    if (n == 0 || n_chunks == 0) return hipSuccess;
-    hipLaunchKernelGGL(hdp_plane_kernel<T>, dim3(n_chunks), dim3(256), 0, st, a, tasks, chunk0, n);
+hipError_t launch_hdp_plane(const Args<T> &a, const int32_t *entries, const int32_t *groups, int n_groups,
+                            const int32_t *ctask, const int32_t *chunk0, int c0, int c1, int n_cu, hipStream_t st) {
+    n_cu = n_cu > 0 ? n_cu : 256;
+    if (c1 <= c0) return hipSuccess;
+    if (n_groups > 0) hipLaunchKernelGGL(hdp_columns_kernel<T>, dim3(n_groups), dim3(256), 0, st, a, entries, groups);
+    const int grid = c1 - c0 < 8 * n_cu ? c1 - c0 : 8 * n_cu;
+    hipLaunchKernelGGL(hdp_plane_kernel<T>, dim3(grid), dim3(256), 0, st, a, ctask, chunk0, c0, c1);
     return hipGetLastError();
 }
-template hipError_t launch_hdp_plane<float>(const Args<float> &, const int32_t *, const int32_t *, int, int,
-                                            hipStream_t);
-template hipError_t launch_hdp_plane<double>(const Args<double> &, const int32_t *, const int32_t *, int, int,
-                                             hipStream_t);
+template hipError_t launch_hdp_plane<float>(const Args<float> &, const int32_t *, const int32_t *, int,
+                                            const int32_t *, const int32_t *, int, int, int, hipStream_t);
+template hipError_t launch_hdp_plane<double>(const Args<double> &, const int32_t *, const int32_t *, int,
+                                             const int32_t *, const int32_t *, int, int, int, hipStream_t);
 
 /* ------------------------------------------------------------------------ */
 /* launchers                                                                 */

@@ -146,7 +146,7 @@ struct sa_context {
     std::shared_ptr<sa_rt::PinnedPool> pair_pool = std::make_shared<sa_rt::PinnedPool>();
     std::mutex mu;
     char err[512] = {};        /* text of the last error (fixed storage: setting it allocates nothing) */
-    hipEvent_t ev[16] = {};
+    hipEvent_t ev[24] = {};    /* run steps; 16..23: each stream lane's emission plane */
     hipEvent_t gev[160] = {};  /* start/end of each forward and backward launch */
     int n_cu = 0;
     /* device address of the no-allocation entry point's scratch array on this
@@ -169,6 +169,8 @@ struct sa_model {
     double inv_dx = 0.0;
     double grid0 = 0.0, grid1 = 0.0, gridn = 0.0; /* first, second and last grid knots */
     int32_t ek = 1;          /* register-kernel emission variant: 0 standard Gaussian, 1 any Gaussian, 2 HDP */
+    std::vector<int32_t> hrow_host; /* HDP: density row of each k-mer (the emission plane groups by it) */
+    int32_t n_rows = 0;
     double gx_const = -2.3025850929940455;
     DBuf d_lut;
     DevTables<double> t64;

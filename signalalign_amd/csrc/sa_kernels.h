@@ -75,6 +75,11 @@ struct Args {
        offsets (doff) the plane kernel needs before any forward sweep runs */
     T *em;
     int32_t em_plane;
+    /* the plane's column pass: per position slot (pos_off + task index + x, x = 0..lX)
+       the first and last diagonal holding column x and the column's offset in the
+       column plane (prep); the column plane (cell_off-based, column after column) */
+    int32_t *col_a, *col_b, *col_off;
+    T *emc;
     /* window outputs: 3 int32 (p, x, y) per pair */
     int32_t *out;
     int32_t *win_count;
@@ -156,12 +161,16 @@ hipError_t launch_forward_reg(const Args<T> &a, const int32_t *order, int n, int
 template <typename T>
 hipError_t launch_backward_reg(const Args<T> &a, const int32_t *order, int n, int cpt, int ek, size_t shmem,
                                hipStream_t st);
-/* the HDP emission plane of n listed sub-alignments (tasks[i]); their chunks of
-   kPlaneDiags diagonals are numbered from chunk0[i] (chunk0[n] in all) */
+/* The HDP emission plane of a set of sub-alignments, in two passes: the column pass
+   over column groups (3 ints each: density row or -1 for columns with no k-mer, first
+   entry, entries; entries are 2 ints: task, column x, grouped by row), then the
+   transpose into the plane's layout over chunks c0..c1-1 of kPlaneDiags diagonals
+   (chunk c belongs to task ctask[c], whose first chunk is chunk0[task]). */
 constexpr int kPlaneDiags = 32;
+constexpr int kColumnGroup = 64;
 template <typename T>
-hipError_t launch_hdp_plane(const Args<T> &a, const int32_t *tasks, const int32_t *chunk0, int n, int n_chunks,
-                            hipStream_t st);
+hipError_t launch_hdp_plane(const Args<T> &a, const int32_t *entries, const int32_t *groups, int n_groups,
+                            const int32_t *ctask, const int32_t *chunk0, int c0, int c1, int n_cu, hipStream_t st);
 /* zeroes n ints (a kernel, so that it can sit in a captured graph) */
 hipError_t launch_zero(int32_t *p, int n, hipStream_t st);
 hipError_t launch_collect(const WinDev *wins, const int32_t *win_count, const int64_t *win_dst, const int32_t *src,

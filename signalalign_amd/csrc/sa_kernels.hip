@@ -93,31 +93,49 @@ __global__ void __launch_bounds__(256) prep_kernel(Args<T> a) {
     if (err) atomicOr(&s_err, err);
     __syncthreads();
     if (a.em_plane) {
-        /* the forward match plane's per-diagonal offsets (exclusive prefix sums of the
-           band widths), which the forward sweeps also write as they go: the HDP
-           emission plane shares the layout and is filled before they run */
+        /* exclusive prefix sums over i = 0..n-1 of w(i), stored by put(i, sum) */
         __shared__ int s_wsum[4];
         __shared__ int s_carry;
-        if (tid == 0) s_carry = 0;
-        __syncthreads();
-        const int32_t *bl = a.xlo + t.diag_off, *bh = a.xhi + t.diag_off;
-        for (int64_t base = 0; base <= N; base += nt) {
-            const int64_t d = base + tid;
-            const int w = d <= N ? bh[d] - bl[d] + 1 : 0;
-            int v = w; /* inclusive scan over the wavefront */
-            for (int o = 1; o < 64; o <<= 1) {
-                const int u = __shfl_up(v, o, 64);
-                if ((tid & 63) >= o) v += u;
+        auto block_scan = [&](int64_t n, auto w_of, auto put) {
+            if (tid == 0) s_carry = 0;
+            __syncthreads();
+            for (int64_t base = 0; base < n; base += nt) {
+                const int64_t i = base + tid;
+                const int w = i < n ? w_of(i) : 0;
+                int v = w; /* inclusive scan over the wavefront */
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int u = __shfl_up(v, o, 64);
+                    if ((tid & 63) >= o) v += u;
+                }
+                if ((tid & 63) == 63) s_wsum[tid >> 6] = v;
+                __syncthreads();
+                int pre = s_carry;
+                for (int q = 0; q < (tid >> 6); q++) pre += s_wsum[q];
+                if (i < n) put(i, pre + v - w);
+                __syncthreads();
+                if (tid == nt - 1) s_carry = pre + v;
+                __syncthreads();
             }
-            if ((tid & 63) == 63) s_wsum[tid >> 6] = v;
-            __syncthreads();
-            int pre = s_carry;
-            for (int k = 0; k < (tid >> 6); k++) pre += s_wsum[k];
-            if (d <= N) a.doff[t.diag_off + d] = pre + v - w;
-            __syncthreads();
-            if (tid == nt - 1) s_carry = pre + v;
-            __syncthreads();
+        };
+        const int32_t *bl = a.xlo + t.diag_off, *bh = a.xhi + t.diag_off;
+        /* the forward match plane's per-diagonal offsets (prefix sums of the band
+           widths), which the forward sweeps also write as they go: the HDP emission
+           plane shares the layout and is filled before they run */
+        block_scan(N + 1, [&](int64_t d) { return bh[d] - bl[d] + 1; },
+                   [&](int64_t d, int v) { a.doff[t.diag_off + d] = v; });
+        /* per column x = 0..lX the diagonals holding it, [first d with xhi(d) >= x, last
+           d with xlo(d) <= x] (bounds are non-decreasing), and its offset in the column
+           plane the emission plane is built through */
+        int32_t *ca = a.col_a + t.pos_off + ti, *cb = a.col_b + t.pos_off + ti, *co = a.col_off + t.pos_off + ti;
+        for (int64_t d = tid; d <= N; d += nt) {
+            const int h = bh[d], hp = d > 0 ? bh[d - 1] : -1;
+            for (int x = max(hp + 1, 0); x <= min(h, t.lX); x++) ca[x] = (int)d;
+            const int l = bl[d], ln = d < N ? bl[d + 1] : t.lX + 1;
+            for (int x = max(l, 0); x < min(ln, t.lX + 1); x++) cb[x] = (int)d;
         }
+        __syncthreads();
+        block_scan((int64_t)t.lX + 1, [&](int64_t x) { return max(0, cb[x] - ca[x] + 1); },
+                   [&](int64_t x, int v) { co[x] = v; });
     }
     if (tid == 0) {
         const int32_t *xlo = a.xlo + t.diag_off;

@@ -67,7 +67,10 @@ struct sa_batch {
     DBuf d_fM, d_fX, d_fY;
     DBuf d_wins, d_task_order, d_win_order, d_out, d_win_count, d_win_total, d_win_dst, d_packed;
     DBuf d_ring_g; /* LDS sweeps' rings in HBM when a band is too wide for LDS */
-    DBuf d_em, d_em_tasks, d_em_chunk0; /* HDP emission plane, its sub-alignments and chunk numbering */
+    /* HDP emission plane, its sub-alignments and chunk numbering; the column pass's
+       per-position column ranges and offsets, its (task, column) entries grouped by
+       density row and the groups */
+    DBuf d_em, d_em_tasks, d_em_chunk0, d_col_a, d_col_b, d_col_off, d_em_ent, d_em_grp;
     /* degenerate references ('X' with options): multi-path sweeps, FP64 */
     int32_t paths = 0;
     std::string options;      /* the bases an 'X' stands for */
@@ -88,6 +91,7 @@ struct sa_batch {
         std::vector<sa_launch> launches; /* shapes; durations filled from the replay's events */
         int64_t nw = 0, total_pairs = 0, cells = 0;
         int32_t ek = 0;
+        int64_t em_c[5] = {0, 0, 0, 0, 0}; /* emission plane chunks per stream lane (prefix) */
         void reset() {
             if (exec) (void)hipGraphExecDestroy(exec);
             exec = nullptr;
@@ -103,7 +107,7 @@ struct sa_batch {
                        &d_task_xycells, &d_task_maxw, &d_task_span,
                        &d_task_nwin, &d_win_slots, &d_err, &d_fM, &d_fX, &d_fY, &d_wins, &d_task_order,
                        &d_win_order, &d_out, &d_win_count, &d_win_total, &d_win_dst, &d_packed, &d_ring_g,
-                       &d_em, &d_em_tasks, &d_em_chunk0,
+                       &d_em, &d_em_tasks, &d_em_chunk0, &d_col_a, &d_col_b, &d_col_off, &d_em_ent, &d_em_grp,
                        &d_ex_trans, &d_ex_s0, &d_ex_s1, &d_ex_obs, &d_ex_rec, &d_ex_count,
                        &d_pcol, &d_pent, &d_pplane, &d_pp, &d_psa, &d_psb, &d_pecol, &d_pekid, &d_pdoff,
                        &d_pdiag, &d_pentr, &d_pmaxent};
@@ -163,6 +167,20 @@ static int fetch_pairs(sa_context *ctx, sa_result *res, const void *d_packed, in
     SA_HIP(ctx, dst->ensure(bytes));
     if (n) SA_HIP(ctx, hipMemcpyAsync(dst->p, d_packed, sizeof(sa_pair) * (size_t)n, hipMemcpyDeviceToHost, st));
     return SA_OK;
+}
+
+/* The emission plane's span on the device: the longest of the stream lanes' plane
+   launches (they overlap one another and the sweeps) */
+static double plane_ms(sa_context *ctx, const int64_t *em_c) {
+    double best = 0.0;
+    for (int lane = 0; lane < 4; lane++) {
+        float ms = 0.f;
+        if (em_c[lane + 1] > em_c[lane] &&
+            hipEventElapsedTime(&ms, ctx->ev[16 + 2 * lane], ctx->ev[17 + 2 * lane]) == hipSuccess)
+            best = std::max(best, (double)ms);
+    }
+    (void)hipGetLastError();
+    return best;
 }
 
 static void finish_pairs(sa_context *ctx, sa_result *res, int64_t n) {
@@ -433,6 +451,11 @@ int sa_model_create(sa_context *ctx, const sa_model_desc *d, sa_model **out) {
         }
         m->gx_const = d->gapx ? d->gapx[0] : -2.3025850929940455;
         m->ek = std_tables ? 0 : 1;
+    }
+    if (d->kind == 2) {
+        m->hrow_host.assign(m->n_kmers, -1);
+        for (int64_t i = 0; i < m->n_kmers; i++) m->hrow_host[i] = (int32_t)d->kmer_row[i];
+        m->n_rows = (int32_t)d->n_rows;
     }
     std::vector<int8_t> lut(m->lut, m->lut + 256);
     int rc = upload(ctx, m->d_lut, lut);
@@ -829,6 +852,16 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
         const char *hp = getenv("SA_HDP_PLANE");
         a.em_plane = model->ek == 2 && !force_lds && !(hp && std::strcmp(hp, "0") == 0) ? 1 : 0;
     }
+    if (a.em_plane) {
+        /* per position slot pos_off + task index + x, x = 0..lX */
+        const size_t ncs = sizeof(int32_t) * (size_t)(b->total_pos + nt + 64);
+        SA_HIP(ctx, b->d_col_a.ensure(ncs));
+        SA_HIP(ctx, b->d_col_b.ensure(ncs));
+        SA_HIP(ctx, b->d_col_off.ensure(ncs));
+        a.col_a = b->d_col_a.as<int32_t>();
+        a.col_b = b->d_col_b.as<int32_t>();
+        a.col_off = b->d_col_off.as<int32_t>();
+    }
 
     SA_HIP(ctx, hipEventRecord(ctx->ev[0], st));
     SA_HIP(ctx, sa::launch_prep<T>(a, (int)nt, st));
@@ -1042,28 +1075,76 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     SA_HIP(ctx, b->d_fX.ensure(sizeof(T) * std::max<int64_t>(total_xy, 1)));
     SA_HIP(ctx, b->d_fY.ensure(sizeof(T) * std::max<int64_t>(total_xy, 1)));
     int rc;
-    /* HDP emission plane over the sub-alignments of the register sweeps, numbered in
-       chunks of kPlaneDiags diagonals */
-    std::vector<int32_t> em_tasks, em_chunk0;
+    /* HDP emission plane over the sub-alignments of the register sweeps: their columns
+       grouped by density row for the column pass (k-mer ids read back from prep),
+       their diagonals numbered in chunks of kPlaneDiags for the transpose.  The column
+       plane lives in the forward match plane, which the sweeps fill only after it. */
+    /* Per stream lane (each lane's stream builds the plane of its own sub-alignments
+       before its sweeps, so the long sweeps start at once and the plane of the others
+       overlaps them): column groups and chunks of each lane, in lane order. */
+    std::vector<int32_t> em_ctask, em_chunk0(nt, 0), em_ent, em_grp;
+    int64_t em_g[kLanes + 1] = {0, 0, 0, 0, 0}, em_c[kLanes + 1] = {0, 0, 0, 0, 0};
+    int n_em = 0;
     if (a.em_plane) {
-        int64_t nch = 0;
-        for (int64_t i = 0; i < nt; i++)
-            if (fgrp[i] != 0) {
-                em_tasks.push_back((int32_t)i);
-                em_chunk0.push_back((int32_t)nch);
-                nch += (nd[i] + 1 + sa::kPlaneDiags - 1) / sa::kPlaneDiags;
-            }
-        if (nch > INT32_MAX) return fail(ctx, SA_ERR_UNSUPPORTED, "emission plane: too many diagonals");
-        em_chunk0.push_back((int32_t)nch);
-        if (!em_tasks.empty()) {
-            SA_HIP(ctx, b->d_em.ensure(sizeof(T) * std::max<int64_t>(total_cells, 1)));
-            if ((rc = upload(ctx, b->d_em_tasks, em_tasks)) || (rc = upload(ctx, b->d_em_chunk0, em_chunk0)))
-                return rc;
-            a.em = b->d_em.as<T>();
-        }
+        for (int64_t i = 0; i < nt; i++) n_em += fgrp[i] != 0;
     }
-    const int n_em = (int)em_tasks.size();
-    const int em_chunks = n_em ? em_chunk0.back() : 0;
+    if (n_em) {
+        std::vector<int32_t> kid(b->total_pos);
+        if (b->total_pos) {
+            SA_HIP(ctx, hipMemcpyAsync(kid.data(), b->d_kid.p, sizeof(int32_t) * b->total_pos, hipMemcpyDeviceToHost,
+                                       st));
+            SA_HIP(ctx, hipStreamSynchronize(st));
+        }
+        const int32_t nrow = model->n_rows;
+        auto row_of = [&](int64_t i, int32_t x) -> int32_t { /* -1: no k-mer */
+            if (x == 0) return -1;
+            const int32_t r = model->hrow_host[kid[b->tasks[i].pos_off + x - 1]];
+            return r >= 0 && r < nrow ? r : -1;
+        };
+        int64_t n_ent = 0;
+        for (int lane = 0; lane < kLanes; lane++) {
+            std::vector<int32_t> lt;
+            for (int64_t i = 0; i < nt; i++)
+                if (fgrp[i] != 0 && tlane[i] == lane) lt.push_back((int32_t)i);
+            for (int32_t i : lt) {
+                em_chunk0[i] = (int32_t)em_ctask.size();
+                const int64_t nc = (nd[i] + 1 + sa::kPlaneDiags - 1) / sa::kPlaneDiags;
+                if ((int64_t)em_ctask.size() + nc > INT32_MAX)
+                    return fail(ctx, SA_ERR_UNSUPPORTED, "emission plane: too many diagonals");
+                em_ctask.insert(em_ctask.end(), (size_t)nc, i);
+            }
+            em_c[lane + 1] = (int64_t)em_ctask.size();
+            /* the lane's columns, grouped by density row (counting sort) */
+            std::vector<int64_t> cnt((size_t)nrow + 2, 0);
+            for (int32_t i : lt)
+                for (int32_t x = 0; x <= b->tasks[i].lX; x++) cnt[(size_t)row_of(i, x) + 2]++;
+            for (size_t r = 1; r < cnt.size(); r++) cnt[r] += cnt[r - 1]; /* cnt[r + 1]: first entry of row r */
+            const int64_t le = cnt.back();
+            if (2 * (n_ent + le) > INT32_MAX) return fail(ctx, SA_ERR_UNSUPPORTED, "emission plane: too many columns");
+            em_ent.resize((size_t)(2 * (n_ent + le)));
+            std::vector<int64_t> cur(cnt.begin(), cnt.end() - 1);
+            for (int32_t i : lt)
+                for (int32_t x = 0; x <= b->tasks[i].lX; x++) {
+                    const int64_t q = n_ent + cur[(size_t)row_of(i, x) + 1]++;
+                    em_ent[(size_t)(2 * q)] = i;
+                    em_ent[(size_t)(2 * q + 1)] = x;
+                }
+            for (int32_t r = -1; r < nrow; r++)
+                for (int64_t e = cnt[(size_t)r + 1]; e < cnt[(size_t)r + 2]; e += sa::kColumnGroup) {
+                    em_grp.push_back(r);
+                    em_grp.push_back((int32_t)(n_ent + e));
+                    em_grp.push_back((int32_t)std::min<int64_t>(sa::kColumnGroup, cnt[(size_t)r + 2] - e));
+                }
+            em_g[lane + 1] = (int64_t)(em_grp.size() / 3);
+            n_ent += le;
+        }
+        SA_HIP(ctx, b->d_em.ensure(sizeof(T) * std::max<int64_t>(total_cells, 1)));
+        if ((rc = upload(ctx, b->d_em_tasks, em_ctask)) || (rc = upload(ctx, b->d_em_chunk0, em_chunk0)) ||
+            (rc = upload(ctx, b->d_em_ent, em_ent)) || (rc = upload(ctx, b->d_em_grp, em_grp)))
+            return rc;
+        a.em = b->d_em.as<T>();
+        a.emc = b->d_fM.as<T>();
+    }
     if ((rc = upload(ctx, b->d_tasks, tasks)) || (rc = upload(ctx, b->d_wins, wins)) ||
         (rc = upload(ctx, b->d_task_order, task_order)) || (rc = upload(ctx, b->d_win_order, win_order)))
         return rc;
@@ -1146,17 +1227,20 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     /* fork on ev[2], per-launch events gev[2k], gev[2k+1] (forward launches first),
        join on ev[4] */
     auto run_sweeps = [&](bool forward) -> int {
-        if (forward && n_em) {
-            SA_HIP(ctx, hipEventRecord(ctx->ev[14], st));
-            SA_HIP(ctx, sa::launch_hdp_plane<T>(a, b->d_em_tasks.as<int32_t>(), b->d_em_chunk0.as<int32_t>(), n_em,
-                                                em_chunks, st));
-            SA_HIP(ctx, hipEventRecord(ctx->ev[15], st));
-        }
         SA_HIP(ctx, hipEventRecord(ctx->ev[2], st));
         for (int q = 1; q < kLanes; q++)
             if (gst[q] != st) SA_HIP(ctx, hipStreamWaitEvent(gst[q], ctx->ev[2], 0));
         for (int lane = 0; lane < kLanes; lane++) {
             hipStream_t sg = gst[lane];
+            if (forward && em_c[lane + 1] > em_c[lane]) {
+                SA_HIP(ctx, hipEventRecord(ctx->ev[16 + 2 * lane], sg));
+                SA_HIP(ctx, sa::launch_hdp_plane<T>(a, b->d_em_ent.as<int32_t>(),
+                                                    b->d_em_grp.as<int32_t>() + 3 * em_g[lane],
+                                                    (int)(em_g[lane + 1] - em_g[lane]), b->d_em_tasks.as<int32_t>(),
+                                                    b->d_em_chunk0.as<int32_t>(), (int)em_c[lane],
+                                                    (int)em_c[lane + 1], ctx->n_cu, sg));
+                SA_HIP(ctx, hipEventRecord(ctx->ev[17 + 2 * lane], sg));
+            }
             if (forward)
                 for (size_t k = 0; k < fl.size(); k++) {
                     if (fl[k].lane != lane) continue;
@@ -1258,11 +1342,7 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
         SA_HIP(ctx, hipStreamSynchronize(st));
     }
     tm.emission_path = ek;
-    if (n_em) {
-        float ems = 0.f;
-        (void)hipEventElapsedTime(&ems, ctx->ev[14], ctx->ev[15]);
-        tm.emission_ms = ems;
-    }
+    tm.emission_ms = plane_ms(ctx, em_c);
     /* final positions in the reference's list order: per read, sub-alignments in
        split order, each in reverse emission order */
     std::vector<int64_t> win_dst(nw);
@@ -1392,6 +1472,7 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
             gc.total_pairs = total_pairs;
             gc.cells = total_cells;
             gc.ek = ek;
+            std::copy(em_c, em_c + 5, gc.em_c);
         } else {
             gc.reset();
             (void)hipGetLastError();
@@ -1924,7 +2005,7 @@ int replay_batch(sa_batch *b, sa_result *res) {
     tm.backward_launches = (int64_t)(gc.launches.size() - nf);
     tm.cells = gc.cells;
     tm.emission_path = gc.ek;
-    if (gc.ek == 3 && hipEventElapsedTime(&ms, ctx->ev[14], ctx->ev[15]) == hipSuccess) tm.emission_ms = ms;
+    if (gc.ek == 3) tm.emission_ms = plane_ms(ctx, gc.em_c);
     tm.replayed = 1;
     return SA_OK;
 }
