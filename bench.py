@@ -281,17 +281,23 @@ def hdp_leg(native, synth, ctx, args):
         return {"gcups": round(cells / float(np.mean(ms)) / 1e6, 3), "ms_per_step": round(float(np.mean(ms)), 3),
                 "cells": cells, "roofline": kernel_roofline(tms, "f32", FLOP_FWD_HDP, FLOP_BWD_HDP)}
 
-    per_strand = []
-    for strand, m in ((0, mt), (1, mc)):
-        dm = ctx.model(m)
-        rs = [r for r in reads if r["strand"] == strand]
-        per_strand.append((dm, native.Batch(ctx, dm, make_native_reads(native, rs), params)))
-    sched = {"per strand": timed("per-strand", per_strand)}
-    dmm = ctx.model(models.merge_strand_models(mt, mc))
-    merged = [native.Read(r["ref"] if r["strand"] == 0 else r["ref"].lower(), r["events"], r["anchors"], r["scale"],
-                          r["shift"], r["var"]) for r in reads]
-    sched["one batch"] = timed("one-batch", [(dmm, native.Batch(ctx, dmm, merged, params))])
+    sched = {}
+    if args.hdp_schedule in ("both", "per-strand"):
+        per_strand = []
+        for strand, m in ((0, mt), (1, mc)):
+            dm = ctx.model(m)
+            rs = [r for r in reads if r["strand"] == strand]
+            per_strand.append((dm, native.Batch(ctx, dm, make_native_reads(native, rs), params)))
+        sched["per strand"] = timed("per-strand", per_strand)
+    if args.hdp_schedule in ("both", "one-batch"):
+        dmm = ctx.model(models.merge_strand_models(mt, mc))
+        merged = [native.Read(r["ref"] if r["strand"] == 0 else r["ref"].lower(), r["events"], r["anchors"],
+                              r["scale"], r["shift"], r["var"]) for r in reads]
+        sched["one batch"] = timed("one-batch", [(dmm, native.Batch(ctx, dmm, merged, params))])
     best = max(sched, key=lambda k: sched[k]["gcups"])
+    if best == "one batch" and args.hdp_reads == 2048:  # the profiled schedule
+        r = sched[best]["roofline"]
+        r["traffic"] = traffic_for(r["kernel"], r["kernel_cells_per_launch"], args.hdp_traffic_file)
     out = {"workload": f"HDP: {args.hdp_reads} reads x 2 strands, log-normal lengths median 5000 nt, ACEGT 5-mer "
                        "template/complement HDP grids (3125 k-mers x 2000 points), band expansion 50, fp32",
            "schedule": best, **sched[best],
@@ -494,14 +500,12 @@ def serving_leg(native, synth, ctx, gmodel, args):
     return out
 
 
-def traffic_for(kname, cells, args):
+def traffic_for(kname, cells, path):
     """HBM bytes per launch of `kname`: the calibrated rocprofv3 FETCH_SIZE +
-    WRITE_SIZE bytes per band cell kept with the profiles, times the launch's cells
-    (None if that kernel is not listed or the workload is not the profiled one)."""
-    if args.reads != 4096 or args.median != 8000 or args.expansion != 50:
-        return None
+    WRITE_SIZE bytes per band cell kept with the profiles (the file `path` of the
+    profiled workload), times the launch's cells (None if that kernel is not listed)."""
     try:
-        with open(args.traffic_file) as f:
+        with open(path) as f:
             for line in f:
                 parts = line.strip().split("\t")
                 if len(parts) >= 2 and parts[0] == kname:
@@ -560,6 +564,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="timed steps without graph replay")
     ap.add_argument("--hdp-reads", type=int, default=2048)
     ap.add_argument("--hdp-only", action="store_true", help="run only the HDP configuration (for profiling it)")
+    ap.add_argument("--hdp-schedule", choices=["both", "per-strand", "one-batch"], default="both",
+                    help="the HDP leg's schedules to time (one of them: for profiling it)")
+    ap.add_argument("--hdp-traffic-file", default=os.path.join(ROOT, "profiles", "traffic_hdp_2048x2reads.txt"))
     ap.add_argument("--no-serve", action="store_true", help="skip the serving configuration")
     ap.add_argument("--no-expect", action="store_true", help="skip the EM expectations configuration")
     ap.add_argument("--expect-reads", type=int, default=512)
@@ -651,7 +658,8 @@ def main():
     # its average launch duration (HIP events on the stream each launch runs on) and
     # average cells per launch, as rocprofv3 averages per kernel name
     roofline = kernel_roofline(tms, dtype, FLOP_FWD, FLOP_BWD)
-    roofline["traffic"] = traffic_for(roofline["kernel"], roofline["kernel_cells_per_launch"], args)
+    if args.reads == 4096 and args.median == 8000 and args.expansion == 50:  # the profiled workload
+        roofline["traffic"] = traffic_for(roofline["kernel"], roofline["kernel_cells_per_launch"], args.traffic_file)
     roofline["forward_ms"] = round(float(np.mean(fwd_ms)), 3)
     roofline["backward_ms"] = round(float(np.mean(bwd_ms)), 3)
     roofline["sweeps_ms"] = round(float(np.mean([t["fb_ms"] for t in tms])), 3)
