@@ -126,6 +126,51 @@ def test_exact_logadd_switch(oracle_lib):
     assert oracle.logadd(10.0, 0.0) == 10.0
 
 
+def test_lookup_logadd_knot_sensitivity(oracle_lib):
+    """Why an FP32 recurrence is held to a looser posterior bar than FP64 (CPU experiment).
+
+    The reference's lookup logAdd (impl/pairwiseAligner.c:234-254) jumps at a difference
+    of 7.5: just below it a term still adds lookup(7.5) - 7.5 = 4.47e-4 to the log sum,
+    at 7.5 and beyond it adds nothing. A total refresh (impl/pairwiseAligner.c:1367-1379)
+    with a term that close to the knot moves by 4.47e-4 when the term moves by any amount
+    that crosses it, and every posterior of the following diagonals scales with it.
+    Read 6 of the batched workload (4644 nt, seed 2024): rounding its event means to FP32
+    (a relative change below 6e-8, the least any FP32 recurrence must make) moves the FP64
+    oracle's posteriors by more than 3e-4; the same rounding moves them by under 1e-5
+    when the oracle uses the exact log-sum-exp. So the 1e-4 posterior bar cannot hold on
+    every pair for arithmetic narrower than FP64, and the excess comes from the knots,
+    not from the rounding drift itself."""
+    from signalalign_amd import native, synth
+    from tests import _parity as P
+
+    m = synth.gaussian_model("ACEGT", 6, seed=1)
+    r = synth.batch_reads(m, 7, 8000, seed=2024, methyl_cpg=0.3)[6]
+    assert len(r["ref"]) == 4644
+    r32 = dict(r)
+    r32["events"] = np.asarray(r["events"], dtype=np.float32).astype(np.float64)
+    assert np.max(np.abs(r32["events"][:, 0] / r["events"][:, 0] - 1)) < 6e-8
+    params = native.Params(diagonal_expansion=50)
+
+    def max_dp(exact):
+        oracle.set_exact_logadd(exact)
+        try:
+            a, sa = P.run_oracle(m, r, params)
+            b, sb = P.run_oracle(m, r32, params)
+        finally:
+            oracle.set_exact_logadd(False)
+        ga = {(x, y): p / 1e7 for p, x, y in a}
+        gb = {(x, y): p / 1e7 for p, x, y in b}
+        common = [k for k in ga if k in gb]
+        assert len(common) > 5000
+        return max(abs(ga[k] - gb[k]) for k in common), P.rel(sb["log_likelihood"], sa["log_likelihood"])
+
+    dp_lookup, ll_lookup = max_dp(False)
+    dp_exact, _ = max_dp(True)
+    assert dp_lookup > 3e-4, dp_lookup
+    assert dp_exact < 1e-5, dp_exact
+    assert ll_lookup < 1e-8  # the log-likelihood keeps the FP64 bar
+
+
 def test_emission_pdfs_closed_form(oracle_lib):
     """tests/signalPairwiseAlignerTest.c:81-111 at 1e-3, here against scipy at 1e-12."""
     rng = np.random.default_rng(9)
