@@ -212,15 +212,20 @@ def kernel_roofline(tms, dtype, flop_fwd, flop_bwd):
     agg = {}
     for t in tms:
         for L in t["launches"]:
-            if L["direction"] == 0:
+            if L["direction"] == 0 and L.get("persistent", 0):
+                name = f"sa::fb_persistent_kernel<{tname}, {ek}>"
+            elif L["direction"] == 0:
                 name = (f"sa::forward_sw_kernel<{tname}, {L['cpt']}, {ek}>" if L["cpt"] and L["waves"] == 1
+                        else f"sa::forward_halo_kernel<{tname}, {ek}, {L['waves']}>" if L.get("sync_every", 0) > 1
                         else f"sa::forward_reg_kernel<{tname}, {L['cpt']}, {ek}, {L['waves']}>" if L["cpt"]
                         else f"sa::forward_kernel<{tname}, ...>")
             else:
                 name = (f"sa::backward_reg_kernel<{tname}, {L['cpt']}, {ek}>" if L["cpt"]
                         else f"sa::backward_kernel<{tname}, ...>")
-            a_ = agg.setdefault(name, {"n": 0, "ms": 0.0, "cells": 0, "flop": flop_fwd if L["direction"] == 0
-                                       else flop_bwd})
+            # the persistent launch runs the forward sweeps and the windows of its items
+            flop = (flop_fwd + flop_bwd if L.get("persistent", 0) else flop_fwd if L["direction"] == 0
+                    else flop_bwd)
+            a_ = agg.setdefault(name, {"n": 0, "ms": 0.0, "cells": 0, "flop": flop})
             a_["n"] += 1
             a_["ms"] += L["ms"]
             a_["cells"] += L["cells"]
@@ -647,7 +652,9 @@ def main():
             f"{cells / tm['total_ms'] / 1e6:.1f} GCUPS")
     for L in tms[-1]["launches"]:
         log(f"  launch stream {L['stream']} {'fwd' if L['direction'] == 0 else 'bwd'} cpt {L['cpt']} waves "
-            f"{L['waves']} items {L['items']} cells {L['cells']} start {L['start_ms']:.1f} ms dur {L['ms']:.1f} ms")
+            f"{L['waves']} sync {L['sync_every']} persistent {L['persistent']} items {L['items']} cells {L['cells']} "
+            f"start {L['start_ms']:.1f} ms "
+            f"dur {L['ms']:.1f} ms")
     res, _ = batch.run(prec, fetch=True)  # the pairs, for the error reports
     n_pairs = sum(len(r.pairs) for r in res)
     ms = float(np.mean(dev_ms))

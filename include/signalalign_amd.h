@@ -178,6 +178,11 @@ typedef struct {
     int64_t cells;      /* band cells those items cover */
     double start_ms;    /* from the start of the sweeps (HIP events on its stream) */
     double ms;          /* duration */
+    int32_t sync_every; /* several-wavefront forward sweeps: diagonals between two
+                           synchronisations of the wavefronts (1, or 8 with halo lanes);
+                           0 otherwise */
+    int32_t persistent; /* 1: the persistent launch (direction 0), which also runs the
+                           traceback windows of its items (cells = forward cells) */
 } sa_launch;
 
 typedef struct sa_context sa_context;
@@ -242,12 +247,19 @@ void sa_batch_destroy(sa_batch *batch);
  * cap of the one-wavefront sweeps (0: none).  long_factor: sub-alignments with
  * lX + lY >= max(20000, long_factor x median) run their forward sweep on two to four
  * wavefronts, the fewest whose window holds the band (0: 2.5; negative: never).  stream_split: fraction of the one-wavefront forward
- * cells (longest sub-alignments first) on stream 0, the rest on stream 3 (0: 0.5). */
+ * cells (longest sub-alignments first) on stream 0, the rest on stream 3 (0: 0.5).
+ * persist_waves: 0 or -1, a forward launch and then backward launches per stream
+ * (stream_split applies).  1-8 would ask for a persistent launch (the one-wavefront
+ * sub-alignments' forward sweeps longest first, then the traceback windows of those
+ * done, by worker wavefronts, this many per SIMD); it hung on the device with its
+ * cause not found, so sa_batch_set_tuning refuses it with SA_ERR_UNSUPPORTED. */
 typedef struct {
     int32_t use_graph;
     int32_t waves_per_simd;
     double long_factor;
     double stream_split;
+    int32_t persist_waves;
+    int32_t reserved;
 } sa_tuning;
 void sa_tuning_default(sa_tuning *t);
 int sa_batch_set_tuning(sa_batch *batch, const sa_tuning *t);

@@ -17,7 +17,7 @@ constexpr int kPairProb1 = 10000000;      /* PAIR_ALIGNMENT_PROB_1 */
 constexpr int kMaxRing = 1024;           /* widest band diagonal the kernels accept */
 
 /* Error bits raised by device prep. */
-enum : int32_t { ERR_BAD_KMER = 1, ERR_BAND = 2, ERR_WINDOWS = 4 };
+enum : int32_t { ERR_BAD_KMER = 1, ERR_BAND = 2, ERR_WINDOWS = 4, ERR_QUEUE = 8 };
 
 /* One sub-alignment (a split block of one read). */
 struct TaskDev {

@@ -121,13 +121,29 @@ __device__ __forceinline__ float logadd_lut(float a, float b, const LutEntry<flo
 }
 
 /* The FP32 form with its coefficients selected by compares (no table): bit-identical
-   to logadd_lut(float); the LDS sweeps use it. */
+   to logadd_lut(float).  The constants are the table's, folded at compile time with
+   the same float operations as lut_coeffs_f32. */
+namespace la32 {
+constexpr float C[4][4] = {
+    {-0.009350833524763f, 0.130659527668286f, 0.498799810682272f, 0.693203116424741f},
+    {-0.014532321752540f, 0.139942324101744f, 0.495635523139337f, 0.692140569840976f},
+    {-0.004605031767994f, 0.063427417320019f, 0.695956496475118f, 0.514272634594009f},
+    {-0.000458661602210f, 0.009695946122598f, 0.930734667215156f, 0.168037164329057f}};
+constexpr float k0(int p) { return C[p][0] * 0.125f; }
+constexpr float k1(int p) { return C[p][1] * 0.25f; }
+constexpr float k2(int p) { return (C[p][2] - 1.0f) * 0.5f; }
+constexpr float k3(int p) { return C[p][3]; }
+} // namespace la32
 __device__ __forceinline__ float logadd(float a, float b) {
     const float hi = __builtin_fmaxf(a, b);
     const float u = la_u(a - b);
-    float c[4];
-    lut_coeffs_f32(u <= 2.0f ? 0 : u <= 5.0f ? 1 : u <= 9.0f ? 2 : u <= 15.0f ? 3 : 4, c);
-    return hi + __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(c[0], u, c[1]), u, c[2]), u, c[3]);
+    const bool p0 = u <= 2.0f, p1 = u <= 5.0f, p2 = u <= 9.0f, p3 = u <= 15.0f;
+    using namespace la32;
+    const float c0 = p0 ? k0(0) : p1 ? k0(1) : p2 ? k0(2) : p3 ? k0(3) : 0.0f;
+    const float c1 = p0 ? k1(0) : p1 ? k1(1) : p2 ? k1(2) : p3 ? k1(3) : 0.0f;
+    const float c2 = p0 ? k2(0) : p1 ? k2(1) : p2 ? k2(2) : p3 ? k2(3) : 0.0f;
+    const float c3 = p0 ? k3(0) : p1 ? k3(1) : p2 ? k3(2) : p3 ? k3(3) : 0.0f;
+    return hi + __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(c0, u, c1), u, c2), u, c3);
 }
 
 __device__ __forceinline__ double logadd_lut(double a, double b, const LutEntry<double> *lut) {

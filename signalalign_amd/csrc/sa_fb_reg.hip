@@ -224,6 +224,16 @@ __device__ __forceinline__ int ld_scalar_byte(const int8_t *p, int i) {
     const int32_t w = ld_scalar((const int32_t *)(ad & ~(uintptr_t)3), 0);
     return (w >> (int)((ad & 3) * 8)) & 0xff;
 }
+/* the aligned dword holding flag byte i, extracted later (get) so that the load's wait
+   falls where the flag is used */
+struct FlagWord {
+    int32_t w, sh;
+    __device__ __forceinline__ int get() const { return (w >> sh) & 0xff; }
+};
+__device__ __forceinline__ FlagWord ld_flag_word(const int8_t *p, int i) {
+    const uintptr_t ad = (uintptr_t)p + (uintptr_t)__builtin_amdgcn_readfirstlane(i);
+    return FlagWord{ld_scalar((const int32_t *)(ad & ~(uintptr_t)3), 0), (int)((ad & 3) * 8)};
+}
 template <typename T>
 __device__ __forceinline__ void load_econ_uniform(ECon<T, 2> &e, const Args<T> &a, int64_t base, int j, bool ok) {
     e.m = ok ? ld_scalar(a.evm + base, j) : (T)0;
@@ -582,6 +592,294 @@ __global__ void __launch_bounds__(64 * NW) forward_reg_kernel(Args<T> a, const i
 }
 
 /* ------------------------------------------------------------------------ */
+/* forward, several wavefronts synchronised every kHaloK diagonals           */
+/* ------------------------------------------------------------------------ */
+
+/* Wave w's lane i holds position X0 + kHaloS*w + i; for w > 0 its first kHaloH lanes
+ * repeat the last positions of wave w-1 (a halo), the other lanes are its own.  Between
+ * two synchronisations a wavefront runs on its own: its first lane takes -inf for the
+ * positions left of it.  A cell reads x-1 on d-1 and d-2 and x on d-1, so the wrong
+ * values spread by at most one position per diagonal and stay inside the halo for
+ * kHaloK = kHaloH diagonals.  Every kHaloK-th diagonal (where the FP32 rows are also
+ * renormalised) each wavefront writes the states of d and d-1 at the positions it owns
+ * to LDS, the window slides if the next kHaloK diagonals need it, and every lane reads
+ * its position's states back: the halo lanes are exact again.  One barrier per
+ * kHaloK diagonals instead of one per diagonal; the arithmetic of every stored cell is
+ * the other sweeps' own, so the planes are bit-identical to theirs.  Each diagonal step
+ * is one cell's dependency chain; its scalar loads (band bounds and flag, the first
+ * lane's event) are issued two diagonals ahead, so that the step's LDS waits (the
+ * log-add table; they share a counter with scalar loads) find them done.  The table
+ * form of the log-add measured faster here than selecting the cubic by compares
+ * (0.70 against 0.76 us per diagonal on a 50 kb read alone).  The host picks
+ * this sweep when the band fits the window at every synchronisation:
+ * xhi(min(d + kHaloK, N)) - min(xlo(d-1), xlo(d)) + 1 <= W at d = 0, kHaloK, 2 kHaloK, ...
+ * (prep_kernel's second span per sub-alignment). */
+template <typename T, int EK, int NW>
+__global__ void __launch_bounds__(64 * NW) forward_halo_kernel(Args<T> a, const int32_t *__restrict__ order) {
+    static_assert(NW > 1, "one-wavefront sweeps are forward_sw_kernel");
+    constexpr bool NORM = sizeof(T) == 4;
+    constexpr int W = 64 + kHaloS * (NW - 1); /* positions of the window */
+    /* states of d and d-1 by position, double-buffered by synchronisation parity (a
+       wavefront writes the next buffer only after the barrier every wavefront reaches
+       once it has read this one) */
+    __shared__ T shb[2][6 * W];
+    __shared__ T red[2][NW];
+    __shared__ LutEntry<T> lut[LutSize<T>::n];
+    set_priority(a.prio);
+    lut_fill(lut, (int)threadIdx.x);
+    __syncthreads();
+    const int ti = order[blockIdx.x];
+    const TaskDev t = a.tasks[ti];
+    const Model<T> &m = a.m;
+    const int gl = threadIdx.x;
+    const int lane = gl & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(gl >> 6); /* uniform: scalar branches on it */
+    const int woff = kHaloS * wave;                            /* the wavefront's first position */
+    const bool own = wave == 0 || lane >= kHaloH;              /* position owned (stored, exchanged) */
+    const int lX = t.lX, lY = t.lY, N = lX + lY;
+    const int32_t *xlo = a.xlo + t.diag_off;
+    const int32_t *xhi = a.xhi + t.diag_off;
+    const int8_t *dfl = a.dflag + t.diag_off;
+    int32_t *doff = a.doff + t.diag_off;
+    int32_t *doffxy = a.doffxy + t.diag_off;
+    double *foff = a.foff + t.diag_off;
+    const int32_t *kid = a.kid + t.pos_off;
+    const int64_t evc = t.evc_off;
+    T *fM = a.fM + t.cell_off;
+    T *fX = a.fX + t.xy_off;
+    T *fY = a.fY + t.xy_off;
+    const T s = (T)(m.kind == 1 ? 1.0 : t.scale / t.var);
+    const T iv = (T)(1.0 / t.var);
+    const T tMC = m.trans[0], tMFX = m.trans[1], tMFY = m.trans[2], tGOX = m.trans[3], tGOY = m.trans[4];
+    const T tGEX = m.trans[5], tGEY = m.trans[6], tSWX = m.trans[7];
+    const T NI = ninf<T>();
+    const bool useSWX = m.use_swx != 0;
+
+    /* plane offsets of each diagonal, written 64 diagonals at a time by wave 0 (lane j
+       holds diagonal dc + j) */
+    int dc = 0;
+    int md_doff = 0, md_dxy = -1;
+    double md_foff = 0.0;
+
+    int X0 = 0;
+    KCon<T, EK> kc;
+    ECon<T, EK> ev;
+    T pM, pX, pY, qM, qX, qY;
+    T LpM = NI, LpX = NI, LpY = NI;
+
+    /* diagonal 0: start or ragged-start values (impl/stateMachine.c:1022-1031); every
+       lane evaluates its own position, so the halo starts exact */
+    const int lo0 = ld_scalar(xlo, 0), hi0 = ld_scalar(xhi, 0), f0 = ld_scalar_byte(dfl, 0);
+    /* bounds and flag of d+1 and d+2: each diagonal's scalar loads are issued two
+       diagonals before their use */
+    int nlo = 0, nhi = -1, nlo2 = 0, nhi2 = -1;
+    FlagWord nfl{0, 0}, nfl2{0, 0};
+    if (N >= 1) { nlo = ld_scalar(xlo, 1); nhi = ld_scalar(xhi, 1); nfl = ld_flag_word(dfl, 1); }
+    if (N >= 2) { nlo2 = ld_scalar(xlo, 2); nhi2 = ld_scalar(xhi, 2); nfl2 = ld_flag_word(dfl, 2); }
+    {
+        const int x = woff + lane;
+        const bool v = x >= lo0 && x <= hi0;
+        pM = v ? (t.ragged_left ? NI : (T)0) : NI;
+        pX = v ? (t.ragged_left ? (T)0 : NI) : NI;
+        pY = pX;
+        qM = NI; qX = NI; qY = NI;
+        if (v && own) {
+            fM[x - lo0] = pM;
+            if (f0) { fX[x - lo0] = pX; fY[x - lo0] = pY; }
+        }
+        load_kcon(kc, m, kid, x - 1, x >= 1 && x <= lX);
+        const int j = 1 - x - 1;
+        load_econ(ev, a, evc + j, j >= 0 && j < lY);
+        /* complete the loads here: a load still pending at the loop's entry makes every
+           iteration wait on the vector memory counter, which also counts the stores */
+        sink(kc);
+        sink(ev);
+    }
+    if (gl == 0) { md_doff = 0; md_dxy = f0 ? 0 : -1; md_foff = 0.0; }
+    int64_t run = hi0 - lo0 + 1;
+    int64_t runxy = f0 ? run : 0;
+    double off1 = 0.0, off2 = 0.0;
+    int lo1 = lo0;
+    /* pre: the event the wavefront's first lane reads on d+1 (cell x on d reads event
+       d - x - 1); pre2, the one of d+2, loaded a diagonal earlier through the scalar
+       cache.  Diagonal 1's events are loaded above, so pre2 starts as diagonal 2's. */
+    ECon<T, EK> pre, pre2;
+    load_econ_uniform(pre2, a, evc, 1 - X0 - woff, 1 - X0 - woff >= 0 && 1 - X0 - woff < lY);
+    pre = pre2;
+    bool fresh = true;
+    constexpr bool PF = EK == 2;
+    HdpFetch<T> gp;
+    bool have_gp = false;
+    constexpr bool PL = EK == 3;
+    const T *em = PL ? a.em + t.cell_off : nullptr;
+    T pe = (T)0;
+    bool have_pe = false;
+    bool bad = false;
+    int sp = 0; /* synchronisation parity */
+
+    for (int d = 1; d <= N; d++) {
+        int j = d - dc;
+        if (j == 64) {
+            if (wave == 0 && dc + lane <= N) {
+                doff[dc + lane] = md_doff; doffxy[dc + lane] = md_dxy; foff[dc + lane] = md_foff;
+            }
+            dc += 64;
+            j = 0;
+        }
+        const int lo = nlo, hi = nhi;
+        const FlagWord flw = nfl;
+        nlo = nlo2; nhi = nhi2; nfl = nfl2;
+        if (d + 2 <= N) { nlo2 = ld_scalar(xlo, d + 2); nhi2 = ld_scalar(xhi, d + 2); nfl2 = ld_flag_word(dfl, d + 2); }
+        const bool sync = (d % kHaloK) == 0;
+        /* the band's right end kHaloK diagonals on (the next synchronisation's window) */
+        const int hnext = sync ? ld_scalar(xhi, d + kHaloK < N ? d + kHaloK : N) : 0;
+        bad |= lo < X0; /* bounds are non-decreasing; the sweep runs on to a single exit */
+        if (!fresh) {
+            /* cell x on d reads event d-x-1: its left neighbour's event of d-1 */
+            ev = shr1(ev, pre);
+        }
+        fresh = false;
+        pre = pre2; /* the first lane's event on d+1 */
+        {
+            const int jn = d + 1 - X0 - woff; /* on d+2 */
+            load_econ_uniform(pre2, a, evc, jn, jn >= 0 && jn < lY);
+        }
+        const T LM = shr1(pM, NI), LX = shr1(pX, NI), LY = shr1(pY, NI);
+        if (NORM && off2 != off1) {
+            /* d-2 carries its own offset (the diagonal after a renormalised one): bring
+               its states to d-1's */
+            const T dl2 = (T)(off2 - off1);
+            qM += dl2; qX += dl2; qY += dl2;
+            LpM += dl2; LpX += dl2; LpY += dl2;
+        }
+        T nM, nX, nY;
+        {
+            const int x = X0 + woff + lane;
+            const bool valid = x >= lo && x <= hi;
+            const T eX = em_gapx<T>(m, kc);
+            T eM, eY;
+            if constexpr (PL) {
+                eM = have_pe ? pe : valid ? em[run + (x - lo)] : (T)0;
+                eY = eM;
+            } else if (PF && have_gp) {
+                eM = hdp_finish(m, gp, iv);
+                eY = eM;
+            } else {
+                em_both<T>(m, kc, ev, s, iv, eM, eY);
+            }
+            /* lower -> gapX, middle -> match, upper -> gapY */
+            T X = LM + (eX + tGOX);
+            X = logadd_lut(X, LX + (eX + tGEX), lut);
+            if (useSWX) X = logadd_lut(X, LY + (eX + tSWX), lut);
+            T M = LpM + (eM + tMC);
+            M = logadd_lut(M, LpX + (eM + tMFX), lut);
+            M = logadd_lut(M, LpY + (eM + tMFY), lut);
+            T Y = pM + (eY + tGOY);
+            Y = logadd_lut(Y, pY + (eY + tGEY), lut);
+            nM = valid ? M : NI;
+            nX = valid ? X : NI;
+            nY = valid ? Y : NI;
+        }
+        if constexpr (PF) {
+            /* HDP: the density gathers of diagonal d+1, issued a diagonal ahead */
+            const ECon<T, EK> inc = shr1(ev, pre);
+            gp = hdp_fetch(m, kc.row, kc.hmu, inc.m, s);
+            have_gp = true;
+        }
+        if constexpr (PL) {
+            /* the emissions of d+1, read a diagonal ahead */
+            const int64_t nrun = run + (hi - lo + 1);
+            const int x = X0 + woff + lane;
+            pe = d < N && x >= nlo && x <= nhi ? em[nrun + (x - nlo)] : (T)0;
+            have_pe = true;
+        }
+        double offd = off1;
+        if (sync) {
+            /* renormalisation (FP32) over the positions owned; the states of d and d-1
+               at every owned position cross through LDS */
+            if constexpr (NORM) {
+                T m3 = nM > nX ? nM : nX;
+                m3 = m3 > nY ? m3 : nY;
+                const T mx = wave_max(own ? m3 : NI);
+                if (lane == 0) red[sp][wave] = mx;
+            }
+            T *sb = shb[sp];
+            if (own) {
+                const int i = woff + lane;
+                sb[0 * W + i] = nM; sb[1 * W + i] = nX; sb[2 * W + i] = nY;
+                sb[3 * W + i] = pM; sb[4 * W + i] = pX; sb[5 * W + i] = pY;
+            }
+            __syncthreads();
+            /* slide as far as d and d-1 allow when the next kHaloK diagonals need it */
+            int X0n = X0;
+            if (hnext > X0 + W - 1) {
+                X0n = lo1 < lo ? lo1 : lo;
+                X0n = X0n > X0 ? X0n : X0;
+                bad |= hnext > X0n + W - 1;
+            }
+            {
+                const int i = X0n - X0 + woff + lane;
+                const bool in = i < W;
+                const int ii = in ? i : 0;
+                nM = in ? sb[0 * W + ii] : NI; nX = in ? sb[1 * W + ii] : NI; nY = in ? sb[2 * W + ii] : NI;
+                pM = in ? sb[3 * W + ii] : NI; pX = in ? sb[4 * W + ii] : NI; pY = in ? sb[5 * W + ii] : NI;
+            }
+            if constexpr (NORM) {
+                T mx = red[sp][0];
+#pragma unroll
+                for (int w = 1; w < NW; w++) mx = red[sp][w] > mx ? red[sp][w] : mx;
+                if (mx > NI) {
+                    nM -= mx; nX -= mx; nY -= mx;
+                    offd = off1 + (double)mx;
+                }
+            }
+            sp ^= 1;
+            if (X0n != X0) {
+                X0 = X0n;
+                /* constants of the new positions for diagonal d+1 (cell x reads k-mer
+                   x-1 and event d-x) */
+                const int x = X0 + woff + lane;
+                load_kcon(kc, m, kid, x - 1, x >= 1 && x <= lX);
+                const int jj = d - x;
+                load_econ(ev, a, evc + jj, jj >= 0 && jj < lY);
+                sink(kc);
+                sink(ev);
+                /* the first lane's event on d+2 from the new window */
+                const int jn = d + 1 - X0 - woff;
+                load_econ_uniform(pre2, a, evc, jn, jn >= 0 && jn < lY);
+                fresh = true;
+                have_gp = false;
+                have_pe = false;
+            }
+        }
+        const int fl = flw.get(); /* extracted here, a diagonal after its load was issued */
+        {
+            const int x = X0 + woff + lane;
+            if (own && x >= lo && x <= hi) {
+                fM[run + (x - lo)] = nM;
+                if (fl) { fX[runxy + (x - lo)] = nX; fY[runxy + (x - lo)] = nY; }
+            }
+        }
+        if (gl == j) { md_doff = (int32_t)run; md_dxy = fl ? (int32_t)runxy : -1; md_foff = offd; }
+        run += hi - lo + 1;
+        if (fl) runxy += hi - lo + 1;
+        qM = pM; qX = pX; qY = pY;
+        pM = nM; pX = nX; pY = nY;
+        if (sync) {
+            /* the middle neighbours of d+1 from the exchanged states of d-1 */
+            LpM = shr1(qM, NI); LpX = shr1(qX, NI); LpY = shr1(qY, NI);
+        } else {
+            LpM = LM; LpX = LX; LpY = LY;
+        }
+        off2 = off1;
+        off1 = offd;
+        lo1 = lo;
+    }
+    if (wave == 0 && dc + lane <= N) { doff[dc + lane] = md_doff; doffxy[dc + lane] = md_dxy; foff[dc + lane] = md_foff; }
+    if (bad && gl == 0) atomicOr(a.err, ERR_WINDOWS);
+}
+
+/* ------------------------------------------------------------------------ */
 /* forward, one wavefront, window width following the band                  */
 /* ------------------------------------------------------------------------ */
 
@@ -906,15 +1204,10 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
 
 /* One wavefront per sub-alignment; the window runs at 1, 2 or 3 cells per lane as the
  * band's width allows (block classes from prep_kernel), never wider than CMAX. */
+/* The one-wavefront forward sweep of one sub-alignment (the kernel below and the
+ * persistent sweeps' forward item). */
 template <typename T, int CMAX, int EK>
-__global__ void __launch_bounds__(64) forward_sw_kernel(Args<T> a, const int32_t *__restrict__ order) {
-    extern __shared__ __align__(16) unsigned char dsm[];
-    T *xs = reinterpret_cast<T *>(dsm); /* [6][64 * CMAX]: states across a width change */
-    __shared__ LutEntry<T> lut[LutSize<T>::n];
-    set_priority(a.prio);
-    lut_fill(lut, (int)threadIdx.x);
-    __syncthreads();
-    const TaskDev t = a.tasks[order[blockIdx.x]];
+__device__ __forceinline__ void fwd_task(const Args<T> &a, const TaskDev t, T *xs, const LutEntry<T> *lut) {
     const int lane = threadIdx.x;
     const int N = t.lX + t.lY;
     const int8_t *dcl = a.dcls + t.diag_off;
@@ -939,6 +1232,17 @@ __global__ void __launch_bounds__(64) forward_sw_kernel(Args<T> a, const int32_t
     double *foff = a.foff + t.diag_off;
     if (st.dc + lane <= N) { doff[st.dc + lane] = st.md_doff; doffxy[st.dc + lane] = st.md_dxy; foff[st.dc + lane] = st.md_foff; }
     if (st.bad && lane == 0) atomicOr(a.err, ERR_WINDOWS);
+}
+
+template <typename T, int CMAX, int EK>
+__global__ void __launch_bounds__(64) forward_sw_kernel(Args<T> a, const int32_t *__restrict__ order) {
+    extern __shared__ __align__(16) unsigned char dsm[];
+    T *xs = reinterpret_cast<T *>(dsm); /* [6][64 * CMAX]: states across a width change */
+    __shared__ LutEntry<T> lut[LutSize<T>::n];
+    set_priority(a.prio);
+    lut_fill(lut, (int)threadIdx.x);
+    __syncthreads();
+    fwd_task<T, CMAX, EK>(a, a.tasks[order[blockIdx.x]], xs, lut);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1460,18 +1764,10 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
    to scratch and still run 10% faster.  Otherwise the compiler's choice. */
 template <typename T, int CMAX> struct BwdWaves { static constexpr int n = sizeof(T) == 8 && CMAX == 3 ? 2 : 1; };
 
+/* One traceback window (the kernel below and the persistent sweeps' window item). */
 template <typename T, int CMAX, int EK>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BwdWaves<T, CMAX>::n)))
-backward_reg_kernel(Args<T> a, const int32_t *__restrict__ order) {
-    extern __shared__ __align__(16) unsigned char dsm[];
-    T *xs = reinterpret_cast<T *>(dsm); /* [4][64 * CMAX]: states across a width change */
-    __shared__ LutEntry<T> lut[LutSize<T>::n];
-    __shared__ LutEntry<double> lutd2[sizeof(T) == 8 ? 1 : 16]; /* FP64 table for the totals */
-    lut_fill(lut, (int)threadIdx.x);
-    if constexpr (sizeof(T) != 8) lut_fill(lutd2, (int)threadIdx.x);
-    __syncthreads();
-    const LutEntry<double> *lutd = sizeof(T) == 8 ? reinterpret_cast<const LutEntry<double> *>(lut) : lutd2;
-    const int wi = order[blockIdx.x];
+__device__ __forceinline__ void bwd_window(const Args<T> &a, const int wi, T *xs, const LutEntry<T> *lut,
+                                           const LutEntry<double> *lutd) {
     const WinDev Wd = a.wins[wi];
     const TaskDev t = a.tasks[Wd.task];
     const int lane = threadIdx.x;
@@ -1498,6 +1794,120 @@ backward_reg_kernel(Args<T> a, const int32_t *__restrict__ order) {
     }
     if (lane == 0) a.win_count[wi] = st.cursor;
     if (st.bad && lane == 0) atomicOr(a.err, ERR_WINDOWS);
+}
+
+template <typename T, int CMAX, int EK>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(BwdWaves<T, CMAX>::n)))
+backward_reg_kernel(Args<T> a, const int32_t *__restrict__ order) {
+    extern __shared__ __align__(16) unsigned char dsm[];
+    T *xs = reinterpret_cast<T *>(dsm); /* [4][64 * CMAX]: states across a width change */
+    __shared__ LutEntry<T> lut[LutSize<T>::n];
+    __shared__ LutEntry<double> lutd2[sizeof(T) == 8 ? 1 : 16]; /* FP64 table for the totals */
+    lut_fill(lut, (int)threadIdx.x);
+    if constexpr (sizeof(T) != 8) lut_fill(lutd2, (int)threadIdx.x);
+    __syncthreads();
+    const LutEntry<double> *lutd = sizeof(T) == 8 ? reinterpret_cast<const LutEntry<double> *>(lut) : lutd2;
+    bwd_window<T, CMAX, EK>(a, order[blockIdx.x], xs, lut, lutd);
+}
+
+/* ------------------------------------------------------------------------ */
+/* persistent sweeps: forward items and their traceback windows, one launch  */
+/* ------------------------------------------------------------------------ */
+
+/* Worker wavefronts claim forward items in the launch's order (longest first).  A
+ * worker whose sweep is done makes the sub-alignment's planes and per-diagonal offsets
+ * visible device-wide (agent-scope release) and publishes it in the next slot of
+ * `done`.  Once every forward item is claimed, workers take the traceback windows of
+ * the published sub-alignments in publication order (an agent-scope acquire before
+ * each), each window once.  A worker waits only for a sub-alignment whose forward
+ * sweep another running worker has claimed, and forward sweeps never wait, so every
+ * wait ends; a worker that makes no progress for wait_ticks of the real-time clock
+ * (seconds; a step takes milliseconds) raises ERR_QUEUE and leaves, so the launch ends
+ * whatever the queue holds.  The sub-alignments' per-diagonal arrays
+ * start on 128-byte boundaries (sa_batch_create), so no scalar-cache line a window's
+ * sweep reads holds another sub-alignment's offsets. */
+/* registers for three workers per SIMD in FP32, as the separate sweeps get (more spill to scratch),
+   two in FP64 (as the backward sweep at three positions per lane) */
+template <typename T> struct PersistWaves { static constexpr int n = sizeof(T) == 8 ? 2 : 3; };
+
+template <typename T, int EK>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PersistWaves<T>::n)))
+fb_persistent_kernel(Args<T> a, PersistQ q) {
+    constexpr int CMAX = 3;
+    extern __shared__ __align__(16) unsigned char dsm[];
+    T *xs = reinterpret_cast<T *>(dsm); /* [6][64 * CMAX] forward, [4][64 * CMAX] backward */
+    __shared__ LutEntry<T> lut[LutSize<T>::n];
+    __shared__ LutEntry<double> lutd2[sizeof(T) == 8 ? 1 : 16]; /* FP64 table for the totals */
+    lut_fill(lut, (int)threadIdx.x);
+    if constexpr (sizeof(T) != 8) lut_fill(lutd2, (int)threadIdx.x);
+    __syncthreads();
+    const LutEntry<double> *lutd = sizeof(T) == 8 ? reinterpret_cast<const LutEntry<double> *>(lut) : lutd2;
+    const int lane = threadIdx.x;
+    /* forward items: their chains are the launch's longest, so they issue first on a
+       shared SIMD */
+    __builtin_amdgcn_s_setprio(1);
+    while (true) {
+        int f = 0;
+        if (lane == 0) f = atomicAdd(&q.ctr[0], 1);
+        f = __builtin_amdgcn_readfirstlane(f);
+        if (f >= q.n_fwd) break;
+        const int ti = q.order[f];
+        fwd_task<T, CMAX, EK>(a, a.tasks[ti], xs, lut);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (lane == 0) {
+            const int slot = atomicAdd(&q.ctr[1], 1);
+            atomicExch(&q.done[slot], ti + 1); /* a read-modify-write, as every queue access */
+        }
+    }
+    __builtin_amdgcn_s_setprio(0);
+    /* windows of the published sub-alignments.  Every access to the queue, reads
+       included, is an atomic read-modify-write (an OR with a zero the compiler cannot
+       see, so that no read becomes a plain load): all of them meet at the one point
+       where read-modify-writes are ordered.  The cursor moves by the compare-and-swap's
+       own result.  A worker that makes no progress for q.wait_ticks of the 100 MHz
+       real-time clock raises ERR_QUEUE and leaves, however long its naps are. */
+    const int zero = q.zero;
+    uint64_t last = __builtin_amdgcn_s_memrealtime();
+    int k = 0;
+    if (lane == 0) k = atomicOr(&q.ctr[2], zero);
+    k = __builtin_amdgcn_readfirstlane(k);
+    int naps = 1;
+    while (k < q.n_fwd) {
+        int t1 = 0;
+        if (lane == 0) t1 = atomicOr(&q.done[k], zero);
+        t1 = __builtin_amdgcn_readfirstlane(t1);
+        if (t1 != 0) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            const int ti = t1 - 1;
+            int w = 0;
+            if (lane == 0) w = atomicAdd(&q.wclaim[ti], 1);
+            w = __builtin_amdgcn_readfirstlane(w);
+            if (w < a.task_nwin[ti]) {
+                bwd_window<T, CMAX, EK>(a, q.first_win[ti] + w, xs, lut, lutd);
+                last = __builtin_amdgcn_s_memrealtime();
+                naps = 1;
+                continue; /* the same publication slot may hold more windows */
+            }
+            /* this sub-alignment's windows are all taken: move the cursor past it (or
+               follow whoever did) */
+            int old = 0;
+            if (lane == 0) old = atomicCAS(&q.ctr[2], k, k + 1);
+            old = __builtin_amdgcn_readfirstlane(old);
+            k = old == k ? k + 1 : old;
+            last = __builtin_amdgcn_s_memrealtime();
+        } else {
+            /* the k-th forward sweep is still running: back off so that the waiting
+               workers do not crowd one address */
+            for (int z = 0; z < naps; z++) __builtin_amdgcn_s_sleep(127);
+            naps = naps < 8 ? naps + 1 : 8;
+            if (lane == 0) k = atomicOr(&q.ctr[2], zero);
+            k = __builtin_amdgcn_readfirstlane(k);
+            if (__builtin_amdgcn_s_memrealtime() - last > (uint64_t)q.wait_ticks) {
+                if (lane == 0) atomicOr(a.err, ERR_QUEUE);
+                break;
+            }
+        }
+    }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1777,6 +2187,41 @@ hipError_t launch_forward_reg(const Args<T> &a, const int32_t *order, int n, int
     }
 }
 
+template <typename T, int EK>
+static hipError_t fwd_halo_inst(const Args<T> &a, const int32_t *order, int n, int nw, hipStream_t st) {
+    switch (nw) {
+    case 2: hipLaunchKernelGGL((forward_halo_kernel<T, EK, 2>), dim3(n), dim3(128), 0, st, a, order); break;
+    case 3: hipLaunchKernelGGL((forward_halo_kernel<T, EK, 3>), dim3(n), dim3(192), 0, st, a, order); break;
+    default: hipLaunchKernelGGL((forward_halo_kernel<T, EK, 4>), dim3(n), dim3(256), 0, st, a, order); break;
+    }
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_forward_halo(const Args<T> &a, const int32_t *order, int n, int nw, int ek, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    if (nw < 2 || nw > 4) return hipErrorInvalidValue;
+    switch (ek) {
+    case 0: return fwd_halo_inst<T, 0>(a, order, n, nw, st);
+    case 1: return fwd_halo_inst<T, 1>(a, order, n, nw, st);
+    case 2: return fwd_halo_inst<T, 2>(a, order, n, nw, st);
+    default: return fwd_halo_inst<T, 3>(a, order, n, nw, st);
+    }
+}
+
+template <typename T>
+hipError_t launch_fb_persistent(const Args<T> &a, const PersistQ &q, int workers, int ek, hipStream_t st) {
+    if (q.n_fwd == 0 || workers <= 0) return hipSuccess;
+    const size_t shmem = sizeof(T) * 6 * 64 * 3;
+    switch (ek) {
+    case 0: hipLaunchKernelGGL((fb_persistent_kernel<T, 0>), dim3(workers), dim3(64), shmem, st, a, q); break;
+    case 1: hipLaunchKernelGGL((fb_persistent_kernel<T, 1>), dim3(workers), dim3(64), shmem, st, a, q); break;
+    case 2: hipLaunchKernelGGL((fb_persistent_kernel<T, 2>), dim3(workers), dim3(64), shmem, st, a, q); break;
+    default: hipLaunchKernelGGL((fb_persistent_kernel<T, 3>), dim3(workers), dim3(64), shmem, st, a, q); break;
+    }
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_backward_reg(const Args<T> &a, const int32_t *order, int n, int cpt, int ek, size_t shmem,
                                hipStream_t st) {
@@ -1796,6 +2241,10 @@ template hipError_t launch_forward_reg<float>(const Args<float> &, const int32_t
                                               hipStream_t);
 template hipError_t launch_forward_reg<double>(const Args<double> &, const int32_t *, int, int, int, int, size_t,
                                                hipStream_t);
+template hipError_t launch_fb_persistent<float>(const Args<float> &, const PersistQ &, int, int, hipStream_t);
+template hipError_t launch_fb_persistent<double>(const Args<double> &, const PersistQ &, int, int, hipStream_t);
+template hipError_t launch_forward_halo<float>(const Args<float> &, const int32_t *, int, int, int, hipStream_t);
+template hipError_t launch_forward_halo<double>(const Args<double> &, const int32_t *, int, int, int, hipStream_t);
 template hipError_t launch_backward_reg<float>(const Args<float> &, const int32_t *, int, int, int, size_t,
                                               hipStream_t);
 template hipError_t launch_backward_reg<double>(const Args<double> &, const int32_t *, int, int, int, size_t,

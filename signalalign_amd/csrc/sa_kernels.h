@@ -64,6 +64,7 @@ struct Args {
     int64_t *task_xycells;
     int32_t *task_maxw;
     int32_t *task_span;     /* positions a register window must cover (see prep_kernel) */
+    int32_t *task_hspan;    /* positions the halo sweep's window must cover (nullable) */
     int32_t *task_nwin;
     WinDev *win_slots;
     int32_t *err;
@@ -158,6 +159,29 @@ template <typename T> hipError_t launch_backward(const Args<T> &a, int n_wins, h
 template <typename T>
 hipError_t launch_forward_reg(const Args<T> &a, const int32_t *order, int n, int cpt, int nw, int ek, size_t shmem,
                               hipStream_t st);
+/* several wavefronts (2, 3 or 4) per sub-alignment, one position per lane, synchronised
+   every kHaloK diagonals through halo lanes (forward_halo_kernel) */
+constexpr int kHaloH = 8, kHaloK = 8, kHaloS = 64 - kHaloH;
+inline int halo_window(int nw) { return 64 + kHaloS * (nw - 1); }
+template <typename T>
+hipError_t launch_forward_halo(const Args<T> &a, const int32_t *order, int n, int nw, int ek, hipStream_t st);
+/* Work queue of one persistent launch (fb_persistent_kernel): ctr[0] next forward item,
+   ctr[1] publication slots taken, ctr[2] the publication slot whose windows are being
+   claimed; done[k] = the k-th sub-alignment to finish its forward sweep + 1 (0 until
+   published); wclaim per sub-alignment: windows claimed.  ctr, done and wclaim start
+   at zero (3 + n_fwd + n_tasks ints, contiguous). */
+struct PersistQ {
+    int32_t *ctr;
+    int32_t *done;
+    int32_t *wclaim;
+    const int32_t *first_win; /* per sub-alignment: its first window in Args::wins */
+    const int32_t *order;     /* the n_fwd forward items, longest first */
+    int32_t n_fwd;
+    int32_t zero;             /* 0 (read-modify-write reads OR it in) */
+    int64_t wait_ticks;       /* 100 MHz real-time ticks a worker waits without progress before it gives up */
+};
+template <typename T>
+hipError_t launch_fb_persistent(const Args<T> &a, const PersistQ &q, int workers, int ek, hipStream_t st);
 template <typename T>
 hipError_t launch_backward_reg(const Args<T> &a, const int32_t *order, int n, int cpt, int ek, size_t shmem,
                                hipStream_t st);

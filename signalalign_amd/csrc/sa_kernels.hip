@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(256) prep_kernel(Args<T> a) {
        (1, 2 or 3 positions per lane; 4: wider) holding the forward span (bits 0-2)
        and the backward span (bits 3-5) of every diagonal in the block.  The sweeps
        change window width at block boundaries. */
-    int span = 1;
+    int span = 1, hspan = 1;
     const int32_t *XL = a.xlo + t.diag_off, *XH = a.xhi + t.diag_off;
     auto span_class = [](int sp) { return sp <= 64 ? 1 : sp <= 127 ? 2 : sp <= 190 ? 3 : 4; };
     auto wave_class = [](int c) {
@@ -194,6 +194,12 @@ __global__ void __launch_bounds__(256) prep_kernel(Args<T> a) {
             if (d + 2 <= N) hmax = max(hmax, XH[d + 2]);
             const int fsp = XH[d] - lmin + 1, bsp = hmax - XL[d] + 1;
             span = max(span, max(fsp, bsp));
+            if (d % kHaloK == 0) {
+                /* the halo sweep's window at a synchronisation: d and d-1 kept, the
+                   next kHaloK diagonals computed */
+                const int lk = d >= 1 ? min(XL[d], XL[d - 1]) : XL[d];
+                hspan = max(hspan, XH[d + kHaloK < N ? d + kHaloK : N] - lk + 1);
+            }
             cf = span_class(fsp);
             cb = span_class(bsp);
         }
@@ -202,10 +208,15 @@ __global__ void __launch_bounds__(256) prep_kernel(Args<T> a) {
         if ((tid & 63) == 0 && d <= N) a.dcls[t.diag_off + d] = (int8_t)(cf | cb << 3);
     }
     atomicMax(&s_span, span);
+    __shared__ int s_hspan;
+    if (tid == 0) s_hspan = 1;
+    __syncthreads();
+    atomicMax(&s_hspan, hspan);
     __syncthreads();
     if (tid == 0) {
         a.task_xycells[ti] = s_xy;
         a.task_span[ti] = s_span;
+        if (a.task_hspan) a.task_hspan[ti] = s_hspan;
     }
     /* per traceback window: the backward span over its diagonals (T, D] (each
        window's sweep picks the narrowest register window that holds it) and the

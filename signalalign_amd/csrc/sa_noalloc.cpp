@@ -55,7 +55,7 @@ struct HostPlan {
     TaskDev tasks[kMaxTasks];
     int32_t x1[kMaxTasks], y1[kMaxTasks];
     int64_t cells[kMaxTasks], xycells[kMaxTasks];
-    int32_t maxw[kMaxTasks], span[kMaxTasks], nwin[kMaxTasks], first_win[kMaxTasks];
+    int32_t maxw[kMaxTasks], span[kMaxTasks], hspan[kMaxTasks], nwin[kMaxTasks], first_win[kMaxTasks];
     int32_t fgrp[kMaxTasks];
     int32_t ancx[kMaxAnchors], ancy[kMaxAnchors];
     WinDev slots[kMaxWins];
@@ -205,6 +205,7 @@ int align_noalloc(sa_context *ctx, HostPlan &H, const sa_model *model, const sa_
     int8_t *d_dflag = ar.take<int8_t>(total_diag + 4), *d_dcls = ar.take<int8_t>(total_diag + 4);
     int64_t *d_cells = ar.take<int64_t>(nt), *d_xycells = ar.take<int64_t>(nt);
     int32_t *d_maxw = ar.take<int32_t>(nt), *d_span = ar.take<int32_t>(nt), *d_nwin = ar.take<int32_t>(nt);
+    int32_t *d_hspan = ar.take<int32_t>(nt);
     WinDev *d_slots = ar.take<WinDev>(total_wslots);
     int32_t *d_err = ar.take<int32_t>(1);
     if (ar.over) return capacity_error(ctx, "scratch bytes for the inputs and band", (int64_t)kScratchBytes + 1,
@@ -247,6 +248,7 @@ int align_noalloc(sa_context *ctx, HostPlan &H, const sa_model *model, const sa_
     a.task_xycells = d_xycells;
     a.task_maxw = d_maxw;
     a.task_span = d_span;
+    a.task_hspan = d_hspan;
     a.task_nwin = d_nwin;
     a.win_slots = d_slots;
     a.err = d_err;
@@ -262,6 +264,7 @@ int align_noalloc(sa_context *ctx, HostPlan &H, const sa_model *model, const sa_
     SA_HIP(ctx, hipMemcpyAsync(H.xycells, d_xycells, sizeof(int64_t) * nt, hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipMemcpyAsync(H.maxw, d_maxw, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipMemcpyAsync(H.span, d_span, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, st));
+    SA_HIP(ctx, hipMemcpyAsync(H.hspan, d_hspan, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipMemcpyAsync(H.nwin, d_nwin, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipMemcpyAsync(H.slots, d_slots, sizeof(WinDev) * total_wslots, hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipMemcpyAsync(&err, d_err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -303,6 +306,11 @@ int align_noalloc(sa_context *ctx, HostPlan &H, const sa_model *model, const sa_
         for (int i = 0; i < nt; i++) {
             const int mw = force_lds ? -1 : mw_class(H.span[i]);
             H.fgrp[i] = mw >= 0 ? 1 + mw : 0; /* 0: LDS sweep */
+            /* 5 + k: the halo sweep with k + 2 wavefronts, where its window holds the band */
+            const char *halo_env = getenv("SA_HALO"); /* "0": no halo sweeps, as in sa_align */
+            const bool use_halo = !(halo_env && std::strcmp(halo_env, "0") == 0);
+            for (int hw = 2; use_halo && !force_lds && hw <= 4; hw++)
+                if (sa::halo_window(hw) >= H.hspan[i]) { H.fgrp[i] = 5 + hw - 2; break; }
             if (H.fgrp[i] == 0) { n_lds_f++; lds_maxw = std::max(lds_maxw, H.maxw[i]); }
         }
         for (int i = 0; i < nt; i++)
@@ -314,7 +322,7 @@ int align_noalloc(sa_context *ctx, HostPlan &H, const sa_model *model, const sa_
             }
         /* orders grouped by sweep shape (stable within a group) */
         int no = 0;
-        for (int g = 0; g <= 4; g++)
+        for (int g = 0; g <= 7; g++)
             for (int i = 0; i < nt; i++)
                 if (H.fgrp[i] == g) H.task_order[no++] = i;
         int nwo = 0;
@@ -368,6 +376,8 @@ int align_noalloc(sa_context *ctx, HostPlan &H, const sa_model *model, const sa_
                 al.ring_g = f_hbm ? d_ring : nullptr;
                 al.ring_g_stride = (int64_t)f_stride;
                 SA_HIP(ctx, sa::launch_forward<T>(al, e - s, lds_fnt, st));
+            } else if (g >= 5) {
+                SA_HIP(ctx, sa::launch_forward_halo<T>(a, d_task_order + s, e - s, g - 5 + 2, ek, st));
             } else {
                 SA_HIP(ctx, sa::launch_forward_reg<T>(a, d_task_order + s, e - s, kMW[g - 1][1], kMW[g - 1][0], ek, 0, st));
             }

@@ -63,9 +63,10 @@ struct sa_batch {
     DBuf d_ref, d_events, d_ancx, d_ancy, d_tasks, d_x1, d_y1;
     /* derived */
     DBuf d_kid, d_evm, d_evn, d_evl, d_evi, d_xlo, d_xhi, d_doff, d_foff, d_dflag, d_dcls, d_doffxy;
-    DBuf d_task_cells, d_task_xycells, d_task_maxw, d_task_span, d_task_nwin, d_win_slots, d_err;
+    DBuf d_task_cells, d_task_xycells, d_task_maxw, d_task_span, d_task_hspan, d_task_nwin, d_win_slots, d_err;
     DBuf d_fM, d_fX, d_fY;
     DBuf d_wins, d_task_order, d_win_order, d_out, d_win_count, d_win_total, d_win_dst, d_packed;
+    DBuf d_pq, d_first_win; /* persistent sweeps: work queue, first window per sub-alignment */
     DBuf d_ring_g; /* LDS sweeps' rings in HBM when a band is too wide for LDS */
     /* HDP emission plane, its sub-alignments and chunk numbering; the column pass's
        per-position column ranges and offsets, its (task, column) entries grouped by
@@ -81,7 +82,7 @@ struct sa_batch {
     int32_t expect = 0;
     double ex_rec_threshold = 1.0;
     DBuf d_ex_trans, d_ex_s0, d_ex_s1, d_ex_obs, d_ex_rec, d_ex_count;
-    sa_tuning tune{1, 0, 2.5, 0.5};
+    sa_tuning tune{1, 0, 2.5, 0.5, 0, 0};
     int32_t nruns[2] = {0, 0}; /* runs since the batch or its tuning last changed, per precision */
     /* a recorded run per precision (0 FP64, 1 FP32), replayed while the batch is unchanged */
     struct GraphCache {
@@ -104,9 +105,10 @@ struct sa_batch {
         for (auto &g : gc) g.reset();
         DBuf *all[] = {&d_ref, &d_events, &d_ancx, &d_ancy, &d_tasks, &d_x1, &d_y1, &d_kid, &d_evm, &d_evn,
                        &d_evl, &d_evi, &d_xlo, &d_xhi, &d_doff, &d_foff, &d_dflag, &d_dcls, &d_doffxy, &d_task_cells,
-                       &d_task_xycells, &d_task_maxw, &d_task_span,
+                       &d_task_xycells, &d_task_maxw, &d_task_span, &d_task_hspan,
                        &d_task_nwin, &d_win_slots, &d_err, &d_fM, &d_fX, &d_fY, &d_wins, &d_task_order,
                        &d_win_order, &d_out, &d_win_count, &d_win_total, &d_win_dst, &d_packed, &d_ring_g,
+                       &d_pq, &d_first_win,
                        &d_em, &d_em_tasks, &d_em_chunk0, &d_col_a, &d_col_b, &d_col_off, &d_em_ent, &d_em_grp,
                        &d_ex_trans, &d_ex_s0, &d_ex_s1, &d_ex_obs, &d_ex_rec, &d_ex_count,
                        &d_pcol, &d_pent, &d_pplane, &d_pp, &d_psa, &d_psb, &d_pecol, &d_pekid, &d_pdoff,
@@ -589,7 +591,10 @@ int batch_fill(sa_context *ctx, sa_batch *b, const sa_model *model, const sa_rea
             t.var = rd.var;
             b->total_pos += t.lX;
             b->total_ev += t.lY;
-            b->total_diag += N + 1;
+            /* per-diagonal arrays of a sub-alignment start on 128-byte boundaries (32
+               slots): the persistent sweeps read one sub-alignment's offsets through the
+               scalar cache while another's are being written */
+            b->total_diag += (N + 1 + 31) / 32 * 32;
             b->total_wslots += t.win_cap;
             if (N > 0) {
                 b->tasks.push_back(t);
@@ -687,12 +692,16 @@ void sa_tuning_default(sa_tuning *t) {
     t->waves_per_simd = 0;
     t->long_factor = 2.5;
     t->stream_split = 0.5;
+    t->persist_waves = 0;
+    t->reserved = 0;
 }
 
 int sa_batch_set_tuning(sa_batch *b, const sa_tuning *t) {
     if (!b || !t || t->waves_per_simd < 0 || t->waves_per_simd > 8 || !(t->stream_split >= 0.0) ||
-        !(t->stream_split < 1.0))
+        !(t->stream_split < 1.0) || t->persist_waves < -1 || t->persist_waves > 8)
         return SA_ERR_INVALID_ARGUMENT;
+    /* the persistent launch hung on the device with its cause not found: refused */
+    if (t->persist_waves > 0) return SA_ERR_UNSUPPORTED;
     std::lock_guard<std::mutex> lk(b->ctx->mu);
     (void)hipSetDevice(b->ctx->device);
     for (auto &g : b->gc) g.reset();
@@ -732,14 +741,14 @@ int sa_batch_autotune(sa_batch *b, sa_precision precision, sa_tuning *best, doub
     const double sp[] = {0.5, 0.35, 0.65};
     const int wp[] = {0, 2, 3, 4};
     for (int knob = 0; knob < 3; knob++) {
-        const int n = knob == 0 ? 4 : knob == 1 ? 3 : 4;
+        const int n = knob == 1 ? 3 : 4;
         for (int i = 0; i < n; i++) {
             sa_tuning t = cur;
             if (knob == 0) t.long_factor = lf[i];
             else if (knob == 1) t.stream_split = sp[i];
             else t.waves_per_simd = wp[i];
             if (t.long_factor == cur.long_factor && t.stream_split == cur.stream_split &&
-                t.waves_per_simd == cur.waves_per_simd)
+                t.waves_per_simd == cur.waves_per_simd && t.persist_waves == cur.persist_waves)
                 continue;
             double ms = 0.0;
             if ((rc = time_it(t, &ms))) return rc;
@@ -809,6 +818,7 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     SA_HIP(ctx, b->d_task_xycells.ensure(sizeof(int64_t) * nt));
     SA_HIP(ctx, b->d_task_maxw.ensure(sizeof(int32_t) * nt));
     SA_HIP(ctx, b->d_task_span.ensure(sizeof(int32_t) * nt));
+    SA_HIP(ctx, b->d_task_hspan.ensure(sizeof(int32_t) * nt));
     SA_HIP(ctx, b->d_task_nwin.ensure(sizeof(int32_t) * nt));
     SA_HIP(ctx, b->d_win_slots.ensure(sizeof(WinDev) * std::max<int64_t>(b->total_wslots, 1)));
     SA_HIP(ctx, b->d_err.ensure(sizeof(int32_t)));
@@ -836,6 +846,7 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     a.task_xycells = b->d_task_xycells.as<int64_t>();
     a.task_maxw = b->d_task_maxw.as<int32_t>();
     a.task_span = b->d_task_span.as<int32_t>();
+    a.task_hspan = b->d_task_hspan.as<int32_t>();
     a.task_nwin = b->d_task_nwin.as<int32_t>();
     a.win_slots = b->d_win_slots.as<WinDev>();
     a.err = b->d_err.as<int32_t>();
@@ -867,13 +878,14 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     SA_HIP(ctx, sa::launch_prep<T>(a, (int)nt, st));
     SA_HIP(ctx, hipEventRecord(ctx->ev[1], st));
     std::vector<int64_t> cells(nt), xycells(nt);
-    std::vector<int32_t> maxw(nt), nwin(nt), span(nt);
+    std::vector<int32_t> maxw(nt), nwin(nt), span(nt), hspan(nt);
     std::vector<WinDev> slots(b->total_wslots);
     int32_t err = 0;
     SA_HIP(ctx, hipMemcpyAsync(cells.data(), b->d_task_cells.p, sizeof(int64_t) * nt, hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipMemcpyAsync(xycells.data(), b->d_task_xycells.p, sizeof(int64_t) * nt, hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipMemcpyAsync(maxw.data(), b->d_task_maxw.p, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipMemcpyAsync(span.data(), b->d_task_span.p, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, st));
+    SA_HIP(ctx, hipMemcpyAsync(hspan.data(), b->d_task_hspan.p, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipMemcpyAsync(nwin.data(), b->d_task_nwin.p, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipMemcpyAsync(slots.data(), b->d_win_slots.p, sizeof(WinDev) * b->total_wslots,
                                hipMemcpyDeviceToHost, st));
@@ -949,8 +961,17 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
         if (fw && std::strcmp(fw, "4") == 0) long_n = 0;
     }
     /* forward group key: 0 = LDS; 1 + k = one wavefront, kCpt[k]; 7 + k = kMW[k] (the
-       narrowest several-wavefront window that holds the band) */
-    constexpr int kFG = 11, kBG = 7;
+       narrowest several-wavefront window that holds the band); 11 + k = the halo sweep
+       with k + 2 wavefronts (preferred where its window holds the band at every
+       synchronisation) */
+    constexpr int kFG = 15, kBG = 7, kPG = 14; /* kPG: the persistent launch */
+    const char *halo_env = getenv("SA_HALO"); /* "0": the several-wavefront sweeps synchronise every diagonal */
+    const bool use_halo = !(halo_env && std::strcmp(halo_env, "0") == 0);
+    auto halo_shape = [&](int hsp) {
+        for (int w = 2; w <= 4; w++)
+            if (sa::halo_window(w) >= hsp) return w;
+        return 0;
+    };
     std::vector<int> fgrp(nt), bgrp(nt);
     for (int64_t i = 0; i < nt; i++) {
         int f = 0, bb = 0;
@@ -961,6 +982,8 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
                 f = bb;
             }
             if (mw >= 0 && nd[i] >= long_n) f = 7 + mw;
+            const int hw = use_halo ? halo_shape(hspan[i]) : 0;
+            if (hw && nd[i] >= long_n) f = 11 + hw - 2;
         }
         fgrp[i] = f;
         bgrp[i] = lds_backward ? 0 : bb;
@@ -974,6 +997,23 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
             const int c = bgrp[i] == 0 ? 0 : cpt1(wins[wi].span);
             wgrp[wi] = c == 0 ? bgrp[i] : std::min(bgrp[i], 1 + (int)(std::find(kCpt, kCpt + 6, c) - kCpt));
         }
+    /* Persistent sweeps: the one-wavefront sub-alignments whose register windows hold at
+       most three positions per lane in both directions run as one persistent launch
+       (their forward sweeps, then the windows of those done) on stream 3, enqueued
+       after the other streams' forward launches; their windows leave the backward
+       launches. */
+    /* Off: sa_batch_set_tuning refuses persist_waves 1-8 (the persistent launch hung on
+       the GPU with its cause not found). */
+    const bool persist = !expect && b->tune.persist_waves > 0;
+    int64_t n_pers = 0, n_pers_win = 0;
+    if (persist)
+        for (int64_t i = 0; i < nt; i++)
+            if (fgrp[i] >= 1 && fgrp[i] <= 3 && bgrp[i] >= 1 && bgrp[i] <= 3) {
+                fgrp[i] = kPG;
+                n_pers++;
+                n_pers_win += nwin[i];
+                for (int32_t w = 0; w < nwin[i]; w++) wgrp[task_first_win[i] + w] = kBG;
+            }
     /* Streams: sub-alignments whose forward sweep uses several wavefronts (the long
        ones) run on stream 1, LDS sweeps on stream 2, one-wavefront forward sweeps
        on streams 0 and 3 (the sub-alignments holding the first half of their cells,
@@ -986,7 +1026,7 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
         std::vector<int32_t> nw1;
         int64_t nw1_cells = 0;
         for (int64_t i = 0; i < nt; i++) {
-            tlane[i] = fgrp[i] == 0 ? 2 : fgrp[i] >= 7 ? 1 : 0;
+            tlane[i] = fgrp[i] == 0 ? 2 : fgrp[i] == kPG ? 3 : fgrp[i] >= 7 ? 1 : 0;
             if (tlane[i] == 0) { nw1.push_back((int32_t)i); nw1_cells += cells[i]; }
         }
         std::stable_sort(nw1.begin(), nw1.end(), [&](int32_t x, int32_t y) { return cells[x] > cells[y]; });
@@ -1006,13 +1046,13 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
         int64_t mw_max = 0;
         for (int64_t i = 0; i < nt; i++) {
             any_lds_task |= fgrp[i] == 0;
-            if (fgrp[i] >= 7) mw_max = std::max(mw_max, nd[i]);
+            if (fgrp[i] >= 7 && fgrp[i] != kPG) mw_max = std::max(mw_max, nd[i]);
         }
         if (streams4 && !any_lds_task && mw_max > 0) {
             int n_head = 0, n_mw = 0;
-            for (int64_t i = 0; i < nt; i++) n_mw += fgrp[i] >= 7;
+            for (int64_t i = 0; i < nt; i++) n_mw += fgrp[i] >= 7 && fgrp[i] != kPG;
             for (int64_t i = 0; i < nt; i++)
-                if (fgrp[i] >= 7 && (double)nd[i] >= 0.85 * (double)mw_max && n_head < n_mw - 1) {
+                if (fgrp[i] >= 7 && fgrp[i] != kPG && (double)nd[i] >= 0.85 * (double)mw_max && n_head < n_mw - 1) {
                     tlane[i] = 2;
                     n_head++;
                 }
@@ -1196,7 +1236,30 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
        waves_per_simd of them share a SIMD (160 KiB of LDS per CU, four SIMDs) */
     const int wps = b->tune.waves_per_simd;
     const size_t shmem1 = wps > 0 && wps < 8 ? (size_t)std::max(0, 163840 / (4 * wps) - 4096) : 0;
+    if (n_pers) {
+        SA_HIP(ctx, b->d_pq.ensure(sizeof(int32_t) * (size_t)(3 + n_pers + nt)));
+        if ((rc = upload(ctx, b->d_first_win, task_first_win))) return rc;
+    }
     auto launch_fwd = [&](const Launch &L, hipStream_t sg) -> hipError_t {
+        if (L.group == kPG) {
+            /* queue counters zeroed by a kernel (graph-safe), then the workers: enough
+               for every item, at most persist_waves per SIMD */
+            int32_t *pq = b->d_pq.as<int32_t>();
+            hipError_t e = sa::launch_zero(pq, (int)(3 + L.n + nt), sg);
+            if (e != hipSuccess) return e;
+            sa::PersistQ q;
+            q.ctr = pq;
+            q.done = pq + 3;
+            q.wclaim = pq + 3 + L.n;
+            q.first_win = b->d_first_win.as<int32_t>();
+            q.order = d_to + L.off;
+            q.n_fwd = (int32_t)L.n;
+            q.zero = 0;
+            q.wait_ticks = 200000000; /* 2 s; a step takes milliseconds */
+            const int pw = b->tune.persist_waves;
+            const int64_t workers = std::min<int64_t>(L.n + n_pers_win, (int64_t)4 * std::max(ctx->n_cu, 1) * pw);
+            return sa::launch_fb_persistent<T>(a, q, (int)workers, ek, sg);
+        }
         if (L.group == 0) {
             sa::Args<T> al = a;
             al.task_order = d_to + L.off;
@@ -1211,6 +1274,7 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
         ap.prio = L.group >= 7 ? (L.lane == 2 ? 3 : 2) : L.lane == 0 ? 1 : 0;
         if (L.group < 7)
             return sa::launch_forward_reg<T>(ap, d_to + L.off, (int)L.n, kCpt[L.group - 1], 1, ek, shmem1, sg);
+        if (L.group >= 11) return sa::launch_forward_halo<T>(ap, d_to + L.off, (int)L.n, L.group - 11 + 2, ek, sg);
         return sa::launch_forward_reg<T>(ap, d_to + L.off, (int)L.n, kMW[L.group - 7][1], kMW[L.group - 7][0], ek,
                                          0, sg);
     };
@@ -1272,6 +1336,7 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     SA_HIP(ctx, hipMemcpyAsync(&err, b->d_err.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     SA_HIP(ctx, hipStreamSynchronize(st));
     if (err & sa::ERR_WINDOWS) return kRetryLds;
+    if (err & sa::ERR_QUEUE) return fail(ctx, SA_ERR_HIP, "persistent sweeps: a worker waited too long for a forward sweep");
     {
         float fb = 0.f;
         (void)hipEventElapsedTime(&fb, ctx->ev[2], ctx->ev[4]);
@@ -1282,10 +1347,15 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
             (void)hipEventElapsedTime(&t0, ctx->ev[2], ctx->gev[e]);
             (void)hipEventElapsedTime(&t1, ctx->ev[2], ctx->gev[e + 1]);
             sa_launch li;
+            std::memset(&li, 0, sizeof(li));
             li.direction = dir;
-            li.cpt = L.group == 0 ? 0 : dir == 0 && L.group >= 7 ? kMW[L.group - 7][1] : kCpt[L.group - 1];
-            li.waves = dir == 0 && L.group >= 7 ? kMW[L.group - 7][0] : 1;
+            const bool pl = dir == 0 && L.group == kPG;
+            li.cpt = L.group == 0 ? 0 : pl ? 3 : dir == 0 && L.group >= 11 ? 1 : dir == 0 && L.group >= 7 ? kMW[L.group - 7][1]
+                                                                                                        : kCpt[L.group - 1];
+            li.waves = pl ? 1 : dir == 0 && L.group >= 11 ? L.group - 11 + 2 : dir == 0 && L.group >= 7 ? kMW[L.group - 7][0] : 1;
+            li.persistent = pl ? 1 : 0;
             li.stream = L.lane;
+            li.sync_every = pl ? 0 : dir == 0 && L.group >= 11 ? sa::kHaloK : dir == 0 && L.group >= 7 ? 1 : 0;
             li.items = L.n;
             li.cells = L.cells;
             li.start_ms = t0;

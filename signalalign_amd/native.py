@@ -100,13 +100,14 @@ class _ServerStats(ctypes.Structure):
 
 class _Tuning(ctypes.Structure):
     _fields_ = [("use_graph", ctypes.c_int32), ("waves_per_simd", ctypes.c_int32), ("long_factor", ctypes.c_double),
-                ("stream_split", ctypes.c_double)]
+                ("stream_split", ctypes.c_double), ("persist_waves", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class _Launch(ctypes.Structure):
     _fields_ = [("direction", ctypes.c_int32), ("cpt", ctypes.c_int32), ("waves", ctypes.c_int32),
                 ("stream", ctypes.c_int32), ("items", ctypes.c_int64), ("cells", ctypes.c_int64),
-                ("start_ms", ctypes.c_double), ("ms", ctypes.c_double)]
+                ("start_ms", ctypes.c_double), ("ms", ctypes.c_double), ("sync_every", ctypes.c_int32),
+                ("persistent", ctypes.c_int32)]
 
 
 

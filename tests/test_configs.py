@@ -176,3 +176,49 @@ def test_batched_acegt_6mer_uneven_lengths():
         _check_batch(ctx, m, reads, native.Params(diagonal_expansion=50), "ACEGT batch")
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("expansion", [20, 50, 100])
+def test_halo_sweep_bit_equal_to_barrier_sweep(monkeypatch, expansion):
+    """The several-wavefront forward sweep that synchronises every 8th diagonal through
+    halo lanes gives the same pairs and log-likelihood, bit for bit, as the one that
+    synchronises every diagonal (SA_HALO=0): bands of about 45, 93 and 190 cells (two,
+    two or three and four wavefronts), FP32 and FP64, the model file's Gaussian tables,
+    general Gaussian tables, HDP gathers and the HDP emission plane."""
+    from signalalign_amd import models
+    monkeypatch.setenv("SA_FWD_WAVES", "4")
+    ctx = native.Context(0)
+    try:
+        g = synth.gaussian_model("ACGT", 6, seed=1)
+        h = synth.hdp_model("ACGT", 4, seed=3)
+        reads = [synth.single_read(g, n, seed=900 + n) for n in (6000, 2500)]
+        hreads = [synth.single_read(h, 4000, seed=901)]
+        params = native.Params(diagonal_expansion=expansion)
+        gen = models.GaussianModel(g.alphabet, 6, g.transitions, g.match, g.gapy.copy(), g.gapx.copy())
+        gen.gapy[:, 4] *= 1.3
+        gen.gapx[::7] = np.log(0.05)
+        cases = [("model file", g, reads), ("general tables", gen, reads[:1]), ("hdp plane", h, hreads),
+                 ("hdp gathers", h, hreads)]
+        for label, m, rs in cases:
+            dm = ctx.model(m)
+            nr = [P.native_read(r) for r in rs]
+            for prec in (native.SA_FP32, native.SA_FP64):
+                if label == "hdp gathers":
+                    monkeypatch.setenv("SA_HDP_PLANE", "0")
+                monkeypatch.setenv("SA_HALO", "1")
+                got, tm = ctx.align_batch(dm, nr, params, prec)
+                fwd = [(L["waves"], L["sync_every"]) for L in tm["launches"] if L["direction"] == 0]
+                monkeypatch.setenv("SA_HALO", "0")
+                want, tm0 = ctx.align_batch(dm, nr, params, prec)
+                fwd0 = [(L["waves"], L["sync_every"]) for L in tm0["launches"] if L["direction"] == 0]
+                monkeypatch.delenv("SA_HDP_PLANE", raising=False)
+                print(f"e={expansion} {label} precision {prec}: forward (waves, sync) halo {fwd}, barrier {fwd0}",
+                      flush=True)
+                assert fwd and all(w >= 2 and se == 8 for w, se in fwd), fwd
+                assert fwd0 and all(w >= 2 and se == 1 for w, se in fwd0), fwd0
+                for a, b in zip(got, want):
+                    assert np.array_equal(a.pairs, b.pairs), label
+                    assert a.log_likelihood == b.log_likelihood, label
+            dm.close()
+    finally:
+        ctx.close()

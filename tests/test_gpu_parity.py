@@ -15,16 +15,18 @@ from tests import _parity as P
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(params=["reg1", "reg4", "lds"], autouse=True)
+@pytest.fixture(params=["reg1", "reg4", "reg4b", "lds"], autouse=True)
 def kernel_path(request, monkeypatch):
     """Every case runs through each sweep: the register-resident kernels with one
-    wavefront per forward sweep, with four, and the LDS-ring kernels (the fallback
+    wavefront per forward sweep, with several (synchronised every 8th diagonal through
+    halo lanes, and -- reg4b -- every diagonal), and the LDS-ring kernels (the fallback
     for bands wider than a register window)."""
     if request.param == "lds":
         monkeypatch.setenv("SA_KERNELS", "lds")
     else:
         monkeypatch.setenv("SA_KERNELS", "reg")
-        monkeypatch.setenv("SA_FWD_WAVES", request.param[-1])
+        monkeypatch.setenv("SA_FWD_WAVES", request.param[3])
+        monkeypatch.setenv("SA_HALO", "0" if request.param.endswith("b") else "1")
     return request.param
 
 
