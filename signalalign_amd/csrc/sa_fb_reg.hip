@@ -1665,18 +1665,30 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
             T sv[CPT];
             bool cand[CPT];
             bool anyc = false;
+            if (have_fpre && lo > 0 && hi < e) {
+                /* no cell of e lies in row 0 or column 0, and the prefetched forward
+                   states are NaN outside the band: the exponent is NaN there and fails
+                   the test, so no per-cell bounds check is needed */
 #pragma unroll
-            for (int c = 0; c < CPT; c++) {
-                const int x = X0 + CPT * lane + c;
-                const int y = e - x;
-                cand[c] = false;
-                sv[c] = NI;
-                if (x >= lo && x <= hi && x > 0 && y > 0) {
-                    const T fm = have_fpre ? fpre[c] : fM[dof + (x - lo)];
-                    sv[c] = NORM ? (fm + nM[c]) + cst : (fm + nM[c]) - (T)total;
+                for (int c = 0; c < CPT; c++) {
+                    sv[c] = NORM ? (fpre[c] + nM[c]) + cst : (fpre[c] + nM[c]) - (T)total;
                     cand[c] = sv[c] >= lthr;
+                    anyc |= cand[c];
                 }
-                anyc |= cand[c];
+            } else {
+#pragma unroll
+                for (int c = 0; c < CPT; c++) {
+                    const int x = X0 + CPT * lane + c;
+                    const int y = e - x;
+                    cand[c] = false;
+                    sv[c] = NI;
+                    if (x >= lo && x <= hi && x > 0 && y > 0) {
+                        const T fm = have_fpre ? fpre[c] : fM[dof + (x - lo)];
+                        sv[c] = NORM ? (fm + nM[c]) + cst : (fm + nM[c]) - (T)total;
+                        cand[c] = sv[c] >= lthr;
+                    }
+                    anyc |= cand[c];
+                }
             }
             if (__ballot(anyc)) {
                 bool keep[CPT];
@@ -1719,7 +1731,7 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
             for (int c = 0; c < CPT; c++) {
                 const int x = X0 + CPT * lane + c;
                 const bool v = x >= nlo && x <= nhi;
-                fpre[c] = v ? fM[ndo + (x - nlo)] : NI;
+                fpre[c] = v ? fM[ndo + (x - nlo)] : __builtin_nanf(""); /* NaN: see the posterior test */
             }
             have_fpre = true;
             if (posterior_count % 10 == 0) { /* e-1 refreshes the total */
