@@ -1021,6 +1021,7 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
     double md_foff = st.md_foff;
     int ncls = st.ncls;
     int next = 0;
+    const int d0 = d; /* the segment's first diagonal */
 
     for (; d <= N; d++) {
         int j = d - dc;
@@ -1042,33 +1043,16 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
            single exit per segment: early returns cost the loop its register assignment. */
         bad |= lo < X0;
         if (hi > X0 + W - 1) {
-            /* slide the window as far as the two previous diagonals allow */
-            const int minlo = lo1 < lo2 ? lo1 : lo2;
-            int smax = (minlo - X0) / CPT;
-            const int sreq = (hi - (X0 + W - 1) + CPT - 1) / CPT;
-            bad |= smax < sreq;
-            smax = smax < 0 ? 0 : smax;
-#pragma unroll
-            for (int c = 0; c < CPT; c++) {
-                pM[c] = shfl_idx(pM[c], lane + smax, NI); pX[c] = shfl_idx(pX[c], lane + smax, NI);
-                pY[c] = shfl_idx(pY[c], lane + smax, NI);
-                qM[c] = shfl_idx(qM[c], lane + smax, NI); qX[c] = shfl_idx(qX[c], lane + smax, NI);
-                qY[c] = shfl_idx(qY[c], lane + smax, NI);
+            /* the band leaves the window: the segment ends here and starts again at d
+               with the window starting at the leftmost cell of d, d-1 and d-2 (the way
+               a width change re-seats it).  Sliding inside the loop would give the
+               k-mer and event registers a second definition, which costs the loop
+               register copies and memory waits on every diagonal. */
+            if (d != d0) {
+                next = CPT;
+                break;
             }
-            LpM = shr1(qM[CPT - 1], NI); LpX = shr1(qX[CPT - 1], NI); LpY = shr1(qY[CPT - 1], NI);
-            X0 += smax * CPT;
-#pragma unroll
-            for (int c = 0; c < CPT; c++) {
-                const int x = X0 + CPT * lane + c;
-                load_kcon(kc[c], m, kid, x - 1, x >= 1 && x <= lX);
-                const int jj = d - x - 1;
-                load_econ(ev[c], a, evc + jj, jj >= 0 && jj < lY);
-            }
-#pragma unroll
-            for (int c = 0; c < CPT; c++) { sink(kc[c]); sink(ev[c]); }
-            fresh = true;
-            have_gp = false;
-            have_pe = false;
+            bad = true; /* the window holds fewer positions than the band: the LDS sweeps redo it */
         }
         if (!fresh) {
             /* cell x on d reads event d-x-1: its left neighbour's event of d-1 */
@@ -1482,34 +1466,16 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
             fresh = false;
         } else {
             if (lo < X0) {
-                /* slide the window left as far as the two later diagonals allow */
-                const int maxhi = hi1 > hi2 ? hi1 : hi2;
-                int smax = (X0 + W - 1 - maxhi) / CPT;
-                const int sreq = (X0 - lo + CPT - 1) / CPT;
-                bad |= smax < sreq; /* single loop exit, as in the forward sweep */
-                smax = smax < 0 ? 0 : smax;
-#pragma unroll
-                for (int c = 0; c < CPT; c++) {
-                    pM[c] = shfl_idx(pM[c], lane - smax, NI); pX[c] = shfl_idx(pX[c], lane - smax, NI);
-                    pY[c] = shfl_idx(pY[c], lane - smax, NI); qM[c] = shfl_idx(qM[c], lane - smax, NI);
+                /* the band leaves the window: the segment ends here and starts again at
+                   e with the window ending at the rightmost cell of e, e+1 and e+2 (the
+                   way a width change re-seats it).  Sliding inside the loop would give
+                   the k-mer and event registers a second definition, which costs the
+                   loop register copies and memory waits on every diagonal. */
+                if (e != es) {
+                    next = CPT;
+                    break;
                 }
-                X0 -= smax * CPT;
-                RpM = shl1(qM[0], NI);
-#pragma unroll
-                for (int c = 0; c < CPT; c++) {
-                    const int x = X0 + CPT * lane + c;
-                    load_kcon(kc[c], m, kid, x - 1, x >= 1 && x <= lX);
-                    const int jj = e - x;
-                    load_econ(ev[c], a, evc + jj, jj >= 0 && jj < lY);
-                }
-#pragma unroll
-                for (int c = 0; c < CPT; c++) { sink(kc[c]); sink(ev[c]); }
-                fresh = true;
-                have_gp = false;
-                have_pe = false;
-                have_eY = false;
-                have_fpre = false;
-                have_rpre = false;
+                bad = true; /* the window holds fewer positions than the band: the LDS sweeps redo it */
             }
             bad |= hi > X0 + W - 1;
             if (!fresh) {
