@@ -229,11 +229,47 @@ __device__ __forceinline__ double wave_max(double v) {
  * rearranged with per-k-mer and per-event constants:
  *   a = (evm - s*mu) / sd,  evm = (mean - shift)/var, s = scale/var (1, 0, 1 undescaled)
  *   e = ca - a^2/2 + evl - nb (noise - nmu)^2 / noise,  evl = -1.5 log(noise). */
+template <typename T> __device__ __forceinline__ T fma_t(T x, T y, T z);
+template <> __device__ __forceinline__ float fma_t(float x, float y, float z) { return __builtin_fmaf(x, y, z); }
+template <> __device__ __forceinline__ double fma_t(double x, double y, double z) { return __builtin_fma(x, y, z); }
+
+/* Every sweep evaluates the emissions through these, with each fused multiply-add
+ * written out and no other contraction, so that one k-mer and event give the same
+ * bits in every kernel and in every context the compiler inlines them into (the
+ * register sweeps, the LDS sweeps and the multi-path sweeps can then meet in one
+ * read, and a pair's posterior does not depend on which of them ran). */
+template <typename T> struct GaussTerms {
+    T a2;    /* a^2 */
+    T noise; /* evl - nb (noise - nmu)^2 / noise */
+};
+template <typename T>
+__device__ __forceinline__ GaussTerms<T> gauss_terms(T mu, T isd, T nmu, T nb, T evm, T evn, T evl, T evi, T s) {
+#pragma clang fp contract(off)
+    const T a = fma_t(-s, mu, evm) * isd;
+    const T u = evn - nmu;
+    return GaussTerms<T>{a * a, fma_t(-nb, (u * u) * evi, evl)};
+}
 template <typename T>
 __device__ __forceinline__ T gauss_emit(T mu, T isd, T ca, T nmu, T nb, T evm, T evn, T evl, T evi, T s) {
-    T a = (evm - s * mu) * isd;
-    T u = evn - nmu;
-    return (ca - (T)0.5 * a * a) + evl - nb * (u * u * evi);
+#pragma clang fp contract(off)
+    const GaussTerms<T> g = gauss_terms(mu, isd, nmu, nb, evm, evn, evl, evi, s);
+    return fma_t((T)-0.5, g.a2, ca) + g.noise;
+}
+/* The extra-event emission of a model file's tables from the match table's constants:
+ * the extra-event table is the match table with sd x 1.75 (impl/stateMachine.c:1410-1412),
+ * so a becomes a/1.75 and ca loses log(1.75). */
+template <typename T> __device__ __forceinline__ T gauss_ext_std(const GaussTerms<T> &g, T ca) {
+#pragma clang fp contract(off)
+    return fma_t((T)-0.16326530612244898, g.a2, ca - (T)0.55961578793542266) + g.noise;
+}
+/* both emissions of a model file's tables at once */
+template <typename T>
+__device__ __forceinline__ void gauss_both_std(T mu, T isd, T ca, T nmu, T nb, T evm, T evn, T evl, T evi, T s,
+                                               T &em, T &ey) {
+#pragma clang fp contract(off)
+    const GaussTerms<T> g = gauss_terms(mu, isd, nmu, nb, evm, evn, evl, evi, s);
+    em = fma_t((T)-0.5, g.a2, ca) + g.noise;
+    ey = gauss_ext_std(g, ca);
 }
 
 /* emissions_signal_getHdpKmerDensity (impl/stateMachine.c:468-495) ->
@@ -332,6 +368,10 @@ struct Emit {
             return;
         }
         T evn = a.evn[j], evl = a.evl[j], evi = a.evi[j];
+        if (m.std_y) {
+            gauss_both_std(m.mmu[kid], m.misd[kid], m.mca[kid], m.mnmu[kid], m.mnb[kid], evm, evn, evl, evi, s, em, ey);
+            return;
+        }
         em = gauss_emit(m.mmu[kid], m.misd[kid], m.mca[kid], m.mnmu[kid], m.mnb[kid], evm, evn, evl, evi, s);
         ey = gauss_emit(m.ymu[kid], m.yisd[kid], m.yca[kid], m.ynmu[kid], m.ynb[kid], evm, evn, evl, evi, s);
     }
@@ -346,6 +386,9 @@ struct Emit {
                                              T inv_var) {
         T evm = a.evm[j];
         if (m.kind == 2) return hdp_emit(m, kid, evm, s, inv_var);
+        if (m.std_y)
+            return gauss_ext_std(gauss_terms(m.mmu[kid], m.misd[kid], m.mnmu[kid], m.mnb[kid], evm, a.evn[j],
+                                             a.evl[j], a.evi[j], s), m.mca[kid]);
         return gauss_emit(m.ymu[kid], m.yisd[kid], m.yca[kid], m.ynmu[kid], m.ynb[kid], evm, a.evn[j], a.evl[j],
                           a.evi[j], s);
     }

@@ -150,27 +150,14 @@ template <typename T> __device__ __forceinline__ KCon<T, 2> shl1_match(const KCo
 }
 template <typename T> __device__ __forceinline__ KCon<T, 3> shl1_match(const KCon<T, 3> &) { return KCon<T, 3>{}; }
 
-/* emissions from registers */
-template <typename T>
-__device__ __forceinline__ T em_match(const Model<T> &, const KCon<T, 0> &k, const ECon<T, 0> &e, T s, T) {
-    return gauss_emit(k.mmu, k.misd, k.mca, k.mnmu, k.mnb, e.m, e.n, e.l, e.i, s);
-}
-/* extra-event table = match table with sd x 1.75: a/1.75 and -log(1.75) */
-template <typename T>
-__device__ __forceinline__ T em_gapy(const Model<T> &, const KCon<T, 0> &k, const ECon<T, 0> &e, T s, T) {
-    const T a = (e.m - s * k.mmu) * k.misd;
-    const T u = e.n - k.mnmu;
-    return ((k.mca - (T)0.55961578793542266) - (T)0.16326530612244898 * (a * a)) + e.l - k.mnb * (u * u * e.i);
-}
+/* emissions from registers.  Model-file tables (EK 0): both emissions of one k-mer and
+   event at once, sharing the standardised mean and the noise term.  The backward sweep
+   takes a cell's match emission from its right neighbour's evaluation on the diagonal
+   before. */
 template <typename T>
 __device__ __forceinline__ void em_both(const Model<T> &, const KCon<T, 0> &k, const ECon<T, 0> &e, T s, T,
                                         T &eM, T &eY) {
-    const T a = (e.m - s * k.mmu) * k.misd;
-    const T u = e.n - k.mnmu;
-    const T noise = e.l - k.mnb * (u * u * e.i);
-    const T a2 = a * a;
-    eM = (k.mca - (T)0.5 * a2) + noise;
-    eY = ((k.mca - (T)0.55961578793542266) - (T)0.16326530612244898 * a2) + noise;
+    gauss_both_std(k.mmu, k.misd, k.mca, k.mnmu, k.mnb, e.m, e.n, e.l, e.i, s, eM, eY);
 }
 template <typename T>
 __device__ __forceinline__ T em_gapx(const Model<T> &m, const KCon<T, 0> &k) {
@@ -1385,20 +1372,23 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
     bool fresh = true;
     bool bad = st.bad;
     int next = 0;
-    /* HDP: the match and extra-event emissions are the same density, and the match
-       emission of cell x on e (k-mer x+1, event e-x) is the extra-event emission its
-       right neighbour computed on e+1; so one density per cell, its gathers issued a
-       diagonal ahead */
+    /* The match emission of cell x on e (k-mer x+1, event e-x) is the emission of the
+       k-mer and event its right neighbour evaluated on e+1: each cell evaluates its own
+       k-mer against its event once (both emissions, em_both) and keeps the match one
+       for its left neighbour's next diagonal.  HDP: the match and extra-event emissions
+       are the same density, its gathers issued a diagonal ahead */
     constexpr bool PF = EK == 2;
     HdpFetch<T> gp[PF ? CPT : 1];
     /* emission plane (EK 3): the same reuse, the extra-event emissions of e-1's cells
        (E(e, x)) loaded a diagonal ahead */
     constexpr bool PL = EK == 3;
-    constexpr bool PE = PF || PL;
     const T *em = PL ? a.em + t.cell_off : nullptr;
     T pe[PL ? CPT : 1];
     bool have_pe = false;
-    T eYp[PE ? CPT : 1];
+    /* FP64 Gaussian sweeps evaluate every match emission: the registers the reuse takes
+       cost them more spills than the evaluation */
+    constexpr bool RE = EK >= 2 || sizeof(T) == 4;
+    T eYp[CPT]; /* the match emission of each cell's own k-mer and event, from e+1 */
     bool have_gp = false, have_eY = false;
     /* E(dd, xx) straight from the plane (window tops and slides) */
     auto plane_at = [&](int dd, int xx) -> T {
@@ -1455,7 +1445,7 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
         T nM[CPT], nX[CPT], nY[CPT];
         T LowM = NI, LowX = NI; /* lane i+1's first cell of e+1 */
         double obe = 0.0;
-        T eYc[PE ? CPT : 1];
+        T eYc[CPT];
         if (e == D) {
 #pragma unroll
             for (int c = 0; c < CPT; c++) {
@@ -1489,7 +1479,7 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
             LowM = shl1(pM[0], NI);
             LowX = shl1(pX[0], NI);
             const KCon<T, EK> kn = shl1_match(kc[0]); /* k-mer x of lane's last cell */
-            const T eYn = PE && have_eY ? shl1(eYp[0], NI) : NI;
+            const T eYn = have_eY ? shl1(eYp[0], NI) : NI;
             const bool renorm = NORM && ((D - e) & 7) == 0;
             T mx = NI;
 #pragma unroll
@@ -1511,8 +1501,15 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                     eM = have_eY ? (c < CPT - 1 ? eYp[c < CPT - 1 ? c + 1 : 0] : eYn) : em_match<T>(m, kx, ev[c], s, iv);
                     eYc[c] = eY;
                 } else {
-                    eM = em_match<T>(m, kx, ev[c], s, iv);
-                    eY = em_gapy<T>(m, kc[c], ev[c], s, iv);
+                    T mself; /* (FP64: the refresh below evaluates it again) */
+                    em_both<T>(m, kc[c], ev[c], s, iv, mself, eY);
+                    if (RE && have_eY) {
+                        eM = c < CPT - 1 ? eYp[c < CPT - 1 ? c + 1 : 0] : eYn;
+                    } else {
+                        T unused;
+                        em_both<T>(m, kx, ev[c], s, iv, eM, unused);
+                    }
+                    eYc[c] = mself;
                 }
                 /* contributions arrive middle (e+2), then upper, then lower (e+1) */
                 T M = CmidM + (eM + tMC);
@@ -1532,11 +1529,9 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                     mx = m3 > mx ? m3 : mx;
                 }
             }
-            if constexpr (PE) {
 #pragma unroll
-                for (int c = 0; c < CPT; c++) eYp[c] = eYc[c];
-                have_eY = true;
-            }
+            for (int c = 0; c < CPT; c++) eYp[c] = eYc[c];
+            have_eY = true;
             obe = ob1;
             if (renorm) {
                 /* every 8th diagonal: the largest state becomes 0, its log carried in the
@@ -1607,9 +1602,14 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                     for (int c = 0; c < CPT; c++) {
                         const int x = X0 + CPT * lane + c; /* cell (x, e+1-x); middle (x-1, e-x) on e-1 */
                         if (x >= lo1e && x <= hi1e) {
+                            /* the match emission of k-mer x, event e-x */
                             T eM;
-                            if constexpr (PL) eM = eYc[PL ? c : 0]; /* E(e+1, x) */
-                            else eM = em_match<T>(m, kc[c], ev[c], s, iv);
+                            if constexpr (RE) {
+                                eM = eYc[c];
+                            } else {
+                                T unused;
+                                em_both<T>(m, kc[c], ev[c], s, iv, eM, unused);
+                            }
                             T Mv = rgm[c] + (eM + tMC);
                             Mv = logadd_lut(Mv, rgx[c] + (eM + tMFX), lut);
                             Mv = logadd_lut(Mv, rgy[c] + (eM + tMFY), lut);
@@ -1739,8 +1739,11 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
  * the band's width allows (block classes from prep_kernel), never wider than CMAX. */
 /* Waves per SIMD the backward sweep is compiled for: the FP64 sweeps at three cells
    per lane would otherwise hold one wave per SIMD; at two they spill a few registers
-   to scratch and still run 10% faster.  Otherwise the compiler's choice. */
-template <typename T, int CMAX> struct BwdWaves { static constexpr int n = sizeof(T) == 8 && CMAX == 3 ? 2 : 1; };
+   to scratch and still run 10% faster.  The FP32 ones at three cells are held to 168
+   registers, three waves per SIMD.  Otherwise the compiler's choice. */
+template <typename T, int CMAX> struct BwdWaves {
+    static constexpr int n = CMAX != 3 ? 1 : sizeof(T) == 8 ? 2 : 3;
+};
 
 /* One traceback window (the kernel below and the persistent sweeps' window item). */
 template <typename T, int CMAX, int EK>

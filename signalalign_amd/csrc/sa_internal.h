@@ -222,6 +222,7 @@ sa::Model<T> device_model(const sa_model *m, const DevTables<T> &t) {
     r.ys = (const T *)t.ys.p;
     r.grid_length = m->grid_length;
     r.use_swx = m->trans[7] > -HUGE_VAL ? 1 : 0;
+    r.std_y = m->kind != SA_EMIT_HDP && m->ek == 0 ? 1 : 0;
     r.inv_dx = (T)m->inv_dx;
     r.grid_x0 = (T)m->grid0;
     r.grid_xn = (T)m->gridn;

@@ -29,6 +29,8 @@ struct Model {
     const T *ys;
     int32_t grid_length;
     int32_t use_swx;        /* GAP_SWITCH_TO_X above -inf: the gapY -> gapX term is summed */
+    int32_t std_y;          /* Gaussian tables as a model file gives them: the extra-event
+                               emission comes from the match table's constants */
     T inv_dx;
     T grid_x0, grid_xn, grid_dx; /* first and last knot, knot spacing (knot 1 - knot 0) */
     T gx_const;             /* the skip emission when every k-mer has the same one */

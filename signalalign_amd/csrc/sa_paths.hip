@@ -147,8 +147,13 @@ __device__ __forceinline__ void fwd_step(const Args<double> &a, const LutEntry<d
             if (gauss) {
                 eX = Rc.gx;
                 if (y >= 1) {
-                    eM = gauss_emit(Rc.mmu, Rc.misd, Rc.mca, Rc.mnmu, Rc.mnb, Rc.evm, Rc.evn, Rc.evl, Rc.evi, s);
-                    eY = gauss_emit(Rc.ymu, Rc.yisd, Rc.yca, Rc.ynmu, Rc.ynb, Rc.evm, Rc.evn, Rc.evl, Rc.evi, s);
+                    if (m.std_y) {
+                        gauss_both_std(Rc.mmu, Rc.misd, Rc.mca, Rc.mnmu, Rc.mnb, Rc.evm, Rc.evn, Rc.evl, Rc.evi, s, eM,
+                                       eY);
+                    } else {
+                        eM = gauss_emit(Rc.mmu, Rc.misd, Rc.mca, Rc.mnmu, Rc.mnb, Rc.evm, Rc.evn, Rc.evl, Rc.evi, s);
+                        eY = gauss_emit(Rc.ymu, Rc.yisd, Rc.yca, Rc.ynmu, Rc.ynb, Rc.evm, Rc.evn, Rc.evl, Rc.evi, s);
+                    }
                 }
             } else {
                 eX = Emit<double>::gapx(m, Ec.kk);

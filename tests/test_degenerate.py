@@ -194,7 +194,12 @@ def test_gpu_degenerate_batch_mixed_and_refusals(gpu_ctx, oracle_lib):
     print("degenerate batch", flush=True)
     res, _ = gpu_ctx.align_batch(dm, reads, params, native.SA_FP64)
     plain, _ = gpu_ctx.align_batch(dm, [P.native_read(r2)], params, native.SA_FP64)
-    assert np.array_equal(res[1].pairs, plain[0].pairs) and np.all(res[1].paths == 0)
+    # the multi-path sweeps and the single-path sweeps order the posterior's additions
+    # differently; a posterior is truncated to 1e-7 steps, so one within an ulp of a
+    # step (most often p = 1) can land one step apart
+    got, want = res[1].pairs.astype(np.int64), plain[0].pairs.astype(np.int64)
+    assert np.array_equal(got[:, 1:], want[:, 1:]) and np.all(res[1].paths == 0)
+    assert np.max(np.abs(got[:, 0] - want[:, 0])) <= 1
     assert res[1].log_likelihood == plain[0].log_likelihood
     with pytest.raises(native.SignalAlignError):
         gpu_ctx.align_batch(dm, reads, params, native.SA_FP32)
