@@ -429,7 +429,8 @@ def serving_leg(native, synth, ctx, gmodel, args):
         pools.append((ctx.model(m), objs, [o._struct() for o in objs]))
     log(f"serving: pools of {args.serve_pool} reads per model in {time.time() - t0:.1f}s")
     ps = native.Params(diagonal_expansion=args.expansion)._struct()
-    srv = native.Server(0, hbm_budget_bytes=96 << 30, lanes=args.serve_lanes, max_wave_reads=args.serve_max_wave)
+    srv = native.Server(0, hbm_budget_bytes=96 << 30, lanes=args.serve_lanes, max_wave_reads=args.serve_max_wave,
+                        length_ratio=args.serve_length_ratio)
     rng = np.random.default_rng(args.seed + 13)
 
     def pick(n):
@@ -500,7 +501,7 @@ def serving_leg(native, synth, ctx, gmodel, args):
            "achieved_reads_per_s": round(len(reqs) / span, 1), "waves": nw,
            "mean_wave_reads": round(len(reqs) / max(nw, 1), 1), "submitter_max_lag_ms": round(lag * 1e3, 3),
            "max_wave_reads": st["max_wave_reads"], "lanes": args.serve_lanes,
-           "wave_cap_reads": args.serve_max_wave}
+           "wave_cap_reads": args.serve_max_wave, "length_ratio": args.serve_length_ratio}
     log(f"serving: {out}")
     return out
 
@@ -573,6 +574,7 @@ def main():
                     help="the HDP leg's schedules to time (one of them: for profiling it)")
     ap.add_argument("--hdp-traffic-file", default=os.path.join(ROOT, "profiles", "traffic_hdp_2048x2reads.txt"))
     ap.add_argument("--no-serve", action="store_true", help="skip the serving configuration")
+    ap.add_argument("--serve-only", action="store_true", help="run only the serving configuration")
     ap.add_argument("--no-expect", action="store_true", help="skip the EM expectations configuration")
     ap.add_argument("--expect-reads", type=int, default=512)
     ap.add_argument("--no-degenerate", action="store_true", help="skip the degenerate-reference configuration")
@@ -582,6 +584,8 @@ def main():
     ap.add_argument("--serve-burst", type=int, default=8192)
     ap.add_argument("--serve-load", type=float, default=0.7)
     ap.add_argument("--serve-lanes", type=int, default=3)
+    ap.add_argument("--serve-length-ratio", type=float, default=2.0,
+                    help="a wave's reads are within this factor of the oldest request's length (negative: any)")
     ap.add_argument("--serve-max-wave", type=int, default=1024)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run a device copy of this many MiB (HBM byte-counter calibration under rocprofv3)")
@@ -594,6 +598,11 @@ def main():
 
     from signalalign_amd import native, synth
 
+    if args.serve_only:
+        ctx = native.Context(0)
+        print(json.dumps({"serving": serving_leg(native, synth, ctx, synth.gaussian_model("ACEGT", 6, seed=1), args)}),
+              flush=True)
+        return
     if args.hdp_only:
         if args.copy_calibration_mib:
             copy_calibration(args.copy_calibration_mib)

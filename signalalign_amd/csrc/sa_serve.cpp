@@ -121,8 +121,10 @@ void sa_server::schedule_loop() {
             Request *q = *it;
             const bool fits = wave.empty() ? true : inflight_bytes + bytes + q->est_bytes <= cfg.hbm_budget_bytes;
             /* a wave finishes with its longest read: keep reads of one length class
-               (within a factor of two of the oldest request) together */
-            const bool same_class = 2 * q->n_diag >= first->n_diag && q->n_diag <= 2 * first->n_diag;
+               (within length_ratio of the oldest request) together */
+            const double lr = cfg.length_ratio;
+            const bool same_class = lr < 0.0 || ((double)q->n_diag * lr >= (double)first->n_diag &&
+                                                 (double)q->n_diag <= lr * (double)first->n_diag);
             if (q->model == first->model && q->precision == first->precision && q->options == first->options &&
                 same_params(q->params, first->params) && same_class && fits) {
                 wave.push_back(q);
@@ -329,6 +331,8 @@ int sa_server_create(int device, const sa_server_config *cfg, sa_server **out) {
     if (c.hbm_budget_bytes <= 0) c.hbm_budget_bytes = (int64_t)32 << 30;
     if (c.lanes <= 0) c.lanes = 3;
     if (c.max_wave_reads <= 0) c.max_wave_reads = 2048;
+    if (c.length_ratio == 0.0) c.length_ratio = 2.0;
+    if (c.length_ratio > 0.0 && c.length_ratio < 1.0) return SA_ERR_INVALID_ARGUMENT;
     if (c.lanes > 4) return SA_ERR_INVALID_ARGUMENT;
     auto *s = new sa_server();
     s->cfg = c;
