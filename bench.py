@@ -430,7 +430,7 @@ def serving_leg(native, synth, ctx, gmodel, args):
     log(f"serving: pools of {args.serve_pool} reads per model in {time.time() - t0:.1f}s")
     ps = native.Params(diagonal_expansion=args.expansion)._struct()
     srv = native.Server(0, hbm_budget_bytes=96 << 30, lanes=args.serve_lanes, max_wave_reads=args.serve_max_wave,
-                        length_ratio=args.serve_length_ratio)
+                        length_ratio=args.serve_length_ratio, batch_delay_ms=args.serve_batch_delay)
     rng = np.random.default_rng(args.seed + 13)
 
     def pick(n):
@@ -443,6 +443,8 @@ def serving_leg(native, synth, ctx, gmodel, args):
         q = queue.Queue()
         lag = [0.0]
 
+        sub_s = [0.0]
+
         def submitter():
             start = time.perf_counter()
             arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs))) if rate else np.zeros(len(reqs))
@@ -454,6 +456,7 @@ def serving_leg(native, synth, ctx, gmodel, args):
                     lag[0] = max(lag[0], now - at)
                 dm, _, structs = pools[mi]
                 q.put(srv.submit_struct(dm, structs[ri], ps, native.SA_FP32))
+            sub_s[0] = time.perf_counter() - start
             q.put(None)
 
         th = threading.Thread(target=submitter)
@@ -471,6 +474,7 @@ def serving_leg(native, synth, ctx, gmodel, args):
             t_last = max(t_last, tm["done_s"])
             waves.add(tm["wave"])
         th.join()
+        log(f"serving: {len(reqs)} submissions took {sub_s[0] * 1e3:.0f} ms")
         return np.array(lat), cells, t_last - t_first, len(waves), lag[0]
 
     burst = pick(args.serve_burst)
@@ -501,7 +505,8 @@ def serving_leg(native, synth, ctx, gmodel, args):
            "achieved_reads_per_s": round(len(reqs) / span, 1), "waves": nw,
            "mean_wave_reads": round(len(reqs) / max(nw, 1), 1), "submitter_max_lag_ms": round(lag * 1e3, 3),
            "max_wave_reads": st["max_wave_reads"], "lanes": args.serve_lanes,
-           "wave_cap_reads": args.serve_max_wave, "length_ratio": args.serve_length_ratio}
+           "wave_cap_reads": args.serve_max_wave, "length_ratio": args.serve_length_ratio,
+           "batch_delay_ms": args.serve_batch_delay}
     log(f"serving: {out}")
     return out
 
@@ -583,7 +588,9 @@ def main():
     ap.add_argument("--serve-pool", type=int, default=1024)
     ap.add_argument("--serve-burst", type=int, default=8192)
     ap.add_argument("--serve-load", type=float, default=0.7)
-    ap.add_argument("--serve-lanes", type=int, default=3)
+    ap.add_argument("--serve-lanes", type=int, default=4)
+    ap.add_argument("--serve-batch-delay", type=float, default=30.0,
+                    help="ms a free lane waits for requests to gather while other waves run (negative: none)")
     ap.add_argument("--serve-length-ratio", type=float, default=2.0,
                     help="a wave's reads are within this factor of the oldest request's length (negative: any)")
     ap.add_argument("--serve-max-wave", type=int, default=1024)

@@ -101,18 +101,40 @@ void sa_server::schedule_loop() {
     std::unique_lock<std::mutex> lk(mu);
     for (;;) {
         Lane *idle = nullptr;
-        cv_sched.wait(lk, [&] {
+        double due = -1.0;
+        auto ready = [&] {
+            due = -1.0;
             if (stop) return true;
             if (pending.empty()) return false;
             idle = nullptr;
-            for (auto &l : lanes)
-                if (!l->busy) { idle = l.get(); break; }
+            bool others = false;
+            for (auto &l : lanes) {
+                if (!l->busy) { if (!idle) idle = l.get(); }
+                else others = true;
+            }
             if (!idle) return false;
             /* memory: the head request must fit beside the waves in flight (alone, it is
                admitted once nothing else is in flight) */
             const int64_t head = pending.front()->est_bytes;
-            return inflight_bytes == 0 || inflight_bytes + head <= cfg.hbm_budget_bytes;
-        });
+            if (!(inflight_bytes == 0 || inflight_bytes + head <= cfg.hbm_budget_bytes)) return false;
+            /* continuous batching: an idle device takes the wave at once; a busy one lets
+               it gather requests until the oldest has waited batch_delay_ms */
+            if (others && cfg.batch_delay_ms > 0.0 && (int64_t)pending.size() < cfg.max_wave_reads) {
+                const double d = pending.front()->timing.submit_s + 1e-3 * cfg.batch_delay_ms;
+                if (now() < d) {
+                    due = d;
+                    return false;
+                }
+            }
+            return true;
+        };
+        while (!ready()) {
+            if (due > 0.0)
+                cv_sched.wait_until(lk, t0 + std::chrono::duration_cast<std::chrono::steady_clock::duration>(
+                                                 std::chrono::duration<double>(due)));
+            else
+                cv_sched.wait(lk);
+        }
         if (stop) return;
         Request *first = pending.front();
         std::vector<Request *> wave;
@@ -329,9 +351,10 @@ int sa_server_create(int device, const sa_server_config *cfg, sa_server **out) {
     sa_server_config c{};
     if (cfg) c = *cfg;
     if (c.hbm_budget_bytes <= 0) c.hbm_budget_bytes = (int64_t)32 << 30;
-    if (c.lanes <= 0) c.lanes = 3;
+    if (c.lanes <= 0) c.lanes = 4;
     if (c.max_wave_reads <= 0) c.max_wave_reads = 2048;
     if (c.length_ratio == 0.0) c.length_ratio = 2.0;
+    if (c.batch_delay_ms == 0.0) c.batch_delay_ms = 30.0;
     if (c.length_ratio > 0.0 && c.length_ratio < 1.0) return SA_ERR_INVALID_ARGUMENT;
     if (c.lanes > 4) return SA_ERR_INVALID_ARGUMENT;
     auto *s = new sa_server();

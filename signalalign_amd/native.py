@@ -84,7 +84,7 @@ class _Timing(ctypes.Structure):
 
 class _ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("lanes", ctypes.c_int32), ("max_wave_reads", ctypes.c_int32),
-                ("length_ratio", ctypes.c_double)]
+                ("length_ratio", ctypes.c_double), ("batch_delay_ms", ctypes.c_double)]
 
 
 class _ReqTiming(ctypes.Structure):
@@ -518,9 +518,11 @@ class Server:
     """Request loop with continuous batching (sa_server_*): submit reads one at a
     time, collect results as waves finish."""
 
-    def __init__(self, device=0, hbm_budget_bytes=0, lanes=0, max_wave_reads=0, length_ratio=0.0):
+    def __init__(self, device=0, hbm_budget_bytes=0, lanes=0, max_wave_reads=0, length_ratio=0.0,
+                 batch_delay_ms=0.0):
         L = load()
-        cfg = _ServerConfig(int(hbm_budget_bytes), int(lanes), int(max_wave_reads), float(length_ratio))
+        cfg = _ServerConfig(int(hbm_budget_bytes), int(lanes), int(max_wave_reads), float(length_ratio),
+                            float(batch_delay_ms))
         h = ctypes.c_void_p()
         rc = L.sa_server_create(device, ctypes.byref(cfg), ctypes.byref(h))
         if rc != 0:

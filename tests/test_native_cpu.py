@@ -28,6 +28,15 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(L, n), f"{n} declared in include/signalalign_amd.h but not exported"
 
 
+@pytest.mark.parametrize("cfg", [dict(length_ratio=0.5), dict(lanes=5)])
+def test_server_config_checked_before_any_device_call(cfg):
+    """A length class narrower than one length, or more than four lanes, is refused
+    before the server touches a device."""
+    with pytest.raises(native.SignalAlignError) as e:
+        native.Server(0, **cfg)
+    assert e.value.code == -1  # SA_ERR_INVALID_ARGUMENT
+
+
 def test_no_device_is_an_error_not_a_fallback():
     if native.device_count() > 0:
         pytest.skip("a device is present")
