@@ -38,7 +38,7 @@ namespace sa {
 
 template <typename T>
 __global__ void __launch_bounds__(256) prep_kernel(Args<T> a) {
-    const int ti = blockIdx.x;
+    const int ti = a.task_order ? a.task_order[blockIdx.x] : (int)blockIdx.x;
     const TaskDev t = a.tasks[ti];
     const int tid = threadIdx.x;
     const int nt = blockDim.x;
@@ -49,6 +49,19 @@ __global__ void __launch_bounds__(256) prep_kernel(Args<T> a) {
     __shared__ int s_err;
     __shared__ int s_nwin;
     __shared__ int s_span;
+    /* the character table and (up to kPrepAnc of) the sub-alignment's anchors in LDS:
+       every k-mer reads the table k times and every diagonal binary-searches the
+       anchors, each a chain of dependent loads */
+    constexpr int kPrepAnc = 2048;
+    __shared__ int8_t s_lut[256];
+    __shared__ int32_t s_ax[kPrepAnc], s_ay[kPrepAnc];
+    const bool anc_lds = t.n_anc <= kPrepAnc;
+    for (int i = tid; i < 256; i += nt) s_lut[i] = a.lut[i];
+    if (anc_lds)
+        for (int i = tid; i < t.n_anc; i += nt) {
+            s_ax[i] = a.anc_x[t.anc_off + i];
+            s_ay[i] = a.anc_y[t.anc_off + i];
+        }
     if (tid == 0) { s_cells = 0; s_xy = 0; s_maxw = 0; s_err = 0; s_nwin = 0; s_span = 1; }
     __syncthreads();
     int err = 0;
@@ -57,7 +70,7 @@ __global__ void __launch_bounds__(256) prep_kernel(Args<T> a) {
         bool bad = false;
         for (int j = 0; j < k; j++) {
             const char ch = a.ref[t.ref_off + p + j];
-            int c = a.lut[(uint8_t)ch];
+            int c = s_lut[(uint8_t)ch];
             bad |= c < 0 && !(a.paths && ch == 'X'); /* an 'X' k-mer's ids come per path */
             id = id * A + (c < 0 ? 0 : c);
         }
@@ -79,7 +92,9 @@ __global__ void __launch_bounds__(256) prep_kernel(Args<T> a) {
     int maxw = 0;
     for (int64_t d = tid; d <= N; d += nt) {
         int64_t L = 0, R = 0;
-        bool ok = band_at(a.anc_x + t.anc_off, a.anc_y + t.anc_off, t.n_anc, t.lX, t.lY, a.expansion, d, &L, &R);
+        const bool ok = anc_lds ? band_at(s_ax, s_ay, t.n_anc, t.lX, t.lY, a.expansion, d, &L, &R)
+                                : band_at(a.anc_x + t.anc_off, a.anc_y + t.anc_off, t.n_anc, t.lX, t.lY, a.expansion,
+                                          d, &L, &R);
         if (!ok) { err |= ERR_BAND; L = -d; R = -d; }
         int32_t lo = (int32_t)div2(d + L), hi = (int32_t)div2(d + R);
         a.xlo[t.diag_off + d] = lo;

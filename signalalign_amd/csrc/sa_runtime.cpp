@@ -61,6 +61,7 @@ struct sa_batch {
     int64_t stride = 4;
     /* raw inputs resident in HBM */
     DBuf d_ref, d_events, d_ancx, d_ancy, d_tasks, d_x1, d_y1;
+    DBuf d_prep_order; /* sub-alignments longest first: the order prep's workgroups start in */
     /* derived */
     DBuf d_kid, d_evm, d_evn, d_evl, d_evi, d_xlo, d_xhi, d_doff, d_foff, d_dflag, d_dcls, d_doffxy;
     DBuf d_task_cells, d_task_xycells, d_task_maxw, d_task_span, d_task_hspan, d_task_nwin, d_win_slots, d_err;
@@ -103,7 +104,8 @@ struct sa_batch {
     } gc[2];
     ~sa_batch() {
         for (auto &g : gc) g.reset();
-        DBuf *all[] = {&d_ref, &d_events, &d_ancx, &d_ancy, &d_tasks, &d_x1, &d_y1, &d_kid, &d_evm, &d_evn,
+        DBuf *all[] = {&d_ref, &d_events, &d_ancx, &d_ancy, &d_tasks, &d_x1, &d_y1, &d_prep_order, &d_kid, &d_evm,
+                       &d_evn,
                        &d_evl, &d_evi, &d_xlo, &d_xhi, &d_doff, &d_foff, &d_dflag, &d_dcls, &d_doffxy, &d_task_cells,
                        &d_task_xycells, &d_task_maxw, &d_task_span, &d_task_hspan,
                        &d_task_nwin, &d_win_slots, &d_err, &d_fM, &d_fX, &d_fY, &d_wins, &d_task_order,
@@ -637,10 +639,17 @@ int batch_fill(sa_context *ctx, sa_batch *b, const sa_model *model, const sa_rea
     SA_HIP(ctx, hipStreamSynchronize(ctx->stream));
     std::vector<int32_t> x1(b->tasks.size()), y1(b->tasks.size());
     for (size_t i = 0; i < b->tasks.size(); i++) { x1[i] = b->thost[i].x1; y1[i] = b->thost[i].y1; }
+    /* prep's workgroups take the sub-alignments longest first: a long one's
+       workgroup then starts in the first round instead of holding the kernel's tail */
+    std::vector<int32_t> porder(b->tasks.size());
+    for (size_t i = 0; i < porder.size(); i++) porder[i] = (int32_t)i;
+    std::stable_sort(porder.begin(), porder.end(), [&](int32_t u, int32_t v) {
+        return (int64_t)b->tasks[u].lX + b->tasks[u].lY > (int64_t)b->tasks[v].lX + b->tasks[v].lY;
+    });
     int rc;
     if ((rc = upload(ctx, b->d_ancx, ancx)) || (rc = upload(ctx, b->d_ancy, ancy)) ||
         (rc = upload(ctx, b->d_tasks, b->tasks)) || (rc = upload(ctx, b->d_x1, x1)) ||
-        (rc = upload(ctx, b->d_y1, y1)))
+        (rc = upload(ctx, b->d_y1, y1)) || (rc = upload(ctx, b->d_prep_order, porder)))
         return rc;
     return SA_OK;
 
@@ -829,6 +838,7 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     std::memset(&a, 0, sizeof(a));
     a.m = device_model<T>(model, tab);
     a.tasks = b->d_tasks.as<TaskDev>();
+    a.task_order = b->d_prep_order.as<int32_t>(); /* (the sweeps' launches set their own) */
     a.ref = b->d_ref.as<char>();
     a.events = b->d_events.as<double>();
     a.event_stride = b->stride;
@@ -1601,6 +1611,7 @@ int run_paths(sa_batch *b, sa_result *res) {
     std::memset(&a, 0, sizeof(a));
     a.m = device_model<double>(model, tab);
     a.tasks = b->d_tasks.as<TaskDev>();
+    a.task_order = b->d_prep_order.as<int32_t>(); /* (the sweeps' launches set their own) */
     a.ref = b->d_ref.as<char>();
     a.events = b->d_events.as<double>();
     a.event_stride = b->stride;
