@@ -50,6 +50,9 @@ template <typename T> struct KCon<T, 2> {
 
 template <typename T> struct KCon<T, 3> {};
 
+/* the three states of one diagonal's cells of a lane (forward sweeps) */
+template <typename T, int CPT> struct FStates { T M[CPT], X[CPT], Y[CPT]; };
+
 template <typename T, int EK> struct ECon;
 template <typename T> struct ECon<T, 0> { T m, n, l, i; };
 template <typename T> struct ECon<T, 1> { T m, n, l, i; };
@@ -928,6 +931,7 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
     ECon<T, EK> ev[CPT];
     T pM[CPT], pX[CPT], pY[CPT], qM[CPT], qX[CPT], qY[CPT];
     T LpM, LpX, LpY;
+    using FSt = FStates<T, CPT>;
     int d = st.d;
     int X0;
     if (d == 0) {
@@ -1010,7 +1014,11 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
     int next = 0;
     const int d0 = d; /* the segment's first diagonal */
 
-    for (; d <= N; d++) {
+    /* One diagonal: P holds d-1's states, Q d-2's; Q receives d's (cells from the
+       right, so that each of d-2's states is read before its register is reused).
+       False: the segment ends before d.  The loop below alternates the two roles, so
+       no state is copied between diagonals. */
+    auto step = [&](FSt &P, FSt &Q) -> bool {
         int j = d - dc;
         const int lo = nlo, hi = nhi, fl = nfl;
         if (j == 64) {
@@ -1021,7 +1029,7 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
             ncls = d + 64 <= N ? ld_scalar_byte(dcl, d + 64) : 0;
             if (want != CPT) {
                 next = want;
-                break;
+                return false;
             }
         }
         if (d < N) { nlo = ld_scalar(xlo, d + 1); nhi = ld_scalar(xhi, d + 1); nfl = ld_scalar_byte(dfl, d + 1); }
@@ -1037,7 +1045,7 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
                register copies and memory waits on every diagonal. */
             if (d != d0) {
                 next = CPT;
-                break;
+                return false;
             }
             bad = true; /* the window holds fewer positions than the band: the LDS sweeps redo it */
         }
@@ -1053,16 +1061,15 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
             const int jn = d - X0; /* event of the first cell of the window on d+1 */
             load_econ_uniform(pre, a, evc, jn, jn >= 0 && jn < lY);
         }
-        T LM = shr1(pM[CPT - 1], NI), LX = shr1(pX[CPT - 1], NI), LY = shr1(pY[CPT - 1], NI);
+        T LM = shr1(P.M[CPT - 1], NI), LX = shr1(P.X[CPT - 1], NI), LY = shr1(P.Y[CPT - 1], NI);
         const bool renorm = NORM && (d & 7) == 0;
-        T nM[CPT], nX[CPT], nY[CPT];
         T mx = NI;
 #pragma unroll
-        for (int c = 0; c < CPT; c++) {
+        for (int c = CPT - 1; c >= 0; c--) {
             const int x = X0 + CPT * lane + c;
             const bool valid = x >= lo && x <= hi;
-            const T lM = c ? pM[c - 1] : LM, lXv = c ? pX[c - 1] : LX, lYv = c ? pY[c - 1] : LY;
-            const T mM = c ? qM[c - 1] : LpM, mXv = c ? qX[c - 1] : LpX, mYv = c ? qY[c - 1] : LpY;
+            const T lM = c ? P.M[c - 1] : LM, lXv = c ? P.X[c - 1] : LX, lYv = c ? P.Y[c - 1] : LY;
+            const T mM = c ? Q.M[c - 1] : LpM, mXv = c ? Q.X[c - 1] : LpX, mYv = c ? Q.Y[c - 1] : LpY;
             const T eX = em_gapx<T>(m, kc[c]);
             T eM, eY;
             if constexpr (PL) {
@@ -1081,14 +1088,14 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
             T M = mM + (eM + tMC);
             M = logadd_lut(M, mXv + (eM + tMFX), lut);
             M = logadd_lut(M, mYv + (eM + tMFY), lut);
-            T Y = pM[c] + (eY + tGOY);
-            Y = logadd_lut(Y, pY[c] + (eY + tGEY), lut);
-            nM[c] = valid ? M : NI;
-            nX[c] = valid ? X : NI;
-            nY[c] = valid ? Y : NI;
+            T Y = P.M[c] + (eY + tGOY);
+            Y = logadd_lut(Y, P.Y[c] + (eY + tGEY), lut);
+            Q.M[c] = valid ? M : NI;
+            Q.X[c] = valid ? X : NI;
+            Q.Y[c] = valid ? Y : NI;
             if (renorm) {
-                T m3 = nM[c] > nX[c] ? nM[c] : nX[c];
-                m3 = m3 > nY[c] ? m3 : nY[c];
+                T m3 = Q.M[c] > Q.X[c] ? Q.M[c] : Q.X[c];
+                m3 = m3 > Q.Y[c] ? m3 : Q.Y[c];
                 mx = m3 > mx ? m3 : mx;
             }
         }
@@ -1101,8 +1108,8 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
             if (mx > NI) {
 #pragma unroll
                 for (int c = 0; c < CPT; c++) {
-                    nM[c] -= mx; nX[c] -= mx; nY[c] -= mx;
-                    pM[c] -= mx; pX[c] -= mx; pY[c] -= mx;
+                    Q.M[c] -= mx; Q.X[c] -= mx; Q.Y[c] -= mx;
+                    P.M[c] -= mx; P.X[c] -= mx; P.Y[c] -= mx;
                 }
                 LM -= mx; LX -= mx; LY -= mx;
                 offd = off1 + (double)mx;
@@ -1134,29 +1141,40 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
         for (int c = 0; c < CPT; c++) {
             const int x = X0 + CPT * lane + c;
             if (x >= lo && x <= hi) {
-                fM[run + (x - lo)] = nM[c];
-                if (fl) { fX[runxy + (x - lo)] = nX[c]; fY[runxy + (x - lo)] = nY[c]; }
+                fM[run + (x - lo)] = Q.M[c];
+                if (fl) { fX[runxy + (x - lo)] = Q.X[c]; fY[runxy + (x - lo)] = Q.Y[c]; }
             }
         }
         if (lane == j) { md_doff = (int32_t)run; md_dxy = fl ? (int32_t)runxy : -1; md_foff = offd; }
         run += hi - lo + 1;
         if (fl) runxy += hi - lo + 1;
         LpM = LM; LpX = LX; LpY = LY;
-#pragma unroll
-        for (int c = 0; c < CPT; c++) {
-            qM[c] = pM[c]; qX[c] = pX[c]; qY[c] = pY[c];
-            pM[c] = nM[c]; pX[c] = nX[c]; pY[c] = nY[c];
-        }
         off1 = offd;
         lo2 = lo1;
         lo1 = lo;
+        d++;
+        return true;
+    };
+    FSt A, B; /* the states of d-1 and d-2 */
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+        A.M[c] = pM[c]; A.X[c] = pX[c]; A.Y[c] = pY[c];
+        B.M[c] = qM[c]; B.X[c] = qX[c]; B.Y[c] = qY[c];
+    }
+    bool swapped = false; /* d-1's states are in B */
+    while (d <= N) {
+        if (!step(A, B)) break;
+        swapped = true;
+        if (d > N || !step(B, A)) break;
+        swapped = false;
     }
     if (next) {
+        const FSt &P = swapped ? B : A, &Q = swapped ? A : B;
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
             const int i = CPT * lane + c;
-            xs[0 * xsw + i] = pM[c]; xs[1 * xsw + i] = pX[c]; xs[2 * xsw + i] = pY[c];
-            xs[3 * xsw + i] = qM[c]; xs[4 * xsw + i] = qX[c]; xs[5 * xsw + i] = qY[c];
+            xs[0 * xsw + i] = P.M[c]; xs[1 * xsw + i] = P.X[c]; xs[2 * xsw + i] = P.Y[c];
+            xs[3 * xsw + i] = Q.M[c]; xs[4 * xsw + i] = Q.X[c]; xs[5 * xsw + i] = Q.Y[c];
         }
         __syncthreads();
     }
