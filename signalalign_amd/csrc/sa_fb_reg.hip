@@ -1340,6 +1340,7 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
     ECon<T, EK> ev[CPT];
     T pM[CPT], pX[CPT], pY[CPT], qM[CPT];
     T RpM; /* lane i+1's first cell of e+2, match state */
+    using FSt = FStates<T, CPT>;
     if (es == D) {
         X0 = nhi - W + 1;
 #pragma unroll
@@ -1406,7 +1407,10 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
     /* FP64 Gaussian sweeps evaluate every match emission: the registers the reuse takes
        cost them more spills than the evaluation */
     constexpr bool RE = EK >= 2 || sizeof(T) == 4;
-    T eYp[CPT]; /* the match emission of each cell's own k-mer and event, from e+1 */
+    /* the match emission of each cell's own k-mer and event: from e+1 until the cell
+       reads its right neighbour's, then its own of e (cells run from the left, so each
+       value is read before its register is rewritten) */
+    T eYp[CPT];
     bool have_gp = false, have_eY = false;
     /* E(dd, xx) straight from the plane (window tops and slides) */
     auto plane_at = [&](int dd, int xx) -> T {
@@ -1444,14 +1448,18 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
     };
 
     int e = es;
-    for (; e > TB; e--) {
+    /* One diagonal: P holds e+1's states, Q e+2's (its match states only); Q receives
+       e's (cells from the left, so that each of e+2's match states is read before its
+       register is reused).  False: the segment ends before e.  The loop below
+       alternates the two roles, so no state is copied between diagonals. */
+    auto step = [&](FSt &P, FSt &Q) -> bool {
         if ((e & 63) == 63 && e != es) {
             /* top of block e/64: its class decides the width */
             const int want = window_class((ncls >> 3) & 7, cmax);
             ncls = e - 64 > TB ? ld_scalar_byte(dcl, e - 127) : 0;
             if (want != CPT) {
                 next = want;
-                break;
+                return false;
             }
         }
         const int lo = nlo, hi = nhi, dof = ndo;
@@ -1460,16 +1468,14 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
             nlo = ld_scalar(xlo, e - 1); nhi = ld_scalar(xhi, e - 1); ndo = ld_scalar(doff, e - 1);
             nfo = ld_scalar(foff, e - 1);
         }
-        T nM[CPT], nX[CPT], nY[CPT];
         T LowM = NI, LowX = NI; /* lane i+1's first cell of e+1 */
         double obe = 0.0;
-        T eYc[CPT];
         if (e == D) {
 #pragma unroll
             for (int c = 0; c < CPT; c++) {
                 const int x = X0 + CPT * lane + c;
                 const bool valid = x >= lo && x <= hi;
-                nM[c] = valid ? e0 : NI; nX[c] = valid ? e1 : NI; nY[c] = valid ? e2 : NI;
+                Q.M[c] = valid ? e0 : NI; Q.X[c] = valid ? e1 : NI; Q.Y[c] = valid ? e2 : NI;
             }
             fresh = false;
         } else {
@@ -1481,7 +1487,7 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                    loop register copies and memory waits on every diagonal. */
                 if (e != es) {
                     next = CPT;
-                    break;
+                    return false;
                 }
                 bad = true; /* the window holds fewer positions than the band: the LDS sweeps redo it */
             }
@@ -1494,8 +1500,8 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                 ev[CPT - 1] = in;
             }
             fresh = false;
-            LowM = shl1(pM[0], NI);
-            LowX = shl1(pX[0], NI);
+            LowM = shl1(P.M[0], NI);
+            LowX = shl1(P.X[0], NI);
             const KCon<T, EK> kn = shl1_match(kc[0]); /* k-mer x of lane's last cell */
             const T eYn = have_eY ? shl1(eYp[0], NI) : NI;
             const bool renorm = NORM && ((D - e) & 7) == 0;
@@ -1505,19 +1511,19 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                 const int x = X0 + CPT * lane + c;
                 const bool valid = x >= lo && x <= hi;
                 const KCon<T, EK> &kx = c < CPT - 1 ? kc[c + 1] : kn;
-                const T ClowX = c < CPT - 1 ? pX[c + 1] : LowX;
-                const T CupY = pY[c];
-                const T CmidM = c < CPT - 1 ? qM[c + 1] : RpM;
+                const T ClowX = c < CPT - 1 ? P.X[c + 1] : LowX;
+                const T CupY = P.Y[c];
+                const T CmidM = c < CPT - 1 ? Q.M[c + 1] : RpM;
                 const T eX = em_gapx<T>(m, kx);
                 T eM, eY;
                 if constexpr (PL) {
                     eY = have_pe ? pe[PL ? c : 0] : plane_at(e + 1, x);
                     eM = have_eY ? (c < CPT - 1 ? eYp[c < CPT - 1 ? c + 1 : 0] : eYn) : plane_at(e + 2, x + 1);
-                    eYc[c] = eY;
+                    eYp[c] = eY;
                 } else if constexpr (PF) {
                     eY = have_gp ? hdp_finish(m, gp[c], iv) : em_gapy<T>(m, kc[c], ev[c], s, iv);
                     eM = have_eY ? (c < CPT - 1 ? eYp[c < CPT - 1 ? c + 1 : 0] : eYn) : em_match<T>(m, kx, ev[c], s, iv);
-                    eYc[c] = eY;
+                    eYp[c] = eY;
                 } else {
                     T mself; /* (FP64: the refresh below evaluates it again) */
                     em_both<T>(m, kc[c], ev[c], s, iv, mself, eY);
@@ -1527,7 +1533,7 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                         T unused;
                         em_both<T>(m, kx, ev[c], s, iv, eM, unused);
                     }
-                    eYc[c] = mself;
+                    eYp[c] = mself;
                 }
                 /* contributions arrive middle (e+2), then upper, then lower (e+1) */
                 T M = CmidM + (eM + tMC);
@@ -1538,17 +1544,15 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                 T Y = CmidM + (eM + tMFY);
                 Y = logadd_lut(Y, CupY + (eY + tGEY), lut);
                 if (useSWX) Y = logadd_lut(Y, ClowX + (eX + tSWX), lut);
-                nM[c] = valid ? M : NI;
-                nX[c] = valid ? X : NI;
-                nY[c] = valid ? Y : NI;
+                Q.M[c] = valid ? M : NI;
+                Q.X[c] = valid ? X : NI;
+                Q.Y[c] = valid ? Y : NI;
                 if (renorm) {
-                    T m3 = nM[c] > nX[c] ? nM[c] : nX[c];
-                    m3 = m3 > nY[c] ? m3 : nY[c];
+                    T m3 = Q.M[c] > Q.X[c] ? Q.M[c] : Q.X[c];
+                    m3 = m3 > Q.Y[c] ? m3 : Q.Y[c];
                     mx = m3 > mx ? m3 : mx;
                 }
             }
-#pragma unroll
-            for (int c = 0; c < CPT; c++) eYp[c] = eYc[c];
             have_eY = true;
             obe = ob1;
             if (renorm) {
@@ -1559,8 +1563,8 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                 if (mx > NI) {
 #pragma unroll
                     for (int c = 0; c < CPT; c++) {
-                        nM[c] -= mx; nX[c] -= mx; nY[c] -= mx;
-                        pM[c] -= mx;
+                        Q.M[c] -= mx; Q.X[c] -= mx; Q.Y[c] -= mx;
+                        P.M[c] -= mx;
                     }
                     LowM -= mx;
                     obe = ob1 + (double)mx;
@@ -1604,9 +1608,9 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                     tA[c] = -__builtin_huge_val();
                     tB[c] = -__builtin_huge_val();
                     if (x >= lo && x <= hi) {
-                        T cell = (have_fpre ? fpre[c] : fM[dof + (x - lo)]) + nM[c];
-                        cell = logadd_lut(cell, rfx[c] + nX[c], lut);
-                        cell = logadd_lut(cell, rfy[c] + nY[c], lut);
+                        T cell = (have_fpre ? fpre[c] : fM[dof + (x - lo)]) + Q.M[c];
+                        cell = logadd_lut(cell, rfx[c] + Q.X[c], lut);
+                        cell = logadd_lut(cell, rfy[c] + Q.Y[c], lut);
                         tA[c] = (double)cell + offe;
                     }
                 }
@@ -1623,7 +1627,7 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                             /* the match emission of k-mer x, event e-x */
                             T eM;
                             if constexpr (RE) {
-                                eM = eYc[c];
+                                eM = eYp[c];
                             } else {
                                 T unused;
                                 em_both<T>(m, kc[c], ev[c], s, iv, eM, unused);
@@ -1631,7 +1635,7 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                             T Mv = rgm[c] + (eM + tMC);
                             Mv = logadd_lut(Mv, rgx[c] + (eM + tMFX), lut);
                             Mv = logadd_lut(Mv, rgy[c] + (eM + tMFY), lut);
-                            tB[c] = (double)(Mv + pM[c]) + offm;
+                            tB[c] = (double)(Mv + P.M[c]) + offm;
                         }
                     }
                 }
@@ -1655,7 +1659,7 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                    the test, so no per-cell bounds check is needed */
 #pragma unroll
                 for (int c = 0; c < CPT; c++) {
-                    sv[c] = NORM ? (fpre[c] + nM[c]) + cst : (fpre[c] + nM[c]) - (T)total;
+                    sv[c] = NORM ? (fpre[c] + Q.M[c]) + cst : (fpre[c] + Q.M[c]) - (T)total;
                     cand[c] = sv[c] >= lthr;
                     anyc |= cand[c];
                 }
@@ -1668,7 +1672,7 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                     sv[c] = NI;
                     if (x >= lo && x <= hi && x > 0 && y > 0) {
                         const T fm = have_fpre ? fpre[c] : fM[dof + (x - lo)];
-                        sv[c] = NORM ? (fm + nM[c]) + cst : (fm + nM[c]) - (T)total;
+                        sv[c] = NORM ? (fm + Q.M[c]) + cst : (fm + Q.M[c]) - (T)total;
                         cand[c] = sv[c] >= lthr;
                     }
                     anyc |= cand[c];
@@ -1724,20 +1728,44 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
             }
         }
         RpM = LowM;
-#pragma unroll
-        for (int c = 0; c < CPT; c++) {
-            qM[c] = pM[c];
-            pM[c] = nM[c]; pX[c] = nX[c]; pY[c] = nY[c];
-        }
         ob1 = obe;
         hi2 = hi1;
         hi1 = hi;
+        e--;
+        return true;
+    };
+    FSt A, B; /* the states of e+1 and e+2 */
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+        A.M[c] = pM[c]; A.X[c] = pX[c]; A.Y[c] = pY[c];
+        B.M[c] = qM[c]; B.X[c] = NI; B.Y[c] = NI;
+    }
+    bool swapped = false; /* e+1's states are in B */
+    if constexpr (sizeof(T) == 4) {
+        while (e > TB) {
+            if (!step(A, B)) break;
+            swapped = true;
+            if (e <= TB || !step(B, A)) break;
+            swapped = false;
+        }
+    } else {
+        /* FP64: two full sets of states cost the sweep more spills than the copies */
+        while (e > TB) {
+            if (!step(A, B)) break;
+#pragma unroll
+            for (int c = 0; c < CPT; c++) {
+                const T q = A.M[c];
+                A.M[c] = B.M[c]; A.X[c] = B.X[c]; A.Y[c] = B.Y[c];
+                B.M[c] = q;
+            }
+        }
     }
     if (next) {
+        const FSt &P = swapped ? B : A, &Q = swapped ? A : B;
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
             const int i = CPT * lane + c;
-            xs[0 * xsw + i] = pM[c]; xs[1 * xsw + i] = pX[c]; xs[2 * xsw + i] = pY[c]; xs[3 * xsw + i] = qM[c];
+            xs[0 * xsw + i] = P.M[c]; xs[1 * xsw + i] = P.X[c]; xs[2 * xsw + i] = P.Y[c]; xs[3 * xsw + i] = Q.M[c];
         }
         __syncthreads();
     }
