@@ -1378,6 +1378,9 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
 #pragma unroll
         for (int c = 0; c < CPT; c++) { sink(kc[c]); sink(ev[c]); }
     }
+    /* the extra-event skip emission of k-mer x+1 for the lane's last cell (a segment
+       keeps its k-mers) */
+    const T eXn = em_gapx<T>(m, shl1_match(kc[0]));
     ECon<T, EK> pre;
     {
         const int jn = (es - 1) - (X0 + W - 1); /* event of lane 63's last cell on es-1 */
@@ -1502,7 +1505,6 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
             fresh = false;
             LowM = shl1(P.M[0], NI);
             LowX = shl1(P.X[0], NI);
-            const KCon<T, EK> kn = shl1_match(kc[0]); /* k-mer x of lane's last cell */
             const T eYn = have_eY ? shl1(eYp[0], NI) : NI;
             const bool renorm = NORM && ((D - e) & 7) == 0;
             T mx = NI;
@@ -1510,11 +1512,13 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
             for (int c = 0; c < CPT; c++) {
                 const int x = X0 + CPT * lane + c;
                 const bool valid = x >= lo && x <= hi;
-                const KCon<T, EK> &kx = c < CPT - 1 ? kc[c + 1] : kn;
+                /* k-mer x+1 (the lane's last cell: its right neighbour's first, shifted
+                   in only where an emission is evaluated from it) */
+                auto kx = [&]() { return c < CPT - 1 ? kc[c < CPT - 1 ? c + 1 : 0] : shl1_match(kc[0]); };
                 const T ClowX = c < CPT - 1 ? P.X[c + 1] : LowX;
                 const T CupY = P.Y[c];
                 const T CmidM = c < CPT - 1 ? Q.M[c + 1] : RpM;
-                const T eX = em_gapx<T>(m, kx);
+                const T eX = c < CPT - 1 ? em_gapx<T>(m, kc[c < CPT - 1 ? c + 1 : 0]) : eXn;
                 T eM, eY;
                 if constexpr (PL) {
                     eY = have_pe ? pe[PL ? c : 0] : plane_at(e + 1, x);
@@ -1522,7 +1526,7 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                     eYp[c] = eY;
                 } else if constexpr (PF) {
                     eY = have_gp ? hdp_finish(m, gp[c], iv) : em_gapy<T>(m, kc[c], ev[c], s, iv);
-                    eM = have_eY ? (c < CPT - 1 ? eYp[c < CPT - 1 ? c + 1 : 0] : eYn) : em_match<T>(m, kx, ev[c], s, iv);
+                    eM = have_eY ? (c < CPT - 1 ? eYp[c < CPT - 1 ? c + 1 : 0] : eYn) : em_match<T>(m, kx(), ev[c], s, iv);
                     eYp[c] = eY;
                 } else {
                     T mself; /* (FP64: the refresh below evaluates it again) */
@@ -1531,7 +1535,7 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                         eM = c < CPT - 1 ? eYp[c < CPT - 1 ? c + 1 : 0] : eYn;
                     } else {
                         T unused;
-                        em_both<T>(m, kx, ev[c], s, iv, eM, unused);
+                        em_both<T>(m, kx(), ev[c], s, iv, eM, unused);
                     }
                     eYp[c] = mself;
                 }
