@@ -585,7 +585,7 @@ def main():
     ap.add_argument("--no-degenerate", action="store_true", help="skip the degenerate-reference configuration")
     ap.add_argument("--degenerate-reads", type=int, default=256)
     ap.add_argument("--serve-requests", type=int, default=100000)
-    ap.add_argument("--serve-pool", type=int, default=1024)
+    ap.add_argument("--serve-pool", type=int, default=4096)
     ap.add_argument("--serve-burst", type=int, default=8192)
     ap.add_argument("--serve-load", type=float, default=0.7)
     ap.add_argument("--serve-lanes", type=int, default=4)
