@@ -24,6 +24,8 @@
  */
 #include "sa_device.h"
 
+#include <type_traits>
+
 namespace sa {
 
 /* ------------------------------------------------------------------------ */
@@ -1014,11 +1016,19 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
     int next = 0;
     const int d0 = d; /* the segment's first diagonal */
 
-    /* One diagonal: P holds d-1's states, Q d-2's; Q receives d's (cells from the
-       right, so that each of d-2's states is read before its register is reused).
-       False: the segment ends before d.  The loop below alternates the two roles, so
-       no state is copied between diagonals. */
-    auto step = [&](FSt &P, FSt &Q) -> bool {
+    /* One diagonal: P holds d-1's states, Q d-2's, Nw receives d's (at three cells per
+       lane with rotated events a third set, so that the roles repeat every third diagonal
+       as the events do; otherwise Q itself, cells from the right, so that each of d-2's
+       states is read before its register is reused).  False: the segment ends before
+       d.  The loop below rotates the roles, so no state is copied between diagonals.
+       Events: at two or three cells per lane cell c's event is in ev[(c - K) mod CPT]
+       on the loop's K-th diagonal (K = ph), so the shift by one cell per diagonal is a
+       renaming too; elsewhere the registers are shifted. */
+    /* (FP64 at three cells per lane: the third set costs more than the copies) */
+    constexpr bool ROT = (CPT == 2 || (CPT == 3 && sizeof(T) == 4)) && EK != 3;
+    auto step = [&](FSt &P, FSt &Q, FSt &Nw, auto ph) -> bool {
+        constexpr int K = decltype(ph)::value;
+        auto evr = [&](int c) -> ECon<T, EK> & { return ev[ROT ? ((c - K) % CPT + CPT) % CPT : c]; };
         int j = d - dc;
         const int lo = nlo, hi = nhi, fl = nfl;
         if (j == 64) {
@@ -1050,11 +1060,16 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
             bad = true; /* the window holds fewer positions than the band: the LDS sweeps redo it */
         }
         if (!fresh) {
-            /* cell x on d reads event d-x-1: its left neighbour's event of d-1 */
-            const ECon<T, EK> inc = shr1(ev[CPT - 1], pre);
+            /* cell x on d reads event d-x-1: its left neighbour's event of d-1 (the
+               register of the lane's last cell on d-1 becomes the first cell's) */
+            if constexpr (ROT) {
+                evr(0) = shr1(evr(0), pre);
+            } else {
+                const ECon<T, EK> inc = shr1(ev[CPT - 1], pre);
 #pragma unroll
-            for (int c = CPT - 1; c >= 1; c--) ev[c] = ev[c - 1];
-            ev[0] = inc;
+                for (int c = CPT - 1; c >= 1; c--) ev[c] = ev[c - 1];
+                ev[0] = inc;
+            }
         }
         fresh = false;
         {
@@ -1079,7 +1094,7 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
                 eM = hdp_finish(m, gp[PF ? c : 0], iv);
                 eY = eM;
             } else {
-                em_both<T>(m, kc[c], ev[c], s, iv, eM, eY);
+                em_both<T>(m, kc[c], evr(c), s, iv, eM, eY);
             }
             /* lower -> gapX, middle -> match, upper -> gapY */
             T X = lM + (eX + tGOX);
@@ -1090,12 +1105,12 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
             M = logadd_lut(M, mYv + (eM + tMFY), lut);
             T Y = P.M[c] + (eY + tGOY);
             Y = logadd_lut(Y, P.Y[c] + (eY + tGEY), lut);
-            Q.M[c] = valid ? M : NI;
-            Q.X[c] = valid ? X : NI;
-            Q.Y[c] = valid ? Y : NI;
+            Nw.M[c] = valid ? M : NI;
+            Nw.X[c] = valid ? X : NI;
+            Nw.Y[c] = valid ? Y : NI;
             if (renorm) {
-                T m3 = Q.M[c] > Q.X[c] ? Q.M[c] : Q.X[c];
-                m3 = m3 > Q.Y[c] ? m3 : Q.Y[c];
+                T m3 = Nw.M[c] > Nw.X[c] ? Nw.M[c] : Nw.X[c];
+                m3 = m3 > Nw.Y[c] ? m3 : Nw.Y[c];
                 mx = m3 > mx ? m3 : mx;
             }
         }
@@ -1108,7 +1123,7 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
             if (mx > NI) {
 #pragma unroll
                 for (int c = 0; c < CPT; c++) {
-                    Q.M[c] -= mx; Q.X[c] -= mx; Q.Y[c] -= mx;
+                    Nw.M[c] -= mx; Nw.X[c] -= mx; Nw.Y[c] -= mx;
                     P.M[c] -= mx; P.X[c] -= mx; P.Y[c] -= mx;
                 }
                 LM -= mx; LX -= mx; LY -= mx;
@@ -1119,10 +1134,10 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
             /* HDP: the density gathers of diagonal d+1 are issued at the end of diagonal
                d, before its stores, and used a diagonal later.  The events of d+1 are
                this diagonal's shifted one lane; the first lane's comes from `pre` */
-            const ECon<T, EK> inc = shr1(ev[CPT - 1], pre);
+            const ECon<T, EK> inc = shr1(evr(CPT - 1), pre);
 #pragma unroll
             for (int c = 0; c < CPT; c++) {
-                const T evm = c ? ev[c - 1].m : inc.m;
+                const T evm = c ? evr(c - 1).m : inc.m;
                 gp[c] = hdp_fetch(m, kc[c].row, kc[c].hmu, evm, s);
             }
             have_gp = true;
@@ -1141,8 +1156,8 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
         for (int c = 0; c < CPT; c++) {
             const int x = X0 + CPT * lane + c;
             if (x >= lo && x <= hi) {
-                fM[run + (x - lo)] = Q.M[c];
-                if (fl) { fX[runxy + (x - lo)] = Q.X[c]; fY[runxy + (x - lo)] = Q.Y[c]; }
+                fM[run + (x - lo)] = Nw.M[c];
+                if (fl) { fX[runxy + (x - lo)] = Nw.X[c]; fY[runxy + (x - lo)] = Nw.Y[c]; }
             }
         }
         if (lane == j) { md_doff = (int32_t)run; md_dxy = fl ? (int32_t)runxy : -1; md_foff = offd; }
@@ -1155,21 +1170,39 @@ __device__ __forceinline__ int fwd_segment(const Args<T> &a, const TaskDev &t, F
         d++;
         return true;
     };
-    FSt A, B; /* the states of d-1 and d-2 */
+    FSt A, B, C; /* the states of d-1 and d-2, and (three cells per lane, rotated events) of d */
 #pragma unroll
     for (int c = 0; c < CPT; c++) {
         A.M[c] = pM[c]; A.X[c] = pX[c]; A.Y[c] = pY[c];
         B.M[c] = qM[c]; B.X[c] = qX[c]; B.Y[c] = qY[c];
     }
-    bool swapped = false; /* d-1's states are in B */
-    while (d <= N) {
-        if (!step(A, B)) break;
-        swapped = true;
-        if (d > N || !step(B, A)) break;
-        swapped = false;
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    int role = 0; /* d-1's and d-2's states: 0 in A and B, 1 in B and A (C and A with the
+                     third set), 2 in B and C */
+    if constexpr (CPT == 3 && ROT) {
+        while (d <= N) {
+            role = 0;
+            if (!step(A, B, C, I0{})) break;
+            role = 1;
+            if (d > N || !step(C, A, B, I1{})) break;
+            role = 2;
+            if (d > N || !step(B, C, A, I2{})) break;
+            role = 0;
+        }
+    } else {
+        while (d <= N) {
+            role = 0;
+            if (!step(A, B, B, I0{})) break;
+            role = 1;
+            if (d > N || !step(B, A, A, I1{})) break;
+            role = 0;
+        }
     }
     if (next) {
-        const FSt &P = swapped ? B : A, &Q = swapped ? A : B;
+        const FSt &P = role == 0 ? A : CPT == 3 && ROT ? (role == 1 ? C : B) : B;
+        const FSt &Q = role == 0 ? B : CPT == 3 && ROT ? (role == 1 ? A : C) : A;
 #pragma unroll
         for (int c = 0; c < CPT; c++) {
             const int i = CPT * lane + c;
