@@ -1412,8 +1412,11 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
         for (int c = 0; c < CPT; c++) { sink(kc[c]); sink(ev[c]); }
     }
     /* the extra-event skip emission of k-mer x+1 for the lane's last cell (a segment
-       keeps its k-mers) */
-    const T eXn = em_gapx<T>(m, shl1_match(kc[0]));
+       keeps its k-mers).  FP64 Gaussian sweeps, which evaluate every match emission,
+       shift that k-mer in on every diagonal instead: one register more costs them
+       spills */
+    constexpr bool XN = EK >= 2 || sizeof(T) == 4;
+    const T eXn = XN ? em_gapx<T>(m, shl1_match(kc[0])) : (T)0;
     ECon<T, EK> pre;
     {
         const int jn = (es - 1) - (X0 + W - 1); /* event of lane 63's last cell on es-1 */
@@ -1539,6 +1542,7 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
             LowM = shl1(P.M[0], NI);
             LowX = shl1(P.X[0], NI);
             const T eYn = have_eY ? shl1(eYp[0], NI) : NI;
+            const KCon<T, EK> knd = XN ? kc[0] : shl1_match(kc[0]); /* (FP64 Gaussian only) */
             const bool renorm = NORM && ((D - e) & 7) == 0;
             T mx = NI;
 #pragma unroll
@@ -1547,11 +1551,11 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                 const bool valid = x >= lo && x <= hi;
                 /* k-mer x+1 (the lane's last cell: its right neighbour's first, shifted
                    in only where an emission is evaluated from it) */
-                auto kx = [&]() { return c < CPT - 1 ? kc[c < CPT - 1 ? c + 1 : 0] : shl1_match(kc[0]); };
+                auto kx = [&]() { return c < CPT - 1 ? kc[c < CPT - 1 ? c + 1 : 0] : XN ? shl1_match(kc[0]) : knd; };
                 const T ClowX = c < CPT - 1 ? P.X[c + 1] : LowX;
                 const T CupY = P.Y[c];
                 const T CmidM = c < CPT - 1 ? Q.M[c + 1] : RpM;
-                const T eX = c < CPT - 1 ? em_gapx<T>(m, kc[c < CPT - 1 ? c + 1 : 0]) : eXn;
+                const T eX = c < CPT - 1 ? em_gapx<T>(m, kc[c < CPT - 1 ? c + 1 : 0]) : XN ? eXn : em_gapx<T>(m, knd);
                 T eM, eY;
                 if constexpr (PL) {
                     eY = have_pe ? pe[PL ? c : 0] : plane_at(e + 1, x);
