@@ -387,6 +387,11 @@ int sa_server_create(int device, const sa_server_config *config, sa_server **out
  * the same options. */
 int sa_server_submit(sa_server *server, const sa_model *model, const sa_read *read, const sa_params *params,
                      sa_precision precision, int64_t *ticket);
+/* Queues n reads at once (copied on several threads), tickets[i] for reads[i]; the
+ * scheduler sees them together, as one burst.  No counterpart in the reference: for
+ * callers holding many reads.  A bad read fails the call before any is queued. */
+int sa_server_submit_batch(sa_server *server, const sa_model *model, const sa_read *reads, int64_t n,
+                           const sa_params *params, sa_precision precision, int64_t *tickets);
 /* Waits up to timeout_s (0: poll) for a request: SA_OK with its result (valid until
  * sa_server_release), SA_PENDING, or the request's error status. */
 int sa_server_wait(sa_server *server, int64_t ticket, double timeout_s, sa_read_result *out,

@@ -623,13 +623,32 @@ int batch_fill(sa_context *ctx, sa_batch *b, const sa_model *model, const sa_rea
     SA_HIP(ctx, ctx->stage.ensure(ref_pad + ev_bytes));
     char *h_ref = ctx->stage.as<char>();
     double *h_ev = reinterpret_cast<double *>(h_ref + ref_pad);
-    int64_t ro = 0, eo = 0;
-    for (int64_t r = 0; r < n_reads; r++) {
-        if (reads[r].ref_length) std::memcpy(h_ref + ro, reads[r].ref, (size_t)reads[r].ref_length);
-        if (reads[r].n_events)
-            std::memcpy(h_ev + eo * stride, reads[r].events, sizeof(double) * (size_t)(reads[r].n_events * stride));
-        ro += reads[r].ref_length;
-        eo += reads[r].n_events;
+    {
+        /* a large batch (a serving wave, a bench batch) is packed by several threads,
+           whole reads each: one thread copies a few GB/s into pinned memory */
+        std::vector<int64_t> ro(n_reads + 1, 0), eo(n_reads + 1, 0);
+        for (int64_t r = 0; r < n_reads; r++) {
+            ro[r + 1] = ro[r] + reads[r].ref_length;
+            eo[r + 1] = eo[r] + reads[r].n_events;
+        }
+        auto pack = [&](int64_t r0, int64_t r1) {
+            for (int64_t r = r0; r < r1; r++) {
+                if (reads[r].ref_length) std::memcpy(h_ref + ro[r], reads[r].ref, (size_t)reads[r].ref_length);
+                if (reads[r].n_events)
+                    std::memcpy(h_ev + eo[r] * stride, reads[r].events,
+                                sizeof(double) * (size_t)(reads[r].n_events * stride));
+            }
+        };
+        const int64_t bytes = ro[n_reads] + (int64_t)sizeof(double) * eo[n_reads] * stride;
+        const int nth = (int)std::min<int64_t>(std::min<int64_t>(8, n_reads), bytes >> 23); /* 8 MB a thread */
+        if (nth <= 1) {
+            pack(0, n_reads);
+        } else {
+            std::vector<std::thread> th;
+            for (int i = 1; i < nth; i++) th.emplace_back(pack, n_reads * i / nth, n_reads * (i + 1) / nth);
+            pack(0, n_reads / nth);
+            for (auto &x : th) x.join();
+        }
     }
     if (b->paths) b->href.assign(h_ref, (size_t)ref_off);
     SA_HIP(ctx, b->d_ref.ensure(ref_bytes));

@@ -380,22 +380,28 @@ int sa_server_create(int device, const sa_server_config *cfg, sa_server **out) {
     return SA_OK;
 }
 
-int sa_server_submit(sa_server *s, const sa_model *model, const sa_read *read, const sa_params *params,
-                     sa_precision precision, int64_t *ticket) {
-    if (!s || !model || !read || !ticket || (precision != SA_FP32 && precision != SA_FP64))
-        return SA_ERR_INVALID_ARGUMENT;
-    if (read->ref_length < 0 || read->n_events < 0 || read->n_anchors < 0 || read->event_stride < 2)
-        return SA_ERR_INVALID_ARGUMENT;
+namespace {
+
+/* A request's own copy of a read (the sweeps read an event's mean and noise only: those
+   two are kept), or nullptr with *rc set for a read the server refuses. */
+Request *make_request(const sa_model *model, const sa_read *read, const sa_params *params, sa_precision precision,
+                      int *rc) {
+    if (read->ref_length < 0 || read->n_events < 0 || read->n_anchors < 0 || read->event_stride < 2) {
+        *rc = SA_ERR_INVALID_ARGUMENT;
+        return nullptr;
+    }
     /* degenerate references (FP64 only) run in waves of their own options */
     const bool degen = read->degenerate_options && read->degenerate_options[0] && read->ref_length > 0 &&
                        memchr(read->ref, 'X', (size_t)read->ref_length);
-    if (degen && precision != SA_FP64) return SA_ERR_UNSUPPORTED;
+    if (degen && precision != SA_FP64) {
+        *rc = SA_ERR_UNSUPPORTED;
+        return nullptr;
+    }
     auto *q = new Request();
     q->model = model;
     if (params) q->params = *params; else sa_params_default(&q->params);
     q->precision = precision;
     q->ref.assign(read->ref, (size_t)read->ref_length);
-    /* the sweeps read an event's mean and noise only: keep those two */
     q->n_events = read->n_events;
     q->stride = 2;
     q->events.resize((size_t)(2 * read->n_events));
@@ -413,13 +419,63 @@ int sa_server_submit(sa_server *s, const sa_model *model, const sa_read *read, c
     const int k = sa_model_kmer_length(model);
     q->est_bytes = estimate_bytes(read, k, &q->params, precision,
                                   degen ? mean_paths(read, k, (int64_t)q->options.size()) : 0.0, model->ek == 2);
-    q->n_diag = std::max<int64_t>(read->ref_length - sa_model_kmer_length(model) + 1, 0) + read->n_events;
+    q->n_diag = std::max<int64_t>(read->ref_length - k + 1, 0) + read->n_events;
+    *rc = SA_OK;
+    return q;
+}
+
+} // namespace
+
+int sa_server_submit(sa_server *s, const sa_model *model, const sa_read *read, const sa_params *params,
+                     sa_precision precision, int64_t *ticket) {
+    if (!s || !model || !read || !ticket || (precision != SA_FP32 && precision != SA_FP64))
+        return SA_ERR_INVALID_ARGUMENT;
+    int rc;
+    Request *q = make_request(model, read, params, precision, &rc);
+    if (!q) return rc;
     std::lock_guard<std::mutex> lk(s->mu);
     q->ticket = s->next_ticket++;
     q->timing.submit_s = s->now();
     s->reqs[q->ticket] = q;
     s->pending.push_back(q);
     *ticket = q->ticket;
+    s->cv_sched.notify_all();
+    return SA_OK;
+}
+
+int sa_server_submit_batch(sa_server *s, const sa_model *model, const sa_read *reads, int64_t n,
+                           const sa_params *params, sa_precision precision, int64_t *tickets) {
+    if (!s || !model || (!reads && n > 0) || (!tickets && n > 0) || n < 0 ||
+        (precision != SA_FP32 && precision != SA_FP64))
+        return SA_ERR_INVALID_ARGUMENT;
+    std::vector<Request *> qs((size_t)n, nullptr);
+    std::vector<int> rcs((size_t)n, SA_OK);
+    /* the copies on up to 8 threads, whole reads each */
+    const int nth = (int)std::max<int64_t>(1, std::min<int64_t>(8, n / 64));
+    auto work = [&](int64_t i0, int64_t i1) {
+        for (int64_t i = i0; i < i1; i++) qs[i] = make_request(model, &reads[i], params, precision, &rcs[i]);
+    };
+    {
+        std::vector<std::thread> th;
+        for (int i = 1; i < nth; i++) th.emplace_back(work, n * i / nth, n * (i + 1) / nth);
+        work(0, n / nth);
+        for (auto &x : th) x.join();
+    }
+    for (int64_t i = 0; i < n; i++)
+        if (!qs[i]) {
+            for (Request *q : qs) delete q;
+            return rcs[i];
+        }
+    std::lock_guard<std::mutex> lk(s->mu);
+    const double t = s->now();
+    for (int64_t i = 0; i < n; i++) {
+        Request *q = qs[i];
+        q->ticket = s->next_ticket++;
+        q->timing.submit_s = t;
+        s->reqs[q->ticket] = q;
+        s->pending.push_back(q);
+        tickets[i] = q->ticket;
+    }
     s->cv_sched.notify_all();
     return SA_OK;
 }

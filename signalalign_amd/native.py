@@ -147,6 +147,8 @@ def load():
     L.sa_server_create.argtypes = [ctypes.c_int, ctypes.POINTER(_ServerConfig), ctypes.POINTER(vp)]
     L.sa_server_submit.argtypes = [vp, vp, ctypes.POINTER(_Read), ctypes.POINTER(_Params), ctypes.c_int,
                                    ctypes.POINTER(ctypes.c_int64)]
+    L.sa_server_submit_batch.argtypes = [vp, vp, ctypes.POINTER(_Read), ctypes.c_int64, ctypes.POINTER(_Params),
+                                         ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
     L.sa_server_wait.argtypes = [vp, ctypes.c_int64, ctypes.c_double, ctypes.POINTER(_ReadResult),
                                  ctypes.POINTER(_ReqTiming)]
     L.sa_server_release.argtypes = [vp, ctypes.c_int64]
@@ -536,6 +538,16 @@ class Server:
         if rc != 0:
             raise SignalAlignError(rc, "sa_server_submit")
         return t.value
+
+    def submit_structs(self, model, read_structs, params_struct, precision=SA_FP32):
+        """Queue many reads at once (sa_server_submit_batch): one ticket per read."""
+        n = len(read_structs)
+        arr = (_Read * n)(*read_structs)
+        t = (ctypes.c_int64 * max(n, 1))()
+        rc = self._L.sa_server_submit_batch(self._h, model._h, arr, n, ctypes.byref(params_struct), precision, t)
+        if rc != 0:
+            raise SignalAlignError(rc, "sa_server_submit_batch")
+        return list(t[:n])
 
     def submit(self, model, read, params=None, precision=SA_FP32):
         return self.submit_struct(model, read._struct(), (params or Params())._struct(), precision)

@@ -75,6 +75,36 @@ def test_alone_equals_mixed(models_):
         srv.close()
 
 
+def test_batch_submission_equals_single(models_):
+    """Reads queued by one sa_server_submit_batch call give the results they give
+    submitted one by one (FP32, exact equality); a refused read in the batch fails the
+    call and queues nothing."""
+    g6, hd, dg, dh = models_
+    params = native.Params(diagonal_expansion=50)
+    rs = _reads(g6, [600 + 173 * i for i in range(150)], seed=306)
+    srv = native.Server(0, lanes=2, max_wave_reads=64)
+    try:
+        one = [r for r, _ in _collect(srv, [srv.submit(dg, P.native_read(r), params, native.SA_FP32) for r in rs])]
+        objs = [P.native_read(r) for r in rs]
+        tickets = srv.submit_structs(dg, [o._struct() for o in objs], params._struct(), native.SA_FP32)
+        assert len(tickets) == len(rs) and len(set(tickets)) == len(rs)
+        many = _collect(srv, tickets)
+        assert max(tm["wave_reads"] for _, tm in many) > 1
+        for a, (b, _) in zip(one, many):
+            assert np.array_equal(a.pairs, b.pairs) and a.log_likelihood == b.log_likelihood
+        st0 = srv.stats()
+        bad = native.Read(rs[0]["ref"], rs[0]["events"], rs[0]["anchors"], rs[0]["scale"], rs[0]["shift"],
+                          rs[0]["var"])
+        bs = bad._struct()
+        bs.event_stride = 1  # refused: fewer than two values per event
+        with pytest.raises(native.SignalAlignError) as e:
+            srv.submit_structs(dg, [objs[1]._struct(), bs], params._struct(), native.SA_FP32)
+        assert e.value.code == -1
+        assert srv.stats()["pending"] == 0 and srv.stats()["reads"] == st0["reads"]
+    finally:
+        srv.close()
+
+
 def test_served_results_match_oracle(models_):
     """FP64 requests through the server against the FP64 oracle (the drop-in bar)."""
     g6, hd, dg, dh = models_
