@@ -92,7 +92,11 @@ struct HBuf {
    again neither copies its pairs a second time on the host nor pins new memory.
    Results hold the pool by shared_ptr: one may outlive its context. */
 struct PinnedPool {
-    static constexpr size_t kKeep = 2; /* free buffers kept; more are unpinned */
+    /* free buffers kept: at least kKeep, up to kKeepMax while they total at most
+       kKeepBytes (a serving lane's waves finish while earlier waves' results are still
+       held, and pinning or unpinning host memory waits for the whole device) */
+    static constexpr size_t kKeep = 2, kKeepMax = 8;
+    static constexpr size_t kKeepBytes = (size_t)2 << 30;
     std::mutex mu;
     std::vector<HBuf> idle;
     HBuf take() {
@@ -109,7 +113,10 @@ struct PinnedPool {
         if (!h.p) return;
         std::lock_guard<std::mutex> lk(mu);
         idle.push_back(h);
-        if (idle.size() > kKeep) {
+        for (;;) {
+            size_t total = 0;
+            for (const HBuf &b : idle) total += b.cap;
+            if (idle.size() <= kKeep || (idle.size() <= kKeepMax && total <= kKeepBytes)) break;
             auto it = std::min_element(idle.begin(), idle.end(),
                                        [](const HBuf &a, const HBuf &b) { return a.cap < b.cap; });
             it->release();
