@@ -765,11 +765,11 @@ int sa_batch_autotune(sa_batch *b, sa_precision precision, sa_tuning *best, doub
     double cur_ms = 0.0;
     int rc = time_it(cur, &cur_ms);
     if (rc) return rc;
-    const double lf[] = {2.5, 2.0, 1.5, 4.0, -1.0};
-    const double sp[] = {0.5, 0.4, 0.6, 0.3, 0.7};
+    const double lf[] = {2.5, 2.0, 3.0, 1.5, 4.0, -1.0};
+    const double sp[] = {0.5, 0.55, 0.6, 0.65, 0.4, 0.7, 0.3};
     const int wp[] = {0, 2, 3, 4};
     for (int knob = 0; knob < 3; knob++) {
-        const int n = knob == 2 ? 4 : 5;
+        const int n = knob == 0 ? 6 : knob == 1 ? 7 : 4;
         for (int i = 0; i < n; i++) {
             sa_tuning t = cur;
             if (knob == 0) t.long_factor = lf[i];
