@@ -1763,7 +1763,9 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                 fpre[c] = v ? fM[ndo + (x - nlo)] : __builtin_nanf(""); /* NaN: see the posterior test */
             }
             have_fpre = true;
-            if (posterior_count % 10 == 0) { /* e-1 refreshes the total */
+            /* e-1 refreshes the total (FP64: its 5·CPT states are loaded there instead; held
+               across the diagonal they cost spills) */
+            if (sizeof(T) == 4 && posterior_count % 10 == 0) {
                 load_refresh(e - 1);
                 have_rpre = true;
             }
