@@ -52,6 +52,18 @@ template <typename T> struct KCon<T, 2> {
 
 template <typename T> struct KCon<T, 3> {};
 
+/* Element i of one sub-alignment's plane, addressed as the plane's base (uniform, in
+   scalar registers) plus a 32-bit byte offset in one vector register rather than a
+   64-bit vector sum (the backward sweep's loads; the forward sweeps, short of scalar
+   registers, keep 64-bit addresses); the runtime gives the register sweeps only
+   sub-alignments whose planes stay under 4 GB. */
+template <typename T> __device__ __forceinline__ T &pl(T *base, int64_t i) {
+    return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (uint32_t)i * (uint32_t)sizeof(T));
+}
+template <typename T> __device__ __forceinline__ const T &pl(const T *base, int64_t i) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + (uint32_t)i * (uint32_t)sizeof(T));
+}
+
 /* the three states of one diagonal's cells of a lane (forward sweeps) */
 template <typename T, int CPT> struct FStates { T M[CPT], X[CPT], Y[CPT]; };
 
@@ -1455,7 +1467,7 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
     auto plane_at = [&](int dd, int xx) -> T {
         if (dd > lX + lY) return (T)0;
         const int l = ld_scalar(xlo, dd), h = ld_scalar(xhi, dd), o = ld_scalar(doff, dd);
-        return xx >= l && xx <= h ? em[o + (xx - l)] : (T)0;
+        return xx >= l && xx <= h ? pl(em, o + (xx - l)) : (T)0;
     };
     /* the forward match states of the next diagonal's cells for its posteriors, loaded
        a diagonal ahead (they would otherwise stall every diagonal on HBM latency) */
@@ -1475,14 +1487,14 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
         for (int c = 0; c < CPT; c++) {
             const int x = X0 + CPT * lane + c;
             const bool in = x >= lor && x <= hir;
-            rfx[c] = in ? fX[dxy + (x - lor)] : NI;
-            rfy[c] = in ? fY[dxy + (x - lor)] : NI;
+            rfx[c] = in ? pl(fX, dxy + (x - lor)) : NI;
+            rfy[c] = in ? pl(fY, dxy + (x - lor)) : NI;
             /* cell (x, r+1-x) of r+1; its middle (x-1, r-x) on r-1 */
             const bool hm = x >= lo1 && x <= hi1r && x - 1 >= lom && x - 1 <= him && x >= 1 && r + 1 - x >= 1;
             const int fi = x - 1 - lom;
-            rgm[c] = hm ? fM[dm + fi] : NI;
-            rgx[c] = hm ? fX[dxym + fi] : NI;
-            rgy[c] = hm ? fY[dxym + fi] : NI;
+            rgm[c] = hm ? pl(fM, dm + fi) : NI;
+            rgx[c] = hm ? pl(fX, dxym + fi) : NI;
+            rgy[c] = hm ? pl(fY, dxym + fi) : NI;
         }
     };
 
@@ -1631,7 +1643,7 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
 #pragma unroll
             for (int c = 0; c < CPT; c++) {
                 const int x = X0 + CPT * lane + c;
-                pe[c] = x >= lo && x <= hi ? em[dof + (x - lo)] : (T)0;
+                pe[c] = x >= lo && x <= hi ? pl(em, dof + (x - lo)) : (T)0;
             }
             have_pe = true;
         }
@@ -1649,7 +1661,7 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                     tA[c] = -__builtin_huge_val();
                     tB[c] = -__builtin_huge_val();
                     if (x >= lo && x <= hi) {
-                        T cell = (have_fpre ? fpre[c] : fM[dof + (x - lo)]) + Q.M[c];
+                        T cell = (have_fpre ? fpre[c] : pl(fM, dof + (x - lo))) + Q.M[c];
                         cell = logadd_lut(cell, rfx[c] + Q.X[c], lut);
                         cell = logadd_lut(cell, rfy[c] + Q.Y[c], lut);
                         tA[c] = (double)cell + offe;
@@ -1712,7 +1724,7 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
                     cand[c] = false;
                     sv[c] = NI;
                     if (x >= lo && x <= hi && x > 0 && y > 0) {
-                        const T fm = have_fpre ? fpre[c] : fM[dof + (x - lo)];
+                        const T fm = have_fpre ? fpre[c] : pl(fM, dof + (x - lo));
                         sv[c] = NORM ? (fm + Q.M[c]) + cst : (fm + Q.M[c]) - (T)total;
                         cand[c] = sv[c] >= lthr;
                     }
@@ -1760,7 +1772,7 @@ __device__ __forceinline__ int bwd_segment(const Args<T> &a, const int wi, const
             for (int c = 0; c < CPT; c++) {
                 const int x = X0 + CPT * lane + c;
                 const bool v = x >= nlo && x <= nhi;
-                fpre[c] = v ? fM[ndo + (x - nlo)] : __builtin_nanf(""); /* NaN: see the posterior test */
+                fpre[c] = v ? pl(fM, ndo + (x - nlo)) : __builtin_nanf(""); /* NaN: see the posterior test */
             }
             have_fpre = true;
             /* e-1 refreshes the total (FP64: its 5·CPT states are loaded there instead; held

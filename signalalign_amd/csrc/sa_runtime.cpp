@@ -1004,7 +1004,9 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     std::vector<int> fgrp(nt), bgrp(nt);
     for (int64_t i = 0; i < nt; i++) {
         int f = 0, bb = 0;
-        if (!force_lds) {
+        /* the register sweeps address a sub-alignment's planes with 32-bit byte offsets:
+           one whose planes reach 4 GB runs on the LDS sweeps */
+        if (!force_lds && (uint64_t)cells[i] * sizeof(T) < ((uint64_t)1 << 32)) {
             const int c1 = cpt1(span[i]), mw = mw_shape(span[i]);
             if (c1) {
                 bb = 1 + (int)(std::find(kCpt, kCpt + 6, c1) - kCpt);
