@@ -176,7 +176,6 @@ struct sa_model {
     double inv_dx = 0.0;
     double grid0 = 0.0, grid1 = 0.0, gridn = 0.0; /* first, second and last grid knots */
     int32_t ek = 1;          /* register-kernel emission variant: 0 standard Gaussian, 1 any Gaussian, 2 HDP */
-    std::vector<int32_t> hrow_host; /* HDP: density row of each k-mer (the emission plane groups by it) */
     int32_t n_rows = 0;
     double gx_const = -2.3025850929940455;
     DBuf d_lut;

@@ -197,6 +197,21 @@ constexpr int kColumnGroup = 64;
 template <typename T>
 hipError_t launch_hdp_plane(const Args<T> &a, const int32_t *entries, const int32_t *groups, int n_groups,
                             const int32_t *ctask, const int32_t *chunk0, int c0, int c1, int n_cu, hipStream_t st);
+/* The column entries of the plane's sub-alignments, grouped on the device: one
+   PlaneTask per sub-alignment (in stream-lane order, lane 0-3), whose columns 0..lX
+   take the flat indices col0..col0+lX.  Writes the entries (task, x int32 pairs)
+   stably sorted by key lane * (nrow + 1) + density row + 1, and the runs of equal
+   keys (key, count; their number in *n_runs) in ascending key order. */
+constexpr int kPlaneLanes = 4;
+struct PlaneTask {
+    int64_t pos_off, col0;
+    int32_t task, lane, lX, pad;
+};
+size_t plane_sort_temp_bytes(int64_t n_cols, int32_t nrow); /* 0: the sort cannot be sized */
+hipError_t launch_plane_sort(const PlaneTask *pt, int n_tasks, int64_t n_cols, const int32_t *kid,
+                             const int32_t *hrow, int32_t nrow, uint32_t *keys, uint32_t *keys_sorted,
+                             uint64_t *vals, uint64_t *entries, uint32_t *run_keys, int32_t *run_counts,
+                             int32_t *n_runs, void *temp, size_t temp_bytes, hipStream_t st);
 /* zeroes n ints (a kernel, so that it can sit in a captured graph) */
 hipError_t launch_zero(int32_t *p, int n, hipStream_t st);
 hipError_t launch_collect(const WinDev *wins, const int32_t *win_count, const int64_t *win_dst, const int32_t *src,

@@ -73,6 +73,7 @@ struct sa_batch {
        per-position column ranges and offsets, its (task, column) entries grouped by
        density row and the groups */
     DBuf d_em, d_em_tasks, d_em_chunk0, d_col_a, d_col_b, d_col_off, d_em_ent, d_em_grp;
+    DBuf d_em_pt, d_pk, d_pv, d_runs, d_sort_tmp; /* the device grouping: tasks, keys, values, runs, sort scratch */
     /* degenerate references ('X' with options): multi-path sweeps, FP64 */
     int32_t paths = 0;
     std::string options;      /* the bases an 'X' stands for */
@@ -112,7 +113,7 @@ struct sa_batch {
                        &d_win_order, &d_out, &d_win_count, &d_win_total, &d_win_dst, &d_packed, &d_ring_g,
                        &d_pq, &d_first_win,
                        &d_em, &d_em_tasks, &d_em_chunk0, &d_col_a, &d_col_b, &d_col_off, &d_em_ent, &d_em_grp,
-                       &d_ex_trans, &d_ex_s0, &d_ex_s1, &d_ex_obs, &d_ex_rec, &d_ex_count,
+                       &d_em_pt, &d_pk, &d_pv, &d_runs, &d_sort_tmp, &d_ex_trans, &d_ex_s0, &d_ex_s1, &d_ex_obs, &d_ex_rec, &d_ex_count,
                        &d_pcol, &d_pent, &d_pplane, &d_pp, &d_psa, &d_psb, &d_pecol, &d_pekid, &d_pdoff,
                        &d_pdiag, &d_pentr, &d_pmaxent};
         for (DBuf *b : all) b->release();
@@ -457,8 +458,6 @@ int sa_model_create(sa_context *ctx, const sa_model_desc *d, sa_model **out) {
         m->ek = std_tables ? 0 : 1;
     }
     if (d->kind == 2) {
-        m->hrow_host.assign(m->n_kmers, -1);
-        for (int64_t i = 0; i < m->n_kmers; i++) m->hrow_host[i] = (int32_t)d->kmer_row[i];
         m->n_rows = (int32_t)d->n_rows;
     }
     std::vector<int8_t> lut(m->lut, m->lut + 256);
@@ -1147,71 +1146,91 @@ int run_batch(sa_batch *b, sa_result *res, bool force_lds) {
     SA_HIP(ctx, b->d_fY.ensure(sizeof(T) * std::max<int64_t>(total_xy, 1)));
     int rc;
     /* HDP emission plane over the sub-alignments of the register sweeps: their columns
-       grouped by density row for the column pass (k-mer ids read back from prep),
+       grouped by density row for the column pass (sorted on the device by the rows of
+       the k-mer ids prep wrote),
        their diagonals numbered in chunks of kPlaneDiags for the transpose.  The column
        plane lives in the forward match plane, which the sweeps fill only after it. */
     /* Per stream lane (each lane's stream builds the plane of its own sub-alignments
        before its sweeps, so the long sweeps start at once and the plane of the others
        overlaps them): column groups and chunks of each lane, in lane order. */
-    std::vector<int32_t> em_ctask, em_chunk0(nt, 0), em_ent, em_grp;
+    std::vector<int32_t> em_ctask, em_chunk0(nt, 0), em_grp;
     int64_t em_g[kLanes + 1] = {0, 0, 0, 0, 0}, em_c[kLanes + 1] = {0, 0, 0, 0, 0};
     int n_em = 0;
     if (a.em_plane) {
         for (int64_t i = 0; i < nt; i++) n_em += fgrp[i] != 0;
     }
     if (n_em) {
-        std::vector<int32_t> kid(b->total_pos);
-        if (b->total_pos) {
-            SA_HIP(ctx, hipMemcpyAsync(kid.data(), b->d_kid.p, sizeof(int32_t) * b->total_pos, hipMemcpyDeviceToHost,
-                                       st));
-            SA_HIP(ctx, hipStreamSynchronize(st));
-        }
-        const int32_t nrow = model->n_rows;
-        auto row_of = [&](int64_t i, int32_t x) -> int32_t { /* -1: no k-mer */
-            if (x == 0) return -1;
-            const int32_t r = model->hrow_host[kid[b->tasks[i].pos_off + x - 1]];
-            return r >= 0 && r < nrow ? r : -1;
-        };
-        int64_t n_ent = 0;
+        /* the columns of each lane's sub-alignments, in lane order: chunks for the
+           transpose here, the grouping by density row on the device */
+        std::vector<sa::PlaneTask> pts;
+        int64_t n_cols = 0;
         for (int lane = 0; lane < kLanes; lane++) {
-            std::vector<int32_t> lt;
-            for (int64_t i = 0; i < nt; i++)
-                if (fgrp[i] != 0 && tlane[i] == lane) lt.push_back((int32_t)i);
-            for (int32_t i : lt) {
+            for (int64_t i = 0; i < nt; i++) {
+                if (fgrp[i] == 0 || tlane[i] != lane) continue;
                 em_chunk0[i] = (int32_t)em_ctask.size();
                 const int64_t nc = (nd[i] + 1 + sa::kPlaneDiags - 1) / sa::kPlaneDiags;
                 if ((int64_t)em_ctask.size() + nc > INT32_MAX)
                     return fail(ctx, SA_ERR_UNSUPPORTED, "emission plane: too many diagonals");
-                em_ctask.insert(em_ctask.end(), (size_t)nc, i);
+                em_ctask.insert(em_ctask.end(), (size_t)nc, (int32_t)i);
+                sa::PlaneTask q{};
+                q.pos_off = b->tasks[i].pos_off;
+                q.col0 = n_cols;
+                q.task = (int32_t)i;
+                q.lane = lane;
+                q.lX = b->tasks[i].lX;
+                pts.push_back(q);
+                n_cols += (int64_t)b->tasks[i].lX + 1;
             }
             em_c[lane + 1] = (int64_t)em_ctask.size();
-            /* the lane's columns, grouped by density row (counting sort) */
-            std::vector<int64_t> cnt((size_t)nrow + 2, 0);
-            for (int32_t i : lt)
-                for (int32_t x = 0; x <= b->tasks[i].lX; x++) cnt[(size_t)row_of(i, x) + 2]++;
-            for (size_t r = 1; r < cnt.size(); r++) cnt[r] += cnt[r - 1]; /* cnt[r + 1]: first entry of row r */
-            const int64_t le = cnt.back();
-            if (2 * (n_ent + le) > INT32_MAX) return fail(ctx, SA_ERR_UNSUPPORTED, "emission plane: too many columns");
-            em_ent.resize((size_t)(2 * (n_ent + le)));
-            std::vector<int64_t> cur(cnt.begin(), cnt.end() - 1);
-            for (int32_t i : lt)
-                for (int32_t x = 0; x <= b->tasks[i].lX; x++) {
-                    const int64_t q = n_ent + cur[(size_t)row_of(i, x) + 1]++;
-                    em_ent[(size_t)(2 * q)] = i;
-                    em_ent[(size_t)(2 * q + 1)] = x;
-                }
-            for (int32_t r = -1; r < nrow; r++)
-                for (int64_t e = cnt[(size_t)r + 1]; e < cnt[(size_t)r + 2]; e += sa::kColumnGroup) {
-                    em_grp.push_back(r);
-                    em_grp.push_back((int32_t)(n_ent + e));
-                    em_grp.push_back((int32_t)std::min<int64_t>(sa::kColumnGroup, cnt[(size_t)r + 2] - e));
-                }
-            em_g[lane + 1] = (int64_t)(em_grp.size() / 3);
-            n_ent += le;
         }
+        if (2 * n_cols > INT32_MAX) return fail(ctx, SA_ERR_UNSUPPORTED, "emission plane: too many columns");
+        const int32_t nrow = model->n_rows;
+        const size_t nkey = (size_t)sa::kPlaneLanes * (size_t)(nrow + 1);
+        const size_t tmp = sa::plane_sort_temp_bytes(n_cols, nrow);
+        if (!tmp) return fail(ctx, SA_ERR_HIP, "emission plane: column sort");
+        SA_HIP(ctx, b->d_pk.ensure(sizeof(uint32_t) * 2 * (size_t)n_cols));
+        SA_HIP(ctx, b->d_pv.ensure(sizeof(uint64_t) * (size_t)n_cols));
+        SA_HIP(ctx, b->d_em_ent.ensure(sizeof(uint64_t) * (size_t)n_cols));
+        SA_HIP(ctx, b->d_runs.ensure(sizeof(int32_t) * (2 * nkey + 1)));
+        SA_HIP(ctx, b->d_sort_tmp.ensure(tmp));
+        if ((rc = upload(ctx, b->d_em_pt, pts))) return rc;
+        uint32_t *keys = b->d_pk.as<uint32_t>();
+        int32_t *runs = b->d_runs.as<int32_t>();
+        SA_HIP(ctx, sa::launch_plane_sort(b->d_em_pt.as<sa::PlaneTask>(), (int)pts.size(), n_cols, b->d_kid.as<int32_t>(),
+                                          a.m.hrow, nrow, keys, keys + n_cols,
+                                          b->d_pv.as<uint64_t>(), b->d_em_ent.as<uint64_t>(), (uint32_t *)runs,
+                                          runs + nkey, runs + 2 * nkey, b->d_sort_tmp.p, tmp, st));
+        int32_t nr = 0;
+        SA_HIP(ctx, hipMemcpyAsync(&nr, runs + 2 * nkey, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        SA_HIP(ctx, hipStreamSynchronize(st));
+        if (nr < 0 || (size_t)nr > nkey) return fail(ctx, SA_ERR_HIP, "emission plane: column sort");
+        std::vector<int32_t> rk((size_t)nr), rn((size_t)nr);
+        if (nr) {
+            SA_HIP(ctx, hipMemcpyAsync(rk.data(), runs, sizeof(int32_t) * nr, hipMemcpyDeviceToHost, st));
+            SA_HIP(ctx, hipMemcpyAsync(rn.data(), runs + nkey, sizeof(int32_t) * nr, hipMemcpyDeviceToHost, st));
+            SA_HIP(ctx, hipStreamSynchronize(st));
+        }
+        /* groups of at most kColumnGroup columns of one (lane, row) key; keys ascend,
+           so each lane's groups are contiguous and in row order */
+        int64_t e0 = 0;
+        int lane_done = 0;
+        for (int32_t k = 0; k < nr; k++) {
+            const int lane = (int)((uint32_t)rk[k] / (uint32_t)(nrow + 1));
+            const int32_t r = (int32_t)((uint32_t)rk[k] % (uint32_t)(nrow + 1)) - 1;
+            if (lane >= kLanes || rn[k] <= 0) return fail(ctx, SA_ERR_HIP, "emission plane: column sort");
+            for (; lane_done < lane; lane_done++) em_g[lane_done + 1] = (int64_t)(em_grp.size() / 3);
+            for (int64_t e = 0; e < rn[k]; e += sa::kColumnGroup) {
+                em_grp.push_back(r);
+                em_grp.push_back((int32_t)(e0 + e));
+                em_grp.push_back((int32_t)std::min<int64_t>(sa::kColumnGroup, rn[k] - e));
+            }
+            e0 += rn[k];
+        }
+        for (; lane_done < kLanes; lane_done++) em_g[lane_done + 1] = (int64_t)(em_grp.size() / 3);
+        if (e0 != n_cols) return fail(ctx, SA_ERR_HIP, "emission plane: column sort");
         SA_HIP(ctx, b->d_em.ensure(sizeof(T) * std::max<int64_t>(total_cells, 1)));
         if ((rc = upload(ctx, b->d_em_tasks, em_ctask)) || (rc = upload(ctx, b->d_em_chunk0, em_chunk0)) ||
-            (rc = upload(ctx, b->d_em_ent, em_ent)) || (rc = upload(ctx, b->d_em_grp, em_grp)))
+            (rc = upload(ctx, b->d_em_grp, em_grp)))
             return rc;
         a.em = b->d_em.as<T>();
         a.emc = b->d_fM.as<T>();
