@@ -2,9 +2,10 @@
  * sa_serve.cpp -- request loop with continuous batching on one device.
  *
  * Clients submit single reads (inputs are copied); a scheduler thread forms
- * "waves" from the pending requests -- same model, parameters and precision, in
- * arrival order -- admitting each read only while the band-matrix bytes of all
- * waves in flight stay within the HBM budget.  Each wave runs on one of several
+ * "waves" from the pending requests -- same model, parameters, precision and length
+ * class, in arrival order, the class set by the longest request that arrived within
+ * the batching delay of the oldest -- admitting each read only while the band-matrix
+ * bytes of all waves in flight stay within the HBM budget.  Each wave runs on one of several
  * lanes (a context with one stream and a batch whose device buffers are reused),
  * so waves overlap on the device: a request that arrives while others run is
  * admitted as soon as a lane and memory free up.  Model tables stay resident.
@@ -101,7 +102,11 @@ void sa_server::schedule_loop() {
     std::unique_lock<std::mutex> lk(mu);
     for (;;) {
         Lane *idle = nullptr;
+        Request *first = nullptr;
         double due = -1.0;
+        auto fits_now = [&](const Request *q) {
+            return inflight_bytes == 0 || inflight_bytes + q->est_bytes <= cfg.hbm_budget_bytes;
+        };
         auto ready = [&] {
             due = -1.0;
             if (stop) return true;
@@ -113,10 +118,21 @@ void sa_server::schedule_loop() {
                 else others = true;
             }
             if (!idle) return false;
-            /* memory: the head request must fit beside the waves in flight (alone, it is
+            /* the wave's first request sets its model and length class: the longest of
+               the requests that arrived within the batching delay of the oldest (a burst
+               starts its long waves first and fills in behind them with short ones; the
+               oldest waits at most one wave per length class among them) */
+            first = pending.front();
+            const double win = first->timing.submit_s + 1e-3 * std::max(cfg.batch_delay_ms, 0.0);
+            Request *lp = first;
+            for (Request *q : pending) {
+                if (q->timing.submit_s > win) break;
+                if (q->n_diag > lp->n_diag) lp = q;
+            }
+            /* memory: the first request must fit beside the waves in flight (alone, it is
                admitted once nothing else is in flight) */
-            const int64_t head = pending.front()->est_bytes;
-            if (!(inflight_bytes == 0 || inflight_bytes + head <= cfg.hbm_budget_bytes)) return false;
+            if (fits_now(lp)) first = lp;
+            if (!fits_now(first)) return false;
             /* continuous batching: an idle device takes the wave at once; a busy one lets
                it gather requests until the oldest has waited batch_delay_ms */
             if (others && cfg.batch_delay_ms > 0.0 && (int64_t)pending.size() < cfg.max_wave_reads) {
@@ -136,14 +152,13 @@ void sa_server::schedule_loop() {
                 cv_sched.wait(lk);
         }
         if (stop) return;
-        Request *first = pending.front();
         std::vector<Request *> wave;
         int64_t bytes = 0;
         for (auto it = pending.begin(); it != pending.end() && (int64_t)wave.size() < cfg.max_wave_reads;) {
             Request *q = *it;
-            const bool fits = wave.empty() ? true : inflight_bytes + bytes + q->est_bytes <= cfg.hbm_budget_bytes;
+            const bool fits = inflight_bytes + bytes + q->est_bytes <= cfg.hbm_budget_bytes || (q == first && wave.empty());
             /* a wave finishes with its longest read: keep reads of one length class
-               (within length_ratio of the oldest request) together */
+               (within length_ratio of the first request) together */
             const double lr = cfg.length_ratio;
             const bool same_class = lr < 0.0 || ((double)q->n_diag * lr >= (double)first->n_diag &&
                                                  (double)q->n_diag <= lr * (double)first->n_diag);
