@@ -358,12 +358,14 @@ typedef struct {
     int64_t hbm_budget_bytes; /* band-matrix bytes of the waves in flight (0: 32 GiB) */
     int32_t lanes;            /* waves in flight, 1-4 (0: 4) */
     int32_t max_wave_reads;   /* reads per wave at most (0: 2048) */
-    double length_ratio;      /* a wave takes reads whose lX + lY is within this factor of the
-                                 oldest pending request's (a wave lasts as long as its longest
-                                 read); 0: 2; negative: any length */
+    double length_ratio;      /* a wave takes reads whose lX + lY is within this factor of its
+                                 first request's (a wave lasts as long as its longest read);
+                                 0: 2; negative: any length */
     double batch_delay_ms;    /* while other waves keep the device busy, a free lane waits up to
                                  this long after the oldest pending request's arrival for
-                                 max_wave_reads requests to gather; 0: 30; negative: no wait */
+                                 max_wave_reads requests to gather; 0: 30; negative: no wait.
+                                 A wave's first request is the longest of those that arrived
+                                 within this delay of the oldest pending one */
 } sa_server_config;
 typedef struct {
     double submit_s, admit_s, done_s; /* seconds on the server's clock */
