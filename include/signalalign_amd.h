@@ -223,7 +223,10 @@ int sa_align_noalloc(sa_context *ctx, const sa_model *model, const sa_read *read
                      sa_precision precision, sa_pair *pairs, int64_t pairs_cap, int64_t *n_pairs,
                      double *log_likelihood);
 
-/* Batched call: all reads in one length-sorted launch sequence. */
+/* Batched call: all reads in one length-sorted launch sequence.  The context keeps the
+   call's batch and refills it on its next sa_align_batch or sa_align call, so its device
+   buffers (sized for the largest call so far) are allocated once and freed with the
+   context. */
 int sa_align_batch(sa_context *ctx, const sa_model *model, const sa_read *reads, int64_t n_reads,
                    const sa_params *params, sa_precision precision, sa_result **out);
 

@@ -150,6 +150,7 @@ struct sa_context {
     hipStream_t aux[3] = {};   /* sub-alignment groups of different window sizes run side by side */
     int n_streams = 4;         /* 1: everything on `stream` (aux[] alias it) */
     HBuf stage;                /* pinned staging for inputs and pairs (calls are serialised by mu) */
+    sa_batch *batch = nullptr; /* sa_align_batch's batch, refilled call after call */
     std::shared_ptr<sa_rt::PinnedPool> pair_pool = std::make_shared<sa_rt::PinnedPool>();
     std::mutex mu;
     char err[512] = {};        /* text of the last error (fixed storage: setting it allocates nothing) */

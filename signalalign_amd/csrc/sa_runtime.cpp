@@ -379,6 +379,8 @@ int sa_context_create(int device, sa_context **out) { return sa_context_create_e
 
 void sa_context_destroy(sa_context *ctx) {
     if (!ctx) return;
+    if (ctx->batch) sa_batch_destroy(ctx->batch);
+    ctx->batch = nullptr;
     sa_rt::enter(ctx);
     for (auto &e : ctx->ev) if (e) (void)hipEventDestroy(e);
     for (auto &e : ctx->gev) if (e) (void)hipEventDestroy(e);
@@ -2179,11 +2181,23 @@ int sa_batch_run(sa_batch *b, sa_precision precision, sa_result **out) {
 int sa_align_batch(sa_context *ctx, const sa_model *model, const sa_read *reads, int64_t n_reads,
                    const sa_params *params, sa_precision precision, sa_result **out) {
     *out = nullptr;
-    sa_batch *b = nullptr;
-    int rc = sa_batch_create(ctx, model, reads, n_reads, params, &b);
-    if (rc) return rc;
-    rc = sa_batch_run(b, precision, out);
-    sa_batch_destroy(b);
+    if (!ctx) return SA_ERR_INVALID_ARGUMENT;
+    /* one batch per context, refilled call after call: its device buffers only grow, so
+       repeated calls do not allocate again (a large batch's planes are tens of GB, and
+       allocating them from the stream-ordered pool can take seconds per call) */
+    int rc;
+    if (ctx->batch) {
+        rc = sa_batch_refill(ctx->batch, model, reads, n_reads, params);
+    } else {
+        sa_batch *b = nullptr;
+        rc = sa_batch_create(ctx, model, reads, n_reads, params, &b);
+        if (rc == SA_OK) ctx->batch = b;
+    }
+    if (rc == SA_OK) rc = sa_batch_run(ctx->batch, precision, out);
+    if (rc != SA_OK && ctx->batch) { /* start the next call from a fresh batch */
+        sa_batch_destroy(ctx->batch);
+        ctx->batch = nullptr;
+    }
     return rc;
 }
 

@@ -383,12 +383,17 @@ class Context:
             L.sa_expect_free(res)
 
     def align_batch(self, model, reads, params=None, precision=SA_FP64):
-        """Returns ([ReadResult], timing dict)."""
-        b = Batch(self, model, reads, params)
-        try:
-            return b.run(precision)
-        finally:
-            b.close()
+        """sa_align_batch (the context refills one batch call after call).  Returns
+        ([ReadResult], timing dict)."""
+        reads = list(reads)
+        arr = (_Read * max(1, len(reads)))()
+        for i, r in enumerate(reads):
+            arr[i] = r._struct()
+        ps = (params or Params())._struct()
+        res = ctypes.c_void_p()
+        self._check(self._L.sa_align_batch(self._h, model._h, arr, len(reads), ctypes.byref(ps), precision,
+                                           ctypes.byref(res)), "sa_align_batch")
+        return _take_result(self._L, res)
 
 
 class DeviceModel:
@@ -436,6 +441,36 @@ class DeviceModel:
             pass
 
 
+def _take_result(L, res, fetch=True):
+    """([ReadResult], timing dict) of an sa_result, which is freed."""
+    try:
+        t = _Timing()
+        L.sa_result_timing(res, ctypes.byref(t))
+        timing = {k: getattr(t, k) for k, _ in _Timing._fields_}
+        n = L.sa_result_launches(res, None, 0)
+        arr = (_Launch * max(n, 1))()
+        L.sa_result_launches(res, arr, n)
+        timing["launches"] = [{k: getattr(arr[i], k) for k, _ in _Launch._fields_} for i in range(n)]
+        out = []
+        if fetch:
+            rr = _ReadResult()
+            for i in range(L.sa_result_n_reads(res)):
+                L.sa_result_read(res, i, ctypes.byref(rr))
+                if rr.n_pairs:
+                    buf = np.ctypeslib.as_array(ctypes.cast(rr.pairs, ctypes.POINTER(ctypes.c_int32)),
+                                                shape=(rr.n_pairs * 3,))
+                    pairs = buf.reshape(-1, 3).astype(np.int64)
+                else:
+                    pairs = np.zeros((0, 3), dtype=np.int64)
+                paths = (np.ctypeslib.as_array(rr.paths, shape=(rr.n_pairs,)).copy()
+                         if rr.n_pairs and bool(rr.paths) else None)
+                out.append(ReadResult(pairs, rr.log_likelihood, rr.cells, rr.n_windows, rr.n_subalignments,
+                                      paths))
+        return out, timing
+    finally:
+        L.sa_result_free(res)
+
+
 class Batch:
     """Reads uploaded to HBM once, runnable any number of times (sa_batch_*)."""
 
@@ -457,32 +492,7 @@ class Batch:
         L = self.ctx._L
         res = ctypes.c_void_p()
         self.ctx._check(L.sa_batch_run(self._h, precision, ctypes.byref(res)), "sa_batch_run")
-        try:
-            t = _Timing()
-            L.sa_result_timing(res, ctypes.byref(t))
-            timing = {k: getattr(t, k) for k, _ in _Timing._fields_}
-            n = L.sa_result_launches(res, None, 0)
-            arr = (_Launch * max(n, 1))()
-            L.sa_result_launches(res, arr, n)
-            timing["launches"] = [{k: getattr(arr[i], k) for k, _ in _Launch._fields_} for i in range(n)]
-            out = []
-            if fetch:
-                rr = _ReadResult()
-                for i in range(L.sa_result_n_reads(res)):
-                    L.sa_result_read(res, i, ctypes.byref(rr))
-                    if rr.n_pairs:
-                        buf = np.ctypeslib.as_array(ctypes.cast(rr.pairs, ctypes.POINTER(ctypes.c_int32)),
-                                                    shape=(rr.n_pairs * 3,))
-                        pairs = buf.reshape(-1, 3).astype(np.int64)
-                    else:
-                        pairs = np.zeros((0, 3), dtype=np.int64)
-                    paths = (np.ctypeslib.as_array(rr.paths, shape=(rr.n_pairs,)).copy()
-                             if rr.n_pairs and bool(rr.paths) else None)
-                    out.append(ReadResult(pairs, rr.log_likelihood, rr.cells, rr.n_windows, rr.n_subalignments,
-                                          paths))
-            return out, timing
-        finally:
-            L.sa_result_free(res)
+        return _take_result(L, res, fetch)
 
     def tuning(self):
         t = _Tuning()
